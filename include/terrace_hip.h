@@ -1,0 +1,160 @@
+/*
+ * terrace_hip.h — C ABI of the MI355X (gfx950) Terrace hot path.
+ *
+ * A drop-in for the batch-update + traversal path of the reference's
+ * header-only C++ container (namespace graphstore, include/graph.h) and its
+ * Ligra-style traversal (include/Map.cpp, include/VertexSubset.cpp) plus the
+ * algorithm entry points the benchmark driver calls (src/graph_bm.cc).
+ *
+ * Conventions
+ *   - Every function returns an int status: TG_OK (0) or a negative code.
+ *     Nothing aborts and no C++ exception crosses the ABI; tg_last_error()
+ *     returns the message of the last failure on the calling thread.
+ *   - Plain pointers and sizes only.  Functions ending in _dev take device
+ *     pointers (HBM); all others take host pointers.
+ *   - Calls on one tg_graph are serialised internally; each graph owns one
+ *     HIP stream (default priority).
+ *   - Vertex ids are uint32 in [0, n); edges are directed (src, dst).
+ */
+#ifndef TERRACE_HIP_H
+#define TERRACE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TG_OK 0
+#define TG_EINVAL (-1)   /* bad argument (id out of range, null pointer) */
+#define TG_ENOMEM (-2)   /* device allocation failed; the graph is unchanged */
+#define TG_EDEVICE (-3)  /* HIP error; message via tg_last_error() */
+#define TG_EINTERNAL (-4)
+
+typedef struct tg_graph tg_graph;
+typedef struct tg_frontier tg_frontier;
+
+const char *tg_last_error(void);
+
+/* ---- container lifetime: Graph(uint32_t) / ~Graph()
+ *      include/graph.h:118-120, 232-239 ------------------------------- */
+int tg_create(uint32_t num_vertices, tg_graph **out);
+int tg_destroy(tg_graph *g);
+
+/* ---- batch updates ---------------------------------------------------
+ * tg_insert_batch replaces Graph::add_edge_batch (include/graph.h:434-502).
+ * Unlike the reference it accepts any order and duplicates (it sorts and
+ * deduplicates on the device), does not modify its inputs and needs no perm
+ * vector.  Edges already present are ignored.  Tier routing is the
+ * reference's: the 13 smallest neighbours inline, the rest in the PMA while
+ * degree + run length <= 1024, else in the vertex's B-tree.
+ * tg_delete_batch is the batched Graph::remove_edge (include/graph.h:
+ * 675-791); deleting an absent edge is a no-op. */
+int tg_insert_batch(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t count);
+int tg_delete_batch(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t count);
+int tg_insert_batch_dev(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t count);
+int tg_delete_batch_dev(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t count);
+/* Graph::add_edge / remove_edge (include/graph.h:507-641, 675-791):
+ * *result = 0 when the edge was added/removed, -1 when it was already
+ * present/absent. */
+int tg_add_edge(tg_graph *g, uint32_t s, uint32_t d, int *result);
+int tg_remove_edge(tg_graph *g, uint32_t s, uint32_t d, int *result);
+
+/* ---- queries ---------------------------------------------------------- */
+/* Graph::is_edge (include/graph.h:643-673): *out = 1 or 0 */
+int tg_is_edge(tg_graph *g, uint32_t s, uint32_t d, uint32_t *out);
+int tg_is_edge_batch(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t count, uint8_t *out);
+/* Graph::degree (include/graph.h:797-803) */
+int tg_degree(tg_graph *g, uint32_t v, uint32_t *out);
+int tg_degrees_all(tg_graph *g, uint32_t *out /* [n] */);
+/* Graph::get_num_edges (include/graph.h:817-822), as uint64 */
+int tg_num_edges(tg_graph *g, uint64_t *out);
+/* Graph::get_num_vertices (include/graph.h:824-826) */
+int tg_num_vertices(tg_graph *g, uint32_t *out);
+/* Graph::get_size (include/graph.h:805-815): device bytes held */
+int tg_memory_bytes(tg_graph *g, uint64_t *out);
+/* NeighborIterator (include/graph.h:844-899) for every vertex at once:
+ * offsets[n+1], nbrs[num_edges], each list in sorted order. */
+int tg_export_adjacency(tg_graph *g, uint64_t *offsets, uint32_t *nbrs);
+int tg_export_adjacency_dev(tg_graph *g, uint64_t *offsets, uint32_t *nbrs);
+/* Graph::neighbors(v): copies min(degree, cap) neighbours, sorted. */
+int tg_neighbors(tg_graph *g, uint32_t v, uint32_t *out, uint32_t cap, uint32_t *degree);
+/* Tier of every vertex: 0 inline only, 1 PMA, 2 B-tree. */
+int tg_tiers(tg_graph *g, uint8_t *out /* [n] */);
+
+/* ---- algorithms (run_algorithm's tests, src/graph_bm.cc:86-229) ------- */
+/* BFS_with_edge_map (include/BFS.h:41-84): parents[n] (-1 unreached),
+ * levels[n] (UINT32_MAX unreached); either may be NULL. */
+int tg_bfs(tg_graph *g, uint32_t src, int32_t *parents, uint32_t *levels);
+/* PR_S<double> (include/Pagerank.h:106-184), undamped as in the reference */
+int tg_pagerank(tg_graph *g, uint32_t iters, double *out);
+/* CC (include/Components.h:92-120): minimum vertex id per component */
+int tg_cc(tg_graph *g, uint32_t *labels);
+/* BC (include/BC.h:99-158): unreachable vertices are NaN as in the reference */
+int tg_bc(tg_graph *g, uint32_t src, double *out);
+/* TC (include/TC.h:73-87) */
+int tg_tc(tg_graph *g, uint64_t *count);
+/* Device-output variants (pointers into HBM) */
+int tg_bfs_dev(tg_graph *g, uint32_t src, int32_t *parents, uint32_t *levels, uint32_t *depth);
+int tg_pagerank_dev(tg_graph *g, uint32_t iters, double *out);
+int tg_cc_dev(tg_graph *g, uint32_t *labels, uint32_t *rounds);
+int tg_bc_dev(tg_graph *g, uint32_t src, double *out);
+
+/* ---- frontiers and edgeMap (include/VertexSubset.cpp, include/Map.cpp) */
+enum tg_op { TG_OP_BFS = 1, TG_OP_CC = 2, TG_OP_BC_FWD = 3, TG_OP_BC_BACK = 4, TG_OP_PR = 5 };
+/* Device arrays an op reads and writes (all device pointers):
+ *   BFS     parent (int32, -1 = unvisited), level (optional), depth
+ *   CC      ids, prev (prevIDs; output = vertices whose id changed)
+ *   BC_*    val (NumPaths / Dependencies), visited (uint8)
+ *   PR      src_val (normalised p_curr), val (p_next, assigned) */
+typedef struct {
+  int32_t *parent;
+  uint32_t *level;
+  uint32_t depth;
+  uint32_t *ids;
+  const uint32_t *prev;
+  double *val;
+  const double *src_val;
+  const uint8_t *visited;
+} tg_op_args;
+/* VertexSubset(el_t e, uint64_t n, bool all) */
+int tg_frontier_create(tg_graph *g, tg_frontier **out);
+int tg_frontier_destroy(tg_frontier *f);
+int tg_frontier_set(tg_frontier *f, const uint32_t *verts, uint64_t count); /* host list */
+int tg_frontier_set_all(tg_frontier *f);
+int tg_frontier_size(tg_frontier *f, uint64_t *out);                       /* get_n */
+/* sorted members into out[cap] (host); *count = size */
+int tg_frontier_get(tg_frontier *f, uint32_t *out, uint64_t cap, uint64_t *count);
+/* edgeMap(G, vs, F, output, threshold) (include/Map.cpp:66-74); out may be
+ * NULL when output is false. */
+int tg_edge_map(tg_graph *g, tg_frontier *in, int op, const tg_op_args *args, tg_frontier *out,
+                uint32_t threshold);
+
+/* ---- workload and device memory helpers -------------------------------
+ * rMat<uint32_t>(n, seed, a, b, c)(i) for i in [start, start+count)
+ * (include/rmat_util.h:81-115) written straight into device arrays.
+ * nominal=0 is the reference's divisor (INT32_MAX), 1 divides by 2^32.
+ * flags: 1 = also write every reverse edge, 2 = drop self-loops.
+ * *written = entries written (output order is not index order). */
+int tg_rmat_dev(uint32_t n, uint32_t seed, double a, double b, double c, int nominal, uint64_t start,
+                uint64_t count, int flags, uint32_t *src, uint32_t *dst, uint64_t *written);
+int tg_dev_alloc(uint64_t bytes, void **out);
+int tg_dev_free(void *p);
+int tg_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
+int tg_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+int tg_device_sync(void);
+
+/* ---- measurement -------------------------------------------------------
+ * With profiling on, kernels of interest are bracketed by hipEvents on
+ * the graph's stream; tg_profile_get sums their durations by name. */
+int tg_profile_enable(tg_graph *g, int on);
+int tg_profile_reset(tg_graph *g);
+int tg_profile_get(tg_graph *g, const char *name, double *ms, uint64_t *launches);
+/* internal counters: [0] PMA rebuilds, [1] PMA windows, [2] window leaves,
+ * [3] tree blocks written, [4] promotions, [5] trees, [6] PMA slots */
+int tg_stats(tg_graph *g, uint64_t *out /* [8] */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,557 @@
+// extern "C" boundary of libterrace_hip.so (declared in include/terrace_hip.h).
+// Every entry point catches all C++ exceptions and returns a status code.
+#include "../../include/terrace_hip.h"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "gen.hpp"
+#include "traverse.hpp"
+
+struct tg_graph {
+  tg::Graph g;
+  tg::Traversal t;
+  // host-call staging buffers
+  tg::DevBuf<uint32_t> s_src, s_dst, s_u32;
+  tg::DevBuf<uint8_t> s_u8;
+  tg::DevBuf<uint64_t> s_u64;
+  tg::DevBuf<int32_t> s_i32;
+  tg::DevBuf<double> s_f64;
+  explicit tg_graph(uint32_t n) : g(n), t(g) {}
+};
+
+struct tg_frontier {
+  tg_graph *owner;
+  tg::Frontier f;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F &&f) {
+  try {
+    f();
+    return TG_OK;
+  } catch (const tg::Error &e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_err = "host allocation failed";
+    return TG_ENOMEM;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return TG_EINTERNAL;
+  } catch (...) {
+    g_err = "unknown error";
+    return TG_EINTERNAL;
+  }
+}
+
+#define NEED(cond, msg) \
+  if (!(cond)) throw tg::Error(TG_EINVAL, msg)
+
+void h2d(void *d, const void *h, uint64_t bytes, hipStream_t s) {
+  if (bytes) TG_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+}
+void d2h(void *h, const void *d, uint64_t bytes, hipStream_t s) {
+  if (bytes) TG_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
+  TG_HIP(hipStreamSynchronize(s));
+}
+
+void stage_pair(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint64_t count) {
+  hipStream_t s = G->g.stream();
+  G->s_src.ensure(count + 1, s);
+  G->s_dst.ensure(count + 1, s);
+  h2d(G->s_src.get(), src, count * 4, s);
+  h2d(G->s_dst.get(), dst, count * 4, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tg_last_error(void) { return g_err.c_str(); }
+
+int tg_create(uint32_t num_vertices, tg_graph **out) {
+  return guard([&] {
+    NEED(out, "null out");
+    *out = nullptr;
+    NEED(num_vertices > 0, "num_vertices must be > 0");
+    *out = new tg_graph(num_vertices);
+  });
+}
+
+int tg_destroy(tg_graph *g) {
+  return guard([&] { delete g; });
+}
+
+int tg_insert_batch(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint64_t count) {
+  return guard([&] {
+    NEED(G && (count == 0 || (src && dst)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    if (!count) return;
+    stage_pair(G, src, dst, count);
+    G->g.insert_batch(G->s_src.get(), G->s_dst.get(), count);
+  });
+}
+
+int tg_delete_batch(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint64_t count) {
+  return guard([&] {
+    NEED(G && (count == 0 || (src && dst)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    if (!count) return;
+    stage_pair(G, src, dst, count);
+    G->g.delete_batch(G->s_src.get(), G->s_dst.get(), count);
+  });
+}
+
+int tg_insert_batch_dev(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint64_t count) {
+  return guard([&] {
+    NEED(G && (count == 0 || (src && dst)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->g.insert_batch(src, dst, count);
+  });
+}
+
+int tg_delete_batch_dev(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint64_t count) {
+  return guard([&] {
+    NEED(G && (count == 0 || (src && dst)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->g.delete_batch(src, dst, count);
+  });
+}
+
+static int single_edge(tg_graph *G, uint32_t s, uint32_t d, int *result, bool insert) {
+  return guard([&] {
+    NEED(G && result, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    NEED(s < G->g.num_vertices() && d < G->g.num_vertices(), "vertex id out of range");
+    hipStream_t st = G->g.stream();
+    stage_pair(G, &s, &d, 1);
+    G->s_u8.ensure(2, st);
+    G->g.is_edge_batch(G->s_src.get(), G->s_dst.get(), 1, G->s_u8.get());
+    uint8_t present = 0;
+    d2h(&present, G->s_u8.get(), 1, st);
+    if (insert) {
+      *result = present ? -1 : 0;
+      if (!present) G->g.insert_batch(G->s_src.get(), G->s_dst.get(), 1);
+    } else {
+      *result = present ? 0 : -1;
+      if (present) G->g.delete_batch(G->s_src.get(), G->s_dst.get(), 1);
+    }
+  });
+}
+
+int tg_add_edge(tg_graph *G, uint32_t s, uint32_t d, int *result) { return single_edge(G, s, d, result, true); }
+int tg_remove_edge(tg_graph *G, uint32_t s, uint32_t d, int *result) { return single_edge(G, s, d, result, false); }
+
+int tg_is_edge(tg_graph *G, uint32_t s, uint32_t d, uint32_t *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    NEED(s < G->g.num_vertices() && d < G->g.num_vertices(), "vertex id out of range");
+    hipStream_t st = G->g.stream();
+    stage_pair(G, &s, &d, 1);
+    G->s_u8.ensure(2, st);
+    G->g.is_edge_batch(G->s_src.get(), G->s_dst.get(), 1, G->s_u8.get());
+    uint8_t r = 0;
+    d2h(&r, G->s_u8.get(), 1, st);
+    *out = r;
+  });
+}
+
+int tg_is_edge_batch(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint64_t count, uint8_t *out) {
+  return guard([&] {
+    NEED(G && (count == 0 || (src && dst && out)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    if (!count) return;
+    hipStream_t st = G->g.stream();
+    stage_pair(G, src, dst, count);
+    G->s_u8.ensure(count + 1, st);
+    G->g.is_edge_batch(G->s_src.get(), G->s_dst.get(), count, G->s_u8.get());
+    d2h(out, G->s_u8.get(), count, st);
+  });
+}
+
+int tg_degree(tg_graph *G, uint32_t v, uint32_t *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    NEED(v < G->g.num_vertices(), "vertex id out of range");
+    hipStream_t st = G->g.stream();
+    uint32_t n = G->g.num_vertices();
+    G->s_u32.ensure(n, st);
+    G->g.degrees(G->s_u32.get());
+    d2h(out, G->s_u32.get() + v, 4, st);
+  });
+}
+
+int tg_degrees_all(tg_graph *G, uint32_t *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    hipStream_t st = G->g.stream();
+    uint32_t n = G->g.num_vertices();
+    G->s_u32.ensure(n, st);
+    G->g.degrees(G->s_u32.get());
+    d2h(out, G->s_u32.get(), static_cast<uint64_t>(n) * 4, st);
+  });
+}
+
+int tg_num_edges(tg_graph *G, uint64_t *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    *out = G->g.num_edges();
+  });
+}
+
+int tg_num_vertices(tg_graph *G, uint32_t *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    *out = G->g.num_vertices();
+  });
+}
+
+int tg_memory_bytes(tg_graph *G, uint64_t *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    *out = G->g.memory_bytes();
+  });
+}
+
+int tg_export_adjacency_dev(tg_graph *G, uint64_t *offsets, uint32_t *nbrs) {
+  return guard([&] {
+    NEED(G && offsets && nbrs, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->g.export_adjacency(offsets, nbrs);
+  });
+}
+
+int tg_export_adjacency(tg_graph *G, uint64_t *offsets, uint32_t *nbrs) {
+  return guard([&] {
+    NEED(G && offsets, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    hipStream_t st = G->g.stream();
+    uint32_t n = G->g.num_vertices();
+    uint64_t m = G->g.num_edges();
+    NEED(m == 0 || nbrs, "null neighbours");
+    G->s_u64.ensure(static_cast<uint64_t>(n) + 1, st);
+    G->s_u32.ensure(m + 1, st);
+    G->g.export_adjacency(G->s_u64.get(), G->s_u32.get());
+    d2h(offsets, G->s_u64.get(), (static_cast<uint64_t>(n) + 1) * 8, st);
+    d2h(nbrs, G->s_u32.get(), m * 4, st);
+  });
+}
+
+int tg_neighbors(tg_graph *G, uint32_t v, uint32_t *out, uint32_t cap, uint32_t *degree) {
+  return guard([&] {
+    NEED(G && degree, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    NEED(v < G->g.num_vertices(), "vertex id out of range");
+    hipStream_t st = G->g.stream();
+    uint32_t n = G->g.num_vertices();
+    G->s_u32.ensure(n, st);
+    G->g.degrees(G->s_u32.get());
+    uint32_t deg = 0;
+    d2h(&deg, G->s_u32.get() + v, 4, st);
+    *degree = deg;
+    if (!deg || !cap) return;
+    NEED(out, "null out");
+    G->s_dst.ensure(static_cast<uint64_t>(deg) + 64, st);
+    G->g.neighbors_of(v, G->s_dst.get());
+    d2h(out, G->s_dst.get(), static_cast<uint64_t>(std::min(deg, cap)) * 4, st);
+  });
+}
+
+int tg_tiers(tg_graph *G, uint8_t *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    hipStream_t st = G->g.stream();
+    uint32_t n = G->g.num_vertices();
+    G->s_u8.ensure(n, st);
+    G->g.tiers(G->s_u8.get());
+    d2h(out, G->s_u8.get(), n, st);
+  });
+}
+
+// ---------------------------------------------------------------- algorithms
+int tg_bfs_dev(tg_graph *G, uint32_t src, int32_t *parents, uint32_t *levels, uint32_t *depth) {
+  return guard([&] {
+    NEED(G && parents, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    NEED(src < G->g.num_vertices(), "source out of range");
+    uint32_t d = G->t.bfs(src, parents, levels);
+    if (depth) *depth = d;
+  });
+}
+
+int tg_bfs(tg_graph *G, uint32_t src, int32_t *parents, uint32_t *levels) {
+  return guard([&] {
+    NEED(G, "null graph");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    uint32_t n = G->g.num_vertices();
+    NEED(src < n, "source out of range");
+    hipStream_t st = G->g.stream();
+    G->s_i32.ensure(n, st);
+    G->s_u32.ensure(n, st);
+    G->t.bfs(src, G->s_i32.get(), G->s_u32.get());
+    if (parents) d2h(parents, G->s_i32.get(), static_cast<uint64_t>(n) * 4, st);
+    if (levels) d2h(levels, G->s_u32.get(), static_cast<uint64_t>(n) * 4, st);
+  });
+}
+
+int tg_pagerank_dev(tg_graph *G, uint32_t iters, double *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->t.pagerank(iters, out);
+  });
+}
+
+int tg_pagerank(tg_graph *G, uint32_t iters, double *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    uint32_t n = G->g.num_vertices();
+    hipStream_t st = G->g.stream();
+    G->s_f64.ensure(n, st);
+    G->t.pagerank(iters, G->s_f64.get());
+    d2h(out, G->s_f64.get(), static_cast<uint64_t>(n) * 8, st);
+  });
+}
+
+int tg_cc_dev(tg_graph *G, uint32_t *labels, uint32_t *rounds) {
+  return guard([&] {
+    NEED(G && labels, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    uint32_t r = G->t.cc(labels);
+    if (rounds) *rounds = r;
+  });
+}
+
+int tg_cc(tg_graph *G, uint32_t *labels) {
+  return guard([&] {
+    NEED(G && labels, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    uint32_t n = G->g.num_vertices();
+    hipStream_t st = G->g.stream();
+    G->s_u32.ensure(n, st);
+    G->t.cc(G->s_u32.get());
+    d2h(labels, G->s_u32.get(), static_cast<uint64_t>(n) * 4, st);
+  });
+}
+
+int tg_bc_dev(tg_graph *G, uint32_t src, double *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    NEED(src < G->g.num_vertices(), "source out of range");
+    G->t.bc(src, out);
+  });
+}
+
+int tg_bc(tg_graph *G, uint32_t src, double *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    uint32_t n = G->g.num_vertices();
+    NEED(src < n, "source out of range");
+    hipStream_t st = G->g.stream();
+    G->s_f64.ensure(n, st);
+    G->t.bc(src, G->s_f64.get());
+    d2h(out, G->s_f64.get(), static_cast<uint64_t>(n) * 8, st);
+  });
+}
+
+int tg_tc(tg_graph *G, uint64_t *count) {
+  return guard([&] {
+    NEED(G && count, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    *count = G->t.tc();
+  });
+}
+
+// ---------------------------------------------------------------- frontiers
+int tg_frontier_create(tg_graph *G, tg_frontier **out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    auto *f = new tg_frontier;
+    f->owner = G;
+    f->f.n = G->g.num_vertices();
+    f->f.count = 0;
+    f->f.has_queue = true;
+    f->f.queue.ensure(f->f.n, G->g.stream());
+    *out = f;
+  });
+}
+
+int tg_frontier_destroy(tg_frontier *f) {
+  return guard([&] { delete f; });
+}
+
+int tg_frontier_set(tg_frontier *F, const uint32_t *verts, uint64_t count) {
+  return guard([&] {
+    NEED(F && (count == 0 || verts), "null argument");
+    std::lock_guard<std::mutex> l(F->owner->g.mu);
+    uint32_t n = F->f.n;
+    std::vector<uint32_t> v(verts, verts + count);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    NEED(v.empty() || v.back() < n, "vertex id out of range");
+    hipStream_t st = F->owner->g.stream();
+    F->f.queue.ensure(n, st);
+    h2d(F->f.queue.get(), v.data(), v.size() * 4, st);
+    F->f.count = v.size();
+    F->f.all = false;
+    F->f.has_queue = true;
+    F->f.has_bits = false;
+    TG_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int tg_frontier_set_all(tg_frontier *F) {
+  return guard([&] {
+    NEED(F, "null argument");
+    F->f.all = true;
+    F->f.count = F->f.n;
+    F->f.has_queue = false;
+    F->f.has_bits = false;
+  });
+}
+
+int tg_frontier_size(tg_frontier *F, uint64_t *out) {
+  return guard([&] {
+    NEED(F && out, "null argument");
+    *out = F->f.count;
+  });
+}
+
+int tg_frontier_get(tg_frontier *F, uint32_t *out, uint64_t cap, uint64_t *count) {
+  return guard([&] {
+    NEED(F && count, "null argument");
+    std::lock_guard<std::mutex> l(F->owner->g.mu);
+    F->owner->t.to_queue(F->f);
+    *count = F->f.count;
+    if (!cap || !F->f.count) return;
+    NEED(out, "null out");
+    std::vector<uint32_t> v(F->f.count);
+    d2h(v.data(), F->f.queue.get(), F->f.count * 4, F->owner->g.stream());
+    std::sort(v.begin(), v.end());
+    std::copy(v.begin(), v.begin() + std::min<uint64_t>(cap, v.size()), out);
+  });
+}
+
+int tg_edge_map(tg_graph *G, tg_frontier *in, int op, const tg_op_args *a, tg_frontier *out, uint32_t threshold) {
+  return guard([&] {
+    NEED(G && in && a, "null argument");
+    NEED(in->owner == G && (!out || out->owner == G), "frontier of another graph");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    tg::OpState st;
+    st.parent = a->parent;
+    st.level = a->level;
+    st.depth = a->depth;
+    st.ids = a->ids;
+    st.prev = a->prev;
+    st.val = a->val;
+    st.src_val = a->src_val;
+    st.visited = a->visited;
+    tg::Op o;
+    switch (op) {
+      case TG_OP_BFS: NEED(st.parent, "BFS needs parent"); o = tg::Op::BFS; break;
+      case TG_OP_CC: NEED(st.ids, "CC needs ids"); o = tg::Op::CC; break;
+      case TG_OP_BC_FWD: NEED(st.val && st.visited, "BC needs val and visited"); o = tg::Op::BC_FWD; break;
+      case TG_OP_BC_BACK: NEED(st.val && st.visited, "BC needs val and visited"); o = tg::Op::BC_BACK; break;
+      case TG_OP_PR: NEED(st.val && st.src_val, "PR needs val and src_val"); o = tg::Op::PR; break;
+      default: throw tg::Error(TG_EINVAL, "unknown op");
+    }
+    G->t.edge_map(in->f, o, st, out ? &out->f : nullptr, threshold);
+    TG_HIP(hipStreamSynchronize(G->g.stream()));
+  });
+}
+
+// ---------------------------------------------------------------- helpers
+int tg_rmat_dev(uint32_t n, uint32_t seed, double a, double b, double c, int nominal, uint64_t start,
+                uint64_t count, int flags, uint32_t *src, uint32_t *dst, uint64_t *written) {
+  return guard([&] {
+    NEED(count == 0 || (src && dst), "null argument");
+    uint64_t w = tg::rmat_device(n, seed, a, b, c, nominal, start, count, flags, src, dst, nullptr);
+    if (written) *written = w;
+  });
+}
+
+int tg_dev_alloc(uint64_t bytes, void **out) {
+  return guard([&] {
+    NEED(out, "null out");
+    *out = nullptr;
+    TG_HIP(hipMalloc(out, bytes ? bytes : 1));
+  });
+}
+
+int tg_dev_free(void *p) {
+  return guard([&] {
+    if (p) TG_HIP(hipFree(p));
+  });
+}
+
+int tg_memcpy_h2d(void *dst, const void *src, uint64_t bytes) {
+  return guard([&] { TG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+
+int tg_memcpy_d2h(void *dst, const void *src, uint64_t bytes) {
+  return guard([&] { TG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+
+int tg_device_sync(void) {
+  return guard([&] { TG_HIP(hipDeviceSynchronize()); });
+}
+
+int tg_profile_enable(tg_graph *G, int on) {
+  return guard([&] {
+    NEED(G, "null graph");
+    G->t.profiling = on != 0;
+  });
+}
+
+int tg_profile_reset(tg_graph *G) {
+  return guard([&] {
+    NEED(G, "null graph");
+    G->t.ktimes.clear();
+  });
+}
+
+int tg_profile_get(tg_graph *G, const char *name, double *ms, uint64_t *launches) {
+  return guard([&] {
+    NEED(G && name && ms && launches, "null argument");
+    *ms = 0;
+    *launches = 0;
+    for (auto &k : G->t.ktimes)
+      if (std::strcmp(k.name, name) == 0) {
+        *ms = k.ms;
+        *launches = k.launches;
+      }
+  });
+}
+
+int tg_stats(tg_graph *G, uint64_t *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    const tg::Stats &s = G->g.stats();
+    out[0] = s.pma_rebuilds;
+    out[1] = s.pma_windows;
+    out[2] = s.pma_window_leaves;
+    out[3] = s.tree_blocks_written;
+    out[4] = s.promotions;
+    out[5] = G->g.num_trees();
+    out[6] = G->g.pma_slots();
+    out[7] = 0;
+  });
+}
+
+}  // extern "C"

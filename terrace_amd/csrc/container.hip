@@ -1,0 +1,1788 @@
+// Degree-tiered dynamic graph on gfx950: batch insert / delete across the
+// inline, PMA and B-tree tiers, plus the read-side queries.
+//
+// Reference semantics restated here (include/graph.h):
+//   * add_inplace (graph.h:256-372): the 13 smallest neighbours stay inline,
+//     the rest of a source run is "kicked" to the secondary tier; the run
+//     goes to the B-tree when the vertex already has one or degree + run
+//     length > 1024 (graph.h:276-277,344-345), otherwise to the PMA.
+//   * add_btree (graph.h:379-426): a vertex routed to the B-tree for the
+//     first time moves its PMA neighbours into its new tree.
+//   * remove_edge (graph.h:675-791): deleting an inline neighbour refills
+//     inline from the smallest secondary neighbour; B-trees are never
+//     demoted.
+// The batch is processed as whole-array kernels instead of per-edge locks:
+// sort + unique, existence filter, one thread per source run for the
+// inline merge, then one pass per secondary tier.
+#include "graph.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace tg {
+
+namespace {
+
+constexpr uint64_t kInvalidKey = ~0ull;
+constexpr uint8_t kRouteNone = 0, kRoutePma = 1, kRouteTree = 2;
+
+// ------------------------------------------------------------ device search
+__device__ inline uint64_t pma_find_leaf(const GView &g, uint64_t key) {
+  uint64_t lo = 0, hi = 1ull << g.H;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (g.fence[mid] <= key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo == 0 ? 0 : lo - 1;
+}
+
+// Slot of (s,d) in the PMA, or ~0 when absent.
+__device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d) {
+  uint64_t f = pma_find_leaf(g, make_key(s, d));
+  uint64_t base = f << kLeafShift;
+  for (uint32_t i = 0; i < kLeafSlots; i++) {
+    if (g.pdst[base + i] == d && g.psrc[base + i] == s) return base + i;
+  }
+  return ~0ull;
+}
+
+// Index position j of the leaf block of tree t that may hold d
+// (last block whose first key <= d, or 0).
+__device__ inline uint32_t tree_block_pos(const GView &g, uint32_t t, uint32_t d) {
+  uint64_t off = g.toff[t];
+  uint32_t lo = 0, hi = g.tnblk[t];
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (g.imin[off + mid] <= d) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo == 0 ? 0 : lo - 1;
+}
+
+__device__ inline bool block_has(const uint32_t *keys, uint32_t cnt, uint32_t d) {
+  uint32_t lo = 0, hi = cnt;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (keys[mid] < d) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < cnt && keys[lo] == d;
+}
+
+__device__ inline bool tree_has(const GView &g, uint32_t t, uint32_t d) {
+  uint32_t nb = g.tnblk[t];
+  if (nb == 0) return false;
+  uint32_t j = tree_block_pos(g, t, d);
+  uint32_t b = g.iblk[g.toff[t] + j];
+  return block_has(g.bkeys + static_cast<uint64_t>(b) * kBlk, g.bcnt[b], d);
+}
+
+// Graph::is_edge (graph.h:643-673): inline first, then the secondary tier.
+__device__ inline bool edge_exists(const GView &g, uint32_t s, uint32_t d) {
+  const VBlock &b = g.vb[s];
+  uint32_t D = b.degree;
+  uint32_t lim = D < kInline ? D : kInline;
+  for (uint32_t i = 0; i < lim; i++) {
+    uint32_t x = b.nbr[i];
+    if (x == d) return true;
+    if (x > d) return false;
+  }
+  if (D <= kInline) return false;
+  uint64_t aux = b.aux;
+  if (aux & kAuxTree) return tree_has(g, static_cast<uint32_t>(aux), d);
+  return pma_slot_of(g, s, d) != ~0ull;
+}
+
+// Merge path: number of A elements among the first m of merge(A, B)
+// (A and B sorted, disjoint).
+template <typename TA, typename TB>
+__device__ inline uint64_t merge_split(const TA &A, uint64_t na, const TB &B, uint64_t nb, uint64_t m) {
+  uint64_t lo = m > nb ? m - nb : 0;
+  uint64_t hi = m < na ? m : na;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (A(mid) < B(m - 1 - mid)) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+struct KeyArr {
+  const uint64_t *p;
+  __device__ uint64_t operator()(uint64_t i) const { return p[i]; }
+};
+struct DstArr {  // dst part of a key array
+  const uint64_t *p;
+  __device__ uint32_t operator()(uint64_t i) const { return static_cast<uint32_t>(p[i]); }
+};
+struct U32Arr {
+  const uint32_t *p;
+  __device__ uint32_t operator()(uint64_t i) const { return p[i]; }
+};
+
+template <typename T>
+__device__ inline uint64_t lower_bound_dev(const T *a, uint64_t n, T x) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+template <typename T>
+__device__ inline uint64_t upper_bound_dev(const T *a, uint64_t n, T x) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+#define GRID_STRIDE(i, n)                                                                  \
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < (n); \
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+
+// ------------------------------------------------------------ batch prep
+__global__ void k_pack(const uint32_t *src, const uint32_t *dst, uint64_t n, uint32_t nv, uint64_t *keys,
+                       uint64_t *err) {
+  GRID_STRIDE(i, n) {
+    uint32_t s = src[i], d = dst[i];
+    if (s >= nv || d >= nv) {
+      atomicOr(reinterpret_cast<unsigned long long *>(err), 1ull);
+      keys[i] = 0;
+    } else {
+      keys[i] = make_key(s, d);
+    }
+  }
+}
+
+__global__ void k_flag_unique(const uint64_t *keys, uint64_t n, uint32_t *flags) {
+  GRID_STRIDE(i, n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_scatter_flagged(const uint64_t *in, const uint32_t *flags, const uint32_t *pos, uint64_t n,
+                                  uint64_t *out) {
+  GRID_STRIDE(i, n) if (flags[i]) out[pos[i]] = in[i];
+}
+
+// exists flag per (sorted unique) key
+__global__ void k_exists(GView g, const uint64_t *keys, uint64_t n, uint32_t *ex) {
+  GRID_STRIDE(i, n) ex[i] = edge_exists(g, key_src(keys[i]), key_dst(keys[i])) ? 1u : 0u;
+}
+
+__global__ void k_not(const uint32_t *a, uint64_t n, uint32_t *out) {
+  GRID_STRIDE(i, n) out[i] = a[i] ? 0u : 1u;
+}
+
+// run boundaries over sorted keys (by src)
+__global__ void k_flag_runs(const uint64_t *keys, uint64_t n, uint32_t *flags) {
+  GRID_STRIDE(i, n) flags[i] = (i == 0 || key_src(keys[i]) != key_src(keys[i - 1])) ? 1u : 0u;
+}
+
+__global__ void k_run_starts(const uint32_t *flags, const uint32_t *pos, uint64_t n, uint32_t *run_beg,
+                             uint32_t nruns) {
+  GRID_STRIDE(i, n) if (flags[i]) run_beg[pos[i]] = static_cast<uint32_t>(i);
+  if (blockIdx.x == 0 && threadIdx.x == 0) run_beg[nruns] = static_cast<uint32_t>(n);
+}
+
+// ------------------------------------------------------------ inline merge
+// One thread per source run of the unique sorted batch R.  ex[] marks keys
+// already present; nidx[] is the exclusive count of new keys (size |R|+1).
+// Writes the new inline slots and degree, the run's route, and the old
+// inline entries that are kicked out (kick_old[run*13 ..], count in
+// run_nko).
+__global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
+                               const uint32_t *run_beg, uint32_t nruns, uint8_t *run_route,
+                               uint32_t *run_c, uint32_t *run_nko, uint32_t *kick_old, uint32_t *promote) {
+  GRID_STRIDE(r, nruns) {
+    uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
+    uint32_t s = key_src(R[b0]);
+    VBlock &vb = g.vb[s];
+    uint32_t D = vb.degree;
+    uint32_t lim = D < kInline ? D : kInline;
+    uint32_t inl[kInline];
+    for (uint32_t i = 0; i < kInline; i++) inl[i] = i < lim ? vb.nbr[i] : 0;
+    uint64_t size = b1 - b0;  // run length including keys already present
+    uint32_t nnew = nidx[b1] - nidx[b0];
+    // k = |inline ∪ run| as the reference counts it (graph.h:290-330)
+    uint32_t dup_inline = 0;
+    for (uint32_t p = 0; p < lim; p++) {
+      uint64_t x = make_key(s, inl[p]);
+      uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, x);
+      if (lb < b1 && R[lb] == x) dup_inline++;
+    }
+    uint64_t kk = static_cast<uint64_t>(lim) + size - dup_inline;
+    bool is_tree = (vb.aux & kAuxTree) != 0;
+    bool overflow = (D == 0) ? (size > kInline) : (kk > kInline);
+    uint8_t route = kRouteNone;
+    if (overflow) route = (is_tree || static_cast<uint64_t>(D) + size > kMediumDegree) ? kRouteTree : kRoutePma;
+    // merge the inline slots with the new keys, keep the 13 smallest
+    uint32_t out[kInline];
+    uint32_t ni = 0, p = 0, c = 0;
+    uint32_t q = b0;
+    while (ni < kInline) {
+      while (q < b1 && ex[q]) q++;
+      bool have_a = p < lim, have_b = q < b1;
+      if (!have_a && !have_b) break;
+      if (have_a && (!have_b || inl[p] < key_dst(R[q]))) {
+        out[ni++] = inl[p++];
+      } else {
+        out[ni++] = key_dst(R[q]);
+        q++;
+        c++;
+      }
+    }
+    uint32_t nko = lim - p;
+    for (uint32_t i = 0; i < nko; i++) kick_old[static_cast<uint64_t>(r) * kInline + i] = inl[p + i];
+    for (uint32_t i = 0; i < ni; i++) vb.nbr[i] = out[i];
+    vb.degree = D + nnew;
+    run_route[r] = route;
+    run_c[r] = c;
+    run_nko[r] = nko;
+    promote[r] = (route == kRouteTree && !is_tree) ? 1u : 0u;
+  }
+}
+
+// Place the kicked keys of every run, sorted, into kick[run_beg ..]:
+// new keys beyond the first c of the run, merged with the old inline
+// entries that were pushed out.
+__global__ void k_place_kicked(const uint64_t *R, const uint32_t *ex, const uint32_t *nidx, uint64_t nR,
+                               const uint32_t *run_beg, const uint32_t *run_of, const uint8_t *run_route,
+                               const uint32_t *run_c, const uint32_t *run_nko, const uint32_t *kick_old,
+                               uint64_t *kick, uint8_t *kroute) {
+  GRID_STRIDE(i, nR) {
+    if (ex[i]) continue;
+    uint32_t r = run_of[i];
+    uint8_t route = run_route[r];
+    if (route == kRouteNone) continue;
+    uint32_t b0 = run_beg[r];
+    uint32_t j = nidx[i] - nidx[b0];
+    uint32_t c = run_c[r];
+    if (j < c) continue;
+    uint32_t d = key_dst(R[i]);
+    uint32_t nko = run_nko[r];
+    const uint32_t *ko = kick_old + static_cast<uint64_t>(r) * kInline;
+    uint32_t less = 0;
+    while (less < nko && ko[less] < d) less++;
+    uint64_t pos = b0 + (j - c) + less;
+    kick[pos] = R[i];
+    kroute[pos] = route;
+  }
+}
+
+__global__ void k_place_kicked_old(const uint64_t *R, const uint32_t *nidx, const uint32_t *run_beg,
+                                   uint32_t nruns, const uint8_t *run_route, const uint32_t *run_c,
+                                   const uint32_t *run_nko, const uint32_t *kick_old, uint64_t *kick,
+                                   uint8_t *kroute) {
+  GRID_STRIDE(r, nruns) {
+    uint8_t route = run_route[r];
+    uint32_t nko = run_nko[r];
+    if (route == kRouteNone || nko == 0) continue;
+    uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
+    uint32_t s = key_src(R[b0]);
+    uint32_t c = run_c[r];
+    for (uint32_t e = 0; e < nko; e++) {
+      uint32_t x = kick_old[static_cast<uint64_t>(r) * kInline + e];
+      uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, make_key(s, x));
+      uint32_t new_less = nidx[lb] - nidx[b0] - c;  // new tail keys below x
+      uint64_t pos = b0 + e + new_less;
+      kick[pos] = make_key(s, x);
+      kroute[pos] = route;
+    }
+  }
+}
+
+__global__ void k_run_of(const uint32_t *flags, const uint32_t *pos, uint64_t n, uint32_t *run_of) {
+  GRID_STRIDE(i, n) run_of[i] = pos[i] + flags[i] - 1;
+}
+
+__global__ void k_flag_route(const uint8_t *kroute, uint64_t n, uint8_t want, uint32_t *flags) {
+  GRID_STRIDE(i, n) flags[i] = kroute[i] == want ? 1u : 0u;
+}
+
+// ------------------------------------------------------------ PMA kernels
+__global__ void k_leaf_of(GView g, const uint64_t *keys, uint64_t k, uint32_t *leaf) {
+  GRID_STRIDE(i, k) leaf[i] = static_cast<uint32_t>(pma_find_leaf(g, keys[i]));
+}
+
+__global__ void k_flag_change_u32(const uint32_t *a, uint64_t n, uint32_t *flags) {
+  GRID_STRIDE(i, n) flags[i] = (i == 0 || a[i] != a[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_touched(const uint32_t *leaf, const uint32_t *flags, const uint32_t *pos, uint64_t k,
+                          uint32_t *T, uint32_t *Tstart, uint32_t nt) {
+  GRID_STRIDE(i, k) if (flags[i]) {
+    T[pos[i]] = leaf[i];
+    Tstart[pos[i]] = static_cast<uint32_t>(i);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) Tstart[nt] = static_cast<uint32_t>(k);
+}
+
+__global__ void k_climb(GView g, Bounds bd, const uint32_t *T, const uint32_t *Tstart, uint32_t nt,
+                        uint8_t *mark, uint32_t *chosen) {
+  GRID_STRIDE(t, nt) {
+    uint32_t f = T[t];
+    uint32_t L = g.H;
+    for (uint32_t l = 0; l <= g.H; l++) {
+      uint64_t lo = (static_cast<uint64_t>(f) >> l) << l;
+      uint64_t hi = lo + (1ull << l);
+      uint64_t a = lower_bound_dev<uint32_t>(T, nt, static_cast<uint32_t>(lo));
+      uint64_t b = hi >= (1ull << g.H) ? nt : lower_bound_dev<uint32_t>(T, nt, static_cast<uint32_t>(hi));
+      uint64_t newc = Tstart[b] - Tstart[a];
+      uint64_t old = g.pcnt[tree_level_off(g.H, l) + (f >> l)];
+      if (old + newc <= bd.up[l]) {
+        L = l;
+        break;
+      }
+    }
+    mark[tree_level_off(g.H, L) + (f >> L)] = 1;
+    chosen[t] = L;
+  }
+}
+
+__global__ void k_cover(GView g, const uint32_t *T, uint32_t nt, const uint8_t *mark, const uint32_t *chosen,
+                        uint32_t *cover) {
+  GRID_STRIDE(t, nt) {
+    uint32_t f = T[t];
+    uint32_t L = chosen[t];
+    for (uint32_t l = g.H; l > L; l--) {
+      if (mark[tree_level_off(g.H, l) + (f >> l)]) {
+        L = l;
+        break;
+      }
+    }
+    cover[t] = L;
+  }
+}
+
+__global__ void k_owner(const uint32_t *T, uint32_t nt, const uint32_t *cover, uint32_t *flags) {
+  GRID_STRIDE(t, nt) {
+    uint32_t L = cover[t];
+    flags[t] = (t == 0 || (static_cast<uint64_t>(T[t - 1]) >> L) != (static_cast<uint64_t>(T[t]) >> L)) ? 1u : 0u;
+  }
+}
+
+// Window record: lo leaf, level, key range, old element count.
+struct Win {
+  uint64_t lo;
+  uint32_t L;
+  uint32_t kb, ke;
+  uint32_t old;
+};
+
+__global__ void k_make_windows(GView g, const uint32_t *T, const uint32_t *Tstart, uint32_t nt,
+                               const uint32_t *cover, const uint32_t *flags, const uint32_t *pos, Win *win,
+                               uint32_t *win_leaves) {
+  GRID_STRIDE(t, nt) if (flags[t]) {
+    uint32_t L = cover[t];
+    uint64_t lo = (static_cast<uint64_t>(T[t]) >> L) << L;
+    uint64_t hi = lo + (1ull << L);
+    uint64_t b = hi >= (1ull << g.H) ? nt : lower_bound_dev<uint32_t>(T, nt, static_cast<uint32_t>(hi));
+    Win w;
+    w.lo = lo;
+    w.L = L;
+    w.kb = Tstart[t];
+    w.ke = Tstart[b];
+    w.old = g.pcnt[tree_level_off(g.H, L) + (lo >> L)];
+    win[pos[t]] = w;
+    win_leaves[pos[t]] = 1u << L;
+  }
+}
+
+__global__ void k_clear_marks(GView g, const uint32_t *T, uint32_t nt, const uint32_t *chosen, uint8_t *mark) {
+  GRID_STRIDE(t, nt) {
+    uint32_t L = chosen[t];
+    mark[tree_level_off(g.H, L) + (T[t] >> L)] = 0;
+  }
+}
+
+__device__ inline uint32_t task_window(const uint32_t *task_base, uint32_t nw, uint64_t j) {
+  // last w with task_base[w] <= j
+  uint32_t lo = 0, hi = nw;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (task_base[mid] <= j) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo - 1;
+}
+
+__global__ void k_task_counts(GView g, const Win *win, const uint32_t *task_base, uint32_t nw, uint64_t ntasks,
+                              uint32_t *taskcnt) {
+  GRID_STRIDE(j, ntasks) {
+    uint32_t w = task_window(task_base, nw, j);
+    uint64_t leaf = win[w].lo + (j - task_base[w]);
+    taskcnt[j] = g.pcnt[leaf];
+  }
+}
+
+// One wave per leaf task: compact the leaf's elements into tmp_old and drop
+// begin pointers that point into the window (they are re-derived by the
+// fill kernel with atomicMin).
+__global__ void __launch_bounds__(256) k_gather(GView g, const Win *win, const uint32_t *task_base, uint32_t nw,
+                                                uint64_t ntasks, const uint32_t *taskoff, uint64_t *tmp_old,
+                                                uint64_t *err) {
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint32_t lane = threadIdx.x & 63;
+  for (uint64_t j = wave; j < ntasks; j += nwaves) {
+    uint32_t w = task_window(task_base, nw, j);
+    uint64_t leaf = win[w].lo + (j - task_base[w]);
+    uint64_t slot = (leaf << kLeafShift) + lane;
+    uint32_t d = g.pdst[slot];
+    bool v = d != kEmpty;
+    uint64_t m = __ballot(v);
+    if (lane == 0 && static_cast<uint32_t>(__popcll(m)) != g.pcnt[leaf])
+      atomicOr(reinterpret_cast<unsigned long long *>(err), 2ull);
+    if (v) {
+      uint32_t s = g.psrc[slot];
+      uint32_t r = __popcll(m & lanemask_lt());
+      tmp_old[taskoff[j] + r] = make_key(s, d);
+      if (g.vb[s].aux == slot) g.vb[s].aux = kPmaNone;
+    }
+  }
+}
+
+// One wave per leaf task: even redistribution of merge(old, new) over the
+// window (count per leaf q or q+1, extras to the first leaves, packed
+// left, as redistribute does in src/PMA.cc:419-544), fences, count tree.
+__global__ void __launch_bounds__(256) k_fill(GView g, const Win *win, const uint32_t *task_base, uint32_t nw,
+                                              uint64_t ntasks, const uint32_t *taskoff, const uint64_t *tmp_old,
+                                              const uint64_t *keys) {
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint32_t lane = threadIdx.x & 63;
+  for (uint64_t j = wave; j < ntasks; j += nwaves) {
+    uint32_t w = task_window(task_base, nw, j);
+    Win W = win[w];
+    uint64_t li = j - task_base[w];
+    uint64_t leaf = W.lo + li;
+    uint64_t na = W.old, nb = W.ke - W.kb;
+    const uint64_t *A = tmp_old + taskoff[task_base[w]];
+    const uint64_t *B = keys + W.kb;
+    uint64_t tot = na + nb;
+    uint64_t q = tot >> W.L, rem = tot & ((1ull << W.L) - 1);
+    uint64_t c = q + (li < rem ? 1 : 0);
+    uint64_t m0 = li * q + (li < rem ? li : rem);
+    uint64_t slot = (leaf << kLeafShift) + lane;
+    KeyArr AA{A}, BB{B};
+    if (lane < c) {
+      uint64_t m = m0 + lane;
+      uint64_t i = merge_split(AA, na, BB, nb, m);
+      uint64_t jb = m - i;
+      uint64_t e = (i < na && (jb >= nb || A[i] < B[jb])) ? A[i] : B[jb];
+      // the element before e is the larger of the last taken from A and B
+      bool has_prev = m > 0;
+      uint64_t prev = 0;
+      if (i > 0) prev = A[i - 1];
+      if (jb > 0 && (i == 0 || B[jb - 1] > prev)) prev = B[jb - 1];
+      uint32_t s = key_src(e);
+      g.pdst[slot] = key_dst(e);
+      g.psrc[slot] = s;
+      if (!has_prev || key_src(prev) != s)
+        atomicMin(reinterpret_cast<unsigned long long *>(&g.vb[s].aux), static_cast<unsigned long long>(slot));
+      if (lane == 0) g.fence[leaf] = e;
+    } else {
+      g.pdst[slot] = kEmpty;
+    }
+    if (lane == 0) {
+      g.pcnt[leaf] = static_cast<uint32_t>(c);
+      if (c == 0) {
+        // empty leaf: fence just above the window's last key
+        uint64_t m = tot - 1;
+        uint64_t i = merge_split(AA, na, BB, nb, m);
+        uint64_t jb = m - i;
+        uint64_t e = (i < na && (jb >= nb || A[i] < B[jb])) ? A[i] : B[jb];
+        g.fence[leaf] = e + 1;
+      }
+      // internal nodes of the count tree inside the window, closed form
+      for (uint32_t l = 1; l <= W.L; l++) {
+        uint64_t span = 1ull << l;
+        if (li & (span - 1)) break;
+        uint64_t a = li, b = li + span;
+        uint64_t cnt = span * q + ((b < rem ? b : rem) - (a < rem ? a : rem));
+        g.pcnt[tree_level_off(g.H, l) + (leaf >> l)] = static_cast<uint32_t>(cnt);
+      }
+      if (li == 0) {
+        for (uint32_t l = W.L + 1; l <= g.H; l++)
+          atomicAdd(&g.pcnt[tree_level_off(g.H, l) + (W.lo >> l)], static_cast<uint32_t>(nb));
+      }
+    }
+  }
+}
+
+// PMA delete: clear the slot, decrement the count tree, flag begin fixes.
+__global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t k, uint64_t *fix_slot, uint64_t *err) {
+  GRID_STRIDE(i, k) {
+    uint32_t s = key_src(keys[i]), d = key_dst(keys[i]);
+    uint64_t slot = pma_slot_of(g, s, d);
+    fix_slot[i] = ~0ull;
+    if (slot == ~0ull) {
+      atomicOr(reinterpret_cast<unsigned long long *>(err), 4ull);
+      continue;
+    }
+    g.pdst[slot] = kEmpty;
+    uint64_t leaf = slot >> kLeafShift;
+    for (uint32_t l = 0; l <= g.H; l++) atomicSub(&g.pcnt[tree_level_off(g.H, l) + (leaf >> l)], 1u);
+    if (g.vb[s].aux == slot) fix_slot[i] = slot;
+  }
+}
+
+// New begin for vertices whose first PMA element was deleted: the next
+// occupied slot if it still belongs to the vertex.
+__global__ void k_pma_fix_begin(GView g, const uint64_t *keys, uint64_t k, const uint64_t *fix_slot) {
+  GRID_STRIDE(i, k) {
+    uint64_t slot = fix_slot[i];
+    if (slot == ~0ull) continue;
+    uint32_t s = key_src(keys[i]);
+    uint64_t p = slot + 1;
+    uint64_t nb = kPmaNone;
+    while (p < g.N) {
+      if ((p & (kLeafSlots - 1)) == 0 && g.pcnt[p >> kLeafShift] == 0) {
+        p += kLeafSlots;
+        continue;
+      }
+      if (g.pdst[p] != kEmpty) {
+        if (g.psrc[p] == s) nb = p;
+        break;
+      }
+      p++;
+    }
+    g.vb[s].aux = nb;
+  }
+}
+
+// ------------------------------------------------------------ promotion
+// One wave per promoted vertex: copy its PMA run (D_old - 13 elements
+// from the begin slot) into ext[ext_off ..].
+__global__ void __launch_bounds__(256) k_extract_pma(GView g, const uint32_t *pv, const uint32_t *pcount,
+                                                     const uint32_t *ext_off, uint32_t np, uint64_t *ext) {
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint32_t lane = threadIdx.x & 63;
+  for (uint64_t w = wave; w < np; w += nwaves) {
+    uint32_t s = pv[w];
+    uint32_t need = pcount[w];
+    if (need == 0) continue;
+    uint64_t p = g.vb[s].aux;
+    uint32_t got = 0;
+    while (got < need) {
+      uint64_t slot = p + lane;
+      uint32_t d = slot < g.N ? g.pdst[slot] : kEmpty;
+      bool v = d != kEmpty;
+      uint64_t m = __ballot(v);
+      uint32_t r = __popcll(m & lanemask_lt());
+      if (v && got + r < need) ext[ext_off[w] + got + r] = make_key(s, d);
+      got += __popcll(m);
+      p += 64;
+      if (p >= g.N) break;
+    }
+  }
+}
+
+__global__ void k_promote_info(const uint64_t *R, const uint32_t *run_beg, const uint32_t *promote,
+                               const uint32_t *ppos, uint32_t nruns, GView g, uint32_t *pv, uint32_t *pcount) {
+  GRID_STRIDE(r, nruns) if (promote[r]) {
+    uint32_t s = key_src(R[run_beg[r]]);
+    pv[ppos[r]] = s;
+    // degree was already raised by this batch's new keys; the PMA run
+    // holds the old secondary count (old degree - 13).
+    pcount[ppos[r]] = 0;
+  }
+}
+
+__global__ void k_promote_old_counts(const uint32_t *pv, uint32_t np, const uint32_t *old_deg, uint32_t *pcount) {
+  GRID_STRIDE(i, np) {
+    uint32_t D = old_deg[i];
+    pcount[i] = D > kInline ? D - kInline : 0;
+  }
+}
+
+__global__ void k_set_tree(GView g, const uint32_t *pv, uint32_t np, uint32_t first_tree, uint32_t *tvtx) {
+  GRID_STRIDE(i, np) {
+    uint32_t t = first_tree + static_cast<uint32_t>(i);
+    g.vb[pv[i]].aux = kAuxTree | t;
+    g.tnblk[t] = 0;
+    g.toff[t] = 0;
+    tvtx[t] = pv[i];
+  }
+}
+
+// ------------------------------------------------------------ tree kernels
+__global__ void k_tree_loc(GView g, const uint64_t *keys, uint64_t k, uint64_t *gj) {
+  GRID_STRIDE(i, k) {
+    uint32_t s = key_src(keys[i]), d = key_dst(keys[i]);
+    uint32_t t = static_cast<uint32_t>(g.vb[s].aux);
+    uint32_t j = g.tnblk[t] == 0 ? 0 : tree_block_pos(g, t, d);
+    gj[i] = (static_cast<uint64_t>(t) << 32) | j;
+  }
+}
+
+__global__ void k_flag_change_u64(const uint64_t *a, uint64_t n, uint32_t *flags) {
+  GRID_STRIDE(i, n) flags[i] = (i == 0 || a[i] != a[i - 1]) ? 1u : 0u;
+}
+
+struct Grp {
+  uint32_t t, j;
+  uint32_t kb, ke;
+  uint32_t old_b;  // ~0 when the tree is empty
+  uint32_t ocnt;
+  uint32_t ob;     // output blocks
+  uint32_t has_old;
+};
+
+__global__ void k_make_groups(GView g, const uint64_t *gj, const uint32_t *flags, const uint32_t *pos, uint64_t k,
+                              uint32_t ng, Grp *grp, uint32_t *grp_kb) {
+  GRID_STRIDE(i, k) if (flags[i]) {
+    uint32_t x = pos[i];
+    grp_kb[x] = static_cast<uint32_t>(i);
+    Grp G;
+    G.t = static_cast<uint32_t>(gj[i] >> 32);
+    G.j = static_cast<uint32_t>(gj[i]);
+    G.kb = static_cast<uint32_t>(i);
+    grp[x] = G;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) grp_kb[ng] = static_cast<uint32_t>(k);
+}
+
+// insert=true: total = old + new; delete: total = old - deleted.
+__global__ void k_group_sizes(GView g, Grp *grp, const uint32_t *grp_kb, uint32_t ng, bool insert, uint32_t *ob,
+                              uint32_t *has_old) {
+  GRID_STRIDE(x, ng) {
+    Grp G = grp[x];
+    G.ke = grp_kb[x + 1];
+    uint32_t nb = g.tnblk[G.t];
+    G.has_old = nb > 0 ? 1u : 0u;
+    G.old_b = nb > 0 ? g.iblk[g.toff[G.t] + G.j] : ~0u;
+    G.ocnt = nb > 0 ? g.bcnt[G.old_b] : 0u;
+    uint32_t kcount = G.ke - G.kb;
+    uint32_t total = insert ? G.ocnt + kcount : G.ocnt - kcount;
+    if (insert) G.ob = total <= kBlk ? 1u : (total + kBlkFill - 1) / kBlkFill;
+    else G.ob = total > 0 ? 1u : 0u;
+    grp[x] = G;
+    ob[x] = G.ob;
+    has_old[x] = G.has_old;
+  }
+}
+
+__device__ inline uint32_t find_group_of_block(const uint32_t *obpre, uint32_t ng, uint64_t x) {
+  uint32_t lo = 0, hi = ng;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (obpre[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo - 1;
+}
+
+// One 256-thread block per output leaf block of a tree insert.
+__global__ void __launch_bounds__(256) k_tree_fill_insert(GView g, const Grp *grp, const uint32_t *obpre,
+                                                          uint32_t ng, uint64_t nout, uint64_t blk_base,
+                                                          const uint64_t *keys) {
+  for (uint64_t x = blockIdx.x; x < nout; x += gridDim.x) {
+    // groups with ob == 0 never occur on insert, so the search is exact
+    uint32_t gi = find_group_of_block(obpre, ng, x);
+    Grp G = grp[gi];
+    uint64_t bi = x - obpre[gi];
+    uint64_t na = G.ocnt, nb = G.ke - G.kb;
+    uint64_t tot = na + nb;
+    uint64_t q = tot / G.ob, rem = tot % G.ob;
+    uint64_t c = q + (bi < rem ? 1 : 0);
+    uint64_t m0 = bi * q + (bi < rem ? bi : rem);
+    const uint32_t *A = G.has_old ? g.bkeys + static_cast<uint64_t>(G.old_b) * kBlk : nullptr;
+    U32Arr AA{A};
+    DstArr BB{keys + G.kb};
+    uint64_t nbk = blk_base + x;
+    for (uint64_t e = threadIdx.x; e < c; e += blockDim.x) {
+      uint64_t m = m0 + e;
+      uint64_t i = merge_split(AA, na, BB, nb, m);
+      uint64_t jb = m - i;
+      uint32_t v = (i < na && (jb >= nb || A[i] < BB(jb))) ? A[i] : BB(jb);
+      g.bkeys[nbk * kBlk + e] = v;
+    }
+    if (threadIdx.x == 0) g.bcnt[nbk] = static_cast<uint32_t>(c);
+  }
+}
+
+// One 256-thread block per group of a tree delete (groups with ob == 1).
+__global__ void __launch_bounds__(256) k_tree_fill_delete(GView g, const Grp *grp, const uint32_t *obpre,
+                                                          uint32_t ng, uint64_t blk_base, const uint64_t *keys) {
+  __shared__ uint32_t wsum[4];
+  for (uint64_t x = blockIdx.x; x < ng; x += gridDim.x) {
+    Grp G = grp[x];
+    if (G.ob == 0) continue;
+    const uint32_t *A = g.bkeys + static_cast<uint64_t>(G.old_b) * kBlk;
+    uint64_t nbk = blk_base + obpre[x];
+    uint32_t e = threadIdx.x;
+    bool keep = false;
+    uint32_t v = 0;
+    if (e < G.ocnt) {
+      v = A[e];
+      uint32_t lo = G.kb, hi = G.ke;
+      while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (key_dst(keys[mid]) < v) lo = mid + 1;
+        else hi = mid;
+      }
+      keep = !(lo < G.ke && key_dst(keys[lo]) == v);
+    }
+    uint64_t m = __ballot(keep);
+    uint32_t r = __popcll(m & lanemask_lt());
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) base += wsum[w];
+    if (keep) g.bkeys[nbk * kBlk + base + r] = v;
+    if (threadIdx.x == 0) g.bcnt[nbk] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+}
+
+struct TreeRun {
+  uint32_t t;
+  uint32_t g0, g1;
+  uint32_t nb_old;
+  uint64_t new_off;
+  uint32_t nb_new;
+};
+
+__global__ void k_tree_runs(GView g, const Grp *grp, const uint32_t *flags, const uint32_t *pos, uint32_t ng,
+                            uint32_t ntr, TreeRun *tr, uint32_t *tr_start) {
+  GRID_STRIDE(x, ng) if (flags[x]) {
+    tr_start[pos[x]] = static_cast<uint32_t>(x);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) tr_start[ntr] = ng;
+}
+
+__global__ void k_tree_run_sizes(GView g, const Grp *grp, const uint32_t *tr_start, uint32_t ntr,
+                                 const uint32_t *obpre, const uint32_t *hopre, TreeRun *tr, uint32_t *nb_new,
+                                 uint32_t *nb_old) {
+  GRID_STRIDE(r, ntr) {
+    TreeRun R;
+    R.g0 = tr_start[r];
+    R.g1 = tr_start[r + 1];
+    R.t = grp[R.g0].t;
+    R.nb_old = g.tnblk[R.t];
+    int64_t delta = static_cast<int64_t>(obpre[R.g1] - obpre[R.g0]) - static_cast<int64_t>(hopre[R.g1] - hopre[R.g0]);
+    R.nb_new = static_cast<uint32_t>(static_cast<int64_t>(R.nb_old) + delta);
+    R.new_off = 0;
+    tr[r] = R;
+    nb_new[r] = R.nb_new;
+    nb_old[r] = R.nb_old;
+  }
+}
+
+__device__ inline uint32_t find_run(const uint32_t *pre, uint32_t n, uint64_t j) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= j) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo - 1;
+}
+
+// Copy the untouched entries of each touched tree's index into its new
+// segment, shifted by the blocks that earlier groups added or removed.
+__global__ void k_tree_idx_copy(GView g, const TreeRun *tr, const uint32_t *oldpre, uint32_t ntr, uint64_t ntask,
+                                const Grp *grp, const uint32_t *obpre, const uint32_t *hopre,
+                                const uint32_t *newpre, uint64_t idx_base) {
+  GRID_STRIDE(j, ntask) {
+    uint32_t r = find_run(oldpre, ntr, j);
+    TreeRun R = tr[r];
+    uint32_t e = static_cast<uint32_t>(j - oldpre[r]);
+    // first group of this tree with grp.j >= e
+    uint32_t lo = R.g0, hi = R.g1;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (grp[mid].j < e) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < R.g1 && grp[lo].j == e) continue;  // rewritten by its group
+    int64_t shift = static_cast<int64_t>(obpre[lo] - obpre[R.g0]) - static_cast<int64_t>(hopre[lo] - hopre[R.g0]);
+    uint64_t dst = idx_base + newpre[r] + static_cast<uint64_t>(static_cast<int64_t>(e) + shift);
+    uint64_t src = g.toff[R.t] + e;
+    g.iblk[dst] = g.iblk[src];
+    g.imin[dst] = g.imin[src];
+  }
+}
+
+__global__ void k_tree_idx_groups(GView g, const TreeRun *tr, uint32_t ntr, const Grp *grp,
+                                  const uint32_t *grp_run, uint32_t ng, const uint32_t *obpre,
+                                  const uint32_t *hopre, const uint32_t *newpre, uint64_t idx_base,
+                                  uint64_t blk_base) {
+  GRID_STRIDE(x, ng) {
+    Grp G = grp[x];
+    TreeRun R = tr[grp_run[x]];
+    int64_t shift = static_cast<int64_t>(obpre[x] - obpre[R.g0]) - static_cast<int64_t>(hopre[x] - hopre[R.g0]);
+    for (uint32_t b = 0; b < G.ob; b++) {
+      uint64_t nbk = blk_base + obpre[x] + b;
+      uint64_t dst = idx_base + newpre[grp_run[x]] + static_cast<uint64_t>(static_cast<int64_t>(G.j) + shift + b);
+      g.iblk[dst] = static_cast<uint32_t>(nbk);
+      g.imin[dst] = g.bkeys[nbk * kBlk];
+    }
+  }
+}
+
+__global__ void k_grp_run(const uint32_t *flags, const uint32_t *pos, uint32_t ng, uint32_t *grp_run) {
+  GRID_STRIDE(x, ng) grp_run[x] = pos[x] + flags[x] - 1;
+}
+
+__global__ void k_tree_commit(GView g, const TreeRun *tr, uint32_t ntr, const uint32_t *newpre, uint64_t idx_base) {
+  GRID_STRIDE(r, ntr) {
+    TreeRun R = tr[r];
+    g.toff[R.t] = idx_base + newpre[r];
+    g.tnblk[R.t] = R.nb_new;
+  }
+}
+
+// ------------------------------------------------------------ delete batch
+// One thread per run of existing keys: remove inline hits, mark secondary
+// deletes, and record how many inline slots must be refilled.
+__global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_beg, uint32_t nruns,
+                              uint8_t *sec_route, uint32_t *run_bump) {
+  GRID_STRIDE(r, nruns) {
+    uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
+    uint32_t s = key_src(R[b0]);
+    VBlock &vb = g.vb[s];
+    uint32_t D = vb.degree;
+    uint32_t lim = D < kInline ? D : kInline;
+    uint32_t inl[kInline];
+    for (uint32_t i = 0; i < lim; i++) inl[i] = vb.nbr[i];
+    uint8_t tier = (vb.aux & kAuxTree) ? kRouteTree : kRoutePma;
+    bool hit[kInline];
+    for (uint32_t p = 0; p < kInline; p++) hit[p] = false;
+    for (uint32_t q = b0; q < b1; q++) {
+      uint32_t d = key_dst(R[q]);
+      uint8_t rt = tier;
+      for (uint32_t p = 0; p < lim; p++)
+        if (inl[p] == d) {
+          hit[p] = true;
+          rt = kRouteNone;
+        }
+      sec_route[q] = rt;
+    }
+    uint32_t keep = 0;
+    for (uint32_t p = 0; p < lim; p++)
+      if (!hit[p]) inl[keep++] = inl[p];
+    uint32_t x = lim - keep;        // inline entries deleted
+    uint32_t len = b1 - b0;
+    uint32_t sec_before = D > kInline ? D - kInline : 0;
+    uint32_t sec_after = sec_before - (len - x);
+    uint32_t y = x < sec_after ? x : sec_after;
+    for (uint32_t i = 0; i < keep; i++) vb.nbr[i] = inl[i];
+    vb.degree = D - len;
+    run_bump[r] = y;
+  }
+}
+
+// Refill inline slots from the smallest secondary neighbours (graph.h:729-780,
+// batched): thread per run with y > 0; emits the moved keys for deletion
+// from the secondary tier.
+__global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, uint32_t nruns,
+                       const uint32_t *run_bump, const uint32_t *bump_off, uint64_t *bump_keys,
+                       uint8_t *bump_route) {
+  GRID_STRIDE(r, nruns) {
+    uint32_t y = run_bump[r];
+    if (y == 0) continue;
+    uint32_t s = key_src(R[run_beg[r]]);
+    VBlock &vb = g.vb[s];
+    uint32_t D = vb.degree;  // already the post-delete degree
+    // inline currently holds min(D,13) - y entries
+    uint32_t have = (D < kInline ? D : kInline) - y;
+    uint64_t out = bump_off[r];
+    uint32_t got = 0;
+    if (vb.aux & kAuxTree) {
+      uint32_t t = static_cast<uint32_t>(vb.aux);
+      uint64_t off = g.toff[t];
+      uint32_t nb = g.tnblk[t];
+      for (uint32_t j = 0; j < nb && got < y; j++) {
+        uint32_t b = g.iblk[off + j];
+        uint32_t c = g.bcnt[b];
+        for (uint32_t e = 0; e < c && got < y; e++) {
+          uint32_t d = g.bkeys[static_cast<uint64_t>(b) * kBlk + e];
+          vb.nbr[have + got] = d;
+          bump_keys[out + got] = make_key(s, d);
+          bump_route[out + got] = kRouteTree;
+          got++;
+        }
+      }
+    } else {
+      uint64_t p = vb.aux;
+      while (got < y && p < g.N) {
+        uint32_t d = g.pdst[p];
+        if (d != kEmpty) {
+          vb.nbr[have + got] = d;
+          bump_keys[out + got] = make_key(s, d);
+          bump_route[out + got] = kRoutePma;
+          got++;
+        }
+        p++;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ queries
+__global__ void k_degrees(GView g, uint32_t *out) {
+  GRID_STRIDE(v, g.n) out[v] = g.vb[v].degree;
+}
+__global__ void k_degrees_u64(GView g, uint64_t *out) {
+  GRID_STRIDE(v, g.n) out[v] = g.vb[v].degree;
+}
+
+__global__ void k_is_edge(GView g, const uint32_t *src, const uint32_t *dst, uint64_t k, uint8_t *out) {
+  GRID_STRIDE(i, k) {
+    uint32_t s = src[i], d = dst[i];
+    out[i] = (s < g.n && d < g.n && edge_exists(g, s, d)) ? 1 : 0;
+  }
+}
+
+__global__ void k_tiers(GView g, uint8_t *out) {
+  GRID_STRIDE(v, g.n) {
+    const VBlock &b = g.vb[v];
+    out[v] = (b.aux & kAuxTree) ? 2 : (b.degree > kInline ? 1 : 0);
+  }
+}
+
+__global__ void k_flag_tiers(GView g, uint32_t *fp, uint32_t *ft) {
+  GRID_STRIDE(v, g.n) {
+    const VBlock &b = g.vb[v];
+    bool t = (b.aux & kAuxTree) != 0;
+    ft[v] = t ? 1u : 0u;
+    fp[v] = (!t && b.degree > kInline) ? 1u : 0u;
+  }
+}
+
+__global__ void k_scatter_ids(const uint32_t *flags, const uint32_t *pos, uint64_t n, uint32_t *out) {
+  GRID_STRIDE(i, n) if (flags[i]) out[pos[i]] = static_cast<uint32_t>(i);
+}
+
+__global__ void k_export_inline(GView g, const uint64_t *off, uint32_t *nbrs) {
+  GRID_STRIDE(v, g.n) {
+    const VBlock &b = g.vb[v];
+    uint32_t lim = b.degree < kInline ? b.degree : kInline;
+    uint64_t o = off[v];
+    for (uint32_t i = 0; i < lim; i++) nbrs[o + i] = b.nbr[i];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_export_pma(GView g, const uint32_t *list, uint32_t cnt, const uint64_t *off,
+                                                    uint32_t *nbrs) {
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint32_t lane = threadIdx.x & 63;
+  for (uint64_t w = wave; w < cnt; w += nwaves) {
+    uint32_t v = list[w];
+    const VBlock &b = g.vb[v];
+    uint32_t need = b.degree - kInline;
+    uint64_t p = b.aux;
+    uint64_t o = off[v] + kInline;
+    uint32_t got = 0;
+    while (got < need && p < g.N) {
+      uint64_t slot = p + lane;
+      uint32_t d = slot < g.N ? g.pdst[slot] : kEmpty;
+      bool ok = d != kEmpty;
+      uint64_t m = __ballot(ok);
+      uint32_t r = __popcll(m & lanemask_lt());
+      if (ok && got + r < need) nbrs[o + got + r] = d;
+      got += __popcll(m);
+      p += 64;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_export_tree(GView g, const uint32_t *list, uint32_t cnt,
+                                                     const uint64_t *off, uint32_t *nbrs) {
+  for (uint64_t w = blockIdx.x; w < cnt; w += gridDim.x) {
+    uint32_t v = list[w];
+    const VBlock &b = g.vb[v];
+    uint32_t t = static_cast<uint32_t>(b.aux);
+    uint64_t base = off[v] + kInline;
+    uint64_t io = g.toff[t];
+    uint32_t nb = g.tnblk[t];
+    for (uint32_t j = 0; j < nb; j++) {
+      uint32_t blk = g.iblk[io + j];
+      uint32_t c = g.bcnt[blk];
+      for (uint32_t e = threadIdx.x; e < c; e += blockDim.x) nbrs[base + e] = g.bkeys[static_cast<uint64_t>(blk) * kBlk + e];
+      base += c;
+    }
+  }
+}
+
+// One wave: sorted neighbours of vertex v (inline, then PMA run or tree).
+__global__ void k_neighbors_one(GView g, uint32_t v, uint32_t *out) {
+  uint32_t lane = threadIdx.x;
+  const VBlock &b = g.vb[v];
+  uint32_t D = b.degree;
+  uint32_t lim = D < kInline ? D : kInline;
+  if (lane < lim) out[lane] = b.nbr[lane];
+  if (D <= kInline) return;
+  uint32_t need = D - kInline;
+  uint32_t *o = out + kInline;
+  if (b.aux & kAuxTree) {
+    uint32_t t = static_cast<uint32_t>(b.aux);
+    uint64_t io = g.toff[t];
+    uint32_t nb = g.tnblk[t];
+    uint64_t base = 0;
+    for (uint32_t j = 0; j < nb; j++) {
+      uint32_t blk = g.iblk[io + j];
+      uint32_t c = g.bcnt[blk];
+      for (uint32_t e = lane; e < c; e += 64) o[base + e] = g.bkeys[static_cast<uint64_t>(blk) * kBlk + e];
+      base += c;
+    }
+    return;
+  }
+  uint64_t p = b.aux;
+  uint32_t got = 0;
+  while (got < need && p < g.N) {
+    uint64_t slot = p + lane;
+    uint32_t d = slot < g.N ? g.pdst[slot] : kEmpty;
+    bool ok = d != kEmpty;
+    uint64_t m = __ballot(ok);
+    uint32_t r = __popcll(m & lanemask_lt());
+    if (ok && got + r < need) o[got + r] = d;
+    got += __popcll(m);
+    p += 64;
+  }
+}
+
+__global__ void k_init_aux(VBlock *vb, uint32_t n) {
+  GRID_STRIDE(v, n) vb[v].aux = kPmaNone;
+}
+
+__global__ void k_old_degrees(GView g, const uint64_t *R, const uint32_t *run_beg, uint32_t nruns, uint32_t *od) {
+  GRID_STRIDE(r, nruns) od[r] = g.vb[key_src(R[run_beg[r]])].degree;
+}
+
+__global__ void k_gather_u32(const uint32_t *a, const uint32_t *flags, const uint32_t *pos, uint32_t n,
+                             uint32_t *out) {
+  GRID_STRIDE(i, n) if (flags[i]) out[pos[i]] = a[i];
+}
+
+__global__ void k_flag_tree_change(const Grp *grp, uint32_t ng, uint32_t *f) {
+  GRID_STRIDE(x, ng) f[x] = (x == 0 || grp[x].t != grp[x - 1].t) ? 1u : 0u;
+}
+
+}  // namespace
+
+// ===================================================================== host
+Graph::Graph(uint32_t n) : n_(n) {
+  if (n == 0 || n > 0x7FFFFFFFu) throw Error(-1, "vertex count must be in [1, 2^31)");
+  TG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  vb_.ensure(n, stream_);
+  TG_HIP(hipMemsetAsync(vb_.get(), 0, sizeof(VBlock) * static_cast<uint64_t>(n), stream_));
+  k_init_aux<<<grid_for(n, 256), 256, 0, stream_>>>(vb_.get(), n);
+  TG_CHECK_LAUNCH();
+  scal_.ensure(8, stream_);
+  init_pma(2);
+  ensure_trees(16);
+  TG_HIP(hipStreamSynchronize(stream_));
+}
+
+Graph::~Graph() {
+  if (stream_) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamDestroy(stream_);
+  }
+}
+
+GView Graph::view() const {
+  GView g;
+  g.vb = vb_.get();
+  g.n = n_;
+  g.pdst = pdst_.get();
+  g.psrc = psrc_.get();
+  g.N = pma_N_;
+  g.H = pma_H_;
+  g.pcnt = pcnt_.get();
+  g.fence = fence_.get();
+  g.bkeys = bkeys_.get();
+  g.bcnt = bcnt_.get();
+  g.toff = toff_.get();
+  g.tnblk = tnblk_.get();
+  g.iblk = iblk_.get();
+  g.imin = imin_.get();
+  return g;
+}
+
+void Graph::compute_bounds() {
+  // density_bound (src/PMA.cc:382-394) with depth = H - l:
+  //   upper = 0.75 + 0.25 * depth / H, capped at (L-1)/L - 0.001
+  //   lower = 0.25 - 0.125 * depth / H
+  const double limit = static_cast<double>(kLeafSlots - 1) / kLeafSlots;
+  for (uint32_t l = 0; l <= pma_H_ && l < 48; l++) {
+    double depth = static_cast<double>(pma_H_ - l);
+    double H = pma_H_ ? static_cast<double>(pma_H_) : 1.0;
+    double up = 0.75 + 0.25 * depth / H;
+    if (up > limit) up = limit - 0.001;
+    double lo = 0.25 - 0.125 * depth / H;
+    double slots = static_cast<double>(kLeafSlots) * std::ldexp(1.0, static_cast<int>(l));
+    bounds_.up[l] = static_cast<uint64_t>(std::floor(up * slots));
+    bounds_.lo[l] = static_cast<uint64_t>(std::ceil(lo * slots));
+  }
+}
+
+void Graph::init_pma(uint64_t nleaves) {
+  uint32_t H = 0;
+  while ((1ull << H) < nleaves) H++;
+  nleaves = 1ull << H;
+  pma_H_ = H;
+  pma_N_ = nleaves * kLeafSlots;
+  pdst_.ensure(pma_N_, stream_);
+  psrc_.ensure(pma_N_, stream_);
+  pcnt_.ensure(2 * nleaves, stream_);
+  fence_.ensure(nleaves, stream_);
+  mark_.ensure(2 * nleaves, stream_);
+  fill_u32(pdst_.get(), kEmpty, pma_N_, stream_);
+  fill_u32(pcnt_.get(), 0, 2 * nleaves, stream_);
+  fill_u64(fence_.get(), 0, nleaves, stream_);
+  TG_HIP(hipMemsetAsync(mark_.get(), 0, 2 * nleaves, stream_));
+  compute_bounds();
+}
+
+void Graph::ensure_blocks(uint64_t need) {
+  if (need <= bcnt_.cap) return;
+  uint64_t want = need + need / 2 + 64;
+  DevBuf<uint32_t> nk, nc;
+  nk.ensure(want * kBlk, stream_);
+  nc.ensure(want, stream_);
+  if (blk_top_) {
+    TG_HIP(hipMemcpyAsync(nk.get(), bkeys_.get(), blk_top_ * kBlk * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
+    TG_HIP(hipMemcpyAsync(nc.get(), bcnt_.get(), blk_top_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
+  }
+  TG_HIP(hipStreamSynchronize(stream_));
+  std::swap(bkeys_.ptr, nk.ptr);
+  std::swap(bkeys_.cap, nk.cap);
+  std::swap(bcnt_.ptr, nc.ptr);
+  std::swap(bcnt_.cap, nc.cap);
+}
+
+void Graph::ensure_index(uint64_t need) {
+  if (need <= iblk_.cap) return;
+  uint64_t want = need + need / 2 + 64;
+  DevBuf<uint32_t> nb, nm;
+  nb.ensure(want, stream_);
+  nm.ensure(want, stream_);
+  if (idx_top_) {
+    TG_HIP(hipMemcpyAsync(nb.get(), iblk_.get(), idx_top_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
+    TG_HIP(hipMemcpyAsync(nm.get(), imin_.get(), idx_top_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
+  }
+  TG_HIP(hipStreamSynchronize(stream_));
+  std::swap(iblk_.ptr, nb.ptr);
+  std::swap(iblk_.cap, nb.cap);
+  std::swap(imin_.ptr, nm.ptr);
+  std::swap(imin_.cap, nm.cap);
+}
+
+void Graph::ensure_trees(uint64_t need) {
+  if (need <= tnblk_.cap) return;
+  uint64_t want = need + need / 2 + 16;
+  DevBuf<uint64_t> no;
+  DevBuf<uint32_t> nn, nv;
+  no.ensure(want, stream_);
+  nn.ensure(want, stream_);
+  nv.ensure(want, stream_);
+  if (ntrees_) {
+    TG_HIP(hipMemcpyAsync(no.get(), toff_.get(), ntrees_ * sizeof(uint64_t), hipMemcpyDeviceToDevice, stream_));
+    TG_HIP(hipMemcpyAsync(nn.get(), tnblk_.get(), ntrees_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
+    TG_HIP(hipMemcpyAsync(nv.get(), tvtx_.get(), ntrees_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
+  }
+  TG_HIP(hipStreamSynchronize(stream_));
+  std::swap(toff_.ptr, no.ptr);
+  std::swap(toff_.cap, no.cap);
+  std::swap(tnblk_.ptr, nn.ptr);
+  std::swap(tnblk_.cap, nn.cap);
+  std::swap(tvtx_.ptr, nv.ptr);
+  std::swap(tvtx_.cap, nv.cap);
+}
+
+// Compact in[flags] into out; returns the count (synchronises).
+uint64_t Graph::compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out) {
+  if (n == 0) return 0;
+  u4_.ensure(n + 1, stream_);
+  exclusive_scan_u32(flags, u4_.get(), n, u4_.get() + n, scan_scratch, stream_);
+  k_scatter_flagged<<<grid_for(n, 256), 256, 0, stream_>>>(in, flags, u4_.get(), n, out);
+  TG_CHECK_LAUNCH();
+  return read_scalar(u4_.get() + n, stream_);
+}
+
+// Pack, validate, sort and unique the batch into k0_; returns the count.
+uint64_t Graph::prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t count, uint64_t *&keys) {
+  k0_.ensure(count, stream_);
+  k1_.ensure(count, stream_);
+  f0_.ensure(count + 1, stream_);
+  fill_u64(scal_.get(), 0, 1, stream_);
+  k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), scal_.get());
+  TG_CHECK_LAUNCH();
+  if (read_scalar(scal_.get(), stream_)) throw Error(-1, "vertex id out of range");
+  uint32_t bits = 1;
+  while (bits < 32 && (1ull << bits) < n_) bits++;
+  radix_sort_keys(k0_.get(), count, bits, bits, sort_, stream_);
+  k_flag_unique<<<grid_for(count, 256), 256, 0, stream_>>>(k0_.get(), count, f0_.get());
+  TG_CHECK_LAUNCH();
+  uint64_t m = compact_keys(k0_.get(), f0_.get(), count, k1_.get());
+  keys = k1_.get();
+  return m;
+}
+
+// ------------------------------------------------------------ PMA insert
+void Graph::pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total) {
+  // gather every old element, grow, spread merge(old, keys) over the array
+  uint64_t old_total = total - k;
+  tmp_old_.ensure(old_total + 1, stream_);
+  GView ov = view();
+  uint64_t onl = 1ull << pma_H_;
+  // one window covering the whole old array
+  DevBuf<uint8_t> winbuf;
+  winbuf.ensure(sizeof(Win) + 64, stream_);
+  Win w0{0, pma_H_, 0, static_cast<uint32_t>(k), static_cast<uint32_t>(old_total)};
+  TG_HIP(hipMemcpyAsync(winbuf.get(), &w0, sizeof(Win), hipMemcpyHostToDevice, stream_));
+  TG_HIP(hipStreamSynchronize(stream_));
+  u0_.ensure(onl + 1, stream_);
+  u1_.ensure(2, stream_);
+  fill_u32(u1_.get(), 0, 1, stream_);  // task_base = {0}
+  const Win *dw = reinterpret_cast<const Win *>(winbuf.get());
+  k_task_counts<<<grid_for(onl, 256), 256, 0, stream_>>>(ov, dw, u1_.get(), 1, onl, u0_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(u0_.get(), u0_.get(), onl, u0_.get() + onl, scan_scratch, stream_);
+  fill_u64(scal_.get(), 0, 1, stream_);
+  k_gather<<<grid_for(onl * 64, 256), 256, 0, stream_>>>(ov, dw, u1_.get(), 1, onl, u0_.get(), tmp_old_.get(),
+                                                         scal_.get());
+  TG_CHECK_LAUNCH();
+  if (read_scalar(scal_.get(), stream_)) throw Error(-3, "PMA count tree out of sync (gather)");
+  // new geometry: density about 0.5 after the rebuild
+  uint64_t nleaves = 2;
+  while (nleaves * kLeafSlots / 2 < total) nleaves <<= 1;
+  DevBuf<uint32_t> keep_dst, keep_src;  // old arrays released after init
+  std::swap(keep_dst.ptr, pdst_.ptr);
+  std::swap(keep_dst.cap, pdst_.cap);
+  std::swap(keep_src.ptr, psrc_.ptr);
+  std::swap(keep_src.cap, psrc_.cap);
+  pcnt_.release();
+  fence_.release();
+  mark_.release();
+  init_pma(nleaves);
+  GView nv = view();
+  Win w1{0, pma_H_, 0, static_cast<uint32_t>(k), static_cast<uint32_t>(old_total)};
+  TG_HIP(hipMemcpyAsync(winbuf.get(), &w1, sizeof(Win), hipMemcpyHostToDevice, stream_));
+  TG_HIP(hipStreamSynchronize(stream_));
+  uint64_t nl = 1ull << pma_H_;
+  // taskoff for the single window starts at 0: reuse u0_[0] == 0
+  k_fill<<<grid_for(nl * 64, 256), 256, 0, stream_>>>(nv, dw, u1_.get(), 1, nl, u0_.get(), tmp_old_.get(), keys);
+  TG_CHECK_LAUNCH();
+  TG_HIP(hipStreamSynchronize(stream_));
+  stats_.pma_rebuilds++;
+}
+
+void Graph::pma_insert(const uint64_t *keys, uint64_t k) {
+  if (k == 0) return;
+  if (k > 0xFFFFFFFFull) throw Error(-1, "batch too large");
+  GView g = view();
+  uint64_t nl = 1ull << pma_H_;
+  uint64_t root_old = read_scalar(pcnt_.get() + tree_level_off(pma_H_, pma_H_), stream_);
+  uint64_t total = root_old + k;
+  if (total > bounds_.up[pma_H_]) {
+    pma_rebuild(keys, k, total);
+    return;
+  }
+  u0_.ensure(k + 1, stream_);  // leaf of key
+  u1_.ensure(k + 1, stream_);  // flags
+  u2_.ensure(k + 1, stream_);  // positions
+  k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get());
+  TG_CHECK_LAUNCH();
+  k_flag_change_u32<<<grid_for(k, 256), 256, 0, stream_>>>(u0_.get(), k, u1_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(u1_.get(), u2_.get(), k, u2_.get() + k, scan_scratch, stream_);
+  uint32_t nt = read_scalar(u2_.get() + k, stream_);
+  DevBuf<uint32_t> &T = f1_, &Tstart = f2_, &chosen = f3_;
+  T.ensure(nt + 1, stream_);
+  Tstart.ensure(nt + 1, stream_);
+  chosen.ensure(nt + 1, stream_);
+  k_touched<<<grid_for(k, 256), 256, 0, stream_>>>(u0_.get(), u1_.get(), u2_.get(), k, T.get(), Tstart.get(), nt);
+  TG_CHECK_LAUNCH();
+  k_climb<<<grid_for(nt, 256), 256, 0, stream_>>>(g, bounds_, T.get(), Tstart.get(), nt, mark_.get(), chosen.get());
+  TG_CHECK_LAUNCH();
+  u3_.ensure(nt + 1, stream_);  // cover
+  k_cover<<<grid_for(nt, 256), 256, 0, stream_>>>(g, T.get(), nt, mark_.get(), chosen.get(), u3_.get());
+  TG_CHECK_LAUNCH();
+  // owners -> windows (reuse u1_/u2_ for flags/positions over touched)
+  k_owner<<<grid_for(nt, 256), 256, 0, stream_>>>(T.get(), nt, u3_.get(), u1_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(u1_.get(), u2_.get(), nt, u2_.get() + nt, scan_scratch, stream_);
+  uint32_t nw = read_scalar(u2_.get() + nt, stream_);
+  DevBuf<uint8_t> &winbuf = route_;
+  winbuf.ensure(static_cast<uint64_t>(nw) * sizeof(Win) + 64, stream_);
+  Win *win = reinterpret_cast<Win *>(winbuf.get());
+  kick_.ensure(nw + 1, stream_);  // window leaf counts -> task_base
+  k_make_windows<<<grid_for(nt, 256), 256, 0, stream_>>>(g, T.get(), Tstart.get(), nt, u3_.get(), u1_.get(),
+                                                         u2_.get(), win, kick_.get());
+  TG_CHECK_LAUNCH();
+  k_clear_marks<<<grid_for(nt, 256), 256, 0, stream_>>>(g, T.get(), nt, chosen.get(), mark_.get());
+  TG_CHECK_LAUNCH();
+  uint32_t *task_base = kick_.get();
+  exclusive_scan_u32(task_base, task_base, nw, task_base + nw, scan_scratch, stream_);
+  uint64_t ntasks = read_scalar(task_base + nw, stream_);
+  if (ntasks > nl) throw Error(-3, "PMA window tasks exceed leaves");
+  u4_.ensure(ntasks + 1, stream_);
+  k_task_counts<<<grid_for(ntasks, 256), 256, 0, stream_>>>(g, win, task_base, nw, ntasks, u4_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(u4_.get(), u4_.get(), ntasks, u4_.get() + ntasks, scan_scratch, stream_);
+  uint64_t nold = read_scalar(u4_.get() + ntasks, stream_);
+  tmp_old_.ensure(nold + 1, stream_);
+  fill_u64(scal_.get(), 0, 1, stream_);
+  k_gather<<<grid_for(ntasks * 64, 256), 256, 0, stream_>>>(g, win, task_base, nw, ntasks, u4_.get(),
+                                                            tmp_old_.get(), scal_.get());
+  TG_CHECK_LAUNCH();
+  k_fill<<<grid_for(ntasks * 64, 256), 256, 0, stream_>>>(g, win, task_base, nw, ntasks, u4_.get(), tmp_old_.get(),
+                                                          keys);
+  TG_CHECK_LAUNCH();
+  if (read_scalar(scal_.get(), stream_)) throw Error(-3, "PMA count tree out of sync (gather)");
+  stats_.pma_windows += nw;
+  stats_.pma_window_leaves += ntasks;
+}
+
+void Graph::pma_delete(const uint64_t *keys, uint64_t k) {
+  if (k == 0) return;
+  GView g = view();
+  g2_.ensure(k, stream_);
+  fill_u64(scal_.get(), 0, 1, stream_);
+  k_pma_delete<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g2_.get(), scal_.get());
+  TG_CHECK_LAUNCH();
+  k_pma_fix_begin<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g2_.get());
+  TG_CHECK_LAUNCH();
+  if (read_scalar(scal_.get(), stream_)) throw Error(-3, "PMA delete: key not found");
+}
+
+// ------------------------------------------------------------ tree update
+void Graph::tree_insert(const uint64_t *keys, uint64_t k) {
+  if (k == 0) return;
+  GView g = view();
+  g3_.ensure(k + 1, stream_);
+  u0_.ensure(k + 1, stream_);
+  u1_.ensure(k + 1, stream_);
+  k_tree_loc<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g3_.get());
+  TG_CHECK_LAUNCH();
+  k_flag_change_u64<<<grid_for(k, 256), 256, 0, stream_>>>(g3_.get(), k, u0_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(u0_.get(), u1_.get(), k, u1_.get() + k, scan_scratch, stream_);
+  uint32_t ng = read_scalar(u1_.get() + k, stream_);
+  DevBuf<uint8_t> grpbuf;
+  grpbuf.ensure(static_cast<uint64_t>(ng + 1) * sizeof(Grp), stream_);
+  Grp *grp = reinterpret_cast<Grp *>(grpbuf.get());
+  DevBuf<uint32_t> gkb, ob, ho, gflag, gpos, grun;
+  gkb.ensure(ng + 1, stream_);
+  ob.ensure(ng + 1, stream_);
+  ho.ensure(ng + 1, stream_);
+  k_make_groups<<<grid_for(k, 256), 256, 0, stream_>>>(g, g3_.get(), u0_.get(), u1_.get(), k, ng, grp, gkb.get());
+  TG_CHECK_LAUNCH();
+  k_group_sizes<<<grid_for(ng, 256), 256, 0, stream_>>>(g, grp, gkb.get(), ng, true, ob.get(), ho.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(ob.get(), ob.get(), ng, ob.get() + ng, scan_scratch, stream_);
+  exclusive_scan_u32(ho.get(), ho.get(), ng, ho.get() + ng, scan_scratch, stream_);
+  uint64_t nout = read_scalar(ob.get() + ng, stream_);
+  uint64_t blk_base = blk_top_;
+  ensure_blocks(blk_top_ + nout);
+  blk_top_ += nout;
+  g = view();
+  k_tree_fill_insert<<<grid_for(nout * 256, 256, 1u << 16), 256, 0, stream_>>>(g, grp, ob.get(), ng, nout,
+                                                                                blk_base, keys);
+  TG_CHECK_LAUNCH();
+  stats_.tree_blocks_written += nout;
+  // index rebuild for the touched trees
+  gflag.ensure(ng + 1, stream_);
+  gpos.ensure(ng + 1, stream_);
+  grun.ensure(ng + 1, stream_);
+  k_flag_tree_change<<<grid_for(ng, 256), 256, 0, stream_>>>(grp, ng, gflag.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(gflag.get(), gpos.get(), ng, gpos.get() + ng, scan_scratch, stream_);
+  uint32_t ntr = read_scalar(gpos.get() + ng, stream_);
+  k_grp_run<<<grid_for(ng, 256), 256, 0, stream_>>>(gflag.get(), gpos.get(), ng, grun.get());
+  TG_CHECK_LAUNCH();
+  DevBuf<uint8_t> trbuf;
+  trbuf.ensure(static_cast<uint64_t>(ntr + 1) * sizeof(TreeRun), stream_);
+  TreeRun *tr = reinterpret_cast<TreeRun *>(trbuf.get());
+  DevBuf<uint32_t> trs, nbn, nbo;
+  trs.ensure(ntr + 1, stream_);
+  nbn.ensure(ntr + 1, stream_);
+  nbo.ensure(ntr + 1, stream_);
+  k_tree_runs<<<grid_for(ng, 256), 256, 0, stream_>>>(g, grp, gflag.get(), gpos.get(), ng, ntr, tr, trs.get());
+  TG_CHECK_LAUNCH();
+  k_tree_run_sizes<<<grid_for(ntr, 256), 256, 0, stream_>>>(g, grp, trs.get(), ntr, ob.get(), ho.get(), tr,
+                                                            nbn.get(), nbo.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(nbn.get(), nbn.get(), ntr, nbn.get() + ntr, scan_scratch, stream_);
+  exclusive_scan_u32(nbo.get(), nbo.get(), ntr, nbo.get() + ntr, scan_scratch, stream_);
+  uint64_t nidx_new = read_scalar(nbn.get() + ntr, stream_);
+  uint64_t nidx_old = read_scalar(nbo.get() + ntr, stream_);
+  uint64_t idx_base = idx_top_;
+  ensure_index(idx_top_ + nidx_new);
+  idx_top_ += nidx_new;
+  g = view();
+  if (nidx_old)
+    k_tree_idx_copy<<<grid_for(nidx_old, 256), 256, 0, stream_>>>(g, tr, nbo.get(), ntr, nidx_old, grp, ob.get(),
+                                                                  ho.get(), nbn.get(), idx_base);
+  TG_CHECK_LAUNCH();
+  k_tree_idx_groups<<<grid_for(ng, 256), 256, 0, stream_>>>(g, tr, ntr, grp, grun.get(), ng, ob.get(), ho.get(),
+                                                            nbn.get(), idx_base, blk_base);
+  TG_CHECK_LAUNCH();
+  k_tree_commit<<<grid_for(ntr, 256), 256, 0, stream_>>>(g, tr, ntr, nbn.get(), idx_base);
+  TG_CHECK_LAUNCH();
+  TG_HIP(hipStreamSynchronize(stream_));
+}
+
+void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
+  if (k == 0) return;
+  GView g = view();
+  g3_.ensure(k + 1, stream_);
+  u0_.ensure(k + 1, stream_);
+  u1_.ensure(k + 1, stream_);
+  k_tree_loc<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g3_.get());
+  TG_CHECK_LAUNCH();
+  k_flag_change_u64<<<grid_for(k, 256), 256, 0, stream_>>>(g3_.get(), k, u0_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(u0_.get(), u1_.get(), k, u1_.get() + k, scan_scratch, stream_);
+  uint32_t ng = read_scalar(u1_.get() + k, stream_);
+  DevBuf<uint8_t> grpbuf;
+  grpbuf.ensure(static_cast<uint64_t>(ng + 1) * sizeof(Grp), stream_);
+  Grp *grp = reinterpret_cast<Grp *>(grpbuf.get());
+  DevBuf<uint32_t> gkb, ob, ho, gflag, gpos, grun;
+  gkb.ensure(ng + 1, stream_);
+  ob.ensure(ng + 1, stream_);
+  ho.ensure(ng + 1, stream_);
+  k_make_groups<<<grid_for(k, 256), 256, 0, stream_>>>(g, g3_.get(), u0_.get(), u1_.get(), k, ng, grp, gkb.get());
+  TG_CHECK_LAUNCH();
+  k_group_sizes<<<grid_for(ng, 256), 256, 0, stream_>>>(g, grp, gkb.get(), ng, false, ob.get(), ho.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(ob.get(), ob.get(), ng, ob.get() + ng, scan_scratch, stream_);
+  exclusive_scan_u32(ho.get(), ho.get(), ng, ho.get() + ng, scan_scratch, stream_);
+  uint64_t nout = read_scalar(ob.get() + ng, stream_);
+  uint64_t blk_base = blk_top_;
+  ensure_blocks(blk_top_ + nout);
+  blk_top_ += nout;
+  g = view();
+  k_tree_fill_delete<<<grid_for(static_cast<uint64_t>(ng) * 256, 256, 1u << 16), 256, 0, stream_>>>(
+      g, grp, ob.get(), ng, blk_base, keys);
+  TG_CHECK_LAUNCH();
+  stats_.tree_blocks_written += nout;
+  gflag.ensure(ng + 1, stream_);
+  gpos.ensure(ng + 1, stream_);
+  grun.ensure(ng + 1, stream_);
+  k_flag_tree_change<<<grid_for(ng, 256), 256, 0, stream_>>>(grp, ng, gflag.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(gflag.get(), gpos.get(), ng, gpos.get() + ng, scan_scratch, stream_);
+  uint32_t ntr = read_scalar(gpos.get() + ng, stream_);
+  k_grp_run<<<grid_for(ng, 256), 256, 0, stream_>>>(gflag.get(), gpos.get(), ng, grun.get());
+  TG_CHECK_LAUNCH();
+  DevBuf<uint8_t> trbuf;
+  trbuf.ensure(static_cast<uint64_t>(ntr + 1) * sizeof(TreeRun), stream_);
+  TreeRun *tr = reinterpret_cast<TreeRun *>(trbuf.get());
+  DevBuf<uint32_t> trs, nbn, nbo;
+  trs.ensure(ntr + 1, stream_);
+  nbn.ensure(ntr + 1, stream_);
+  nbo.ensure(ntr + 1, stream_);
+  k_tree_runs<<<grid_for(ng, 256), 256, 0, stream_>>>(g, grp, gflag.get(), gpos.get(), ng, ntr, tr, trs.get());
+  TG_CHECK_LAUNCH();
+  k_tree_run_sizes<<<grid_for(ntr, 256), 256, 0, stream_>>>(g, grp, trs.get(), ntr, ob.get(), ho.get(), tr,
+                                                            nbn.get(), nbo.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(nbn.get(), nbn.get(), ntr, nbn.get() + ntr, scan_scratch, stream_);
+  exclusive_scan_u32(nbo.get(), nbo.get(), ntr, nbo.get() + ntr, scan_scratch, stream_);
+  uint64_t nidx_new = read_scalar(nbn.get() + ntr, stream_);
+  uint64_t nidx_old = read_scalar(nbo.get() + ntr, stream_);
+  uint64_t idx_base = idx_top_;
+  ensure_index(idx_top_ + nidx_new + 1);
+  idx_top_ += nidx_new;
+  g = view();
+  if (nidx_old)
+    k_tree_idx_copy<<<grid_for(nidx_old, 256), 256, 0, stream_>>>(g, tr, nbo.get(), ntr, nidx_old, grp, ob.get(),
+                                                                  ho.get(), nbn.get(), idx_base);
+  TG_CHECK_LAUNCH();
+  k_tree_idx_groups<<<grid_for(ng, 256), 256, 0, stream_>>>(g, tr, ntr, grp, grun.get(), ng, ob.get(), ho.get(),
+                                                            nbn.get(), idx_base, blk_base);
+  TG_CHECK_LAUNCH();
+  k_tree_commit<<<grid_for(ntr, 256), 256, 0, stream_>>>(g, tr, ntr, nbn.get(), idx_base);
+  TG_CHECK_LAUNCH();
+  TG_HIP(hipStreamSynchronize(stream_));
+}
+
+// ------------------------------------------------------------ batches
+void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
+  if (count == 0) return;
+  if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
+  lists_dirty_ = true;
+  uint64_t *R = nullptr;
+  uint64_t nR = prepare_keys(src, dst, count, R);  // R in k1_
+  GView g = view();
+  // existence + new-key prefix
+  f0_.ensure(nR + 1, stream_);
+  f1_.ensure(nR + 1, stream_);
+  f2_.ensure(nR + 1, stream_);
+  f3_.ensure(nR + 1, stream_);
+  uint32_t *ex = f0_.get(), *nidx = f1_.get();
+  k_exists<<<grid_for(nR, 256), 256, 0, stream_>>>(g, R, nR, ex);
+  TG_CHECK_LAUNCH();
+  k_not<<<grid_for(nR, 256), 256, 0, stream_>>>(ex, nR, nidx);
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(nidx, nidx, nR, nidx + nR, scan_scratch, stream_);
+  // runs
+  uint32_t *rflag = f2_.get(), *rpos = f3_.get();
+  k_flag_runs<<<grid_for(nR, 256), 256, 0, stream_>>>(R, nR, rflag);
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(rflag, rpos, nR, rpos + nR, scan_scratch, stream_);
+  uint32_t nruns = read_scalar(rpos + nR, stream_);
+  DevBuf<uint32_t> run_beg, run_of, run_c, run_nko, kick_old, promote, ppos;
+  DevBuf<uint8_t> run_route, kroute;
+  run_beg.ensure(nruns + 1, stream_);
+  run_of.ensure(nR + 1, stream_);
+  run_c.ensure(nruns + 1, stream_);
+  run_nko.ensure(nruns + 1, stream_);
+  kick_old.ensure(static_cast<uint64_t>(nruns) * kInline + 1, stream_);
+  promote.ensure(nruns + 1, stream_);
+  ppos.ensure(nruns + 1, stream_);
+  run_route.ensure(nruns + 1, stream_);
+  kroute.ensure(nR + 1, stream_);
+  k_run_starts<<<grid_for(nR, 256), 256, 0, stream_>>>(rflag, rpos, nR, run_beg.get(), nruns);
+  TG_CHECK_LAUNCH();
+  k_run_of<<<grid_for(nR, 256), 256, 0, stream_>>>(rflag, rpos, nR, run_of.get());
+  TG_CHECK_LAUNCH();
+  // the old degrees of promoted vertices are needed after the merge
+  DevBuf<uint32_t> old_deg;
+  old_deg.ensure(nruns + 1, stream_);
+  k_old_degrees<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, old_deg.get());
+  TG_CHECK_LAUNCH();
+  k_inline_merge<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, ex, nidx, run_beg.get(), nruns, run_route.get(),
+                                                            run_c.get(), run_nko.get(), kick_old.get(),
+                                                            promote.get());
+  TG_CHECK_LAUNCH();
+  // kicked keys, sorted per run, placed in kick buffer k2_
+  k2_.ensure(nR + 1, stream_);
+  fill_u64(k2_.get(), kInvalidKey, nR, stream_);
+  TG_HIP(hipMemsetAsync(kroute.get(), 0, nR, stream_));
+  k_place_kicked<<<grid_for(nR, 256), 256, 0, stream_>>>(R, ex, nidx, nR, run_beg.get(), run_of.get(),
+                                                         run_route.get(), run_c.get(), run_nko.get(), kick_old.get(),
+                                                         k2_.get(), kroute.get());
+  TG_CHECK_LAUNCH();
+  k_place_kicked_old<<<grid_for(nruns, 256), 256, 0, stream_>>>(R, nidx, run_beg.get(), nruns, run_route.get(),
+                                                                run_c.get(), run_nko.get(), kick_old.get(), k2_.get(),
+                                                                kroute.get());
+  TG_CHECK_LAUNCH();
+  // promotions: new trees for vertices routed to the B-tier the first time
+  exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_);
+  uint32_t np = read_scalar(ppos.get() + nruns, stream_);
+  uint64_t next = 0;
+  DevBuf<uint64_t> ext;
+  if (np) {
+    DevBuf<uint32_t> pv, pcount, pext, pod;
+    pv.ensure(np + 1, stream_);
+    pcount.ensure(np + 1, stream_);
+    pext.ensure(np + 1, stream_);
+    pod.ensure(np + 1, stream_);
+    k_promote_info<<<grid_for(nruns, 256), 256, 0, stream_>>>(R, run_beg.get(), promote.get(), ppos.get(), nruns, g,
+                                                              pv.get(), pcount.get());
+    TG_CHECK_LAUNCH();
+    k_gather_u32<<<grid_for(nruns, 256), 256, 0, stream_>>>(old_deg.get(), promote.get(), ppos.get(), nruns,
+                                                           pod.get());
+    TG_CHECK_LAUNCH();
+    k_promote_old_counts<<<grid_for(np, 256), 256, 0, stream_>>>(pv.get(), np, pod.get(), pcount.get());
+    TG_CHECK_LAUNCH();
+    exclusive_scan_u32(pcount.get(), pext.get(), np, pext.get() + np, scan_scratch, stream_);
+    next = read_scalar(pext.get() + np, stream_);
+    ext.ensure(next + 1, stream_);
+    if (next) {
+      k_extract_pma<<<grid_for(static_cast<uint64_t>(np) * 64, 256), 256, 0, stream_>>>(g, pv.get(), pcount.get(),
+                                                                                       pext.get(), np, ext.get());
+      TG_CHECK_LAUNCH();
+      pma_delete(ext.get(), next);
+    }
+    ensure_trees(ntrees_ + np);
+    g = view();
+    k_set_tree<<<grid_for(np, 256), 256, 0, stream_>>>(g, pv.get(), np, ntrees_, tvtx_.get());
+    TG_CHECK_LAUNCH();
+    ntrees_ += np;
+    stats_.promotions += np;
+  }
+  // split kicked keys by tier
+  f3_.ensure(nR + 1, stream_);
+  k_flag_route<<<grid_for(nR, 256), 256, 0, stream_>>>(kroute.get(), nR, kRoutePma, f3_.get());
+  TG_CHECK_LAUNCH();
+  k3_.ensure(nR + 1, stream_);
+  uint64_t npma = compact_keys(k2_.get(), f3_.get(), nR, k3_.get());
+  k_flag_route<<<grid_for(nR, 256), 256, 0, stream_>>>(kroute.get(), nR, kRouteTree, f3_.get());
+  TG_CHECK_LAUNCH();
+  g0_.ensure(nR + next + 1, stream_);
+  uint64_t ntree = compact_keys(k2_.get(), f3_.get(), nR, g0_.get());
+  if (next) {
+    TG_HIP(hipMemcpyAsync(g0_.get() + ntree, ext.get(), next * sizeof(uint64_t), hipMemcpyDeviceToDevice, stream_));
+    ntree += next;
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < n_) bits++;
+    radix_sort_keys(g0_.get(), ntree, bits, bits, sort_, stream_);
+  }
+  tree_insert(g0_.get(), ntree);
+  pma_insert(k3_.get(), npma);
+  TG_HIP(hipStreamSynchronize(stream_));
+}
+
+void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
+  if (count == 0) return;
+  if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
+  lists_dirty_ = true;
+  uint64_t *R0 = nullptr;
+  uint64_t nR0 = prepare_keys(src, dst, count, R0);  // k1_
+  GView g = view();
+  f0_.ensure(nR0 + 1, stream_);
+  k_exists<<<grid_for(nR0, 256), 256, 0, stream_>>>(g, R0, nR0, f0_.get());
+  TG_CHECK_LAUNCH();
+  k2_.ensure(nR0 + 1, stream_);
+  uint64_t nR = compact_keys(R0, f0_.get(), nR0, k2_.get());
+  if (nR == 0) return;
+  uint64_t *R = k2_.get();
+  f2_.ensure(nR + 1, stream_);
+  f3_.ensure(nR + 1, stream_);
+  k_flag_runs<<<grid_for(nR, 256), 256, 0, stream_>>>(R, nR, f2_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(f2_.get(), f3_.get(), nR, f3_.get() + nR, scan_scratch, stream_);
+  uint32_t nruns = read_scalar(f3_.get() + nR, stream_);
+  DevBuf<uint32_t> run_beg, run_bump, bump_off;
+  DevBuf<uint8_t> sec_route;
+  run_beg.ensure(nruns + 1, stream_);
+  run_bump.ensure(nruns + 1, stream_);
+  bump_off.ensure(nruns + 1, stream_);
+  sec_route.ensure(nR + 1, stream_);
+  k_run_starts<<<grid_for(nR, 256), 256, 0, stream_>>>(f2_.get(), f3_.get(), nR, run_beg.get(), nruns);
+  TG_CHECK_LAUNCH();
+  k_delete_runs<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, sec_route.get(),
+                                                           run_bump.get());
+  TG_CHECK_LAUNCH();
+  // secondary deletes by tier
+  f0_.ensure(nR + 1, stream_);
+  k3_.ensure(nR + 1, stream_);
+  g0_.ensure(nR + 1, stream_);
+  k_flag_route<<<grid_for(nR, 256), 256, 0, stream_>>>(sec_route.get(), nR, kRoutePma, f0_.get());
+  TG_CHECK_LAUNCH();
+  uint64_t npma = compact_keys(R, f0_.get(), nR, k3_.get());
+  k_flag_route<<<grid_for(nR, 256), 256, 0, stream_>>>(sec_route.get(), nR, kRouteTree, f0_.get());
+  TG_CHECK_LAUNCH();
+  uint64_t ntree = compact_keys(R, f0_.get(), nR, g0_.get());
+  pma_delete(k3_.get(), npma);
+  tree_delete(g0_.get(), ntree);
+  // refill inline slots from the secondary tier
+  exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_);
+  uint64_t nb = read_scalar(bump_off.get() + nruns, stream_);
+  if (nb) {
+    DevBuf<uint64_t> bkeys;
+    DevBuf<uint8_t> broute;
+    bkeys.ensure(nb + 1, stream_);
+    broute.ensure(nb + 1, stream_);
+    g = view();
+    k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, run_bump.get(), bump_off.get(),
+                                                      bkeys.get(), broute.get());
+    TG_CHECK_LAUNCH();
+    f0_.ensure(nb + 1, stream_);
+    k3_.ensure(nb + 1, stream_);
+    g0_.ensure(nb + 1, stream_);
+    k_flag_route<<<grid_for(nb, 256), 256, 0, stream_>>>(broute.get(), nb, kRoutePma, f0_.get());
+    TG_CHECK_LAUNCH();
+    uint64_t bp = compact_keys(bkeys.get(), f0_.get(), nb, k3_.get());
+    k_flag_route<<<grid_for(nb, 256), 256, 0, stream_>>>(broute.get(), nb, kRouteTree, f0_.get());
+    TG_CHECK_LAUNCH();
+    uint64_t bt = compact_keys(bkeys.get(), f0_.get(), nb, g0_.get());
+    pma_delete(k3_.get(), bp);
+    tree_delete(g0_.get(), bt);
+  }
+  TG_HIP(hipStreamSynchronize(stream_));
+}
+
+// ------------------------------------------------------------ queries
+void Graph::degrees(uint32_t *out) const {
+  k_degrees<<<grid_for(n_, 256), 256, 0, stream_>>>(view(), out);
+  TG_CHECK_LAUNCH();
+}
+
+uint64_t Graph::num_edges() {
+  g1_.ensure(static_cast<uint64_t>(n_) + 1, stream_);
+  k_degrees_u64<<<grid_for(n_, 256), 256, 0, stream_>>>(view(), g1_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u64(g1_.get(), g1_.get(), n_, g1_.get() + n_, scan_scratch, stream_);
+  return read_scalar(g1_.get() + n_, stream_);
+}
+
+void Graph::is_edge_batch(const uint32_t *src, const uint32_t *dst, uint64_t count, uint8_t *out) {
+  if (!count) return;
+  k_is_edge<<<grid_for(count, 256), 256, 0, stream_>>>(view(), src, dst, count, out);
+  TG_CHECK_LAUNCH();
+}
+
+void Graph::tiers(uint8_t *out) const {
+  k_tiers<<<grid_for(n_, 256), 256, 0, stream_>>>(view(), out);
+  TG_CHECK_LAUNCH();
+}
+
+const Graph::TierLists &Graph::tier_lists() {
+  if (!lists_dirty_) return lists_;
+  GView g = view();
+  DevBuf<uint32_t> fp, ft, pp, pt;
+  fp.ensure(n_ + 1, stream_);
+  ft.ensure(n_ + 1, stream_);
+  pp.ensure(n_ + 1, stream_);
+  pt.ensure(n_ + 1, stream_);
+  k_flag_tiers<<<grid_for(n_, 256), 256, 0, stream_>>>(g, fp.get(), ft.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(fp.get(), pp.get(), n_, pp.get() + n_, scan_scratch, stream_);
+  exclusive_scan_u32(ft.get(), pt.get(), n_, pt.get() + n_, scan_scratch, stream_);
+  uint32_t np = read_scalar(pp.get() + n_, stream_);
+  uint32_t nt = read_scalar(pt.get() + n_, stream_);
+  list_pma_.ensure(np + 1, stream_);
+  list_tree_.ensure(nt + 1, stream_);
+  k_scatter_ids<<<grid_for(n_, 256), 256, 0, stream_>>>(fp.get(), pp.get(), n_, list_pma_.get());
+  TG_CHECK_LAUNCH();
+  k_scatter_ids<<<grid_for(n_, 256), 256, 0, stream_>>>(ft.get(), pt.get(), n_, list_tree_.get());
+  TG_CHECK_LAUNCH();
+  TG_HIP(hipStreamSynchronize(stream_));
+  lists_.pma = list_pma_.get();
+  lists_.npma = np;
+  lists_.tree = list_tree_.get();
+  lists_.ntree = nt;
+  lists_dirty_ = false;
+  return lists_;
+}
+
+void Graph::export_adjacency(uint64_t *offsets, uint32_t *nbrs) {
+  GView g = view();
+  k_degrees_u64<<<grid_for(n_, 256), 256, 0, stream_>>>(g, offsets);
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u64(offsets, offsets, n_, offsets + n_, scan_scratch, stream_);
+  const TierLists &L = tier_lists();
+  k_export_inline<<<grid_for(n_, 256), 256, 0, stream_>>>(g, offsets, nbrs);
+  TG_CHECK_LAUNCH();
+  if (L.npma) {
+    k_export_pma<<<grid_for(static_cast<uint64_t>(L.npma) * 64, 256), 256, 0, stream_>>>(g, L.pma, L.npma, offsets,
+                                                                                         nbrs);
+    TG_CHECK_LAUNCH();
+  }
+  if (L.ntree) {
+    k_export_tree<<<grid_for(static_cast<uint64_t>(L.ntree) * 256, 256, 1u << 16), 256, 0, stream_>>>(
+        g, L.tree, L.ntree, offsets, nbrs);
+    TG_CHECK_LAUNCH();
+  }
+  TG_HIP(hipStreamSynchronize(stream_));
+}
+
+void Graph::neighbors_of(uint32_t v, uint32_t *out) {
+  if (v >= n_) throw Error(-1, "vertex out of range");
+  k_neighbors_one<<<1, 64, 0, stream_>>>(view(), v, out);
+  TG_CHECK_LAUNCH();
+  TG_HIP(hipStreamSynchronize(stream_));
+}
+
+uint64_t Graph::memory_bytes() const {
+  return vb_.cap * sizeof(VBlock) + (pdst_.cap + psrc_.cap + pcnt_.cap) * 4 + fence_.cap * 8 + mark_.cap +
+         (bkeys_.cap + bcnt_.cap + iblk_.cap + imin_.cap) * 4 + toff_.cap * 8 + (tnblk_.cap + tvtx_.cap) * 4;
+}
+
+}  // namespace tg

@@ -1,0 +1,152 @@
+// Degree-tiered dynamic graph resident in HBM (gfx950).
+//
+// Tiers, following include/graph.h:61-72 with the thresholds the code
+// actually uses (graph.h:34-39):
+//   * inline  : 13 smallest neighbours in a 64-byte vertex block
+//   * PMA     : the rest of every vertex with degree <= 1024, in one shared
+//               packed memory array sorted by (src,dst) with empty slots
+//   * B-tree  : the rest of every vertex routed past 1024, as sorted leaf
+//               blocks of 256 ids under a per-vertex index (a height-2
+//               B+-tree whose root is the index array)
+#pragma once
+
+#include "prims.hpp"
+
+#include <mutex>
+#include <vector>
+
+namespace tg {
+
+// Raw device view handed to kernels.
+struct GView {
+  VBlock *vb;
+  uint32_t n;
+  // PMA
+  uint32_t *pdst;
+  uint32_t *psrc;
+  uint64_t N;       // slots = nleaves * 64
+  uint32_t H;       // nleaves = 1 << H
+  uint32_t *pcnt;   // count tree, level l at tree_off(l)
+  uint64_t *fence;  // [nleaves] lower key fence of each leaf
+  // B-tree tier
+  uint32_t *bkeys;  // [blocks * 256]
+  uint32_t *bcnt;   // [blocks]
+  uint64_t *toff;   // [trees] index segment offset
+  uint32_t *tnblk;  // [trees] blocks in the index
+  uint32_t *iblk;   // index pool: block id
+  uint32_t *imin;   // index pool: first key of the block
+};
+
+__host__ __device__ inline uint64_t tree_level_off(uint32_t H, uint32_t l) {
+  uint64_t nl = 1ull << H;
+  return 2 * nl - ((2 * nl) >> l);
+}
+
+struct Bounds {
+  uint64_t up[48];  // max elements a window at level l may hold
+  uint64_t lo[48];  // min elements a window at level l should hold
+};
+
+struct Stats {
+  uint64_t pma_rebuilds = 0;
+  uint64_t pma_windows = 0;
+  uint64_t pma_window_leaves = 0;
+  uint64_t tree_blocks_written = 0;
+  uint64_t promotions = 0;
+};
+
+class Graph {
+ public:
+  explicit Graph(uint32_t n);
+  ~Graph();
+  Graph(const Graph &) = delete;
+  Graph &operator=(const Graph &) = delete;
+
+  uint32_t num_vertices() const { return n_; }
+  hipStream_t stream() const { return stream_; }
+  GView view() const;
+
+  // Batch updates.  src/dst are device pointers of count ids each; the
+  // inputs are not modified.  Any order, duplicates allowed.
+  void insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t count);
+  void delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t count);
+
+  // Queries (device pointers).
+  void degrees(uint32_t *out) const;
+  uint64_t num_edges();
+  void is_edge_batch(const uint32_t *src, const uint32_t *dst, uint64_t count, uint8_t *out);
+  // offsets[n+1] (device, u64) and neighbours (device, u32), sorted per vertex.
+  void export_adjacency(uint64_t *offsets, uint32_t *nbrs);
+  // Sorted neighbours of one vertex into out (device, >= degree entries).
+  void neighbors_of(uint32_t v, uint32_t *out);
+  // 0 = inline only, 1 = PMA, 2 = B-tree, per vertex (device u8[n]).
+  void tiers(uint8_t *out) const;
+  uint64_t memory_bytes() const;
+  uint32_t num_trees() const { return ntrees_; }
+  uint64_t pma_slots() const { return pma_N_; }
+  const Stats &stats() const { return stats_; }
+
+  // Vertex lists by tier, rebuilt lazily after updates (device arrays).
+  struct TierLists {
+    uint32_t *pma = nullptr;   // vertices with a PMA run
+    uint32_t npma = 0;
+    uint32_t *tree = nullptr;  // vertices with a B-tree
+    uint32_t ntree = 0;
+  };
+  const TierLists &tier_lists();
+
+  std::mutex mu;  // serialises calls on one handle
+  ScanScratch scan_scratch;
+
+ private:
+  uint32_t n_;
+  hipStream_t stream_ = nullptr;
+  DevBuf<VBlock> vb_;
+  // PMA
+  uint64_t pma_N_ = 0;
+  uint32_t pma_H_ = 0;
+  DevBuf<uint32_t> pdst_, psrc_, pcnt_;
+  DevBuf<uint64_t> fence_;
+  DevBuf<uint8_t> mark_;
+  Bounds bounds_{};
+  // B-tree tier
+  DevBuf<uint32_t> bkeys_, bcnt_;
+  uint64_t blk_top_ = 0;
+  DevBuf<uint32_t> iblk_, imin_;
+  uint64_t idx_top_ = 0;
+  DevBuf<uint64_t> toff_;
+  DevBuf<uint32_t> tnblk_, tvtx_;
+  uint32_t ntrees_ = 0;
+  Stats stats_;
+
+  // tier lists
+  bool lists_dirty_ = true;
+  TierLists lists_;
+  DevBuf<uint32_t> list_pma_, list_tree_;
+
+  // scratch
+  SortScratch sort_;
+  DevBuf<uint64_t> k0_, k1_, k2_, k3_;
+  DevBuf<uint32_t> f0_, f1_, f2_, f3_;
+  DevBuf<uint64_t> g0_, g1_, g2_, g3_;
+  DevBuf<uint32_t> u0_, u1_, u2_, u3_, u4_;
+  DevBuf<uint32_t> kick_;
+  DevBuf<uint8_t> route_;
+  DevBuf<uint64_t> scal_;  // small device scalars
+  DevBuf<uint64_t> tmp_old_;
+
+  void init_pma(uint64_t nleaves);
+  void compute_bounds();
+  uint64_t prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t count, uint64_t *&keys);
+  uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
+  void pma_insert(const uint64_t *keys, uint64_t k);
+  void pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total);
+  void pma_delete(const uint64_t *keys, uint64_t k);
+  void tree_insert(const uint64_t *keys, uint64_t k);
+  void tree_delete(const uint64_t *keys, uint64_t k);
+  void ensure_blocks(uint64_t need);
+  void ensure_index(uint64_t need);
+  void ensure_trees(uint64_t need);
+};
+
+}  // namespace tg

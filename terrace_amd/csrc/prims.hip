@@ -1,0 +1,291 @@
+// Device primitives for gfx950: reduce-then-scan exclusive scan and a
+// stable LSD radix sort of 64-bit (src,dst) keys with wave64 multisplit
+// ranking.  The sort replaces the reference's integerSort_y/integerSort_x
+// pair (include/integerSort/blockRadixSort/blockRadixSort.h:287-292), whose
+// output is the lexicographic (src,dst) order produced here.
+#include "prims.hpp"
+
+namespace tg {
+
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;  // 2048
+
+template <typename T>
+__device__ T wave_incl_scan(T v) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    T o = __shfl_up(v, off, 64);
+    if (lane >= static_cast<uint32_t>(off)) v += o;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Exclusive scan of the 256 per-thread values of a block; returns the
+// thread's exclusive prefix and writes the block total into *total.
+template <typename T>
+__device__ T block_excl_scan(T v, T *lds_waves, T *total) {
+  const int tid = threadIdx.x;
+  const int w = tid >> 6;
+  T inc = wave_incl_scan(v);
+  if ((tid & 63) == 63) lds_waves[w] = inc;
+  __syncthreads();
+  T wave_off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kScanBlock / 64; i++) {
+    T x = lds_waves[i];
+    if (i < w) wave_off += x;
+    tot += x;
+  }
+  *total = tot;
+  __syncthreads();
+  return wave_off + inc - v;
+}
+
+template <typename InT, typename AccT>
+__global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const InT *in, uint64_t n, AccT *sums) {
+  __shared__ AccT lds[kScanBlock / 64];
+  uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile;
+  AccT acc = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    uint64_t idx = base + static_cast<uint64_t>(i) * kScanBlock + threadIdx.x;
+    if (idx < n) acc += static_cast<AccT>(in[idx]);
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    AccT t = 0;
+    for (int i = 0; i < kScanBlock / 64; i++) t += lds[i];
+    sums[blockIdx.x] = t;
+  }
+}
+
+// Block-level scan of a tile with per-block offsets[blockIdx.x] added.
+template <typename InT, typename AccT>
+__global__ void __launch_bounds__(kScanBlock) k_scan_tile(const InT *in, InT *out, uint64_t n,
+                                                          const AccT *offsets, InT *total_dev) {
+  __shared__ AccT lds[kScanBlock / 64];
+  uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile +
+                  static_cast<uint64_t>(threadIdx.x) * kScanItems;
+  AccT vals[kScanItems];
+  AccT sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    uint64_t idx = base + i;
+    vals[i] = idx < n ? static_cast<AccT>(in[idx]) : 0;
+    sum += vals[i];
+  }
+  AccT tot;
+  AccT pre = block_excl_scan(sum, lds, &tot);
+  AccT off = offsets ? offsets[blockIdx.x] : 0;
+  pre += off;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    uint64_t idx = base + i;
+    if (idx < n) out[idx] = static_cast<InT>(pre);
+    pre += vals[i];
+  }
+  if (total_dev && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_dev = static_cast<InT>(off + tot);
+}
+
+template <typename T>
+__global__ void k_set_zero(T *p) { *p = 0; }
+
+template <typename InT>
+void scan_impl(const InT *in, InT *out, uint64_t n, InT *total_dev, ScanScratch &ss, int level,
+               hipStream_t s) {
+  if (n == 0) {
+    if (total_dev) {
+      k_set_zero<InT><<<1, 1, 0, s>>>(total_dev);
+      TG_CHECK_LAUNCH();
+    }
+    return;
+  }
+  uint64_t nblocks = (n + kScanTile - 1) / kScanTile;
+  if (nblocks == 1) {
+    k_scan_tile<InT, uint64_t><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, total_dev);
+    TG_CHECK_LAUNCH();
+    return;
+  }
+  if (level >= 5) throw Error(-1, "scan: input too large");
+  DevBuf<uint64_t> &sums = ss.levels[level];
+  sums.ensure(nblocks, s);
+  k_scan_reduce<InT, uint64_t><<<static_cast<unsigned>(nblocks), kScanBlock, 0, s>>>(in, n, sums.get());
+  TG_CHECK_LAUNCH();
+  scan_impl<uint64_t>(sums.get(), sums.get(), nblocks, nullptr, ss, level + 1, s);
+  k_scan_tile<InT, uint64_t><<<static_cast<unsigned>(nblocks), kScanBlock, 0, s>>>(in, out, n, sums.get(),
+                                                                                  total_dev);
+  TG_CHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------------ sort
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortBlock * kSortItems;  // 2048 keys
+constexpr int kWaves = kSortBlock / 64;
+
+// Key index of item j of a lane: waves own contiguous 512-key sub-tiles,
+// iteration j covers 64 consecutive keys, so (wave, j, lane) is index order.
+__device__ inline uint64_t sort_index(uint64_t tile_base, int w, int j, int lane) {
+  return tile_base + static_cast<uint64_t>(w) * (kSortItems * 64) + j * 64 + lane;
+}
+
+__global__ void __launch_bounds__(kSortBlock) k_radix_hist(const uint64_t *keys, uint64_t n, int shift,
+                                                           uint32_t *hist, uint32_t nblocks) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    uint64_t idx = sort_index(tile, w, j, lane);
+    if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & 0xFF], 1u);
+  }
+  __syncthreads();
+  hist[static_cast<uint64_t>(threadIdx.x) * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const uint64_t *keys, uint64_t *out, uint64_t n,
+                                                              int shift, const uint32_t *offs,
+                                                              uint32_t nblocks) {
+  __shared__ uint32_t wave_cnt[kWaves][256];
+  __shared__ uint32_t digit_base[256];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < kWaves * 256; i += kSortBlock) (&wave_cnt[0][0])[i] = 0;
+  __syncthreads();
+  uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+  uint64_t k[kSortItems];
+  uint32_t rank[kSortItems];
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    uint64_t idx = sort_index(tile, w, j, lane);
+    bool valid = idx < n;
+    k[j] = valid ? keys[idx] : 0;
+    uint32_t d = static_cast<uint32_t>((k[j] >> shift) & 0xFF);
+    // peers = lanes of this wave holding the same digit (valid only)
+    uint64_t active = __ballot(valid);
+    uint64_t peers = active;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    uint32_t r = __popcll(peers & lt);
+    int leader = valid ? __ffsll(static_cast<unsigned long long>(peers)) - 1 : 0;
+    uint32_t base = 0;
+    if (valid && lane == leader) {
+      base = wave_cnt[w][d];
+      wave_cnt[w][d] = base + static_cast<uint32_t>(__popcll(peers));
+    }
+    base = __shfl(base, leader, 64);
+    rank[j] = base + r;
+  }
+  __syncthreads();
+  {
+    const int d = threadIdx.x;  // 256 threads == 256 digits
+    uint32_t run = offs[static_cast<uint64_t>(d) * nblocks + blockIdx.x];
+    digit_base[d] = run;
+    for (int ww = 0; ww < kWaves; ww++) {
+      uint32_t c = wave_cnt[ww][d];
+      wave_cnt[ww][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    uint64_t idx = sort_index(tile, w, j, lane);
+    if (idx < n) {
+      uint32_t d = static_cast<uint32_t>((k[j] >> shift) & 0xFF);
+      out[wave_cnt[w][d] + rank[j]] = k[j];
+    }
+  }
+  (void)digit_base;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_f64(double *p, double v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace
+
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *total_dev,
+                        ScanScratch &ss, hipStream_t s) {
+  scan_impl<uint32_t>(in, out, n, total_dev, ss, 0, s);
+}
+
+void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *total_dev,
+                        ScanScratch &ss, hipStream_t s) {
+  scan_impl<uint64_t>(in, out, n, total_dev, ss, 0, s);
+}
+
+void radix_sort_keys(uint64_t *keys, uint64_t n, uint32_t lo_bits, uint32_t hi_bits, SortScratch &ss,
+                     hipStream_t s) {
+  if (n < 2) return;
+  if (n > 0xFFFFFFFFull) throw Error(-1, "radix_sort_keys: more than 2^32 keys");
+  uint32_t nblocks = static_cast<uint32_t>((n + kSortTile - 1) / kSortTile);
+  ss.alt.ensure(n, s);
+  ss.hist.ensure(static_cast<uint64_t>(nblocks) * 256, s);
+  uint64_t *src = keys, *dst = ss.alt.get();
+  int passes = 0;
+  auto pass = [&](int shift) {
+    k_radix_hist<<<nblocks, kSortBlock, 0, s>>>(src, n, shift, ss.hist.get(), nblocks);
+    TG_CHECK_LAUNCH();
+    exclusive_scan_u32(ss.hist.get(), ss.hist.get(), static_cast<uint64_t>(nblocks) * 256, nullptr,
+                       ss.scan, s);
+    k_radix_scatter<<<nblocks, kSortBlock, 0, s>>>(src, dst, n, shift, ss.hist.get(), nblocks);
+    TG_CHECK_LAUNCH();
+    uint64_t *t = src;
+    src = dst;
+    dst = t;
+    passes++;
+  };
+  for (uint32_t b = 0; b < lo_bits; b += 8) pass(static_cast<int>(b));
+  for (uint32_t b = 0; b < hi_bits; b += 8) pass(static_cast<int>(32 + b));
+  if (src != keys)
+    TG_HIP(hipMemcpyAsync(keys, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+}
+
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  k_fill_u32<<<grid_for(n, 256, 65536), 256, 0, s>>>(p, v, n);
+  TG_CHECK_LAUNCH();
+}
+void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  k_fill_u64<<<grid_for(n, 256, 65536), 256, 0, s>>>(p, v, n);
+  TG_CHECK_LAUNCH();
+}
+void fill_f64(double *p, double v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  k_fill_f64<<<grid_for(n, 256, 65536), 256, 0, s>>>(p, v, n);
+  TG_CHECK_LAUNCH();
+}
+
+}  // namespace tg

@@ -1,0 +1,925 @@
+// edgeMap over the tiered graph on gfx950.
+//
+// Direction switch as include/Map.cpp:66-74 (dense pull when
+// n / threshold <= |frontier|).  Each direction is split by tier:
+//   inline  : one thread per vertex (13 slots in its 64-byte block)
+//   PMA     : one wave64 per vertex, 64 slots per step, ballot compaction
+//   B-tree  : pull = one 256-thread workgroup per (vertex, leaf block);
+//             push = one workgroup per frontier vertex over its blocks
+// Sparse outputs are deduplicated through the output bitmap and appended
+// with one atomicAdd per wave.
+#include "traverse.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace tg {
+
+namespace {
+
+#define GRID_STRIDE(i, n)                                                                  \
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < (n); \
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+
+__device__ inline bool bit_get(const uint32_t *bits, uint32_t v) { return (bits[v >> 5] >> (v & 31)) & 1u; }
+__device__ inline bool bit_set(uint32_t *bits, uint32_t v) {
+  uint32_t m = 1u << (v & 31);
+  return (atomicOr(&bits[v >> 5], m) & m) == 0;
+}
+
+// Wave-aggregated queue append (lanes with pred write v).
+__device__ inline void queue_push(uint32_t *q, uint32_t *cnt, bool pred, uint32_t v) {
+  uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  uint32_t lane = __lane_id();
+  int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+  uint32_t base = 0;
+  if (static_cast<int>(lane) == leader) base = atomicAdd(cnt, static_cast<uint32_t>(__popcll(m)));
+  base = __shfl(base, leader, 64);
+  if (pred) q[base + __popcll(m & lanemask_lt())] = v;
+}
+
+template <typename T>
+__device__ inline T wave_min(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    T o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ inline double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+enum Mode { kFirst = 0, kExcl = 1, kShared = 2 };
+
+// ---------------------------------------------------------------- pull ops
+// BFS pull (BFS_F, include/BFS.h:27-39; map_dense_vs_not_all early exit,
+// include/graph.h:1490-1613): parent = first frontier neighbour in sorted
+// order = the smallest frontier neighbour.
+struct PullBFS {
+  const uint32_t *fbits;
+  int32_t *parent;
+  uint32_t *level;
+  uint32_t depth;
+  uint32_t *obits;
+  using Acc = uint32_t;
+  static constexpr bool kEarly = true;
+  __device__ static Acc ident() { return 0xFFFFFFFFu; }
+  __device__ bool skip(uint32_t i) const { return parent[i] != -1; }
+  __device__ bool member(uint32_t v) const { return bit_get(fbits, v); }
+  __device__ Acc value(uint32_t v) const { return v; }
+  __device__ static Acc combine(Acc a, Acc b) { return a < b ? a : b; }
+  __device__ static Acc wave_reduce(Acc a) { return wave_min(a); }
+  __device__ static bool found(Acc a) { return a != 0xFFFFFFFFu; }
+  __device__ void commit(uint32_t i, Acc a, Mode m) const {
+    if (a == 0xFFFFFFFFu) return;
+    if (m == kShared) {
+      if (atomicCAS(&parent[i], -1, static_cast<int32_t>(a)) != -1) return;
+    } else {
+      parent[i] = static_cast<int32_t>(a);
+    }
+    if (level) level[i] = depth;
+    bit_set(obits, i);
+  }
+};
+
+// CC pull (CC_F, include/Components.h:45-66): IDs[i] = min over active
+// neighbours.
+struct PullCC {
+  const uint32_t *fbits;  // null = all
+  uint32_t *ids;
+  using Acc = uint32_t;
+  static constexpr bool kEarly = false;
+  __device__ static Acc ident() { return 0xFFFFFFFFu; }
+  __device__ bool skip(uint32_t) const { return false; }
+  __device__ bool member(uint32_t v) const { return fbits == nullptr || bit_get(fbits, v); }
+  __device__ Acc value(uint32_t v) const { return ids[v]; }
+  __device__ static Acc combine(Acc a, Acc b) { return a < b ? a : b; }
+  __device__ static Acc wave_reduce(Acc a) { return wave_min(a); }
+  __device__ static bool found(Acc) { return false; }
+  __device__ void commit(uint32_t i, Acc a, Mode m) const {
+    if (a == 0xFFFFFFFFu) return;
+    if (m == kShared) atomicMin(&ids[i], a);
+    else if (a < ids[i]) ids[i] = a;
+  }
+};
+
+struct SumAcc {
+  double s;
+  uint32_t any;
+};
+
+// BC forward / backward pull (BC_F / BC_Back_F, include/BC.h:34-71):
+// unvisited i adds the values of its frontier neighbours; PR pull
+// (PR_F, include/Pagerank.h:32-58): p_next[i] = Σ p[v] over all
+// neighbours.
+struct PullSum {
+  const uint32_t *fbits;  // null = all
+  const double *src;      // values read from neighbours
+  double *dst;            // values accumulated into
+  const uint8_t *visited; // null = no condition
+  uint32_t *obits;        // null = no output
+  bool assign;            // PR: the first pass assigns instead of adding
+  using Acc = SumAcc;
+  static constexpr bool kEarly = false;
+  __device__ static Acc ident() { return SumAcc{0.0, 0}; }
+  __device__ bool skip(uint32_t i) const { return visited != nullptr && visited[i]; }
+  __device__ bool member(uint32_t v) const { return fbits == nullptr || bit_get(fbits, v); }
+  __device__ Acc value(uint32_t v) const { return SumAcc{src[v], 1}; }
+  __device__ static Acc combine(Acc a, Acc b) { return SumAcc{a.s + b.s, a.any | b.any}; }
+  __device__ static Acc wave_reduce(Acc a) {
+    a.s = wave_sum_d(a.s);
+    a.any = __ballot(a.any != 0) != 0 ? 1u : 0u;
+    return a;
+  }
+  __device__ static bool found(Acc) { return false; }
+  __device__ void commit(uint32_t i, Acc a, Mode m) const {
+    if (assign && m == kFirst) {
+      dst[i] = a.s;
+      return;
+    }
+    if (!a.any) return;
+    if (m == kShared) unsafeAtomicAdd(&dst[i], a.s);
+    else dst[i] += a.s;
+    if (obits) bit_set(obits, i);
+  }
+};
+
+template <class Op>
+__global__ void __launch_bounds__(256) k_pull_inline(GView g, Op op) {
+  GRID_STRIDE(i, g.n) {
+    uint32_t v = static_cast<uint32_t>(i);
+    if (op.skip(v)) continue;
+    const VBlock &b = g.vb[v];
+    uint32_t D = b.degree;
+    uint32_t lim = D < kInline ? D : kInline;
+    typename Op::Acc a = Op::ident();
+    for (uint32_t k = 0; k < lim; k++) {
+      uint32_t u = b.nbr[k];
+      if (op.member(u)) {
+        a = Op::combine(a, op.value(u));
+        if (Op::kEarly && Op::found(a)) break;
+      }
+    }
+    op.commit(v, a, kFirst);
+  }
+}
+
+// One wave per PMA vertex of the list.
+template <class Op>
+__global__ void __launch_bounds__(256) k_pull_pma(GView g, Op op, const uint32_t *list, uint32_t cnt) {
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint32_t lane = threadIdx.x & 63;
+  for (uint64_t w = wave; w < cnt; w += nwaves) {
+    uint32_t v = list[w];
+    if (op.skip(v)) continue;  // uniform: same v for the wave
+    const VBlock &b = g.vb[v];
+    uint32_t need = b.degree - kInline;
+    uint64_t p = b.aux;
+    uint32_t got = 0;
+    typename Op::Acc a = Op::ident();
+    while (got < need && p < g.N) {
+      uint64_t slot = p + lane;
+      uint32_t d = slot < g.N ? g.pdst[slot] : kEmpty;
+      bool valid = d != kEmpty;
+      uint64_t m = __ballot(valid);
+      uint32_t r = __popcll(m & lanemask_lt());
+      bool in = valid && got + r < need && op.member(d);
+      if (in) a = Op::combine(a, op.value(d));
+      got += __popcll(m);
+      p += 64;
+      if (Op::kEarly && __ballot(in) != 0) break;
+    }
+    a = Op::wave_reduce(a);
+    if (lane == 0) op.commit(v, a, kExcl);
+  }
+}
+
+// One workgroup per (tree vertex, leaf block) task.
+template <class Op>
+__global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_t *list, uint32_t cnt,
+                                                   const uint64_t *task_off, uint64_t ntasks) {
+  __shared__ typename Op::Acc part[4];
+  for (uint64_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    // vertex of task t: last list index with task_off <= t
+    uint32_t lo = 0, hi = cnt;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (task_off[mid] <= t) lo = mid + 1;
+      else hi = mid;
+    }
+    uint32_t li = lo - 1;
+    uint32_t v = list[li];
+    if (op.skip(v)) continue;  // uniform across the block
+    uint32_t tr = static_cast<uint32_t>(g.vb[v].aux);
+    uint64_t j = t - task_off[li];
+    uint32_t blk = g.iblk[g.toff[tr] + j];
+    uint32_t c = g.bcnt[blk];
+    typename Op::Acc a = Op::ident();
+    if (threadIdx.x < c) {
+      uint32_t d = g.bkeys[static_cast<uint64_t>(blk) * kBlk + threadIdx.x];
+      if (op.member(d)) a = op.value(d);
+    }
+    a = Op::wave_reduce(a);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      typename Op::Acc r = part[0];
+      for (int w = 1; w < 4; w++) r = Op::combine(r, part[w]);
+      op.commit(v, r, kShared);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- push ops
+// BFS push: CAS(-1 -> s) (BFS_F::updateAtomic, include/BFS.h:34-36).
+struct PushBFS {
+  int32_t *parent;
+  uint32_t *level;
+  uint32_t depth;
+  __device__ bool apply(uint32_t s, uint32_t d) const {
+    if (parent[d] != -1) return false;
+    if (atomicCAS(&parent[d], -1, static_cast<int32_t>(s)) != -1) return false;
+    if (level) level[d] = depth;
+    return true;
+  }
+};
+// CC push: writeMin (CC_F::updateAtomic, include/Components.h:61-64).
+struct PushCC {
+  uint32_t *ids;
+  __device__ bool apply(uint32_t s, uint32_t d) const {
+    uint32_t ls = ids[s];
+    if (ls >= ids[d]) return false;
+    return atomicMin(&ids[d], ls) > ls;
+  }
+};
+// BC push: CAS-add, output on the first contribution (BC_F::updateAtomic,
+// include/BC.h:44-50).
+struct PushSum {
+  double *val;
+  const uint8_t *visited;
+  __device__ bool apply(uint32_t s, uint32_t d) const {
+    if (visited[d]) return false;
+    double old = atomicAdd(&val[d], val[s]);
+    return old == 0.0;
+  }
+};
+
+struct PushOut {
+  uint32_t *obits;  // null = no output
+  uint32_t *queue;
+  uint32_t *count;
+};
+
+template <class Op>
+__device__ inline void push_emit(const Op &op, const PushOut &o, uint32_t s, uint32_t d, bool valid) {
+  bool hit = valid && op.apply(s, d);
+  if (o.obits) {
+    bool fresh = hit && bit_set(o.obits, d);
+    queue_push(o.queue, o.count, fresh, d);
+  }
+}
+
+// Thread per frontier vertex: inline neighbours; vertices with a secondary
+// run are appended to the PMA / tree lists.
+template <class Op>
+__global__ void __launch_bounds__(256) k_push_inline(GView g, Op op, PushOut o, const uint32_t *fq, uint32_t fcount,
+                                                     uint32_t *sec_pma, uint32_t *sec_tree, uint32_t *sec_cnt) {
+  for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < fcount;
+       i0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint64_t i = i0 + threadIdx.x;
+    bool ok = i < fcount;
+    uint32_t s = ok ? fq[i] : 0;
+    uint32_t D = ok ? g.vb[s].degree : 0;
+    uint32_t lim = D < kInline ? D : kInline;
+    for (uint32_t k = 0; k < kInline; k++) {
+      bool valid = k < lim;
+      uint32_t d = valid ? g.vb[s].nbr[k] : 0;
+      push_emit(op, o, s, d, valid);
+    }
+    bool sec = ok && D > kInline;
+    bool tree = sec && (g.vb[s].aux & kAuxTree);
+    queue_push(sec_pma, sec_cnt + 0, sec && !tree, s);
+    queue_push(sec_tree, sec_cnt + 1, tree, s);
+  }
+}
+
+template <class Op>
+__global__ void __launch_bounds__(256) k_push_pma(GView g, Op op, PushOut o, const uint32_t *list,
+                                                  const uint32_t *cnt_ptr) {
+  uint32_t cnt = *cnt_ptr;
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint32_t lane = threadIdx.x & 63;
+  for (uint64_t w = wave; w < cnt; w += nwaves) {
+    uint32_t s = list[w];
+    const VBlock &b = g.vb[s];
+    uint32_t need = b.degree - kInline;
+    uint64_t p = b.aux;
+    uint32_t got = 0;
+    while (got < need && p < g.N) {
+      uint64_t slot = p + lane;
+      uint32_t d = slot < g.N ? g.pdst[slot] : kEmpty;
+      bool valid = d != kEmpty;
+      uint64_t m = __ballot(valid);
+      uint32_t r = __popcll(m & lanemask_lt());
+      push_emit(op, o, s, d, valid && got + r < need);
+      got += __popcll(m);
+      p += 64;
+    }
+  }
+}
+
+template <class Op>
+__global__ void __launch_bounds__(256) k_push_tree(GView g, Op op, PushOut o, const uint32_t *list,
+                                                   const uint32_t *cnt_ptr) {
+  uint32_t cnt = *cnt_ptr;
+  for (uint64_t w = blockIdx.x; w < cnt; w += gridDim.x) {
+    uint32_t s = list[w];
+    uint32_t tr = static_cast<uint32_t>(g.vb[s].aux);
+    uint64_t off = g.toff[tr];
+    uint32_t nb = g.tnblk[tr];
+    for (uint32_t j = 0; j < nb; j++) {
+      uint32_t blk = g.iblk[off + j];
+      uint32_t c = g.bcnt[blk];
+      bool valid = threadIdx.x < c;
+      uint32_t d = valid ? g.bkeys[static_cast<uint64_t>(blk) * kBlk + threadIdx.x] : 0;
+      push_emit(op, o, s, d, valid);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- helpers
+__global__ void k_bits_from_queue(const uint32_t *q, uint64_t cnt, uint32_t *bits) {
+  GRID_STRIDE(i, cnt) bit_set(bits, q[i]);
+}
+
+__global__ void k_word_pop(const uint32_t *bits, uint64_t words, uint32_t n, uint32_t *pop) {
+  GRID_STRIDE(w, words) {
+    uint32_t x = bits[w];
+    uint64_t hi = (w + 1) * 32;
+    if (hi > n) {
+      uint32_t keep = n - static_cast<uint32_t>(w * 32);
+      x &= keep >= 32 ? 0xFFFFFFFFu : ((1u << keep) - 1u);
+    }
+    pop[w] = __popc(x);
+  }
+}
+
+__global__ void k_queue_from_bits(const uint32_t *bits, uint64_t words, uint32_t n, const uint32_t *pos,
+                                  uint32_t *q) {
+  GRID_STRIDE(w, words) {
+    uint32_t x = bits[w];
+    uint32_t o = pos[w];
+    while (x) {
+      uint32_t b = __ffs(x) - 1;
+      uint32_t v = static_cast<uint32_t>(w * 32 + b);
+      if (v < n) q[o++] = v;
+      x &= x - 1;
+    }
+  }
+}
+
+__global__ void k_set_u32(uint32_t *p, uint64_t i, uint32_t v) { p[i] = v; }
+__global__ void k_set_i32(int32_t *p, uint64_t i, int32_t v) { p[i] = v; }
+__global__ void k_set_f64(double *p, uint64_t i, double v) { p[i] = v; }
+__global__ void k_set_u8(uint8_t *p, uint64_t i, uint8_t v) { p[i] = v; }
+
+__global__ void k_iota(uint32_t *p, uint32_t n) { GRID_STRIDE(i, n) p[i] = static_cast<uint32_t>(i); }
+__global__ void k_fill_i32(int32_t *p, int32_t v, uint64_t n) { GRID_STRIDE(i, n) p[i] = v; }
+__global__ void k_copy_u32(const uint32_t *a, uint32_t *b, uint64_t n) { GRID_STRIDE(i, n) b[i] = a[i]; }
+
+// p[i] / deg(i) (PR_Vertex, include/Pagerank.h:61-69)
+__global__ void k_pr_norm(GView g, const double *p, double *q) {
+  GRID_STRIDE(i, g.n) q[i] = p[i] / static_cast<double>(g.vb[i].degree);
+}
+
+// CC: output = vertices whose label changed this round.
+__global__ void k_cc_changed(const uint32_t *ids, const uint32_t *prev, uint32_t n, uint32_t *obits) {
+  GRID_STRIDE(i, n) if (ids[i] != prev[i]) bit_set(obits, static_cast<uint32_t>(i));
+}
+
+__global__ void k_mark_visited(const uint32_t *q, uint64_t cnt, uint8_t *visited) {
+  GRID_STRIDE(i, cnt) visited[q[i]] = 1;
+}
+
+// BC_Back_Vertex_F (include/BC.h:84-94) over a level list.
+__global__ void k_bc_back_vertex(const uint32_t *q, uint64_t cnt, uint8_t *visited, double *dep, const double *inv) {
+  GRID_STRIDE(i, cnt) {
+    uint32_t v = q[i];
+    visited[v] = 1;
+    dep[v] += inv[v];
+  }
+}
+
+__global__ void k_recip(double *p, uint32_t n) { GRID_STRIDE(i, n) p[i] = 1 / p[i]; }
+
+__global__ void k_bc_final(double *dep, const double *inv, uint32_t n) {
+  GRID_STRIDE(i, n) dep[i] = (dep[i] - inv[i]) / inv[i];
+}
+
+__global__ void k_tree_task_counts(GView g, const uint32_t *list, uint32_t cnt, uint64_t *out) {
+  GRID_STRIDE(i, cnt) out[i] = g.tnblk[static_cast<uint32_t>(g.vb[list[i]].aux)];
+}
+
+// TC over an exported CSR: every edge (i, v) with v > i counts
+// |N(i) ∩ N(v) ∩ [0, i)| (countF + count_common, include/TC.h:51-87).
+__global__ void __launch_bounds__(256) k_tc(const uint64_t *off, const uint32_t *nbr, uint32_t n, uint64_t m,
+                                            unsigned long long *total) {
+  __shared__ unsigned long long part[4];
+  unsigned long long local = 0;
+  for (uint64_t e0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); e0 < m;
+       e0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint64_t e = e0 + threadIdx.x;
+    if (e < m) {
+      // vertex owning edge e
+      uint32_t lo = 0, hi = n;
+      while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (off[mid + 1] <= e) lo = mid + 1;
+        else hi = mid;
+      }
+      uint32_t i = lo;
+      uint32_t v = nbr[e];
+      if (v > i) {
+        uint64_t ab = off[i], ae = off[i + 1], bb = off[v], be = off[v + 1];
+        // restrict N(v) to ids below i
+        uint64_t l2 = bb, h2 = be;
+        while (l2 < h2) {
+          uint64_t mid = (l2 + h2) >> 1;
+          if (nbr[mid] < i) l2 = mid + 1;
+          else h2 = mid;
+        }
+        be = l2;
+        uint64_t pa = ab, pb = bb;
+        while (pa < ae && pb < be) {
+          uint32_t x = nbr[pa], y = nbr[pb];
+          if (x >= i) break;
+          if (x == y) {
+            local++;
+            pa++;
+            pb++;
+          } else if (x < y) {
+            pa++;
+          } else {
+            pb++;
+          }
+        }
+      }
+    }
+  }
+  for (int off2 = 32; off2 > 0; off2 >>= 1) local += __shfl_xor(local, off2, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
+}
+
+}  // namespace
+
+// ===================================================================== host
+void Traversal::record(const char *name, hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  TG_HIP(hipEventSynchronize(b));
+  TG_HIP(hipEventElapsedTime(&ms, a, b));
+  for (auto &k : ktimes)
+    if (std::string(k.name) == name) {
+      k.ms += ms;
+      k.launches++;
+      return;
+    }
+  ktimes.push_back(KTime{name, ms, 1});
+}
+
+void Traversal::frontier_single(Frontier &f, uint32_t v) {
+  hipStream_t s = g_.stream();
+  f.n = g_.num_vertices();
+  f.all = false;
+  f.queue.ensure(f.n, s);
+  k_set_u32<<<1, 1, 0, s>>>(f.queue.get(), 0, v);
+  TG_CHECK_LAUNCH();
+  f.count = 1;
+  f.has_queue = true;
+  f.has_bits = false;
+}
+
+void Traversal::frontier_all(Frontier &f) {
+  f.n = g_.num_vertices();
+  f.all = true;
+  f.count = f.n;
+  f.has_queue = f.has_bits = false;
+}
+
+void Traversal::frontier_empty(Frontier &f) {
+  hipStream_t s = g_.stream();
+  f.n = g_.num_vertices();
+  f.all = false;
+  f.count = 0;
+  f.bits.ensure(f.words() + 1, s);
+  f.queue.ensure(f.n, s);
+  fill_u32(f.bits.get(), 0, f.words(), s);
+  f.has_bits = true;
+  f.has_queue = true;
+}
+
+void Traversal::to_bits(Frontier &f) {
+  if (f.all || f.has_bits) return;
+  hipStream_t s = g_.stream();
+  f.bits.ensure(f.words() + 1, s);
+  fill_u32(f.bits.get(), 0, f.words(), s);
+  if (f.count) {
+    k_bits_from_queue<<<grid_for(f.count, 256), 256, 0, s>>>(f.queue.get(), f.count, f.bits.get());
+    TG_CHECK_LAUNCH();
+  }
+  f.has_bits = true;
+}
+
+void Traversal::to_queue(Frontier &f) {
+  hipStream_t s = g_.stream();
+  if (f.has_queue) return;
+  f.queue.ensure(f.n, s);
+  if (f.all) {
+    k_iota<<<grid_for(f.n, 256), 256, 0, s>>>(f.queue.get(), f.n);
+    TG_CHECK_LAUNCH();
+    f.count = f.n;
+    f.has_queue = true;
+    return;
+  }
+  uint64_t words = f.words();
+  utmp_.ensure(words + 1, s);
+  k_word_pop<<<grid_for(words, 256), 256, 0, s>>>(f.bits.get(), words, f.n, utmp_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(utmp_.get(), utmp_.get(), words, utmp_.get() + words, g_.scan_scratch, s);
+  k_queue_from_bits<<<grid_for(words, 256), 256, 0, s>>>(f.bits.get(), words, f.n, utmp_.get(), f.queue.get());
+  TG_CHECK_LAUNCH();
+  f.count = read_scalar(utmp_.get() + words, s);
+  f.has_queue = true;
+}
+
+uint64_t Traversal::popcount_bits(const Frontier &f) {
+  hipStream_t s = g_.stream();
+  uint64_t words = f.words();
+  utmp_.ensure(words + 1, s);
+  k_word_pop<<<grid_for(words, 256), 256, 0, s>>>(f.bits.get(), words, f.n, utmp_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(utmp_.get(), utmp_.get(), words, utmp_.get() + words, g_.scan_scratch, s);
+  return read_scalar(utmp_.get() + words, s);
+}
+
+void Traversal::prepare_tree_tasks() {
+  const Graph::TierLists &L = g_.tier_lists();
+  hipStream_t s = g_.stream();
+  tree_list_ = L.tree;
+  ntree_list_ = L.ntree;
+  ttask_off_.ensure(static_cast<uint64_t>(L.ntree) + 1, s);
+  if (L.ntree) {
+    k_tree_task_counts<<<grid_for(L.ntree, 256), 256, 0, s>>>(g_.view(), L.tree, L.ntree, ttask_off_.get());
+    TG_CHECK_LAUNCH();
+  }
+  exclusive_scan_u64(ttask_off_.get(), ttask_off_.get(), L.ntree, ttask_off_.get() + L.ntree, g_.scan_scratch, s);
+  ntree_tasks_ = read_scalar(ttask_off_.get() + L.ntree, s);
+}
+
+template <class Op>
+static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierLists &L, const uint32_t *tree_list,
+                        uint32_t ntree, const uint64_t *task_off, uint64_t ntasks) {
+  hipStream_t s = g.stream();
+  GView v = g.view();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (T.profiling) {
+    TG_HIP(hipEventCreate(&e0));
+    TG_HIP(hipEventCreate(&e1));
+    TG_HIP(hipEventRecord(e0, s));
+  }
+  k_pull_inline<Op><<<grid_for(v.n, 256), 256, 0, s>>>(v, op);
+  TG_CHECK_LAUNCH();
+  if (L.npma) {
+    k_pull_pma<Op><<<grid_for(static_cast<uint64_t>(L.npma) * 64, 256), 256, 0, s>>>(v, op, L.pma, L.npma);
+    TG_CHECK_LAUNCH();
+  }
+  if (ntasks) {
+    k_pull_tree<Op><<<grid_for(ntasks * 256, 256, 1u << 20), 256, 0, s>>>(v, op, tree_list, ntree, task_off, ntasks);
+    TG_CHECK_LAUNCH();
+  }
+  if (T.profiling) {
+    TG_HIP(hipEventRecord(e1, s));
+    T.record("edge_map_pull", e0, e1);
+    TG_HIP(hipEventDestroy(e0));
+    TG_HIP(hipEventDestroy(e1));
+  }
+}
+
+void Traversal::pull(Frontier &in, Op op, const OpState &st, Frontier *out) {
+  hipStream_t s = g_.stream();
+  const Graph::TierLists &L = g_.tier_lists();
+  prepare_tree_tasks();
+  if (!in.all) to_bits(in);
+  const uint32_t *fb = in.all ? nullptr : in.bits.get();
+  uint32_t *ob = nullptr;
+  if (out) {
+    frontier_empty(*out);
+    out->has_queue = false;
+    ob = out->bits.get();
+  }
+  switch (op) {
+    case Op::BFS: {
+      PullBFS p{fb, st.parent, st.level, st.depth, ob};
+      if (!fb) throw Error(-1, "BFS needs an explicit frontier");
+      launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+      break;
+    }
+    case Op::CC: {
+      PullCC p{fb, st.ids};
+      launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+      if (ob) {
+        if (!st.prev) throw Error(-1, "CC edge_map with output needs prevIDs");
+        k_cc_changed<<<grid_for(g_.num_vertices(), 256), 256, 0, s>>>(st.ids, st.prev, g_.num_vertices(), ob);
+        TG_CHECK_LAUNCH();
+      }
+      break;
+    }
+    case Op::BC_FWD:
+    case Op::BC_BACK: {
+      PullSum p{fb, st.val, st.val, st.visited, ob, false};
+      launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+      break;
+    }
+    case Op::PR: {
+      PullSum p{nullptr, st.src_val, st.val, nullptr, nullptr, true};
+      launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+      break;
+    }
+    default:
+      throw Error(-1, "edge_map: unsupported op");
+  }
+  if (out) out->count = popcount_bits(*out);
+}
+
+template <class Op>
+static void launch_push(Traversal &T, Graph &g, const Op &op, Frontier &in, Frontier *out, DevBuf<uint32_t> &cnt,
+                        DevBuf<uint32_t> &sp, DevBuf<uint32_t> &st) {
+  hipStream_t s = g.stream();
+  GView v = g.view();
+  uint32_t n = g.num_vertices();
+  cnt.ensure(4, s);
+  fill_u32(cnt.get(), 0, 4, s);
+  sp.ensure(in.count + 1, s);
+  st.ensure(in.count + 1, s);
+  PushOut o{nullptr, nullptr, nullptr};
+  if (out) {
+    T.frontier_empty(*out);
+    o.obits = out->bits.get();
+    o.queue = out->queue.get();
+    o.count = cnt.get() + 2;
+  }
+  (void)n;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (T.profiling) {
+    TG_HIP(hipEventCreate(&e0));
+    TG_HIP(hipEventCreate(&e1));
+    TG_HIP(hipEventRecord(e0, s));
+  }
+  uint32_t fc = static_cast<uint32_t>(in.count);
+  k_push_inline<Op><<<grid_for(fc, 256), 256, 0, s>>>(v, op, o, in.queue.get(), fc, sp.get(), st.get(), cnt.get());
+  TG_CHECK_LAUNCH();
+  k_push_pma<Op><<<grid_for(static_cast<uint64_t>(fc) * 64, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
+  TG_CHECK_LAUNCH();
+  k_push_tree<Op><<<grid_for(static_cast<uint64_t>(fc) * 256, 256, 1u << 16), 256, 0, s>>>(v, op, o, st.get(),
+                                                                                         cnt.get() + 1);
+  TG_CHECK_LAUNCH();
+  if (T.profiling) {
+    TG_HIP(hipEventRecord(e1, s));
+    T.record("edge_map_push", e0, e1);
+    TG_HIP(hipEventDestroy(e0));
+    TG_HIP(hipEventDestroy(e1));
+  }
+  if (out) {
+    out->count = read_scalar(cnt.get() + 2, s);
+    out->has_queue = true;
+    out->has_bits = true;
+  }
+}
+
+void Traversal::push(Frontier &in, Op op, const OpState &st, Frontier *out) {
+  to_queue(in);
+  switch (op) {
+    case Op::BFS: {
+      PushBFS p{st.parent, st.level, st.depth};
+      launch_push(*this, g_, p, in, out, cnt_, sec_pma_, sec_tree_);
+      break;
+    }
+    case Op::CC: {
+      PushCC p{st.ids};
+      launch_push(*this, g_, p, in, out, cnt_, sec_pma_, sec_tree_);
+      break;
+    }
+    case Op::BC_FWD:
+    case Op::BC_BACK: {
+      PushSum p{st.val, st.visited};
+      launch_push(*this, g_, p, in, out, cnt_, sec_pma_, sec_tree_);
+      break;
+    }
+    default:
+      throw Error(-1, "edge_map: op has no push form");
+  }
+}
+
+void Traversal::edge_map(Frontier &in, Op op, const OpState &st, Frontier *out, uint32_t threshold) {
+  uint32_t n = g_.num_vertices();
+  if (threshold == 0) threshold = 1;
+  bool dense = in.all || (n / threshold) <= in.count || op == Op::PR;
+  if (dense) pull(in, op, st, out);
+  else push(in, op, st, out);
+}
+
+// ---------------------------------------------------------------- BFS
+uint32_t Traversal::bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t threshold) {
+  hipStream_t s = g_.stream();
+  uint32_t n = g_.num_vertices();
+  if (src >= n) throw Error(-1, "source out of range");
+  k_fill_i32<<<grid_for(n, 256), 256, 0, s>>>(parent, -1, n);
+  TG_CHECK_LAUNCH();
+  if (level) fill_u32(level, 0xFFFFFFFFu, n, s);
+  k_set_i32<<<1, 1, 0, s>>>(parent, src, static_cast<int32_t>(src));
+  if (level) k_set_u32<<<1, 1, 0, s>>>(level, src, 0);
+  Frontier a, b;
+  frontier_single(a, src);
+  Frontier *cur = &a, *nxt = &b;
+  uint32_t depth = 0;
+  while (cur->count > 0) {
+    depth++;
+    OpState st;
+    st.parent = parent;
+    st.level = level;
+    st.depth = depth;
+    edge_map(*cur, Op::BFS, st, nxt, threshold);
+    std::swap(cur, nxt);
+  }
+  TG_HIP(hipStreamSynchronize(s));
+  return depth;
+}
+
+// ---------------------------------------------------------------- PR
+void Traversal::pagerank(uint32_t iters, double *out) {
+  hipStream_t s = g_.stream();
+  uint32_t n = g_.num_vertices();
+  dtmp_.ensure(n, s);
+  dtmp2_.ensure(n, s);
+  double *p = out, *q = dtmp_.get(), *pn = dtmp2_.get();
+  fill_f64(p, 1 / static_cast<double>(n), n, s);
+  Frontier all;
+  frontier_all(all);
+  for (uint32_t it = 0; it < iters; it++) {
+    k_pr_norm<<<grid_for(n, 256), 256, 0, s>>>(g_.view(), p, q);
+    TG_CHECK_LAUNCH();
+    OpState st;
+    st.src_val = q;
+    st.val = pn;
+    pull(all, Op::PR, st, nullptr);
+    std::swap(p, pn);
+  }
+  if (p != out) TG_HIP(hipMemcpyAsync(out, p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  TG_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- CC
+uint32_t Traversal::cc(uint32_t *ids, uint32_t threshold) {
+  hipStream_t s = g_.stream();
+  uint32_t n = g_.num_vertices();
+  utmp_.ensure(n + 1, s);
+  DevBuf<uint32_t> prev;
+  prev.ensure(n, s);
+  k_iota<<<grid_for(n, 256), 256, 0, s>>>(ids, n);
+  TG_CHECK_LAUNCH();
+  Frontier a, b;
+  frontier_all(a);
+  Frontier *cur = &a, *nxt = &b;
+  uint32_t rounds = 0;
+  while (cur->count > 0) {
+    rounds++;
+    k_copy_u32<<<grid_for(n, 256), 256, 0, s>>>(ids, prev.get(), n);
+    TG_CHECK_LAUNCH();
+    OpState st;
+    st.ids = ids;
+    bool dense = cur->all || (n / (threshold ? threshold : 1)) <= cur->count;
+    if (dense) {
+      if (!cur->all) to_bits(*cur);
+      const Graph::TierLists &L = g_.tier_lists();
+      prepare_tree_tasks();
+      PullCC p{cur->all ? nullptr : cur->bits.get(), ids};
+      launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+      frontier_empty(*nxt);
+      nxt->has_queue = false;
+      k_cc_changed<<<grid_for(n, 256), 256, 0, s>>>(ids, prev.get(), n, nxt->bits.get());
+      TG_CHECK_LAUNCH();
+      nxt->count = popcount_bits(*nxt);
+    } else {
+      push(*cur, Op::CC, st, nxt);
+    }
+    std::swap(cur, nxt);
+  }
+  TG_HIP(hipStreamSynchronize(s));
+  return rounds;
+}
+
+// ---------------------------------------------------------------- BC
+void Traversal::bc(uint32_t src, double *out, uint32_t threshold) {
+  hipStream_t s = g_.stream();
+  uint32_t n = g_.num_vertices();
+  if (src >= n) throw Error(-1, "source out of range");
+  dtmp_.ensure(n, s);
+  double *paths = dtmp_.get();
+  double *dep = out;
+  btmp_.ensure(n, s);
+  uint8_t *visited = btmp_.get();
+  fill_f64(paths, 0.0, n, s);
+  TG_HIP(hipMemsetAsync(visited, 0, n, s));
+  k_set_u8<<<1, 1, 0, s>>>(visited, src, 1);
+  k_set_f64<<<1, 1, 0, s>>>(paths, src, 1.0);
+  // level lists, concatenated
+  DevBuf<uint32_t> lv;
+  lv.ensure(static_cast<uint64_t>(n) + 1, s);
+  std::vector<uint64_t> loff{0};
+  k_set_u32<<<1, 1, 0, s>>>(lv.get(), 0, src);
+  loff.push_back(1);
+  Frontier a, b;
+  frontier_single(a, src);
+  Frontier *cur = &a, *nxt = &b;
+  while (true) {
+    OpState st;
+    st.val = paths;
+    st.visited = visited;
+    edge_map(*cur, Op::BC_FWD, st, nxt, threshold);
+    if (nxt->count == 0) break;
+    to_queue(*nxt);
+    TG_HIP(hipMemcpyAsync(lv.get() + loff.back(), nxt->queue.get(), nxt->count * sizeof(uint32_t),
+                          hipMemcpyDeviceToDevice, s));
+    loff.push_back(loff.back() + nxt->count);
+    k_mark_visited<<<grid_for(nxt->count, 256), 256, 0, s>>>(nxt->queue.get(), nxt->count, visited);
+    TG_CHECK_LAUNCH();
+    std::swap(cur, nxt);
+  }
+  size_t nlev = loff.size() - 1;  // non-empty levels
+  k_recip<<<grid_for(n, 256), 256, 0, s>>>(paths, n);
+  TG_CHECK_LAUNCH();
+  TG_HIP(hipMemsetAsync(visited, 0, n, s));
+  fill_f64(dep, 0.0, n, s);
+  for (size_t r = nlev; r-- > 0;) {
+    if (r + 1 < nlev) {
+      Frontier f;
+      f.n = n;
+      f.queue.ensure(loff[r + 2] - loff[r + 1] + 1, s);
+      TG_HIP(hipMemcpyAsync(f.queue.get(), lv.get() + loff[r + 1], (loff[r + 2] - loff[r + 1]) * sizeof(uint32_t),
+                            hipMemcpyDeviceToDevice, s));
+      f.count = loff[r + 2] - loff[r + 1];
+      f.has_queue = true;
+      OpState st;
+      st.val = dep;
+      st.visited = visited;
+      edge_map(f, Op::BC_BACK, st, nullptr, threshold);
+    }
+    uint64_t c = loff[r + 1] - loff[r];
+    k_bc_back_vertex<<<grid_for(c, 256), 256, 0, s>>>(lv.get() + loff[r], c, visited, dep, paths);
+    TG_CHECK_LAUNCH();
+  }
+  k_bc_final<<<grid_for(n, 256), 256, 0, s>>>(dep, paths, n);
+  TG_CHECK_LAUNCH();
+  TG_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- TC
+uint64_t Traversal::tc() {
+  hipStream_t s = g_.stream();
+  uint32_t n = g_.num_vertices();
+  uint64_t m = g_.num_edges();
+  csr_off_.ensure(static_cast<uint64_t>(n) + 1, s);
+  csr_nbr_.ensure(m + 1, s);
+  g_.export_adjacency(csr_off_.get(), csr_nbr_.get());
+  red_.ensure(2, s);
+  fill_u64(red_.get(), 0, 1, s);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (profiling) {
+    TG_HIP(hipEventCreate(&e0));
+    TG_HIP(hipEventCreate(&e1));
+    TG_HIP(hipEventRecord(e0, s));
+  }
+  if (m) {
+    k_tc<<<grid_for(m, 256, 1u << 18), 256, 0, s>>>(csr_off_.get(), csr_nbr_.get(), n, m,
+                                                    reinterpret_cast<unsigned long long *>(red_.get()));
+    TG_CHECK_LAUNCH();
+  }
+  if (profiling) {
+    TG_HIP(hipEventRecord(e1, s));
+    record("tc", e0, e1);
+    TG_HIP(hipEventDestroy(e0));
+    TG_HIP(hipEventDestroy(e1));
+  }
+  return read_scalar(red_.get(), s);
+}
+
+}  // namespace tg

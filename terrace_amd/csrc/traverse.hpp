@@ -1,0 +1,92 @@
+// Ligra-style traversal over the tiered graph (include/Map.cpp,
+// include/VertexSubset.cpp): frontiers as a sparse queue and/or a dense
+// bitmap, edgeMap with the reference's push/pull switch, and the five
+// algorithms of include/BFS.h, Pagerank.h, Components.h, BC.h and TC.h.
+#pragma once
+
+#include "graph.hpp"
+
+namespace tg {
+
+// Frontier (VertexSubset): sparse queue and/or dense bitmap, or "all".
+struct Frontier {
+  uint32_t n = 0;
+  bool all = false;
+  bool has_queue = false;  // queue[0..count) valid
+  bool has_bits = false;   // bits valid
+  uint64_t count = 0;      // |frontier|
+  DevBuf<uint32_t> queue;
+  DevBuf<uint32_t> bits;
+  uint64_t words() const { return (static_cast<uint64_t>(n) + 31) / 32; }
+};
+
+enum class Op : int { BFS = 1, CC = 2, BC_FWD = 3, BC_BACK = 4, PR = 5, TC = 6 };
+
+// Device state an op reads and writes (pointers into device memory).
+struct OpState {
+  int32_t *parent = nullptr;    // BFS
+  uint32_t *level = nullptr;    // BFS (optional)
+  uint32_t depth = 0;           // BFS: level assigned to newly reached vertices
+  uint32_t *ids = nullptr;      // CC
+  const uint32_t *prev = nullptr;  // CC prevIDs (output = ids != prev)
+  double *val = nullptr;        // BC NumPaths / Dependencies, PR p_next
+  const double *src_val = nullptr;  // PR normalised p_curr
+  const uint8_t *visited = nullptr;  // BC
+};
+
+class Traversal {
+ public:
+  explicit Traversal(Graph &g) : g_(g) {}
+
+  // Frontier helpers
+  void frontier_single(Frontier &f, uint32_t v);
+  void frontier_all(Frontier &f);
+  void frontier_empty(Frontier &f);
+  void to_bits(Frontier &f);
+  void to_queue(Frontier &f);
+
+  // edgeMap(G, in, F, output, threshold) (include/Map.cpp:66-74): dense
+  // pull when n / threshold <= |in|, else sparse push.  out receives the
+  // output frontier when output is true.
+  void edge_map(Frontier &in, Op op, const OpState &st, Frontier *out, uint32_t threshold = 20);
+
+  // Whole algorithms (device result buffers).
+  uint32_t bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t threshold = 20);
+  void pagerank(uint32_t iters, double *out);
+  uint32_t cc(uint32_t *ids, uint32_t threshold = 20);
+  void bc(uint32_t src, double *out, uint32_t threshold = 20);
+  uint64_t tc();
+
+  // Per-kernel timing on the graph's stream (hipEvents), keyed by name.
+  struct KTime {
+    const char *name;
+    double ms;
+    uint64_t launches;
+  };
+  bool profiling = false;
+  std::vector<KTime> ktimes;
+  void record(const char *name, hipEvent_t a, hipEvent_t b);
+
+  uint64_t last_edges_visited = 0;  // Σ degree over processed frontiers
+
+ private:
+  Graph &g_;
+  DevBuf<uint32_t> cnt_;        // device counters
+  DevBuf<uint32_t> sec_pma_, sec_tree_;  // frontier vertices with secondary runs
+  DevBuf<uint64_t> ttask_off_;  // tree task prefix over listed tree vertices
+  DevBuf<double> dtmp_, dtmp2_;
+  DevBuf<uint32_t> utmp_;
+  DevBuf<uint8_t> btmp_;
+  DevBuf<uint64_t> csr_off_;
+  DevBuf<uint32_t> csr_nbr_;
+  DevBuf<uint64_t> red_;
+  uint64_t ntree_tasks_ = 0;
+  const uint32_t *tree_list_ = nullptr;
+  uint32_t ntree_list_ = 0;
+  void prepare_tree_tasks();
+  void pull(Frontier &in, Op op, const OpState &st, Frontier *out);
+  void push(Frontier &in, Op op, const OpState &st, Frontier *out);
+  uint64_t popcount_bits(const Frontier &f);
+};
+
+}  // namespace tg

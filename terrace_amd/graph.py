@@ -1,0 +1,265 @@
+"""Host-side mirror of the reference's graphstore::Graph and Ligra surface.
+
+Same names and argument meaning as include/graph.h (class Graph),
+include/VertexSubset.cpp (VertexSubset), include/Map.cpp (edgeMap) and the
+algorithm headers (BFS_with_edge_map, PR_S, CC, BC, TC), backed by
+libterrace_hip.so on the GPU.  Every call goes through the C ABI; there is
+no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import (OP_BC_BACK, OP_BC_FWD, OP_BFS, OP_CC, OP_PR, DeviceBuffer, OpArgs,
+                   TerraceError, check, lib, ptr)
+
+__all__ = ["Graph", "VertexSubset", "edgeMap", "BFS_with_edge_map", "PR_S", "CC", "BC", "TC",
+           "rmat_device", "TerraceError", "DeviceBuffer"]
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class Graph:
+    """graphstore::Graph (include/graph.h:73-230) resident in HBM."""
+
+    def __init__(self, size: int):
+        h = ctypes.c_void_p()
+        check(lib().tg_create(int(size), ctypes.byref(h)))
+        self._h = h
+        self._n = int(size)
+
+    # -- lifetime
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().tg_destroy(self._h)
+        self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    # -- updates (graph.h:434-502, 507-641, 675-791)
+    def add_edge_batch(self, srcs, dests, edge_count=None, perm=None) -> None:
+        """Insert edges; any order, duplicates ignored.  perm is accepted for
+        signature compatibility and ignored (the device sorts)."""
+        s, d = _u32(srcs), _u32(dests)
+        if len(s) != len(d):
+            raise ValueError("srcs and dests differ in length")
+        n = len(s) if edge_count is None else int(edge_count)
+        check(lib().tg_insert_batch(self._h, ptr(s), ptr(d), n))
+
+    def remove_edge_batch(self, srcs, dests, edge_count=None) -> None:
+        s, d = _u32(srcs), _u32(dests)
+        if len(s) != len(d):
+            raise ValueError("srcs and dests differ in length")
+        n = len(s) if edge_count is None else int(edge_count)
+        check(lib().tg_delete_batch(self._h, ptr(s), ptr(d), n))
+
+    def add_edge_batch_dev(self, src: DeviceBuffer, dst: DeviceBuffer, count: int) -> None:
+        check(lib().tg_insert_batch_dev(self._h, src.ptr, dst.ptr, int(count)))
+
+    def remove_edge_batch_dev(self, src: DeviceBuffer, dst: DeviceBuffer, count: int) -> None:
+        check(lib().tg_delete_batch_dev(self._h, src.ptr, dst.ptr, int(count)))
+
+    def add_edge(self, s: int, d: int) -> int:
+        r = ctypes.c_int()
+        check(lib().tg_add_edge(self._h, s, d, ctypes.byref(r)))
+        return r.value
+
+    def remove_edge(self, s: int, d: int) -> int:
+        r = ctypes.c_int()
+        check(lib().tg_remove_edge(self._h, s, d, ctypes.byref(r)))
+        return r.value
+
+    # -- queries
+    def is_edge(self, s: int, d: int) -> int:
+        r = ctypes.c_uint32()
+        check(lib().tg_is_edge(self._h, s, d, ctypes.byref(r)))
+        return r.value
+
+    def is_edge_batch(self, srcs, dests) -> np.ndarray:
+        s, d = _u32(srcs), _u32(dests)
+        out = np.zeros(len(s), np.uint8)
+        check(lib().tg_is_edge_batch(self._h, ptr(s), ptr(d), len(s), ptr(out)))
+        return out
+
+    def degree(self, v: int) -> int:
+        r = ctypes.c_uint32()
+        check(lib().tg_degree(self._h, v, ctypes.byref(r)))
+        return r.value
+
+    def degrees(self) -> np.ndarray:
+        out = np.empty(self._n, np.uint32)
+        check(lib().tg_degrees_all(self._h, ptr(out)))
+        return out
+
+    def neighbors(self, v: int) -> np.ndarray:
+        deg = ctypes.c_uint32()
+        check(lib().tg_neighbors(self._h, v, None, 0, ctypes.byref(deg)))
+        out = np.empty(deg.value, np.uint32)
+        if deg.value:
+            check(lib().tg_neighbors(self._h, v, ptr(out), deg.value, ctypes.byref(deg)))
+        return out
+
+    def get_num_edges(self) -> int:
+        r = ctypes.c_uint64()
+        check(lib().tg_num_edges(self._h, ctypes.byref(r)))
+        return r.value
+
+    def get_num_vertices(self) -> int:
+        return self._n
+
+    def get_size(self) -> int:
+        r = ctypes.c_uint64()
+        check(lib().tg_memory_bytes(self._h, ctypes.byref(r)))
+        return r.value
+
+    def export_adjacency(self):
+        """(offsets[n+1] uint64, neighbours[m] uint32), sorted per vertex."""
+        m = self.get_num_edges()
+        off = np.empty(self._n + 1, np.uint64)
+        nbr = np.empty(max(m, 1), np.uint32)
+        check(lib().tg_export_adjacency(self._h, ptr(off), ptr(nbr)))
+        return off, nbr[:m]
+
+    def tiers(self) -> np.ndarray:
+        out = np.empty(self._n, np.uint8)
+        check(lib().tg_tiers(self._h, ptr(out)))
+        return out
+
+    def stats(self) -> dict:
+        out = np.zeros(8, np.uint64)
+        check(lib().tg_stats(self._h, ptr(out)))
+        keys = ["pma_rebuilds", "pma_windows", "pma_window_leaves", "tree_blocks_written",
+                "promotions", "trees", "pma_slots"]
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    # -- profiling
+    def profile(self, on: bool = True) -> None:
+        check(lib().tg_profile_enable(self._h, int(on)))
+
+    def profile_reset(self) -> None:
+        check(lib().tg_profile_reset(self._h))
+
+    def profile_get(self, name: str):
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        check(lib().tg_profile_get(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+class VertexSubset:
+    """VertexSubset (include/VertexSubset.cpp): VertexSubset(e, n) holds one
+    vertex, VertexSubset(0, n, all=True) holds every vertex."""
+
+    def __init__(self, G: Graph, e=None, all: bool = False):
+        h = ctypes.c_void_p()
+        check(lib().tg_frontier_create(G.handle, ctypes.byref(h)))
+        self._h, self._g = h, G
+        if all:
+            check(lib().tg_frontier_set_all(h))
+        elif e is not None:
+            v = _u32(np.atleast_1d(e))
+            check(lib().tg_frontier_set(h, ptr(v), len(v)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get_n(self) -> int:
+        r = ctypes.c_uint64()
+        check(lib().tg_frontier_size(self._h, ctypes.byref(r)))
+        return r.value
+
+    def not_empty(self) -> bool:
+        return self.get_n() > 0
+
+    def to_array(self) -> np.ndarray:
+        c = ctypes.c_uint64()
+        n = self.get_n()
+        out = np.empty(max(n, 1), np.uint32)
+        check(lib().tg_frontier_get(self._h, ptr(out), n, ctypes.byref(c)))
+        return out[:c.value]
+
+    def del_(self) -> None:
+        if self._h is not None and self._h.value:
+            lib().tg_frontier_destroy(self._h)
+        self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.del_()
+        except Exception:
+            pass
+
+
+def edgeMap(G: Graph, vs: VertexSubset, op: int, args: OpArgs, output: bool = True,
+            threshold: int = 20):
+    """edgeMap(G, vs, F, output, threshold) (include/Map.cpp:66-74) with F
+    one of the device ops (OP_BFS, OP_CC, OP_BC_FWD, OP_BC_BACK, OP_PR)."""
+    out = VertexSubset(G) if output else None
+    check(lib().tg_edge_map(G.handle, vs.handle, int(op), ctypes.byref(args),
+                            out.handle if out else None, int(threshold)))
+    return out
+
+
+# -- algorithm entry points (src/graph_bm.cc test_* drivers)
+def BFS_with_edge_map(G: Graph, src: int, with_levels: bool = False):
+    """include/BFS.h:41-84: parents (int32, -1 unreached)."""
+    parents = np.empty(G.get_num_vertices(), np.int32)
+    levels = np.empty(G.get_num_vertices(), np.uint32)
+    check(lib().tg_bfs(G.handle, int(src), ptr(parents), ptr(levels)))
+    return (parents, levels) if with_levels else parents
+
+
+def PR_S(G: Graph, maxIters: int) -> np.ndarray:
+    """include/Pagerank.h:106-184 (undamped, FP64)."""
+    out = np.empty(G.get_num_vertices(), np.float64)
+    check(lib().tg_pagerank(G.handle, int(maxIters), ptr(out)))
+    return out
+
+
+def CC(G: Graph) -> np.ndarray:
+    """include/Components.h:92-120."""
+    out = np.empty(G.get_num_vertices(), np.uint32)
+    check(lib().tg_cc(G.handle, ptr(out)))
+    return out
+
+
+def BC(G: Graph, start: int) -> np.ndarray:
+    """include/BC.h:99-158 (unreachable -> NaN)."""
+    out = np.empty(G.get_num_vertices(), np.float64)
+    check(lib().tg_bc(G.handle, int(start), ptr(out)))
+    return out
+
+
+def TC(G: Graph) -> int:
+    """include/TC.h:73-87."""
+    r = ctypes.c_uint64()
+    check(lib().tg_tc(G.handle, ctypes.byref(r)))
+    return r.value
+
+
+def rmat_device(n: int, seed: int, a: float, b: float, c: float, count: int, start: int = 0,
+                nominal: bool = True, symmetric: bool = False, drop_loops: bool = False):
+    """rMat<uint32_t> edges generated in HBM (include/rmat_util.h:81-115).
+    Returns (src DeviceBuffer, dst DeviceBuffer, entries written)."""
+    cap = count * (2 if symmetric else 1)
+    s, d = DeviceBuffer(cap * 4), DeviceBuffer(cap * 4)
+    w = ctypes.c_uint64()
+    flags = (1 if symmetric else 0) | (2 if drop_loops else 0)
+    check(lib().tg_rmat_dev(int(n), int(seed) & 0xFFFFFFFF, a, b, c, int(nominal), int(start), int(count),
+                            flags, s.ptr, d.ptr, ctypes.byref(w)))
+    return s, d, w.value
+
+
+OPS = {"BFS": OP_BFS, "CC": OP_CC, "BC_FWD": OP_BC_FWD, "BC_BACK": OP_BC_BACK, "PR": OP_PR}

@@ -1,0 +1,264 @@
+"""GPU parity: the HIP path through the C ABI against the oracle and the
+reference-build golden fixtures.
+
+Bars: adjacency, degrees, BFS levels, CC labels, TC and tier assignment
+bit-exact; BFS parents validated; PR and BC within 1e-9 relative L1 (FP64,
+different summation order).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from terrace_amd import graph as tg
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+GRAPHS = ["graph_s10", "graph_skew12", "graph_promote11"]
+UNREACHED = np.iinfo(np.uint32).max
+FP_TOL = 1e-9
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def split(keys):
+    keys = np.asarray(keys, np.uint64)
+    return ((keys >> np.uint64(32)).astype(np.uint32),
+            (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+
+
+def assert_adjacency(G, keys, n, what=""):
+    off, nbr = G.export_adjacency()
+    ref = oracle.CSR(keys, n)
+    deg = (off[1:] - off[:-1]).astype(np.uint32)
+    bad = np.nonzero(deg != ref.degrees())[0]
+    assert len(bad) == 0, f"{what}: degree mismatch at {bad[:10]}"
+    assert np.array_equal(nbr, ref.nbr[:ref.m]), f"{what}: neighbour lists differ"
+    assert np.array_equal(G.degrees(), ref.degrees()), what
+    assert G.get_num_edges() == ref.m
+
+
+def check_algorithms(G, g, src, iters=10, what=""):
+    parents, levels = tg.BFS_with_edge_map(G, src, with_levels=True)
+    lv = oracle.bfs_levels(g, src)
+    assert np.array_equal(levels, lv), f"{what}: BFS levels differ"
+    assert oracle.bfs_check_parents(g, src, parents, lv) == 0, f"{what}: BFS parents invalid"
+    pr = tg.PR_S(G, iters)
+    assert oracle.rel_l1(pr, oracle.pagerank(g, iters)) <= FP_TOL, f"{what}: PR"
+    assert np.array_equal(tg.CC(G), oracle.cc(g)), f"{what}: CC"
+    bc = tg.BC(G, src)
+    ref = oracle.bc(g, src)
+    reach = lv != UNREACHED
+    assert np.all(np.isnan(bc[~reach])), f"{what}: BC unreachable not NaN"
+    assert oracle.rel_l1(bc[reach], ref[reach]) <= FP_TOL, f"{what}: BC"
+    assert tg.TC(G) == oracle.tc(g), f"{what}: TC"
+
+
+# ------------------------------------------------------------------ generator
+@pytest.mark.parametrize("nominal", [False, True])
+def test_rmat_device_matches_oracle(nominal):
+    n, seed, count = 1 << 20, oracle.aspen_ith_rand(100) & 0xFFFFFFFF, 200000
+    say("rmat", nominal, n, count)
+    s, d, w = tg.rmat_device(n, seed, 0.5, 0.1, 0.1, count, nominal=nominal)
+    assert w == count
+    gs, gd = s.download(np.uint32, count), d.download(np.uint32, count)
+    os_, od = oracle.rmat_edges(n, seed, 0.5, 0.1, 0.1, count, nominal=nominal)
+    got = np.sort((gs.astype(np.uint64) << np.uint64(32)) | gd)
+    want = np.sort((os_.astype(np.uint64) << np.uint64(32)) | od)
+    assert np.array_equal(got, want)
+    # symmetric + loop-free form equals the oracle's key build
+    s, d, w = tg.rmat_device(n, seed, 0.5, 0.1, 0.1, count, nominal=nominal, symmetric=True,
+                             drop_loops=True)
+    keys = (s.download(np.uint32, w).astype(np.uint64) << np.uint64(32)) | d.download(np.uint32, w)
+    assert np.array_equal(np.unique(keys), oracle.build_keys(os_, od))
+
+
+# ------------------------------------------------------------------ golden graphs
+@pytest.mark.parametrize("name", GRAPHS)
+def test_golden_graph(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    n = int(z["n"][0])
+    say("golden", name, n)
+    G = tg.Graph(n)
+    keys = z["base"]
+    tiers = oracle.tier_model(n)
+    oracle.tier_insert(tiers, oracle.CSR(np.zeros(0, np.uint64), n), keys)
+    s, d = split(keys)
+    G.add_edge_batch(s, d)
+    for k in range(int(z["nbatches"][0]) + 1):
+        if k:
+            b = z[f"batch{k - 1}"]
+            oracle.tier_insert(tiers, oracle.CSR(keys, n), b)
+            s, d = split(b)
+            G.add_edge_batch(s, d)
+            keys = oracle.set_union(keys, b)
+        off, nbr = G.export_adjacency()
+        assert np.array_equal((off[1:] - off[:-1]).astype(np.uint32), z[f"deg{k}"]), (name, k)
+        assert np.array_equal(nbr, z[f"nbrs{k}"]), (name, k)
+        t = G.tiers()
+        assert np.array_equal(t == 2, tiers["btree"] == 1), (name, k, "B-tree tier")
+    g = oracle.CSR(keys, n)
+    src = int(z["src"][0])
+    parents, levels = tg.BFS_with_edge_map(G, src, with_levels=True)
+    assert oracle.bfs_check_parents(g, src, parents, levels) == 0
+    assert np.array_equal(levels, oracle.bfs_levels(g, src))
+    assert oracle.rel_l1(tg.PR_S(G, int(z["iters"][0])), z["pr"]) <= FP_TOL
+    assert np.array_equal(tg.CC(G), z["cc"])
+    bc = tg.BC(G, src)
+    reach = levels != UNREACHED
+    assert oracle.rel_l1(bc[reach], z["bc"][reach]) <= FP_TOL
+    assert np.all(np.isnan(bc[~reach]))
+    assert tg.TC(G) == int(z["tc"][0])
+    G.close()
+
+
+# ------------------------------------------------------------------ updates vs set model
+@pytest.mark.parametrize("seed,skew", [(1, False), (2, True)])
+def test_mixed_updates_match_set_model(seed, skew):
+    n = 1 << 12
+    a, b, c = (0.57, 0.19, 0.19) if skew else (0.5, 0.1, 0.1)
+    say("mixed", seed, skew)
+    rng = np.random.default_rng(seed)
+    G = tg.Graph(n)
+    keys = oracle.build_keys(*oracle.rmat_edges(n, seed, a, b, c, 40000))
+    G.add_edge_batch(*split(keys))
+    assert_adjacency(G, keys, n, "static")
+    for rnd in range(6):
+        ins = oracle.build_keys(*oracle.rmat_edges(n, 100 * seed + rnd, a, b, c, 3000))
+        # deletes: existing edges (both directions) plus absent ones
+        pick = keys[rng.choice(len(keys), size=min(len(keys), 2500), replace=False)]
+        ps, pd = split(pick)
+        dels = oracle.sort_unique(np.concatenate([pick, (pd.astype(np.uint64) << np.uint64(32)) | ps,
+                                                  oracle.build_keys(*oracle.rmat_edges(n, 999 + rnd, a, b, c, 50))]))
+        G.add_edge_batch(*split(ins))
+        keys = oracle.set_union(keys, ins)
+        assert_adjacency(G, keys, n, f"insert {rnd}")
+        G.remove_edge_batch(*split(dels))
+        keys = oracle.set_difference(keys, dels)
+        assert_adjacency(G, keys, n, f"delete {rnd}")
+    g = oracle.CSR(keys, n)
+    check_algorithms(G, g, int(np.argmax(g.degrees())), what="mixed")
+    G.close()
+
+
+def test_delete_everything_then_reinsert():
+    n = 1 << 11
+    say("delete-all")
+    hub = np.arange(20, 1600, dtype=np.uint32)
+    keys = oracle.sort_unique(np.concatenate([
+        oracle.build_keys(*oracle.rmat_edges(n, 5, 0.57, 0.19, 0.19, 30000)),
+        oracle.build_keys(np.full(len(hub), 3, np.uint32), hub)]))
+    G = tg.Graph(n)
+    G.add_edge_batch(*split(keys))
+    assert (G.tiers() == 2).any()
+    half = keys[::2]
+    G.remove_edge_batch(*split(half))
+    rest = oracle.set_difference(keys, half)
+    assert_adjacency(G, rest, n, "half")
+    G.remove_edge_batch(*split(rest))
+    assert G.get_num_edges() == 0
+    assert_adjacency(G, np.zeros(0, np.uint64), n, "empty")
+    # B-trees are never demoted (include/graph.h); they keep working empty
+    G.add_edge_batch(*split(keys))
+    assert_adjacency(G, keys, n, "reinsert")
+    G.close()
+
+
+def test_promotion_and_hub_shrink():
+    n = 4096
+    say("promotion")
+    G = tg.Graph(n)
+    e = lambda s, ds: np.array([(s << 32) | d for d in ds], np.uint64)  # noqa: E731
+    keys = e(7, range(100, 900))
+    G.add_edge_batch(*split(keys))
+    assert G.tiers()[7] == 1
+    more = e(7, range(900, 1300))
+    G.add_edge_batch(*split(more))
+    keys = oracle.set_union(keys, more)
+    assert G.tiers()[7] == 2
+    assert_adjacency(G, keys, n, "promoted")
+    drop = e(7, range(100, 1290))
+    G.remove_edge_batch(*split(drop))
+    keys = oracle.set_difference(keys, drop)
+    assert_adjacency(G, keys, n, "shrunk")
+    assert G.degree(7) == 10 and G.tiers()[7] == 2
+    assert G.neighbors(7).tolist() == list(range(1290, 1300))
+    G.close()
+
+
+# ------------------------------------------------------------------ API edge cases
+def test_api_edge_cases():
+    say("api")
+    G = tg.Graph(100)
+    G.add_edge_batch([], [])
+    G.remove_edge_batch([], [])
+    assert G.get_num_edges() == 0
+    # duplicates and self-loops inside one batch
+    G.add_edge_batch([1, 1, 1, 5, 5], [2, 2, 1, 6, 6])
+    assert G.get_num_edges() == 3 and G.is_edge(1, 1) == 1 and G.is_edge(1, 2) == 1
+    assert G.is_edge(2, 1) == 0
+    assert G.add_edge(1, 2) == -1 and G.add_edge(2, 1) == 0
+    assert G.remove_edge(2, 1) == 0 and G.remove_edge(2, 1) == -1
+    assert G.neighbors(1).tolist() == [1, 2]
+    with pytest.raises(tg.TerraceError) as ei:
+        G.add_edge_batch([1, 200], [2, 3])
+    assert ei.value.code == -1
+    assert G.get_num_edges() == 3  # rejected batch changed nothing
+    with pytest.raises(tg.TerraceError):
+        G.degree(100)
+    with pytest.raises(tg.TerraceError):
+        tg.BFS_with_edge_map(G, 1000)
+    # BFS from an isolated vertex
+    parents, levels = tg.BFS_with_edge_map(G, 50, with_levels=True)
+    assert parents[50] == 50 and (parents >= 0).sum() == 1 and levels[50] == 0
+    bc = tg.BC(G, 50)
+    assert bc[50] == 0.0 and np.isnan(bc[:50]).all()
+    G.close()
+
+
+def test_edge_map_frontier_api():
+    n = 1 << 10
+    say("edge_map api")
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 9, 0.5, 0.1, 0.1, 5000))
+    G = tg.Graph(n)
+    G.add_edge_batch(*split(keys))
+    g = oracle.CSR(keys, n)
+    src = int(np.argmax(g.degrees()))
+    lv = oracle.bfs_levels(g, src)
+    parent = tg.DeviceBuffer(n * 4)
+    parent.upload(np.where(np.arange(n) == src, src, -1).astype(np.int32))
+    vs = tg.VertexSubset(G, src)
+    depth = 0
+    for threshold in (20,):
+        while vs.not_empty():
+            depth += 1
+            args = tg.OpArgs(parent=parent.ptr, depth=depth)
+            nxt = tg.edgeMap(G, vs, tg.OPS["BFS"], args, True, threshold)
+            assert nxt.to_array().tolist() == np.nonzero(lv == depth)[0].tolist(), depth
+            vs = nxt
+    G.close()
+
+
+# ------------------------------------------------------------------ larger scale
+def test_scale16_static_and_batches():
+    n = 1 << 16
+    say("scale16")
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 77, 0.57, 0.19, 0.19, 600000))
+    G = tg.Graph(n)
+    s, d, w = tg.rmat_device(n, 77, 0.57, 0.19, 0.19, 600000, symmetric=True, drop_loops=True)
+    G.add_edge_batch_dev(s, d, w)
+    assert_adjacency(G, keys, n, "static dev")
+    for t in range(3):
+        b = oracle.build_keys(*oracle.rmat_edges(n, 1000 + t, 0.57, 0.19, 0.19, 20000 * (t + 1)))
+        G.add_edge_batch(*split(b))
+        keys = oracle.set_union(keys, b)
+    assert_adjacency(G, keys, n, "after batches")
+    g = oracle.CSR(keys, n)
+    check_algorithms(G, g, int(np.argmax(g.degrees())), what="scale16")
+    G.close()
