@@ -130,6 +130,30 @@ int tg_frontier_get(tg_frontier *f, uint32_t *out, uint64_t cap, uint64_t *count
 int tg_edge_map(tg_graph *g, tg_frontier *in, int op, const tg_op_args *args, tg_frontier *out,
                 uint32_t threshold);
 
+/* vertexMap(vs, F, output=false) (include/Map.cpp:76-108) for the vertex
+ * functors of the algorithm headers, applied to the members of f:
+ *   PR_VERTEX      val[i] = val[i] / degree(i)        (Pagerank.h:61-69)
+ *   PR_RESET       val[i] = 0                         (Pagerank.h:72-81)
+ *   CC_VERTEX      prev[i] = ids[i]                   (Components.h:37-43)
+ *   BC_VERTEX      visited[i] = 1                     (BC.h:74-81)
+ *   BC_BACK_VERTEX visited[i] = 1; val[i] += inv[i]   (BC.h:84-94)
+ * and the parallel_for loops of BC (BC.h:126-128, 130-132, 140-142):
+ *   INVERT         val[i] = 1 / val[i]
+ *   CLEAR_VISITED  visited[i] = 0
+ *   BC_FINAL       val[i] = (val[i] - inv[i]) / inv[i] */
+enum tg_vop { TG_VOP_PR_VERTEX = 1, TG_VOP_PR_RESET = 2, TG_VOP_CC_VERTEX = 3, TG_VOP_BC_VERTEX = 4,
+              TG_VOP_BC_BACK_VERTEX = 5, TG_VOP_INVERT = 6, TG_VOP_CLEAR_VISITED = 7, TG_VOP_BC_FINAL = 8 };
+typedef struct {
+  double *val;
+  const double *inv;
+  const uint32_t *ids;
+  uint32_t *prev;
+  uint8_t *visited;
+} tg_vmap_args;
+int tg_vertex_map(tg_graph *g, tg_frontier *f, int vop, const tg_vmap_args *args);
+/* Copy of a frontier (VertexSubset(other, true)). */
+int tg_frontier_copy(tg_frontier *dst, tg_frontier *src);
+
 /* ---- workload and device memory helpers -------------------------------
  * rMat<uint32_t>(n, seed, a, b, c)(i) for i in [start, start+count)
  * (include/rmat_util.h:81-115) written straight into device arrays.

@@ -78,6 +78,8 @@ _SIGS = {
     "tg_frontier_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "tg_frontier_get": (ctypes.c_int, [_P, _P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     "tg_edge_map": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(OpArgs), _P, ctypes.c_uint32]),
+    "tg_vertex_map": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
+    "tg_frontier_copy": (ctypes.c_int, [_P, _P]),
     "tg_rmat_dev": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
                                    ctypes.c_int, _P, _P, ctypes.POINTER(ctypes.c_uint64)]),

@@ -53,8 +53,17 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    # C++ benchmark driver over the graphstore shim (host code only)
+    driver = os.path.join(PKG, "host", "graph_bm")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(PKG, "host", "graph_bm.cpp"), "-L" + PKG, "-lterrace_hip",
+           "-Wl,-rpath,$ORIGIN/..", "-o", driver + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"graph_bm build failed:\n{r.stderr[-4000:]}")
+    os.replace(driver + ".tmp", driver)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {driver}")
     return LIB
 
 

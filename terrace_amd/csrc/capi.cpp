@@ -476,6 +476,31 @@ int tg_edge_map(tg_graph *G, tg_frontier *in, int op, const tg_op_args *a, tg_fr
   });
 }
 
+int tg_vertex_map(tg_graph *G, tg_frontier *f, int vop, const tg_vmap_args *a) {
+  return guard([&] {
+    NEED(G && f && a, "null argument");
+    NEED(f->owner == G, "frontier of another graph");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    tg::Traversal::VArgs v;
+    v.val = a->val;
+    v.inv = a->inv;
+    v.ids = a->ids;
+    v.prev = a->prev;
+    v.visited = a->visited;
+    G->t.vertex_map(f->f, vop, v);
+    TG_HIP(hipStreamSynchronize(G->g.stream()));
+  });
+}
+
+int tg_frontier_copy(tg_frontier *dst, tg_frontier *src) {
+  return guard([&] {
+    NEED(dst && src && dst->owner == src->owner, "frontiers of different graphs");
+    std::lock_guard<std::mutex> l(src->owner->g.mu);
+    src->owner->t.copy_frontier(dst->f, src->f);
+    TG_HIP(hipStreamSynchronize(src->owner->g.stream()));
+  });
+}
+
 // ---------------------------------------------------------------- helpers
 int tg_rmat_dev(uint32_t n, uint32_t seed, double a, double b, double c, int nominal, uint64_t start,
                 uint64_t count, int flags, uint32_t *src, uint32_t *dst, uint64_t *written) {

@@ -400,8 +400,11 @@ __global__ void k_clear_marks(GView g, const uint32_t *T, uint32_t nt, const uin
   }
 }
 
-__device__ inline uint32_t task_window(const uint32_t *task_base, uint32_t nw, uint64_t j) {
-  // last w with task_base[w] <= j
+// Window of leaf task j.  tw (task -> window map) is filled by
+// k_task_map; without it, a binary search over task_base.
+__device__ inline uint32_t task_window(const uint32_t *task_base, uint32_t nw, uint64_t j,
+                                       const uint32_t *tw = nullptr) {
+  if (tw) return tw[j];
   uint32_t lo = 0, hi = nw;
   while (lo < hi) {
     uint32_t mid = (lo + hi) >> 1;
@@ -411,10 +414,17 @@ __device__ inline uint32_t task_window(const uint32_t *task_base, uint32_t nw, u
   return lo - 1;
 }
 
+__global__ void k_task_map(const uint32_t *task_base, uint32_t nw, uint32_t *tw) {
+  GRID_STRIDE(w, nw) {
+    uint32_t b = task_base[w], e = task_base[w + 1];
+    for (uint32_t j = b; j < e; j++) tw[j] = static_cast<uint32_t>(w);
+  }
+}
+
 __global__ void k_task_counts(GView g, const Win *win, const uint32_t *task_base, uint32_t nw, uint64_t ntasks,
-                              uint32_t *taskcnt) {
+                              uint32_t *taskcnt, const uint32_t *tw) {
   GRID_STRIDE(j, ntasks) {
-    uint32_t w = task_window(task_base, nw, j);
+    uint32_t w = task_window(task_base, nw, j, tw);
     uint64_t leaf = win[w].lo + (j - task_base[w]);
     taskcnt[j] = g.pcnt[leaf];
   }
@@ -425,12 +435,12 @@ __global__ void k_task_counts(GView g, const Win *win, const uint32_t *task_base
 // fill kernel with atomicMin).
 __global__ void __launch_bounds__(256) k_gather(GView g, const Win *win, const uint32_t *task_base, uint32_t nw,
                                                 uint64_t ntasks, const uint32_t *taskoff, uint64_t *tmp_old,
-                                                uint64_t *err) {
+                                                uint64_t *err, const uint32_t *tw) {
   uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
   uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
   uint32_t lane = threadIdx.x & 63;
   for (uint64_t j = wave; j < ntasks; j += nwaves) {
-    uint32_t w = task_window(task_base, nw, j);
+    uint32_t w = task_window(task_base, nw, j, tw);
     uint64_t leaf = win[w].lo + (j - task_base[w]);
     uint64_t slot = (leaf << kLeafShift) + lane;
     uint32_t d = g.pdst[slot];
@@ -452,12 +462,12 @@ __global__ void __launch_bounds__(256) k_gather(GView g, const Win *win, const u
 // left, as redistribute does in src/PMA.cc:419-544), fences, count tree.
 __global__ void __launch_bounds__(256) k_fill(GView g, const Win *win, const uint32_t *task_base, uint32_t nw,
                                               uint64_t ntasks, const uint32_t *taskoff, const uint64_t *tmp_old,
-                                              const uint64_t *keys) {
+                                              const uint64_t *keys, const uint32_t *tw) {
   uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
   uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
   uint32_t lane = threadIdx.x & 63;
   for (uint64_t j = wave; j < ntasks; j += nwaves) {
-    uint32_t w = task_window(task_base, nw, j);
+    uint32_t w = task_window(task_base, nw, j, tw);
     Win W = win[w];
     uint64_t li = j - task_base[w];
     uint64_t leaf = W.lo + li;
@@ -499,19 +509,41 @@ __global__ void __launch_bounds__(256) k_fill(GView g, const Win *win, const uin
         uint64_t e = (i < na && (jb >= nb || A[i] < B[jb])) ? A[i] : B[jb];
         g.fence[leaf] = e + 1;
       }
-      // internal nodes of the count tree inside the window, closed form
-      for (uint32_t l = 1; l <= W.L; l++) {
-        uint64_t span = 1ull << l;
-        if (li & (span - 1)) break;
-        uint64_t a = li, b = li + span;
-        uint64_t cnt = span * q + ((b < rem ? b : rem) - (a < rem ? a : rem));
-        g.pcnt[tree_level_off(g.H, l) + (leaf >> l)] = static_cast<uint32_t>(cnt);
-      }
-      if (li == 0) {
-        for (uint32_t l = W.L + 1; l <= g.H; l++)
-          atomicAdd(&g.pcnt[tree_level_off(g.H, l) + (W.lo >> l)], static_cast<uint32_t>(nb));
-      }
     }
+  }
+}
+
+// Count tree rebuild, bottom-up from level L0: each block folds 512
+// level-L0 nodes through up to 9 levels in LDS.  Replaces per-window /
+// per-key atomics on shared ancestors.
+__global__ void __launch_bounds__(256) k_tree_up(uint32_t *pcnt, uint32_t H, uint32_t L0) {
+  __shared__ uint32_t sm[256];
+  const uint64_t nin = 1ull << (H - L0);
+  const uint64_t base_in = static_cast<uint64_t>(blockIdx.x) * 512;
+  const uint32_t t = threadIdx.x;
+  const uint64_t i0 = base_in + 2 * t;
+  uint32_t v = 0;
+  if (i0 < nin) v = pcnt[tree_level_off(H, L0) + i0] + (i0 + 1 < nin ? pcnt[tree_level_off(H, L0) + i0 + 1] : 0u);
+  uint32_t lvl = L0 + 1;
+  {
+    uint64_t idx = (base_in >> 1) + t;
+    if (idx < (1ull << (H - lvl))) pcnt[tree_level_off(H, lvl) + idx] = v;
+  }
+  sm[t] = v;
+  __syncthreads();
+  uint32_t width = 256;
+  while (width > 1 && lvl < H) {
+    width >>= 1;
+    lvl++;
+    uint32_t nv = 0;
+    if (t < width) nv = sm[2 * t] + sm[2 * t + 1];
+    __syncthreads();
+    if (t < width) {
+      sm[t] = nv;
+      uint64_t idx = (base_in >> (lvl - L0)) + t;
+      if (idx < (1ull << (H - lvl))) pcnt[tree_level_off(H, lvl) + idx] = nv;
+    }
+    __syncthreads();
   }
 }
 
@@ -527,7 +559,7 @@ __global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t k, uint64_t
     }
     g.pdst[slot] = kEmpty;
     uint64_t leaf = slot >> kLeafShift;
-    for (uint32_t l = 0; l <= g.H; l++) atomicSub(&g.pcnt[tree_level_off(g.H, l) + (leaf >> l)], 1u);
+    atomicSub(&g.pcnt[leaf], 1u);
     if (g.vb[s].aux == slot) fix_slot[i] = slot;
   }
 }
@@ -1231,6 +1263,16 @@ uint64_t Graph::prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t 
 }
 
 // ------------------------------------------------------------ PMA insert
+void Graph::rebuild_count_tree() {
+  if (!tree_dirty_) return;
+  tree_dirty_ = false;
+  for (uint32_t L0 = 0; L0 < pma_H_; L0 += 9) {
+    uint64_t nin = 1ull << (pma_H_ - L0);
+    k_tree_up<<<grid_for(nin, 512, 1ull << 31), 256, 0, stream_>>>(pcnt_.get(), pma_H_, L0);
+    TG_CHECK_LAUNCH();
+  }
+}
+
 void Graph::pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total) {
   // gather every old element, grow, spread merge(old, keys) over the array
   uint64_t old_total = total - k;
@@ -1238,7 +1280,7 @@ void Graph::pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total) {
   GView ov = view();
   uint64_t onl = 1ull << pma_H_;
   // one window covering the whole old array
-  DevBuf<uint8_t> winbuf;
+  DevBuf<uint8_t> &winbuf = s8_[0];
   winbuf.ensure(sizeof(Win) + 64, stream_);
   Win w0{0, pma_H_, 0, static_cast<uint32_t>(k), static_cast<uint32_t>(old_total)};
   TG_HIP(hipMemcpyAsync(winbuf.get(), &w0, sizeof(Win), hipMemcpyHostToDevice, stream_));
@@ -1247,12 +1289,12 @@ void Graph::pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total) {
   u1_.ensure(2, stream_);
   fill_u32(u1_.get(), 0, 1, stream_);  // task_base = {0}
   const Win *dw = reinterpret_cast<const Win *>(winbuf.get());
-  k_task_counts<<<grid_for(onl, 256), 256, 0, stream_>>>(ov, dw, u1_.get(), 1, onl, u0_.get());
+  k_task_counts<<<grid_for(onl, 256), 256, 0, stream_>>>(ov, dw, u1_.get(), 1, onl, u0_.get(), nullptr);
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(u0_.get(), u0_.get(), onl, u0_.get() + onl, scan_scratch, stream_);
   fill_u64(scal_.get(), 0, 1, stream_);
   k_gather<<<grid_for(onl * 64, 256), 256, 0, stream_>>>(ov, dw, u1_.get(), 1, onl, u0_.get(), tmp_old_.get(),
-                                                         scal_.get());
+                                                         scal_.get(), nullptr);
   TG_CHECK_LAUNCH();
   if (read_scalar(scal_.get(), stream_)) throw Error(-3, "PMA count tree out of sync (gather)");
   // new geometry: density about 0.5 after the rebuild
@@ -1273,8 +1315,10 @@ void Graph::pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total) {
   TG_HIP(hipStreamSynchronize(stream_));
   uint64_t nl = 1ull << pma_H_;
   // taskoff for the single window starts at 0: reuse u0_[0] == 0
-  k_fill<<<grid_for(nl * 64, 256), 256, 0, stream_>>>(nv, dw, u1_.get(), 1, nl, u0_.get(), tmp_old_.get(), keys);
+  k_fill<<<grid_for(nl * 64, 256), 256, 0, stream_>>>(nv, dw, u1_.get(), 1, nl, u0_.get(), tmp_old_.get(), keys,
+                                                     nullptr);
   TG_CHECK_LAUNCH();
+  tree_dirty_ = true;
   TG_HIP(hipStreamSynchronize(stream_));
   stats_.pma_rebuilds++;
 }
@@ -1282,6 +1326,7 @@ void Graph::pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total) {
 void Graph::pma_insert(const uint64_t *keys, uint64_t k) {
   if (k == 0) return;
   if (k > 0xFFFFFFFFull) throw Error(-1, "batch too large");
+  rebuild_count_tree();
   GView g = view();
   uint64_t nl = 1ull << pma_H_;
   uint64_t root_old = read_scalar(pcnt_.get() + tree_level_off(pma_H_, pma_H_), stream_);
@@ -1329,18 +1374,23 @@ void Graph::pma_insert(const uint64_t *keys, uint64_t k) {
   uint64_t ntasks = read_scalar(task_base + nw, stream_);
   if (ntasks > nl) throw Error(-3, "PMA window tasks exceed leaves");
   u4_.ensure(ntasks + 1, stream_);
-  k_task_counts<<<grid_for(ntasks, 256), 256, 0, stream_>>>(g, win, task_base, nw, ntasks, u4_.get());
+  u3_.ensure(ntasks + 1, stream_);
+  uint32_t *tw = u3_.get();  // cover[] is no longer needed
+  k_task_map<<<grid_for(nw, 256), 256, 0, stream_>>>(task_base, nw, tw);
+  TG_CHECK_LAUNCH();
+  k_task_counts<<<grid_for(ntasks, 256), 256, 0, stream_>>>(g, win, task_base, nw, ntasks, u4_.get(), tw);
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(u4_.get(), u4_.get(), ntasks, u4_.get() + ntasks, scan_scratch, stream_);
   uint64_t nold = read_scalar(u4_.get() + ntasks, stream_);
   tmp_old_.ensure(nold + 1, stream_);
   fill_u64(scal_.get(), 0, 1, stream_);
   k_gather<<<grid_for(ntasks * 64, 256), 256, 0, stream_>>>(g, win, task_base, nw, ntasks, u4_.get(),
-                                                            tmp_old_.get(), scal_.get());
+                                                            tmp_old_.get(), scal_.get(), tw);
   TG_CHECK_LAUNCH();
   k_fill<<<grid_for(ntasks * 64, 256), 256, 0, stream_>>>(g, win, task_base, nw, ntasks, u4_.get(), tmp_old_.get(),
-                                                          keys);
+                                                          keys, tw);
   TG_CHECK_LAUNCH();
+  tree_dirty_ = true;
   if (read_scalar(scal_.get(), stream_)) throw Error(-3, "PMA count tree out of sync (gather)");
   stats_.pma_windows += nw;
   stats_.pma_window_leaves += ntasks;
@@ -1353,6 +1403,7 @@ void Graph::pma_delete(const uint64_t *keys, uint64_t k) {
   fill_u64(scal_.get(), 0, 1, stream_);
   k_pma_delete<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g2_.get(), scal_.get());
   TG_CHECK_LAUNCH();
+  tree_dirty_ = true;
   k_pma_fix_begin<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g2_.get());
   TG_CHECK_LAUNCH();
   if (read_scalar(scal_.get(), stream_)) throw Error(-3, "PMA delete: key not found");
@@ -1371,10 +1422,11 @@ void Graph::tree_insert(const uint64_t *keys, uint64_t k) {
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(u0_.get(), u1_.get(), k, u1_.get() + k, scan_scratch, stream_);
   uint32_t ng = read_scalar(u1_.get() + k, stream_);
-  DevBuf<uint8_t> grpbuf;
+  DevBuf<uint8_t> &grpbuf = s8_[1];
   grpbuf.ensure(static_cast<uint64_t>(ng + 1) * sizeof(Grp), stream_);
   Grp *grp = reinterpret_cast<Grp *>(grpbuf.get());
-  DevBuf<uint32_t> gkb, ob, ho, gflag, gpos, grun;
+  DevBuf<uint32_t> &gkb = s32_[20], &ob = s32_[21], &ho = s32_[22], &gflag = s32_[23], &gpos = s32_[24],
+                   &grun = s32_[25];
   gkb.ensure(ng + 1, stream_);
   ob.ensure(ng + 1, stream_);
   ho.ensure(ng + 1, stream_);
@@ -1403,10 +1455,10 @@ void Graph::tree_insert(const uint64_t *keys, uint64_t k) {
   uint32_t ntr = read_scalar(gpos.get() + ng, stream_);
   k_grp_run<<<grid_for(ng, 256), 256, 0, stream_>>>(gflag.get(), gpos.get(), ng, grun.get());
   TG_CHECK_LAUNCH();
-  DevBuf<uint8_t> trbuf;
+  DevBuf<uint8_t> &trbuf = s8_[2];
   trbuf.ensure(static_cast<uint64_t>(ntr + 1) * sizeof(TreeRun), stream_);
   TreeRun *tr = reinterpret_cast<TreeRun *>(trbuf.get());
-  DevBuf<uint32_t> trs, nbn, nbo;
+  DevBuf<uint32_t> &trs = s32_[26], &nbn = s32_[27], &nbo = s32_[28];
   trs.ensure(ntr + 1, stream_);
   nbn.ensure(ntr + 1, stream_);
   nbo.ensure(ntr + 1, stream_);
@@ -1447,10 +1499,11 @@ void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(u0_.get(), u1_.get(), k, u1_.get() + k, scan_scratch, stream_);
   uint32_t ng = read_scalar(u1_.get() + k, stream_);
-  DevBuf<uint8_t> grpbuf;
+  DevBuf<uint8_t> &grpbuf = s8_[1];
   grpbuf.ensure(static_cast<uint64_t>(ng + 1) * sizeof(Grp), stream_);
   Grp *grp = reinterpret_cast<Grp *>(grpbuf.get());
-  DevBuf<uint32_t> gkb, ob, ho, gflag, gpos, grun;
+  DevBuf<uint32_t> &gkb = s32_[20], &ob = s32_[21], &ho = s32_[22], &gflag = s32_[23], &gpos = s32_[24],
+                   &grun = s32_[25];
   gkb.ensure(ng + 1, stream_);
   ob.ensure(ng + 1, stream_);
   ho.ensure(ng + 1, stream_);
@@ -1478,10 +1531,10 @@ void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
   uint32_t ntr = read_scalar(gpos.get() + ng, stream_);
   k_grp_run<<<grid_for(ng, 256), 256, 0, stream_>>>(gflag.get(), gpos.get(), ng, grun.get());
   TG_CHECK_LAUNCH();
-  DevBuf<uint8_t> trbuf;
+  DevBuf<uint8_t> &trbuf = s8_[2];
   trbuf.ensure(static_cast<uint64_t>(ntr + 1) * sizeof(TreeRun), stream_);
   TreeRun *tr = reinterpret_cast<TreeRun *>(trbuf.get());
-  DevBuf<uint32_t> trs, nbn, nbo;
+  DevBuf<uint32_t> &trs = s32_[26], &nbn = s32_[27], &nbo = s32_[28];
   trs.ensure(ntr + 1, stream_);
   nbn.ensure(ntr + 1, stream_);
   nbo.ensure(ntr + 1, stream_);
@@ -1535,8 +1588,9 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(rflag, rpos, nR, rpos + nR, scan_scratch, stream_);
   uint32_t nruns = read_scalar(rpos + nR, stream_);
-  DevBuf<uint32_t> run_beg, run_of, run_c, run_nko, kick_old, promote, ppos;
-  DevBuf<uint8_t> run_route, kroute;
+  DevBuf<uint32_t> &run_beg = s32_[0], &run_of = s32_[1], &run_c = s32_[2], &run_nko = s32_[3],
+                   &kick_old = s32_[4], &promote = s32_[5], &ppos = s32_[6];
+  DevBuf<uint8_t> &run_route = s8_[3], &kroute = s8_[4];
   run_beg.ensure(nruns + 1, stream_);
   run_of.ensure(nR + 1, stream_);
   run_c.ensure(nruns + 1, stream_);
@@ -1551,7 +1605,7 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   k_run_of<<<grid_for(nR, 256), 256, 0, stream_>>>(rflag, rpos, nR, run_of.get());
   TG_CHECK_LAUNCH();
   // the old degrees of promoted vertices are needed after the merge
-  DevBuf<uint32_t> old_deg;
+  DevBuf<uint32_t> &old_deg = s32_[7];
   old_deg.ensure(nruns + 1, stream_);
   k_old_degrees<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, old_deg.get());
   TG_CHECK_LAUNCH();
@@ -1575,9 +1629,9 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_);
   uint32_t np = read_scalar(ppos.get() + nruns, stream_);
   uint64_t next = 0;
-  DevBuf<uint64_t> ext;
+  DevBuf<uint64_t> &ext = s64_[0];
   if (np) {
-    DevBuf<uint32_t> pv, pcount, pext, pod;
+    DevBuf<uint32_t> &pv = s32_[8], &pcount = s32_[9], &pext = s32_[10], &pod = s32_[11];
     pv.ensure(np + 1, stream_);
     pcount.ensure(np + 1, stream_);
     pext.ensure(np + 1, stream_);
@@ -1648,8 +1702,8 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(f2_.get(), f3_.get(), nR, f3_.get() + nR, scan_scratch, stream_);
   uint32_t nruns = read_scalar(f3_.get() + nR, stream_);
-  DevBuf<uint32_t> run_beg, run_bump, bump_off;
-  DevBuf<uint8_t> sec_route;
+  DevBuf<uint32_t> &run_beg = s32_[0], &run_bump = s32_[12], &bump_off = s32_[13];
+  DevBuf<uint8_t> &sec_route = s8_[5];
   run_beg.ensure(nruns + 1, stream_);
   run_bump.ensure(nruns + 1, stream_);
   bump_off.ensure(nruns + 1, stream_);
@@ -1675,8 +1729,8 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_);
   uint64_t nb = read_scalar(bump_off.get() + nruns, stream_);
   if (nb) {
-    DevBuf<uint64_t> bkeys;
-    DevBuf<uint8_t> broute;
+    DevBuf<uint64_t> &bkeys = s64_[1];
+    DevBuf<uint8_t> &broute = s8_[6];
     bkeys.ensure(nb + 1, stream_);
     broute.ensure(nb + 1, stream_);
     g = view();
@@ -1726,7 +1780,7 @@ void Graph::tiers(uint8_t *out) const {
 const Graph::TierLists &Graph::tier_lists() {
   if (!lists_dirty_) return lists_;
   GView g = view();
-  DevBuf<uint32_t> fp, ft, pp, pt;
+  DevBuf<uint32_t> &fp = s32_[14], &ft = s32_[15], &pp = s32_[16], &pt = s32_[17];
   fp.ensure(n_ + 1, stream_);
   ft.ensure(n_ + 1, stream_);
   pp.ensure(n_ + 1, stream_);

@@ -109,6 +109,7 @@ class Graph {
   DevBuf<uint64_t> fence_;
   DevBuf<uint8_t> mark_;
   Bounds bounds_{};
+  bool tree_dirty_ = false;  // count tree levels above the leaves are stale
   // B-tree tier
   DevBuf<uint32_t> bkeys_, bcnt_;
   uint64_t blk_top_ = 0;
@@ -134,6 +135,11 @@ class Graph {
   DevBuf<uint8_t> route_;
   DevBuf<uint64_t> scal_;  // small device scalars
   DevBuf<uint64_t> tmp_old_;
+  // per-batch scratch kept across calls (no allocation on the update path
+  // once the buffers have grown)
+  DevBuf<uint32_t> s32_[32];
+  DevBuf<uint8_t> s8_[8];
+  DevBuf<uint64_t> s64_[4];
 
   void init_pma(uint64_t nleaves);
   void compute_bounds();
@@ -141,6 +147,7 @@ class Graph {
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
   void pma_insert(const uint64_t *keys, uint64_t k);
   void pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total);
+  void rebuild_count_tree();
   void pma_delete(const uint64_t *keys, uint64_t k);
   void tree_insert(const uint64_t *keys, uint64_t k);
   void tree_delete(const uint64_t *keys, uint64_t k);

@@ -72,19 +72,25 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const InT *in, uint6
   }
 }
 
-// Block-level scan of a tile with per-block offsets[blockIdx.x] added.
+// Block-level scan of a tile with per-block offsets[blockIdx.x] added.  The
+// tile is staged through LDS so global loads and stores stay coalesced.
 template <typename InT, typename AccT>
 __global__ void __launch_bounds__(kScanBlock) k_scan_tile(const InT *in, InT *out, uint64_t n,
                                                           const AccT *offsets, InT *total_dev) {
   __shared__ AccT lds[kScanBlock / 64];
-  uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile +
-                  static_cast<uint64_t>(threadIdx.x) * kScanItems;
+  __shared__ InT tile[kScanTile];
+  const uint64_t tbase = static_cast<uint64_t>(blockIdx.x) * kScanTile;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    uint64_t idx = tbase + static_cast<uint64_t>(i) * kScanBlock + threadIdx.x;
+    tile[i * kScanBlock + threadIdx.x] = idx < n ? in[idx] : InT(0);
+  }
+  __syncthreads();
   AccT vals[kScanItems];
   AccT sum = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    uint64_t idx = base + i;
-    vals[i] = idx < n ? static_cast<AccT>(in[idx]) : 0;
+    vals[i] = static_cast<AccT>(tile[threadIdx.x * kScanItems + i]);
     sum += vals[i];
   }
   AccT tot;
@@ -93,9 +99,14 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_tile(const InT *in, InT *ou
   pre += off;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    uint64_t idx = base + i;
-    if (idx < n) out[idx] = static_cast<InT>(pre);
+    tile[threadIdx.x * kScanItems + i] = static_cast<InT>(pre);
     pre += vals[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    uint64_t idx = tbase + static_cast<uint64_t>(i) * kScanBlock + threadIdx.x;
+    if (idx < n) out[idx] = tile[i * kScanBlock + threadIdx.x];
   }
   if (total_dev && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_dev = static_cast<InT>(off + tot);
 }

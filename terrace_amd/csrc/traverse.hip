@@ -48,6 +48,15 @@ __device__ inline T wave_min(T v) {
   }
   return v;
 }
+template <typename T>
+__device__ inline T group16_min(T v) {
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) {
+    T o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
 __device__ inline double wave_sum_d(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -74,6 +83,7 @@ struct PullBFS {
   __device__ Acc value(uint32_t v) const { return v; }
   __device__ static Acc combine(Acc a, Acc b) { return a < b ? a : b; }
   __device__ static Acc wave_reduce(Acc a) { return wave_min(a); }
+  __device__ static Acc reduce16(Acc a, uint64_t) { return group16_min(a); }
   __device__ static bool found(Acc a) { return a != 0xFFFFFFFFu; }
   __device__ void commit(uint32_t i, Acc a, Mode m) const {
     if (a == 0xFFFFFFFFu) return;
@@ -100,6 +110,7 @@ struct PullCC {
   __device__ Acc value(uint32_t v) const { return ids[v]; }
   __device__ static Acc combine(Acc a, Acc b) { return a < b ? a : b; }
   __device__ static Acc wave_reduce(Acc a) { return wave_min(a); }
+  __device__ static Acc reduce16(Acc a, uint64_t) { return group16_min(a); }
   __device__ static bool found(Acc) { return false; }
   __device__ void commit(uint32_t i, Acc a, Mode m) const {
     if (a == 0xFFFFFFFFu) return;
@@ -136,6 +147,12 @@ struct PullSum {
     a.any = __ballot(a.any != 0) != 0 ? 1u : 0u;
     return a;
   }
+  __device__ static Acc reduce16(Acc a, uint64_t gmask) {
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) a.s += __shfl_xor(a.s, off, 64);
+    a.any = (__ballot(a.any != 0) & gmask) != 0 ? 1u : 0u;
+    return a;
+  }
   __device__ static bool found(Acc) { return false; }
   __device__ void commit(uint32_t i, Acc a, Mode m) const {
     if (assign && m == kFirst) {
@@ -149,54 +166,86 @@ struct PullSum {
   }
 };
 
+// Loads a 64-byte vertex block with four 16-byte loads.
+__device__ inline void load_vblock(const VBlock *vb, uint32_t v, uint32_t (&w)[16]) {
+  const uint4 *p = reinterpret_cast<const uint4 *>(vb + v);
+  uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+  w[0] = q0.x; w[1] = q0.y; w[2] = q0.z; w[3] = q0.w;
+  w[4] = q1.x; w[5] = q1.y; w[6] = q1.z; w[7] = q1.w;
+  w[8] = q2.x; w[9] = q2.y; w[10] = q2.z; w[11] = q2.w;
+  w[12] = q3.x; w[13] = q3.y; w[14] = q3.z; w[15] = q3.w;
+}
+
 template <class Op>
 __global__ void __launch_bounds__(256) k_pull_inline(GView g, Op op) {
   GRID_STRIDE(i, g.n) {
     uint32_t v = static_cast<uint32_t>(i);
     if (op.skip(v)) continue;
-    const VBlock &b = g.vb[v];
-    uint32_t D = b.degree;
+    uint32_t w[16];
+    load_vblock(g.vb, v, w);
+    uint32_t D = w[0];
     uint32_t lim = D < kInline ? D : kInline;
     typename Op::Acc a = Op::ident();
-    for (uint32_t k = 0; k < lim; k++) {
-      uint32_t u = b.nbr[k];
-      if (op.member(u)) {
+    bool done = false;
+#pragma unroll
+    for (uint32_t k = 0; k < kInline; k++) {
+      uint32_t u = w[1 + k];
+      if (!done && k < lim && op.member(u)) {
         a = Op::combine(a, op.value(u));
-        if (Op::kEarly && Op::found(a)) break;
+        if (Op::kEarly && Op::found(a)) done = true;
       }
     }
     op.commit(v, a, kFirst);
   }
 }
 
-// One wave per PMA vertex of the list.
+// 16-lane group per PMA vertex; each lane covers 4 slots with one 16-byte
+// load, so a group reads 64 slots (256 bytes) per step and a wave works on
+// four vertices at once.
 template <class Op>
 __global__ void __launch_bounds__(256) k_pull_pma(GView g, Op op, const uint32_t *list, uint32_t cnt) {
-  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
-  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
-  uint32_t lane = threadIdx.x & 63;
-  for (uint64_t w = wave; w < cnt; w += nwaves) {
+  const uint32_t lane = threadIdx.x & 63, gl = lane & 15;
+  const uint64_t gmask = 0xFFFFull << (lane & 48);
+  const uint64_t lt = lanemask_lt() & gmask;
+  uint64_t grp = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 4;
+  uint64_t ngrp = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 4;
+  for (uint64_t w = grp; w < cnt; w += ngrp) {
     uint32_t v = list[w];
-    if (op.skip(v)) continue;  // uniform: same v for the wave
+    if (op.skip(v)) continue;  // uniform within the group
     const VBlock &b = g.vb[v];
     uint32_t need = b.degree - kInline;
-    uint64_t p = b.aux;
+    uint64_t begin = b.aux;
+    uint64_t p = begin & ~3ull;
     uint32_t got = 0;
     typename Op::Acc a = Op::ident();
     while (got < need && p < g.N) {
-      uint64_t slot = p + lane;
-      uint32_t d = slot < g.N ? g.pdst[slot] : kEmpty;
-      bool valid = d != kEmpty;
-      uint64_t m = __ballot(valid);
-      uint32_t r = __popcll(m & lanemask_lt());
-      bool in = valid && got + r < need && op.member(d);
-      if (in) a = Op::combine(a, op.value(d));
-      got += __popcll(m);
+      uint64_t slot = p + gl * 4;
+      uint4 q = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+      if (slot < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + slot);
+      uint32_t d[4] = {q.x, q.y, q.z, q.w};
+      bool val[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) val[j] = d[j] != kEmpty && slot + j >= begin;
+      uint64_t B0 = __ballot(val[0]) & gmask, B1 = __ballot(val[1]) & gmask;
+      uint64_t B2 = __ballot(val[2]) & gmask, B3 = __ballot(val[3]) & gmask;
+      uint32_t r = __popcll(B0 & lt) + __popcll(B1 & lt) + __popcll(B2 & lt) + __popcll(B3 & lt);
+      bool hit = false;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (val[j]) {
+          if (got + r < need && op.member(d[j])) {
+            a = Op::combine(a, op.value(d[j]));
+            hit = true;
+          }
+          r++;
+        }
+      }
+      got += __popcll(B0) + __popcll(B1) + __popcll(B2) + __popcll(B3);
       p += 64;
-      if (Op::kEarly && __ballot(in) != 0) break;
+      if (Op::kEarly && (__ballot(hit) & gmask) != 0) break;
     }
-    a = Op::wave_reduce(a);
-    if (lane == 0) op.commit(v, a, kExcl);
+    a = Op::reduce16(a, gmask);
+    if (gl == 0) op.commit(v, a, kExcl);
   }
 }
 
@@ -314,23 +363,36 @@ template <class Op>
 __global__ void __launch_bounds__(256) k_push_pma(GView g, Op op, PushOut o, const uint32_t *list,
                                                   const uint32_t *cnt_ptr) {
   uint32_t cnt = *cnt_ptr;
-  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
-  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
-  uint32_t lane = threadIdx.x & 63;
-  for (uint64_t w = wave; w < cnt; w += nwaves) {
+  const uint32_t lane = threadIdx.x & 63, gl = lane & 15;
+  const uint64_t gmask = 0xFFFFull << (lane & 48);
+  const uint64_t lt = lanemask_lt() & gmask;
+  uint64_t grp = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 4;
+  uint64_t ngrp = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 4;
+  for (uint64_t w = grp; w < cnt; w += ngrp) {
     uint32_t s = list[w];
     const VBlock &b = g.vb[s];
     uint32_t need = b.degree - kInline;
-    uint64_t p = b.aux;
+    uint64_t begin = b.aux;
+    uint64_t p = begin & ~3ull;
     uint32_t got = 0;
     while (got < need && p < g.N) {
-      uint64_t slot = p + lane;
-      uint32_t d = slot < g.N ? g.pdst[slot] : kEmpty;
-      bool valid = d != kEmpty;
-      uint64_t m = __ballot(valid);
-      uint32_t r = __popcll(m & lanemask_lt());
-      push_emit(op, o, s, d, valid && got + r < need);
-      got += __popcll(m);
+      uint64_t slot = p + gl * 4;
+      uint4 q = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+      if (slot < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + slot);
+      uint32_t d[4] = {q.x, q.y, q.z, q.w};
+      bool val[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) val[j] = d[j] != kEmpty && slot + j >= begin;
+      uint64_t B0 = __ballot(val[0]) & gmask, B1 = __ballot(val[1]) & gmask;
+      uint64_t B2 = __ballot(val[2]) & gmask, B3 = __ballot(val[3]) & gmask;
+      uint32_t r = __popcll(B0 & lt) + __popcll(B1 & lt) + __popcll(B2 & lt) + __popcll(B3 & lt);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        bool in = val[j] && got + r < need;
+        push_emit(op, o, s, d[j], in);
+        if (val[j]) r++;
+      }
+      got += __popcll(B0) + __popcll(B1) + __popcll(B2) + __popcll(B3);
       p += 64;
     }
   }
@@ -480,9 +542,70 @@ __global__ void __launch_bounds__(256) k_tc(const uint64_t *off, const uint32_t 
   if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
 }
 
+// vertexMap functors of the algorithm headers over a vertex list (or all).
+__global__ void k_vertex_map(GView g, const uint32_t *list, uint64_t cnt, int vop, double *val, const double *inv,
+                             const uint32_t *ids, uint32_t *prev, uint8_t *visited) {
+  GRID_STRIDE(i, cnt) {
+    uint32_t v = list ? list[i] : static_cast<uint32_t>(i);
+    switch (vop) {
+      case 1: val[v] = val[v] / static_cast<double>(g.vb[v].degree); break;
+      case 2: val[v] = 0.0; break;
+      case 3: prev[v] = ids[v]; break;
+      case 4: visited[v] = 1; break;
+      case 5:
+        visited[v] = 1;
+        val[v] += inv[v];
+        break;
+      case 6: val[v] = 1 / val[v]; break;
+      case 7: visited[v] = 0; break;
+      case 8: val[v] = (val[v] - inv[v]) / inv[v]; break;
+      default: break;
+    }
+  }
+}
+
 }  // namespace
 
 // ===================================================================== host
+void Traversal::vertex_map(Frontier &f, int vop, const VArgs &a) {
+  hipStream_t s = g_.stream();
+  if (vop < 1 || vop > 8) throw Error(-1, "unknown vertex op");
+  if ((vop == 1 || vop == 2 || vop == 6) && !a.val) throw Error(-1, "vertex op needs val");
+  if (vop == 7 && !a.visited) throw Error(-1, "CLEAR_VISITED needs visited");
+  if (vop == 8 && !(a.val && a.inv)) throw Error(-1, "BC_FINAL needs val and inv");
+  if (vop == 3 && !(a.ids && a.prev)) throw Error(-1, "CC_VERTEX needs ids and prev");
+  if (vop == 4 && !a.visited) throw Error(-1, "BC_VERTEX needs visited");
+  if (vop == 5 && !(a.visited && a.val && a.inv)) throw Error(-1, "BC_BACK_VERTEX needs visited, val, inv");
+  if (f.all) {
+    k_vertex_map<<<grid_for(f.n, 256), 256, 0, s>>>(g_.view(), nullptr, f.n, vop, a.val, a.inv, a.ids, a.prev,
+                                                    a.visited);
+  } else {
+    to_queue(f);
+    if (f.count == 0) return;
+    k_vertex_map<<<grid_for(f.count, 256), 256, 0, s>>>(g_.view(), f.queue.get(), f.count, vop, a.val, a.inv, a.ids,
+                                                        a.prev, a.visited);
+  }
+  TG_CHECK_LAUNCH();
+}
+
+void Traversal::copy_frontier(Frontier &dst, Frontier &src) {
+  hipStream_t s = g_.stream();
+  dst.n = src.n;
+  dst.all = src.all;
+  dst.count = src.count;
+  dst.has_queue = src.has_queue;
+  dst.has_bits = src.has_bits;
+  if (src.has_queue) {
+    dst.queue.ensure(src.n, s);
+    if (src.count)
+      TG_HIP(hipMemcpyAsync(dst.queue.get(), src.queue.get(), src.count * 4, hipMemcpyDeviceToDevice, s));
+  }
+  if (src.has_bits) {
+    dst.bits.ensure(src.words() + 1, s);
+    TG_HIP(hipMemcpyAsync(dst.bits.get(), src.bits.get(), src.words() * 4, hipMemcpyDeviceToDevice, s));
+  }
+}
+
 void Traversal::record(const char *name, hipEvent_t a, hipEvent_t b) {
   float ms = 0;
   TG_HIP(hipEventSynchronize(b));
@@ -599,7 +722,7 @@ static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierL
   k_pull_inline<Op><<<grid_for(v.n, 256), 256, 0, s>>>(v, op);
   TG_CHECK_LAUNCH();
   if (L.npma) {
-    k_pull_pma<Op><<<grid_for(static_cast<uint64_t>(L.npma) * 64, 256), 256, 0, s>>>(v, op, L.pma, L.npma);
+    k_pull_pma<Op><<<grid_for(static_cast<uint64_t>(L.npma) * 16, 256), 256, 0, s>>>(v, op, L.pma, L.npma);
     TG_CHECK_LAUNCH();
   }
   if (ntasks) {
@@ -687,7 +810,7 @@ static void launch_push(Traversal &T, Graph &g, const Op &op, Frontier &in, Fron
   uint32_t fc = static_cast<uint32_t>(in.count);
   k_push_inline<Op><<<grid_for(fc, 256), 256, 0, s>>>(v, op, o, in.queue.get(), fc, sp.get(), st.get(), cnt.get());
   TG_CHECK_LAUNCH();
-  k_push_pma<Op><<<grid_for(static_cast<uint64_t>(fc) * 64, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
+  k_push_pma<Op><<<grid_for(static_cast<uint64_t>(fc) * 16, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
   TG_CHECK_LAUNCH();
   k_push_tree<Op><<<grid_for(static_cast<uint64_t>(fc) * 256, 256, 1u << 16), 256, 0, s>>>(v, op, o, st.get(),
                                                                                          cnt.get() + 1);

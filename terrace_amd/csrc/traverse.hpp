@@ -50,6 +50,17 @@ class Traversal {
   // output frontier when output is true.
   void edge_map(Frontier &in, Op op, const OpState &st, Frontier *out, uint32_t threshold = 20);
 
+  // vertexMap over the members of f (op codes of tg_vop).
+  struct VArgs {
+    double *val = nullptr;
+    const double *inv = nullptr;
+    const uint32_t *ids = nullptr;
+    uint32_t *prev = nullptr;
+    uint8_t *visited = nullptr;
+  };
+  void vertex_map(Frontier &f, int vop, const VArgs &a);
+  void copy_frontier(Frontier &dst, Frontier &src);
+
   // Whole algorithms (device result buffers).
   uint32_t bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t threshold = 20);
   void pagerank(uint32_t iters, double *out);
