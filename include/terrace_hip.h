@@ -154,6 +154,55 @@ int tg_vertex_map(tg_graph *g, tg_frontier *f, int vop, const tg_vmap_args *args
 /* Copy of a frontier (VertexSubset(other, true)). */
 int tg_frontier_copy(tg_frontier *dst, tg_frontier *src);
 
+/* ---- multi-source BFS and local triangle counts -----------------------
+ * No counterpart in the reference (serving / batching layer).
+ * tg_bfs_multi: k <= 64 BFS from srcs[] in one traversal (one bit per
+ * source); levels[j*n + v] = level of v from srcs[j] (UINT32_MAX
+ * unreached), identical to tg_bfs's levels for each source (host arrays).
+ * tg_ltc: triangles through each of verts[0..q) (host arrays). */
+int tg_bfs_multi(tg_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *levels);
+int tg_ltc(tg_graph *g, const uint32_t *verts, uint32_t q, uint64_t *counts);
+
+/* ---- request loop (serving) --------------------------------------------
+ * No counterpart in the reference.  A server replays submitted requests at
+ * their arrival offsets (seconds from tg_server_run) against one graph.
+ * batched = 0: one request at a time in arrival order (the plain path).
+ * batched = 1: continuous batching; every query still runs on the version
+ * that contains exactly the updates that arrived before it.  Requests whose
+ * footprint exceeds mem_budget bytes are rejected (status TG_ENOMEM). */
+typedef struct tg_server tg_server;
+enum tg_req { TG_REQ_UPDATE = 1, TG_REQ_BFS = 2, TG_REQ_CC = 3, TG_REQ_PR = 4, TG_REQ_LTC = 5 };
+typedef struct {
+  int type;
+  double arrival_s;
+  uint32_t src;    /* BFS source / LTC vertex */
+  uint32_t iters;  /* PR iterations */
+  const uint32_t *ins_src, *ins_dst;
+  uint64_t n_ins;
+  const uint32_t *del_src, *del_dst;
+  uint64_t n_del;
+} tg_request;
+typedef struct {
+  int status; /* 0 done, 1 pending, < 0 error code */
+  double arrival_s, start_s, done_s;
+  uint64_t version; /* update batches applied before it ran */
+  uint64_t digest;  /* order-independent digest of levels / labels / ranks */
+  uint64_t value;   /* LTC count */
+  double fvalue;    /* PR: sum of ranks */
+  uint64_t group;
+} tg_result;
+int tg_server_create(tg_graph *g, int batched, uint64_t mem_budget, int keep_results, tg_server **out);
+int tg_server_destroy(tg_server *s);
+int tg_server_submit(tg_server *s, const tg_request *r, uint64_t *id);
+int tg_server_run(tg_server *s);
+int tg_server_result(tg_server *s, uint64_t id, tg_result *out);
+/* kept result array (keep_results=1): BFS levels / CC labels as uint32[n],
+ * PR ranks as double[n] */
+int tg_server_result_data(tg_server *s, uint64_t id, void *out, uint64_t bytes);
+/* [0] makespan s, [1] requests/s, [2] p50 s, [3] p99 s, [4] p50 update,
+ * [5] p99 update, [6] p50 query, [7] p99 query, [8] groups, [9] rejected */
+int tg_server_summary(tg_server *s, double *out /* [10] */);
+
 /* ---- workload and device memory helpers -------------------------------
  * rMat<uint32_t>(n, seed, a, b, c)(i) for i in [start, start+count)
  * (include/rmat_util.h:81-115) written straight into device arrays.

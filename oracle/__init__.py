@@ -83,6 +83,8 @@ def lib():
     L.orc_bc.argtypes = [c.c_uint32, _u64p, _u32p, c.c_uint32, _f64p]
     L.orc_tc.restype = c.c_uint64
     L.orc_tc.argtypes = [c.c_uint32, _u64p, _u32p]
+    L.orc_ltc.restype = c.c_uint64
+    L.orc_ltc.argtypes = [_u64p, _u32p, c.c_uint32]
     L.orc_count_common.restype = c.c_uint32
     L.orc_count_common.argtypes = [_u64p, _u32p, c.c_uint32, c.c_uint32]
     _lib = L
@@ -214,6 +216,10 @@ def tc(g: CSR) -> int:
 
 def count_common(g: CSR, s: int, d: int) -> int:
     return int(lib().orc_count_common(g.off, g.nbr, s, d))
+
+
+def ltc(g: CSR, v: int) -> int:
+    return int(lib().orc_ltc(g.off, g.nbr, v))
 
 
 def rel_l1(a: np.ndarray, b: np.ndarray) -> float:

@@ -402,3 +402,25 @@ uint64_t orc_tc(uint32_t n, const uint64_t *off, const uint32_t *nbr) {
       if (nbr[e] > i) total += orc_count_common(off, nbr, nbr[e], i);
   return total;
 }
+
+uint64_t orc_ltc(const uint64_t *off, const uint32_t *nbr, uint32_t v) {
+  /* triangles through v: half of Σ_{u ∈ N(v)} |N(u) ∩ N(v) \ {u, v}| */
+  uint64_t total = 0;
+  for (uint64_t e = off[v]; e < off[v + 1]; e++) {
+    uint32_t u = nbr[e];
+    if (u == v) continue;
+    uint64_t i = off[u], ie = off[u + 1], j = off[v], je = off[v + 1];
+    while (i < ie && j < je) {
+      if (nbr[i] == nbr[j]) {
+        if (nbr[i] != u && nbr[i] != v) total++;
+        i++;
+        j++;
+      } else if (nbr[i] < nbr[j]) {
+        i++;
+      } else {
+        j++;
+      }
+    }
+  }
+  return total / 2;
+}

@@ -93,6 +93,10 @@ uint64_t orc_tc(uint32_t n, const uint64_t *off, const uint32_t *nbr);
 uint32_t orc_count_common(const uint64_t *off, const uint32_t *nbr,
                           uint32_t s, uint32_t d);
 
+/* Triangles through v (the local triangle count query of the serving
+ * workload; no counterpart in the reference). */
+uint64_t orc_ltc(const uint64_t *off, const uint32_t *nbr, uint32_t v);
+
 #ifdef __cplusplus
 }
 #endif

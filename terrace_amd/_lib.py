@@ -38,6 +38,35 @@ class OpArgs(ctypes.Structure):
     ]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int),
+        ("arrival_s", ctypes.c_double),
+        ("src", ctypes.c_uint32),
+        ("iters", ctypes.c_uint32),
+        ("ins_src", ctypes.c_void_p),
+        ("ins_dst", ctypes.c_void_p),
+        ("n_ins", ctypes.c_uint64),
+        ("del_src", ctypes.c_void_p),
+        ("del_dst", ctypes.c_void_p),
+        ("n_del", ctypes.c_uint64),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("arrival_s", ctypes.c_double),
+        ("start_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+        ("version", ctypes.c_uint64),
+        ("digest", ctypes.c_uint64),
+        ("value", ctypes.c_uint64),
+        ("fvalue", ctypes.c_double),
+        ("group", ctypes.c_uint64),
+    ]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -80,6 +109,15 @@ _SIGS = {
     "tg_edge_map": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(OpArgs), _P, ctypes.c_uint32]),
     "tg_vertex_map": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "tg_frontier_copy": (ctypes.c_int, [_P, _P]),
+    "tg_bfs_multi": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
+    "tg_ltc": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
+    "tg_server_create": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_P)]),
+    "tg_server_destroy": (ctypes.c_int, [_P]),
+    "tg_server_submit": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint64)]),
+    "tg_server_run": (ctypes.c_int, [_P]),
+    "tg_server_result": (ctypes.c_int, [_P, ctypes.c_uint64, _P]),
+    "tg_server_result_data": (ctypes.c_int, [_P, ctypes.c_uint64, _P, ctypes.c_uint64]),
+    "tg_server_summary": (ctypes.c_int, [_P, _P]),
     "tg_rmat_dev": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
                                    ctypes.c_int, _P, _P, ctypes.POINTER(ctypes.c_uint64)]),

@@ -4,10 +4,12 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 
 #include "gen.hpp"
+#include "serve.hpp"
 #include "traverse.hpp"
 
 struct tg_graph {
@@ -498,6 +500,127 @@ int tg_frontier_copy(tg_frontier *dst, tg_frontier *src) {
     std::lock_guard<std::mutex> l(src->owner->g.mu);
     src->owner->t.copy_frontier(dst->f, src->f);
     TG_HIP(hipStreamSynchronize(src->owner->g.stream()));
+  });
+}
+
+// ---------------------------------------------------------------- batching
+int tg_bfs_multi(tg_graph *G, const uint32_t *srcs, uint32_t k, uint32_t *levels) {
+  return guard([&] {
+    NEED(G && srcs && levels, "null argument");
+    NEED(k >= 1 && k <= 64, "k must be in [1, 64]");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    uint32_t n = G->g.num_vertices();
+    hipStream_t st = G->g.stream();
+    G->s_u32.ensure(static_cast<uint64_t>(k) * n, st);
+    G->t.bfs_multi(srcs, k, G->s_u32.get());
+    d2h(levels, G->s_u32.get(), static_cast<uint64_t>(k) * n * 4, st);
+  });
+}
+
+int tg_ltc(tg_graph *G, const uint32_t *verts, uint32_t q, uint64_t *counts) {
+  return guard([&] {
+    NEED(G && (q == 0 || (verts && counts)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->t.ltc(verts, q, counts);
+  });
+}
+
+}  // extern "C"
+
+struct tg_server {
+  tg_graph *owner;
+  std::unique_ptr<tg::Server> s;
+};
+
+extern "C" {
+
+int tg_server_create(tg_graph *G, int batched, uint64_t mem_budget, int keep_results, tg_server **out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    auto *sv = new tg_server;
+    sv->owner = G;
+    sv->s = std::make_unique<tg::Server>(G->g, batched != 0, mem_budget, keep_results != 0);
+    *out = sv;
+  });
+}
+
+int tg_server_destroy(tg_server *s) {
+  return guard([&] { delete s; });
+}
+
+int tg_server_submit(tg_server *S, const tg_request *r, uint64_t *id) {
+  return guard([&] {
+    NEED(S && r, "null argument");
+    NEED(r->type >= TG_REQ_UPDATE && r->type <= TG_REQ_LTC, "unknown request type");
+    uint32_t n = S->owner->g.num_vertices();
+    tg::Request q;
+    q.type = r->type;
+    q.arrival = r->arrival_s;
+    q.src = r->src;
+    q.iters = r->iters;
+    if (r->type == TG_REQ_UPDATE) {
+      NEED((r->n_ins == 0 || (r->ins_src && r->ins_dst)) && (r->n_del == 0 || (r->del_src && r->del_dst)),
+           "null update arrays");
+      q.us.assign(r->ins_src, r->ins_src + r->n_ins);
+      q.ud.assign(r->ins_dst, r->ins_dst + r->n_ins);
+      q.uop.assign(r->n_ins, 0u);
+      q.us.insert(q.us.end(), r->del_src, r->del_src + r->n_del);
+      q.ud.insert(q.ud.end(), r->del_dst, r->del_dst + r->n_del);
+      q.uop.insert(q.uop.end(), r->n_del, 1u);
+      for (size_t i = 0; i < q.us.size(); i++) NEED(q.us[i] < n && q.ud[i] < n, "vertex id out of range");
+    } else {
+      NEED(r->type == TG_REQ_CC || r->type == TG_REQ_PR || r->src < n, "vertex id out of range");
+    }
+    uint64_t i = S->s->submit(std::move(q));
+    if (id) *id = i;
+  });
+}
+
+int tg_server_run(tg_server *S) {
+  return guard([&] {
+    NEED(S, "null server");
+    std::lock_guard<std::mutex> l(S->owner->g.mu);
+    S->s->run();
+  });
+}
+
+int tg_server_result(tg_server *S, uint64_t id, tg_result *out) {
+  return guard([&] {
+    NEED(S && out, "null argument");
+    const tg::Request &r = S->s->result(id);
+    out->status = r.status;
+    out->arrival_s = r.arrival;
+    out->start_s = r.start;
+    out->done_s = r.done;
+    out->version = r.version;
+    out->digest = r.digest;
+    out->value = r.value;
+    out->fvalue = r.fvalue;
+    out->group = r.group;
+  });
+}
+
+int tg_server_result_data(tg_server *S, uint64_t id, void *out, uint64_t bytes) {
+  return guard([&] {
+    NEED(S && out, "null argument");
+    const tg::Request &r = S->s->result(id);
+    if (!r.r32.empty()) {
+      NEED(bytes >= r.r32.size() * 4, "buffer too small");
+      std::memcpy(out, r.r32.data(), r.r32.size() * 4);
+    } else if (!r.r64.empty()) {
+      NEED(bytes >= r.r64.size() * 8, "buffer too small");
+      std::memcpy(out, r.r64.data(), r.r64.size() * 8);
+    } else {
+      throw tg::Error(TG_EINVAL, "no kept result for this request");
+    }
+  });
+}
+
+int tg_server_summary(tg_server *S, double *out) {
+  return guard([&] {
+    NEED(S && out, "null argument");
+    std::vector<double> v = S->s->summary();
+    for (size_t i = 0; i < v.size(); i++) out[i] = v[i];
   });
 }
 

@@ -1567,9 +1567,15 @@ void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
 void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
   if (count == 0) return;
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
-  lists_dirty_ = true;
   uint64_t *R = nullptr;
   uint64_t nR = prepare_keys(src, dst, count, R);  // R in k1_
+  insert_sorted(R, nR);
+}
+
+// Insert sorted unique keys (R must not alias the batch scratch buffers).
+void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
+  if (nR == 0) return;
+  lists_dirty_ = true;
   GView g = view();
   // existence + new-key prefix
   f0_.ensure(nR + 1, stream_);
@@ -1685,9 +1691,15 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
 void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
   if (count == 0) return;
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
-  lists_dirty_ = true;
   uint64_t *R0 = nullptr;
   uint64_t nR0 = prepare_keys(src, dst, count, R0);  // k1_
+  delete_sorted(R0, nR0);
+}
+
+// Delete sorted unique keys (absent keys are ignored).
+void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
+  if (nR0 == 0) return;
+  lists_dirty_ = true;
   GView g = view();
   f0_.ensure(nR0 + 1, stream_);
   k_exists<<<grid_for(nR0, 256), 256, 0, stream_>>>(g, R0, nR0, f0_.get());
@@ -1750,6 +1762,45 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
     tree_delete(g0_.get(), bt);
   }
   TG_HIP(hipStreamSynchronize(stream_));
+}
+
+// ------------------------------------------------------------ coalesced ops
+__global__ void k_last_op_flags(const uint64_t *keys, const uint32_t *ops, uint64_t n, uint32_t *fins,
+                                uint32_t *fdel) {
+  GRID_STRIDE(i, n) {
+    bool last = (i + 1 == n) || keys[i] != keys[i + 1];
+    fins[i] = (last && ops[i] == 0) ? 1u : 0u;
+    fdel[i] = (last && ops[i] != 0) ? 1u : 0u;
+  }
+}
+
+// A coalesced run of update batches: ops[i] = 0 insert / 1 delete, in
+// arrival order.  The stable sort keeps arrival order inside each key, so
+// the last op of a key decides, which equals applying the batches one by
+// one (both ops are idempotent set updates).
+void Graph::apply_ops(const uint32_t *src, const uint32_t *dst, const uint32_t *ops, uint64_t count) {
+  if (count == 0) return;
+  if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
+  k0_.ensure(count, stream_);
+  fill_u64(scal_.get(), 0, 1, stream_);
+  k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), scal_.get());
+  TG_CHECK_LAUNCH();
+  if (read_scalar(scal_.get(), stream_)) throw Error(-1, "vertex id out of range");
+  ops_val_.ensure(count, stream_);
+  TG_HIP(hipMemcpyAsync(ops_val_.get(), ops, count * 4, hipMemcpyDeviceToDevice, stream_));
+  uint32_t bits = 1;
+  while (bits < 32 && (1ull << bits) < n_) bits++;
+  radix_sort_pairs(k0_.get(), ops_val_.get(), count, bits, bits, sort_, stream_);
+  f0_.ensure(count + 1, stream_);
+  f1_.ensure(count + 1, stream_);
+  k_last_op_flags<<<grid_for(count, 256), 256, 0, stream_>>>(k0_.get(), ops_val_.get(), count, f0_.get(), f1_.get());
+  TG_CHECK_LAUNCH();
+  ops_ins_.ensure(count + 1, stream_);
+  ops_del_.ensure(count + 1, stream_);
+  uint64_t ni = compact_keys(k0_.get(), f0_.get(), count, ops_ins_.get());
+  uint64_t nd = compact_keys(k0_.get(), f1_.get(), count, ops_del_.get());
+  insert_sorted(ops_ins_.get(), ni);
+  delete_sorted(ops_del_.get(), nd);
 }
 
 // ------------------------------------------------------------ queries

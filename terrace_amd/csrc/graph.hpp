@@ -70,6 +70,12 @@ class Graph {
   // inputs are not modified.  Any order, duplicates allowed.
   void insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t count);
   void delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t count);
+  // Sorted unique keys (src<<32|dst) already in HBM.
+  void insert_sorted(const uint64_t *keys, uint64_t n);
+  void delete_sorted(const uint64_t *keys, uint64_t n);
+  // Coalesced update batches: ops[i] 0 = insert, 1 = delete, arrival order;
+  // the result equals applying them one after another.
+  void apply_ops(const uint32_t *src, const uint32_t *dst, const uint32_t *ops, uint64_t count);
 
   // Queries (device pointers).
   void degrees(uint32_t *out) const;
@@ -135,6 +141,8 @@ class Graph {
   DevBuf<uint8_t> route_;
   DevBuf<uint64_t> scal_;  // small device scalars
   DevBuf<uint64_t> tmp_old_;
+  DevBuf<uint32_t> ops_val_;
+  DevBuf<uint64_t> ops_ins_, ops_del_;
   // per-batch scratch kept across calls (no allocation on the update path
   // once the buffers have grown)
   DevBuf<uint32_t> s32_[32];

@@ -171,7 +171,8 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_hist(const uint64_t *keys,
 
 __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const uint64_t *keys, uint64_t *out, uint64_t n,
                                                               int shift, const uint32_t *offs,
-                                                              uint32_t nblocks) {
+                                                              uint32_t nblocks, const uint32_t *vals,
+                                                              uint32_t *vout) {
   __shared__ uint32_t wave_cnt[kWaves][256];
   __shared__ uint32_t digit_base[256];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -222,7 +223,9 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const uint64_t *ke
     uint64_t idx = sort_index(tile, w, j, lane);
     if (idx < n) {
       uint32_t d = static_cast<uint32_t>((k[j] >> shift) & 0xFF);
-      out[wave_cnt[w][d] + rank[j]] = k[j];
+      uint32_t pos = wave_cnt[w][d] + rank[j];
+      out[pos] = k[j];
+      if (vals) vout[pos] = vals[idx];
     }
   }
   (void)digit_base;
@@ -256,31 +259,37 @@ void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t 
   scan_impl<uint64_t>(in, out, n, total_dev, ss, 0, s);
 }
 
-void radix_sort_keys(uint64_t *keys, uint64_t n, uint32_t lo_bits, uint32_t hi_bits, SortScratch &ss,
-                     hipStream_t s) {
+void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t lo_bits, uint32_t hi_bits,
+                      SortScratch &ss, hipStream_t s) {
   if (n < 2) return;
-  if (n > 0xFFFFFFFFull) throw Error(-1, "radix_sort_keys: more than 2^32 keys");
+  if (n > 0xFFFFFFFFull) throw Error(-1, "radix sort: more than 2^32 keys");
   uint32_t nblocks = static_cast<uint32_t>((n + kSortTile - 1) / kSortTile);
   ss.alt.ensure(n, s);
+  if (vals) ss.alt_vals.ensure(n, s);
   ss.hist.ensure(static_cast<uint64_t>(nblocks) * 256, s);
   uint64_t *src = keys, *dst = ss.alt.get();
-  int passes = 0;
+  uint32_t *vsrc = vals, *vdst = vals ? ss.alt_vals.get() : nullptr;
   auto pass = [&](int shift) {
     k_radix_hist<<<nblocks, kSortBlock, 0, s>>>(src, n, shift, ss.hist.get(), nblocks);
     TG_CHECK_LAUNCH();
     exclusive_scan_u32(ss.hist.get(), ss.hist.get(), static_cast<uint64_t>(nblocks) * 256, nullptr,
                        ss.scan, s);
-    k_radix_scatter<<<nblocks, kSortBlock, 0, s>>>(src, dst, n, shift, ss.hist.get(), nblocks);
+    k_radix_scatter<<<nblocks, kSortBlock, 0, s>>>(src, dst, n, shift, ss.hist.get(), nblocks, vsrc, vdst);
     TG_CHECK_LAUNCH();
-    uint64_t *t = src;
-    src = dst;
-    dst = t;
-    passes++;
+    std::swap(src, dst);
+    std::swap(vsrc, vdst);
   };
   for (uint32_t b = 0; b < lo_bits; b += 8) pass(static_cast<int>(b));
   for (uint32_t b = 0; b < hi_bits; b += 8) pass(static_cast<int>(32 + b));
-  if (src != keys)
+  if (src != keys) {
     TG_HIP(hipMemcpyAsync(keys, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    if (vals) TG_HIP(hipMemcpyAsync(vals, vsrc, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  }
+}
+
+void radix_sort_keys(uint64_t *keys, uint64_t n, uint32_t lo_bits, uint32_t hi_bits, SortScratch &ss,
+                     hipStream_t s) {
+  radix_sort_pairs(keys, nullptr, n, lo_bits, hi_bits, ss, s);
 }
 
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {

@@ -55,6 +55,7 @@ void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t 
 
 struct SortScratch {
   DevBuf<uint64_t> alt;
+  DevBuf<uint32_t> alt_vals;
   DevBuf<uint32_t> hist;
   ScanScratch scan;
 };
@@ -63,6 +64,10 @@ struct SortScratch {
 // dst bits [0, lo_bits) and src bits [32, 32 + hi_bits).  8-bit digits.
 void radix_sort_keys(uint64_t *keys, uint64_t n, uint32_t lo_bits, uint32_t hi_bits,
                      SortScratch &ss, hipStream_t s);
+
+// Same, carrying one u32 value per key (stable).
+void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t lo_bits, uint32_t hi_bits,
+                      SortScratch &ss, hipStream_t s);
 
 // Fill helpers (kernels, not hipMemset, so they are safe under capture).
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);

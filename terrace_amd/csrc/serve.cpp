@@ -1,0 +1,390 @@
+// Request loop: plain (one at a time) and batched (continuous batching) modes.
+#include "serve.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+
+namespace tg {
+
+namespace {
+
+double wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+double pct(std::vector<double> v, double p) {
+  if (v.empty()) return 0;
+  std::sort(v.begin(), v.end());
+  double pos = p * (v.size() - 1);
+  size_t lo = static_cast<size_t>(std::floor(pos));
+  size_t hi = std::min(lo + 1, v.size() - 1);
+  return v[lo] + (v[hi] - v[lo]) * (pos - lo);
+}
+
+}  // namespace
+
+Server::Server(Graph &g, bool batched, uint64_t mem_budget, bool keep_results)
+    : g_(g), batched_(batched), budget_(mem_budget), keep_(keep_results) {
+  for (int i = 0; i < kLanes; i++) {
+    if (batched_) {
+      TG_HIP(hipStreamCreateWithFlags(&streams_[i], hipStreamNonBlocking));
+      trav_[i] = std::make_unique<Traversal>(g_, streams_[i]);
+    } else {
+      trav_[i] = std::make_unique<Traversal>(g_);
+    }
+  }
+}
+
+Server::~Server() {
+  for (int i = 0; i < kLanes; i++) {
+    trav_[i].reset();
+    if (streams_[i]) {
+      (void)hipStreamSynchronize(streams_[i]);
+      (void)hipStreamDestroy(streams_[i]);
+    }
+  }
+}
+
+double Server::now() const { return wall() - t0_; }
+
+uint64_t Server::submit(Request r) {
+  r.status = 1;
+  r.footprint = footprint(r);
+  reqs_.push_back(std::move(r));
+  return reqs_.size() - 1;
+}
+
+// Device bytes a request needs while it runs (scratch, outputs, growth).
+uint64_t Server::footprint(const Request &r) const {
+  const uint64_t n = g_.num_vertices();
+  switch (r.type) {
+    case kReqUpdate: {
+      uint64_t e = r.us.size();
+      // keys, sort buffers, flags and kicked lists; PMA growth when the
+      // array may double (16 B per slot of the current array)
+      return 64 * e + (e > g_.pma_slots() / 8 ? 16 * g_.pma_slots() : 0);
+    }
+    case kReqBfs: return 4 * n + n / 2;   // one lane of a multi-source run
+    case kReqCc: return 12 * n;
+    case kReqPr: return 24 * n;
+    case kReqLtc: return 1 << 16;
+    default: return 0;
+  }
+}
+
+void Server::exec_update_group(const std::vector<uint64_t> &ids) {
+  hipStream_t s = g_.stream();
+  uint64_t total = 0;
+  for (uint64_t id : ids) total += reqs_[id].us.size();
+  double st = now();
+  if (!batched_) {
+    // plain: the request's own inserts, then its deletes
+    Request &r = reqs_[ids[0]];
+    std::vector<uint32_t> is, id, ds, dd;
+    for (size_t i = 0; i < r.us.size(); i++) {
+      if (r.uop[i] == 0) {
+        is.push_back(r.us[i]);
+        id.push_back(r.ud[i]);
+      } else {
+        ds.push_back(r.us[i]);
+        dd.push_back(r.ud[i]);
+      }
+    }
+    stage_s_.ensure(std::max<uint64_t>(total, 1), s);
+    stage_d_.ensure(std::max<uint64_t>(total, 1), s);
+    if (!is.empty()) {
+      TG_HIP(hipMemcpyAsync(stage_s_.get(), is.data(), is.size() * 4, hipMemcpyHostToDevice, s));
+      TG_HIP(hipMemcpyAsync(stage_d_.get(), id.data(), id.size() * 4, hipMemcpyHostToDevice, s));
+      g_.insert_batch(stage_s_.get(), stage_d_.get(), is.size());
+    }
+    if (!ds.empty()) {
+      TG_HIP(hipMemcpyAsync(stage_s_.get(), ds.data(), ds.size() * 4, hipMemcpyHostToDevice, s));
+      TG_HIP(hipMemcpyAsync(stage_d_.get(), dd.data(), dd.size() * 4, hipMemcpyHostToDevice, s));
+      g_.delete_batch(stage_s_.get(), stage_d_.get(), ds.size());
+    }
+  } else {
+    std::vector<uint32_t> hs, hd, ho;
+    hs.reserve(total);
+    hd.reserve(total);
+    ho.reserve(total);
+    for (uint64_t rid : ids) {
+      const Request &r = reqs_[rid];
+      hs.insert(hs.end(), r.us.begin(), r.us.end());
+      hd.insert(hd.end(), r.ud.begin(), r.ud.end());
+      ho.insert(ho.end(), r.uop.begin(), r.uop.end());
+    }
+    stage_s_.ensure(std::max<uint64_t>(total, 1), s);
+    stage_d_.ensure(std::max<uint64_t>(total, 1), s);
+    stage_o_.ensure(std::max<uint64_t>(total, 1), s);
+    if (total) {
+      TG_HIP(hipMemcpyAsync(stage_s_.get(), hs.data(), total * 4, hipMemcpyHostToDevice, s));
+      TG_HIP(hipMemcpyAsync(stage_d_.get(), hd.data(), total * 4, hipMemcpyHostToDevice, s));
+      TG_HIP(hipMemcpyAsync(stage_o_.get(), ho.data(), total * 4, hipMemcpyHostToDevice, s));
+      g_.apply_ops(stage_s_.get(), stage_d_.get(), stage_o_.get(), total);
+    }
+  }
+  TG_HIP(hipStreamSynchronize(s));
+  // versions count update requests: the k-th batch of a coalesced run is
+  // version base+k+1, and no query runs between them
+  double t = now();
+  for (uint64_t id : ids) {
+    Request &r = reqs_[id];
+    r.start = st;
+    r.done = t;
+    r.version = ++version_;
+    r.status = 0;
+    r.group = groups_;
+  }
+  groups_++;
+}
+
+void Server::exec_query_plain(Request &r) {
+  const uint32_t n = g_.num_vertices();
+  Traversal &T = *trav_[0];
+  hipStream_t s = T.stream();
+  r.start = now();
+  switch (r.type) {
+    case kReqBfs: {
+      par_.ensure(n, s);
+      res32_[0].ensure(n, s);
+      T.bfs(r.src, par_.get(), res32_[0].get());
+      r.digest = T.digest(res32_[0].get(), nullptr, n);
+      if (keep_) {
+        r.r32.resize(n);
+        TG_HIP(hipMemcpy(r.r32.data(), res32_[0].get(), n * 4ull, hipMemcpyDeviceToHost));
+      }
+      break;
+    }
+    case kReqCc: {
+      res32_[1].ensure(n, s);
+      T.cc(res32_[1].get());
+      r.digest = T.digest(res32_[1].get(), nullptr, n);
+      if (keep_) {
+        r.r32.resize(n);
+        TG_HIP(hipMemcpy(r.r32.data(), res32_[1].get(), n * 4ull, hipMemcpyDeviceToHost));
+      }
+      break;
+    }
+    case kReqPr: {
+      res64_.ensure(n, s);
+      T.pagerank(r.iters, res64_.get());
+      r.digest = T.digest(nullptr, reinterpret_cast<const uint64_t *>(res64_.get()), n);
+      std::vector<double> h(n);
+      TG_HIP(hipMemcpy(h.data(), res64_.get(), n * 8ull, hipMemcpyDeviceToHost));
+      double sum = 0;
+      for (double x : h) sum += x;
+      r.fvalue = sum;
+      if (keep_) r.r64 = std::move(h);
+      break;
+    }
+    case kReqLtc: {
+      uint64_t c = 0;
+      T.ltc(&r.src, 1, &c);
+      r.value = c;
+      break;
+    }
+    default:
+      throw Error(-1, "unknown request type");
+  }
+  r.done = now();
+  r.version = version_;
+  r.status = 0;
+  r.group = groups_++;
+}
+
+void Server::exec_query_group(const std::vector<uint64_t> &ids) {
+  const uint32_t n = g_.num_vertices();
+  std::vector<uint64_t> bfs, cc, pr, ltc;
+  for (uint64_t id : ids) {
+    switch (reqs_[id].type) {
+      case kReqBfs: bfs.push_back(id); break;
+      case kReqCc: cc.push_back(id); break;
+      case kReqPr: pr.push_back(id); break;
+      case kReqLtc: ltc.push_back(id); break;
+      default: throw Error(-1, "unknown request type");
+    }
+  }
+  (void)g_.tier_lists();  // shared, read-only while the group runs
+  const uint64_t ver = version_, grp = groups_++;
+  double st = now();
+  auto finish = [&](uint64_t id) {
+    Request &r = reqs_[id];
+    r.start = st;
+    r.done = now();
+    r.version = ver;
+    r.group = grp;
+    r.status = 0;
+  };
+  std::vector<std::thread> th;
+  std::vector<std::string> errors(kLanes);
+  if (!bfs.empty())
+    th.emplace_back([&] {
+      try {
+        Traversal &T = *trav_[0];
+        hipStream_t s = T.stream();
+        for (size_t c0 = 0; c0 < bfs.size(); c0 += 64) {
+          uint32_t k = static_cast<uint32_t>(std::min<size_t>(64, bfs.size() - c0));
+          std::vector<uint32_t> srcs(k);
+          for (uint32_t j = 0; j < k; j++) srcs[j] = reqs_[bfs[c0 + j]].src;
+          res32_[0].ensure(static_cast<uint64_t>(k) * n, s);
+          T.bfs_multi(srcs.data(), k, res32_[0].get());
+          for (uint32_t j = 0; j < k; j++) {
+            Request &r = reqs_[bfs[c0 + j]];
+            const uint32_t *lv = res32_[0].get() + static_cast<uint64_t>(j) * n;
+            r.digest = T.digest(lv, nullptr, n);
+            if (keep_) {
+              r.r32.resize(n);
+              TG_HIP(hipMemcpyAsync(r.r32.data(), lv, n * 4ull, hipMemcpyDeviceToHost, s));
+              TG_HIP(hipStreamSynchronize(s));
+            }
+            finish(bfs[c0 + j]);
+          }
+        }
+      } catch (const std::exception &e) {
+        errors[0] = e.what();
+      }
+    });
+  if (!cc.empty())
+    th.emplace_back([&] {
+      try {
+        Traversal &T = *trav_[1];
+        hipStream_t s = T.stream();
+        res32_[1].ensure(n, s);
+        T.cc(res32_[1].get());
+        uint64_t d = T.digest(res32_[1].get(), nullptr, n);
+        std::vector<uint32_t> h;
+        if (keep_) {
+          h.resize(n);
+          TG_HIP(hipMemcpyAsync(h.data(), res32_[1].get(), n * 4ull, hipMemcpyDeviceToHost, s));
+          TG_HIP(hipStreamSynchronize(s));
+        }
+        for (uint64_t id : cc) {
+          reqs_[id].digest = d;
+          if (keep_) reqs_[id].r32 = h;
+          finish(id);
+        }
+      } catch (const std::exception &e) {
+        errors[1] = e.what();
+      }
+    });
+  if (!pr.empty())
+    th.emplace_back([&] {
+      try {
+        Traversal &T = *trav_[2];
+        hipStream_t s = T.stream();
+        std::map<uint32_t, std::vector<uint64_t>> by_iters;
+        for (uint64_t id : pr) by_iters[reqs_[id].iters].push_back(id);
+        res64_.ensure(n, s);
+        for (auto &kv : by_iters) {
+          T.pagerank(kv.first, res64_.get());
+          uint64_t d = T.digest(nullptr, reinterpret_cast<const uint64_t *>(res64_.get()), n);
+          std::vector<double> h(n);
+          TG_HIP(hipMemcpyAsync(h.data(), res64_.get(), n * 8ull, hipMemcpyDeviceToHost, s));
+          TG_HIP(hipStreamSynchronize(s));
+          double sum = 0;
+          for (double x : h) sum += x;
+          for (uint64_t id : kv.second) {
+            reqs_[id].digest = d;
+            reqs_[id].fvalue = sum;
+            if (keep_) reqs_[id].r64 = h;
+            finish(id);
+          }
+        }
+      } catch (const std::exception &e) {
+        errors[2] = e.what();
+      }
+    });
+  if (!ltc.empty())
+    th.emplace_back([&] {
+      try {
+        Traversal &T = *trav_[3];
+        std::vector<uint32_t> vs(ltc.size());
+        for (size_t i = 0; i < ltc.size(); i++) vs[i] = reqs_[ltc[i]].src;
+        std::vector<uint64_t> out(ltc.size());
+        T.ltc(vs.data(), static_cast<uint32_t>(vs.size()), out.data());
+        for (size_t i = 0; i < ltc.size(); i++) {
+          reqs_[ltc[i]].value = out[i];
+          finish(ltc[i]);
+        }
+      } catch (const std::exception &e) {
+        errors[3] = e.what();
+      }
+    });
+  for (auto &t : th) t.join();
+  for (auto &e : errors)
+    if (!e.empty()) throw Error(-3, "query group: " + e);
+}
+
+void Server::run() {
+  std::vector<uint64_t> order(reqs_.size());
+  for (uint64_t i = 0; i < order.size(); i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint64_t a, uint64_t b) { return reqs_[a].arrival < reqs_[b].arrival; });
+  (void)g_.tier_lists();
+  t0_ = wall();
+  size_t next = 0;  // first request (in arrival order) not yet executed
+  while (next < order.size()) {
+    double t = now();
+    Request &head = reqs_[order[next]];
+    if (head.arrival > t) {
+      double wait = head.arrival - t;
+      if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
+      continue;
+    }
+    // a request larger than the whole budget can never be admitted
+    if (head.footprint > budget_) {
+      head.status = -2;
+      head.start = head.done = now();
+      head.version = version_;
+      next++;
+      continue;
+    }
+    if (!batched_) {
+      if (head.type == kReqUpdate) exec_update_group({order[next]});
+      else exec_query_plain(head);
+      next++;
+      continue;
+    }
+    // maximal arrival-ordered run of the head's kind that has arrived and fits
+    bool upd = head.type == kReqUpdate;
+    std::vector<uint64_t> ids;
+    uint64_t used = 0;
+    size_t j = next;
+    while (j < order.size()) {
+      Request &r = reqs_[order[j]];
+      if (r.arrival > t || (r.type == kReqUpdate) != upd) break;
+      if (r.footprint > budget_) break;  // rejected on its own turn
+      if (!ids.empty() && used + r.footprint > budget_) break;
+      used += r.footprint;
+      ids.push_back(order[j]);
+      j++;
+    }
+    if (upd) exec_update_group(ids);
+    else exec_query_group(ids);
+    next = j;
+  }
+}
+
+std::vector<double> Server::summary() const {
+  std::vector<double> all, up, q;
+  double first = 1e300, last = 0;
+  double rejected = 0;
+  for (const Request &r : reqs_) {
+    if (r.status != 0) {
+      rejected++;
+      continue;
+    }
+    double lat = r.done - r.arrival;
+    all.push_back(lat);
+    (r.type == kReqUpdate ? up : q).push_back(lat);
+    first = std::min(first, r.arrival);
+    last = std::max(last, r.done);
+  }
+  double span = all.empty() ? 0 : last - first;
+  return {span, span > 0 ? all.size() / span : 0, pct(all, 0.5), pct(all, 0.99), pct(up, 0.5), pct(up, 0.99),
+          pct(q, 0.5), pct(q, 0.99), static_cast<double>(groups_), rejected};
+}
+
+}  // namespace tg

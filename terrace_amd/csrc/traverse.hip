@@ -542,6 +542,206 @@ __global__ void __launch_bounds__(256) k_tc(const uint64_t *off, const uint32_t 
   if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
 }
 
+// ---------------------------------------------------------------- multi-source BFS
+// 64 BFS sources share one traversal: bit j of a vertex's mask is source j.
+// Pull: a vertex ORs its neighbours' frontier masks; push: a frontier vertex
+// ORs its mask (minus what the target has seen) into each neighbour.
+struct PullMS {
+  const uint64_t *fr;
+  uint64_t *nx;
+  const uint64_t *vis;
+  uint64_t full;
+  using Acc = uint64_t;
+  static constexpr bool kEarly = false;
+  __device__ static Acc ident() { return 0ull; }
+  __device__ bool skip(uint32_t i) const { return (vis[i] & full) == full; }
+  __device__ bool member(uint32_t) const { return true; }
+  __device__ Acc value(uint32_t v) const { return fr[v]; }
+  __device__ static Acc combine(Acc a, Acc b) { return a | b; }
+  __device__ static Acc wave_reduce(Acc a) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a |= __shfl_xor(a, off, 64);
+    return a;
+  }
+  __device__ static Acc reduce16(Acc a, uint64_t) {
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) a |= __shfl_xor(a, off, 64);
+    return a;
+  }
+  __device__ static bool found(Acc) { return false; }
+  __device__ void commit(uint32_t i, Acc a, Mode m) const {
+    a &= ~vis[i];
+    if (!a) return;
+    if (m == kShared) atomicOr(reinterpret_cast<unsigned long long *>(&nx[i]), static_cast<unsigned long long>(a));
+    else nx[i] |= a;
+  }
+};
+
+struct PushMS {
+  const uint64_t *fr;
+  uint64_t *nx;
+  const uint64_t *vis;
+  __device__ bool apply(uint32_t s, uint32_t d) const {
+    uint64_t m = fr[s] & ~vis[d];
+    if (!m) return false;
+    atomicOr(reinterpret_cast<unsigned long long *>(&nx[d]), static_cast<unsigned long long>(m));
+    return true;
+  }
+};
+
+__global__ void k_ms_init(const uint32_t *srcs, uint32_t k, uint32_t n, uint64_t *vis, uint64_t *fr,
+                          uint32_t *levels) {
+  GRID_STRIDE(j, k) {
+    uint32_t s = srcs[j];
+    atomicOr(reinterpret_cast<unsigned long long *>(&vis[s]), 1ull << j);
+    atomicOr(reinterpret_cast<unsigned long long *>(&fr[s]), 1ull << j);
+    levels[j * static_cast<uint64_t>(n) + s] = 0;
+  }
+}
+
+// New bits = reached this level for the first time; they become the next
+// frontier, their level is written, and the vertex joins the active set.
+__global__ void k_ms_finalize(uint32_t n, uint64_t *vis, uint64_t *fr, uint64_t *nx, uint32_t *levels, uint32_t depth,
+                              uint32_t *obits) {
+  GRID_STRIDE(i, n) {
+    uint64_t nw = nx[i] & ~vis[i];
+    nx[i] = 0;
+    fr[i] = nw;
+    if (nw) {
+      vis[i] |= nw;
+      bit_set(obits, static_cast<uint32_t>(i));
+      while (nw) {
+        uint32_t b = __ffsll(static_cast<unsigned long long>(nw)) - 1;
+        levels[b * static_cast<uint64_t>(n) + i] = depth;
+        nw &= nw - 1;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- local triangle count
+// Wave-wide walk over the sorted neighbours of u: f(w, valid) per lane.
+template <class F>
+__device__ inline void wave_foreach_nbr(const GView &g, uint32_t u, F f) {
+  const uint32_t lane = __lane_id();
+  const VBlock &b = g.vb[u];
+  uint32_t D = b.degree;
+  uint32_t lim = D < kInline ? D : kInline;
+  f(lane < lim ? b.nbr[lane < kInline ? lane : 0] : 0u, lane < lim);
+  if (D <= kInline) return;
+  uint32_t need = D - kInline;
+  if (b.aux & kAuxTree) {
+    uint32_t t = static_cast<uint32_t>(b.aux);
+    uint64_t io = g.toff[t];
+    uint32_t nb = g.tnblk[t];
+    for (uint32_t j = 0; j < nb; j++) {
+      uint32_t blk = g.iblk[io + j];
+      uint32_t c = g.bcnt[blk];
+      for (uint32_t e0 = 0; e0 < c; e0 += 64) {
+        uint32_t e = e0 + lane;
+        f(e < c ? g.bkeys[static_cast<uint64_t>(blk) * kBlk + e] : 0u, e < c);
+      }
+    }
+    return;
+  }
+  uint64_t p = b.aux;
+  uint32_t got = 0;
+  while (got < need && p < g.N) {
+    uint64_t slot = p + lane;
+    uint32_t d = slot < g.N ? g.pdst[slot] : kEmpty;
+    bool ok = d != kEmpty;
+    uint64_t m = __ballot(ok);
+    uint32_t r = __popcll(m & lanemask_lt());
+    f(d, ok && got + r < need);
+    got += __popcll(m);
+    p += 64;
+  }
+}
+
+// One wave per queried vertex: its sorted neighbour list into buf[off[q]..].
+__global__ void __launch_bounds__(256) k_gather_lists(GView g, const uint32_t *verts, uint32_t q, const uint64_t *off,
+                                                      uint32_t *buf) {
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t w = wave; w < q; w += nwaves) {
+    uint64_t base = off[w];
+    uint32_t pos = 0;
+    wave_foreach_nbr(g, verts[w], [&](uint32_t d, bool valid) {
+      uint64_t m = __ballot(valid);
+      if (valid) buf[base + pos + __popcll(m & lanemask_lt())] = d;
+      pos += __popcll(m);
+    });
+    (void)lane;
+  }
+}
+
+// One wave per (query, neighbour u): |N(u) ∩ N(v)| by binary search in the
+// gathered N(v); the sum over u is twice the triangles through v.
+__global__ void __launch_bounds__(256) k_ltc_count(GView g, const uint32_t *verts, uint32_t q, const uint64_t *off,
+                                                   const uint32_t *buf, uint64_t ntasks,
+                                                   unsigned long long *counts) {
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t t = wave; t < ntasks; t += nwaves) {
+    uint32_t lo = 0, hi = q;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (off[mid + 1] <= t) lo = mid + 1;
+      else hi = mid;
+    }
+    uint32_t qi = lo;
+    uint32_t v = verts[qi];
+    const uint32_t *nv = buf + off[qi];
+    uint64_t dv = off[qi + 1] - off[qi];
+    uint32_t u = buf[t];
+    unsigned long long c = 0;
+    if (u != v) {
+      wave_foreach_nbr(g, u, [&](uint32_t w, bool valid) {
+        if (valid && w != u && w != v) {
+          uint64_t l2 = 0, h2 = dv;
+          while (l2 < h2) {
+            uint64_t mid = (l2 + h2) >> 1;
+            if (nv[mid] < w) l2 = mid + 1;
+            else h2 = mid;
+          }
+          if (l2 < dv && nv[l2] == w) c++;
+        }
+      });
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0 && c) atomicAdd(&counts[qi], c);
+  }
+}
+
+__global__ void k_deg_of(GView g, const uint32_t *verts, uint32_t q, uint64_t *out) {
+  GRID_STRIDE(i, q) out[i] = g.vb[verts[i]].degree;
+}
+
+// Order-independent 64-bit digest of an array: Σ mix(i, a[i]).
+__device__ inline unsigned long long mix64(unsigned long long x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+__global__ void __launch_bounds__(256) k_digest(const uint32_t *a32, const uint64_t *a64, uint64_t n,
+                                                unsigned long long *out) {
+  __shared__ unsigned long long part[4];
+  unsigned long long h = 0;
+  GRID_STRIDE(i, n) {
+    unsigned long long x = a32 ? a32[i] : a64[i];
+    h += mix64((static_cast<unsigned long long>(i) << 1) * 0x9E3779B97F4A7C15ull ^ mix64(x + 0x632BE59BD9B4E019ull));
+  }
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+
 // vertexMap functors of the algorithm headers over a vertex list (or all).
 __global__ void k_vertex_map(GView g, const uint32_t *list, uint64_t cnt, int vop, double *val, const double *inv,
                              const uint32_t *ids, uint32_t *prev, uint8_t *visited) {
@@ -568,7 +768,7 @@ __global__ void k_vertex_map(GView g, const uint32_t *list, uint64_t cnt, int vo
 
 // ===================================================================== host
 void Traversal::vertex_map(Frontier &f, int vop, const VArgs &a) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   if (vop < 1 || vop > 8) throw Error(-1, "unknown vertex op");
   if ((vop == 1 || vop == 2 || vop == 6) && !a.val) throw Error(-1, "vertex op needs val");
   if (vop == 7 && !a.visited) throw Error(-1, "CLEAR_VISITED needs visited");
@@ -589,7 +789,7 @@ void Traversal::vertex_map(Frontier &f, int vop, const VArgs &a) {
 }
 
 void Traversal::copy_frontier(Frontier &dst, Frontier &src) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   dst.n = src.n;
   dst.all = src.all;
   dst.count = src.count;
@@ -620,7 +820,7 @@ void Traversal::record(const char *name, hipEvent_t a, hipEvent_t b) {
 }
 
 void Traversal::frontier_single(Frontier &f, uint32_t v) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   f.n = g_.num_vertices();
   f.all = false;
   f.queue.ensure(f.n, s);
@@ -639,7 +839,7 @@ void Traversal::frontier_all(Frontier &f) {
 }
 
 void Traversal::frontier_empty(Frontier &f) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   f.n = g_.num_vertices();
   f.all = false;
   f.count = 0;
@@ -652,7 +852,7 @@ void Traversal::frontier_empty(Frontier &f) {
 
 void Traversal::to_bits(Frontier &f) {
   if (f.all || f.has_bits) return;
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   f.bits.ensure(f.words() + 1, s);
   fill_u32(f.bits.get(), 0, f.words(), s);
   if (f.count) {
@@ -663,7 +863,7 @@ void Traversal::to_bits(Frontier &f) {
 }
 
 void Traversal::to_queue(Frontier &f) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   if (f.has_queue) return;
   f.queue.ensure(f.n, s);
   if (f.all) {
@@ -677,7 +877,7 @@ void Traversal::to_queue(Frontier &f) {
   utmp_.ensure(words + 1, s);
   k_word_pop<<<grid_for(words, 256), 256, 0, s>>>(f.bits.get(), words, f.n, utmp_.get());
   TG_CHECK_LAUNCH();
-  exclusive_scan_u32(utmp_.get(), utmp_.get(), words, utmp_.get() + words, g_.scan_scratch, s);
+  exclusive_scan_u32(utmp_.get(), utmp_.get(), words, utmp_.get() + words, scan_, s);
   k_queue_from_bits<<<grid_for(words, 256), 256, 0, s>>>(f.bits.get(), words, f.n, utmp_.get(), f.queue.get());
   TG_CHECK_LAUNCH();
   f.count = read_scalar(utmp_.get() + words, s);
@@ -685,18 +885,18 @@ void Traversal::to_queue(Frontier &f) {
 }
 
 uint64_t Traversal::popcount_bits(const Frontier &f) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   uint64_t words = f.words();
   utmp_.ensure(words + 1, s);
   k_word_pop<<<grid_for(words, 256), 256, 0, s>>>(f.bits.get(), words, f.n, utmp_.get());
   TG_CHECK_LAUNCH();
-  exclusive_scan_u32(utmp_.get(), utmp_.get(), words, utmp_.get() + words, g_.scan_scratch, s);
+  exclusive_scan_u32(utmp_.get(), utmp_.get(), words, utmp_.get() + words, scan_, s);
   return read_scalar(utmp_.get() + words, s);
 }
 
 void Traversal::prepare_tree_tasks() {
   const Graph::TierLists &L = g_.tier_lists();
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   tree_list_ = L.tree;
   ntree_list_ = L.ntree;
   ttask_off_.ensure(static_cast<uint64_t>(L.ntree) + 1, s);
@@ -704,14 +904,14 @@ void Traversal::prepare_tree_tasks() {
     k_tree_task_counts<<<grid_for(L.ntree, 256), 256, 0, s>>>(g_.view(), L.tree, L.ntree, ttask_off_.get());
     TG_CHECK_LAUNCH();
   }
-  exclusive_scan_u64(ttask_off_.get(), ttask_off_.get(), L.ntree, ttask_off_.get() + L.ntree, g_.scan_scratch, s);
+  exclusive_scan_u64(ttask_off_.get(), ttask_off_.get(), L.ntree, ttask_off_.get() + L.ntree, scan_, s);
   ntree_tasks_ = read_scalar(ttask_off_.get() + L.ntree, s);
 }
 
 template <class Op>
 static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierLists &L, const uint32_t *tree_list,
                         uint32_t ntree, const uint64_t *task_off, uint64_t ntasks) {
-  hipStream_t s = g.stream();
+  hipStream_t s = T.stream();
   GView v = g.view();
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (T.profiling) {
@@ -738,7 +938,7 @@ static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierL
 }
 
 void Traversal::pull(Frontier &in, Op op, const OpState &st, Frontier *out) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   const Graph::TierLists &L = g_.tier_lists();
   prepare_tree_tasks();
   if (!in.all) to_bits(in);
@@ -786,7 +986,7 @@ void Traversal::pull(Frontier &in, Op op, const OpState &st, Frontier *out) {
 template <class Op>
 static void launch_push(Traversal &T, Graph &g, const Op &op, Frontier &in, Frontier *out, DevBuf<uint32_t> &cnt,
                         DevBuf<uint32_t> &sp, DevBuf<uint32_t> &st) {
-  hipStream_t s = g.stream();
+  hipStream_t s = T.stream();
   GView v = g.view();
   uint32_t n = g.num_vertices();
   cnt.ensure(4, s);
@@ -862,7 +1062,7 @@ void Traversal::edge_map(Frontier &in, Op op, const OpState &st, Frontier *out, 
 
 // ---------------------------------------------------------------- BFS
 uint32_t Traversal::bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t threshold) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   if (src >= n) throw Error(-1, "source out of range");
   k_fill_i32<<<grid_for(n, 256), 256, 0, s>>>(parent, -1, n);
@@ -889,7 +1089,7 @@ uint32_t Traversal::bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t
 
 // ---------------------------------------------------------------- PR
 void Traversal::pagerank(uint32_t iters, double *out) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   dtmp_.ensure(n, s);
   dtmp2_.ensure(n, s);
@@ -912,7 +1112,7 @@ void Traversal::pagerank(uint32_t iters, double *out) {
 
 // ---------------------------------------------------------------- CC
 uint32_t Traversal::cc(uint32_t *ids, uint32_t threshold) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   utmp_.ensure(n + 1, s);
   DevBuf<uint32_t> prev;
@@ -952,7 +1152,7 @@ uint32_t Traversal::cc(uint32_t *ids, uint32_t threshold) {
 
 // ---------------------------------------------------------------- BC
 void Traversal::bc(uint32_t src, double *out, uint32_t threshold) {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   if (src >= n) throw Error(-1, "source out of range");
   dtmp_.ensure(n, s);
@@ -1015,9 +1215,113 @@ void Traversal::bc(uint32_t src, double *out, uint32_t threshold) {
   TG_HIP(hipStreamSynchronize(s));
 }
 
+// ---------------------------------------------------------------- multi-source BFS
+uint32_t Traversal::bfs_multi(const uint32_t *srcs_host, uint32_t k, uint32_t *levels, uint32_t threshold) {
+  hipStream_t s = stream_;
+  uint32_t n = g_.num_vertices();
+  if (k == 0 || k > 64) throw Error(-1, "bfs_multi: 1..64 sources");
+  std::vector<uint32_t> hs(srcs_host, srcs_host + k);
+  for (uint32_t v : hs)
+    if (v >= n) throw Error(-1, "source out of range");
+  ms_vis_.ensure(n, s);
+  ms_fr_.ensure(n, s);
+  ms_nx_.ensure(n, s);
+  utmp_.ensure(n + 64, s);
+  fill_u64(ms_vis_.get(), 0, n, s);
+  fill_u64(ms_fr_.get(), 0, n, s);
+  fill_u64(ms_nx_.get(), 0, n, s);
+  fill_u32(levels, 0xFFFFFFFFu, static_cast<uint64_t>(k) * n, s);
+  DevBuf<uint32_t> dsrc;
+  dsrc.ensure(k, s);
+  TG_HIP(hipMemcpyAsync(dsrc.get(), hs.data(), k * 4, hipMemcpyHostToDevice, s));
+  k_ms_init<<<1, 64, 0, s>>>(dsrc.get(), k, n, ms_vis_.get(), ms_fr_.get(), levels);
+  TG_CHECK_LAUNCH();
+  std::vector<uint32_t> uniq(hs);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  Frontier a, b;
+  a.n = n;
+  a.queue.ensure(n, s);
+  TG_HIP(hipMemcpyAsync(a.queue.get(), uniq.data(), uniq.size() * 4, hipMemcpyHostToDevice, s));
+  TG_HIP(hipStreamSynchronize(s));
+  a.count = uniq.size();
+  a.has_queue = true;
+  Frontier *cur = &a, *nxt = &b;
+  uint64_t full = k == 64 ? ~0ull : ((1ull << k) - 1);
+  uint32_t depth = 0;
+  while (cur->count > 0) {
+    depth++;
+    bool dense = (n / (threshold ? threshold : 1)) <= cur->count;
+    if (dense) {
+      const Graph::TierLists &L = g_.tier_lists();
+      prepare_tree_tasks();
+      PullMS p{ms_fr_.get(), ms_nx_.get(), ms_vis_.get(), full};
+      launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+    } else {
+      to_queue(*cur);
+      PushMS p{ms_fr_.get(), ms_nx_.get(), ms_vis_.get()};
+      launch_push(*this, g_, p, *cur, nullptr, cnt_, sec_pma_, sec_tree_);
+    }
+    frontier_empty(*nxt);
+    nxt->has_queue = false;
+    k_ms_finalize<<<grid_for(n, 256), 256, 0, s>>>(n, ms_vis_.get(), ms_fr_.get(), ms_nx_.get(), levels, depth,
+                                                   nxt->bits.get());
+    TG_CHECK_LAUNCH();
+    nxt->count = popcount_bits(*nxt);
+    std::swap(cur, nxt);
+  }
+  TG_HIP(hipStreamSynchronize(s));
+  return depth;
+}
+
+// ---------------------------------------------------------------- local triangle count
+void Traversal::ltc(const uint32_t *verts_host, uint32_t q, uint64_t *out_host) {
+  hipStream_t s = stream_;
+  uint32_t n = g_.num_vertices();
+  if (q == 0) return;
+  for (uint32_t i = 0; i < q; i++)
+    if (verts_host[i] >= n) throw Error(-1, "vertex out of range");
+  DevBuf<uint32_t> dv;
+  DevBuf<uint64_t> off;
+  DevBuf<unsigned long long> cnt;
+  dv.ensure(q, s);
+  off.ensure(q + 1, s);
+  cnt.ensure(q, s);
+  TG_HIP(hipMemcpyAsync(dv.get(), verts_host, q * 4, hipMemcpyHostToDevice, s));
+  k_deg_of<<<grid_for(q, 256), 256, 0, s>>>(g_.view(), dv.get(), q, off.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u64(off.get(), off.get(), q, off.get() + q, scan_, s);
+  uint64_t total = read_scalar(off.get() + q, s);
+  utmp_.ensure(total + 1, s);
+  fill_u64(reinterpret_cast<uint64_t *>(cnt.get()), 0, q, s);
+  if (total) {
+    k_gather_lists<<<grid_for(static_cast<uint64_t>(q) * 64, 256), 256, 0, s>>>(g_.view(), dv.get(), q, off.get(),
+                                                                                utmp_.get());
+    TG_CHECK_LAUNCH();
+    k_ltc_count<<<grid_for(total * 64, 256), 256, 0, s>>>(g_.view(), dv.get(), q, off.get(), utmp_.get(), total,
+                                                          cnt.get());
+    TG_CHECK_LAUNCH();
+  }
+  std::vector<unsigned long long> h(q);
+  TG_HIP(hipMemcpyAsync(h.data(), cnt.get(), q * 8, hipMemcpyDeviceToHost, s));
+  TG_HIP(hipStreamSynchronize(s));
+  for (uint32_t i = 0; i < q; i++) out_host[i] = h[i] / 2;
+}
+
+uint64_t Traversal::digest(const uint32_t *a32, const uint64_t *a64, uint64_t n) {
+  hipStream_t s = stream_;
+  red_.ensure(2, s);
+  fill_u64(red_.get(), 0, 1, s);
+  if (n) {
+    k_digest<<<grid_for(n, 256, 4096), 256, 0, s>>>(a32, a64, n, reinterpret_cast<unsigned long long *>(red_.get()));
+    TG_CHECK_LAUNCH();
+  }
+  return read_scalar(red_.get(), s);
+}
+
 // ---------------------------------------------------------------- TC
 uint64_t Traversal::tc() {
-  hipStream_t s = g_.stream();
+  hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   uint64_t m = g_.num_edges();
   csr_off_.ensure(static_cast<uint64_t>(n) + 1, s);

@@ -36,7 +36,10 @@ struct OpState {
 
 class Traversal {
  public:
-  explicit Traversal(Graph &g) : g_(g) {}
+  // s = null: use the graph's stream.  Traversals with their own streams
+  // may run concurrently on one graph while no update is in flight.
+  explicit Traversal(Graph &g, hipStream_t s = nullptr) : g_(g), stream_(s ? s : g.stream()) {}
+  hipStream_t stream() const { return stream_; }
 
   // Frontier helpers
   void frontier_single(Frontier &f, uint32_t v);
@@ -67,6 +70,12 @@ class Traversal {
   uint32_t cc(uint32_t *ids, uint32_t threshold = 20);
   void bc(uint32_t src, double *out, uint32_t threshold = 20);
   uint64_t tc();
+  // 64 BFS in one traversal: levels[j*n + v] for source j (k <= 64).
+  uint32_t bfs_multi(const uint32_t *srcs_host, uint32_t k, uint32_t *levels, uint32_t threshold = 20);
+  // Triangles through each queried vertex (host in / host out).
+  void ltc(const uint32_t *verts_host, uint32_t q, uint64_t *out_host);
+  // Order-independent digest of a device array (u32 or u64 words).
+  uint64_t digest(const uint32_t *a32, const uint64_t *a64, uint64_t n);
 
   // Per-kernel timing on the graph's stream (hipEvents), keyed by name.
   struct KTime {
@@ -82,6 +91,8 @@ class Traversal {
 
  private:
   Graph &g_;
+  hipStream_t stream_;
+  ScanScratch scan_;
   DevBuf<uint32_t> cnt_;        // device counters
   DevBuf<uint32_t> sec_pma_, sec_tree_;  // frontier vertices with secondary runs
   DevBuf<uint64_t> ttask_off_;  // tree task prefix over listed tree vertices
@@ -91,6 +102,7 @@ class Traversal {
   DevBuf<uint64_t> csr_off_;
   DevBuf<uint32_t> csr_nbr_;
   DevBuf<uint64_t> red_;
+  DevBuf<uint64_t> ms_vis_, ms_fr_, ms_nx_;
   uint64_t ntree_tasks_ = 0;
   const uint32_t *tree_list_ = nullptr;
   uint32_t ntree_list_ = 0;

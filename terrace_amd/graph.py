@@ -12,11 +12,11 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (OP_BC_BACK, OP_BC_FWD, OP_BFS, OP_CC, OP_PR, DeviceBuffer, OpArgs,
-                   TerraceError, check, lib, ptr)
+from ._lib import (OP_BC_BACK, OP_BC_FWD, OP_BFS, OP_CC, OP_PR, DeviceBuffer, OpArgs, Request,
+                   Result, TerraceError, check, lib, ptr)
 
 __all__ = ["Graph", "VertexSubset", "edgeMap", "BFS_with_edge_map", "PR_S", "CC", "BC", "TC",
-           "rmat_device", "TerraceError", "DeviceBuffer"]
+           "bfs_multi", "local_triangles", "Server", "rmat_device", "TerraceError", "DeviceBuffer"]
 
 
 def _u32(a) -> np.ndarray:
@@ -247,6 +247,84 @@ def TC(G: Graph) -> int:
     r = ctypes.c_uint64()
     check(lib().tg_tc(G.handle, ctypes.byref(r)))
     return r.value
+
+
+def bfs_multi(G: Graph, sources) -> np.ndarray:
+    """BFS levels from up to 64 sources in one traversal: array [k, n]."""
+    srcs = _u32(sources)
+    out = np.empty((len(srcs), G.get_num_vertices()), np.uint32)
+    check(lib().tg_bfs_multi(G.handle, ptr(srcs), len(srcs), ptr(out)))
+    return out
+
+
+def local_triangles(G: Graph, verts) -> np.ndarray:
+    """Triangles through each vertex of verts."""
+    v = _u32(verts)
+    out = np.zeros(len(v), np.uint64)
+    check(lib().tg_ltc(G.handle, ptr(v), len(v), ptr(out)))
+    return out
+
+
+REQ_UPDATE, REQ_BFS, REQ_CC, REQ_PR, REQ_LTC = 1, 2, 3, 4, 5
+
+
+class Server:
+    """Request loop over one graph (tg_server_*): batched=False replays the
+    stream one request at a time, batched=True with continuous batching."""
+
+    def __init__(self, G: Graph, batched: bool, mem_budget: int = 1 << 40, keep_results: bool = False):
+        h = ctypes.c_void_p()
+        check(lib().tg_server_create(G.handle, int(batched), int(mem_budget), int(keep_results),
+                                     ctypes.byref(h)))
+        self._h, self._g = h, G
+        self._keep = []  # host arrays referenced until submit returns
+
+    def submit(self, type: int, arrival: float, src: int = 0, iters: int = 10,
+               ins=None, dels=None) -> int:
+        r = Request(type=type, arrival_s=float(arrival), src=int(src), iters=int(iters))
+        if ins is not None:
+            s, d = _u32(ins[0]), _u32(ins[1])
+            r.ins_src, r.ins_dst, r.n_ins = ptr(s), ptr(d), len(s)
+            self._keep += [s, d]
+        if dels is not None:
+            s, d = _u32(dels[0]), _u32(dels[1])
+            r.del_src, r.del_dst, r.n_del = ptr(s), ptr(d), len(s)
+            self._keep += [s, d]
+        rid = ctypes.c_uint64()
+        check(lib().tg_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
+        self._keep.clear()
+        return rid.value
+
+    def run(self) -> None:
+        check(lib().tg_server_run(self._h))
+
+    def result(self, rid: int) -> dict:
+        r = Result()
+        check(lib().tg_server_result(self._h, int(rid), ctypes.byref(r)))
+        return {f: getattr(r, f) for f, _ in Result._fields_}
+
+    def result_data(self, rid: int, dtype) -> np.ndarray:
+        out = np.empty(self._g.get_num_vertices(), dtype)
+        check(lib().tg_server_result_data(self._h, int(rid), ptr(out), out.nbytes))
+        return out
+
+    def summary(self) -> dict:
+        out = np.zeros(10, np.float64)
+        check(lib().tg_server_summary(self._h, ptr(out)))
+        keys = ["makespan_s", "requests_per_s", "p50_s", "p99_s", "p50_update_s", "p99_update_s",
+                "p50_query_s", "p99_query_s", "groups", "rejected"]
+        return dict(zip(keys, out.tolist()))
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            lib().tg_server_destroy(self._h)
+        self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def rmat_device(n: int, seed: int, a: float, b: float, c: float, count: int, start: int = 0,

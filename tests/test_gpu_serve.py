@@ -1,0 +1,180 @@
+"""GPU tests of the batching / serving layer: multi-source BFS equals
+single-source BFS, batched local triangle counts equal the oracle, and every
+request's result is the same alone (plain server) and in the mix (batched
+server), on the same version of the graph."""
+import numpy as np
+import pytest
+
+import oracle
+from terrace_amd import graph as tg
+
+pytestmark = pytest.mark.gpu
+UNREACHED = np.iinfo(np.uint32).max
+
+
+def split(keys):
+    keys = np.asarray(keys, np.uint64)
+    return ((keys >> np.uint64(32)).astype(np.uint32),
+            (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+
+
+def make_graph(n, keys):
+    G = tg.Graph(n)
+    G.add_edge_batch(*split(keys))
+    return G
+
+
+@pytest.mark.parametrize("skew", [False, True])
+def test_bfs_multi_matches_single(skew):
+    n = 1 << 12
+    a, b, c = (0.57, 0.19, 0.19) if skew else (0.5, 0.1, 0.1)
+    print("[gpu-test] bfs_multi", skew, flush=True)
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 41, a, b, c, 60000))
+    G = make_graph(n, keys)
+    g = oracle.CSR(keys, n)
+    rng = np.random.default_rng(5)
+    srcs = rng.integers(0, n, 64).astype(np.uint32)
+    srcs[1] = srcs[0]  # duplicate source in two lanes
+    srcs[2] = int(np.argmax(g.degrees()))
+    lv = tg.bfs_multi(G, srcs)
+    for j in range(len(srcs)):
+        want = oracle.bfs_levels(g, int(srcs[j]))
+        assert np.array_equal(lv[j], want), j
+    # fewer lanes than 64 and a single lane
+    lv3 = tg.bfs_multi(G, srcs[:3])
+    assert np.array_equal(lv3, lv[:3])
+    _, single = tg.BFS_with_edge_map(G, int(srcs[5]), with_levels=True)
+    assert np.array_equal(tg.bfs_multi(G, srcs[5:6])[0], single)
+    G.close()
+
+
+def test_local_triangles_match_oracle():
+    n = 1 << 12
+    print("[gpu-test] ltc", flush=True)
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 43, 0.57, 0.19, 0.19, 80000))
+    G = make_graph(n, keys)
+    g = oracle.CSR(keys, n)
+    deg = g.degrees()
+    verts = np.concatenate([np.argsort(deg)[-5:], np.arange(0, n, 97), [np.argmin(deg)]]).astype(np.uint32)
+    got = tg.local_triangles(G, verts)
+    want = np.array([oracle.ltc(g, int(v)) for v in verts], np.uint64)
+    assert np.array_equal(got, want)
+    # the per-vertex counts sum to three times the triangle count
+    allv = np.arange(n, dtype=np.uint32)
+    assert int(tg.local_triangles(G, allv).sum()) == 3 * oracle.tc(g)
+    G.close()
+
+
+def build_stream(n, keys, rng):
+    """Interleaved updates and queries; conflicting ops on one edge across
+    batches (insert, delete, re-insert)."""
+    e = keys[rng.choice(len(keys), 30, replace=False)]
+    reqs = []
+    t = 0.0
+
+    def upd(ins_keys, del_keys):
+        i_s, i_d = split(ins_keys)
+        d_s, d_d = split(del_keys)
+        return ("U", (np.concatenate([i_s, i_d]), np.concatenate([i_d, i_s])),
+                (np.concatenate([d_s, d_d]), np.concatenate([d_d, d_s])))
+
+    for r in range(5):
+        fresh = oracle.build_keys(*oracle.rmat_edges(n, 500 + r, 0.57, 0.19, 0.19, 300))
+        reqs.append(upd(fresh, e[r * 5:(r + 1) * 5]))
+        # same edges deleted by one batch and re-inserted by the next
+        reqs.append(upd(e[r * 5:(r + 1) * 5][:2], np.zeros(0, np.uint64)))
+        for q in range(12):
+            kind = rng.choice(["BFS", "BFS", "BFS", "CC", "PR", "LTC"])
+            reqs.append((kind, int(rng.integers(0, n))))
+    out = []
+    for req in reqs:
+        t += 1e-5
+        out.append((t, req))
+    return out
+
+
+def submit_all(S, stream):
+    ids = []
+    for t, req in stream:
+        if req[0] == "U":
+            ids.append(S.submit(tg.REQ_UPDATE, t, ins=req[1], dels=req[2]))
+        elif req[0] == "BFS":
+            ids.append(S.submit(tg.REQ_BFS, t, src=req[1]))
+        elif req[0] == "CC":
+            ids.append(S.submit(tg.REQ_CC, t))
+        elif req[0] == "PR":
+            ids.append(S.submit(tg.REQ_PR, t, iters=10))
+        else:
+            ids.append(S.submit(tg.REQ_LTC, t, src=req[1]))
+    return ids
+
+
+def test_server_alone_vs_mix():
+    n = 1 << 12
+    print("[gpu-test] server alone vs mix", flush=True)
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 47, 0.57, 0.19, 0.19, 50000))
+    rng = np.random.default_rng(11)
+    stream = build_stream(n, keys, rng)
+    G1, G2 = make_graph(n, keys), make_graph(n, keys)
+    plain = tg.Server(G1, batched=False, keep_results=True)
+    batched = tg.Server(G2, batched=True, keep_results=True)
+    ids1, ids2 = submit_all(plain, stream), submit_all(batched, stream)
+    plain.run()
+    batched.run()
+    # versions: the oracle state after each update batch, in order
+    states = [keys]
+    for t, req in stream:
+        if req[0] == "U":
+            k = states[-1]
+            ins = oracle.sort_unique((req[1][0].astype(np.uint64) << np.uint64(32)) | req[1][1])
+            dels = oracle.sort_unique((req[2][0].astype(np.uint64) << np.uint64(32)) | req[2][1])
+            k = oracle.set_difference(oracle.set_union(k, ins), dels)
+            states.append(k)
+    groups = set()
+    for (t, req), a, b in zip(stream, ids1, ids2):
+        ra, rb = plain.result(a), batched.result(b)
+        assert ra["status"] == 0 and rb["status"] == 0
+        assert ra["version"] == rb["version"], (req[0], ra, rb)
+        groups.add(rb["group"])
+        if req[0] == "U":
+            continue
+        g = oracle.CSR(states[ra["version"]], n)
+        if req[0] == "BFS":
+            la, lb = plain.result_data(a, np.uint32), batched.result_data(b, np.uint32)
+            assert np.array_equal(la, lb) and np.array_equal(la, oracle.bfs_levels(g, req[1]))
+            assert ra["digest"] == rb["digest"]
+        elif req[0] == "CC":
+            la, lb = plain.result_data(a, np.uint32), batched.result_data(b, np.uint32)
+            assert np.array_equal(la, lb) and np.array_equal(la, oracle.cc(g))
+        elif req[0] == "PR":
+            pa, pb = plain.result_data(a, np.float64), batched.result_data(b, np.float64)
+            assert oracle.rel_l1(pb, pa) <= 1e-9 and oracle.rel_l1(pa, oracle.pagerank(g, 10)) <= 1e-9
+        else:
+            assert ra["value"] == rb["value"] == oracle.ltc(g, req[1])
+    # batching actually grouped requests
+    assert len(groups) < len(stream)
+    # both graphs end in the same state, equal to the oracle
+    for G in (G1, G2):
+        off, nbr = G.export_adjacency()
+        ref = oracle.CSR(states[-1], n)
+        assert np.array_equal(nbr, ref.nbr[:ref.m]) and np.array_equal(off, ref.off)
+    s1, s2 = plain.summary(), batched.summary()
+    assert s1["rejected"] == 0 and s2["rejected"] == 0
+    plain.close(), batched.close(), G1.close(), G2.close()
+
+
+def test_server_admission_rejects_oversized():
+    n = 1 << 12
+    print("[gpu-test] server admission", flush=True)
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 49, 0.5, 0.1, 0.1, 20000))
+    G = make_graph(n, keys)
+    # budget fits a BFS lane but not a PageRank (24 n bytes)
+    S = tg.Server(G, batched=True, mem_budget=20 * n)
+    a = S.submit(tg.REQ_BFS, 0.0, src=1)
+    b = S.submit(tg.REQ_PR, 0.0, iters=3)
+    c = S.submit(tg.REQ_BFS, 0.0, src=2)
+    S.run()
+    assert S.result(a)["status"] == 0 and S.result(c)["status"] == 0
+    assert S.result(b)["status"] == -2
+    assert S.summary()["rejected"] == 1
+    S.close(), G.close()
