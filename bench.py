@@ -28,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_pull_pma<PullSum> launch at the default workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (profiles/), or None.
+PMA_PULL_TRAFFIC = None
 
 
 def log(*a):
@@ -124,6 +127,137 @@ def cpu_baseline(quick: bool):
             "sample": "oracle set-union insert, rMAT scale 20, 5 batches of 1e5"}
 
 
+def serve_bench(args):
+    """Mixed request stream through the plain and the batched server.
+
+    rMAT (skewed a=.57 b=.19 c=.19) graph built twice on the device.  R
+    requests arrive as a Poisson process: 30% update batches (10..1e6 edges,
+    log-uniform, 90% inserts of fresh rMAT edges / 10% deletes of existing
+    edges, both directions), 70% queries (60% BFS, 20% CC, 10% PageRank 10
+    iterations FP64, 10% local triangle count).  The same stream is replayed
+    one request at a time (plain) and with continuous batching; every query
+    result digest must agree between the two."""
+    from terrace_amd import graph as tg
+
+    n = 1 << args.scale
+    probs = (0.57, 0.19, 0.19)
+    seed = aspen(100) & 0xFFFFFFFF
+    rng = np.random.default_rng(2024)
+
+    def build():
+        G = tg.Graph(n)
+        s, d, w = tg.rmat_device(n, seed, *probs, args.edges, nominal=True, symmetric=True, drop_loops=True)
+        G.add_edge_batch_dev(s, d, w)
+        del s, d
+        return G
+
+    log(f"serve: building scale {args.scale} skewed graph, {args.edges} generated edges (x2)")
+    t0 = time.perf_counter()
+    G1 = build()
+    log(f"serve: graph 1 built in {time.perf_counter() - t0:.2f} s, {G1.get_num_edges()} edges, "
+        f"{G1.get_size() / 2**30:.1f} GiB")
+    off, nbr = G1.export_adjacency()
+    G2 = build()
+    m = len(nbr)
+    deg = (off[1:] - off[:-1])
+    active = np.nonzero(deg)[0].astype(np.uint32)
+
+    # request stream
+    R = args.requests
+    gaps = rng.exponential(1.0 / args.rate, R)
+    arrivals = np.cumsum(gaps)
+    reqs = []
+    t_gen = 0
+    for i in range(R):
+        u = rng.random()
+        if u < 0.3:
+            size = int(round(10 ** rng.uniform(1, 6)))
+            npairs = max(1, int(size * 0.9) // 2)
+            s, d, w = tg.rmat_device(n, aspen(7000 + t_gen) & 0xFFFFFFFF, *probs, npairs, nominal=True,
+                                     symmetric=True, drop_loops=True)
+            t_gen += 1
+            ins = (s.download(np.uint32, w), d.download(np.uint32, w))
+            nd = max(1, int(size * 0.1) // 2)
+            idx = rng.integers(0, m, nd)
+            ds = (np.searchsorted(off, idx, side="right") - 1).astype(np.uint32)
+            dd = nbr[idx]
+            dels = (np.concatenate([ds, dd]), np.concatenate([dd, ds]))
+            reqs.append(("U", arrivals[i], ins, dels))
+        else:
+            q = rng.random()
+            v = int(active[rng.integers(0, len(active))])
+            if q < 0.6:
+                reqs.append(("BFS", arrivals[i], v))
+            elif q < 0.8:
+                reqs.append(("CC", arrivals[i]))
+            elif q < 0.9:
+                reqs.append(("PR", arrivals[i]))
+            else:
+                reqs.append(("LTC", arrivals[i], v))
+    del off, nbr
+
+    def replay(G, batched):
+        S = tg.Server(G, batched=batched, mem_budget=args.budget << 30)
+        ids = []
+        for r in reqs:
+            if r[0] == "U":
+                ids.append(S.submit(tg.REQ_UPDATE, r[1], ins=r[2], dels=r[3]))
+            elif r[0] == "BFS":
+                ids.append(S.submit(tg.REQ_BFS, r[1], src=r[2]))
+            elif r[0] == "CC":
+                ids.append(S.submit(tg.REQ_CC, r[1]))
+            elif r[0] == "PR":
+                ids.append(S.submit(tg.REQ_PR, r[1], iters=10))
+            else:
+                ids.append(S.submit(tg.REQ_LTC, r[1], src=r[2]))
+        t0 = time.perf_counter()
+        S.run()
+        wall = time.perf_counter() - t0
+        res = [S.result(i) for i in ids]
+        summ = S.summary()
+        summ["wall_s"] = wall
+        S.close()
+        return summ, res
+
+    log(f"serve: replaying {R} requests at {args.rate}/s, plain")
+    sp, rp = replay(G1, False)
+    log(f"serve: plain {sp}")
+    log("serve: batched")
+    sb, rb = replay(G2, True)
+    log(f"serve: batched {sb}")
+    mism = 0
+    for r, a, b in zip(reqs, rp, rb):
+        if a["version"] != b["version"]:
+            mism += 1
+        elif r[0] in ("BFS", "CC") and a["digest"] != b["digest"]:
+            mism += 1
+        elif r[0] == "LTC" and a["value"] != b["value"]:
+            mism += 1
+        elif r[0] == "PR" and abs(a["fvalue"] - b["fvalue"]) > 1e-9 * max(abs(a["fvalue"]), 1e-300):
+            mism += 1
+    kinds = {k: sum(1 for r in reqs if r[0] == k) for k in ("U", "BFS", "CC", "PR", "LTC")}
+    out = {
+        "metric": "mixed-stream requests/s (batched server)",
+        "value": sb["requests_per_s"],
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "steps": R,
+        "warmup": 0,
+        "ms_per_step": sb["makespan_s"] / R * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/f64",
+        "data": "synthetic rMAT generated on the device (seeded); Poisson arrivals",
+        "config": {"workload": f"serving: rmat scale {args.scale} skewed (.57/.19/.19), {args.edges} generated "
+                               f"edges; {R} requests at {args.rate}/s", "vertices": n,
+                   "directed_edges": G1.get_num_edges(), "requests": kinds, "parallelism": "single GPU"},
+        "serve": {"plain": sp, "batched": sb, "throughput_x": sb["requests_per_s"] / max(sp["requests_per_s"], 1e-9),
+                  "p99_ratio": sb["p99_s"] / max(sp["p99_s"], 1e-12), "result_mismatches": mism},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,11 +269,19 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller graph for smoke runs")
+    ap.add_argument("--serve", action="store_true", help="mixed-stream serving comparison instead")
+    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--rate", type=float, default=200.0, help="request arrivals per second")
+    ap.add_argument("--budget", type=int, default=64, help="server memory budget, GiB")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.serve:
+        serve_bench(args)
+        return
 
     from terrace_amd import graph as tg
 
@@ -203,15 +345,27 @@ def main():
         m_now = G.get_num_edges()
         extra["pr_ms"] = t_pr * 1e3
         extra["pr_gteps"] = 10 * m_now / t_pr / 1e9
-        pull_ms, pull_n = G.profile_get("edge_map_pull")
-        # PR pull per iteration: algorithmic bytes 4m + 8n (metadata) + 16n
-        per_iter_bytes = 4 * m_now + 8 * n + 16 * n
-        avg_ms = pull_ms / max(pull_n, 1)
-        achieved = per_iter_bytes / (avg_ms * 1e-3) / 1e9
+        # Roofline of the dominant kernel, k_pull_pma<PullSum> (one launch per
+        # PageRank iteration), timed with HIP events on the traversal's stream.
+        # Algorithmic bytes per launch: the PageRank per-iteration figure
+        # (4 B per edge + 8 B metadata + 16 B p/p_next per vertex) over the
+        # units one launch processes: the PMA-tier vertices and their edges
+        # beyond the 13 inline slots.
+        tc_ = G.tier_counts()
+        pma_ms, pma_n = G.profile_get("pull_pma_sum")
+        pma_bytes = 4 * tc_["pma_edges"] + 24 * tc_["pma_vertices"]
+        pma_avg = pma_ms / max(pma_n, 1)
+        achieved = pma_bytes / (pma_avg * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                    "kernel": "PageRank dense pull (k_pull_inline + k_pull_pma + k_pull_tree), one iteration",
-                    "avg_ms": avg_ms}
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": PMA_PULL_TRAFFIC,
+                    "kernel": "k_pull_pma<PullSum> (PageRank dense pull over PMA-tier vertices)",
+                    "avg_ms": pma_avg, "launches": pma_n, "algorithmic_bytes": pma_bytes,
+                    "units": tc_}
+        pull_ms, pull_n = G.profile_get("edge_map_pull")
+        extra["pr_pull_all_tiers_avg_ms"] = pull_ms / max(pull_n, 1)
+        for kn in ("pull_inline_sum", "pull_tree_sum"):
+            ms_, n_ = G.profile_get(kn)
+            extra[kn + "_avg_ms"] = ms_ / max(n_, 1)
         # CC
         t0 = time.perf_counter()
         tg.CC(G)

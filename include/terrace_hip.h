@@ -81,6 +81,10 @@ int tg_export_adjacency_dev(tg_graph *g, uint64_t *offsets, uint32_t *nbrs);
 int tg_neighbors(tg_graph *g, uint32_t v, uint32_t *out, uint32_t cap, uint32_t *degree);
 /* Tier of every vertex: 0 inline only, 1 PMA, 2 B-tree. */
 int tg_tiers(tg_graph *g, uint8_t *out /* [n] */);
+/* Tier sizes: out[0] PMA-tier vertices, out[1] their edges beyond the 13
+ * inline slots, out[2] B-tree vertices, out[3] their edges beyond the inline
+ * slots.  Graph::get_size's breakdown (include/graph.h:805-815) by tier. */
+int tg_tier_counts(tg_graph *g, uint64_t *out /* [4] */);
 
 /* ---- algorithms (run_algorithm's tests, src/graph_bm.cc:86-229) ------- */
 /* BFS_with_edge_map (include/BFS.h:41-84): parents[n] (-1 unreached),

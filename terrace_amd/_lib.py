@@ -131,6 +131,7 @@ _SIGS = {
     "tg_profile_get": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_uint64)]),
     "tg_stats": (ctypes.c_int, [_P, _P]),
+    "tg_tier_counts": (ctypes.c_int, [_P, _P]),
 }
 
 

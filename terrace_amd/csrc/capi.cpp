@@ -525,6 +525,14 @@ int tg_ltc(tg_graph *G, const uint32_t *verts, uint32_t q, uint64_t *counts) {
   });
 }
 
+int tg_tier_counts(tg_graph *G, uint64_t *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    std::lock_guard<std::mutex> lk(G->g.mu);
+    G->g.tier_counts(out);
+  });
+}
+
 }  // extern "C"
 
 struct tg_server {
@@ -680,7 +688,7 @@ int tg_profile_get(tg_graph *G, const char *name, double *ms, uint64_t *launches
     *ms = 0;
     *launches = 0;
     for (auto &k : G->t.ktimes)
-      if (std::strcmp(k.name, name) == 0) {
+      if (k.name == name) {
         *ms = k.ms;
         *launches = k.launches;
       }

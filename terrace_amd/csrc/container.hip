@@ -15,6 +15,7 @@
 // sort + unique, existence filter, one thread per source run for the
 // inline merge, then one pass per secondary tier.
 #include "graph.hpp"
+#include "select.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -174,6 +175,14 @@ __global__ void k_exists(GView g, const uint64_t *keys, uint64_t n, uint32_t *ex
   GRID_STRIDE(i, n) ex[i] = edge_exists(g, key_src(keys[i]), key_dst(keys[i])) ? 1u : 0u;
 }
 
+__global__ void k_exists2(GView g, const uint64_t *keys, uint64_t n, uint32_t *ex, uint32_t *nex) {
+  GRID_STRIDE(i, n) {
+    bool e = edge_exists(g, key_src(keys[i]), key_dst(keys[i]));
+    ex[i] = e ? 1u : 0u;
+    nex[i] = e ? 0u : 1u;
+  }
+}
+
 __global__ void k_not(const uint32_t *a, uint64_t n, uint32_t *out) {
   GRID_STRIDE(i, n) out[i] = a[i] ? 0u : 1u;
 }
@@ -251,12 +260,19 @@ __global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, c
 // new keys beyond the first c of the run, merged with the old inline
 // entries that were pushed out.
 __global__ void k_place_kicked(const uint64_t *R, const uint32_t *ex, const uint32_t *nidx, uint64_t nR,
-                               const uint32_t *run_beg, const uint32_t *run_of, const uint8_t *run_route,
+                               const uint32_t *run_beg, uint32_t nruns, const uint8_t *run_route,
                                const uint32_t *run_c, const uint32_t *run_nko, const uint32_t *kick_old,
                                uint64_t *kick, uint8_t *kroute) {
   GRID_STRIDE(i, nR) {
     if (ex[i]) continue;
-    uint32_t r = run_of[i];
+    // run of i: last r with run_beg[r] <= i
+    uint32_t lo = 0, hi = nruns;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (run_beg[mid] <= i) lo = mid + 1;
+      else hi = mid;
+    }
+    uint32_t r = lo - 1;
     uint8_t route = run_route[r];
     if (route == kRouteNone) continue;
     uint32_t b0 = run_beg[r];
@@ -322,8 +338,9 @@ __global__ void k_touched(const uint32_t *leaf, const uint32_t *flags, const uin
   if (blockIdx.x == 0 && threadIdx.x == 0) Tstart[nt] = static_cast<uint32_t>(k);
 }
 
-__global__ void k_climb(GView g, Bounds bd, const uint32_t *T, const uint32_t *Tstart, uint32_t nt,
+__global__ void k_climb(GView g, Bounds bd, const uint32_t *T, const uint32_t *Tstart, const uint64_t *ntp,
                         uint8_t *mark, uint32_t *chosen) {
+  const uint32_t nt = static_cast<uint32_t>(*ntp);
   GRID_STRIDE(t, nt) {
     uint32_t f = T[t];
     uint32_t L = g.H;
@@ -344,8 +361,9 @@ __global__ void k_climb(GView g, Bounds bd, const uint32_t *T, const uint32_t *T
   }
 }
 
-__global__ void k_cover(GView g, const uint32_t *T, uint32_t nt, const uint8_t *mark, const uint32_t *chosen,
+__global__ void k_cover(GView g, const uint32_t *T, const uint64_t *ntp, const uint8_t *mark, const uint32_t *chosen,
                         uint32_t *cover) {
+  const uint32_t nt = static_cast<uint32_t>(*ntp);
   GRID_STRIDE(t, nt) {
     uint32_t f = T[t];
     uint32_t L = chosen[t];
@@ -359,13 +377,6 @@ __global__ void k_cover(GView g, const uint32_t *T, uint32_t nt, const uint8_t *
   }
 }
 
-__global__ void k_owner(const uint32_t *T, uint32_t nt, const uint32_t *cover, uint32_t *flags) {
-  GRID_STRIDE(t, nt) {
-    uint32_t L = cover[t];
-    flags[t] = (t == 0 || (static_cast<uint64_t>(T[t - 1]) >> L) != (static_cast<uint64_t>(T[t]) >> L)) ? 1u : 0u;
-  }
-}
-
 // Window record: lo leaf, level, key range, old element count.
 struct Win {
   uint64_t lo;
@@ -374,10 +385,84 @@ struct Win {
   uint32_t old;
 };
 
-__global__ void k_make_windows(GView g, const uint32_t *T, const uint32_t *Tstart, uint32_t nt,
-                               const uint32_t *cover, const uint32_t *flags, const uint32_t *pos, Win *win,
-                               uint32_t *win_leaves) {
-  GRID_STRIDE(t, nt) if (flags[t]) {
+__global__ void k_clear_marks(GView g, const uint32_t *T, const uint64_t *ntp, const uint32_t *chosen, uint8_t *mark) {
+  const uint32_t nt = static_cast<uint32_t>(*ntp);
+  GRID_STRIDE(t, nt) {
+    uint32_t L = chosen[t];
+    mark[tree_level_off(g.H, L) + (T[t] >> L)] = 0;
+  }
+}
+
+// Tiny helpers that move device counts around without a host round trip.
+__global__ void k_put_u32(uint32_t *p, const uint64_t *idx, uint32_t v) { p[*idx] = v; }
+__global__ void k_u32_to_u64(const uint32_t *a, uint64_t *b) { *b = *a; }
+__global__ void k_put_u32_at(uint32_t *p, uint64_t i, uint32_t v) { p[i] = v; }
+
+// ---- selectors (select.hpp): predicate + emit, no flag arrays
+struct SelUnique {
+  const uint64_t *k;
+  uint64_t *out;
+  __device__ bool pred(uint64_t i) const { return i == 0 || k[i] != k[i - 1]; }
+  __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = k[i]; }
+};
+struct SelRunStart {
+  const uint64_t *k;
+  uint32_t *beg;
+  __device__ bool pred(uint64_t i) const { return i == 0 || key_src(k[i]) != key_src(k[i - 1]); }
+  __device__ void emit(uint64_t i, uint64_t pos) const { beg[pos] = static_cast<uint32_t>(i); }
+};
+struct SelRoute {
+  const uint64_t *k;
+  const uint8_t *route;
+  uint8_t want;
+  uint64_t *out;
+  __device__ bool pred(uint64_t i) const { return route[i] == want; }
+  __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = k[i]; }
+};
+struct SelExisting {
+  GView g;
+  const uint64_t *k;
+  uint64_t *out;
+  __device__ bool pred(uint64_t i) const { return edge_exists(g, key_src(k[i]), key_dst(k[i])); }
+  __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = k[i]; }
+};
+struct SelTouched {
+  const uint32_t *leaf;
+  uint32_t *T, *Tstart;
+  __device__ bool pred(uint64_t i) const { return i == 0 || leaf[i] != leaf[i - 1]; }
+  __device__ void emit(uint64_t i, uint64_t pos) const {
+    T[pos] = leaf[i];
+    Tstart[pos] = static_cast<uint32_t>(i);
+  }
+};
+// last op of each key (keys sorted, arrival order kept inside a key)
+struct SelLastOp {
+  const uint64_t *k;
+  const uint32_t *ops;
+  uint64_t n;
+  uint32_t want;
+  uint64_t *out;
+  __device__ bool pred(uint64_t i) const {
+    bool last = (i + 1 == n) || k[i] != k[i + 1];
+    return last && ((ops[i] != 0) == (want != 0));
+  }
+  __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = k[i]; }
+};
+// Window owner: the first touched leaf of each covering window; emits the
+// window record and its leaf count.
+struct SelOwner {
+  GView g;
+  const uint32_t *T, *Tstart, *cover;
+  const uint64_t *ntp;
+  Win *win;
+  uint32_t *win_leaves;
+  __device__ bool pred(uint64_t t) const {
+    if (t >= *ntp) return false;
+    uint32_t L = cover[t];
+    return t == 0 || (static_cast<uint64_t>(T[t - 1]) >> L) != (static_cast<uint64_t>(T[t]) >> L);
+  }
+  __device__ void emit(uint64_t t, uint64_t pos) const {
+    const uint32_t nt = static_cast<uint32_t>(*ntp);
     uint32_t L = cover[t];
     uint64_t lo = (static_cast<uint64_t>(T[t]) >> L) << L;
     uint64_t hi = lo + (1ull << L);
@@ -388,17 +473,10 @@ __global__ void k_make_windows(GView g, const uint32_t *T, const uint32_t *Tstar
     w.kb = Tstart[t];
     w.ke = Tstart[b];
     w.old = g.pcnt[tree_level_off(g.H, L) + (lo >> L)];
-    win[pos[t]] = w;
-    win_leaves[pos[t]] = 1u << L;
+    win[pos] = w;
+    win_leaves[pos] = 1u << L;
   }
-}
-
-__global__ void k_clear_marks(GView g, const uint32_t *T, uint32_t nt, const uint32_t *chosen, uint8_t *mark) {
-  GRID_STRIDE(t, nt) {
-    uint32_t L = chosen[t];
-    mark[tree_level_off(g.H, L) + (T[t] >> L)] = 0;
-  }
-}
+};
 
 // Window of leaf task j.  tw (task -> window map) is filled by
 // k_task_map; without it, a binary search over task_base.
@@ -990,6 +1068,30 @@ __global__ void k_flag_tiers(GView g, uint32_t *fp, uint32_t *ft) {
   }
 }
 
+// [0] PMA-tier vertices, [1] their secondary edges, [2] B-tree vertices,
+// [3] their secondary edges (device counters, zeroed by the caller)
+__global__ void k_tier_counts(GView g, unsigned long long *out) {
+  unsigned long long c[4] = {0, 0, 0, 0};
+  GRID_STRIDE(v, g.n) {
+    const VBlock &b = g.vb[v];
+    bool t = (b.aux & kAuxTree) != 0;
+    uint32_t sec = b.degree > kInline ? b.degree - kInline : 0;
+    if (t) {
+      c[2]++;
+      c[3] += sec;
+    } else if (b.degree > kInline) {
+      c[0]++;
+      c[1] += sec;
+    }
+  }
+  for (int j = 0; j < 4; j++) {
+    unsigned long long x = c[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(out + j, x);
+  }
+}
+
 __global__ void k_scatter_ids(const uint32_t *flags, const uint32_t *pos, uint64_t n, uint32_t *out) {
   GRID_STRIDE(i, n) if (flags[i]) out[pos[i]] = static_cast<uint32_t>(i);
 }
@@ -1111,6 +1213,8 @@ Graph::Graph(uint32_t n) : n_(n) {
   k_init_aux<<<grid_for(n, 256), 256, 0, stream_>>>(vb_.get(), n);
   TG_CHECK_LAUNCH();
   scal_.ensure(8, stream_);
+  dsc_.ensure(kScalars, stream_);
+  fill_u64(dsc_.get(), 0, kScalars, stream_);
   init_pma(2);
   ensure_trees(16);
   TG_HIP(hipStreamSynchronize(stream_));
@@ -1247,19 +1351,27 @@ uint64_t Graph::compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t
 uint64_t Graph::prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t count, uint64_t *&keys) {
   k0_.ensure(count, stream_);
   k1_.ensure(count, stream_);
-  f0_.ensure(count + 1, stream_);
-  fill_u64(scal_.get(), 0, 1, stream_);
-  k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), scal_.get());
+  fill_u64(dsc_.get(), 0, kScalars, stream_);
+  k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
   TG_CHECK_LAUNCH();
-  if (read_scalar(scal_.get(), stream_)) throw Error(-1, "vertex id out of range");
   uint32_t bits = 1;
   while (bits < 32 && (1ull << bits) < n_) bits++;
   radix_sort_keys(k0_.get(), count, bits, bits, sort_, stream_);
-  k_flag_unique<<<grid_for(count, 256), 256, 0, stream_>>>(k0_.get(), count, f0_.get());
-  TG_CHECK_LAUNCH();
-  uint64_t m = compact_keys(k0_.get(), f0_.get(), count, k1_.get());
+  select(SelUnique{k0_.get(), k1_.get()}, count, dsc_.get() + 1, scan_scratch, stream_);
+  uint64_t h[2];
+  read_scalars(dsc_.get(), h, 2, stream_);
+  if (h[0] & 1) throw Error(-1, "vertex id out of range");
   keys = k1_.get();
-  return m;
+  return h[1];
+}
+
+// Internal consistency flags raised by kernels during the batch.
+void Graph::check_batch_flags() {
+  uint64_t f = 0;
+  read_scalars(dsc_.get(), &f, 1, stream_);
+  if (f) fill_u64(dsc_.get(), 0, 1, stream_);  // leave the flags clear for the next batch
+  if (f & 2) throw Error(-4, "PMA count tree out of sync (gather)");
+  if (f & 4) throw Error(-4, "PMA delete: key not found");
 }
 
 // ------------------------------------------------------------ PMA insert
@@ -1335,44 +1447,43 @@ void Graph::pma_insert(const uint64_t *keys, uint64_t k) {
     pma_rebuild(keys, k, total);
     return;
   }
+  // touched leaves and window selection; counts stay on the device and the
+  // kernels are sized by the upper bound k
+  DevBuf<uint32_t> &T = f1_, &Tstart = f2_, &chosen = f3_;
   u0_.ensure(k + 1, stream_);  // leaf of key
-  u1_.ensure(k + 1, stream_);  // flags
-  u2_.ensure(k + 1, stream_);  // positions
+  u3_.ensure(k + 1, stream_);  // cover, then task -> window map
+  T.ensure(k + 1, stream_);
+  Tstart.ensure(k + 1, stream_);
+  chosen.ensure(k + 1, stream_);
+  uint64_t *ntp = dsc_.get() + 8, *nwp = dsc_.get() + 9;
   k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get());
   TG_CHECK_LAUNCH();
-  k_flag_change_u32<<<grid_for(k, 256), 256, 0, stream_>>>(u0_.get(), k, u1_.get());
+  select(SelTouched{u0_.get(), T.get(), Tstart.get()}, k, ntp, scan_scratch, stream_);
+  k_put_u32<<<1, 1, 0, stream_>>>(Tstart.get(), ntp, static_cast<uint32_t>(k));
   TG_CHECK_LAUNCH();
-  exclusive_scan_u32(u1_.get(), u2_.get(), k, u2_.get() + k, scan_scratch, stream_);
-  uint32_t nt = read_scalar(u2_.get() + k, stream_);
-  DevBuf<uint32_t> &T = f1_, &Tstart = f2_, &chosen = f3_;
-  T.ensure(nt + 1, stream_);
-  Tstart.ensure(nt + 1, stream_);
-  chosen.ensure(nt + 1, stream_);
-  k_touched<<<grid_for(k, 256), 256, 0, stream_>>>(u0_.get(), u1_.get(), u2_.get(), k, T.get(), Tstart.get(), nt);
+  k_climb<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_, T.get(), Tstart.get(), ntp, mark_.get(), chosen.get());
   TG_CHECK_LAUNCH();
-  k_climb<<<grid_for(nt, 256), 256, 0, stream_>>>(g, bounds_, T.get(), Tstart.get(), nt, mark_.get(), chosen.get());
+  k_cover<<<grid_for(k, 256), 256, 0, stream_>>>(g, T.get(), ntp, mark_.get(), chosen.get(), u3_.get());
   TG_CHECK_LAUNCH();
-  u3_.ensure(nt + 1, stream_);  // cover
-  k_cover<<<grid_for(nt, 256), 256, 0, stream_>>>(g, T.get(), nt, mark_.get(), chosen.get(), u3_.get());
-  TG_CHECK_LAUNCH();
-  // owners -> windows (reuse u1_/u2_ for flags/positions over touched)
-  k_owner<<<grid_for(nt, 256), 256, 0, stream_>>>(T.get(), nt, u3_.get(), u1_.get());
-  TG_CHECK_LAUNCH();
-  exclusive_scan_u32(u1_.get(), u2_.get(), nt, u2_.get() + nt, scan_scratch, stream_);
-  uint32_t nw = read_scalar(u2_.get() + nt, stream_);
   DevBuf<uint8_t> &winbuf = route_;
-  winbuf.ensure(static_cast<uint64_t>(nw) * sizeof(Win) + 64, stream_);
+  winbuf.ensure((k + 1) * sizeof(Win) + 64, stream_);
   Win *win = reinterpret_cast<Win *>(winbuf.get());
-  kick_.ensure(nw + 1, stream_);  // window leaf counts -> task_base
-  k_make_windows<<<grid_for(nt, 256), 256, 0, stream_>>>(g, T.get(), Tstart.get(), nt, u3_.get(), u1_.get(),
-                                                         u2_.get(), win, kick_.get());
-  TG_CHECK_LAUNCH();
-  k_clear_marks<<<grid_for(nt, 256), 256, 0, stream_>>>(g, T.get(), nt, chosen.get(), mark_.get());
+  kick_.ensure(k + 2, stream_);  // window leaf counts -> task_base (zeros past nw)
+  fill_u32(kick_.get(), 0, k + 1, stream_);
+  select(SelOwner{g, T.get(), Tstart.get(), u3_.get(), ntp, win, kick_.get()}, k, nwp, scan_scratch, stream_);
+  k_clear_marks<<<grid_for(k, 256), 256, 0, stream_>>>(g, T.get(), ntp, chosen.get(), mark_.get());
   TG_CHECK_LAUNCH();
   uint32_t *task_base = kick_.get();
-  exclusive_scan_u32(task_base, task_base, nw, task_base + nw, scan_scratch, stream_);
-  uint64_t ntasks = read_scalar(task_base + nw, stream_);
-  if (ntasks > nl) throw Error(-3, "PMA window tasks exceed leaves");
+  // scanning k entries (zeros past nw) keeps nw on the device: task_base[w]
+  // for w >= nw equals the task total, so window searches over k are exact
+  exclusive_scan_u32(task_base, task_base, k, task_base + k, scan_scratch, stream_);
+  k_u32_to_u64<<<1, 1, 0, stream_>>>(task_base + k, dsc_.get() + 10);
+  TG_CHECK_LAUNCH();
+  uint64_t h[3];
+  read_scalars(dsc_.get() + 8, h, 3, stream_);  // nt, nw, ntasks
+  uint32_t nw = static_cast<uint32_t>(h[1]);
+  uint64_t ntasks = h[2];
+  if (ntasks > nl) throw Error(-4, "PMA window tasks exceed leaves");
   u4_.ensure(ntasks + 1, stream_);
   u3_.ensure(ntasks + 1, stream_);
   uint32_t *tw = u3_.get();  // cover[] is no longer needed
@@ -1381,17 +1492,14 @@ void Graph::pma_insert(const uint64_t *keys, uint64_t k) {
   k_task_counts<<<grid_for(ntasks, 256), 256, 0, stream_>>>(g, win, task_base, nw, ntasks, u4_.get(), tw);
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(u4_.get(), u4_.get(), ntasks, u4_.get() + ntasks, scan_scratch, stream_);
-  uint64_t nold = read_scalar(u4_.get() + ntasks, stream_);
-  tmp_old_.ensure(nold + 1, stream_);
-  fill_u64(scal_.get(), 0, 1, stream_);
+  tmp_old_.ensure(root_old + 1, stream_);  // the windows hold at most every element
   k_gather<<<grid_for(ntasks * 64, 256), 256, 0, stream_>>>(g, win, task_base, nw, ntasks, u4_.get(),
-                                                            tmp_old_.get(), scal_.get(), tw);
+                                                            tmp_old_.get(), dsc_.get(), tw);
   TG_CHECK_LAUNCH();
   k_fill<<<grid_for(ntasks * 64, 256), 256, 0, stream_>>>(g, win, task_base, nw, ntasks, u4_.get(), tmp_old_.get(),
                                                           keys, tw);
   TG_CHECK_LAUNCH();
   tree_dirty_ = true;
-  if (read_scalar(scal_.get(), stream_)) throw Error(-3, "PMA count tree out of sync (gather)");
   stats_.pma_windows += nw;
   stats_.pma_window_leaves += ntasks;
 }
@@ -1400,13 +1508,11 @@ void Graph::pma_delete(const uint64_t *keys, uint64_t k) {
   if (k == 0) return;
   GView g = view();
   g2_.ensure(k, stream_);
-  fill_u64(scal_.get(), 0, 1, stream_);
-  k_pma_delete<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g2_.get(), scal_.get());
+  k_pma_delete<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g2_.get(), dsc_.get());
   TG_CHECK_LAUNCH();
   tree_dirty_ = true;
   k_pma_fix_begin<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g2_.get());
   TG_CHECK_LAUNCH();
-  if (read_scalar(scal_.get(), stream_)) throw Error(-3, "PMA delete: key not found");
 }
 
 // ------------------------------------------------------------ tree update
@@ -1575,30 +1681,25 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
 // Insert sorted unique keys (R must not alias the batch scratch buffers).
 void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
   if (nR == 0) return;
+  if (read_only_) throw Error(-1, "graph is a query-only snapshot");
   lists_dirty_ = true;
   GView g = view();
   // existence + new-key prefix
   f0_.ensure(nR + 1, stream_);
   f1_.ensure(nR + 1, stream_);
-  f2_.ensure(nR + 1, stream_);
-  f3_.ensure(nR + 1, stream_);
   uint32_t *ex = f0_.get(), *nidx = f1_.get();
-  k_exists<<<grid_for(nR, 256), 256, 0, stream_>>>(g, R, nR, ex);
-  TG_CHECK_LAUNCH();
-  k_not<<<grid_for(nR, 256), 256, 0, stream_>>>(ex, nR, nidx);
+  k_exists2<<<grid_for(nR, 256), 256, 0, stream_>>>(g, R, nR, ex, nidx);
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(nidx, nidx, nR, nidx + nR, scan_scratch, stream_);
-  // runs
-  uint32_t *rflag = f2_.get(), *rpos = f3_.get();
-  k_flag_runs<<<grid_for(nR, 256), 256, 0, stream_>>>(R, nR, rflag);
-  TG_CHECK_LAUNCH();
-  exclusive_scan_u32(rflag, rpos, nR, rpos + nR, scan_scratch, stream_);
-  uint32_t nruns = read_scalar(rpos + nR, stream_);
-  DevBuf<uint32_t> &run_beg = s32_[0], &run_of = s32_[1], &run_c = s32_[2], &run_nko = s32_[3],
-                   &kick_old = s32_[4], &promote = s32_[5], &ppos = s32_[6];
+  // source runs
+  DevBuf<uint32_t> &run_beg = s32_[0], &run_c = s32_[2], &run_nko = s32_[3], &kick_old = s32_[4],
+                   &promote = s32_[5], &ppos = s32_[6];
   DevBuf<uint8_t> &run_route = s8_[3], &kroute = s8_[4];
-  run_beg.ensure(nruns + 1, stream_);
-  run_of.ensure(nR + 1, stream_);
+  run_beg.ensure(nR + 1, stream_);
+  select(SelRunStart{R, run_beg.get()}, nR, dsc_.get() + 2, scan_scratch, stream_);
+  k_put_u32<<<1, 1, 0, stream_>>>(run_beg.get(), dsc_.get() + 2, static_cast<uint32_t>(nR));
+  TG_CHECK_LAUNCH();
+  uint32_t nruns = static_cast<uint32_t>(read_scalar(dsc_.get() + 2, stream_));
   run_c.ensure(nruns + 1, stream_);
   run_nko.ensure(nruns + 1, stream_);
   kick_old.ensure(static_cast<uint64_t>(nruns) * kInline + 1, stream_);
@@ -1606,10 +1707,6 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
   ppos.ensure(nruns + 1, stream_);
   run_route.ensure(nruns + 1, stream_);
   kroute.ensure(nR + 1, stream_);
-  k_run_starts<<<grid_for(nR, 256), 256, 0, stream_>>>(rflag, rpos, nR, run_beg.get(), nruns);
-  TG_CHECK_LAUNCH();
-  k_run_of<<<grid_for(nR, 256), 256, 0, stream_>>>(rflag, rpos, nR, run_of.get());
-  TG_CHECK_LAUNCH();
   // the old degrees of promoted vertices are needed after the merge
   DevBuf<uint32_t> &old_deg = s32_[7];
   old_deg.ensure(nruns + 1, stream_);
@@ -1623,17 +1720,27 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
   k2_.ensure(nR + 1, stream_);
   fill_u64(k2_.get(), kInvalidKey, nR, stream_);
   TG_HIP(hipMemsetAsync(kroute.get(), 0, nR, stream_));
-  k_place_kicked<<<grid_for(nR, 256), 256, 0, stream_>>>(R, ex, nidx, nR, run_beg.get(), run_of.get(),
-                                                         run_route.get(), run_c.get(), run_nko.get(), kick_old.get(),
-                                                         k2_.get(), kroute.get());
+  k_place_kicked<<<grid_for(nR, 256), 256, 0, stream_>>>(R, ex, nidx, nR, run_beg.get(), nruns, run_route.get(),
+                                                         run_c.get(), run_nko.get(), kick_old.get(), k2_.get(),
+                                                         kroute.get());
   TG_CHECK_LAUNCH();
   k_place_kicked_old<<<grid_for(nruns, 256), 256, 0, stream_>>>(R, nidx, run_beg.get(), nruns, run_route.get(),
                                                                 run_c.get(), run_nko.get(), kick_old.get(), k2_.get(),
                                                                 kroute.get());
   TG_CHECK_LAUNCH();
-  // promotions: new trees for vertices routed to the B-tier the first time
+  // split kicked keys by tier and count promotions: one host read
+  k3_.ensure(nR + 1, stream_);
+  g0_.ensure(nR + 1, stream_);
+  select(SelRoute{k2_.get(), kroute.get(), kRoutePma, k3_.get()}, nR, dsc_.get() + 3, scan_scratch, stream_);
+  select(SelRoute{k2_.get(), kroute.get(), kRouteTree, g0_.get()}, nR, dsc_.get() + 4, scan_scratch, stream_);
   exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_);
-  uint32_t np = read_scalar(ppos.get() + nruns, stream_);
+  k_u32_to_u64<<<1, 1, 0, stream_>>>(ppos.get() + nruns, dsc_.get() + 5);
+  TG_CHECK_LAUNCH();
+  uint64_t h[3];
+  read_scalars(dsc_.get() + 3, h, 3, stream_);
+  uint64_t npma = h[0], ntree = h[1];
+  uint32_t np = static_cast<uint32_t>(h[2]);
+  // promotions: new trees for vertices routed to the B-tier the first time
   uint64_t next = 0;
   DevBuf<uint64_t> &ext = s64_[0];
   if (np) {
@@ -1666,17 +1773,8 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
     ntrees_ += np;
     stats_.promotions += np;
   }
-  // split kicked keys by tier
-  f3_.ensure(nR + 1, stream_);
-  k_flag_route<<<grid_for(nR, 256), 256, 0, stream_>>>(kroute.get(), nR, kRoutePma, f3_.get());
-  TG_CHECK_LAUNCH();
-  k3_.ensure(nR + 1, stream_);
-  uint64_t npma = compact_keys(k2_.get(), f3_.get(), nR, k3_.get());
-  k_flag_route<<<grid_for(nR, 256), 256, 0, stream_>>>(kroute.get(), nR, kRouteTree, f3_.get());
-  TG_CHECK_LAUNCH();
-  g0_.ensure(nR + next + 1, stream_);
-  uint64_t ntree = compact_keys(k2_.get(), f3_.get(), nR, g0_.get());
   if (next) {
+    g0_.ensure(ntree + next + 1, stream_, true, ntree);
     TG_HIP(hipMemcpyAsync(g0_.get() + ntree, ext.get(), next * sizeof(uint64_t), hipMemcpyDeviceToDevice, stream_));
     ntree += next;
     uint32_t bits = 1;
@@ -1685,7 +1783,7 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
   }
   tree_insert(g0_.get(), ntree);
   pma_insert(k3_.get(), npma);
-  TG_HIP(hipStreamSynchronize(stream_));
+  check_batch_flags();
 }
 
 void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
@@ -1699,47 +1797,42 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
 // Delete sorted unique keys (absent keys are ignored).
 void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
   if (nR0 == 0) return;
+  if (read_only_) throw Error(-1, "graph is a query-only snapshot");
   lists_dirty_ = true;
   GView g = view();
-  f0_.ensure(nR0 + 1, stream_);
-  k_exists<<<grid_for(nR0, 256), 256, 0, stream_>>>(g, R0, nR0, f0_.get());
-  TG_CHECK_LAUNCH();
+  // keep the keys that exist (absent deletes are no-ops)
   k2_.ensure(nR0 + 1, stream_);
-  uint64_t nR = compact_keys(R0, f0_.get(), nR0, k2_.get());
+  select(SelExisting{g, R0, k2_.get()}, nR0, dsc_.get() + 2, scan_scratch, stream_);
+  uint64_t nR = read_scalar(dsc_.get() + 2, stream_);
   if (nR == 0) return;
   uint64_t *R = k2_.get();
-  f2_.ensure(nR + 1, stream_);
-  f3_.ensure(nR + 1, stream_);
-  k_flag_runs<<<grid_for(nR, 256), 256, 0, stream_>>>(R, nR, f2_.get());
-  TG_CHECK_LAUNCH();
-  exclusive_scan_u32(f2_.get(), f3_.get(), nR, f3_.get() + nR, scan_scratch, stream_);
-  uint32_t nruns = read_scalar(f3_.get() + nR, stream_);
   DevBuf<uint32_t> &run_beg = s32_[0], &run_bump = s32_[12], &bump_off = s32_[13];
   DevBuf<uint8_t> &sec_route = s8_[5];
-  run_beg.ensure(nruns + 1, stream_);
+  run_beg.ensure(nR + 1, stream_);
+  select(SelRunStart{R, run_beg.get()}, nR, dsc_.get() + 3, scan_scratch, stream_);
+  k_put_u32<<<1, 1, 0, stream_>>>(run_beg.get(), dsc_.get() + 3, static_cast<uint32_t>(nR));
+  TG_CHECK_LAUNCH();
+  uint32_t nruns = static_cast<uint32_t>(read_scalar(dsc_.get() + 3, stream_));
   run_bump.ensure(nruns + 1, stream_);
   bump_off.ensure(nruns + 1, stream_);
   sec_route.ensure(nR + 1, stream_);
-  k_run_starts<<<grid_for(nR, 256), 256, 0, stream_>>>(f2_.get(), f3_.get(), nR, run_beg.get(), nruns);
-  TG_CHECK_LAUNCH();
   k_delete_runs<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, sec_route.get(),
                                                            run_bump.get());
   TG_CHECK_LAUNCH();
-  // secondary deletes by tier
-  f0_.ensure(nR + 1, stream_);
+  // secondary deletes by tier, and how many inline slots to refill
   k3_.ensure(nR + 1, stream_);
   g0_.ensure(nR + 1, stream_);
-  k_flag_route<<<grid_for(nR, 256), 256, 0, stream_>>>(sec_route.get(), nR, kRoutePma, f0_.get());
+  select(SelRoute{R, sec_route.get(), kRoutePma, k3_.get()}, nR, dsc_.get() + 4, scan_scratch, stream_);
+  select(SelRoute{R, sec_route.get(), kRouteTree, g0_.get()}, nR, dsc_.get() + 5, scan_scratch, stream_);
+  exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_);
+  k_u32_to_u64<<<1, 1, 0, stream_>>>(bump_off.get() + nruns, dsc_.get() + 6);
   TG_CHECK_LAUNCH();
-  uint64_t npma = compact_keys(R, f0_.get(), nR, k3_.get());
-  k_flag_route<<<grid_for(nR, 256), 256, 0, stream_>>>(sec_route.get(), nR, kRouteTree, f0_.get());
-  TG_CHECK_LAUNCH();
-  uint64_t ntree = compact_keys(R, f0_.get(), nR, g0_.get());
+  uint64_t h[3];
+  read_scalars(dsc_.get() + 4, h, 3, stream_);
+  uint64_t npma = h[0], ntree = h[1], nb = h[2];
   pma_delete(k3_.get(), npma);
   tree_delete(g0_.get(), ntree);
   // refill inline slots from the secondary tier
-  exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_);
-  uint64_t nb = read_scalar(bump_off.get() + nruns, stream_);
   if (nb) {
     DevBuf<uint64_t> &bkeys = s64_[1];
     DevBuf<uint8_t> &broute = s8_[6];
@@ -1749,19 +1842,15 @@ void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
     k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, run_bump.get(), bump_off.get(),
                                                       bkeys.get(), broute.get());
     TG_CHECK_LAUNCH();
-    f0_.ensure(nb + 1, stream_);
     k3_.ensure(nb + 1, stream_);
     g0_.ensure(nb + 1, stream_);
-    k_flag_route<<<grid_for(nb, 256), 256, 0, stream_>>>(broute.get(), nb, kRoutePma, f0_.get());
-    TG_CHECK_LAUNCH();
-    uint64_t bp = compact_keys(bkeys.get(), f0_.get(), nb, k3_.get());
-    k_flag_route<<<grid_for(nb, 256), 256, 0, stream_>>>(broute.get(), nb, kRouteTree, f0_.get());
-    TG_CHECK_LAUNCH();
-    uint64_t bt = compact_keys(bkeys.get(), f0_.get(), nb, g0_.get());
-    pma_delete(k3_.get(), bp);
-    tree_delete(g0_.get(), bt);
+    select(SelRoute{bkeys.get(), broute.get(), kRoutePma, k3_.get()}, nb, dsc_.get() + 4, scan_scratch, stream_);
+    select(SelRoute{bkeys.get(), broute.get(), kRouteTree, g0_.get()}, nb, dsc_.get() + 5, scan_scratch, stream_);
+    read_scalars(dsc_.get() + 4, h, 2, stream_);
+    pma_delete(k3_.get(), h[0]);
+    tree_delete(g0_.get(), h[1]);
   }
-  TG_HIP(hipStreamSynchronize(stream_));
+  check_batch_flags();
 }
 
 // ------------------------------------------------------------ coalesced ops
@@ -1782,25 +1871,25 @@ void Graph::apply_ops(const uint32_t *src, const uint32_t *dst, const uint32_t *
   if (count == 0) return;
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
   k0_.ensure(count, stream_);
-  fill_u64(scal_.get(), 0, 1, stream_);
-  k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), scal_.get());
+  fill_u64(dsc_.get(), 0, kScalars, stream_);
+  k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
   TG_CHECK_LAUNCH();
-  if (read_scalar(scal_.get(), stream_)) throw Error(-1, "vertex id out of range");
   ops_val_.ensure(count, stream_);
   TG_HIP(hipMemcpyAsync(ops_val_.get(), ops, count * 4, hipMemcpyDeviceToDevice, stream_));
   uint32_t bits = 1;
   while (bits < 32 && (1ull << bits) < n_) bits++;
   radix_sort_pairs(k0_.get(), ops_val_.get(), count, bits, bits, sort_, stream_);
-  f0_.ensure(count + 1, stream_);
-  f1_.ensure(count + 1, stream_);
-  k_last_op_flags<<<grid_for(count, 256), 256, 0, stream_>>>(k0_.get(), ops_val_.get(), count, f0_.get(), f1_.get());
-  TG_CHECK_LAUNCH();
   ops_ins_.ensure(count + 1, stream_);
   ops_del_.ensure(count + 1, stream_);
-  uint64_t ni = compact_keys(k0_.get(), f0_.get(), count, ops_ins_.get());
-  uint64_t nd = compact_keys(k0_.get(), f1_.get(), count, ops_del_.get());
-  insert_sorted(ops_ins_.get(), ni);
-  delete_sorted(ops_del_.get(), nd);
+  select(SelLastOp{k0_.get(), ops_val_.get(), count, 0, ops_ins_.get()}, count, dsc_.get() + 1, scan_scratch,
+         stream_);
+  select(SelLastOp{k0_.get(), ops_val_.get(), count, 1, ops_del_.get()}, count, dsc_.get() + 2, scan_scratch,
+         stream_);
+  uint64_t h[3];
+  read_scalars(dsc_.get(), h, 3, stream_);
+  if (h[0] & 1) throw Error(-1, "vertex id out of range");
+  insert_sorted(ops_ins_.get(), h[1]);
+  delete_sorted(ops_del_.get(), h[2]);
 }
 
 // ------------------------------------------------------------ queries
@@ -1826,6 +1915,14 @@ void Graph::is_edge_batch(const uint32_t *src, const uint32_t *dst, uint64_t cou
 void Graph::tiers(uint8_t *out) const {
   k_tiers<<<grid_for(n_, 256), 256, 0, stream_>>>(view(), out);
   TG_CHECK_LAUNCH();
+}
+
+void Graph::tier_counts(uint64_t out[4]) {
+  g1_.ensure(4, stream_);
+  fill_u64(g1_.get(), 0, 4, stream_);
+  k_tier_counts<<<grid_for(n_, 256, 4096), 256, 0, stream_>>>(view(), reinterpret_cast<unsigned long long *>(g1_.get()));
+  TG_CHECK_LAUNCH();
+  read_scalars(g1_.get(), out, 4, stream_);
 }
 
 const Graph::TierLists &Graph::tier_lists() {
@@ -1883,6 +1980,58 @@ void Graph::neighbors_of(uint32_t v, uint32_t *out) {
   k_neighbors_one<<<1, 64, 0, stream_>>>(view(), v, out);
   TG_CHECK_LAUNCH();
   TG_HIP(hipStreamSynchronize(stream_));
+}
+
+void Graph::copy_from(Graph &src, bool query_only) {
+  if (src.n_ != n_) throw Error(-1, "snapshot of a graph with another vertex count");
+  src.rebuild_count_tree();
+  hipStream_t s = src.stream_;
+  auto cp = [&](void *d, const void *p, uint64_t bytes) {
+    if (bytes) TG_HIP(hipMemcpyAsync(d, p, bytes, hipMemcpyDeviceToDevice, s));
+  };
+  TG_HIP(hipStreamSynchronize(stream_));
+  cp(vb_.get(), src.vb_.get(), sizeof(VBlock) * static_cast<uint64_t>(n_));
+  if (pma_N_ != src.pma_N_) {
+    pdst_.release();
+    psrc_.release();
+    pcnt_.release();
+    fence_.release();
+    mark_.release();
+    pma_H_ = src.pma_H_;
+    pma_N_ = src.pma_N_;
+    uint64_t nl = 1ull << pma_H_;
+    pdst_.ensure(pma_N_, s);
+    psrc_.ensure(pma_N_, s);
+    pcnt_.ensure(2 * nl, s);
+    fence_.ensure(nl, s);
+    mark_.ensure(2 * nl, s);
+    TG_HIP(hipMemsetAsync(mark_.get(), 0, 2 * nl, s));
+    bounds_ = src.bounds_;
+  }
+  uint64_t nl = 1ull << pma_H_;
+  cp(pdst_.get(), src.pdst_.get(), pma_N_ * 4);
+  if (!query_only) {
+    cp(psrc_.get(), src.psrc_.get(), pma_N_ * 4);
+    cp(pcnt_.get(), src.pcnt_.get(), 2 * nl * 4);
+    cp(fence_.get(), src.fence_.get(), nl * 8);
+  }
+  tree_dirty_ = false;
+  read_only_ = query_only;
+  ensure_blocks(src.blk_top_);
+  ensure_index(src.idx_top_);
+  ensure_trees(src.ntrees_ + 1);
+  cp(bkeys_.get(), src.bkeys_.get(), src.blk_top_ * kBlk * 4);
+  cp(bcnt_.get(), src.bcnt_.get(), src.blk_top_ * 4);
+  cp(iblk_.get(), src.iblk_.get(), src.idx_top_ * 4);
+  cp(imin_.get(), src.imin_.get(), src.idx_top_ * 4);
+  cp(toff_.get(), src.toff_.get(), src.ntrees_ * 8ull);
+  cp(tnblk_.get(), src.tnblk_.get(), src.ntrees_ * 4ull);
+  cp(tvtx_.get(), src.tvtx_.get(), src.ntrees_ * 4ull);
+  blk_top_ = src.blk_top_;
+  idx_top_ = src.idx_top_;
+  ntrees_ = src.ntrees_;
+  lists_dirty_ = true;
+  TG_HIP(hipStreamSynchronize(s));
 }
 
 uint64_t Graph::memory_bytes() const {

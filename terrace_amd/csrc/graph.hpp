@@ -87,10 +87,19 @@ class Graph {
   void neighbors_of(uint32_t v, uint32_t *out);
   // 0 = inline only, 1 = PMA, 2 = B-tree, per vertex (device u8[n]).
   void tiers(uint8_t *out) const;
+  // vertices and secondary edges per tier: PMA count, PMA edges, tree count, tree edges
+  void tier_counts(uint64_t out[4]);
   uint64_t memory_bytes() const;
   uint32_t num_trees() const { return ntrees_; }
   uint64_t pma_slots() const { return pma_N_; }
   const Stats &stats() const { return stats_; }
+
+  // Versioned snapshot: make this graph a read-only copy of src's current
+  // state (device-to-device copies ordered on src's stream).  Queries can
+  // then run on this copy while src takes the next update batch.
+  // query_only: skip what only updates read (PMA sources, count tree,
+  // fences); such a snapshot rejects updates and existence checks.
+  void copy_from(Graph &src, bool query_only = false);
 
   // Vertex lists by tier, rebuilt lazily after updates (device arrays).
   struct TierLists {
@@ -116,6 +125,7 @@ class Graph {
   DevBuf<uint8_t> mark_;
   Bounds bounds_{};
   bool tree_dirty_ = false;  // count tree levels above the leaves are stale
+  bool read_only_ = false;   // query-only snapshot
   // B-tree tier
   DevBuf<uint32_t> bkeys_, bcnt_;
   uint64_t blk_top_ = 0;
@@ -140,6 +150,9 @@ class Graph {
   DevBuf<uint32_t> kick_;
   DevBuf<uint8_t> route_;
   DevBuf<uint64_t> scal_;  // small device scalars
+  // batch scalars: [0] error bits, [1..] counts read back in one copy
+  static constexpr int kScalars = 16;
+  DevBuf<uint64_t> dsc_;
   DevBuf<uint64_t> tmp_old_;
   DevBuf<uint32_t> ops_val_;
   DevBuf<uint64_t> ops_ins_, ops_del_;
@@ -153,6 +166,7 @@ class Graph {
   void compute_bounds();
   uint64_t prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t count, uint64_t *&keys);
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
+  void check_batch_flags();
   void pma_insert(const uint64_t *keys, uint64_t k);
   void pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total);
   void rebuild_count_tree();

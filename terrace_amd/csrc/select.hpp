@@ -1,0 +1,108 @@
+// Predicate-fused stream compaction: F::pred(i) is evaluated inside the
+// count and the scatter kernels, so no flag array and no separate flag
+// kernel are needed.  out positions follow index order (stable).
+//
+//   struct F { __device__ bool pred(uint64_t i) const;
+//              __device__ void emit(uint64_t i, uint64_t pos) const; };
+//   select(f, n, total_dev, scratch, stream)   // *total_dev = count
+#pragma once
+
+#include "prims.hpp"
+
+namespace tg {
+
+namespace sel {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;  // 2048, items blocked per thread
+
+template <class F>
+__global__ void __launch_bounds__(kBlock) k_count(F f, uint64_t n, uint64_t *sums) {
+  __shared__ uint32_t lds[kBlock / 64];
+  uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; i++) {
+    uint64_t idx = base + static_cast<uint64_t>(i) * kBlock + threadIdx.x;
+    if (idx < n && f.pred(idx)) c++;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) sums[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
+// Items are visited in wave-strided order (coalesced); ranks come from
+// ballots, so the emitted order is index order within the tile.
+template <class F>
+__global__ void __launch_bounds__(kBlock) k_emit(F f, uint64_t n, const uint64_t *offs, uint64_t *total) {
+  __shared__ uint32_t wcnt[kItems][kBlock / 64];
+  __shared__ uint32_t wpre[kItems][kBlock / 64];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint64_t lt = lanemask_lt();
+  uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile;
+  bool p[kItems];
+  uint32_t r[kItems];
+#pragma unroll
+  for (int i = 0; i < kItems; i++) {
+    uint64_t idx = base + static_cast<uint64_t>(i) * kBlock + threadIdx.x;
+    p[i] = idx < n && f.pred(idx);
+    uint64_t m = __ballot(p[i]);
+    r[i] = __popcll(m & lt);
+    if ((threadIdx.x & 63) == 0) wcnt[i][w] = __popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int i = 0; i < kItems; i++)
+      for (int ww = 0; ww < kBlock / 64; ww++) {
+        wpre[i][ww] = run;
+        run += wcnt[i][ww];
+      }
+    if (total && blockIdx.x == gridDim.x - 1) *total = (offs ? offs[blockIdx.x] : 0) + run;
+  }
+  __syncthreads();
+  uint64_t boff = offs ? offs[blockIdx.x] : 0;
+#pragma unroll
+  for (int i = 0; i < kItems; i++) {
+    if (p[i]) {
+      uint64_t idx = base + static_cast<uint64_t>(i) * kBlock + threadIdx.x;
+      f.emit(idx, boff + wpre[i][w] + r[i]);
+    }
+  }
+}
+
+}  // namespace sel
+
+// Compacts the indices with f.pred into f.emit positions; *total_dev gets
+// the count (device scalar, not synchronised).
+template <class F>
+void select(const F &f, uint64_t n, uint64_t *total_dev, ScanScratch &ss, hipStream_t s) {
+  if (n == 0) {
+    fill_u64(total_dev, 0, 1, s);
+    return;
+  }
+  uint64_t nblocks = (n + sel::kTile - 1) / sel::kTile;
+  if (nblocks == 1) {
+    sel::k_emit<F><<<1, sel::kBlock, 0, s>>>(f, n, nullptr, total_dev);
+    TG_CHECK_LAUNCH();
+    return;
+  }
+  DevBuf<uint64_t> &sums = ss.levels[5];
+  sums.ensure(nblocks + 1, s);
+  sel::k_count<F><<<static_cast<unsigned>(nblocks), sel::kBlock, 0, s>>>(f, n, sums.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u64(sums.get(), sums.get(), nblocks, nullptr, ss, s);
+  sel::k_emit<F><<<static_cast<unsigned>(nblocks), sel::kBlock, 0, s>>>(f, n, sums.get(), total_dev);
+  TG_CHECK_LAUNCH();
+}
+
+// Several device scalars in one copy (synchronises the stream).
+inline void read_scalars(const uint64_t *dev, uint64_t *host, int count, hipStream_t s) {
+  TG_HIP(hipMemcpyAsync(host, dev, count * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  TG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace tg

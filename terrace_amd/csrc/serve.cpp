@@ -27,24 +27,54 @@ double pct(std::vector<double> v, double p) {
 
 Server::Server(Graph &g, bool batched, uint64_t mem_budget, bool keep_results)
     : g_(g), batched_(batched), budget_(mem_budget), keep_(keep_results) {
-  for (int i = 0; i < kLanes; i++) {
-    if (batched_) {
-      TG_HIP(hipStreamCreateWithFlags(&streams_[i], hipStreamNonBlocking));
-      trav_[i] = std::make_unique<Traversal>(g_, streams_[i]);
-    } else {
-      trav_[i] = std::make_unique<Traversal>(g_);
+  for (int i = 0; i < kLanes; i++) trav_[i] = std::make_unique<Traversal>(g_);
+  if (batched_) {
+    // snapshots in flight: bounded by the budget (each holds a full copy)
+    uint64_t per = std::max<uint64_t>(g_.memory_bytes(), 1);
+    size_t k = static_cast<size_t>(std::min<uint64_t>(6, std::max<uint64_t>(1, budget_ / (2 * per))));
+    for (size_t j = 0; j < k; j++) {
+      auto sl = std::make_unique<QuerySlot>();
+      sl->snap = std::make_unique<Graph>(g_.num_vertices());
+      for (int i = 0; i < kLanes; i++) {
+        TG_HIP(hipStreamCreateWithFlags(&sl->streams[i], hipStreamNonBlocking));
+        sl->trav[i] = std::make_unique<Traversal>(*sl->snap, sl->streams[i]);
+      }
+      slots_.push_back(std::move(sl));
     }
   }
 }
 
 Server::~Server() {
-  for (int i = 0; i < kLanes; i++) {
-    trav_[i].reset();
-    if (streams_[i]) {
-      (void)hipStreamSynchronize(streams_[i]);
-      (void)hipStreamDestroy(streams_[i]);
+  try {
+    drain();
+  } catch (...) {
+  }
+  for (auto &sl : slots_) {
+    for (int i = 0; i < kLanes; i++) {
+      sl->trav[i].reset();
+      if (sl->streams[i]) {
+        (void)hipStreamSynchronize(sl->streams[i]);
+        (void)hipStreamDestroy(sl->streams[i]);
+      }
     }
   }
+}
+
+QuerySlot &Server::acquire_slot() {
+  QuerySlot &sl = *slots_[next_slot_];
+  next_slot_ = (next_slot_ + 1) % slots_.size();
+  if (sl.worker.joinable()) sl.worker.join();
+  if (!sl.error.empty()) throw Error(-3, "query group: " + sl.error);
+  return sl;
+}
+
+void Server::drain() {
+  std::string err;
+  for (auto &sl : slots_) {
+    if (sl->worker.joinable()) sl->worker.join();
+    if (!sl->error.empty()) err = sl->error;
+  }
+  if (!err.empty()) throw Error(-3, "query group: " + err);
 }
 
 double Server::now() const { return wall() - t0_; }
@@ -194,7 +224,7 @@ void Server::exec_query_plain(Request &r) {
   r.group = groups_++;
 }
 
-void Server::exec_query_group(const std::vector<uint64_t> &ids) {
+void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64_t ver, uint64_t grp) {
   const uint32_t n = g_.num_vertices();
   std::vector<uint64_t> bfs, cc, pr, ltc;
   for (uint64_t id : ids) {
@@ -206,8 +236,7 @@ void Server::exec_query_group(const std::vector<uint64_t> &ids) {
       default: throw Error(-1, "unknown request type");
     }
   }
-  (void)g_.tier_lists();  // shared, read-only while the group runs
-  const uint64_t ver = version_, grp = groups_++;
+  (void)slot.snap->tier_lists();  // shared, read-only while the group runs
   double st = now();
   auto finish = [&](uint64_t id) {
     Request &r = reqs_[id];
@@ -222,17 +251,17 @@ void Server::exec_query_group(const std::vector<uint64_t> &ids) {
   if (!bfs.empty())
     th.emplace_back([&] {
       try {
-        Traversal &T = *trav_[0];
+        Traversal &T = *slot.trav[0];
         hipStream_t s = T.stream();
         for (size_t c0 = 0; c0 < bfs.size(); c0 += 64) {
           uint32_t k = static_cast<uint32_t>(std::min<size_t>(64, bfs.size() - c0));
           std::vector<uint32_t> srcs(k);
           for (uint32_t j = 0; j < k; j++) srcs[j] = reqs_[bfs[c0 + j]].src;
-          res32_[0].ensure(static_cast<uint64_t>(k) * n, s);
-          T.bfs_multi(srcs.data(), k, res32_[0].get());
+          slot.res32[0].ensure(static_cast<uint64_t>(k) * n, s);
+          T.bfs_multi(srcs.data(), k, slot.res32[0].get());
           for (uint32_t j = 0; j < k; j++) {
             Request &r = reqs_[bfs[c0 + j]];
-            const uint32_t *lv = res32_[0].get() + static_cast<uint64_t>(j) * n;
+            const uint32_t *lv = slot.res32[0].get() + static_cast<uint64_t>(j) * n;
             r.digest = T.digest(lv, nullptr, n);
             if (keep_) {
               r.r32.resize(n);
@@ -249,15 +278,15 @@ void Server::exec_query_group(const std::vector<uint64_t> &ids) {
   if (!cc.empty())
     th.emplace_back([&] {
       try {
-        Traversal &T = *trav_[1];
+        Traversal &T = *slot.trav[1];
         hipStream_t s = T.stream();
-        res32_[1].ensure(n, s);
-        T.cc(res32_[1].get());
-        uint64_t d = T.digest(res32_[1].get(), nullptr, n);
+        slot.res32[1].ensure(n, s);
+        T.cc(slot.res32[1].get());
+        uint64_t d = T.digest(slot.res32[1].get(), nullptr, n);
         std::vector<uint32_t> h;
         if (keep_) {
           h.resize(n);
-          TG_HIP(hipMemcpyAsync(h.data(), res32_[1].get(), n * 4ull, hipMemcpyDeviceToHost, s));
+          TG_HIP(hipMemcpyAsync(h.data(), slot.res32[1].get(), n * 4ull, hipMemcpyDeviceToHost, s));
           TG_HIP(hipStreamSynchronize(s));
         }
         for (uint64_t id : cc) {
@@ -272,16 +301,16 @@ void Server::exec_query_group(const std::vector<uint64_t> &ids) {
   if (!pr.empty())
     th.emplace_back([&] {
       try {
-        Traversal &T = *trav_[2];
+        Traversal &T = *slot.trav[2];
         hipStream_t s = T.stream();
         std::map<uint32_t, std::vector<uint64_t>> by_iters;
         for (uint64_t id : pr) by_iters[reqs_[id].iters].push_back(id);
-        res64_.ensure(n, s);
+        slot.res64.ensure(n, s);
         for (auto &kv : by_iters) {
-          T.pagerank(kv.first, res64_.get());
-          uint64_t d = T.digest(nullptr, reinterpret_cast<const uint64_t *>(res64_.get()), n);
+          T.pagerank(kv.first, slot.res64.get());
+          uint64_t d = T.digest(nullptr, reinterpret_cast<const uint64_t *>(slot.res64.get()), n);
           std::vector<double> h(n);
-          TG_HIP(hipMemcpyAsync(h.data(), res64_.get(), n * 8ull, hipMemcpyDeviceToHost, s));
+          TG_HIP(hipMemcpyAsync(h.data(), slot.res64.get(), n * 8ull, hipMemcpyDeviceToHost, s));
           TG_HIP(hipStreamSynchronize(s));
           double sum = 0;
           for (double x : h) sum += x;
@@ -299,7 +328,7 @@ void Server::exec_query_group(const std::vector<uint64_t> &ids) {
   if (!ltc.empty())
     th.emplace_back([&] {
       try {
-        Traversal &T = *trav_[3];
+        Traversal &T = *slot.trav[3];
         std::vector<uint32_t> vs(ltc.size());
         for (size_t i = 0; i < ltc.size(); i++) vs[i] = reqs_[ltc[i]].src;
         std::vector<uint64_t> out(ltc.size());
@@ -361,10 +390,24 @@ void Server::run() {
       ids.push_back(order[j]);
       j++;
     }
-    if (upd) exec_update_group(ids);
-    else exec_query_group(ids);
+    if (upd) {
+      exec_update_group(ids);
+    } else {
+      QuerySlot &sl = acquire_slot();
+      sl.snap->copy_from(g_, true);  // version_ as of now, on the live graph's stream
+      uint64_t ver = version_, grp = groups_++;
+      sl.error.clear();
+      sl.worker = std::thread([this, &sl, ids, ver, grp] {
+        try {
+          exec_query_group(sl, ids, ver, grp);
+        } catch (const std::exception &e) {
+          sl.error = e.what();
+        }
+      });
+    }
     next = j;
   }
+  drain();
 }
 
 std::vector<double> Server::summary() const {

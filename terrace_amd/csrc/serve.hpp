@@ -46,6 +46,20 @@ struct Request {
   std::vector<double> r64;    // kept results (ranks)
 };
 
+// A versioned snapshot and the traversal lanes that query it.
+struct QuerySlot {
+  std::unique_ptr<Graph> snap;
+  static constexpr int kLanes = 4;  // BFS, CC, PR, LTC
+  hipStream_t streams[kLanes] = {};
+  std::unique_ptr<Traversal> trav[kLanes];
+  DevBuf<uint32_t> res32[kLanes];
+  DevBuf<int32_t> par;
+  DevBuf<double> res64;
+  std::thread worker;
+  bool busy = false;
+  std::string error;
+};
+
 class Server {
  public:
   Server(Graph &g, bool batched, uint64_t mem_budget, bool keep_results);
@@ -65,20 +79,25 @@ class Server {
   std::vector<Request> reqs_;
   uint64_t version_ = 0;
   uint64_t groups_ = 0;
-  static constexpr int kLanes = 4;  // BFS, CC, PR, LTC
-  hipStream_t streams_[kLanes] = {};
+  static constexpr int kLanes = QuerySlot::kLanes;
+  // plain mode: one lane set on the live graph
   std::unique_ptr<Traversal> trav_[kLanes];
   DevBuf<uint32_t> stage_s_, stage_d_, stage_o_;
   DevBuf<uint32_t> res32_[kLanes];
   DevBuf<int32_t> par_;
   DevBuf<double> res64_;
+  // batched mode: query groups run on snapshots while the live graph moves on
+  std::vector<std::unique_ptr<QuerySlot>> slots_;
+  size_t next_slot_ = 0;
   double t0_ = 0;
 
   double now() const;
   uint64_t footprint(const Request &r) const;
   void exec_update_group(const std::vector<uint64_t> &ids);
   void exec_query_plain(Request &r);
-  void exec_query_group(const std::vector<uint64_t> &ids);
+  void exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64_t ver, uint64_t grp);
+  QuerySlot &acquire_slot();
+  void drain();
 };
 
 }  // namespace tg

@@ -77,6 +77,7 @@ struct PullBFS {
   uint32_t *obits;
   using Acc = uint32_t;
   static constexpr bool kEarly = true;
+  static constexpr const char *kName = "bfs";
   __device__ static Acc ident() { return 0xFFFFFFFFu; }
   __device__ bool skip(uint32_t i) const { return parent[i] != -1; }
   __device__ bool member(uint32_t v) const { return bit_get(fbits, v); }
@@ -104,6 +105,7 @@ struct PullCC {
   uint32_t *ids;
   using Acc = uint32_t;
   static constexpr bool kEarly = false;
+  static constexpr const char *kName = "cc";
   __device__ static Acc ident() { return 0xFFFFFFFFu; }
   __device__ bool skip(uint32_t) const { return false; }
   __device__ bool member(uint32_t v) const { return fbits == nullptr || bit_get(fbits, v); }
@@ -137,6 +139,7 @@ struct PullSum {
   bool assign;            // PR: the first pass assigns instead of adding
   using Acc = SumAcc;
   static constexpr bool kEarly = false;
+  static constexpr const char *kName = "sum";
   __device__ static Acc ident() { return SumAcc{0.0, 0}; }
   __device__ bool skip(uint32_t i) const { return visited != nullptr && visited[i]; }
   __device__ bool member(uint32_t v) const { return fbits == nullptr || bit_get(fbits, v); }
@@ -326,12 +329,41 @@ struct PushOut {
   uint32_t *count;
 };
 
+// Per-wave output buffer in LDS: appends are ranked with a ballot and
+// flushed to the global queue with one atomicAdd per 256 entries.  Every
+// call must be made by the whole wave (wave-uniform control flow), which
+// keeps the register count n uniform.
+constexpr uint32_t kWQ = 256;
+struct WaveQueue {
+  uint32_t *buf;  // this wave's kWQ LDS entries
+  uint32_t *gq;   // global queue
+  uint32_t *gcnt; // global count
+  uint32_t n;
+  __device__ void flush() {
+    if (n == 0) return;
+    const uint32_t lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(gcnt, n);
+    base = __shfl(base, 0, 64);
+    for (uint32_t i = lane; i < n; i += 64) gq[base + i] = buf[i];
+    n = 0;
+  }
+  __device__ void push(bool pred, uint32_t v) {
+    uint64_t m = __ballot(pred);
+    uint32_t c = __popcll(m);
+    if (c == 0) return;
+    if (n + c > kWQ) flush();
+    if (pred) buf[n + __popcll(m & lanemask_lt())] = v;
+    n += c;
+  }
+};
+
 template <class Op>
-__device__ inline void push_emit(const Op &op, const PushOut &o, uint32_t s, uint32_t d, bool valid) {
+__device__ inline void push_emit(const Op &op, const PushOut &o, WaveQueue &wq, uint32_t s, uint32_t d, bool valid) {
   bool hit = valid && op.apply(s, d);
   if (o.obits) {
     bool fresh = hit && bit_set(o.obits, d);
-    queue_push(o.queue, o.count, fresh, d);
+    wq.push(fresh, d);
   }
 }
 
@@ -340,68 +372,88 @@ __device__ inline void push_emit(const Op &op, const PushOut &o, uint32_t s, uin
 template <class Op>
 __global__ void __launch_bounds__(256) k_push_inline(GView g, Op op, PushOut o, const uint32_t *fq, uint32_t fcount,
                                                      uint32_t *sec_pma, uint32_t *sec_tree, uint32_t *sec_cnt) {
+  __shared__ uint32_t lq[3][4][kWQ];
+  const uint32_t w = threadIdx.x >> 6;
+  WaveQueue out{lq[0][w], o.queue, o.count, 0};
+  WaveQueue qp{lq[1][w], sec_pma, sec_cnt + 0, 0};
+  WaveQueue qt{lq[2][w], sec_tree, sec_cnt + 1, 0};
   for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < fcount;
        i0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     uint64_t i = i0 + threadIdx.x;
     bool ok = i < fcount;
     uint32_t s = ok ? fq[i] : 0;
-    uint32_t D = ok ? g.vb[s].degree : 0;
+    uint32_t wv[16];
+    load_vblock(g.vb, s, wv);
+    uint32_t D = ok ? wv[0] : 0;
     uint32_t lim = D < kInline ? D : kInline;
-    for (uint32_t k = 0; k < kInline; k++) {
-      bool valid = k < lim;
-      uint32_t d = valid ? g.vb[s].nbr[k] : 0;
-      push_emit(op, o, s, d, valid);
-    }
+#pragma unroll
+    for (uint32_t k = 0; k < kInline; k++) push_emit(op, o, out, s, wv[1 + k], k < lim);
     bool sec = ok && D > kInline;
-    bool tree = sec && (g.vb[s].aux & kAuxTree);
-    queue_push(sec_pma, sec_cnt + 0, sec && !tree, s);
-    queue_push(sec_tree, sec_cnt + 1, tree, s);
+    uint64_t aux = (static_cast<uint64_t>(wv[15]) << 32) | wv[14];
+    bool tree = sec && (aux & kAuxTree);
+    qp.push(sec && !tree, s);
+    qt.push(tree, s);
   }
+  out.flush();
+  qp.flush();
+  qt.flush();
 }
 
+// Four frontier vertices per wave (16-lane groups, 4 slots per lane); the
+// groups step in lockstep so the wave's output buffer stays uniform.
 template <class Op>
 __global__ void __launch_bounds__(256) k_push_pma(GView g, Op op, PushOut o, const uint32_t *list,
                                                   const uint32_t *cnt_ptr) {
-  uint32_t cnt = *cnt_ptr;
-  const uint32_t lane = threadIdx.x & 63, gl = lane & 15;
+  __shared__ uint32_t lq[4][kWQ];
+  const uint32_t cnt = *cnt_ptr;
+  const uint32_t lane = threadIdx.x & 63, gl = lane & 15, gi = lane >> 4;
   const uint64_t gmask = 0xFFFFull << (lane & 48);
   const uint64_t lt = lanemask_lt() & gmask;
-  uint64_t grp = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 4;
-  uint64_t ngrp = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 4;
-  for (uint64_t w = grp; w < cnt; w += ngrp) {
-    uint32_t s = list[w];
-    const VBlock &b = g.vb[s];
-    uint32_t need = b.degree - kInline;
-    uint64_t begin = b.aux;
+  WaveQueue out{lq[threadIdx.x >> 6], o.queue, o.count, 0};
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  for (uint64_t base = wave * 4; base < cnt; base += nwaves * 4) {
+    uint64_t w = base + gi;
+    bool have = w < cnt;
+    uint32_t s = have ? list[w] : 0;
+    uint32_t need = have ? g.vb[s].degree - kInline : 0;
+    uint64_t begin = have ? g.vb[s].aux : 0;
     uint64_t p = begin & ~3ull;
     uint32_t got = 0;
-    while (got < need && p < g.N) {
+    while (__ballot(got < need && p < g.N) != 0) {
+      bool act = got < need && p < g.N;
       uint64_t slot = p + gl * 4;
       uint4 q = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
-      if (slot < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + slot);
+      // slot is 4-aligned and N a multiple of 64, so slot < N covers slot+3
+      if (act && slot < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + slot);
       uint32_t d[4] = {q.x, q.y, q.z, q.w};
       bool val[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) val[j] = d[j] != kEmpty && slot + j >= begin;
+      for (int j = 0; j < 4; j++) val[j] = act && d[j] != kEmpty && slot + j >= begin;
       uint64_t B0 = __ballot(val[0]) & gmask, B1 = __ballot(val[1]) & gmask;
       uint64_t B2 = __ballot(val[2]) & gmask, B3 = __ballot(val[3]) & gmask;
       uint32_t r = __popcll(B0 & lt) + __popcll(B1 & lt) + __popcll(B2 & lt) + __popcll(B3 & lt);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        bool in = val[j] && got + r < need;
-        push_emit(op, o, s, d[j], in);
+        push_emit(op, o, out, s, d[j], val[j] && got + r < need);
         if (val[j]) r++;
       }
-      got += __popcll(B0) + __popcll(B1) + __popcll(B2) + __popcll(B3);
-      p += 64;
+      if (act) {
+        got += __popcll(B0) + __popcll(B1) + __popcll(B2) + __popcll(B3);
+        p += 64;
+      }
     }
   }
+  out.flush();
 }
 
+// One workgroup per frontier B-tree vertex, 256 ids (one leaf block) per step.
 template <class Op>
 __global__ void __launch_bounds__(256) k_push_tree(GView g, Op op, PushOut o, const uint32_t *list,
                                                    const uint32_t *cnt_ptr) {
-  uint32_t cnt = *cnt_ptr;
+  __shared__ uint32_t lq[4][kWQ];
+  const uint32_t cnt = *cnt_ptr;
+  WaveQueue out{lq[threadIdx.x >> 6], o.queue, o.count, 0};
   for (uint64_t w = blockIdx.x; w < cnt; w += gridDim.x) {
     uint32_t s = list[w];
     uint32_t tr = static_cast<uint32_t>(g.vb[s].aux);
@@ -412,9 +464,10 @@ __global__ void __launch_bounds__(256) k_push_tree(GView g, Op op, PushOut o, co
       uint32_t c = g.bcnt[blk];
       bool valid = threadIdx.x < c;
       uint32_t d = valid ? g.bkeys[static_cast<uint64_t>(blk) * kBlk + threadIdx.x] : 0;
-      push_emit(op, o, s, d, valid);
+      push_emit(op, o, out, s, d, valid);
     }
   }
+  out.flush();
 }
 
 // ---------------------------------------------------------------- helpers
@@ -553,6 +606,7 @@ struct PullMS {
   uint64_t full;
   using Acc = uint64_t;
   static constexpr bool kEarly = false;
+  static constexpr const char *kName = "ms";
   __device__ static Acc ident() { return 0ull; }
   __device__ bool skip(uint32_t i) const { return (vis[i] & full) == full; }
   __device__ bool member(uint32_t) const { return true; }
@@ -811,12 +865,12 @@ void Traversal::record(const char *name, hipEvent_t a, hipEvent_t b) {
   TG_HIP(hipEventSynchronize(b));
   TG_HIP(hipEventElapsedTime(&ms, a, b));
   for (auto &k : ktimes)
-    if (std::string(k.name) == name) {
+    if (k.name == name) {
       k.ms += ms;
       k.launches++;
       return;
     }
-  ktimes.push_back(KTime{name, ms, 1});
+  ktimes.push_back(KTime{std::string(name), ms, 1});
 }
 
 void Traversal::frontier_single(Frontier &f, uint32_t v) {
@@ -913,27 +967,32 @@ static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierL
                         uint32_t ntree, const uint64_t *task_off, uint64_t ntasks) {
   hipStream_t s = T.stream();
   GView v = g.view();
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  // events: [0] start, [1] after inline, [2] after PMA, [3] after tree
+  hipEvent_t e[4] = {};
   if (T.profiling) {
-    TG_HIP(hipEventCreate(&e0));
-    TG_HIP(hipEventCreate(&e1));
-    TG_HIP(hipEventRecord(e0, s));
+    for (auto &x : e) TG_HIP(hipEventCreate(&x));
+    TG_HIP(hipEventRecord(e[0], s));
   }
   k_pull_inline<Op><<<grid_for(v.n, 256), 256, 0, s>>>(v, op);
   TG_CHECK_LAUNCH();
+  if (T.profiling) TG_HIP(hipEventRecord(e[1], s));
   if (L.npma) {
     k_pull_pma<Op><<<grid_for(static_cast<uint64_t>(L.npma) * 16, 256), 256, 0, s>>>(v, op, L.pma, L.npma);
     TG_CHECK_LAUNCH();
   }
+  if (T.profiling) TG_HIP(hipEventRecord(e[2], s));
   if (ntasks) {
     k_pull_tree<Op><<<grid_for(ntasks * 256, 256, 1u << 20), 256, 0, s>>>(v, op, tree_list, ntree, task_off, ntasks);
     TG_CHECK_LAUNCH();
   }
   if (T.profiling) {
-    TG_HIP(hipEventRecord(e1, s));
-    T.record("edge_map_pull", e0, e1);
-    TG_HIP(hipEventDestroy(e0));
-    TG_HIP(hipEventDestroy(e1));
+    TG_HIP(hipEventRecord(e[3], s));
+    std::string k = Op::kName;
+    T.record("edge_map_pull", e[0], e[3]);
+    T.record(("pull_inline_" + k).c_str(), e[0], e[1]);
+    if (L.npma) T.record(("pull_pma_" + k).c_str(), e[1], e[2]);
+    if (ntasks) T.record(("pull_tree_" + k).c_str(), e[2], e[3]);
+    for (auto &x : e) TG_HIP(hipEventDestroy(x));
   }
 }
 

@@ -4,6 +4,8 @@
 // algorithms of include/BFS.h, Pagerank.h, Components.h, BC.h and TC.h.
 #pragma once
 
+#include <string>
+
 #include "graph.hpp"
 
 namespace tg {
@@ -79,7 +81,7 @@ class Traversal {
 
   // Per-kernel timing on the graph's stream (hipEvents), keyed by name.
   struct KTime {
-    const char *name;
+    std::string name;
     double ms;
     uint64_t launches;
   };

@@ -144,6 +144,12 @@ class Graph:
                 "promotions", "trees", "pma_slots"]
         return {k: int(v) for k, v in zip(keys, out)}
 
+    def tier_counts(self) -> dict:
+        out = np.zeros(4, np.uint64)
+        check(lib().tg_tier_counts(self._h, ptr(out)))
+        return {"pma_vertices": int(out[0]), "pma_edges": int(out[1]),
+                "tree_vertices": int(out[2]), "tree_edges": int(out[3])}
+
     # -- profiling
     def profile(self, on: bool = True) -> None:
         check(lib().tg_profile_enable(self._h, int(on)))
