@@ -28,9 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_pull_pma<PullSum> launch at the default workload, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (profiles/), or None.
-PMA_PULL_TRAFFIC = None
+# HBM bytes per k_pull_pma<PullSum> launch at the default workload (scale 22,
+# 69M generated edges), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# scaled by a 1 GiB copy kernel (FETCH x2.000, WRITE x1.000).
+PMA_PULL_TRAFFIC_DEFAULT = 6.144e9
 
 
 def log(*a):
@@ -77,6 +78,21 @@ def apply_batch(G, bt):
     s, d, w, bs, bd, nd = bt
     G.add_edge_batch_dev(s, d, w)
     G.remove_edge_batch_dev(bs, bd, nd)
+
+
+def copy_probe():
+    """Three copies of 1 GiB between device buffers by the library's plain
+    copy kernel (k_copy16): a kernel of known HBM traffic (1 GiB read +
+    1 GiB written per launch) that scales the FETCH_SIZE / WRITE_SIZE
+    counters."""
+    from terrace_amd._lib import DeviceBuffer, lib, check
+    nb = 1 << 30
+    a, b = DeviceBuffer(nb), DeviceBuffer(nb)
+    for _ in range(3):
+        check(lib().tg_memcpy_d2d(b.ptr, a.ptr, nb))
+    a.close()
+    b.close()
+    log(f"copy probe: 3 x {nb} bytes")
 
 
 def cpu_baseline(quick: bool):
@@ -267,8 +283,11 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edges", type=int, default=69_000_000)
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-extra-batches", action="store_true", help="skip the 1e3 / 1e7 batch sizes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller graph for smoke runs")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="finish with three 1 GiB device copies (HBM counter calibration)")
     ap.add_argument("--serve", action="store_true", help="mixed-stream serving comparison instead")
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--rate", type=float, default=200.0, help="request arrivals per second")
@@ -290,6 +309,7 @@ def main():
     if args.quick:
         scale, m0 = 18, 4_000_000
     n = 1 << scale
+    default_workload = (scale, m0) == (22, 69_000_000)
     probs = (0.5, 0.1, 0.1)
     seed = aspen(100) & 0xFFFFFFFF
     rng = np.random.default_rng(7)
@@ -357,7 +377,8 @@ def main():
         pma_avg = pma_ms / max(pma_n, 1)
         achieved = pma_bytes / (pma_avg * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": PMA_PULL_TRAFFIC,
+                    "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": PMA_PULL_TRAFFIC_DEFAULT if default_workload else None,
                     "kernel": "k_pull_pma<PullSum> (PageRank dense pull over PMA-tier vertices)",
                     "avg_ms": pma_avg, "launches": pma_n, "algorithmic_bytes": pma_bytes,
                     "units": tc_}
@@ -371,7 +392,7 @@ def main():
         tg.CC(G)
         extra["cc_ms"] = (time.perf_counter() - t0) * 1e3
         G.profile(False)
-        for bsz in (1000, 10_000_000):
+        for bsz in (() if args.no_extra_batches else (1000, 10_000_000)):
             bt = make_batches(tg, G, n, probs, bsz, 2, 5000 + bsz % 97, rng, off, nbr)
             apply_batch(G, bt[0])
             tg.lib().tg_device_sync()
@@ -383,6 +404,8 @@ def main():
         extra["memory_bytes"] = G.get_size()
         extra["stats"] = G.stats()
 
+    if args.copy_probe:
+        copy_probe()
     cpu = None if args.no_cpu else cpu_baseline(args.quick)
     out = {
         "metric": "inserted edges/s",

@@ -219,6 +219,8 @@ int tg_dev_alloc(uint64_t bytes, void **out);
 int tg_dev_free(void *p);
 int tg_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
 int tg_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+/* Device-to-device copy by a plain vectorised kernel (synchronous). */
+int tg_memcpy_d2d(void *dst, const void *src, uint64_t bytes);
 int tg_device_sync(void);
 
 /* ---- measurement -------------------------------------------------------

@@ -125,6 +125,7 @@ _SIGS = {
     "tg_dev_free": (ctypes.c_int, [_P]),
     "tg_memcpy_h2d": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
     "tg_memcpy_d2h": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
+    "tg_memcpy_d2d": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
     "tg_device_sync": (ctypes.c_int, []),
     "tg_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "tg_profile_reset": (ctypes.c_int, [_P]),

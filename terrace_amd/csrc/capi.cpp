@@ -664,6 +664,13 @@ int tg_memcpy_d2h(void *dst, const void *src, uint64_t bytes) {
   return guard([&] { TG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
 }
 
+int tg_memcpy_d2d(void *dst, const void *src, uint64_t bytes) {
+  return guard([&] {
+    tg::copy_bytes(dst, src, bytes, nullptr);
+    TG_HIP(hipStreamSynchronize(nullptr));
+  });
+}
+
 int tg_device_sync(void) {
   return guard([&] { TG_HIP(hipDeviceSynchronize()); });
 }

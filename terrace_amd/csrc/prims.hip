@@ -308,4 +308,34 @@ void fill_f64(double *p, double v, uint64_t n, hipStream_t s) {
   TG_CHECK_LAUNCH();
 }
 
+namespace {
+__global__ void __launch_bounds__(256) k_copy16(uint4 *dst, const uint4 *src, uint64_t n16) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n16;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+__global__ void k_copy1(uint8_t *dst, const uint8_t *src, uint64_t n) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+void copy_bytes(void *dst, const void *src, uint64_t bytes, hipStream_t s) {
+  if (!bytes) return;
+  auto d = reinterpret_cast<uintptr_t>(dst), r = reinterpret_cast<uintptr_t>(src);
+  uint64_t n16 = ((d | r) & 15) == 0 ? bytes / 16 : 0;
+  if (n16) {
+    k_copy16<<<grid_for(n16, 256, 1u << 16), 256, 0, s>>>(static_cast<uint4 *>(dst), static_cast<const uint4 *>(src),
+                                                          n16);
+    TG_CHECK_LAUNCH();
+  }
+  uint64_t tail = bytes - n16 * 16;
+  if (tail) {
+    k_copy1<<<grid_for(tail, 256), 256, 0, s>>>(static_cast<uint8_t *>(dst) + n16 * 16,
+                                                 static_cast<const uint8_t *>(src) + n16 * 16, tail);
+    TG_CHECK_LAUNCH();
+  }
+}
+
 }  // namespace tg

@@ -74,6 +74,10 @@ void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 void fill_f64(double *p, double v, uint64_t n, hipStream_t s);
 
+// Device-to-device copy with a plain 16-byte-per-lane kernel (its HBM
+// traffic is exactly bytes read + bytes written).
+void copy_bytes(void *dst, const void *src, uint64_t bytes, hipStream_t s);
+
 // Read one device scalar (synchronises the stream).
 template <typename T>
 T read_scalar(const T *dev, hipStream_t s) {
