@@ -146,15 +146,19 @@ def cpu_baseline(quick: bool):
 def serve_bench(args):
     """Mixed request stream through the plain and the batched server.
 
-    rMAT (skewed a=.57 b=.19 c=.19) graph built twice on the device.  R
-    requests arrive as a Poisson process: 30% update batches (10..1e6 edges,
+    rMAT (skewed a=.57 b=.19 c=.19) graph built on the device.  R requests
+    arrive as a Poisson process: 30% update batches (10..1e6 edges,
     log-uniform, 90% inserts of fresh rMAT edges / 10% deletes of existing
     edges, both directions), 70% queries (60% BFS, 20% CC, 10% PageRank 10
-    iterations FP64, 10% local triangle count).  The same stream is replayed
-    one request at a time (plain) and with continuous batching; every query
-    result digest must agree between the two."""
+    iterations FP64, 10% local triangle count).  The stream is replayed one
+    request at a time (plain) and with continuous batching (snapshot at
+    admission).  Every batched result is then checked against the query run
+    alone on the version it reports: the plain server replays the batched
+    server's order (each update at its version, each query after the updates
+    of its version) on a fresh copy of the graph."""
     from terrace_amd import graph as tg
 
+    os.environ.setdefault("TG_SERVE_PROGRESS", "1")
     n = 1 << args.scale
     probs = (0.57, 0.19, 0.19)
     seed = aspen(100) & 0xFFFFFFFF
@@ -165,15 +169,16 @@ def serve_bench(args):
         s, d, w = tg.rmat_device(n, seed, *probs, args.edges, nominal=True, symmetric=True, drop_loops=True)
         G.add_edge_batch_dev(s, d, w)
         del s, d
+        tg.lib().tg_device_sync()
         return G
 
-    log(f"serve: building scale {args.scale} skewed graph, {args.edges} generated edges (x2)")
+    log(f"serve: building scale {args.scale} skewed graph, {args.edges} generated edges")
     t0 = time.perf_counter()
     G1 = build()
-    log(f"serve: graph 1 built in {time.perf_counter() - t0:.2f} s, {G1.get_num_edges()} edges, "
-        f"{G1.get_size() / 2**30:.1f} GiB")
+    m_directed = G1.get_num_edges()
+    log(f"serve: graph built in {time.perf_counter() - t0:.2f} s, {m_directed} edges, "
+        f"{G1.get_size() / 2**30:.1f} GiB, tiers {G1.tier_counts()}")
     off, nbr = G1.export_adjacency()
-    G2 = build()
     m = len(nbr)
     deg = (off[1:] - off[:-1])
     active = np.nonzero(deg)[0].astype(np.uint32)
@@ -210,22 +215,22 @@ def serve_bench(args):
                 reqs.append(("PR", arrivals[i]))
             else:
                 reqs.append(("LTC", arrivals[i], v))
-    del off, nbr
+    del off, nbr, deg, active
 
-    def replay(G, batched):
-        S = tg.Server(G, batched=batched, mem_budget=args.budget << 30)
-        ids = []
-        for r in reqs:
-            if r[0] == "U":
-                ids.append(S.submit(tg.REQ_UPDATE, r[1], ins=r[2], dels=r[3]))
-            elif r[0] == "BFS":
-                ids.append(S.submit(tg.REQ_BFS, r[1], src=r[2]))
-            elif r[0] == "CC":
-                ids.append(S.submit(tg.REQ_CC, r[1]))
-            elif r[0] == "PR":
-                ids.append(S.submit(tg.REQ_PR, r[1], iters=10))
-            else:
-                ids.append(S.submit(tg.REQ_LTC, r[1], src=r[2]))
+    def submit(S, r, arrival):
+        if r[0] == "U":
+            return S.submit(tg.REQ_UPDATE, arrival, ins=r[2], dels=r[3])
+        if r[0] == "BFS":
+            return S.submit(tg.REQ_BFS, arrival, src=r[2])
+        if r[0] == "CC":
+            return S.submit(tg.REQ_CC, arrival)
+        if r[0] == "PR":
+            return S.submit(tg.REQ_PR, arrival, iters=10)
+        return S.submit(tg.REQ_LTC, arrival, src=r[2])
+
+    def replay(G, mode, stream):
+        S = tg.Server(G, mode=mode, mem_budget=args.budget << 30)
+        ids = [submit(S, r, a) for r, a in stream]
         t0 = time.perf_counter()
         S.run()
         wall = time.perf_counter() - t0
@@ -236,21 +241,36 @@ def serve_bench(args):
         return summ, res
 
     log(f"serve: replaying {R} requests at {args.rate}/s, plain")
-    sp, rp = replay(G1, False)
+    sp, rp = replay(G1, "plain", [(r, r[1]) for r in reqs])
+    G1.close()
     log(f"serve: plain {sp}")
-    log("serve: batched")
-    sb, rb = replay(G2, True)
+    G2 = build()
+    log("serve: batched (snapshot at admission)")
+    sb, rb = replay(G2, "admission", [(r, r[1]) for r in reqs])
+    G2.close()
     log(f"serve: batched {sb}")
-    mism = 0
-    for r, a, b in zip(reqs, rp, rb):
-        if a["version"] != b["version"]:
-            mism += 1
-        elif r[0] in ("BFS", "CC") and a["digest"] != b["digest"]:
-            mism += 1
-        elif r[0] == "LTC" and a["value"] != b["value"]:
-            mism += 1
-        elif r[0] == "PR" and abs(a["fvalue"] - b["fvalue"]) > 1e-9 * max(abs(a["fvalue"]), 1e-300):
-            mism += 1
+    mism = verified = 0
+    if not args.no_verify:
+        # the batched server's linearization, replayed one request at a time
+        order = sorted(range(R), key=lambda i: (rb[i]["version"], 0 if reqs[i][0] == "U" else 1, i))
+        G3 = build()
+        log("serve: verification replay (each query alone on its version)")
+        sv, rv = replay(G3, "plain", [(reqs[i], 0.0) for i in order])
+        G3.close()
+        for pos, i in enumerate(order):
+            r, a, b = reqs[i], rv[pos], rb[i]
+            verified += 1
+            if a["status"] != 0 or b["status"] != 0 or a["version"] != b["version"]:
+                mism += 1
+            elif r[0] in ("BFS", "CC") and a["digest"] != b["digest"]:
+                mism += 1
+            elif r[0] == "LTC" and a["value"] != b["value"]:
+                mism += 1
+            elif r[0] == "PR" and abs(a["fvalue"] - b["fvalue"]) > 1e-9 * max(abs(a["fvalue"]), 1e-300):
+                mism += 1
+        log(f"serve: verified {verified} results, {mism} mismatches")
+    # queries admitted later than their arrival-order version
+    ahead = sum(1 for i in range(R) if reqs[i][0] != "U" and rb[i]["version"] != rp[i]["version"])
     kinds = {k: sum(1 for r in reqs if r[0] == k) for k in ("U", "BFS", "CC", "PR", "LTC")}
     out = {
         "metric": "mixed-stream requests/s (batched server)",
@@ -267,9 +287,12 @@ def serve_bench(args):
         "data": "synthetic rMAT generated on the device (seeded); Poisson arrivals",
         "config": {"workload": f"serving: rmat scale {args.scale} skewed (.57/.19/.19), {args.edges} generated "
                                f"edges; {R} requests at {args.rate}/s", "vertices": n,
-                   "directed_edges": G1.get_num_edges(), "requests": kinds, "parallelism": "single GPU"},
-        "serve": {"plain": sp, "batched": sb, "throughput_x": sb["requests_per_s"] / max(sp["requests_per_s"], 1e-9),
-                  "p99_ratio": sb["p99_s"] / max(sp["p99_s"], 1e-12), "result_mismatches": mism},
+                   "directed_edges": m_directed, "requests": kinds, "parallelism": "single GPU"},
+        "serve": {"plain": sp, "batched": sb, "mode": "snapshot at admission",
+                  "throughput_x": sb["requests_per_s"] / max(sp["requests_per_s"], 1e-9),
+                  "p99_ratio": sb["p99_s"] / max(sp["p99_s"], 1e-12),
+                  "verified": verified, "result_mismatches": mism,
+                  "queries_on_later_version": ahead},
     }
     print(json.dumps(out))
 
@@ -280,8 +303,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=100_000)
-    ap.add_argument("--scale", type=int, default=22)
-    ap.add_argument("--edges", type=int, default=69_000_000)
+    ap.add_argument("--scale", type=int, default=None, help="default 22 (26 with --serve)")
+    ap.add_argument("--edges", type=int, default=None, help="generated edges, default 69M (775M with --serve)")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-extra-batches", action="store_true", help="skip the 1e3 / 1e7 batch sizes")
     ap.add_argument("--no-cpu", action="store_true")
@@ -291,13 +314,18 @@ def main():
     ap.add_argument("--serve", action="store_true", help="mixed-stream serving comparison instead")
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--rate", type=float, default=200.0, help="request arrivals per second")
-    ap.add_argument("--budget", type=int, default=64, help="server memory budget, GiB")
+    ap.add_argument("--budget", type=int, default=160, help="server memory budget, GiB")
+    ap.add_argument("--no-verify", action="store_true", help="serve: skip the verification replay")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.scale is None:
+        args.scale = 26 if args.serve else 22
+    if args.edges is None:
+        args.edges = 775_000_000 if args.serve else 69_000_000
     if args.serve:
         serve_bench(args)
         return

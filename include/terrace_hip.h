@@ -172,8 +172,14 @@ int tg_ltc(tg_graph *g, const uint32_t *verts, uint32_t q, uint64_t *counts);
  * their arrival offsets (seconds from tg_server_run) against one graph.
  * batched = 0: one request at a time in arrival order (the plain path).
  * batched = 1: continuous batching; every query still runs on the version
- * that contains exactly the updates that arrived before it.  Requests whose
- * footprint exceeds mem_budget bytes are rejected (status TG_ENOMEM). */
+ * that contains exactly the updates that arrived before it.
+ * batched = 2: continuous batching with snapshot-at-admission; updates are
+ * applied as they arrive, queries are pooled and admitted together on the
+ * version current at admission (it contains every update that arrived
+ * before the query, and may contain updates that arrived while it waited).
+ * tg_result.version names the version; the result equals the query run
+ * alone on it.  Requests whose footprint exceeds mem_budget bytes are
+ * rejected (status TG_ENOMEM). */
 typedef struct tg_server tg_server;
 enum tg_req { TG_REQ_UPDATE = 1, TG_REQ_BFS = 2, TG_REQ_CC = 3, TG_REQ_PR = 4, TG_REQ_LTC = 5 };
 typedef struct {

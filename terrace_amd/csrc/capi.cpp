@@ -547,7 +547,8 @@ int tg_server_create(tg_graph *G, int batched, uint64_t mem_budget, int keep_res
     NEED(G && out, "null argument");
     auto *sv = new tg_server;
     sv->owner = G;
-    sv->s = std::make_unique<tg::Server>(G->g, batched != 0, mem_budget, keep_results != 0);
+    NEED(batched >= 0 && batched <= 2, "batched must be 0, 1 or 2");
+    sv->s = std::make_unique<tg::Server>(G->g, batched, mem_budget, keep_results != 0);
     *out = sv;
   });
 }

@@ -1999,18 +1999,17 @@ void Graph::copy_from(Graph &src, bool query_only) {
     mark_.release();
     pma_H_ = src.pma_H_;
     pma_N_ = src.pma_N_;
-    uint64_t nl = 1ull << pma_H_;
     pdst_.ensure(pma_N_, s);
-    psrc_.ensure(pma_N_, s);
-    pcnt_.ensure(2 * nl, s);
-    fence_.ensure(nl, s);
-    mark_.ensure(2 * nl, s);
-    TG_HIP(hipMemsetAsync(mark_.get(), 0, 2 * nl, s));
     bounds_ = src.bounds_;
   }
   uint64_t nl = 1ull << pma_H_;
   cp(pdst_.get(), src.pdst_.get(), pma_N_ * 4);
   if (!query_only) {
+    // update-side PMA state (a query-only snapshot never allocates it)
+    psrc_.ensure(pma_N_, s);
+    pcnt_.ensure(2 * nl, s);
+    fence_.ensure(nl, s);
+    if (mark_.ensure(2 * nl, s)) TG_HIP(hipMemsetAsync(mark_.get(), 0, 2 * nl, s));
     cp(psrc_.get(), src.psrc_.get(), pma_N_ * 4);
     cp(pcnt_.get(), src.pcnt_.get(), 2 * nl * 4);
     cp(fence_.get(), src.fence_.get(), nl * 8);

@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 
 namespace tg {
@@ -25,13 +27,19 @@ double pct(std::vector<double> v, double p) {
 
 }  // namespace
 
-Server::Server(Graph &g, bool batched, uint64_t mem_budget, bool keep_results)
-    : g_(g), batched_(batched), budget_(mem_budget), keep_(keep_results) {
+Server::Server(Graph &g, int mode, uint64_t mem_budget, bool keep_results)
+    : g_(g), mode_(mode), batched_(mode != kServePlain), budget_(mem_budget), keep_(keep_results) {
+  if (mode < kServePlain || mode > kServeAdmission) throw Error(-1, "unknown serving mode");
+  const char *pv = std::getenv("TG_SERVE_PROGRESS");
+  progress_ = pv && pv[0] == '1';
   for (int i = 0; i < kLanes; i++) trav_[i] = std::make_unique<Traversal>(g_);
   if (batched_) {
-    // snapshots in flight: bounded by the budget (each holds a full copy)
+    // snapshots in flight: bounded by the budget (each holds a full copy).
+    // Admission mode keeps two, so the pending pool grows into large
+    // shared groups while one group runs.
     uint64_t per = std::max<uint64_t>(g_.memory_bytes(), 1);
-    size_t k = static_cast<size_t>(std::min<uint64_t>(6, std::max<uint64_t>(1, budget_ / (2 * per))));
+    uint64_t cap = mode_ == kServeAdmission ? 2 : 6;
+    size_t k = static_cast<size_t>(std::min<uint64_t>(cap, std::max<uint64_t>(1, budget_ / (2 * per))));
     for (size_t j = 0; j < k; j++) {
       auto sl = std::make_unique<QuerySlot>();
       sl->snap = std::make_unique<Graph>(g_.num_vertices());
@@ -60,6 +68,41 @@ Server::~Server() {
   }
 }
 
+// A slot whose group has finished (joined), or null when all are busy.
+QuerySlot *Server::free_slot() {
+  for (auto &p : slots_) {
+    QuerySlot &sl = *p;
+    if (sl.running.load(std::memory_order_acquire)) continue;
+    if (sl.worker.joinable()) sl.worker.join();
+    if (!sl.error.empty()) throw Error(-3, "query group: " + sl.error);
+    return &sl;
+  }
+  return nullptr;
+}
+
+// Snapshot the live graph into the slot and run the group on it.
+void Server::launch_group(QuerySlot &sl, std::vector<uint64_t> ids) {
+  sl.snap->copy_from(g_, true);  // version_ as of now, on the live graph's stream
+  uint64_t ver = version_, grp = groups_++;
+  sl.error.clear();
+  sl.running.store(true, std::memory_order_release);
+  sl.worker = std::thread([this, &sl, ids = std::move(ids), ver, grp] {
+    try {
+      exec_query_group(sl, ids, ver, grp);
+    } catch (const std::exception &e) {
+      sl.error = e.what();
+    }
+    sl.running.store(false, std::memory_order_release);
+  });
+}
+
+void Server::reject(Request &r) {
+  r.status = -2;
+  r.start = r.done = now();
+  r.version = version_;
+  note_done(1);
+}
+
 QuerySlot &Server::acquire_slot() {
   QuerySlot &sl = *slots_[next_slot_];
   next_slot_ = (next_slot_ + 1) % slots_.size();
@@ -78,6 +121,15 @@ void Server::drain() {
 }
 
 double Server::now() const { return wall() - t0_; }
+
+void Server::note_done(uint64_t k) {
+  uint64_t before = completed_.fetch_add(k);
+  if (!progress_) return;
+  uint64_t total = reqs_.size(), step = std::max<uint64_t>(1, total / 10);
+  if ((before + k) / step != before / step || before + k == total)
+    std::fprintf(stderr, "[serve] mode %d: %llu / %llu requests done at %.2f s\n", mode_,
+                 static_cast<unsigned long long>(before + k), static_cast<unsigned long long>(total), now());
+}
 
 uint64_t Server::submit(Request r) {
   r.status = 1;
@@ -168,6 +220,7 @@ void Server::exec_update_group(const std::vector<uint64_t> &ids) {
     r.group = groups_;
   }
   groups_++;
+  note_done(ids.size());
 }
 
 void Server::exec_query_plain(Request &r) {
@@ -222,6 +275,7 @@ void Server::exec_query_plain(Request &r) {
   r.version = version_;
   r.status = 0;
   r.group = groups_++;
+  note_done(1);
 }
 
 void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64_t ver, uint64_t grp) {
@@ -344,6 +398,7 @@ void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64
   for (auto &t : th) t.join();
   for (auto &e : errors)
     if (!e.empty()) throw Error(-3, "query group: " + e);
+  note_done(ids.size());
 }
 
 void Server::run() {
@@ -352,7 +407,12 @@ void Server::run() {
   std::stable_sort(order.begin(), order.end(),
                    [&](uint64_t a, uint64_t b) { return reqs_[a].arrival < reqs_[b].arrival; });
   (void)g_.tier_lists();
+  completed_ = 0;
   t0_ = wall();
+  if (mode_ == kServeAdmission) {
+    run_admission(order);
+    return;
+  }
   size_t next = 0;  // first request (in arrival order) not yet executed
   while (next < order.size()) {
     double t = now();
@@ -364,9 +424,7 @@ void Server::run() {
     }
     // a request larger than the whole budget can never be admitted
     if (head.footprint > budget_) {
-      head.status = -2;
-      head.start = head.done = now();
-      head.version = version_;
+      reject(head);
       next++;
       continue;
     }
@@ -393,19 +451,62 @@ void Server::run() {
     if (upd) {
       exec_update_group(ids);
     } else {
-      QuerySlot &sl = acquire_slot();
-      sl.snap->copy_from(g_, true);  // version_ as of now, on the live graph's stream
-      uint64_t ver = version_, grp = groups_++;
-      sl.error.clear();
-      sl.worker = std::thread([this, &sl, ids, ver, grp] {
-        try {
-          exec_query_group(sl, ids, ver, grp);
-        } catch (const std::exception &e) {
-          sl.error = e.what();
-        }
-      });
+      launch_group(acquire_slot(), ids);
     }
     next = j;
+  }
+  drain();
+}
+
+void Server::run_admission(const std::vector<uint64_t> &order) {
+  size_t next = 0;               // first request (arrival order) not yet taken in
+  std::vector<uint64_t> pool;    // arrived queries waiting for a slot, in arrival order
+  while (next < order.size() || !pool.empty()) {
+    double t = now();
+    // take in every arrival: updates are applied now, in arrival order
+    std::vector<uint64_t> upd;
+    uint64_t used_u = 0;
+    while (next < order.size() && reqs_[order[next]].arrival <= t) {
+      uint64_t id = order[next++];
+      Request &r = reqs_[id];
+      if (r.footprint > budget_) {
+        reject(r);
+        continue;
+      }
+      if (r.type != kReqUpdate) {
+        pool.push_back(id);
+        continue;
+      }
+      if (!upd.empty() && used_u + r.footprint > budget_) {
+        exec_update_group(upd);
+        upd.clear();
+        used_u = 0;
+      }
+      upd.push_back(id);
+      used_u += r.footprint;
+    }
+    if (!upd.empty()) exec_update_group(upd);
+    // admit the pool (oldest first, within the budget) when a slot is free
+    if (!pool.empty()) {
+      if (QuerySlot *sl = free_slot()) {
+        std::vector<uint64_t> ids;
+        uint64_t used = 0;
+        size_t k = 0;
+        for (; k < pool.size(); k++) {
+          const Request &r = reqs_[pool[k]];
+          if (!ids.empty() && used + r.footprint > budget_) break;
+          used += r.footprint;
+          ids.push_back(pool[k]);
+        }
+        pool.erase(pool.begin(), pool.begin() + static_cast<std::ptrdiff_t>(k));
+        launch_group(*sl, std::move(ids));
+        continue;
+      }
+    }
+    // idle: wait for the next arrival, or poll for a slot
+    double wait = next < order.size() ? reqs_[order[next]].arrival - now() : 1.0;
+    if (!pool.empty()) wait = std::min(wait, 50e-6);
+    if (wait > 20e-6) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
   }
   drain();
 }

@@ -5,7 +5,7 @@
 // plain   : one request at a time, in arrival order, through the same
 //           calls a single user makes (insert_batch + delete_batch, bfs,
 //           cc, pagerank, ltc).
-// batched : continuous batching.  Pending requests are cut into maximal
+// strict  : continuous batching.  Pending requests are cut into maximal
 //           arrival-ordered runs of updates or of queries, so every query
 //           sees exactly the updates that arrived before it (the version
 //           it runs on).  A run of updates is coalesced into one device
@@ -15,8 +15,18 @@
 //           lane mask), identical CC / PR queries share one computation,
 //           LTC queries share one launch, and the query kinds run
 //           concurrently on separate streams.
+// admission: continuous batching with snapshot-at-admission.  Updates are
+//           applied in arrival order as soon as they arrive (coalesced);
+//           queries wait in a pending pool and are admitted together, as
+//           slots and memory free up, on a snapshot of the version current
+//           at admission.  Every query sees at least the updates that
+//           arrived before it, and possibly updates that arrived while it
+//           waited (a linearizable order: those updates and the query were
+//           in flight together).  Each result carries its version and
+//           equals the query run alone on that version.
 #pragma once
 
+#include <atomic>
 #include <memory>
 #include <thread>
 #include <vector>
@@ -26,6 +36,7 @@
 namespace tg {
 
 enum ReqType { kReqUpdate = 1, kReqBfs = 2, kReqCc = 3, kReqPr = 4, kReqLtc = 5 };
+enum ServeMode { kServePlain = 0, kServeStrict = 1, kServeAdmission = 2 };
 
 struct Request {
   int type = 0;
@@ -56,13 +67,13 @@ struct QuerySlot {
   DevBuf<int32_t> par;
   DevBuf<double> res64;
   std::thread worker;
-  bool busy = false;
+  std::atomic<bool> running{false};
   std::string error;
 };
 
 class Server {
  public:
-  Server(Graph &g, bool batched, uint64_t mem_budget, bool keep_results);
+  Server(Graph &g, int mode, uint64_t mem_budget, bool keep_results);
   ~Server();
   uint64_t submit(Request r);
   void run();
@@ -73,6 +84,7 @@ class Server {
 
  private:
   Graph &g_;
+  int mode_;
   bool batched_;
   uint64_t budget_;
   bool keep_;
@@ -91,12 +103,21 @@ class Server {
   size_t next_slot_ = 0;
   double t0_ = 0;
 
+  // progress on stderr every tenth of the stream (TG_SERVE_PROGRESS=1)
+  bool progress_ = false;
+  std::atomic<uint64_t> completed_{0};
+  void note_done(uint64_t k);
+
   double now() const;
   uint64_t footprint(const Request &r) const;
   void exec_update_group(const std::vector<uint64_t> &ids);
   void exec_query_plain(Request &r);
   void exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64_t ver, uint64_t grp);
   QuerySlot &acquire_slot();
+  QuerySlot *free_slot();
+  void launch_group(QuerySlot &sl, std::vector<uint64_t> ids);
+  void reject(Request &r);
+  void run_admission(const std::vector<uint64_t> &order);
   void drain();
 };
 

@@ -274,13 +274,21 @@ def local_triangles(G: Graph, verts) -> np.ndarray:
 REQ_UPDATE, REQ_BFS, REQ_CC, REQ_PR, REQ_LTC = 1, 2, 3, 4, 5
 
 
-class Server:
-    """Request loop over one graph (tg_server_*): batched=False replays the
-    stream one request at a time, batched=True with continuous batching."""
+SERVE_MODES = {"plain": 0, "strict": 1, "admission": 2}
 
-    def __init__(self, G: Graph, batched: bool, mem_budget: int = 1 << 40, keep_results: bool = False):
+
+class Server:
+    """Request loop over one graph (tg_server_*).  mode "plain" replays the
+    stream one request at a time; "strict" batches arrival-ordered runs (each
+    query sees exactly the updates that arrived before it); "admission"
+    pools queries and admits them together on the version current at
+    admission.  batched=True is "strict"."""
+
+    def __init__(self, G: Graph, batched: bool = False, mem_budget: int = 1 << 40, keep_results: bool = False,
+                 mode: str | None = None):
+        m = SERVE_MODES[mode] if mode is not None else int(bool(batched))
         h = ctypes.c_void_p()
-        check(lib().tg_server_create(G.handle, int(batched), int(mem_budget), int(keep_results),
+        check(lib().tg_server_create(G.handle, m, int(mem_budget), int(keep_results),
                                      ctypes.byref(h)))
         self._h, self._g = h, G
         self._keep = []  # host arrays referenced until submit returns

@@ -163,6 +163,63 @@ def test_server_alone_vs_mix():
     plain.close(), batched.close(), G1.close(), G2.close()
 
 
+def oracle_states(keys, stream):
+    """The oracle edge set after each update request, in arrival order."""
+    states = [keys]
+    for t, req in stream:
+        if req[0] == "U":
+            ins = oracle.sort_unique((req[1][0].astype(np.uint64) << np.uint64(32)) | req[1][1])
+            dels = oracle.sort_unique((req[2][0].astype(np.uint64) << np.uint64(32)) | req[2][1])
+            states.append(oracle.set_difference(oracle.set_union(states[-1], ins), dels))
+    return states
+
+
+def test_server_snapshot_at_admission():
+    """Admission mode: every query's result equals the query run alone on
+    the version it reports, and that version holds at least every update
+    that arrived before the query."""
+    n = 1 << 12
+    print("[gpu-test] server snapshot-at-admission", flush=True)
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 53, 0.57, 0.19, 0.19, 50000))
+    rng = np.random.default_rng(13)
+    stream = build_stream(n, keys, rng)
+    # spread arrivals so that groups form at several versions
+    stream = [(i * 4e-4, req) for i, (t, req) in enumerate(stream)]
+    G = make_graph(n, keys)
+    S = tg.Server(G, mode="admission", keep_results=True)
+    ids = submit_all(S, stream)
+    S.run()
+    states = oracle_states(keys, stream)
+    seen_updates = 0
+    versions, groups = set(), {}
+    for (t, req), i in zip(stream, ids):
+        r = S.result(i)
+        assert r["status"] == 0
+        if req[0] == "U":
+            seen_updates += 1
+            assert r["version"] == seen_updates  # updates apply in arrival order
+            continue
+        assert seen_updates <= r["version"] <= len(states) - 1, (req, r, seen_updates)
+        versions.add(r["version"])
+        groups.setdefault(r["group"], set()).add(r["version"])
+        g = oracle.CSR(states[r["version"]], n)
+        if req[0] == "BFS":
+            assert np.array_equal(S.result_data(i, np.uint32), oracle.bfs_levels(g, req[1]))
+        elif req[0] == "CC":
+            assert np.array_equal(S.result_data(i, np.uint32), oracle.cc(g))
+        elif req[0] == "PR":
+            assert oracle.rel_l1(S.result_data(i, np.float64), oracle.pagerank(g, 10)) <= 1e-9
+        else:
+            assert r["value"] == oracle.ltc(g, req[1])
+    # one version per group, and queries were pooled
+    assert all(len(v) == 1 for v in groups.values())
+    assert len(groups) < sum(1 for _, req in stream if req[0] != "U")
+    off, nbr = G.export_adjacency()
+    ref = oracle.CSR(states[-1], n)
+    assert np.array_equal(nbr, ref.nbr[:ref.m]) and np.array_equal(off, ref.off)
+    S.close(), G.close()
+
+
 def test_server_admission_rejects_oversized():
     n = 1 << 12
     print("[gpu-test] server admission", flush=True)
