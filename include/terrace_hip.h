@@ -165,6 +165,9 @@ int tg_frontier_copy(tg_frontier *dst, tg_frontier *src);
  * unreached), identical to tg_bfs's levels for each source (host arrays).
  * tg_ltc: triangles through each of verts[0..q) (host arrays). */
 int tg_bfs_multi(tg_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *levels);
+/* k <= 64 single-source BC runs (BC.h:99-158) in one traversal:
+ * out[j*n + v] = BC of v from srcs[j], NaN where unreachable. */
+int tg_bc_multi(tg_graph *g, const uint32_t *srcs, uint32_t k, double *out);
 int tg_ltc(tg_graph *g, const uint32_t *verts, uint32_t q, uint64_t *counts);
 
 /* ---- request loop (serving) --------------------------------------------

@@ -110,6 +110,7 @@ _SIGS = {
     "tg_vertex_map": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "tg_frontier_copy": (ctypes.c_int, [_P, _P]),
     "tg_bfs_multi": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
+    "tg_bc_multi": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
     "tg_ltc": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
     "tg_server_create": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_P)]),
     "tg_server_destroy": (ctypes.c_int, [_P]),

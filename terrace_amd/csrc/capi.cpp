@@ -517,6 +517,19 @@ int tg_bfs_multi(tg_graph *G, const uint32_t *srcs, uint32_t k, uint32_t *levels
   });
 }
 
+int tg_bc_multi(tg_graph *G, const uint32_t *srcs, uint32_t k, double *out) {
+  return guard([&] {
+    NEED(G && srcs && out, "null argument");
+    NEED(k >= 1 && k <= 64, "k must be in [1, 64]");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    uint32_t n = G->g.num_vertices();
+    hipStream_t st = G->g.stream();
+    G->s_f64.ensure(static_cast<uint64_t>(k) * n, st);
+    G->t.bc_multi(srcs, k, G->s_f64.get());
+    d2h(out, G->s_f64.get(), static_cast<uint64_t>(k) * n * 8, st);
+  });
+}
+
 int tg_ltc(tg_graph *G, const uint32_t *verts, uint32_t q, uint64_t *counts) {
   return guard([&] {
     NEED(G && (q == 0 || (verts && counts)), "null argument");

@@ -9,6 +9,7 @@
 // Sparse outputs are deduplicated through the output bitmap and appended
 // with one atomicAdd per wave.
 #include "traverse.hpp"
+#include "select.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -712,6 +713,95 @@ __device__ inline void wave_foreach_nbr(const GView &g, uint32_t u, F f) {
   }
 }
 
+// ---------------------------------------------------------------- multi-source BC
+// 64 single-source BC computations (BC.h:99-158) share one traversal.  The
+// forward MS-BFS keeps, per level, the list of vertices reached there and
+// their lane masks.  NumPaths and Dependencies are [vertex][64] FP64 so a
+// wave (lane j = source j) reads one vertex's 64 values contiguously.
+//
+// One wave per listed vertex v, lanes j in its mask M: acc_j sums X[u][j]
+// over v's neighbours u in sorted order with bit j of dense[u] set; then
+// Y[v][j] = acc_j (+ add[v][j]).  Each sum runs in sorted neighbour order,
+// the order of the per-source restatement, so the results are bit-exact
+// with single-source BC computed that way.
+__global__ void __launch_bounds__(256) k_msbc_level(GView g, const uint32_t *list, const uint64_t *mask, uint64_t cnt,
+                                                    const uint64_t *dense, const double *X, double *Y,
+                                                    const double *add) {
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  const uint32_t j = threadIdx.x & 63;
+  for (uint64_t w = wave; w < cnt; w += nwaves) {
+    uint32_t v = list[w];
+    uint64_t M = mask[w];
+    bool mine = (M >> j) & 1ull;
+    double acc = 0.0;
+    wave_foreach_nbr(g, v, [&](uint32_t u, bool valid) {
+      uint64_t dm = valid ? dense[u] : 0ull;
+      uint64_t hit = __ballot((dm & M) != 0ull);
+      while (hit) {
+        uint32_t t = __ffsll(static_cast<unsigned long long>(hit)) - 1;
+        hit &= hit - 1;
+        uint32_t ut = __shfl(u, static_cast<int>(t), 64);
+        uint64_t dt = __shfl(dm, static_cast<int>(t), 64);
+        if (mine && ((dt >> j) & 1ull)) acc += X[static_cast<uint64_t>(ut) * 64 + j];
+      }
+    });
+    if (mine) {
+      uint64_t o = static_cast<uint64_t>(v) * 64 + j;
+      Y[o] = add ? acc + add[o] : acc;
+    }
+  }
+}
+
+__global__ void k_msbc_init(const uint32_t *srcs, uint32_t k, uint64_t *vis, uint64_t *fr, double *sigma) {
+  GRID_STRIDE(j, k) {
+    uint32_t s = srcs[j];
+    atomicOr(reinterpret_cast<unsigned long long *>(&vis[s]), 1ull << j);
+    atomicOr(reinterpret_cast<unsigned long long *>(&fr[s]), 1ull << j);
+    sigma[static_cast<uint64_t>(s) * 64 + j] = 1.0;
+  }
+}
+
+// Frontier step without level output (k_ms_finalize without levels).
+__global__ void k_msbc_finalize(uint32_t n, uint64_t *vis, uint64_t *fr, uint64_t *nx, uint32_t *obits) {
+  GRID_STRIDE(i, n) {
+    uint64_t nw = nx[i] & ~vis[i];
+    nx[i] = 0;
+    fr[i] = nw;
+    if (nw) {
+      vis[i] |= nw;
+      bit_set(obits, static_cast<uint32_t>(i));
+    }
+  }
+}
+
+struct SelMask {
+  const uint64_t *fr;
+  uint32_t *list;
+  uint64_t *mask;
+  __device__ bool pred(uint64_t i) const { return fr[i] != 0ull; }
+  __device__ void emit(uint64_t i, uint64_t pos) const {
+    list[pos] = static_cast<uint32_t>(i);
+    mask[pos] = fr[i];
+  }
+};
+
+__global__ void k_scatter_masks(const uint32_t *list, const uint64_t *mask, uint64_t cnt, uint64_t *dense, int clear) {
+  GRID_STRIDE(i, cnt) dense[list[i]] = clear ? 0ull : mask[i];
+}
+
+__global__ void k_recip_all(double *p, uint64_t n) { GRID_STRIDE(i, n) p[i] = 1 / p[i]; }
+
+// out[j][v] = (dep - inv) / inv (BC.h:140-142), lane-major for the caller.
+__global__ void k_msbc_final(const double *dep, const double *inv, uint32_t n, uint32_t k, double *out) {
+  GRID_STRIDE(i, static_cast<uint64_t>(n) * k) {
+    uint64_t j = i / n, v = i - j * n;
+    uint64_t o = v * 64 + j;
+    out[i] = (dep[o] - inv[o]) / inv[o];
+  }
+}
+
+
 // One wave per queried vertex: its sorted neighbour list into buf[off[q]..].
 __global__ void __launch_bounds__(256) k_gather_lists(GView g, const uint32_t *verts, uint32_t q, const uint64_t *off,
                                                       uint32_t *buf) {
@@ -1331,6 +1421,110 @@ uint32_t Traversal::bfs_multi(const uint32_t *srcs_host, uint32_t k, uint32_t *l
   }
   TG_HIP(hipStreamSynchronize(s));
   return depth;
+}
+
+// ---------------------------------------------------------------- multi-source BC
+void Traversal::bc_multi(const uint32_t *srcs_host, uint32_t k, double *out, uint32_t threshold) {
+  hipStream_t s = stream_;
+  uint32_t n = g_.num_vertices();
+  if (k == 0 || k > 64) throw Error(-1, "bc_multi: 1..64 sources");
+  std::vector<uint32_t> hs(srcs_host, srcs_host + k);
+  for (uint32_t v : hs)
+    if (v >= n) throw Error(-1, "source out of range");
+  const uint64_t nl = static_cast<uint64_t>(n) * 64;
+  ms_vis_.ensure(n, s);
+  ms_fr_.ensure(n, s);
+  ms_nx_.ensure(n, s);
+  bc_pm_.ensure(n, s);
+  bc_sigma_.ensure(nl, s);
+  bc_dep_.ensure(nl, s);
+  fill_u64(ms_vis_.get(), 0, n, s);
+  fill_u64(ms_fr_.get(), 0, n, s);
+  fill_u64(ms_nx_.get(), 0, n, s);
+  fill_f64(bc_sigma_.get(), 0.0, nl, s);
+  DevBuf<uint32_t> dsrc;
+  dsrc.ensure(k, s);
+  TG_HIP(hipMemcpyAsync(dsrc.get(), hs.data(), k * 4, hipMemcpyHostToDevice, s));
+  k_msbc_init<<<1, 64, 0, s>>>(dsrc.get(), k, ms_vis_.get(), ms_fr_.get(), bc_sigma_.get());
+  TG_CHECK_LAUNCH();
+  // level 0: the sources with their lane masks
+  std::vector<uint64_t> loff{0};
+  bc_list_.ensure(n, s);
+  bc_mask_.ensure(n, s);
+  DevBuf<uint64_t> cntd;
+  cntd.ensure(1, s);
+  select(SelMask{ms_fr_.get(), bc_list_.get(), bc_mask_.get()}, n, cntd.get(), scan_, s);
+  loff.push_back(read_scalar(cntd.get(), s));
+  Frontier a, b;
+  a.n = n;
+  a.queue.ensure(loff[1] + 1, s);
+  TG_HIP(hipMemcpyAsync(a.queue.get(), bc_list_.get(), loff[1] * 4, hipMemcpyDeviceToDevice, s));
+  a.count = loff[1];
+  a.has_queue = true;
+  a.has_bits = false;
+  Frontier *cur = &a, *nxt = &b;
+  uint64_t full = k == 64 ? ~0ull : ((1ull << k) - 1);
+  // forward: MS-BFS levels, NumPaths of each new level from the previous one
+  while (true) {
+    copy_bytes(bc_pm_.get(), ms_fr_.get(), static_cast<uint64_t>(n) * 8, s);  // masks of the last level
+    bool dense = (n / (threshold ? threshold : 1)) <= cur->count;
+    if (dense) {
+      const Graph::TierLists &L = g_.tier_lists();
+      prepare_tree_tasks();
+      PullMS p{ms_fr_.get(), ms_nx_.get(), ms_vis_.get(), full};
+      launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+    } else {
+      to_queue(*cur);
+      PushMS p{ms_fr_.get(), ms_nx_.get(), ms_vis_.get()};
+      launch_push(*this, g_, p, *cur, nullptr, cnt_, sec_pma_, sec_tree_);
+    }
+    frontier_empty(*nxt);
+    nxt->has_queue = false;
+    k_msbc_finalize<<<grid_for(n, 256), 256, 0, s>>>(n, ms_vis_.get(), ms_fr_.get(), ms_nx_.get(), nxt->bits.get());
+    TG_CHECK_LAUNCH();
+    nxt->count = popcount_bits(*nxt);
+    if (nxt->count == 0) break;
+    uint64_t base = loff.back();
+    bc_list_.ensure(base + nxt->count + 1, s, true, base);
+    bc_mask_.ensure(base + nxt->count + 1, s, true, base);
+    select(SelMask{ms_fr_.get(), bc_list_.get() + base, bc_mask_.get() + base}, n, cntd.get(), scan_, s);
+    loff.push_back(base + nxt->count);
+    k_msbc_level<<<grid_for(nxt->count * 64, 256, 1u << 18), 256, 0, s>>>(
+        g_.view(), bc_list_.get() + base, bc_mask_.get() + base, nxt->count, bc_pm_.get(), bc_sigma_.get(),
+        bc_sigma_.get(), nullptr);
+    TG_CHECK_LAUNCH();
+    std::swap(cur, nxt);
+  }
+  // NumPaths -> 1 / NumPaths (BC.h:126-128); unreached lanes become inf
+  k_recip_all<<<grid_for(nl, 256, 1u << 18), 256, 0, s>>>(bc_sigma_.get(), nl);
+  TG_CHECK_LAUNCH();
+  // backward, deepest level first: dense holds the masks of level d + 1
+  fill_f64(bc_dep_.get(), 0.0, nl, s);
+  fill_u64(bc_pm_.get(), 0, n, s);
+  size_t nlev = loff.size() - 1;
+  for (size_t d = nlev; d-- > 0;) {
+    uint64_t c = loff[d + 1] - loff[d];
+    if (d + 1 < nlev) {
+      uint64_t c1 = loff[d + 2] - loff[d + 1];
+      k_scatter_masks<<<grid_for(c1, 256), 256, 0, s>>>(bc_list_.get() + loff[d + 1], bc_mask_.get() + loff[d + 1],
+                                                        c1, bc_pm_.get(), 0);
+      TG_CHECK_LAUNCH();
+    }
+    k_msbc_level<<<grid_for(c * 64, 256, 1u << 18), 256, 0, s>>>(g_.view(), bc_list_.get() + loff[d],
+                                                                  bc_mask_.get() + loff[d], c, bc_pm_.get(),
+                                                                  bc_dep_.get(), bc_dep_.get(), bc_sigma_.get());
+    TG_CHECK_LAUNCH();
+    if (d + 1 < nlev) {
+      uint64_t c1 = loff[d + 2] - loff[d + 1];
+      k_scatter_masks<<<grid_for(c1, 256), 256, 0, s>>>(bc_list_.get() + loff[d + 1], bc_mask_.get() + loff[d + 1],
+                                                        c1, bc_pm_.get(), 1);
+      TG_CHECK_LAUNCH();
+    }
+  }
+  k_msbc_final<<<grid_for(static_cast<uint64_t>(n) * k, 256, 1u << 18), 256, 0, s>>>(bc_dep_.get(), bc_sigma_.get(),
+                                                                                     n, k, out);
+  TG_CHECK_LAUNCH();
+  TG_HIP(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------- local triangle count

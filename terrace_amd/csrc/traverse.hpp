@@ -74,6 +74,9 @@ class Traversal {
   uint64_t tc();
   // 64 BFS in one traversal: levels[j*n + v] for source j (k <= 64).
   uint32_t bfs_multi(const uint32_t *srcs_host, uint32_t k, uint32_t *levels, uint32_t threshold = 20);
+  // 64 BC in one traversal: out[j*n + v] for source j (k <= 64), each
+  // equal to bc(srcs[j]) summed in sorted neighbour order.
+  void bc_multi(const uint32_t *srcs_host, uint32_t k, double *out, uint32_t threshold = 20);
   // Triangles through each queried vertex (host in / host out).
   void ltc(const uint32_t *verts_host, uint32_t q, uint64_t *out_host);
   // Order-independent digest of a device array (u32 or u64 words).
@@ -105,6 +108,9 @@ class Traversal {
   DevBuf<uint32_t> csr_nbr_;
   DevBuf<uint64_t> red_;
   DevBuf<uint64_t> ms_vis_, ms_fr_, ms_nx_;
+  DevBuf<uint64_t> bc_pm_, bc_mask_;  // multi-source BC: dense masks, per-level masks
+  DevBuf<uint32_t> bc_list_;          // per-level vertex lists
+  DevBuf<double> bc_sigma_, bc_dep_;  // [vertex][64]
   uint64_t ntree_tasks_ = 0;
   const uint32_t *tree_list_ = nullptr;
   uint32_t ntree_list_ = 0;

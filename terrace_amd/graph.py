@@ -263,6 +263,15 @@ def bfs_multi(G: Graph, sources) -> np.ndarray:
     return out
 
 
+def bc_multi(G: Graph, sources) -> np.ndarray:
+    """Betweenness from up to 64 sources in one traversal: array [k, n]
+    (NaN where a vertex is unreachable from that source)."""
+    srcs = _u32(sources)
+    out = np.empty((len(srcs), G.get_num_vertices()), np.float64)
+    check(lib().tg_bc_multi(G.handle, ptr(srcs), len(srcs), ptr(out)))
+    return out
+
+
 def local_triangles(G: Graph, verts) -> np.ndarray:
     """Triangles through each vertex of verts."""
     v = _u32(verts)

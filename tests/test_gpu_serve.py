@@ -48,6 +48,36 @@ def test_bfs_multi_matches_single(skew):
     G.close()
 
 
+@pytest.mark.parametrize("skew", [False, True])
+def test_bc_multi_matches_single(skew):
+    """64 BC sources in one traversal: every lane equals the single-source
+    restatement bit for bit (sums in sorted neighbour order, NaN where
+    unreachable) and the single-source GPU BC within 1e-9."""
+    n = 1 << 12
+    a, b, c = (0.57, 0.19, 0.19) if skew else (0.5, 0.1, 0.1)
+    print("[gpu-test] bc_multi", skew, flush=True)
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 59, a, b, c, 60000))
+    G = make_graph(n, keys)
+    g = oracle.CSR(keys, n)
+    rng = np.random.default_rng(17)
+    srcs = rng.integers(0, n, 64).astype(np.uint32)
+    srcs[1] = srcs[0]
+    srcs[2] = int(np.argmax(g.degrees()))
+    srcs[3] = int(np.argmin(g.degrees()))  # possibly isolated
+    got = tg.bc_multi(G, srcs)
+    for j in range(len(srcs)):
+        want = oracle.bc(g, int(srcs[j]))
+        np.testing.assert_array_equal(got[j], want, err_msg=f"lane {j}")
+    for j in (0, 2, 5):
+        single = tg.BC(G, int(srcs[j]))
+        fin = np.isfinite(single)
+        assert np.array_equal(fin, np.isfinite(got[j]))
+        assert oracle.rel_l1(got[j][fin], single[fin]) <= 1e-9
+    # fewer lanes
+    np.testing.assert_array_equal(tg.bc_multi(G, srcs[:5]), got[:5])
+    G.close()
+
+
 def test_local_triangles_match_oracle():
     n = 1 << 12
     print("[gpu-test] ltc", flush=True)
