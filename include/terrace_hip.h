@@ -168,6 +168,29 @@ int tg_bfs_multi(tg_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *levels
 /* k <= 64 single-source BC runs (BC.h:99-158) in one traversal:
  * out[j*n + v] = BC of v from srcs[j], NaN where unreachable. */
 int tg_bc_multi(tg_graph *g, const uint32_t *srcs, uint32_t k, double *out);
+/* Device-output forms: levels / out are device arrays of k*n entries. */
+int tg_bfs_multi_dev(tg_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *levels);
+int tg_bc_multi_dev(tg_graph *g, const uint32_t *srcs, uint32_t k, double *out);
+/* Coalesced updates: ops[i] = 0 insert / 1 delete of (src[i], dst[i]), in
+ * order; one device pass whose result equals applying them one by one (the
+ * last op on an edge wins).  Host arrays.  No counterpart in the reference
+ * (its drivers call add_edge_batch / remove_edge per batch,
+ * src/graph_bm.cc:363-378). */
+int tg_apply_ops(tg_graph *g, const uint32_t *src, const uint32_t *dst, const uint32_t *ops, uint64_t count);
+
+/* ---- autotuner ---------------------------------------------------------
+ * Sweeps the edgeMap push/pull threshold (dense when n / t <= |frontier|,
+ * include/Map.cpp:66-74, reference t = 20) of BFS, multi-source BFS and CC
+ * and the block size of the inline / PMA pull kernels on the current graph;
+ * keeps the fastest for later whole-algorithm calls on this handle.
+ * report[12]: thr_bfs, thr_ms, thr_cc, pull_block, then (reference ms,
+ * tuned ms) for BFS (3 sources), MS-BFS (64 sources), CC and PageRank (3
+ * iterations).  Results never depend on the tuning.
+ * tuning[5]: thr_bfs, thr_ms, thr_cc, pull_block (64/128/256/512), pr_graph
+ * (1 = PageRank iterations replayed from a captured HIP graph). */
+int tg_autotune(tg_graph *g, double *report);
+int tg_get_tuning(tg_graph *g, uint32_t *tuning);
+int tg_set_tuning(tg_graph *g, const uint32_t *tuning);
 int tg_ltc(tg_graph *g, const uint32_t *verts, uint32_t q, uint64_t *counts);
 
 /* ---- request loop (serving) --------------------------------------------

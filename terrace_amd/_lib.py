@@ -111,6 +111,12 @@ _SIGS = {
     "tg_frontier_copy": (ctypes.c_int, [_P, _P]),
     "tg_bfs_multi": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
     "tg_bc_multi": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
+    "tg_bfs_multi_dev": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
+    "tg_bc_multi_dev": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
+    "tg_apply_ops": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_uint64]),
+    "tg_autotune": (ctypes.c_int, [_P, _P]),
+    "tg_get_tuning": (ctypes.c_int, [_P, _P]),
+    "tg_set_tuning": (ctypes.c_int, [_P, _P]),
     "tg_ltc": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P]),
     "tg_server_create": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_P)]),
     "tg_server_destroy": (ctypes.c_int, [_P]),

@@ -517,6 +517,73 @@ int tg_bfs_multi(tg_graph *G, const uint32_t *srcs, uint32_t k, uint32_t *levels
   });
 }
 
+int tg_autotune(tg_graph *G, double *report) {
+  return guard([&] {
+    NEED(G && report, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->t.autotune(report);
+  });
+}
+
+int tg_get_tuning(tg_graph *G, uint32_t *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    const tg::Graph::Tuning &T = G->g.tune;
+    out[0] = T.thr_bfs;
+    out[1] = T.thr_ms;
+    out[2] = T.thr_cc;
+    out[3] = T.pull_block;
+    out[4] = T.pr_graph;
+  });
+}
+
+int tg_set_tuning(tg_graph *G, const uint32_t *in) {
+  return guard([&] {
+    NEED(G && in, "null argument");
+    NEED(in[0] >= 1 && in[1] >= 1 && in[2] >= 1, "thresholds must be >= 1");
+    NEED(in[3] == 64 || in[3] == 128 || in[3] == 256 || in[3] == 512, "pull block must be 64, 128, 256 or 512");
+    NEED(in[4] <= 1, "pr_graph must be 0 or 1");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->g.tune = tg::Graph::Tuning{in[0], in[1], in[2], in[3], in[4]};
+  });
+}
+
+int tg_bfs_multi_dev(tg_graph *G, const uint32_t *srcs, uint32_t k, uint32_t *levels) {
+  return guard([&] {
+    NEED(G && srcs && levels, "null argument");
+    NEED(k >= 1 && k <= 64, "k must be in [1, 64]");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->t.bfs_multi(srcs, k, levels);
+  });
+}
+
+int tg_bc_multi_dev(tg_graph *G, const uint32_t *srcs, uint32_t k, double *out) {
+  return guard([&] {
+    NEED(G && srcs && out, "null argument");
+    NEED(k >= 1 && k <= 64, "k must be in [1, 64]");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->t.bc_multi(srcs, k, out);
+  });
+}
+
+int tg_apply_ops(tg_graph *G, const uint32_t *src, const uint32_t *dst, const uint32_t *ops, uint64_t count) {
+  return guard([&] {
+    NEED(G && (count == 0 || (src && dst && ops)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    if (count == 0) return;
+    for (uint64_t i = 0; i < count; i++) NEED(ops[i] <= 1, "op must be 0 (insert) or 1 (delete)");
+    hipStream_t st = G->g.stream();
+    G->s_src.ensure(count, st);
+    G->s_dst.ensure(count, st);
+    G->s_u32.ensure(count, st);
+    TG_HIP(hipMemcpyAsync(G->s_src.get(), src, count * 4, hipMemcpyHostToDevice, st));
+    TG_HIP(hipMemcpyAsync(G->s_dst.get(), dst, count * 4, hipMemcpyHostToDevice, st));
+    TG_HIP(hipMemcpyAsync(G->s_u32.get(), ops, count * 4, hipMemcpyHostToDevice, st));
+    G->g.apply_ops(G->s_src.get(), G->s_dst.get(), G->s_u32.get(), count);
+    TG_HIP(hipStreamSynchronize(st));
+  });
+}
+
 int tg_bc_multi(tg_graph *G, const uint32_t *srcs, uint32_t k, double *out) {
   return guard([&] {
     NEED(G && srcs && out, "null argument");

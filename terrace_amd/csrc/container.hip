@@ -1683,6 +1683,7 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
   if (nR == 0) return;
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
   lists_dirty_ = true;
+  ++epoch_;
   GView g = view();
   // existence + new-key prefix
   f0_.ensure(nR + 1, stream_);
@@ -1799,6 +1800,7 @@ void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
   if (nR0 == 0) return;
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
   lists_dirty_ = true;
+  ++epoch_;
   GView g = view();
   // keep the keys that exist (absent deletes are no-ops)
   k2_.ensure(nR0 + 1, stream_);
@@ -2016,6 +2018,7 @@ void Graph::copy_from(Graph &src, bool query_only) {
   }
   tree_dirty_ = false;
   read_only_ = query_only;
+  tune = src.tune;
   ensure_blocks(src.blk_top_);
   ensure_index(src.idx_top_);
   ensure_trees(src.ntrees_ + 1);
@@ -2030,6 +2033,7 @@ void Graph::copy_from(Graph &src, bool query_only) {
   idx_top_ = src.idx_top_;
   ntrees_ = src.ntrees_;
   lists_dirty_ = true;
+  ++epoch_;
   TG_HIP(hipStreamSynchronize(s));
 }
 

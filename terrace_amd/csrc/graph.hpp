@@ -110,6 +110,22 @@ class Graph {
   };
   const TierLists &tier_lists();
 
+  // Mutation counter: changes whenever the adjacency may have changed.
+  uint64_t epoch() const { return epoch_; }
+
+  // Launch parameters chosen by the autotuner; the whole-algorithm calls
+  // use them.  thr_*: edgeMap push/pull switch (dense when n / thr <=
+  // |frontier|, include/Map.cpp:66-74, reference value 20); pull_block:
+  // workgroup size of the inline and PMA pull kernels.  Results do not
+  // depend on them.
+  // pr_graph: PageRank iterations replayed from a captured HIP graph.
+  struct Tuning {
+    uint32_t thr_bfs = 20, thr_ms = 20, thr_cc = 20;
+    uint32_t pull_block = 256;
+    uint32_t pr_graph = 1;
+  };
+  Tuning tune;
+
   std::mutex mu;  // serialises calls on one handle
   ScanScratch scan_scratch;
 
@@ -138,6 +154,7 @@ class Graph {
 
   // tier lists
   bool lists_dirty_ = true;
+  uint64_t epoch_ = 0;
   TierLists lists_;
   DevBuf<uint32_t> list_pma_, list_tree_;
 

@@ -12,6 +12,7 @@
 #include "select.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 
 namespace tg {
@@ -78,6 +79,7 @@ struct PullBFS {
   uint32_t *obits;
   using Acc = uint32_t;
   static constexpr bool kEarly = true;
+  static constexpr bool kOrdered = false;
   static constexpr const char *kName = "bfs";
   __device__ static Acc ident() { return 0xFFFFFFFFu; }
   __device__ bool skip(uint32_t i) const { return parent[i] != -1; }
@@ -106,6 +108,7 @@ struct PullCC {
   uint32_t *ids;
   using Acc = uint32_t;
   static constexpr bool kEarly = false;
+  static constexpr bool kOrdered = false;
   static constexpr const char *kName = "cc";
   __device__ static Acc ident() { return 0xFFFFFFFFu; }
   __device__ bool skip(uint32_t) const { return false; }
@@ -140,6 +143,7 @@ struct PullSum {
   bool assign;            // PR: the first pass assigns instead of adding
   using Acc = SumAcc;
   static constexpr bool kEarly = false;
+  static constexpr bool kOrdered = true;  // FP64: tree partials summed in leaf order
   static constexpr const char *kName = "sum";
   __device__ static Acc ident() { return SumAcc{0.0, 0}; }
   __device__ bool skip(uint32_t i) const { return visited != nullptr && visited[i]; }
@@ -181,7 +185,7 @@ __device__ inline void load_vblock(const VBlock *vb, uint32_t v, uint32_t (&w)[1
 }
 
 template <class Op>
-__global__ void __launch_bounds__(256) k_pull_inline(GView g, Op op) {
+__global__ void __launch_bounds__(512) k_pull_inline(GView g, Op op) {
   GRID_STRIDE(i, g.n) {
     uint32_t v = static_cast<uint32_t>(i);
     if (op.skip(v)) continue;
@@ -207,7 +211,7 @@ __global__ void __launch_bounds__(256) k_pull_inline(GView g, Op op) {
 // load, so a group reads 64 slots (256 bytes) per step and a wave works on
 // four vertices at once.
 template <class Op>
-__global__ void __launch_bounds__(256) k_pull_pma(GView g, Op op, const uint32_t *list, uint32_t cnt) {
+__global__ void __launch_bounds__(512) k_pull_pma(GView g, Op op, const uint32_t *list, uint32_t cnt) {
   const uint32_t lane = threadIdx.x & 63, gl = lane & 15;
   const uint64_t gmask = 0xFFFFull << (lane & 48);
   const uint64_t lt = lanemask_lt() & gmask;
@@ -254,9 +258,12 @@ __global__ void __launch_bounds__(256) k_pull_pma(GView g, Op op, const uint32_t
 }
 
 // One workgroup per (tree vertex, leaf block) task.
+// parts != null: the task's partial goes to parts[t] and k_tree_parts adds
+// the partials of each vertex in leaf order (deterministic FP64 sums).
 template <class Op>
 __global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_t *list, uint32_t cnt,
-                                                   const uint64_t *task_off, uint64_t ntasks) {
+                                                   const uint64_t *task_off, uint64_t ntasks,
+                                                   typename Op::Acc *parts) {
   __shared__ typename Op::Acc part[4];
   for (uint64_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
     // vertex of task t: last list index with task_off <= t
@@ -284,9 +291,23 @@ __global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_
     if (threadIdx.x == 0) {
       typename Op::Acc r = part[0];
       for (int w = 1; w < 4; w++) r = Op::combine(r, part[w]);
-      op.commit(v, r, kShared);
+      if (parts) parts[t] = r;
+      else op.commit(v, r, kShared);
     }
     __syncthreads();
+  }
+}
+
+// One thread per tree vertex: its task partials in leaf order, one commit.
+template <class Op>
+__global__ void k_tree_parts(Op op, const uint32_t *list, uint32_t cnt, const uint64_t *task_off,
+                             const typename Op::Acc *parts) {
+  GRID_STRIDE(li, cnt) {
+    uint32_t v = list[li];
+    if (op.skip(v)) continue;
+    typename Op::Acc r = Op::ident();
+    for (uint64_t t = task_off[li]; t < task_off[li + 1]; t++) r = Op::combine(r, parts[t]);
+    op.commit(v, r, kExcl);
   }
 }
 
@@ -607,6 +628,7 @@ struct PullMS {
   uint64_t full;
   using Acc = uint64_t;
   static constexpr bool kEarly = false;
+  static constexpr bool kOrdered = false;
   static constexpr const char *kName = "ms";
   __device__ static Acc ident() { return 0ull; }
   __device__ bool skip(uint32_t i) const { return (vis[i] & full) == full; }
@@ -1040,6 +1062,8 @@ uint64_t Traversal::popcount_bits(const Frontier &f) {
 
 void Traversal::prepare_tree_tasks() {
   const Graph::TierLists &L = g_.tier_lists();
+  if (tasks_epoch_ == g_.epoch() && tree_list_ == L.tree) return;  // unchanged graph
+  tasks_epoch_ = g_.epoch();
   hipStream_t s = stream_;
   tree_list_ = L.tree;
   ntree_list_ = L.ntree;
@@ -1063,17 +1087,29 @@ static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierL
     for (auto &x : e) TG_HIP(hipEventCreate(&x));
     TG_HIP(hipEventRecord(e[0], s));
   }
-  k_pull_inline<Op><<<grid_for(v.n, 256), 256, 0, s>>>(v, op);
+  const unsigned B = g.tune.pull_block;
+  k_pull_inline<Op><<<grid_for(v.n, B), B, 0, s>>>(v, op);
   TG_CHECK_LAUNCH();
   if (T.profiling) TG_HIP(hipEventRecord(e[1], s));
   if (L.npma) {
-    k_pull_pma<Op><<<grid_for(static_cast<uint64_t>(L.npma) * 16, 256), 256, 0, s>>>(v, op, L.pma, L.npma);
+    k_pull_pma<Op><<<grid_for(static_cast<uint64_t>(L.npma) * 16, B), B, 0, s>>>(v, op, L.pma, L.npma);
     TG_CHECK_LAUNCH();
   }
   if (T.profiling) TG_HIP(hipEventRecord(e[2], s));
   if (ntasks) {
-    k_pull_tree<Op><<<grid_for(ntasks * 256, 256, 1u << 20), 256, 0, s>>>(v, op, tree_list, ntree, task_off, ntasks);
+    typename Op::Acc *parts = nullptr;
+    if (Op::kOrdered) {
+      DevBuf<uint8_t> &pb = T.tree_parts();
+      pb.ensure(ntasks * sizeof(typename Op::Acc), s);
+      parts = reinterpret_cast<typename Op::Acc *>(pb.get());
+    }
+    k_pull_tree<Op><<<grid_for(ntasks * 256, 256, 1u << 20), 256, 0, s>>>(v, op, tree_list, ntree, task_off, ntasks,
+                                                                          parts);
     TG_CHECK_LAUNCH();
+    if (parts) {
+      k_tree_parts<Op><<<grid_for(ntree, 256), 256, 0, s>>>(op, tree_list, ntree, task_off, parts);
+      TG_CHECK_LAUNCH();
+    }
   }
   if (T.profiling) {
     TG_HIP(hipEventRecord(e[3], s));
@@ -1211,6 +1247,7 @@ void Traversal::edge_map(Frontier &in, Op op, const OpState &st, Frontier *out, 
 
 // ---------------------------------------------------------------- BFS
 uint32_t Traversal::bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t threshold) {
+  if (!threshold) threshold = g_.tune.thr_bfs;
   hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   if (src >= n) throw Error(-1, "source out of range");
@@ -1246,13 +1283,45 @@ void Traversal::pagerank(uint32_t iters, double *out) {
   fill_f64(p, 1 / static_cast<double>(n), n, s);
   Frontier all;
   frontier_all(all);
-  for (uint32_t it = 0; it < iters; it++) {
-    k_pr_norm<<<grid_for(n, 256), 256, 0, s>>>(g_.view(), p, q);
+  // one iteration (PR_S, include/Pagerank.h:142-170): p / degree into q,
+  // then p_next[i] = sum of q over i's neighbours
+  auto iterate = [&](const double *from, double *to) {
+    k_pr_norm<<<grid_for(n, 256), 256, 0, s>>>(g_.view(), from, q);
     TG_CHECK_LAUNCH();
     OpState st;
     st.src_val = q;
-    st.val = pn;
+    st.val = to;
     pull(all, Op::PR, st, nullptr);
+  };
+  uint32_t done = 0;
+  if (iters) {
+    iterate(p, pn);  // also builds the tier lists and tree tasks
+    std::swap(p, pn);
+    done = 1;
+  }
+  // the rest as a captured pair of iterations (p -> pn -> p), replayed;
+  // the pair holds kernel nodes only (no memset or copy nodes)
+  if (g_.tune.pr_graph && !profiling && iters - done >= 2) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    TG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    iterate(p, pn);
+    iterate(pn, p);
+    TG_HIP(hipStreamEndCapture(s, &graph));
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      TG_HIP(e);
+    }
+    uint32_t reps = (iters - done) / 2;
+    for (uint32_t r = 0; r < reps; r++) TG_HIP(hipGraphLaunch(exec, s));
+    done += 2 * reps;
+    TG_HIP(hipStreamSynchronize(s));
+    TG_HIP(hipGraphExecDestroy(exec));
+    TG_HIP(hipGraphDestroy(graph));
+  }
+  for (; done < iters; done++) {
+    iterate(p, pn);
     std::swap(p, pn);
   }
   if (p != out) TG_HIP(hipMemcpyAsync(out, p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1261,6 +1330,7 @@ void Traversal::pagerank(uint32_t iters, double *out) {
 
 // ---------------------------------------------------------------- CC
 uint32_t Traversal::cc(uint32_t *ids, uint32_t threshold) {
+  if (!threshold) threshold = g_.tune.thr_cc;
   hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   utmp_.ensure(n + 1, s);
@@ -1301,6 +1371,7 @@ uint32_t Traversal::cc(uint32_t *ids, uint32_t threshold) {
 
 // ---------------------------------------------------------------- BC
 void Traversal::bc(uint32_t src, double *out, uint32_t threshold) {
+  if (!threshold) threshold = g_.tune.thr_bfs;
   hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   if (src >= n) throw Error(-1, "source out of range");
@@ -1366,6 +1437,7 @@ void Traversal::bc(uint32_t src, double *out, uint32_t threshold) {
 
 // ---------------------------------------------------------------- multi-source BFS
 uint32_t Traversal::bfs_multi(const uint32_t *srcs_host, uint32_t k, uint32_t *levels, uint32_t threshold) {
+  if (!threshold) threshold = g_.tune.thr_ms;
   hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   if (k == 0 || k > 64) throw Error(-1, "bfs_multi: 1..64 sources");
@@ -1425,6 +1497,7 @@ uint32_t Traversal::bfs_multi(const uint32_t *srcs_host, uint32_t k, uint32_t *l
 
 // ---------------------------------------------------------------- multi-source BC
 void Traversal::bc_multi(const uint32_t *srcs_host, uint32_t k, double *out, uint32_t threshold) {
+  if (!threshold) threshold = g_.tune.thr_ms;
   hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   if (k == 0 || k > 64) throw Error(-1, "bc_multi: 1..64 sources");
@@ -1525,6 +1598,97 @@ void Traversal::bc_multi(const uint32_t *srcs_host, uint32_t k, double *out, uin
                                                                                      n, k, out);
   TG_CHECK_LAUNCH();
   TG_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- autotuner
+namespace {
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+void Traversal::autotune(double *report) {
+  hipStream_t s = stream_;
+  const uint32_t n = g_.num_vertices();
+  if (n == 0) throw Error(-1, "autotune on an empty graph");
+  // sources: the highest-degree vertex and vertices with edges spread over
+  // the id range (deterministic)
+  std::vector<uint32_t> deg(n);
+  utmp_.ensure(n, s);
+  g_.degrees(utmp_.get());
+  TG_HIP(hipMemcpyAsync(deg.data(), utmp_.get(), n * 4ull, hipMemcpyDeviceToHost, s));
+  TG_HIP(hipStreamSynchronize(s));
+  uint32_t hub = static_cast<uint32_t>(std::max_element(deg.begin(), deg.end()) - deg.begin());
+  std::vector<uint32_t> srcs{hub};
+  for (uint32_t q = 1; q <= 2; q++) {
+    uint32_t v = static_cast<uint32_t>((static_cast<uint64_t>(n) * q) / 3);
+    while (v < n && deg[v] == 0) v++;
+    if (v < n) srcs.push_back(v);
+  }
+  std::vector<uint32_t> ms(64);
+  for (uint32_t j = 0; j < 64; j++) {
+    uint32_t v = static_cast<uint32_t>((static_cast<uint64_t>(n) * (2 * j + 1)) / 128);
+    while (v < n && deg[v] == 0) v++;
+    ms[j] = v < n ? v : hub;
+  }
+  DevBuf<int32_t> par;
+  DevBuf<uint32_t> lev, ids;
+  DevBuf<double> pr;
+  par.ensure(n, s);
+  lev.ensure(static_cast<uint64_t>(n) * 64, s);
+  ids.ensure(n, s);
+  pr.ensure(n, s);
+  auto best_of = [&](auto &&run) {  // fastest of 2 runs after a warm-up
+    run();
+    double b = 1e300;
+    for (int r = 0; r < 2; r++) {
+      double t0 = now_s();
+      run();
+      b = std::min(b, now_s() - t0);
+    }
+    return b * 1e3;
+  };
+  const uint32_t thr[] = {5, 10, 20, 40, 80, 160, 320};
+  Graph::Tuning &T = g_.tune;
+  T = Graph::Tuning{};
+  // pull block size on PageRank (every iteration is a full pull)
+  double pr_ref = best_of([&] { pagerank(3, pr.get()); });
+  double pr_best = pr_ref;
+  uint32_t bb = 256;
+  for (uint32_t B : {128u, 512u}) {
+    T.pull_block = B;
+    double t = best_of([&] { pagerank(3, pr.get()); });
+    if (t < pr_best * 0.97) {  // a clear win only; ties keep 256
+      pr_best = t;
+      bb = B;
+    }
+  }
+  T.pull_block = bb;
+  auto sweep = [&](uint32_t &slot, auto &&run) {
+    slot = 20;
+    double ref = best_of(run), best = ref;
+    uint32_t bt = 20;
+    for (uint32_t t : thr) {
+      if (t == 20) continue;
+      slot = t;
+      double x = best_of(run);
+      if (x < best * 0.97) {
+        best = x;
+        bt = t;
+      }
+    }
+    slot = bt;
+    return std::make_pair(ref, best);
+  };
+  auto rb = sweep(T.thr_bfs, [&] {
+    for (uint32_t v : srcs) bfs(v, par.get(), nullptr);
+  });
+  auto rm = sweep(T.thr_ms, [&] { bfs_multi(ms.data(), 64, lev.get()); });
+  auto rc = sweep(T.thr_cc, [&] { cc(ids.get()); });
+  double out[12] = {static_cast<double>(T.thr_bfs), static_cast<double>(T.thr_ms), static_cast<double>(T.thr_cc),
+                    static_cast<double>(T.pull_block), rb.first, rb.second, rm.first, rm.second, rc.first,
+                    rc.second, pr_ref, pr_best};
+  if (report) std::copy(out, out + 12, report);
 }
 
 // ---------------------------------------------------------------- local triangle count

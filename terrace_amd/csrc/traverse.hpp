@@ -66,19 +66,25 @@ class Traversal {
   void vertex_map(Frontier &f, int vop, const VArgs &a);
   void copy_frontier(Frontier &dst, Frontier &src);
 
-  // Whole algorithms (device result buffers).
-  uint32_t bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t threshold = 20);
+  // Whole algorithms (device result buffers).  threshold 0 = the graph's
+  // tuned value (Graph::tune).
+  uint32_t bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t threshold = 0);
   void pagerank(uint32_t iters, double *out);
-  uint32_t cc(uint32_t *ids, uint32_t threshold = 20);
-  void bc(uint32_t src, double *out, uint32_t threshold = 20);
+  uint32_t cc(uint32_t *ids, uint32_t threshold = 0);
+  void bc(uint32_t src, double *out, uint32_t threshold = 0);
   uint64_t tc();
   // 64 BFS in one traversal: levels[j*n + v] for source j (k <= 64).
-  uint32_t bfs_multi(const uint32_t *srcs_host, uint32_t k, uint32_t *levels, uint32_t threshold = 20);
+  uint32_t bfs_multi(const uint32_t *srcs_host, uint32_t k, uint32_t *levels, uint32_t threshold = 0);
   // 64 BC in one traversal: out[j*n + v] for source j (k <= 64), each
   // equal to bc(srcs[j]) summed in sorted neighbour order.
-  void bc_multi(const uint32_t *srcs_host, uint32_t k, double *out, uint32_t threshold = 20);
+  void bc_multi(const uint32_t *srcs_host, uint32_t k, double *out, uint32_t threshold = 0);
   // Triangles through each queried vertex (host in / host out).
   void ltc(const uint32_t *verts_host, uint32_t q, uint64_t *out_host);
+  // Autotuner: sweeps the push/pull thresholds of BFS, multi-source BFS and
+  // CC and the pull block size on the current graph and stores the fastest
+  // in Graph::tune.  report[12]: thr_bfs, thr_ms, thr_cc, pull_block, then
+  // (reference setting ms, tuned ms) for BFS, MS-BFS, CC and PageRank.
+  void autotune(double *report);
   // Order-independent digest of a device array (u32 or u64 words).
   uint64_t digest(const uint32_t *a32, const uint64_t *a64, uint64_t n);
 
@@ -93,6 +99,7 @@ class Traversal {
   void record(const char *name, hipEvent_t a, hipEvent_t b);
 
   uint64_t last_edges_visited = 0;  // Σ degree over processed frontiers
+  DevBuf<uint8_t> &tree_parts() { return tree_parts_; }
 
  private:
   Graph &g_;
@@ -112,6 +119,8 @@ class Traversal {
   DevBuf<uint32_t> bc_list_;          // per-level vertex lists
   DevBuf<double> bc_sigma_, bc_dep_;  // [vertex][64]
   uint64_t ntree_tasks_ = 0;
+  DevBuf<uint8_t> tree_parts_;  // per-task partials of ordered (FP64) tree pulls
+  uint64_t tasks_epoch_ = ~0ull;  // graph epoch the tree tasks were built for
   const uint32_t *tree_list_ = nullptr;
   uint32_t ntree_list_ = 0;
   void prepare_tree_tasks();

@@ -144,6 +144,38 @@ class Graph:
                 "promotions", "trees", "pma_slots"]
         return {k: int(v) for k, v in zip(keys, out)}
 
+    def apply_ops(self, src, dst, ops) -> None:
+        """Coalesced updates: ops[i] 0 = insert, 1 = delete (src[i], dst[i]),
+        applied in order in one device pass (last op on an edge wins)."""
+        s, d, o = _u32(src), _u32(dst), _u32(ops)
+        assert len(s) == len(d) == len(o)
+        check(lib().tg_apply_ops(self._h, ptr(s), ptr(d), ptr(o), len(s)))
+
+    def autotune(self) -> dict:
+        """Sweep push/pull thresholds and the pull block size on this graph;
+        later algorithm calls use the fastest (results are unchanged)."""
+        r = np.zeros(12, np.float64)
+        check(lib().tg_autotune(self._h, ptr(r)))
+        keys = ["thr_bfs", "thr_ms", "thr_cc", "pull_block", "bfs_ref_ms", "bfs_tuned_ms", "ms_ref_ms",
+                "ms_tuned_ms", "cc_ref_ms", "cc_tuned_ms", "pr_ref_ms", "pr_tuned_ms"]
+        out = {k: float(v) for k, v in zip(keys, r)}
+        for k in keys[:4]:
+            out[k] = int(out[k])
+        return out
+
+    TUNING_KEYS = ("thr_bfs", "thr_ms", "thr_cc", "pull_block", "pr_graph")
+
+    def get_tuning(self) -> dict:
+        t = np.zeros(5, np.uint32)
+        check(lib().tg_get_tuning(self._h, ptr(t)))
+        return {k: int(v) for k, v in zip(self.TUNING_KEYS, t)}
+
+    def set_tuning(self, **kw) -> None:
+        cur = self.get_tuning()
+        cur.update(kw)
+        t = np.array([cur[k] for k in self.TUNING_KEYS], np.uint32)
+        check(lib().tg_set_tuning(self._h, ptr(t)))
+
     def tier_counts(self) -> dict:
         out = np.zeros(4, np.uint64)
         check(lib().tg_tier_counts(self._h, ptr(out)))
