@@ -15,6 +15,7 @@ FP64), CC, insert batches of 10^3 and 10^7 edges, and reports them under
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import subprocess
@@ -297,6 +298,180 @@ def serve_bench(args):
     print(json.dumps(out))
 
 
+def batching_bench(args):
+    """Batching on one resident graph (rMAT scale 24, a=.5 b=.1 c=.1):
+    1024 BFS and 256 BC queries from seeded random sources, one traversal per
+    query versus 64 sources per lane mask; 4096 small update batches (1..1000
+    edges, log-uniform, 90% inserts / 10% deletes of existing edges) applied
+    one batch at a time versus coalesced per launch.  The autotuner runs
+    first.  Multi-source levels are compared with the per-source levels for
+    a sample of lanes, multi-source BC with the per-source BC within 1e-9,
+    and the two graphs' adjacency after the updates must be identical."""
+    from terrace_amd import graph as tg
+    from terrace_amd._lib import DeviceBuffer, lib, check, ptr
+
+    n = 1 << args.scale
+    probs = (0.5, 0.1, 0.1)
+    seed = aspen(100) & 0xFFFFFFFF
+    rng = np.random.default_rng(4242)
+
+    def build():
+        G = tg.Graph(n)
+        s, d, w = tg.rmat_device(n, seed, *probs, args.edges, nominal=True, symmetric=True, drop_loops=True)
+        G.add_edge_batch_dev(s, d, w)
+        del s, d
+        tg.lib().tg_device_sync()
+        return G
+
+    log(f"batching: building scale {args.scale}, {args.edges} generated edges")
+    G = build()
+    m_directed = G.get_num_edges()
+    log(f"batching: {m_directed} directed edges, tiers {G.tier_counts()}")
+    t0 = time.perf_counter()
+    tune = G.autotune()
+    tune["autotune_s"] = time.perf_counter() - t0
+    log(f"batching: autotune {tune}")
+    deg = G.degrees()
+    active = np.nonzero(deg)[0].astype(np.uint32)
+    bfs_src = active[rng.integers(0, len(active), 1024)].astype(np.uint32)
+    bc_src = active[rng.integers(0, len(active), 256)].astype(np.uint32)
+    res = {"autotune": tune}
+
+    # ---- BFS: per source vs 64 per lane mask (device outputs)
+    par, lev = DeviceBuffer(4 * n), DeviceBuffer(4 * n)
+    depth = ctypes.c_uint32()
+    t0 = time.perf_counter()
+    for v in bfs_src:
+        check(lib().tg_bfs_dev(G.handle, int(v), par.ptr, lev.ptr, ctypes.byref(depth)))
+    t_single = time.perf_counter() - t0
+    mlev = DeviceBuffer(64 * 4 * n)
+    t0 = time.perf_counter()
+    for c in range(0, len(bfs_src), 64):
+        chunk = np.ascontiguousarray(bfs_src[c:c + 64])
+        check(lib().tg_bfs_multi_dev(G.handle, ptr(chunk), len(chunk), mlev.ptr))
+    t_multi = time.perf_counter() - t0
+    # the last chunk's lanes against per-source levels (sampled lanes)
+    last = np.ascontiguousarray(bfs_src[-64:])
+    bfs_bad = 0
+    for j in range(0, 64, 8):
+        check(lib().tg_bfs_dev(G.handle, int(last[j]), par.ptr, lev.ptr, ctypes.byref(depth)))
+        a = lev.download(np.uint32, n)
+        b = np.empty(n, np.uint32)
+        check(lib().tg_memcpy_d2h(ptr(b), mlev.offset(4 * n * j), 4 * n))
+        bfs_bad += int(not np.array_equal(a, b))
+    res["bfs"] = {"queries": len(bfs_src), "single_s": t_single, "multi_s": t_multi,
+                  "single_per_s": len(bfs_src) / t_single, "multi_per_s": len(bfs_src) / t_multi,
+                  "speedup": t_single / t_multi, "lanes_checked": 8, "lanes_mismatched": bfs_bad}
+    log(f"batching: BFS {res['bfs']}")
+    mlev.close(), par.close(), lev.close()
+
+    # ---- BC: per source vs 64 per lane mask
+    bco = DeviceBuffer(8 * n)
+    t0 = time.perf_counter()
+    for v in bc_src:
+        check(lib().tg_bc_dev(G.handle, int(v), bco.ptr))
+    t_single = time.perf_counter() - t0
+    mbc = DeviceBuffer(64 * 8 * n)
+    t0 = time.perf_counter()
+    for c in range(0, len(bc_src), 64):
+        chunk = np.ascontiguousarray(bc_src[c:c + 64])
+        check(lib().tg_bc_multi_dev(G.handle, ptr(chunk), len(chunk), mbc.ptr))
+    t_multi = time.perf_counter() - t0
+    last = np.ascontiguousarray(bc_src[-64:])
+    worst = 0.0
+    bc_bad = 0
+    for j in range(0, 64, 16):
+        check(lib().tg_bc_dev(G.handle, int(last[j]), bco.ptr))
+        a = bco.download(np.float64, n)
+        b = np.empty(n, np.float64)
+        check(lib().tg_memcpy_d2h(ptr(b), mbc.offset(8 * n * j), 8 * n))
+        fa, fb = np.isfinite(a), np.isfinite(b)
+        if not np.array_equal(fa, fb):
+            bc_bad += 1
+            continue
+        err = float(np.abs(a[fa] - b[fa]).sum() / max(np.abs(a[fa]).sum(), 1e-300))
+        worst = max(worst, err)
+        bc_bad += int(err > 1e-9)
+    res["bc"] = {"queries": len(bc_src), "single_s": t_single, "multi_s": t_multi,
+                 "single_per_s": len(bc_src) / t_single, "multi_per_s": len(bc_src) / t_multi,
+                 "speedup": t_single / t_multi, "lanes_checked": 4, "lanes_out_of_tolerance": bc_bad,
+                 "worst_rel_l1": worst}
+    log(f"batching: BC {res['bc']}")
+    mbc.close(), bco.close()
+
+    # ---- 4096 small update batches: one at a time vs coalesced
+    off, nbr = G.export_adjacency()
+    sizes = np.round(10 ** rng.uniform(0, 3, 4096)).astype(np.int64)
+    n_ins = np.maximum(1, (sizes * 9) // 10)
+    n_del = np.maximum(0, sizes - n_ins)
+    tot_ins = int(n_ins.sum())
+    s_d, d_d, w = tg.rmat_device(n, aspen(9000) & 0xFFFFFFFF, *probs, tot_ins, nominal=True, symmetric=False,
+                                 drop_loops=True)
+    ins_s, ins_d = s_d.download(np.uint32, w), d_d.download(np.uint32, w)
+    del s_d, d_d
+    idx = rng.integers(0, len(nbr), int(n_del.sum()))
+    del_s = (np.searchsorted(off, idx, side="right") - 1).astype(np.uint32)
+    del_d = nbr[idx]
+    del off, nbr
+    batches, io, do = [], 0, 0
+    for b in range(4096):
+        ni = min(int(n_ins[b]), len(ins_s) - io)
+        batches.append((ins_s[io:io + ni], ins_d[io:io + ni], del_s[do:do + n_del[b]], del_d[do:do + n_del[b]]))
+        io += ni
+        do += int(n_del[b])
+    G2 = build()
+    t0 = time.perf_counter()
+    for bs, bd, xs, xd in batches:
+        if len(bs):
+            G2.add_edge_batch(bs, bd)
+        if len(xs):
+            G2.remove_edge_batch(xs, xd)
+    t_single = time.perf_counter() - t0
+    per_launch = 256
+    chunks = []
+    for c in range(0, 4096, per_launch):
+        part = batches[c:c + per_launch]
+        cs = np.concatenate([np.concatenate([p[0], p[2]]) for p in part])
+        cd = np.concatenate([np.concatenate([p[1], p[3]]) for p in part])
+        co = np.concatenate([np.concatenate([np.zeros(len(p[0]), np.uint32), np.ones(len(p[2]), np.uint32)])
+                             for p in part])
+        chunks.append((cs, cd, co))
+    t0 = time.perf_counter()
+    for cs, cd, co in chunks:
+        G.apply_ops(cs, cd, co)
+    t_coal = time.perf_counter() - t0
+    o1, n1 = G.export_adjacency()
+    o2, n2 = G2.export_adjacency()
+    same = bool(np.array_equal(o1, o2) and np.array_equal(n1, n2))
+    del o1, n1, o2, n2
+    edges = int(sizes.sum())
+    res["updates"] = {"batches": 4096, "edges": edges, "batches_per_launch": per_launch,
+                      "single_s": t_single, "coalesced_s": t_coal, "speedup": t_single / t_coal,
+                      "single_edges_per_s": edges / t_single, "coalesced_edges_per_s": edges / t_coal,
+                      "adjacency_identical": same}
+    log(f"batching: updates {res['updates']}")
+    G.close(), G2.close()
+    out = {
+        "metric": "multi-source BFS queries/s (64 per lane mask)",
+        "value": res["bfs"]["multi_per_s"],
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": len(bfs_src),
+        "warmup": 0,
+        "ms_per_step": res["bfs"]["multi_s"] / len(bfs_src) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/f64",
+        "data": "synthetic rMAT generated on the device (seeded)",
+        "config": {"workload": f"batching: rmat scale {args.scale} (.5/.1/.1), {args.edges} generated edges; "
+                               "1024 BFS + 256 BC sources, 4096 update batches of 1..1000 edges",
+                   "vertices": n, "directed_edges": m_directed, "parallelism": "single GPU"},
+        "batching": res,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -312,6 +487,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="finish with three 1 GiB device copies (HBM counter calibration)")
     ap.add_argument("--serve", action="store_true", help="mixed-stream serving comparison instead")
+    ap.add_argument("--batching", action="store_true",
+                    help="multi-source BFS/BC and coalesced small updates instead")
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--rate", type=float, default=200.0, help="request arrivals per second")
     ap.add_argument("--budget", type=int, default=160, help="server memory budget, GiB")
@@ -323,11 +500,14 @@ def main():
         sys.exit(2)
 
     if args.scale is None:
-        args.scale = 26 if args.serve else 22
+        args.scale = 26 if args.serve else 24 if args.batching else 22
     if args.edges is None:
-        args.edges = 775_000_000 if args.serve else 69_000_000
+        args.edges = 775_000_000 if args.serve else 136_000_000 if args.batching else 69_000_000
     if args.serve:
         serve_bench(args)
+        return
+    if args.batching:
+        batching_bench(args)
         return
 
     from terrace_amd import graph as tg
