@@ -609,6 +609,18 @@ def main():
             tg.lib().tg_device_sync()
             dt = time.perf_counter() - t0
             extra[f"update_edges_per_s_b{bsz}"] = (bt[1][2] + bt[1][5]) / dt
+        # the same 1e5 mixed batches handed over as host arrays (PCIe-inclusive)
+        bt = make_batches(tg, G, n, probs, args.batch, 3, 8000, rng, off, nbr)
+        hb = [(s_.download(np.uint32, w_), d_.download(np.uint32, w_), bs_.download(np.uint32, nd_),
+               bd_.download(np.uint32, nd_)) for s_, d_, w_, bs_, bd_, nd_ in bt]
+        tg.lib().tg_device_sync()
+        t0 = time.perf_counter()
+        for hs, hd, xs, xd in hb:
+            G.add_edge_batch(hs, hd)
+            G.remove_edge_batch(xs, xd)
+        tg.lib().tg_device_sync()
+        dt = time.perf_counter() - t0
+        extra["update_edges_per_s_host_arrays_b1e5"] = sum(len(h[0]) + len(h[2]) for h in hb) / dt
         extra["memory_bytes"] = G.get_size()
         extra["stats"] = G.stats()
 
