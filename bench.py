@@ -472,6 +472,134 @@ def batching_bench(args):
     print(json.dumps(out))
 
 
+def skew_bench(args):
+    """Skew stress across all three tiers: rMAT scale 22 with a=.57 b=.19
+    c=.19 (max degree above 1e5; inline, PMA and B-tree tiers populated).
+    BFS, PageRank (10 iterations), CC, BC and TC from the highest-degree
+    vertex, then mixed update batches of 1e6 edges whose sources are the
+    top-1% degree vertices (90% inserts to random active vertices, 10%
+    deletes of existing edges of those sources; both directions).  The
+    adjacency after the first batch is checked against a host set model."""
+    from terrace_amd import graph as tg
+
+    n = 1 << args.scale
+    probs = (0.57, 0.19, 0.19)
+    seed = aspen(100) & 0xFFFFFFFF
+    rng = np.random.default_rng(99)
+    log(f"skew: building scale {args.scale}, {args.edges} generated edges")
+    G = tg.Graph(n)
+    s, d, w = tg.rmat_device(n, seed, *probs, args.edges, nominal=True, symmetric=True, drop_loops=True)
+    tg.lib().tg_device_sync()
+    t0 = time.perf_counter()
+    G.add_edge_batch_dev(s, d, w)
+    tg.lib().tg_device_sync()
+    t_build = time.perf_counter() - t0
+    del s, d
+    m = G.get_num_edges()
+    deg = G.degrees()
+    tiers = G.tier_counts()
+    src = int(np.argmax(deg))
+    log(f"skew: {m} directed edges in {t_build:.3f} s, max degree {int(deg.max())}, tiers {tiers}")
+    res = {"static_build_s": t_build, "directed_edges": m, "max_degree": int(deg.max()),
+           "active_vertices": int((deg > 0).sum()), "tiers": tiers, "source": src}
+    t0 = time.perf_counter()
+    parents, levels = tg.BFS_with_edge_map(G, src, with_levels=True)
+    t = time.perf_counter() - t0
+    reached = levels != np.iinfo(np.uint32).max
+    res["bfs_ms"], res["bfs_gteps"] = t * 1e3, float(deg[reached].astype(np.int64).sum()) / t / 1e9
+    t0 = time.perf_counter()
+    tg.PR_S(G, 10)
+    t = time.perf_counter() - t0
+    res["pr_ms"], res["pr_gteps"] = t * 1e3, 10 * m / t / 1e9
+    t0 = time.perf_counter()
+    tg.CC(G)
+    res["cc_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    tg.BC(G, src)
+    res["bc_ms"] = (time.perf_counter() - t0) * 1e3
+    log(f"skew: BFS {res['bfs_ms']:.1f} ms, PR {res['pr_ms']:.1f} ms, CC {res['cc_ms']:.1f} ms, "
+        f"BC {res['bc_ms']:.1f} ms")
+    t0 = time.perf_counter()
+    res["tc"] = tg.TC(G)
+    res["tc_ms"] = (time.perf_counter() - t0) * 1e3
+    log(f"skew: TC {res['tc']} in {res['tc_ms']:.1f} ms")
+    # hub-targeted 1e6 mixed batches
+    off, nbr = G.export_adjacency()
+    active = np.nonzero(deg)[0].astype(np.uint32)
+    top = np.argsort(deg)[-max(1, int((deg > 0).sum()) // 100):].astype(np.uint32)
+    batches = []
+    for b in range(args.skew_batches + 1):
+        pairs = 450_000
+        hs = top[rng.integers(0, len(top), pairs)]
+        hd = active[rng.integers(0, len(active), pairs)]
+        ins_s, ins_d = np.concatenate([hs, hd]), np.concatenate([hd, hs])
+        vs = top[rng.integers(0, len(top), 50_000)]
+        lo, hi = off[vs].astype(np.int64), off[vs + 1].astype(np.int64)
+        pos = lo + (rng.random(len(vs)) * (hi - lo)).astype(np.int64)
+        dd = nbr[pos]
+        del_s, del_d = np.concatenate([vs, dd]), np.concatenate([dd, vs])
+        dev = [tg.DeviceBuffer(4 * len(x)) for x in (ins_s, ins_d, del_s, del_d)]
+        for buf, x in zip(dev, (ins_s, ins_d, del_s, del_d)):
+            buf.upload(np.ascontiguousarray(x, np.uint32))
+        batches.append((dev, len(ins_s), len(del_s), (ins_s, ins_d, del_s, del_d)))
+    # batch 0 (warm-up) is checked against the host set model
+    dev, ni, nd, host = batches[0]
+    G.add_edge_batch_dev(dev[0], dev[1], ni)
+    G.remove_edge_batch_dev(dev[2], dev[3], nd)
+    keys = (np.repeat(np.arange(n, dtype=np.uint64), np.diff(off).astype(np.int64)) << np.uint64(32)) | nbr
+    del nbr
+    ins = np.unique((host[0].astype(np.uint64) << np.uint64(32)) | host[1])
+    dl = np.unique((host[2].astype(np.uint64) << np.uint64(32)) | host[3])
+    p = np.searchsorted(keys, ins).clip(max=len(keys) - 1)
+    want = np.concatenate([keys, ins[keys[p] != ins]])
+    del keys
+    want.sort()
+    p = np.searchsorted(want, dl).clip(max=len(want) - 1)
+    keep = np.ones(len(want), bool)
+    keep[p[want[p] == dl]] = False
+    want = want[keep]
+    o2, n2 = G.export_adjacency()
+    got = (np.repeat(np.arange(n, dtype=np.uint64), np.diff(o2).astype(np.int64)) << np.uint64(32)) | n2
+    res["update_check_equal"] = bool(np.array_equal(got, want))
+    del got, want, o2, n2
+    log(f"skew: batch 0 adjacency equal to the set model: {res['update_check_equal']}")
+    tg.lib().tg_device_sync()
+    t0 = time.perf_counter()
+    tot = 0
+    for dev, ni, nd, _ in batches[1:]:
+        G.add_edge_batch_dev(dev[0], dev[1], ni)
+        G.remove_edge_batch_dev(dev[2], dev[3], nd)
+        tot += ni + nd
+    tg.lib().tg_device_sync()
+    t = time.perf_counter() - t0
+    res["update_batches"] = args.skew_batches
+    res["update_edges_per_s"] = tot / t
+    res["ms_per_batch"] = t / args.skew_batches * 1e3
+    res["tiers_after"] = G.tier_counts()
+    log(f"skew: {args.skew_batches} hub-targeted batches of {tot // args.skew_batches} edges: "
+        f"{res['update_edges_per_s']:.3e} edges/s")
+    G.close()
+    out = {
+        "metric": "inserted edges/s (hub-targeted 1e6 batches, skewed graph)",
+        "value": res["update_edges_per_s"],
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.skew_batches,
+        "warmup": 1,
+        "ms_per_step": res["ms_per_batch"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic rMAT generated on the device (seeded)",
+        "config": {"workload": f"skew stress: rmat scale {args.scale} (.57/.19/.19), {args.edges} generated edges; "
+                               "1e6-edge mixed batches from the top-1% degree vertices",
+                   "vertices": n, "directed_edges": m, "parallelism": "single GPU"},
+        "skew": res,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -487,6 +615,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="finish with three 1 GiB device copies (HBM counter calibration)")
     ap.add_argument("--serve", action="store_true", help="mixed-stream serving comparison instead")
+    ap.add_argument("--skew", action="store_true", help="skew stress across all three tiers instead")
+    ap.add_argument("--skew-batches", type=int, default=5)
     ap.add_argument("--batching", action="store_true",
                     help="multi-source BFS/BC and coalesced small updates instead")
     ap.add_argument("--requests", type=int, default=400)
@@ -502,7 +632,11 @@ def main():
     if args.scale is None:
         args.scale = 26 if args.serve else 24 if args.batching else 22
     if args.edges is None:
-        args.edges = 775_000_000 if args.serve else 136_000_000 if args.batching else 69_000_000
+        args.edges = (775_000_000 if args.serve else 136_000_000 if args.batching
+                      else 125_000_000 if args.skew else 69_000_000)
+    if args.skew:
+        skew_bench(args)
+        return
     if args.serve:
         serve_bench(args)
         return

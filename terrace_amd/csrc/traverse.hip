@@ -824,6 +824,86 @@ __global__ void k_msbc_final(const double *dep, const double *inv, uint32_t n, u
 }
 
 
+// ---------------------------------------------------------------- TC (degree-ordered)
+// TC.h:51-87 counts, for every edge (i, v) with v > i, the common neighbours
+// below i: every triangle exactly once.  The same total comes from the
+// degree-ordered orientation u -> v when (deg u, u) < (deg v, v): each
+// triangle is found once, at its lowest-ranked vertex u and its middle v, as
+// out(u) ∩ out(v).  Out-lists are short (at most sqrt(2m)), which bounds the
+// per-edge work on skewed graphs.
+__device__ inline bool tc_rank_less(const uint64_t *off, uint32_t a, uint32_t b) {
+  uint64_t da = off[a + 1] - off[a], db = off[b + 1] - off[b];
+  return da < db || (da == db && a < b);
+}
+
+// One wave per vertex: flag[e] = 1 when edge e points up in rank.
+__global__ void __launch_bounds__(256) k_tc_orient_flags(const uint64_t *off, const uint32_t *nbr, uint32_t n,
+                                                         uint32_t *flag) {
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t u = wave; u < n; u += nwaves) {
+    uint64_t b = off[u], e = off[u + 1];
+    for (uint64_t x = b + lane; x < e; x += 64)
+      flag[x] = tc_rank_less(off, static_cast<uint32_t>(u), nbr[x]) ? 1u : 0u;
+  }
+}
+
+__global__ void k_tc_orient_fill(const uint32_t *nbr, const uint32_t *flag, const uint32_t *pos, uint64_t m,
+                                 uint32_t *out) {
+  GRID_STRIDE(x, m) if (flag[x]) out[pos[x]] = nbr[x];
+}
+
+__global__ void k_tc_orient_off(const uint64_t *off, const uint32_t *pos, uint32_t n, uint64_t m, uint64_t *ooff) {
+  GRID_STRIDE(u, static_cast<uint64_t>(n) + 1) ooff[u] = u < n ? pos[off[u]] : pos[m];
+}
+
+constexpr uint32_t kTcLds = 2048;  // out-list entries staged in LDS per wave
+
+// One wave per vertex u: out(u) staged in LDS (searched in place in HBM
+// when longer than kTcLds), then every w in out(v), v in out(u), is looked
+// up by binary search.
+__global__ void __launch_bounds__(256) k_tc_count(const uint64_t *ooff, const uint32_t *out, uint32_t n,
+                                                  unsigned long long *total) {
+  __shared__ uint32_t lds[4][kTcLds];
+  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned long long local = 0;
+  for (uint64_t u = wave; u < n; u += nwaves) {
+    uint64_t ub = ooff[u];
+    uint32_t du = static_cast<uint32_t>(ooff[u + 1] - ub);
+    if (du < 2) continue;  // uniform across the wave
+    const uint32_t *au = out + ub;
+    bool staged = du <= kTcLds;
+    if (staged) {
+      for (uint32_t x = lane; x < du; x += 64) lds[wid][x] = au[x];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+    }
+    const uint32_t *set = staged ? lds[wid] : au;
+    for (uint32_t k = 0; k < du; k++) {
+      uint32_t v = au[k];
+      uint64_t vb = ooff[v], ve = ooff[v + 1];
+      for (uint64_t x = vb + lane; x < ve; x += 64) {
+        uint32_t w = out[x];
+        uint32_t lo = 0, hi = du;
+        while (lo < hi) {
+          uint32_t mid = (lo + hi) >> 1;
+          if (set[mid] < w) lo = mid + 1;
+          else hi = mid;
+        }
+        local += (lo < du && set[lo] == w) ? 1ull : 0ull;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // the next vertex may overwrite the staged list
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
+  if (lane == 0 && local) atomicAdd(total, local);
+}
+
 // One wave per queried vertex: its sorted neighbour list into buf[off[q]..].
 __global__ void __launch_bounds__(256) k_gather_lists(GView g, const uint32_t *verts, uint32_t q, const uint64_t *off,
                                                       uint32_t *buf) {
@@ -1741,6 +1821,7 @@ uint64_t Traversal::tc() {
   hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   uint64_t m = g_.num_edges();
+  if (m > 0xFFFFFFFFull) throw Error(-1, "tc: more than 2^32 directed edges");
   csr_off_.ensure(static_cast<uint64_t>(n) + 1, s);
   csr_nbr_.ensure(m + 1, s);
   g_.export_adjacency(csr_off_.get(), csr_nbr_.get());
@@ -1753,9 +1834,26 @@ uint64_t Traversal::tc() {
     TG_HIP(hipEventRecord(e0, s));
   }
   if (m) {
-    k_tc<<<grid_for(m, 256, 1u << 18), 256, 0, s>>>(csr_off_.get(), csr_nbr_.get(), n, m,
-                                                    reinterpret_cast<unsigned long long *>(red_.get()));
+    DevBuf<uint32_t> flag, pos, out;
+    DevBuf<uint64_t> ooff;
+    flag.ensure(m + 1, s);
+    pos.ensure(m + 1, s);
+    ooff.ensure(static_cast<uint64_t>(n) + 1, s);
+    k_tc_orient_flags<<<grid_for(static_cast<uint64_t>(n) * 64, 256, 1u << 18), 256, 0, s>>>(
+        csr_off_.get(), csr_nbr_.get(), n, flag.get());
     TG_CHECK_LAUNCH();
+    exclusive_scan_u32(flag.get(), pos.get(), m, pos.get() + m, scan_, s);
+    uint32_t mo = read_scalar(pos.get() + m, s);
+    out.ensure(static_cast<uint64_t>(mo) + 1, s);
+    k_tc_orient_fill<<<grid_for(m, 256, 1u << 20), 256, 0, s>>>(csr_nbr_.get(), flag.get(), pos.get(), m, out.get());
+    TG_CHECK_LAUNCH();
+    k_tc_orient_off<<<grid_for(static_cast<uint64_t>(n) + 1, 256), 256, 0, s>>>(csr_off_.get(), pos.get(), n, m,
+                                                                               ooff.get());
+    TG_CHECK_LAUNCH();
+    k_tc_count<<<grid_for(static_cast<uint64_t>(n) * 64, 256, 1u << 16), 256, 0, s>>>(
+        ooff.get(), out.get(), n, reinterpret_cast<unsigned long long *>(red_.get()));
+    TG_CHECK_LAUNCH();
+    TG_HIP(hipStreamSynchronize(s));
   }
   if (profiling) {
     TG_HIP(hipEventRecord(e1, s));

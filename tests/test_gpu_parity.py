@@ -262,3 +262,55 @@ def test_scale16_static_and_batches():
     g = oracle.CSR(keys, n)
     check_algorithms(G, g, int(np.argmax(g.degrees())), what="scale16")
     G.close()
+
+
+def test_tc_complete_graph_closed_form():
+    """K_2100: every vertex in the B-tree tier; degree-ordered out-lists of
+    up to 2099 entries exceed the LDS-staged length (2048), so both search
+    paths run.  TC = C(n, 3), and each vertex lies in C(n-1, 2) triangles."""
+    n = 2100
+    print("[gpu-test] tc complete graph", flush=True)
+    a, b = np.meshgrid(np.arange(n, dtype=np.uint32), np.arange(n, dtype=np.uint32), indexing="ij")
+    keep = a != b
+    G = tg.Graph(n)
+    G.add_edge_batch(a[keep].ravel(), b[keep].ravel())
+    assert G.get_num_edges() == n * (n - 1)
+    assert tg.TC(G) == n * (n - 1) * (n - 2) // 6
+    v = np.array([0, 1, n // 2, n - 1], np.uint32)
+    assert np.array_equal(tg.local_triangles(G, v), np.full(4, (n - 1) * (n - 2) // 2, np.uint64))
+    G.close()
+
+
+def test_skew_all_tiers_hub_targeted_batches():
+    """Skewed rMAT (a=.57 b=.19 c=.19) with all three tiers populated, then
+    mixed batches whose sources are the top-1% degree vertices (B-tree
+    splits, PMA windows around hubs, promotions), checked against the set
+    model after every batch and by every algorithm at the end."""
+    n = 1 << 14
+    say("skew hub-targeted")
+    rng = np.random.default_rng(23)
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 31, 0.57, 0.19, 0.19, 400000))
+    G = tg.Graph(n)
+    G.add_edge_batch(*split(keys))
+    tiers = G.tiers()
+    assert (tiers == 0).any() and (tiers == 1).any() and (tiers == 2).any()
+    assert_adjacency(G, keys, n, "static")
+    for rnd in range(4):
+        deg = oracle.CSR(keys, n).degrees()
+        active = np.nonzero(deg)[0]
+        top = np.argsort(deg)[-max(1, n // 100):]
+        s = top[rng.integers(0, len(top), 20000)].astype(np.uint32)
+        d = active[rng.integers(0, len(active), 20000)].astype(np.uint32)
+        ins = oracle.build_keys(s, d)
+        pick = keys[rng.choice(len(keys), 3000, replace=False)]
+        ps, pd = split(pick)
+        dels = oracle.sort_unique(np.concatenate([pick, (pd.astype(np.uint64) << np.uint64(32)) | ps]))
+        G.add_edge_batch(*split(ins))
+        keys = oracle.set_union(keys, ins)
+        assert_adjacency(G, keys, n, f"hub insert {rnd}")
+        G.remove_edge_batch(*split(dels))
+        keys = oracle.set_difference(keys, dels)
+        assert_adjacency(G, keys, n, f"hub delete {rnd}")
+    g = oracle.CSR(keys, n)
+    check_algorithms(G, g, int(np.argmax(g.degrees())), what="skew")
+    G.close()
