@@ -1215,6 +1215,8 @@ Graph::Graph(uint32_t n) : n_(n) {
   scal_.ensure(8, stream_);
   dsc_.ensure(kScalars, stream_);
   fill_u64(dsc_.get(), 0, kScalars, stream_);
+  scan_scratch.lookback = true;  // single-pass scans / selects / sort on the update path
+  sort_.scan.lookback = true;
   init_pma(2);
   ensure_trees(16);
   TG_HIP(hipStreamSynchronize(stream_));
@@ -1368,7 +1370,17 @@ uint64_t Graph::prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t 
 // Internal consistency flags raised by kernels during the batch.
 void Graph::check_batch_flags() {
   uint64_t f = 0;
+  uint32_t lb[2] = {0, 0};
+  if (scan_scratch.lb.misc.get())
+    TG_HIP(hipMemcpyAsync(&lb[0], scan_scratch.lb.misc.get() + 1, 4, hipMemcpyDeviceToHost, stream_));
+  if (sort_.scan.lb.misc.get())
+    TG_HIP(hipMemcpyAsync(&lb[1], sort_.scan.lb.misc.get() + 1, 4, hipMemcpyDeviceToHost, stream_));
   read_scalars(dsc_.get(), &f, 1, stream_);
+  if (lb[0] | lb[1]) {
+    if (lb[0]) fill_u32(scan_scratch.lb.misc.get() + 1, 0, 1, stream_);
+    if (lb[1]) fill_u32(sort_.scan.lb.misc.get() + 1, 0, 1, stream_);
+    throw Error(-4, "single-pass scan: a look-back wait gave up");
+  }
   if (f) fill_u64(dsc_.get(), 0, 1, stream_);  // leave the flags clear for the next batch
   if (f & 2) throw Error(-4, "PMA count tree out of sync (gather)");
   if (f & 4) throw Error(-4, "PMA delete: key not found");

@@ -111,6 +111,52 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_tile(const InT *in, InT *ou
   if (total_dev && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_dev = static_cast<InT>(off + tot);
 }
 
+// Single-pass scan: tile scan plus the decoupled look-back for the block's
+// prefix (one launch for any n).
+template <typename InT>
+__global__ void __launch_bounds__(kScanBlock) k_scan_lb(const InT *in, InT *out, uint64_t n, InT *total_dev,
+                                                        uint64_t *state, uint32_t *misc, uint32_t nblocks,
+                                                        uint32_t epoch) {
+  __shared__ uint64_t lds[kScanBlock / 64];
+  __shared__ InT tile[kScanTile];
+  __shared__ uint32_t s_ticket;
+  __shared__ uint64_t s_excl;
+  const uint32_t b = lbk::ticket(misc, nblocks, &s_ticket);
+  if (b >= nblocks) return;  // inconsistent ticket counter: reported, no work
+  const uint64_t tbase = static_cast<uint64_t>(b) * kScanTile;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    uint64_t idx = tbase + static_cast<uint64_t>(i) * kScanBlock + threadIdx.x;
+    tile[i * kScanBlock + threadIdx.x] = idx < n ? in[idx] : InT(0);
+  }
+  __syncthreads();
+  uint64_t vals[kScanItems];
+  uint64_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    vals[i] = static_cast<uint64_t>(tile[threadIdx.x * kScanItems + i]);
+    sum += vals[i];
+  }
+  uint64_t tot;
+  uint64_t pre = block_excl_scan(sum, lds, &tot);
+  if (threadIdx.x == 0) s_excl = lbk::lookback(state, 1, b, 0, misc, epoch, static_cast<uint32_t>(tot));
+  __syncthreads();
+  const uint64_t excl = s_excl;
+  pre += excl;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    tile[threadIdx.x * kScanItems + i] = static_cast<InT>(pre);
+    pre += vals[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    uint64_t idx = tbase + static_cast<uint64_t>(i) * kScanBlock + threadIdx.x;
+    if (idx < n) out[idx] = tile[i * kScanBlock + threadIdx.x];
+  }
+  if (total_dev && b == nblocks - 1 && threadIdx.x == 0) *total_dev = static_cast<InT>(excl + tot);
+}
+
 template <typename T>
 __global__ void k_set_zero(T *p) { *p = 0; }
 
@@ -127,6 +173,14 @@ void scan_impl(const InT *in, InT *out, uint64_t n, InT *total_dev, ScanScratch 
   uint64_t nblocks = (n + kScanTile - 1) / kScanTile;
   if (nblocks == 1) {
     k_scan_tile<InT, uint64_t><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, total_dev);
+    TG_CHECK_LAUNCH();
+    return;
+  }
+  // single pass for 32-bit scans (prefixes fit the look-back word)
+  if (ss.lookback && sizeof(InT) == 4 && nblocks <= 0xFFFFFFFFull) {
+    uint32_t nb = static_cast<uint32_t>(nblocks);
+    uint32_t epoch = ss.lb.begin(nb, s);
+    k_scan_lb<InT><<<nb, kScanBlock, 0, s>>>(in, out, n, total_dev, ss.lb.state.get(), ss.lb.misc.get(), nb, epoch);
     TG_CHECK_LAUNCH();
     return;
   }
@@ -231,6 +285,104 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const uint64_t *ke
   (void)digit_base;
 }
 
+// ---- onesweep radix sort: all digit histograms in one pass over the keys,
+// then one kernel per digit pass whose blocks find their per-digit offsets
+// by decoupled look-back (one packed word per (block, digit): epoch << 34 |
+// status << 32 | count).
+struct SortShifts {
+  int s[8];
+  int np;
+};
+
+__global__ void __launch_bounds__(kSortBlock) k_radix_hist_all(const uint64_t *keys, uint64_t n, SortShifts sh,
+                                                               uint32_t *ghist) {
+  __shared__ uint32_t cnt[8][256];
+  for (int p = 0; p < sh.np; p++) cnt[p][threadIdx.x] = 0;
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    uint64_t idx = sort_index(tile, w, j, lane);
+    if (idx < n) {
+      uint64_t k = keys[idx];
+      for (int p = 0; p < sh.np; p++) atomicAdd(&cnt[p][(k >> sh.s[p]) & 0xFF], 1u);
+    }
+  }
+  __syncthreads();
+  for (int p = 0; p < sh.np; p++) {
+    uint32_t c = cnt[p][threadIdx.x];
+    if (c) atomicAdd(&ghist[p * 256 + threadIdx.x], c);
+  }
+}
+
+__global__ void __launch_bounds__(kSortBlock) k_onesweep(const uint64_t *keys, uint64_t *out, uint64_t n, int shift,
+                                                         const uint32_t *ghist, const uint32_t *vals,
+                                                         uint32_t *vout, uint64_t *state, uint32_t *misc,
+                                                         uint32_t nblocks, uint32_t epoch) {
+  __shared__ uint32_t wave_cnt[kWaves][256];
+  __shared__ uint64_t lds[kSortBlock / 64];
+  __shared__ uint32_t s_ticket;
+  const uint32_t b = lbk::ticket(misc, nblocks, &s_ticket);
+  if (b >= nblocks) return;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < kWaves * 256; i += kSortBlock) (&wave_cnt[0][0])[i] = 0;
+  __syncthreads();
+  uint64_t tile = static_cast<uint64_t>(b) * kSortTile;
+  uint64_t k[kSortItems];
+  uint32_t rank[kSortItems];
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    uint64_t idx = sort_index(tile, w, j, lane);
+    bool valid = idx < n;
+    k[j] = valid ? keys[idx] : 0;
+    uint32_t d = static_cast<uint32_t>((k[j] >> shift) & 0xFF);
+    uint64_t active = __ballot(valid);
+    uint64_t peers = active;
+#pragma unroll
+    for (int bb = 0; bb < 8; bb++) {
+      uint64_t m = __ballot((d >> bb) & 1u);
+      peers &= ((d >> bb) & 1u) ? m : ~m;
+    }
+    uint32_t r = __popcll(peers & lt);
+    int leader = valid ? __ffsll(static_cast<unsigned long long>(peers)) - 1 : 0;
+    uint32_t base = 0;
+    if (valid && lane == leader) {
+      base = wave_cnt[w][d];
+      wave_cnt[w][d] = base + static_cast<uint32_t>(__popcll(peers));
+    }
+    base = __shfl(base, leader, 64);
+    rank[j] = base + r;
+  }
+  __syncthreads();
+  const uint32_t d = threadIdx.x;  // 256 threads == 256 digits
+  uint32_t c = 0;
+  for (int ww = 0; ww < kWaves; ww++) c += wave_cnt[ww][d];
+  // global start of digit d: exclusive scan of the pass histogram
+  uint64_t gtot;
+  uint64_t gbase = block_excl_scan(static_cast<uint64_t>(ghist[d]), lds, &gtot);
+  // block prefix of digit d over the blocks before b
+  uint64_t excl = lbk::lookback(state, 256, b, d, misc, epoch, c);
+  uint64_t run = gbase + excl;
+  for (int ww = 0; ww < kWaves; ww++) {
+    uint32_t x = wave_cnt[ww][d];
+    wave_cnt[ww][d] = static_cast<uint32_t>(run);
+    run += x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    uint64_t idx = sort_index(tile, w, j, lane);
+    if (idx < n) {
+      uint32_t dd = static_cast<uint32_t>((k[j] >> shift) & 0xFF);
+      uint32_t pos = wave_cnt[w][dd] + rank[j];
+      out[pos] = k[j];
+      if (vals) vout[pos] = vals[idx];
+    }
+  }
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
@@ -269,6 +421,29 @@ void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t lo_bi
   ss.hist.ensure(static_cast<uint64_t>(nblocks) * 256, s);
   uint64_t *src = keys, *dst = ss.alt.get();
   uint32_t *vsrc = vals, *vdst = vals ? ss.alt_vals.get() : nullptr;
+  if (ss.scan.lookback) {
+    SortShifts sh{};
+    for (uint32_t b = 0; b < lo_bits; b += 8) sh.s[sh.np++] = static_cast<int>(b);
+    for (uint32_t b = 0; b < hi_bits; b += 8) sh.s[sh.np++] = static_cast<int>(32 + b);
+    if (sh.np > 8) throw Error(-1, "radix sort: more than 8 digit passes");
+    ss.hist.ensure(8 * 256, s);
+    fill_u32(ss.hist.get(), 0, static_cast<uint64_t>(sh.np) * 256, s);
+    k_radix_hist_all<<<nblocks, kSortBlock, 0, s>>>(src, n, sh, ss.hist.get());
+    TG_CHECK_LAUNCH();
+    for (int p = 0; p < sh.np; p++) {
+      uint32_t epoch = ss.scan.lb.begin(static_cast<uint64_t>(nblocks) * 256, s);
+      k_onesweep<<<nblocks, kSortBlock, 0, s>>>(src, dst, n, sh.s[p], ss.hist.get() + p * 256, vsrc, vdst,
+                                                ss.scan.lb.state.get(), ss.scan.lb.misc.get(), nblocks, epoch);
+      TG_CHECK_LAUNCH();
+      std::swap(src, dst);
+      std::swap(vsrc, vdst);
+    }
+    if (src != keys) {
+      TG_HIP(hipMemcpyAsync(keys, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+      if (vals) TG_HIP(hipMemcpyAsync(vals, vsrc, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    }
+    return;
+  }
   auto pass = [&](int shift) {
     k_radix_hist<<<nblocks, kSortBlock, 0, s>>>(src, n, shift, ss.hist.get(), nblocks);
     TG_CHECK_LAUNCH();
@@ -336,6 +511,26 @@ void copy_bytes(void *dst, const void *src, uint64_t bytes, hipStream_t s) {
                                                  static_cast<const uint8_t *>(src) + n16 * 16, tail);
     TG_CHECK_LAUNCH();
   }
+}
+
+uint32_t LookBack::begin(uint64_t words, hipStream_t s) {
+  if (misc.ensure(2, s)) fill_u32(misc.get(), 0, 2, s);
+  bool fresh = state.ensure(words, s);
+  if (fresh) fill_u64(state.get(), 0, state.cap, s);
+  if (++epoch >= (1u << 30)) {  // wrap: old words could match again
+    epoch = 1;
+    fill_u64(state.get(), 0, state.cap, s);
+  }
+  return epoch;
+}
+
+uint32_t check_lookback(ScanScratch &ss, hipStream_t s) {
+  if (!ss.lb.misc.get()) return 0;
+  uint32_t e = 0;
+  TG_HIP(hipMemcpyAsync(&e, ss.lb.misc.get() + 1, 4, hipMemcpyDeviceToHost, s));
+  TG_HIP(hipStreamSynchronize(s));
+  if (e) fill_u32(ss.lb.misc.get() + 1, 0, 1, s);
+  return e;
 }
 
 }  // namespace tg

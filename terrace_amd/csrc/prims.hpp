@@ -42,9 +42,102 @@ struct DevBuf {
   operator T *() const { return ptr; }
 };
 
+// Single-pass (decoupled look-back) state.  Blocks take tickets in the
+// order they start, so a block only waits on blocks already running; each
+// publishes its own total before it waits.  Words carry the launch's epoch,
+// so stale words from earlier launches never match and the array needs no
+// clearing between launches.  Every wait is bounded: a block that gives up
+// sets misc[1] and the caller reports it (check_lookback).
+struct LookBack {
+  DevBuf<uint64_t> state;
+  DevBuf<uint32_t> misc;  // [0] ticket counter, [1] error (a wait gave up)
+  uint32_t epoch = 0;
+  // Room for `words` state words; returns the epoch for the next launch.
+  uint32_t begin(uint64_t words, hipStream_t s);
+};
+
 struct ScanScratch {
   DevBuf<uint64_t> levels[6];
+  bool lookback = false;  // single-pass scan / select / sort on this scratch
+  LookBack lb;
 };
+
+namespace lbk {
+
+constexpr uint32_t kSpinLimit = 1u << 22;  // bounded wait (about 10^8 cycles)
+
+// One 64-bit word per published value: epoch << 34 | status << 32 | value
+// (status 1 = the block's own total, 2 = inclusive prefix).  The word is
+// the whole message, so relaxed agent-scope atomics suffice: they read and
+// write the device-coherent copy (XCDs have separate L2 caches) without
+// cache maintenance.
+__device__ inline uint64_t load(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t word(uint32_t epoch, uint32_t status, uint32_t value) {
+  return (static_cast<uint64_t>(epoch) << 34) | (static_cast<uint64_t>(status) << 32) | value;
+}
+
+// Block ticket in start order (one thread takes it, the block shares it).
+// The block that takes the last ticket resets the counter for the next
+// launch on the stream.  Returns nblocks when the counter was inconsistent
+// (the caller skips its work and the error word is set).
+__device__ inline uint32_t ticket(uint32_t *misc, uint32_t nblocks, uint32_t *lds_slot) {
+  if (threadIdx.x == 0) {
+    uint32_t v = atomicAdd(&misc[0], 1u);
+    if (v == nblocks - 1) atomicExch(&misc[0], 0u);
+    if (v >= nblocks) {
+      atomicOr(&misc[1], 2u);
+      v = nblocks;
+    }
+    *lds_slot = v;
+  }
+  __syncthreads();
+  return *lds_slot;
+}
+
+// Publishes agg for slot `me` and returns the sum of the values before it
+// (slots me - stride, me - 2 stride, ... down to index 0).  Totals must fit
+// 32 bits.  Run by one thread per slot.
+__device__ inline uint32_t lookback(uint64_t *state, uint64_t stride, uint32_t b, uint64_t col, uint32_t *misc,
+                                    uint32_t epoch, uint32_t agg) {
+  uint64_t *me = state + static_cast<uint64_t>(b) * stride + col;
+  if (b == 0) {
+    store(me, word(epoch, 2, agg));
+    return 0;
+  }
+  store(me, word(epoch, 1, agg));
+  uint32_t excl = 0;
+  int64_t p = static_cast<int64_t>(b) - 1;
+  uint32_t spins = 0;
+  const uint64_t tag = static_cast<uint64_t>(epoch) << 34;
+  while (p >= 0) {
+    uint64_t f = load(state + static_cast<uint64_t>(p) * stride + col);
+    uint32_t st = static_cast<uint32_t>(f >> 32) & 3u;
+    if ((f & ~((1ull << 34) - 1)) != tag || st == 0) {
+      if (++spins > kSpinLimit) {
+        atomicOr(&misc[1], 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += static_cast<uint32_t>(f);
+    if (st == 2) break;
+    p--;
+  }
+  store(me, word(epoch, 2, excl + agg));
+  return excl;
+}
+
+}  // namespace lbk
+
+// Nonzero when a single-pass kernel on this scratch gave up a wait
+// (synchronises the stream); clears the flag.
+uint32_t check_lookback(ScanScratch &ss, hipStream_t s);
 
 // out[i] = sum(in[0..i)), returns nothing; *total_dev (optional) gets the sum.
 // in and out may alias.  n may be 0.
