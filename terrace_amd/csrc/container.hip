@@ -396,6 +396,10 @@ __global__ void k_clear_marks(GView g, const uint32_t *T, const uint64_t *ntp, c
 // Tiny helpers that move device counts around without a host round trip.
 __global__ void k_put_u32(uint32_t *p, const uint64_t *idx, uint32_t v) { p[*idx] = v; }
 __global__ void k_u32_to_u64(const uint32_t *a, uint64_t *b) { *b = *a; }
+__global__ void k_u32_pair(const uint32_t *a, const uint32_t *b, uint64_t *out) {
+  out[0] = *a;
+  out[1] = *b;
+}
 __global__ void k_put_u32_at(uint32_t *p, uint64_t i, uint32_t v) { p[i] = v; }
 
 // ---- selectors (select.hpp): predicate + emit, no flag arrays
@@ -417,6 +421,13 @@ struct SelRoute {
   uint8_t want;
   uint64_t *out;
   __device__ bool pred(uint64_t i) const { return route[i] == want; }
+  __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = k[i]; }
+};
+struct SelFlagged {
+  const uint64_t *k;
+  const uint32_t *flag;
+  uint64_t *out;
+  __device__ bool pred(uint64_t i) const { return flag[i] != 0; }
   __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = k[i]; }
 };
 struct SelExisting {
@@ -1217,6 +1228,9 @@ Graph::Graph(uint32_t n) : n_(n) {
   fill_u64(dsc_.get(), 0, kScalars, stream_);
   scan_scratch.lookback = true;  // single-pass scans / selects / sort on the update path
   sort_.scan.lookback = true;
+  // their error words live in the batch scalars, read with the batch flags
+  scan_scratch.lb.err = reinterpret_cast<uint32_t *>(dsc_.get() + kErrScan);
+  sort_.scan.lb.err = reinterpret_cast<uint32_t *>(dsc_.get() + kErrSort);
   init_pma(2);
   ensure_trees(16);
   TG_HIP(hipStreamSynchronize(stream_));
@@ -1353,7 +1367,7 @@ uint64_t Graph::compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t
 uint64_t Graph::prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t count, uint64_t *&keys) {
   k0_.ensure(count, stream_);
   k1_.ensure(count, stream_);
-  fill_u64(dsc_.get(), 0, kScalars, stream_);
+  fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
   k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
   TG_CHECK_LAUNCH();
   uint32_t bits = 1;
@@ -1369,19 +1383,14 @@ uint64_t Graph::prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t 
 
 // Internal consistency flags raised by kernels during the batch.
 void Graph::check_batch_flags() {
-  uint64_t f = 0;
-  uint32_t lb[2] = {0, 0};
-  if (scan_scratch.lb.misc.get())
-    TG_HIP(hipMemcpyAsync(&lb[0], scan_scratch.lb.misc.get() + 1, 4, hipMemcpyDeviceToHost, stream_));
-  if (sort_.scan.lb.misc.get())
-    TG_HIP(hipMemcpyAsync(&lb[1], sort_.scan.lb.misc.get() + 1, 4, hipMemcpyDeviceToHost, stream_));
-  read_scalars(dsc_.get(), &f, 1, stream_);
-  if (lb[0] | lb[1]) {
-    if (lb[0]) fill_u32(scan_scratch.lb.misc.get() + 1, 0, 1, stream_);
-    if (lb[1]) fill_u32(sort_.scan.lb.misc.get() + 1, 0, 1, stream_);
+  uint64_t h[kScalars];
+  read_scalars(dsc_.get(), h, kScalars, stream_);
+  uint64_t f = h[0];
+  if (f) fill_u64(dsc_.get(), 0, 1, stream_);  // leave the flags clear for the next batch
+  if (h[kErrScan] | h[kErrSort]) {
+    fill_u64(dsc_.get() + kErrSort, 0, 2, stream_);
     throw Error(-4, "single-pass scan: a look-back wait gave up");
   }
-  if (f) fill_u64(dsc_.get(), 0, 1, stream_);  // leave the flags clear for the next batch
   if (f & 2) throw Error(-4, "PMA count tree out of sync (gather)");
   if (f & 4) throw Error(-4, "PMA delete: key not found");
 }
@@ -1447,13 +1456,15 @@ void Graph::pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total) {
   stats_.pma_rebuilds++;
 }
 
-void Graph::pma_insert(const uint64_t *keys, uint64_t k) {
+void Graph::pma_insert(const uint64_t *keys, uint64_t k, uint64_t root_known) {
   if (k == 0) return;
   if (k > 0xFFFFFFFFull) throw Error(-1, "batch too large");
   rebuild_count_tree();
   GView g = view();
   uint64_t nl = 1ull << pma_H_;
-  uint64_t root_old = read_scalar(pcnt_.get() + tree_level_off(pma_H_, pma_H_), stream_);
+  uint64_t root_old = root_known != kRootUnknown
+                          ? root_known
+                          : read_scalar(pcnt_.get() + tree_level_off(pma_H_, pma_H_), stream_);
   uint64_t total = root_old + k;
   if (total > bounds_.up[pma_H_]) {
     pma_rebuild(keys, k, total);
@@ -1554,7 +1565,19 @@ void Graph::tree_insert(const uint64_t *keys, uint64_t k) {
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(ob.get(), ob.get(), ng, ob.get() + ng, scan_scratch, stream_);
   exclusive_scan_u32(ho.get(), ho.get(), ng, ho.get() + ng, scan_scratch, stream_);
-  uint64_t nout = read_scalar(ob.get() + ng, stream_);
+  gflag.ensure(ng + 1, stream_);
+  gpos.ensure(ng + 1, stream_);
+  grun.ensure(ng + 1, stream_);
+  k_flag_tree_change<<<grid_for(ng, 256), 256, 0, stream_>>>(grp, ng, gflag.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(gflag.get(), gpos.get(), ng, gpos.get() + ng, scan_scratch, stream_);
+  // out-blocks and touched-tree runs: one read
+  k_u32_pair<<<1, 1, 0, stream_>>>(ob.get() + ng, gpos.get() + ng, dsc_.get() + 11);
+  TG_CHECK_LAUNCH();
+  uint64_t h2[2];
+  read_scalars(dsc_.get() + 11, h2, 2, stream_);
+  uint64_t nout = h2[0];
+  uint32_t ntr = static_cast<uint32_t>(h2[1]);
   uint64_t blk_base = blk_top_;
   ensure_blocks(blk_top_ + nout);
   blk_top_ += nout;
@@ -1564,13 +1587,6 @@ void Graph::tree_insert(const uint64_t *keys, uint64_t k) {
   TG_CHECK_LAUNCH();
   stats_.tree_blocks_written += nout;
   // index rebuild for the touched trees
-  gflag.ensure(ng + 1, stream_);
-  gpos.ensure(ng + 1, stream_);
-  grun.ensure(ng + 1, stream_);
-  k_flag_tree_change<<<grid_for(ng, 256), 256, 0, stream_>>>(grp, ng, gflag.get());
-  TG_CHECK_LAUNCH();
-  exclusive_scan_u32(gflag.get(), gpos.get(), ng, gpos.get() + ng, scan_scratch, stream_);
-  uint32_t ntr = read_scalar(gpos.get() + ng, stream_);
   k_grp_run<<<grid_for(ng, 256), 256, 0, stream_>>>(gflag.get(), gpos.get(), ng, grun.get());
   TG_CHECK_LAUNCH();
   DevBuf<uint8_t> &trbuf = s8_[2];
@@ -1587,8 +1603,11 @@ void Graph::tree_insert(const uint64_t *keys, uint64_t k) {
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(nbn.get(), nbn.get(), ntr, nbn.get() + ntr, scan_scratch, stream_);
   exclusive_scan_u32(nbo.get(), nbo.get(), ntr, nbo.get() + ntr, scan_scratch, stream_);
-  uint64_t nidx_new = read_scalar(nbn.get() + ntr, stream_);
-  uint64_t nidx_old = read_scalar(nbo.get() + ntr, stream_);
+  k_u32_pair<<<1, 1, 0, stream_>>>(nbn.get() + ntr, nbo.get() + ntr, dsc_.get() + 11);
+  TG_CHECK_LAUNCH();
+  read_scalars(dsc_.get() + 11, h2, 2, stream_);
+  uint64_t nidx_new = h2[0];
+  uint64_t nidx_old = h2[1];
   uint64_t idx_base = idx_top_;
   ensure_index(idx_top_ + nidx_new);
   idx_top_ += nidx_new;
@@ -1602,7 +1621,6 @@ void Graph::tree_insert(const uint64_t *keys, uint64_t k) {
   TG_CHECK_LAUNCH();
   k_tree_commit<<<grid_for(ntr, 256), 256, 0, stream_>>>(g, tr, ntr, nbn.get(), idx_base);
   TG_CHECK_LAUNCH();
-  TG_HIP(hipStreamSynchronize(stream_));
 }
 
 void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
@@ -1631,7 +1649,19 @@ void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(ob.get(), ob.get(), ng, ob.get() + ng, scan_scratch, stream_);
   exclusive_scan_u32(ho.get(), ho.get(), ng, ho.get() + ng, scan_scratch, stream_);
-  uint64_t nout = read_scalar(ob.get() + ng, stream_);
+  gflag.ensure(ng + 1, stream_);
+  gpos.ensure(ng + 1, stream_);
+  grun.ensure(ng + 1, stream_);
+  k_flag_tree_change<<<grid_for(ng, 256), 256, 0, stream_>>>(grp, ng, gflag.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(gflag.get(), gpos.get(), ng, gpos.get() + ng, scan_scratch, stream_);
+  // out-blocks and touched-tree runs: one read
+  k_u32_pair<<<1, 1, 0, stream_>>>(ob.get() + ng, gpos.get() + ng, dsc_.get() + 11);
+  TG_CHECK_LAUNCH();
+  uint64_t h2[2];
+  read_scalars(dsc_.get() + 11, h2, 2, stream_);
+  uint64_t nout = h2[0];
+  uint32_t ntr = static_cast<uint32_t>(h2[1]);
   uint64_t blk_base = blk_top_;
   ensure_blocks(blk_top_ + nout);
   blk_top_ += nout;
@@ -1640,13 +1670,6 @@ void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
       g, grp, ob.get(), ng, blk_base, keys);
   TG_CHECK_LAUNCH();
   stats_.tree_blocks_written += nout;
-  gflag.ensure(ng + 1, stream_);
-  gpos.ensure(ng + 1, stream_);
-  grun.ensure(ng + 1, stream_);
-  k_flag_tree_change<<<grid_for(ng, 256), 256, 0, stream_>>>(grp, ng, gflag.get());
-  TG_CHECK_LAUNCH();
-  exclusive_scan_u32(gflag.get(), gpos.get(), ng, gpos.get() + ng, scan_scratch, stream_);
-  uint32_t ntr = read_scalar(gpos.get() + ng, stream_);
   k_grp_run<<<grid_for(ng, 256), 256, 0, stream_>>>(gflag.get(), gpos.get(), ng, grun.get());
   TG_CHECK_LAUNCH();
   DevBuf<uint8_t> &trbuf = s8_[2];
@@ -1663,8 +1686,11 @@ void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(nbn.get(), nbn.get(), ntr, nbn.get() + ntr, scan_scratch, stream_);
   exclusive_scan_u32(nbo.get(), nbo.get(), ntr, nbo.get() + ntr, scan_scratch, stream_);
-  uint64_t nidx_new = read_scalar(nbn.get() + ntr, stream_);
-  uint64_t nidx_old = read_scalar(nbo.get() + ntr, stream_);
+  k_u32_pair<<<1, 1, 0, stream_>>>(nbn.get() + ntr, nbo.get() + ntr, dsc_.get() + 11);
+  TG_CHECK_LAUNCH();
+  read_scalars(dsc_.get() + 11, h2, 2, stream_);
+  uint64_t nidx_new = h2[0];
+  uint64_t nidx_old = h2[1];
   uint64_t idx_base = idx_top_;
   ensure_index(idx_top_ + nidx_new + 1);
   idx_top_ += nidx_new;
@@ -1678,7 +1704,6 @@ void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
   TG_CHECK_LAUNCH();
   k_tree_commit<<<grid_for(ntr, 256), 256, 0, stream_>>>(g, tr, ntr, nbn.get(), idx_base);
   TG_CHECK_LAUNCH();
-  TG_HIP(hipStreamSynchronize(stream_));
 }
 
 // ------------------------------------------------------------ batches
@@ -1749,10 +1774,15 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
   exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_);
   k_u32_to_u64<<<1, 1, 0, stream_>>>(ppos.get() + nruns, dsc_.get() + 5);
   TG_CHECK_LAUNCH();
-  uint64_t h[3];
-  read_scalars(dsc_.get() + 3, h, 3, stream_);
+  // the PMA's element count (count tree root) comes back with the counts
+  rebuild_count_tree();
+  k_u32_to_u64<<<1, 1, 0, stream_>>>(pcnt_.get() + tree_level_off(pma_H_, pma_H_), dsc_.get() + 6);
+  TG_CHECK_LAUNCH();
+  uint64_t h[4];
+  read_scalars(dsc_.get() + 3, h, 4, stream_);
   uint64_t npma = h[0], ntree = h[1];
   uint32_t np = static_cast<uint32_t>(h[2]);
+  uint64_t pma_root = h[3];
   // promotions: new trees for vertices routed to the B-tier the first time
   uint64_t next = 0;
   DevBuf<uint64_t> &ext = s64_[0];
@@ -1795,7 +1825,7 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
     radix_sort_keys(g0_.get(), ntree, bits, bits, sort_, stream_);
   }
   tree_insert(g0_.get(), ntree);
-  pma_insert(k3_.get(), npma);
+  pma_insert(k3_.get(), npma, np ? kRootUnknown : pma_root);  // promotions changed the PMA
   check_batch_flags();
 }
 
@@ -1816,7 +1846,10 @@ void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
   GView g = view();
   // keep the keys that exist (absent deletes are no-ops)
   k2_.ensure(nR0 + 1, stream_);
-  select(SelExisting{g, R0, k2_.get()}, nR0, dsc_.get() + 2, scan_scratch, stream_);
+  f0_.ensure(nR0 + 1, stream_);
+  k_exists<<<grid_for(nR0, 256), 256, 0, stream_>>>(g, R0, nR0, f0_.get());  // one search per thread
+  TG_CHECK_LAUNCH();
+  select(SelFlagged{R0, f0_.get(), k2_.get()}, nR0, dsc_.get() + 2, scan_scratch, stream_);
   uint64_t nR = read_scalar(dsc_.get() + 2, stream_);
   if (nR == 0) return;
   uint64_t *R = k2_.get();
@@ -1885,7 +1918,7 @@ void Graph::apply_ops(const uint32_t *src, const uint32_t *dst, const uint32_t *
   if (count == 0) return;
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
   k0_.ensure(count, stream_);
-  fill_u64(dsc_.get(), 0, kScalars, stream_);
+  fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
   k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
   TG_CHECK_LAUNCH();
   ops_val_.ensure(count, stream_);

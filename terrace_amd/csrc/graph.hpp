@@ -169,6 +169,7 @@ class Graph {
   DevBuf<uint64_t> scal_;  // small device scalars
   // batch scalars: [0] error bits, [1..] counts read back in one copy
   static constexpr int kScalars = 16;
+  static constexpr int kErrSort = 14, kErrScan = 15;  // look-back error words (sort, scans/selects)
   DevBuf<uint64_t> dsc_;
   DevBuf<uint64_t> tmp_old_;
   DevBuf<uint32_t> ops_val_;
@@ -184,7 +185,9 @@ class Graph {
   uint64_t prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t count, uint64_t *&keys);
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
   void check_batch_flags();
-  void pma_insert(const uint64_t *keys, uint64_t k);
+  static constexpr uint64_t kRootUnknown = ~0ull;
+  // root_known: the PMA's element count when the caller has just read it
+  void pma_insert(const uint64_t *keys, uint64_t k, uint64_t root_known = kRootUnknown);
   void pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total);
   void rebuild_count_tree();
   void pma_delete(const uint64_t *keys, uint64_t k);

@@ -115,13 +115,13 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_tile(const InT *in, InT *ou
 // prefix (one launch for any n).
 template <typename InT>
 __global__ void __launch_bounds__(kScanBlock) k_scan_lb(const InT *in, InT *out, uint64_t n, InT *total_dev,
-                                                        uint64_t *state, uint32_t *misc, uint32_t nblocks,
-                                                        uint32_t epoch) {
+                                                        uint64_t *state, uint32_t *misc, uint32_t *err,
+                                                        uint32_t nblocks, uint32_t epoch) {
   __shared__ uint64_t lds[kScanBlock / 64];
   __shared__ InT tile[kScanTile];
   __shared__ uint32_t s_ticket;
   __shared__ uint64_t s_excl;
-  const uint32_t b = lbk::ticket(misc, nblocks, &s_ticket);
+  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
   if (b >= nblocks) return;  // inconsistent ticket counter: reported, no work
   const uint64_t tbase = static_cast<uint64_t>(b) * kScanTile;
 #pragma unroll
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_lb(const InT *in, InT *out,
   }
   uint64_t tot;
   uint64_t pre = block_excl_scan(sum, lds, &tot);
-  if (threadIdx.x == 0) s_excl = lbk::lookback(state, 1, b, 0, misc, epoch, static_cast<uint32_t>(tot));
+  if (threadIdx.x == 0) s_excl = lbk::lookback(state, 1, b, 0, err, epoch, static_cast<uint32_t>(tot));
   __syncthreads();
   const uint64_t excl = s_excl;
   pre += excl;
@@ -180,7 +180,8 @@ void scan_impl(const InT *in, InT *out, uint64_t n, InT *total_dev, ScanScratch 
   if (ss.lookback && sizeof(InT) == 4 && nblocks <= 0xFFFFFFFFull) {
     uint32_t nb = static_cast<uint32_t>(nblocks);
     uint32_t epoch = ss.lb.begin(nb, s);
-    k_scan_lb<InT><<<nb, kScanBlock, 0, s>>>(in, out, n, total_dev, ss.lb.state.get(), ss.lb.misc.get(), nb, epoch);
+    k_scan_lb<InT><<<nb, kScanBlock, 0, s>>>(in, out, n, total_dev, ss.lb.state.get(), ss.lb.misc.get(),
+                                             ss.lb.err_word(), nb, epoch);
     TG_CHECK_LAUNCH();
     return;
   }
@@ -319,11 +320,11 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_hist_all(const uint64_t *k
 __global__ void __launch_bounds__(kSortBlock) k_onesweep(const uint64_t *keys, uint64_t *out, uint64_t n, int shift,
                                                          const uint32_t *ghist, const uint32_t *vals,
                                                          uint32_t *vout, uint64_t *state, uint32_t *misc,
-                                                         uint32_t nblocks, uint32_t epoch) {
+                                                         uint32_t *err, uint32_t nblocks, uint32_t epoch) {
   __shared__ uint32_t wave_cnt[kWaves][256];
   __shared__ uint64_t lds[kSortBlock / 64];
   __shared__ uint32_t s_ticket;
-  const uint32_t b = lbk::ticket(misc, nblocks, &s_ticket);
+  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
   if (b >= nblocks) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < kWaves * 256; i += kSortBlock) (&wave_cnt[0][0])[i] = 0;
@@ -363,7 +364,7 @@ __global__ void __launch_bounds__(kSortBlock) k_onesweep(const uint64_t *keys, u
   uint64_t gtot;
   uint64_t gbase = block_excl_scan(static_cast<uint64_t>(ghist[d]), lds, &gtot);
   // block prefix of digit d over the blocks before b
-  uint64_t excl = lbk::lookback(state, 256, b, d, misc, epoch, c);
+  uint64_t excl = lbk::lookback(state, 256, b, d, err, epoch, c);
   uint64_t run = gbase + excl;
   for (int ww = 0; ww < kWaves; ww++) {
     uint32_t x = wave_cnt[ww][d];
@@ -433,7 +434,8 @@ void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t lo_bi
     for (int p = 0; p < sh.np; p++) {
       uint32_t epoch = ss.scan.lb.begin(static_cast<uint64_t>(nblocks) * 256, s);
       k_onesweep<<<nblocks, kSortBlock, 0, s>>>(src, dst, n, sh.s[p], ss.hist.get() + p * 256, vsrc, vdst,
-                                                ss.scan.lb.state.get(), ss.scan.lb.misc.get(), nblocks, epoch);
+                                                ss.scan.lb.state.get(), ss.scan.lb.misc.get(), ss.scan.lb.err_word(),
+                                                nblocks, epoch);
       TG_CHECK_LAUNCH();
       std::swap(src, dst);
       std::swap(vsrc, vdst);
@@ -527,9 +529,9 @@ uint32_t LookBack::begin(uint64_t words, hipStream_t s) {
 uint32_t check_lookback(ScanScratch &ss, hipStream_t s) {
   if (!ss.lb.misc.get()) return 0;
   uint32_t e = 0;
-  TG_HIP(hipMemcpyAsync(&e, ss.lb.misc.get() + 1, 4, hipMemcpyDeviceToHost, s));
+  TG_HIP(hipMemcpyAsync(&e, ss.lb.err_word(), 4, hipMemcpyDeviceToHost, s));
   TG_HIP(hipStreamSynchronize(s));
-  if (e) fill_u32(ss.lb.misc.get() + 1, 0, 1, s);
+  if (e) fill_u32(ss.lb.err_word(), 0, 1, s);
   return e;
 }
 

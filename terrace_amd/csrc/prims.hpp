@@ -51,7 +51,9 @@ struct DevBuf {
 struct LookBack {
   DevBuf<uint64_t> state;
   DevBuf<uint32_t> misc;  // [0] ticket counter, [1] error (a wait gave up)
+  uint32_t *err = nullptr;  // error word elsewhere (null: misc + 1)
   uint32_t epoch = 0;
+  uint32_t *err_word() { return err ? err : misc.get() + 1; }
   // Room for `words` state words; returns the epoch for the next launch.
   uint32_t begin(uint64_t words, hipStream_t s);
 };
@@ -85,12 +87,12 @@ __device__ inline uint64_t word(uint32_t epoch, uint32_t status, uint32_t value)
 // The block that takes the last ticket resets the counter for the next
 // launch on the stream.  Returns nblocks when the counter was inconsistent
 // (the caller skips its work and the error word is set).
-__device__ inline uint32_t ticket(uint32_t *misc, uint32_t nblocks, uint32_t *lds_slot) {
+__device__ inline uint32_t ticket(uint32_t *misc, uint32_t *err, uint32_t nblocks, uint32_t *lds_slot) {
   if (threadIdx.x == 0) {
     uint32_t v = atomicAdd(&misc[0], 1u);
     if (v == nblocks - 1) atomicExch(&misc[0], 0u);
     if (v >= nblocks) {
-      atomicOr(&misc[1], 2u);
+      atomicOr(err, 2u);
       v = nblocks;
     }
     *lds_slot = v;
@@ -102,7 +104,7 @@ __device__ inline uint32_t ticket(uint32_t *misc, uint32_t nblocks, uint32_t *ld
 // Publishes agg for slot `me` and returns the sum of the values before it
 // (slots me - stride, me - 2 stride, ... down to index 0).  Totals must fit
 // 32 bits.  Run by one thread per slot.
-__device__ inline uint32_t lookback(uint64_t *state, uint64_t stride, uint32_t b, uint64_t col, uint32_t *misc,
+__device__ inline uint32_t lookback(uint64_t *state, uint64_t stride, uint32_t b, uint64_t col, uint32_t *err,
                                     uint32_t epoch, uint32_t agg) {
   uint64_t *me = state + static_cast<uint64_t>(b) * stride + col;
   if (b == 0) {
@@ -119,7 +121,7 @@ __device__ inline uint32_t lookback(uint64_t *state, uint64_t stride, uint32_t b
     uint32_t st = static_cast<uint32_t>(f >> 32) & 3u;
     if ((f & ~((1ull << 34) - 1)) != tag || st == 0) {
       if (++spins > kSpinLimit) {
-        atomicOr(&misc[1], 1u);
+        atomicOr(err, 1u);
         break;
       }
       __builtin_amdgcn_s_sleep(1);

@@ -78,22 +78,27 @@ __global__ void __launch_bounds__(kBlock) k_emit(F f, uint64_t n, const uint64_t
 // by decoupled look-back, then the tile is emitted (one launch for any n).
 template <class F>
 __global__ void __launch_bounds__(kBlock) k_select_lb(F f, uint64_t n, uint64_t *total, uint64_t *state,
-                                                      uint32_t *misc, uint32_t nblocks, uint32_t epoch) {
+                                                      uint32_t *misc, uint32_t *err, uint32_t nblocks,
+                                                      uint32_t epoch) {
   __shared__ uint32_t wcnt[kItems][kBlock / 64];
   __shared__ uint32_t wpre[kItems][kBlock / 64];
   __shared__ uint32_t s_ticket;
   __shared__ uint64_t s_off;
-  const uint32_t b = lbk::ticket(misc, nblocks, &s_ticket);
+  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
   if (b >= nblocks) return;
   const uint32_t w = threadIdx.x >> 6;
   const uint64_t lt = lanemask_lt();
   uint64_t base = static_cast<uint64_t>(b) * kTile;
   bool p[kItems];
   uint32_t r[kItems];
+  // all predicates first, so their memory accesses overlap
 #pragma unroll
   for (int i = 0; i < kItems; i++) {
     uint64_t idx = base + static_cast<uint64_t>(i) * kBlock + threadIdx.x;
     p[i] = idx < n && f.pred(idx);
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; i++) {
     uint64_t m = __ballot(p[i]);
     r[i] = __popcll(m & lt);
     if ((threadIdx.x & 63) == 0) wcnt[i][w] = __popcll(m);
@@ -106,7 +111,7 @@ __global__ void __launch_bounds__(kBlock) k_select_lb(F f, uint64_t n, uint64_t 
         wpre[i][ww] = run;
         run += wcnt[i][ww];
       }
-    uint64_t off = lbk::lookback(state, 1, b, 0, misc, epoch, run);
+    uint64_t off = lbk::lookback(state, 1, b, 0, err, epoch, run);
     s_off = off;
     if (total && b == nblocks - 1) *total = off + run;
   }
@@ -140,7 +145,8 @@ void select(const F &f, uint64_t n, uint64_t *total_dev, ScanScratch &ss, hipStr
   if (ss.lookback && n <= 0xFFFFFFFFull) {  // counts fit the look-back word
     uint32_t nb = static_cast<uint32_t>(nblocks);
     uint32_t epoch = ss.lb.begin(nb, s);
-    sel::k_select_lb<F><<<nb, sel::kBlock, 0, s>>>(f, n, total_dev, ss.lb.state.get(), ss.lb.misc.get(), nb, epoch);
+    sel::k_select_lb<F><<<nb, sel::kBlock, 0, s>>>(f, n, total_dev, ss.lb.state.get(), ss.lb.misc.get(),
+                                                   ss.lb.err_word(), nb, epoch);
     TG_CHECK_LAUNCH();
     return;
   }
