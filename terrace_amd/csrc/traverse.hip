@@ -860,30 +860,49 @@ __global__ void k_tc_orient_off(const uint64_t *off, const uint32_t *pos, uint32
 
 constexpr uint32_t kTcLds = 2048;  // out-list entries staged in LDS per wave
 
-// One wave per vertex u: out(u) staged in LDS (searched in place in HBM
-// when longer than kTcLds), then every w in out(v), v in out(u), is looked
-// up by binary search.
-__global__ void __launch_bounds__(256) k_tc_count(const uint64_t *ooff, const uint32_t *out, uint32_t n,
+constexpr uint32_t kTcChunk = 32;  // out-neighbours v per task
+
+// Tasks of vertex u: ceil(|out(u)| / kTcChunk) when u has two or more
+// out-neighbours (a triangle needs two), else none.
+__global__ void k_tc_task_counts(const uint64_t *ooff, uint32_t n, uint32_t *cnt) {
+  GRID_STRIDE(u, n) {
+    uint64_t du = ooff[u + 1] - ooff[u];
+    cnt[u] = du < 2 ? 0u : static_cast<uint32_t>((du + kTcChunk - 1) / kTcChunk);
+  }
+}
+
+__global__ void k_tc_task_map(const uint32_t *toff, uint32_t n, uint32_t *tu) {
+  GRID_STRIDE(u, n) for (uint32_t t = toff[u]; t < toff[u + 1]; t++) tu[t] = static_cast<uint32_t>(u);
+}
+
+// One wave per task (u, chunk of out(u)): out(u) staged in LDS (searched in
+// place in HBM when longer than kTcLds); every w in out(v), v in the chunk,
+// is looked up by binary search.  Chunks bound the work of one wave, so a
+// few heavy vertices do not serialise the launch.
+__global__ void __launch_bounds__(256) k_tc_count(const uint64_t *ooff, const uint32_t *out, const uint32_t *toff,
+                                                  const uint32_t *tu, uint64_t ntasks,
                                                   unsigned long long *total) {
   __shared__ uint32_t lds[4][kTcLds];
   uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
   uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   unsigned long long local = 0;
-  for (uint64_t u = wave; u < n; u += nwaves) {
+  for (uint64_t t = wave; t < ntasks; t += nwaves) {
+    uint32_t u = tu[t];
     uint64_t ub = ooff[u];
     uint32_t du = static_cast<uint32_t>(ooff[u + 1] - ub);
-    if (du < 2) continue;  // uniform across the wave
+    uint32_t kb = static_cast<uint32_t>(t - toff[u]) * kTcChunk;
+    uint32_t ke = kb + kTcChunk < du ? kb + kTcChunk : du;
     const uint32_t *au = out + ub;
     bool staged = du <= kTcLds;
     if (staged) {
+      __builtin_amdgcn_wave_barrier();  // the previous task's searches are done
       for (uint32_t x = lane; x < du; x += 64) lds[wid][x] = au[x];
-      __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
     }
     const uint32_t *set = staged ? lds[wid] : au;
-    for (uint32_t k = 0; k < du; k++) {
+    for (uint32_t k = kb; k < ke; k++) {
       uint32_t v = au[k];
       uint64_t vb = ooff[v], ve = ooff[v + 1];
       for (uint64_t x = vb + lane; x < ve; x += 64) {
@@ -897,7 +916,6 @@ __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *ooff, const ui
         local += (lo < du && set[lo] == w) ? 1ull : 0ull;
       }
     }
-    __builtin_amdgcn_wave_barrier();  // the next vertex may overwrite the staged list
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
@@ -1850,9 +1868,22 @@ uint64_t Traversal::tc() {
     k_tc_orient_off<<<grid_for(static_cast<uint64_t>(n) + 1, 256), 256, 0, s>>>(csr_off_.get(), pos.get(), n, m,
                                                                                ooff.get());
     TG_CHECK_LAUNCH();
-    k_tc_count<<<grid_for(static_cast<uint64_t>(n) * 64, 256, 1u << 16), 256, 0, s>>>(
-        ooff.get(), out.get(), n, reinterpret_cast<unsigned long long *>(red_.get()));
+    // tasks: (vertex, chunk of its out-list)
+    DevBuf<uint32_t> tcnt, toff, tu;
+    tcnt.ensure(static_cast<uint64_t>(n) + 1, s);
+    toff.ensure(static_cast<uint64_t>(n) + 1, s);
+    k_tc_task_counts<<<grid_for(n, 256), 256, 0, s>>>(ooff.get(), n, tcnt.get());
     TG_CHECK_LAUNCH();
+    exclusive_scan_u32(tcnt.get(), toff.get(), n, toff.get() + n, scan_, s);
+    uint32_t ntasks = read_scalar(toff.get() + n, s);
+    if (ntasks) {
+      tu.ensure(ntasks, s);
+      k_tc_task_map<<<grid_for(n, 256), 256, 0, s>>>(toff.get(), n, tu.get());
+      TG_CHECK_LAUNCH();
+      k_tc_count<<<grid_for(static_cast<uint64_t>(ntasks) * 64, 256, 1u << 16), 256, 0, s>>>(
+          ooff.get(), out.get(), toff.get(), tu.get(), ntasks, reinterpret_cast<unsigned long long *>(red_.get()));
+      TG_CHECK_LAUNCH();
+    }
     TG_HIP(hipStreamSynchronize(s));
   }
   if (profiling) {
