@@ -251,6 +251,16 @@ int tg_dev_alloc(uint64_t bytes, void **out);
 int tg_dev_free(void *p);
 int tg_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
 int tg_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+/* Ligra AdjacencyGraph text file (get_edges_from_file_adj_sym,
+ * include/io_util.h:112-145): "AdjacencyGraph", n, m, n offsets, m ids.
+ * tg_read_adjacency_graph fills n and m, and with src/dst given (room for
+ * cap >= m entries) the m edges in file order.  A malformed file returns
+ * TG_EINVAL with the reason in tg_last_error (the reference exits).
+ * tg_load_adjacency_graph creates a graph of n vertices and inserts the
+ * file's edges as one batch (the load in src/graph_bm.cc:236-266). */
+int tg_read_adjacency_graph(const char *path, uint32_t *n, uint64_t *m, uint32_t *src, uint32_t *dst,
+                            uint64_t cap);
+int tg_load_adjacency_graph(const char *path, tg_graph **out);
 /* Device-to-device copy by a plain vectorised kernel (synchronous). */
 int tg_memcpy_d2d(void *dst, const void *src, uint64_t bytes);
 int tg_device_sync(void);

@@ -133,6 +133,9 @@ _SIGS = {
     "tg_memcpy_h2d": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
     "tg_memcpy_d2h": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
     "tg_memcpy_d2d": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
+    "tg_read_adjacency_graph": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32),
+                                               ctypes.POINTER(ctypes.c_uint64), _P, _P, ctypes.c_uint64]),
+    "tg_load_adjacency_graph": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
     "tg_device_sync": (ctypes.c_int, []),
     "tg_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "tg_profile_reset": (ctypes.c_int, [_P]),

@@ -9,6 +9,7 @@
 #include <string>
 
 #include "gen.hpp"
+#include "io.hpp"
 #include "serve.hpp"
 #include "traverse.hpp"
 
@@ -749,6 +750,35 @@ int tg_memcpy_d2d(void *dst, const void *src, uint64_t bytes) {
   return guard([&] {
     tg::copy_bytes(dst, src, bytes, nullptr);
     TG_HIP(hipStreamSynchronize(nullptr));
+  });
+}
+
+int tg_read_adjacency_graph(const char *path, uint32_t *n, uint64_t *m, uint32_t *src, uint32_t *dst, uint64_t cap) {
+  return guard([&] {
+    NEED(path && n && m, "null argument");
+    tg::AdjFile f = tg::read_adjacency_graph(path);
+    *n = f.n;
+    *m = f.m;
+    if (src || dst) {
+      NEED(src && dst, "src and dst must both be given");
+      NEED(cap >= f.m, "capacity below m");
+      std::copy(f.src.begin(), f.src.end(), src);
+      std::copy(f.dst.begin(), f.dst.end(), dst);
+    }
+  });
+}
+
+int tg_load_adjacency_graph(const char *path, tg_graph **out) {
+  return guard([&] {
+    NEED(path && out, "null argument");
+    *out = nullptr;
+    tg::AdjFile f = tg::read_adjacency_graph(path);
+    std::unique_ptr<tg_graph> G(new tg_graph(f.n));
+    if (f.m) {
+      stage_pair(G.get(), f.src.data(), f.dst.data(), f.m);
+      G->g.insert_batch(G->s_src.get(), G->s_dst.get(), f.m);
+    }
+    *out = G.release();
   });
 }
 

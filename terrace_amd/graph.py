@@ -32,6 +32,19 @@ class Graph:
         self._h = h
         self._n = int(size)
 
+    @classmethod
+    def from_adjacency_graph(cls, path: str) -> "Graph":
+        """A graph holding the edges of a Ligra AdjacencyGraph file (the load
+        in src/graph_bm.cc:236-266)."""
+        h = ctypes.c_void_p()
+        check(lib().tg_load_adjacency_graph(str(path).encode(), ctypes.byref(h)))
+        G = cls.__new__(cls)
+        G._h = h
+        n = ctypes.c_uint32()
+        check(lib().tg_num_vertices(h, ctypes.byref(n)))
+        G._n = n.value
+        return G
+
     # -- lifetime
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -293,6 +306,32 @@ def bfs_multi(G: Graph, sources) -> np.ndarray:
     out = np.empty((len(srcs), G.get_num_vertices()), np.uint32)
     check(lib().tg_bfs_multi(G.handle, ptr(srcs), len(srcs), ptr(out)))
     return out
+
+
+def read_adjacency_graph(path: str):
+    """Ligra AdjacencyGraph file (include/io_util.h:112-145): (n, src, dst)
+    with the edges in file order."""
+    n, m = ctypes.c_uint32(), ctypes.c_uint64()
+    check(lib().tg_read_adjacency_graph(str(path).encode(), ctypes.byref(n), ctypes.byref(m), None, None, 0))
+    src = np.empty(m.value, np.uint32)
+    dst = np.empty(m.value, np.uint32)
+    check(lib().tg_read_adjacency_graph(str(path).encode(), ctypes.byref(n), ctypes.byref(m), ptr(src), ptr(dst),
+                                        m.value))
+    return n.value, src, dst
+
+
+def write_adjacency_graph(path: str, n: int, src, dst) -> None:
+    """Writes edges (sorted by source) as a Ligra AdjacencyGraph file."""
+    src, dst = _u32(src), _u32(dst)
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.searchsorted(src, np.arange(n, dtype=np.uint32), side="left").astype(np.int64)
+    with open(path, "w") as f:
+        f.write(f"AdjacencyGraph\n{n}\n{len(dst)}\n")
+        f.write("\n".join(map(str, off.tolist())))
+        f.write("\n")
+        f.write("\n".join(map(str, dst.tolist())))
+        f.write("\n")
 
 
 def bc_multi(G: Graph, sources) -> np.ndarray:

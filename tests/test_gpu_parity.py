@@ -314,3 +314,29 @@ def test_skew_all_tiers_hub_targeted_batches():
     g = oracle.CSR(keys, n)
     check_algorithms(G, g, int(np.argmax(g.degrees())), what="skew")
     G.close()
+
+
+def test_cfg_cpu_small_full_size():
+    """The reference's CPU-runnable workload at full size: rMAT scale 20,
+    8M generated edges (a=.5 b=.1 c=.1, nominal generator, static seed
+    ith_rand(100)), symmetrized; static build, then 10 insert batches of 1e5
+    generated edges (seeds ith_rand(1000 + t)); BFS from 9, PageRank 10
+    iterations, CC, BC from 9 and TC, all against the oracle."""
+    n = 1 << 20
+    say("cfg cpu-small full size")
+    seed = oracle.aspen_ith_rand(100) & 0xFFFFFFFF
+    keys = oracle.build_keys(*oracle.rmat_edges(n, seed, 0.5, 0.1, 0.1, 8_000_000))
+    assert len(keys) == 15_706_884  # the survey's measured count for this workload
+    G = tg.Graph(n)
+    G.add_edge_batch(*split(keys))
+    assert_adjacency(G, keys, n, "static")
+    for t in range(10):
+        ins = oracle.build_keys(*oracle.rmat_edges(n, oracle.aspen_ith_rand(1000 + t) & 0xFFFFFFFF,
+                                                   0.5, 0.1, 0.1, 50_000))
+        G.add_edge_batch(*split(ins))
+        keys = oracle.set_union(keys, ins)
+    assert_adjacency(G, keys, n, "after 10 batches")
+    g = oracle.CSR(keys, n)
+    assert g.degrees()[9] > 0
+    check_algorithms(G, g, 9, what="cfg cpu-small")
+    G.close()
