@@ -98,19 +98,25 @@ int tg_cc(tg_graph *g, uint32_t *labels);
 int tg_bc(tg_graph *g, uint32_t src, double *out);
 /* TC (include/TC.h:73-87) */
 int tg_tc(tg_graph *g, uint64_t *count);
+/* SSSP_BF (include/BellmanFordUnweighted.h:60-130, graph_bm.cc test
+ * "SSSP_BF"): dist[n] = hop distance from src, INT_MAX/2 when unreachable. */
+int tg_sssp_bf(tg_graph *g, uint32_t src, int32_t *dist);
 /* Device-output variants (pointers into HBM) */
 int tg_bfs_dev(tg_graph *g, uint32_t src, int32_t *parents, uint32_t *levels, uint32_t *depth);
 int tg_pagerank_dev(tg_graph *g, uint32_t iters, double *out);
 int tg_cc_dev(tg_graph *g, uint32_t *labels, uint32_t *rounds);
 int tg_bc_dev(tg_graph *g, uint32_t src, double *out);
+int tg_sssp_bf_dev(tg_graph *g, uint32_t src, int32_t *dist, uint32_t *rounds);
 
 /* ---- frontiers and edgeMap (include/VertexSubset.cpp, include/Map.cpp) */
-enum tg_op { TG_OP_BFS = 1, TG_OP_CC = 2, TG_OP_BC_FWD = 3, TG_OP_BC_BACK = 4, TG_OP_PR = 5 };
+enum tg_op { TG_OP_BFS = 1, TG_OP_CC = 2, TG_OP_BC_FWD = 3, TG_OP_BC_BACK = 4, TG_OP_PR = 5, TG_OP_BF = 7 };
 /* Device arrays an op reads and writes (all device pointers):
  *   BFS     parent (int32, -1 = unvisited), level (optional), depth
  *   CC      ids, prev (prevIDs; output = vertices whose id changed)
  *   BC_*    val (NumPaths / Dependencies), visited (uint8)
- *   PR      src_val (normalised p_curr), val (p_next, assigned) */
+ *   PR      src_val (normalised p_curr), val (p_next, assigned)
+ *   BF      dist (int32 ShortestPathLen), visited (claimed: the array is
+ *           written and must be padded to a multiple of 4 bytes) */
 typedef struct {
   int32_t *parent;
   uint32_t *level;
@@ -120,6 +126,7 @@ typedef struct {
   double *val;
   const double *src_val;
   const uint8_t *visited;
+  int32_t *dist;
 } tg_op_args;
 /* VertexSubset(el_t e, uint64_t n, bool all) */
 int tg_frontier_create(tg_graph *g, tg_frontier **out);

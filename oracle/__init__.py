@@ -85,6 +85,8 @@ def lib():
     L.orc_tc.argtypes = [c.c_uint32, _u64p, _u32p]
     L.orc_ltc.restype = c.c_uint64
     L.orc_ltc.argtypes = [_u64p, _u32p, c.c_uint32]
+    L.orc_sssp_bf.restype = None
+    L.orc_sssp_bf.argtypes = [c.c_uint32, _u64p, _u32p, c.c_uint32, _i32p]
     L.orc_count_common.restype = c.c_uint32
     L.orc_count_common.argtypes = [_u64p, _u32p, c.c_uint32, c.c_uint32]
     _lib = L
@@ -220,6 +222,12 @@ def count_common(g: CSR, s: int, d: int) -> int:
 
 def ltc(g: CSR, v: int) -> int:
     return int(lib().orc_ltc(g.off, g.nbr, v))
+
+
+def sssp_bf(g: CSR, src: int) -> np.ndarray:
+    out = np.empty(g.n, np.int32)
+    lib().orc_sssp_bf(g.n, g.off, g.nbr, src, out)
+    return out
 
 
 def rel_l1(a: np.ndarray, b: np.ndarray) -> float:

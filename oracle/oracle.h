@@ -97,6 +97,10 @@ uint32_t orc_count_common(const uint64_t *off, const uint32_t *nbr,
  * workload; no counterpart in the reference). */
 uint64_t orc_ltc(const uint64_t *off, const uint32_t *nbr, uint32_t v);
 
+/* SSSP_BF with unit edge lengths (include/BellmanFordUnweighted.h:60-130):
+ * dist[v] = hop distance from src, INT_MAX/2 when unreachable. */
+void orc_sssp_bf(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint32_t src, int32_t *dist);
+
 #ifdef __cplusplus
 }
 #endif

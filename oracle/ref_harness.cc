@@ -6,7 +6,7 @@
 // time the reference's CPU insert path (bench.py cpu_baseline,
 // kind "reference").  It mirrors run_algorithm in src/graph_bm.cc:231-391:
 // static build with add_edge_batch, insert batches, then BFS / PR / CC /
-// BC / TC through the reference's own algorithm headers.
+// BC / TC / SSSP_BF through the reference's own algorithm headers.
 //
 // Usage:
 //   ref_harness rmat  <n> <seed> <a> <b> <c> <count> <out.bin>
@@ -15,7 +15,7 @@
 //       graph.bin / batch.bin: u64 n, u64 m, then m (u32 src, u32 dst) pairs
 //       sorted by (src,dst), unique.  Writes <out_prefix>.adj<k> after the
 //       static build (k=0) and after every batch (k=1..), then .bfs .pr .cc
-//       .bc .tc for the final graph.
+//       .bc .tc .sssp for the final graph.
 //   ref_harness bench <graph.bin> <batch_size> <batches> <seed>
 //       static build, then `batches` rMAT insert batches of batch_size
 //       edges (sorted + unique, both directions), prints JSON timing.
@@ -41,6 +41,7 @@
 #include "Components.h"
 #include "BC.h"
 #include "TC.h"
+#include "BellmanFordUnweighted.h"
 
 using namespace graphstore;
 
@@ -155,6 +156,9 @@ int cmd_run(int argc, char **argv) {
   free(bc);
   uint64_t tc = TC(g);
   write_blob(prefix + ".tc", &tc, sizeof(tc));
+  int32_t *sp = SSSP_BF(g, src);
+  write_blob(prefix + ".sssp", sp, n * sizeof(int32_t));
+  free(sp);
   return 0;
 }
 

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG, "libterrace_hip.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "terrace_hip.h")
 
 TG_OK, TG_EINVAL, TG_ENOMEM, TG_EDEVICE, TG_EINTERNAL = 0, -1, -2, -3, -4
-OP_BFS, OP_CC, OP_BC_FWD, OP_BC_BACK, OP_PR = 1, 2, 3, 4, 5
+OP_BFS, OP_CC, OP_BC_FWD, OP_BC_BACK, OP_PR, OP_BF = 1, 2, 3, 4, 5, 7
 
 
 class TerraceError(RuntimeError):
@@ -35,6 +35,7 @@ class OpArgs(ctypes.Structure):
         ("val", ctypes.c_void_p),
         ("src_val", ctypes.c_void_p),
         ("visited", ctypes.c_void_p),
+        ("dist", ctypes.c_void_p),
     ]
 
 
@@ -100,6 +101,8 @@ _SIGS = {
     "tg_pagerank_dev": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
     "tg_cc_dev": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint32)]),
     "tg_bc_dev": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+    "tg_sssp_bf": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+    "tg_sssp_bf_dev": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.POINTER(ctypes.c_uint32)]),
     "tg_frontier_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "tg_frontier_destroy": (ctypes.c_int, [_P]),
     "tg_frontier_set": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),

@@ -373,6 +373,29 @@ int tg_bc(tg_graph *G, uint32_t src, double *out) {
   });
 }
 
+int tg_sssp_bf_dev(tg_graph *G, uint32_t src, int32_t *dist, uint32_t *rounds) {
+  return guard([&] {
+    NEED(G && dist, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    NEED(src < G->g.num_vertices(), "source out of range");
+    uint32_t r = G->t.sssp_bf(src, dist);
+    if (rounds) *rounds = r;
+  });
+}
+
+int tg_sssp_bf(tg_graph *G, uint32_t src, int32_t *dist) {
+  return guard([&] {
+    NEED(G && dist, "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    uint32_t n = G->g.num_vertices();
+    NEED(src < n, "source out of range");
+    hipStream_t st = G->g.stream();
+    G->s_i32.ensure(n, st);
+    G->t.sssp_bf(src, G->s_i32.get());
+    d2h(dist, G->s_i32.get(), static_cast<uint64_t>(n) * 4, st);
+  });
+}
+
 int tg_tc(tg_graph *G, uint64_t *count) {
   return guard([&] {
     NEED(G && count, "null argument");
@@ -465,6 +488,7 @@ int tg_edge_map(tg_graph *G, tg_frontier *in, int op, const tg_op_args *a, tg_fr
     st.val = a->val;
     st.src_val = a->src_val;
     st.visited = a->visited;
+    st.dist = a->dist;
     tg::Op o;
     switch (op) {
       case TG_OP_BFS: NEED(st.parent, "BFS needs parent"); o = tg::Op::BFS; break;
@@ -472,6 +496,7 @@ int tg_edge_map(tg_graph *G, tg_frontier *in, int op, const tg_op_args *a, tg_fr
       case TG_OP_BC_FWD: NEED(st.val && st.visited, "BC needs val and visited"); o = tg::Op::BC_FWD; break;
       case TG_OP_BC_BACK: NEED(st.val && st.visited, "BC needs val and visited"); o = tg::Op::BC_BACK; break;
       case TG_OP_PR: NEED(st.val && st.src_val, "PR needs val and src_val"); o = tg::Op::PR; break;
+      case TG_OP_BF: NEED(st.dist && st.visited, "BF needs dist and visited"); o = tg::Op::BF; break;
       default: throw tg::Error(TG_EINVAL, "unknown op");
     }
     G->t.edge_map(in->f, o, st, out ? &out->f : nullptr, threshold);

@@ -125,6 +125,49 @@ struct PullCC {
   }
 };
 
+// Claims byte i of a 4-byte-padded flag array (CAS(&Visited[i], 0, 1)):
+// true for the one caller that finds it 0.
+__device__ inline bool claim_byte(uint8_t *flags, uint32_t i) {
+  uint32_t *w = reinterpret_cast<uint32_t *>(flags + (i & ~3u));
+  uint32_t sh = (i & 3u) * 8;
+  uint32_t old = atomicOr(w, 1u << sh);
+  return ((old >> sh) & 0xFFu) == 0;
+}
+
+// SSSP_BF pull (BF_F::update, include/BellmanFordUnweighted.h:31-41): the
+// smallest dist[s] + 1 over frontier neighbours s lowers dist[i]; a lowered
+// i joins the output when it claims Visited[i] this round.
+struct PullBF {
+  const uint32_t *fbits;  // null = all
+  int32_t *dist;
+  uint8_t *visited;
+  uint32_t *obits;
+  using Acc = uint32_t;  // distances are non-negative during the rounds
+  static constexpr bool kEarly = false;
+  static constexpr bool kOrdered = false;
+  static constexpr const char *kName = "bf";
+  __device__ static Acc ident() { return 0xFFFFFFFFu; }
+  __device__ bool skip(uint32_t) const { return false; }
+  __device__ bool member(uint32_t v) const { return fbits == nullptr || bit_get(fbits, v); }
+  __device__ Acc value(uint32_t v) const { return static_cast<uint32_t>(dist[v] + 1); }
+  __device__ static Acc combine(Acc a, Acc b) { return a < b ? a : b; }
+  __device__ static Acc wave_reduce(Acc a) { return wave_min(a); }
+  __device__ static Acc reduce16(Acc a, uint64_t) { return group16_min(a); }
+  __device__ static bool found(Acc) { return false; }
+  __device__ void commit(uint32_t i, Acc a, Mode m) const {
+    if (a == 0xFFFFFFFFu) return;
+    int32_t nd = static_cast<int32_t>(a);
+    bool lowered;
+    if (m == kShared) {
+      lowered = atomicMin(&dist[i], nd) > nd;
+    } else {
+      lowered = nd < dist[i];
+      if (lowered) dist[i] = nd;
+    }
+    if (lowered && obits && claim_byte(visited, i)) bit_set(obits, i);
+  }
+};
+
 struct SumAcc {
   double s;
   uint32_t any;
@@ -331,6 +374,18 @@ struct PushCC {
     uint32_t ls = ids[s];
     if (ls >= ids[d]) return false;
     return atomicMin(&ids[d], ls) > ls;
+  }
+};
+// SSSP_BF push: writeMin, then CAS on Visited (BF_F::updateAtomic,
+// include/BellmanFordUnweighted.h:42-47).
+struct PushBF {
+  int32_t *dist;
+  uint8_t *visited;
+  __device__ bool apply(uint32_t s, uint32_t d) const {
+    int32_t nd = dist[s] + 1;
+    if (nd >= dist[d]) return false;
+    if (atomicMin(&dist[d], nd) <= nd) return false;
+    return claim_byte(visited, d);
   }
 };
 // BC push: CAS-add, output on the first contribution (BC_F::updateAtomic,
@@ -1260,6 +1315,12 @@ void Traversal::pull(Frontier &in, Op op, const OpState &st, Frontier *out) {
       launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
       break;
     }
+    case Op::BF: {
+      if (!st.dist || !st.visited) throw Error(-1, "BF needs dist and visited");
+      PullBF p{fb, st.dist, const_cast<uint8_t *>(st.visited), ob};
+      launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+      break;
+    }
     default:
       throw Error(-1, "edge_map: unsupported op");
   }
@@ -1327,6 +1388,12 @@ void Traversal::push(Frontier &in, Op op, const OpState &st, Frontier *out) {
     case Op::BC_FWD:
     case Op::BC_BACK: {
       PushSum p{st.val, st.visited};
+      launch_push(*this, g_, p, in, out, cnt_, sec_pma_, sec_tree_);
+      break;
+    }
+    case Op::BF: {
+      if (!st.dist || !st.visited) throw Error(-1, "BF needs dist and visited");
+      PushBF p{st.dist, const_cast<uint8_t *>(st.visited)};
       launch_push(*this, g_, p, in, out, cnt_, sec_pma_, sec_tree_);
       break;
     }
@@ -1531,6 +1598,45 @@ void Traversal::bc(uint32_t src, double *out, uint32_t threshold) {
   k_bc_final<<<grid_for(n, 256), 256, 0, s>>>(dep, paths, n);
   TG_CHECK_LAUNCH();
   TG_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- SSSP_BF
+uint32_t Traversal::sssp_bf(uint32_t src, int32_t *dist, uint32_t threshold) {
+  // BellmanFordUnweighted.h:60-130.  The reference runs push only
+  // (threshold 1); the rounds' result does not depend on the direction, so
+  // the graph's tuned BFS threshold is used.
+  if (!threshold) threshold = g_.tune.thr_bfs;
+  hipStream_t s = stream_;
+  uint32_t n = g_.num_vertices();
+  if (src >= n) throw Error(-1, "source out of range");
+  const int32_t inf = 0x7FFFFFFF / 2;
+  k_fill_i32<<<grid_for(n, 256), 256, 0, s>>>(dist, inf, n);
+  TG_CHECK_LAUNCH();
+  k_set_i32<<<1, 1, 0, s>>>(dist, src, 0);
+  btmp_.ensure((static_cast<uint64_t>(n) + 4) & ~3ull, s);
+  TG_HIP(hipMemsetAsync(btmp_.get(), 0, (static_cast<uint64_t>(n) + 4) & ~3ull, s));
+  Frontier a, b;
+  frontier_single(a, src);
+  Frontier *cur = &a, *nxt = &b;
+  uint32_t round = 0;
+  while (cur->count > 0) {
+    if (round == n) {  // a negative cycle in the reference's rule; unreachable with unit lengths
+      k_fill_i32<<<grid_for(n, 256), 256, 0, s>>>(dist, -inf, n);
+      TG_CHECK_LAUNCH();
+      break;
+    }
+    OpState st;
+    st.dist = dist;
+    st.visited = btmp_.get();
+    edge_map(*cur, Op::BF, st, nxt, threshold);
+    VArgs va;
+    va.visited = btmp_.get();
+    vertex_map(*nxt, 7, va);  // BF_Vertex_F: Visited[i] = 0 for the new frontier
+    std::swap(cur, nxt);
+    round++;
+  }
+  TG_HIP(hipStreamSynchronize(s));
+  return round;
 }
 
 // ---------------------------------------------------------------- multi-source BFS

@@ -22,7 +22,7 @@ struct Frontier {
   uint64_t words() const { return (static_cast<uint64_t>(n) + 31) / 32; }
 };
 
-enum class Op : int { BFS = 1, CC = 2, BC_FWD = 3, BC_BACK = 4, PR = 5, TC = 6 };
+enum class Op : int { BFS = 1, CC = 2, BC_FWD = 3, BC_BACK = 4, PR = 5, TC = 6, BF = 7 };
 
 // Device state an op reads and writes (pointers into device memory).
 struct OpState {
@@ -33,7 +33,8 @@ struct OpState {
   const uint32_t *prev = nullptr;  // CC prevIDs (output = ids != prev)
   double *val = nullptr;        // BC NumPaths / Dependencies, PR p_next
   const double *src_val = nullptr;  // PR normalised p_curr
-  const uint8_t *visited = nullptr;  // BC
+  const uint8_t *visited = nullptr;  // BC; BF claims it (writes), so it is 4-byte padded there
+  int32_t *dist = nullptr;           // BF ShortestPathLen
 };
 
 class Traversal {
@@ -72,6 +73,9 @@ class Traversal {
   void pagerank(uint32_t iters, double *out);
   uint32_t cc(uint32_t *ids, uint32_t threshold = 0);
   void bc(uint32_t src, double *out, uint32_t threshold = 0);
+  // SSSP_BF with unit lengths (include/BellmanFordUnweighted.h:60-130):
+  // dist = hop distance, INT_MAX / 2 unreachable; returns the rounds run.
+  uint32_t sssp_bf(uint32_t src, int32_t *dist, uint32_t threshold = 0);
   uint64_t tc();
   // 64 BFS in one traversal: levels[j*n + v] for source j (k <= 64).
   uint32_t bfs_multi(const uint32_t *srcs_host, uint32_t k, uint32_t *levels, uint32_t threshold = 0);

@@ -12,7 +12,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (OP_BC_BACK, OP_BC_FWD, OP_BFS, OP_CC, OP_PR, DeviceBuffer, OpArgs, Request,
+from ._lib import (OP_BC_BACK, OP_BC_FWD, OP_BF, OP_BFS, OP_CC, OP_PR, DeviceBuffer, OpArgs, Request,
                    Result, TerraceError, check, lib, ptr)
 
 __all__ = ["Graph", "VertexSubset", "edgeMap", "BFS_with_edge_map", "PR_S", "CC", "BC", "TC",
@@ -293,6 +293,14 @@ def BC(G: Graph, start: int) -> np.ndarray:
     return out
 
 
+def SSSP_BF(G: Graph, start: int) -> np.ndarray:
+    """include/BellmanFordUnweighted.h:60-130: int32 hop distances, INT_MAX/2
+    where unreachable."""
+    out = np.empty(G.get_num_vertices(), np.int32)
+    check(lib().tg_sssp_bf(G.handle, int(start), ptr(out)))
+    return out
+
+
 def TC(G: Graph) -> int:
     """include/TC.h:73-87."""
     r = ctypes.c_uint64()
@@ -434,4 +442,4 @@ def rmat_device(n: int, seed: int, a: float, b: float, c: float, count: int, sta
     return s, d, w.value
 
 
-OPS = {"BFS": OP_BFS, "CC": OP_CC, "BC_FWD": OP_BC_FWD, "BC_BACK": OP_BC_BACK, "PR": OP_PR}
+OPS = {"BFS": OP_BFS, "CC": OP_CC, "BC_FWD": OP_BC_FWD, "BC_BACK": OP_BC_BACK, "PR": OP_PR, "BF": OP_BF}

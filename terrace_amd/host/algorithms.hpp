@@ -1,6 +1,7 @@
-// The five algorithms of the reference's benchmark, restated over the
-// graphstore shim (include/terrace/graphstore.hpp).  Same control flow and
-// functor roles as include/BFS.h, Pagerank.h, Components.h, BC.h and TC.h;
+// The algorithms of the reference's benchmark, restated over the graphstore
+// shim (include/terrace/graphstore.hpp).  Same control flow and functor
+// roles as include/BFS.h, Pagerank.h, Components.h, BC.h, TC.h and
+// BellmanFordUnweighted.h;
 // the per-edge work runs on the GPU through the functors' device ops, and
 // the state arrays live in HBM.  Results are malloc'ed host arrays the
 // caller frees, as in the reference.
@@ -284,6 +285,62 @@ inline double *BC(Graph &G, uint32_t start) {
   All.del();
   double *out = static_cast<double *>(malloc(n * sizeof(double)));
   Dependencies.download(out, n);
+  return out;
+}
+
+// ---- SSSP_BF with unit lengths (include/BellmanFordUnweighted.h:29-130) --
+struct BF_F {
+  DeviceArray<int32_t> *ShortestPathLen;
+  DeviceArray<uint8_t> *Visited;  // padded to a multiple of 4 bytes (claimed in words)
+};
+template <>
+struct edge_op_traits<BF_F> {
+  static constexpr int op = TG_OP_BF;
+  static tg_op_args args(const BF_F &f) {
+    tg_op_args a{};
+    a.dist = f.ShortestPathLen->data();
+    a.visited = f.Visited->data();
+    return a;
+  }
+};
+struct BF_Vertex_F {
+  DeviceArray<uint8_t> *Visited;
+};
+template <>
+struct vertex_op_traits<BF_Vertex_F> {
+  static constexpr int op = TG_VOP_CLEAR_VISITED;
+  static tg_vmap_args args(const BF_Vertex_F &f) {
+    tg_vmap_args a{};
+    a.visited = f.Visited->data();
+    return a;
+  }
+};
+
+inline int32_t *SSSP_BF(Graph &G, long start) {
+  const uint64_t n = G.get_num_vertices();
+  const int32_t inf = 0x7FFFFFFF / 2;
+  DeviceArray<int32_t> ShortestPathLen(n);
+  ShortestPathLen.fill(inf);
+  ShortestPathLen.set(static_cast<uint64_t>(start), 0);
+  DeviceArray<uint8_t> Visited((n + 4) & ~3ull);
+  Visited.fill(0);
+  VertexSubset frontier(G, static_cast<uint32_t>(start), n);
+  uint64_t round = 0;
+  while (frontier.not_empty()) {
+    if (round == n) {  // negative weight cycle rule of the reference
+      ShortestPathLen.fill(-inf);
+      break;
+    }
+    // sparse only in the reference (threshold 1)
+    VertexSubset output = edgeMap(G, frontier, BF_F{&ShortestPathLen, &Visited}, true, 1);
+    vertexMap(output, BF_Vertex_F{&Visited}, false);
+    frontier.del();
+    frontier = output;
+    round++;
+  }
+  frontier.del();
+  int32_t *out = static_cast<int32_t *>(malloc(n * sizeof(int32_t)));
+  ShortestPathLen.download(out, n);
   return out;
 }
 

@@ -2,7 +2,7 @@
 // the GPU container, written against the graphstore shim.
 //
 //   graph_bm -s SCALE -m EDGES [-a A -b B -c C] [-src V] [-rounds R]
-//            [-maxiters I] [-TC 1] [-batches 10,100,...] [-trials T]
+//            [-maxiters I] [-TC 1] [-SSSP 1] [-batches 10,100,...] [-trials T]
 //            [-o PREFIX]
 //
 // The static graph is rMat<uint32_t>(2^SCALE, hash64(100), a, b, c) edges
@@ -78,6 +78,7 @@ int main(int argc, char **argv) {
   const long rounds = P.num("-rounds", 4);
   const long maxiters = P.num("-maxiters", 10);
   const bool with_tc = P.num("-TC", 0) != 0;
+  const bool with_sssp = P.num("-SSSP", 0) != 0;  // the test "SSSP_BF" of src/graph_bm.cc:57-84
   const long trials = P.num("-trials", 5);
   const std::string out = P.get("-o", "");
   const uint32_t n = 1u << scale;
@@ -99,6 +100,7 @@ int main(int argc, char **argv) {
 
   std::vector<std::string> tests = {"BFS", "PR", "CC", "BC"};
   if (with_tc) tests.push_back("TC");
+  if (with_sssp) tests.push_back("SSSP_BF");
   for (const std::string &test : tests) {
     double total = 0;
     for (long i = 0; i < rounds; i++) {
@@ -122,6 +124,11 @@ int main(int argc, char **argv) {
         double *r = BC(graph, src);
         total += now() - t0;
         if (!out.empty() && i == 0) dump(out + ".bc", r, n);
+        free(r);
+      } else if (test == "SSSP_BF") {
+        int32_t *r = SSSP_BF(graph, src);
+        total += now() - t0;
+        if (!out.empty() && i == 0) dump(out + ".sssp", r, n);
         free(r);
       } else {
         uint64_t r = TC(graph);

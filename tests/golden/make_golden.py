@@ -102,6 +102,7 @@ def make_graph(name, n, base_keys, batches, src, iters=10):
         data["cc"] = np.fromfile(prefix + ".cc", np.uint32)
         data["bc"] = np.fromfile(prefix + ".bc", np.float64)
         data["tc"] = np.fromfile(prefix + ".tc", np.uint64)
+        data["sssp"] = np.fromfile(prefix + ".sssp", np.int32)
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **data)
     print(name, "n", n, "m", len(base_keys), "tc", int(data["tc"][0]))
 

@@ -58,6 +58,7 @@ def check_algorithms(G, g, src, iters=10, what=""):
     assert np.all(np.isnan(bc[~reach])), f"{what}: BC unreachable not NaN"
     assert oracle.rel_l1(bc[reach], ref[reach]) <= FP_TOL, f"{what}: BC"
     assert tg.TC(G) == oracle.tc(g), f"{what}: TC"
+    assert np.array_equal(tg.SSSP_BF(G, src), oracle.sssp_bf(g, src)), f"{what}: SSSP_BF"
 
 
 # ------------------------------------------------------------------ generator
@@ -115,6 +116,7 @@ def test_golden_graph(name):
     assert oracle.rel_l1(bc[reach], z["bc"][reach]) <= FP_TOL
     assert np.all(np.isnan(bc[~reach]))
     assert tg.TC(G) == int(z["tc"][0])
+    assert np.array_equal(tg.SSSP_BF(G, src), z["sssp"])  # the reference build's SSSP_BF output
     G.close()
 
 

@@ -24,7 +24,7 @@ def test_graph_bm_driver_matches_oracle(tmp_path, scale, m0, probs):
     prefix = str(tmp_path / "out")
     print(f"[gpu-test] graph_bm scale={scale} m0={m0} probs={probs}", flush=True)
     r = subprocess.run([DRIVER, "-s", str(scale), "-m", str(m0), "-a", str(a), "-b", str(b), "-c", str(c),
-                        "-src", "9", "-rounds", "1", "-TC", "1", "-batches", "10,1000", "-trials", "2",
+                        "-src", "9", "-rounds", "1", "-TC", "1", "-SSSP", "1", "-batches", "10,1000", "-trials", "2",
                         "-o", prefix], capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert re.search(r"AVG\ttest=BFS\ttime=", r.stdout)
@@ -42,3 +42,4 @@ def test_graph_bm_driver_matches_oracle(tmp_path, scale, m0, probs):
     reach = lv != np.iinfo(np.uint32).max
     assert oracle.rel_l1(bc[reach], ref[reach]) <= 1e-9 and np.all(np.isnan(bc[~reach]))
     assert int(np.fromfile(prefix + ".tc", np.uint64)[0]) == oracle.tc(g)
+    assert np.array_equal(np.fromfile(prefix + ".sssp", np.int32), oracle.sssp_bf(g, 9))

@@ -163,6 +163,25 @@ def test_tc_matches_reference(name):
     assert oracle.tc(final_graph(z)) == int(z["tc"][0])
 
 
+@pytest.mark.parametrize("name", GRAPHS)
+def test_sssp_bf_matches_reference(name):
+    z = load(name)
+    assert np.array_equal(oracle.sssp_bf(final_graph(z), int(z["src"][0])), z["sssp"])
+
+
+def test_sssp_bf_is_hop_distance():
+    # unit edge lengths: the Bellman-Ford fixed point is the BFS level, and
+    # unreachable vertices keep INT_MAX / 2
+    n = 1 << 11
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 83, 0.57, 0.19, 0.19, 9000))
+    g = oracle.CSR(keys, n)
+    for src in (0, 5, int(np.argmax(g.degrees()))):
+        lv = oracle.bfs_levels(g, src).astype(np.int64)
+        want = np.where(lv == np.iinfo(np.uint32).max, (2**31 - 1) // 2, lv)
+        got = oracle.sssp_bf(g, src)
+        assert np.array_equal(got, want)
+
+
 # --------------------------------------------------------------- closed forms
 def undirected(n, edges):
     ks = []
