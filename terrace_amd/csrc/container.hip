@@ -183,6 +183,22 @@ __global__ void k_exists2(GView g, const uint64_t *keys, uint64_t n, uint32_t *e
   }
 }
 
+// Same, over the first *np keys of a buffer of cap (flags 0 beyond), so
+// the unique count need not be read back first.
+__global__ void k_exists2_n(GView g, const uint64_t *keys, const uint64_t *np, uint64_t cap, uint32_t *ex,
+                            uint32_t *nex) {
+  const uint64_t n = *np;
+  GRID_STRIDE(i, cap) {
+    bool e = i < n && edge_exists(g, key_src(keys[i]), key_dst(keys[i]));
+    ex[i] = e ? 1u : 0u;
+    nex[i] = (i < n && !e) ? 1u : 0u;
+  }
+}
+__global__ void k_exists_n(GView g, const uint64_t *keys, const uint64_t *np, uint64_t cap, uint32_t *ex) {
+  const uint64_t n = *np;
+  GRID_STRIDE(i, cap) ex[i] = (i < n && edge_exists(g, key_src(keys[i]), key_dst(keys[i]))) ? 1u : 0u;
+}
+
 __global__ void k_not(const uint32_t *a, uint64_t n, uint32_t *out) {
   GRID_STRIDE(i, n) out[i] = a[i] ? 0u : 1u;
 }
@@ -413,6 +429,14 @@ struct SelRunStart {
   const uint64_t *k;
   uint32_t *beg;
   __device__ bool pred(uint64_t i) const { return i == 0 || key_src(k[i]) != key_src(k[i - 1]); }
+  __device__ void emit(uint64_t i, uint64_t pos) const { beg[pos] = static_cast<uint32_t>(i); }
+};
+// run starts among the first *np keys
+struct SelRunStartN {
+  const uint64_t *k;
+  uint32_t *beg;
+  const uint64_t *np;
+  __device__ bool pred(uint64_t i) const { return i < *np && (i == 0 || key_src(k[i]) != key_src(k[i - 1])); }
   __device__ void emit(uint64_t i, uint64_t pos) const { beg[pos] = static_cast<uint32_t>(i); }
 };
 struct SelRoute {
@@ -1363,10 +1387,11 @@ uint64_t Graph::compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t
   return read_scalar(u4_.get() + n, stream_);
 }
 
-// Pack, validate, sort and unique the batch into k0_; returns the count.
-uint64_t Graph::prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t count, uint64_t *&keys) {
+// Pack, validate, sort and unique the batch into k1_ without a read back:
+// error bits in dsc_[0], the unique count in dsc_[1].
+void Graph::prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_t count) {
   k0_.ensure(count, stream_);
-  k1_.ensure(count, stream_);
+  k1_.ensure(count + 1, stream_);
   fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
   k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
   TG_CHECK_LAUNCH();
@@ -1374,11 +1399,6 @@ uint64_t Graph::prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t 
   while (bits < 32 && (1ull << bits) < n_) bits++;
   radix_sort_keys(k0_.get(), count, bits, bits, sort_, stream_);
   select(SelUnique{k0_.get(), k1_.get()}, count, dsc_.get() + 1, scan_scratch, stream_);
-  uint64_t h[2];
-  read_scalars(dsc_.get(), h, 2, stream_);
-  if (h[0] & 1) throw Error(-1, "vertex id out of range");
-  keys = k1_.get();
-  return h[1];
 }
 
 // Internal consistency flags raised by kernels during the batch.
@@ -1710,17 +1730,32 @@ void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
 void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
   if (count == 0) return;
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
-  uint64_t *R = nullptr;
-  uint64_t nR = prepare_keys(src, dst, count, R);  // R in k1_
-  insert_sorted(R, nR);
+  if (read_only_) throw Error(-1, "graph is a query-only snapshot");
+  // sort, unique, existence and source runs sized by count; the unique count
+  // stays on the device until the one read below
+  prepare_keys_async(src, dst, count);
+  const uint64_t *R = k1_.get();
+  GView g = view();
+  f0_.ensure(count + 1, stream_);
+  f1_.ensure(count + 1, stream_);
+  k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, R, dsc_.get() + 1, count, f0_.get(), f1_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(f1_.get(), f1_.get(), count, f1_.get() + count, scan_scratch, stream_);
+  s32_[0].ensure(count + 1, stream_);
+  select(SelRunStartN{R, s32_[0].get(), dsc_.get() + 1}, count, dsc_.get() + 2, scan_scratch, stream_);
+  uint64_t h[3];
+  read_scalars(dsc_.get(), h, 3, stream_);
+  if (h[0] & 1) throw Error(-1, "vertex id out of range");
+  if (h[1] == 0) return;
+  k_put_u32_at<<<1, 1, 0, stream_>>>(s32_[0].get(), h[2], static_cast<uint32_t>(h[1]));
+  TG_CHECK_LAUNCH();
+  insert_runs(R, h[1], static_cast<uint32_t>(h[2]));
 }
 
 // Insert sorted unique keys (R must not alias the batch scratch buffers).
 void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
   if (nR == 0) return;
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
-  lists_dirty_ = true;
-  ++epoch_;
   GView g = view();
   // existence + new-key prefix
   f0_.ensure(nR + 1, stream_);
@@ -1730,14 +1765,25 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(nidx, nidx, nR, nidx + nR, scan_scratch, stream_);
   // source runs
-  DevBuf<uint32_t> &run_beg = s32_[0], &run_c = s32_[2], &run_nko = s32_[3], &kick_old = s32_[4],
-                   &promote = s32_[5], &ppos = s32_[6];
-  DevBuf<uint8_t> &run_route = s8_[3], &kroute = s8_[4];
+  DevBuf<uint32_t> &run_beg = s32_[0];
   run_beg.ensure(nR + 1, stream_);
   select(SelRunStart{R, run_beg.get()}, nR, dsc_.get() + 2, scan_scratch, stream_);
   k_put_u32<<<1, 1, 0, stream_>>>(run_beg.get(), dsc_.get() + 2, static_cast<uint32_t>(nR));
   TG_CHECK_LAUNCH();
   uint32_t nruns = static_cast<uint32_t>(read_scalar(dsc_.get() + 2, stream_));
+  insert_runs(R, nR, nruns);
+}
+
+// The insert proper: existence flags in f0_, the new-key prefix in f1_ and
+// the run starts (with run_beg[nruns] = nR) in s32_[0] are ready.
+void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
+  lists_dirty_ = true;
+  ++epoch_;
+  GView g = view();
+  uint32_t *ex = f0_.get(), *nidx = f1_.get();
+  DevBuf<uint32_t> &run_beg = s32_[0], &run_c = s32_[2], &run_nko = s32_[3], &kick_old = s32_[4],
+                   &promote = s32_[5], &ppos = s32_[6];
+  DevBuf<uint8_t> &run_route = s8_[3], &kroute = s8_[4];
   run_c.ensure(nruns + 1, stream_);
   run_nko.ensure(nruns + 1, stream_);
   kick_old.ensure(static_cast<uint64_t>(nruns) * kInline + 1, stream_);
@@ -1832,17 +1878,31 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
 void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
   if (count == 0) return;
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
-  uint64_t *R0 = nullptr;
-  uint64_t nR0 = prepare_keys(src, dst, count, R0);  // k1_
-  delete_sorted(R0, nR0);
+  if (read_only_) throw Error(-1, "graph is a query-only snapshot");
+  // sort, unique, the existing keys and their source runs sized by count;
+  // the three counts come back in one read
+  prepare_keys_async(src, dst, count);
+  GView g = view();
+  k2_.ensure(count + 1, stream_);
+  f0_.ensure(count + 1, stream_);
+  k_exists_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, k1_.get(), dsc_.get() + 1, count, f0_.get());
+  TG_CHECK_LAUNCH();
+  select(SelFlagged{k1_.get(), f0_.get(), k2_.get()}, count, dsc_.get() + 2, scan_scratch, stream_);
+  s32_[0].ensure(count + 1, stream_);
+  select(SelRunStartN{k2_.get(), s32_[0].get(), dsc_.get() + 2}, count, dsc_.get() + 3, scan_scratch, stream_);
+  uint64_t h[4];
+  read_scalars(dsc_.get(), h, 4, stream_);
+  if (h[0] & 1) throw Error(-1, "vertex id out of range");
+  if (h[2] == 0) return;
+  k_put_u32_at<<<1, 1, 0, stream_>>>(s32_[0].get(), h[3], static_cast<uint32_t>(h[2]));
+  TG_CHECK_LAUNCH();
+  delete_runs(k2_.get(), h[2], static_cast<uint32_t>(h[3]));
 }
 
 // Delete sorted unique keys (absent keys are ignored).
 void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
   if (nR0 == 0) return;
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
-  lists_dirty_ = true;
-  ++epoch_;
   GView g = view();
   // keep the keys that exist (absent deletes are no-ops)
   k2_.ensure(nR0 + 1, stream_);
@@ -1853,13 +1913,23 @@ void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
   uint64_t nR = read_scalar(dsc_.get() + 2, stream_);
   if (nR == 0) return;
   uint64_t *R = k2_.get();
-  DevBuf<uint32_t> &run_beg = s32_[0], &run_bump = s32_[12], &bump_off = s32_[13];
-  DevBuf<uint8_t> &sec_route = s8_[5];
+  DevBuf<uint32_t> &run_beg = s32_[0];
   run_beg.ensure(nR + 1, stream_);
   select(SelRunStart{R, run_beg.get()}, nR, dsc_.get() + 3, scan_scratch, stream_);
   k_put_u32<<<1, 1, 0, stream_>>>(run_beg.get(), dsc_.get() + 3, static_cast<uint32_t>(nR));
   TG_CHECK_LAUNCH();
   uint32_t nruns = static_cast<uint32_t>(read_scalar(dsc_.get() + 3, stream_));
+  delete_runs(R, nR, nruns);
+}
+
+// The delete proper: R (in k2_) holds existing keys only and s32_[0] their
+// run starts with run_beg[nruns] = nR.
+void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
+  lists_dirty_ = true;
+  ++epoch_;
+  GView g = view();
+  DevBuf<uint32_t> &run_beg = s32_[0], &run_bump = s32_[12], &bump_off = s32_[13];
+  DevBuf<uint8_t> &sec_route = s8_[5];
   run_bump.ensure(nruns + 1, stream_);
   bump_off.ensure(nruns + 1, stream_);
   sec_route.ensure(nR + 1, stream_);

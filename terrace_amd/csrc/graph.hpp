@@ -182,7 +182,9 @@ class Graph {
 
   void init_pma(uint64_t nleaves);
   void compute_bounds();
-  uint64_t prepare_keys(const uint32_t *src, const uint32_t *dst, uint64_t count, uint64_t *&keys);
+  void prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_t count);
+  void insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
+  void delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
   void check_batch_flags();
   static constexpr uint64_t kRootUnknown = ~0ull;
