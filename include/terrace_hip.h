@@ -40,6 +40,22 @@ const char *tg_last_error(void);
  *      include/graph.h:118-120, 232-239 ------------------------------- */
 int tg_create(uint32_t num_vertices, tg_graph **out);
 int tg_destroy(tg_graph *g);
+/* ---- weighted graphs: the reference's WEIGHTED build (include/graph.h:
+ * 34-38, 76-86, add_edge_batch with weights :430-436).  One uint32 weight
+ * below 2^31 per directed edge.  A batch keeps the first weight given for
+ * an edge; inserting an existing edge leaves its weight; deletes drop it.
+ * On a weighted graph, tg_insert_batch, tg_add_edge and the coalesced ops
+ * return TG_EINVAL (they carry no weights); tg_sssp_bf uses the weights as
+ * edge lengths (include/BellmanFord.h:67-143). */
+int tg_create_weighted(uint32_t num_vertices, tg_graph **out);
+int tg_is_weighted(tg_graph *g, int *out);
+int tg_insert_batch_weighted(tg_graph *g, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
+                             uint64_t count);
+int tg_insert_batch_weighted_dev(tg_graph *g, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
+                                 uint64_t count);
+/* Graph::is_edge of the weighted build returns the weight: out[i] = weight
+ * of (src[i], dst[i]), or 0xFFFFFFFF when the edge is absent. */
+int tg_edge_weights(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t count, uint32_t *out);
 
 /* ---- batch updates ---------------------------------------------------
  * tg_insert_batch replaces Graph::add_edge_batch (include/graph.h:434-502).
@@ -99,7 +115,9 @@ int tg_bc(tg_graph *g, uint32_t src, double *out);
 /* TC (include/TC.h:73-87) */
 int tg_tc(tg_graph *g, uint64_t *count);
 /* SSSP_BF (include/BellmanFordUnweighted.h:60-130, graph_bm.cc test
- * "SSSP_BF"): dist[n] = hop distance from src, INT_MAX/2 when unreachable. */
+ * "SSSP_BF"): dist[n] = hop distance from src, INT_MAX/2 when unreachable.
+ * On a weighted graph: BellmanFord.h's distances with the stored weights as
+ * lengths (sums saturate at INT_MAX/2). */
 int tg_sssp_bf(tg_graph *g, uint32_t src, int32_t *dist);
 /* Device-output variants (pointers into HBM) */
 int tg_bfs_dev(tg_graph *g, uint32_t src, int32_t *parents, uint32_t *levels, uint32_t *depth);

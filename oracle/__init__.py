@@ -87,6 +87,8 @@ def lib():
     L.orc_ltc.argtypes = [_u64p, _u32p, c.c_uint32]
     L.orc_sssp_bf.restype = None
     L.orc_sssp_bf.argtypes = [c.c_uint32, _u64p, _u32p, c.c_uint32, _i32p]
+    L.orc_sssp_bf_weighted.restype = None
+    L.orc_sssp_bf_weighted.argtypes = [c.c_uint32, _u64p, _u32p, _u32p, c.c_uint32, _i32p]
     L.orc_count_common.restype = c.c_uint32
     L.orc_count_common.argtypes = [_u64p, _u32p, c.c_uint32, c.c_uint32]
     _lib = L
@@ -228,6 +230,33 @@ def sssp_bf(g: CSR, src: int) -> np.ndarray:
     out = np.empty(g.n, np.int32)
     lib().orc_sssp_bf(g.n, g.off, g.nbr, src, out)
     return out
+
+
+def sssp_bf_weighted(g: CSR, w: np.ndarray, src: int) -> np.ndarray:
+    """w: one length per CSR entry (g.nbr order)."""
+    out = np.empty(g.n, np.int32)
+    lib().orc_sssp_bf_weighted(g.n, g.off, g.nbr, np.ascontiguousarray(w[:g.m], np.uint32), src, out)
+    return out
+
+
+# ---------------------------------------------------------------- weighted edge sets
+def weighted_insert(keys, wts, ins_keys, ins_w):
+    """The weighted build's add_edge_batch as a set model: within the batch
+    the first weight of a key counts; keys already present keep theirs."""
+    ins_keys = np.asarray(ins_keys, np.uint64)
+    ins_w = np.asarray(ins_w, np.uint32)
+    uk, first = np.unique(ins_keys, return_index=True)
+    uw = ins_w[first]
+    new = ~np.isin(uk, keys)
+    allk = np.concatenate([keys, uk[new]])
+    allw = np.concatenate([wts, uw[new]])
+    order = np.argsort(allk, kind="stable")
+    return allk[order], allw[order]
+
+
+def weighted_delete(keys, wts, del_keys):
+    keep = ~np.isin(keys, np.asarray(del_keys, np.uint64))
+    return keys[keep], wts[keep]
 
 
 def rel_l1(a: np.ndarray, b: np.ndarray) -> float:

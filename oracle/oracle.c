@@ -471,3 +471,48 @@ void orc_sssp_bf(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint32_t 
   free(cur);
   free(nxt);
 }
+
+void orc_sssp_bf_weighted(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *w, uint32_t src,
+                          int32_t *dist) {
+  /* BellmanFord.h:67-143: the rounds of orc_sssp_bf with BF_F::update's
+   * edge length (:30-41) taken from the edge's weight. */
+  const int32_t inf = INT32_MAX / 2;
+  uint8_t *visited = (uint8_t *)calloc(n ? n : 1, 1);
+  uint32_t *cur = (uint32_t *)malloc((uint64_t)(n ? n : 1) * sizeof(uint32_t));
+  uint32_t *nxt = (uint32_t *)malloc((uint64_t)(n ? n : 1) * sizeof(uint32_t));
+  for (uint32_t v = 0; v < n; v++) dist[v] = inf;
+  dist[src] = 0;
+  uint64_t nc = 1, round = 0;
+  cur[0] = src;
+  while (nc > 0) {
+    if (round == n) {
+      for (uint32_t v = 0; v < n; v++) dist[v] = -inf;
+      break;
+    }
+    uint64_t nn = 0;
+    for (uint64_t i = 0; i < nc; i++) {
+      uint32_t s = cur[i];
+      for (uint64_t e = off[s]; e < off[s + 1]; e++) {
+        uint32_t d = nbr[e];
+        int64_t nd64 = (int64_t)dist[s] + w[e];
+        int32_t nd = nd64 >= inf ? inf : (int32_t)nd64;
+        if (dist[d] > nd) {
+          dist[d] = nd;
+          if (!visited[d]) {
+            visited[d] = 1;
+            nxt[nn++] = d;
+          }
+        }
+      }
+    }
+    for (uint64_t i = 0; i < nn; i++) visited[nxt[i]] = 0;
+    uint32_t *t = cur;
+    cur = nxt;
+    nxt = t;
+    nc = nn;
+    round++;
+  }
+  free(visited);
+  free(cur);
+  free(nxt);
+}

@@ -101,6 +101,12 @@ uint64_t orc_ltc(const uint64_t *off, const uint32_t *nbr, uint32_t v);
  * dist[v] = hop distance from src, INT_MAX/2 when unreachable. */
 void orc_sssp_bf(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint32_t src, int32_t *dist);
 
+/* SSSP_BF of the weighted build (include/BellmanFord.h:67-143): w[e] is the
+ * length of CSR entry e (edge off[s] <= e < off[s+1] from s); sums saturate
+ * at INT_MAX/2 (unreachable). */
+void orc_sssp_bf_weighted(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *w, uint32_t src,
+                          int32_t *dist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,11 @@ _SIGS = {
     "tg_last_error": (ctypes.c_char_p, []),
     "tg_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(_P)]),
     "tg_destroy": (ctypes.c_int, [_P]),
+    "tg_create_weighted": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(_P)]),
+    "tg_is_weighted": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
+    "tg_insert_batch_weighted": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_uint64]),
+    "tg_insert_batch_weighted_dev": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_uint64]),
+    "tg_edge_weights": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64, _P]),
     "tg_insert_batch": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64]),
     "tg_delete_batch": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64]),
     "tg_insert_batch_dev": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64]),

@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libterrace_hip.so")
 
-SOURCES = ["prims.hip", "container.hip", "traverse.hip", "gen.hip", "serve.cpp", "io.cpp", "capi.cpp"]
+SOURCES = ["prims.hip", "weights.hip", "container.hip", "traverse.hip", "gen.hip", "serve.cpp", "io.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -88,6 +88,61 @@ int tg_create(uint32_t num_vertices, tg_graph **out) {
   });
 }
 
+int tg_create_weighted(uint32_t num_vertices, tg_graph **out) {
+  return guard([&] {
+    NEED(out, "null out");
+    *out = nullptr;
+    NEED(num_vertices > 0, "num_vertices must be > 0");
+    std::unique_ptr<tg_graph> G(new tg_graph(num_vertices));
+    G->g.enable_weights();
+    *out = G.release();
+  });
+}
+
+int tg_is_weighted(tg_graph *G, int *out) {
+  return guard([&] {
+    NEED(G && out, "null argument");
+    *out = G->g.weighted() ? 1 : 0;
+  });
+}
+
+int tg_insert_batch_weighted(tg_graph *G, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
+                             uint64_t count) {
+  return guard([&] {
+    NEED(G && (count == 0 || (src && dst && w)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    if (!count) return;
+    hipStream_t st = G->g.stream();
+    stage_pair(G, src, dst, count);
+    G->s_u32.ensure(count + 1, st);
+    h2d(G->s_u32.get(), w, count * 4, st);
+    G->g.insert_batch_weighted(G->s_src.get(), G->s_dst.get(), G->s_u32.get(), count);
+  });
+}
+
+int tg_insert_batch_weighted_dev(tg_graph *G, const uint32_t *src, const uint32_t *dst, const uint32_t *w,
+                                 uint64_t count) {
+  return guard([&] {
+    NEED(G && (count == 0 || (src && dst && w)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->g.insert_batch_weighted(src, dst, w, count);
+  });
+}
+
+int tg_edge_weights(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint64_t count, uint32_t *out) {
+  return guard([&] {
+    NEED(G && (count == 0 || (src && dst && out)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    NEED(G->g.weighted(), "graph is not weighted");
+    if (!count) return;
+    hipStream_t st = G->g.stream();
+    stage_pair(G, src, dst, count);
+    G->s_u32.ensure(count + 1, st);
+    G->g.edge_weights(G->s_src.get(), G->s_dst.get(), count, G->s_u32.get());
+    d2h(out, G->s_u32.get(), count * 4, st);
+  });
+}
+
 int tg_destroy(tg_graph *g) {
   return guard([&] { delete g; });
 }

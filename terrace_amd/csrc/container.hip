@@ -171,6 +171,10 @@ __global__ void k_scatter_flagged(const uint64_t *in, const uint32_t *flags, con
 }
 
 // exists flag per (sorted unique) key
+__global__ void k_check_weights(const uint32_t *w, uint64_t n, uint64_t *err) {
+  GRID_STRIDE(i, n) if (w[i] > 0x7FFFFFFFu) atomicOr(reinterpret_cast<unsigned long long *>(err), 8ull);
+}
+
 __global__ void k_exists(GView g, const uint64_t *keys, uint64_t n, uint32_t *ex) {
   GRID_STRIDE(i, n) ex[i] = edge_exists(g, key_src(keys[i]), key_dst(keys[i])) ? 1u : 0u;
 }
@@ -419,6 +423,18 @@ __global__ void k_u32_pair(const uint32_t *a, const uint32_t *b, uint64_t *out) 
 __global__ void k_put_u32_at(uint32_t *p, uint64_t i, uint32_t v) { p[i] = v; }
 
 // ---- selectors (select.hpp): predicate + emit, no flag arrays
+// first key of each run of equal keys, with its weight
+struct SelUniqueW {
+  const uint64_t *k;
+  const uint32_t *w;
+  uint64_t *out;
+  uint32_t *wout;
+  __device__ bool pred(uint64_t i) const { return i == 0 || k[i] != k[i - 1]; }
+  __device__ void emit(uint64_t i, uint64_t pos) const {
+    out[pos] = k[i];
+    wout[pos] = w[i];
+  }
+};
 struct SelUnique {
   const uint64_t *k;
   uint64_t *out;
@@ -1729,6 +1745,7 @@ void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
 // ------------------------------------------------------------ batches
 void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
   if (count == 0) return;
+  if (ws_) throw Error(-1, "weighted graph: insert with weights");
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
   // sort, unique, existence and source runs sized by count; the unique count
@@ -1967,6 +1984,7 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
     pma_delete(k3_.get(), h[0]);
     tree_delete(g0_.get(), h[1]);
   }
+  if (ws_) ws_->erase(R, nR, stream_);  // R holds exactly the deleted (existing) keys
   check_batch_flags();
 }
 
@@ -1984,8 +2002,56 @@ __global__ void k_last_op_flags(const uint64_t *keys, const uint32_t *ops, uint6
 // arrival order.  The stable sort keeps arrival order inside each key, so
 // the last op of a key decides, which equals applying the batches one by
 // one (both ops are idempotent set updates).
+void Graph::enable_weights() {
+  if (ws_) return;
+  if (num_edges() != 0) throw Error(-1, "weights must be enabled on an empty graph");
+  ws_ = std::make_unique<WeightStore>();
+}
+
+void Graph::insert_batch_weighted(const uint32_t *src, const uint32_t *dst, const uint32_t *w, uint64_t count) {
+  if (count == 0) return;
+  if (!ws_) throw Error(-1, "graph is not weighted");
+  if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
+  if (read_only_) throw Error(-1, "graph is a query-only snapshot");
+  k0_.ensure(count, stream_);
+  k1_.ensure(count + 1, stream_);
+  wv0_.ensure(count, stream_);
+  wv1_.ensure(count + 1, stream_);
+  fill_u64(dsc_.get(), 0, kErrSort, stream_);
+  k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
+  TG_CHECK_LAUNCH();
+  k_check_weights<<<grid_for(count, 256), 256, 0, stream_>>>(w, count, dsc_.get());
+  TG_CHECK_LAUNCH();
+  copy_bytes(wv0_.get(), w, count * 4, stream_);
+  uint32_t bits = 1;
+  while (bits < 32 && (1ull << bits) < n_) bits++;
+  radix_sort_pairs(k0_.get(), wv0_.get(), count, bits, bits, sort_, stream_);  // stable: input order within a key
+  select(SelUniqueW{k0_.get(), wv0_.get(), k1_.get(), wv1_.get()}, count, dsc_.get() + 1, scan_scratch, stream_);
+  uint64_t h[2];
+  read_scalars(dsc_.get(), h, 2, stream_);
+  if (h[0] & 1) throw Error(-1, "vertex id out of range");
+  if (h[0] & 8) throw Error(-1, "edge weight must be below 2^31");
+  uint64_t nR = h[1];
+  // the keys absent before the insert get their weights
+  wput_.ensure(nR + 1, stream_);
+  s32_[29].ensure(nR + 1, stream_);
+  k_exists2<<<grid_for(nR, 256), 256, 0, stream_>>>(view(), k1_.get(), nR, s32_[29].get(), wput_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(wput_.get(), s32_[29].get(), nR, s32_[29].get() + nR, scan_scratch, stream_);
+  uint64_t nnew = read_scalar(s32_[29].get() + nR, stream_);
+  insert_sorted(k1_.get(), nR);
+  ws_->insert(k1_.get(), wv1_.get(), wput_.get(), nR, nnew, stream_);
+  TG_HIP(hipStreamSynchronize(stream_));
+}
+
+void Graph::edge_weights(const uint32_t *src, const uint32_t *dst, uint64_t count, uint32_t *out) {
+  if (!ws_) throw Error(-1, "graph is not weighted");
+  ws_->lookup(src, dst, count, out, stream_);
+}
+
 void Graph::apply_ops(const uint32_t *src, const uint32_t *dst, const uint32_t *ops, uint64_t count) {
   if (count == 0) return;
+  if (ws_) throw Error(-1, "weighted graph: coalesced ops carry no weights");
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
   k0_.ensure(count, stream_);
   fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
@@ -2153,7 +2219,7 @@ void Graph::copy_from(Graph &src, bool query_only) {
 }
 
 uint64_t Graph::memory_bytes() const {
-  return vb_.cap * sizeof(VBlock) + (pdst_.cap + psrc_.cap + pcnt_.cap) * 4 + fence_.cap * 8 + mark_.cap +
+  return (ws_ ? ws_->bytes() : 0) + vb_.cap * sizeof(VBlock) + (pdst_.cap + psrc_.cap + pcnt_.cap) * 4 + fence_.cap * 8 + mark_.cap +
          (bkeys_.cap + bcnt_.cap + iblk_.cap + imin_.cap) * 4 + toff_.cap * 8 + (tnblk_.cap + tvtx_.cap) * 4;
 }
 

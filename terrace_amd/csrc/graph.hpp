@@ -11,7 +11,9 @@
 #pragma once
 
 #include "prims.hpp"
+#include "weights.hpp"
 
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -76,6 +78,17 @@ class Graph {
   // Coalesced update batches: ops[i] 0 = insert, 1 = delete, arrival order;
   // the result equals applying them one after another.
   void apply_ops(const uint32_t *src, const uint32_t *dst, const uint32_t *ops, uint64_t count);
+
+  // Weighted graphs (the reference's WEIGHTED build): one uint32 weight per
+  // directed edge, below 2^31 (the Bellman-Ford edge length is an int32).
+  // A batch keeps the first weight given for a key; inserting an edge that
+  // exists leaves its weight unchanged; deleting an edge drops its weight.
+  void enable_weights();
+  bool weighted() const { return ws_ != nullptr; }
+  void insert_batch_weighted(const uint32_t *src, const uint32_t *dst, const uint32_t *w, uint64_t count);
+  WView weight_view() const { return ws_ ? ws_->view() : WView{}; }
+  // out[i] = weight of (src[i], dst[i]), or 0xFFFFFFFF when absent
+  void edge_weights(const uint32_t *src, const uint32_t *dst, uint64_t count, uint32_t *out);
 
   // Queries (device pointers).
   void degrees(uint32_t *out) const;
@@ -155,6 +168,8 @@ class Graph {
   // tier lists
   bool lists_dirty_ = true;
   uint64_t epoch_ = 0;
+  std::unique_ptr<WeightStore> ws_;  // edge weights (weighted graphs only)
+  DevBuf<uint32_t> wv0_, wv1_, wput_;
   TierLists lists_;
   DevBuf<uint32_t> list_pma_, list_tree_;
 

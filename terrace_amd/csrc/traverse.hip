@@ -85,6 +85,7 @@ struct PullBFS {
   __device__ bool skip(uint32_t i) const { return parent[i] != -1; }
   __device__ bool member(uint32_t v) const { return bit_get(fbits, v); }
   __device__ Acc value(uint32_t v) const { return v; }
+  __device__ Acc value_e(uint32_t, uint32_t u) const { return value(u); }
   __device__ static Acc combine(Acc a, Acc b) { return a < b ? a : b; }
   __device__ static Acc wave_reduce(Acc a) { return wave_min(a); }
   __device__ static Acc reduce16(Acc a, uint64_t) { return group16_min(a); }
@@ -114,6 +115,7 @@ struct PullCC {
   __device__ bool skip(uint32_t) const { return false; }
   __device__ bool member(uint32_t v) const { return fbits == nullptr || bit_get(fbits, v); }
   __device__ Acc value(uint32_t v) const { return ids[v]; }
+  __device__ Acc value_e(uint32_t, uint32_t u) const { return value(u); }
   __device__ static Acc combine(Acc a, Acc b) { return a < b ? a : b; }
   __device__ static Acc wave_reduce(Acc a) { return wave_min(a); }
   __device__ static Acc reduce16(Acc a, uint64_t) { return group16_min(a); }
@@ -150,6 +152,48 @@ struct PullBF {
   __device__ bool skip(uint32_t) const { return false; }
   __device__ bool member(uint32_t v) const { return fbits == nullptr || bit_get(fbits, v); }
   __device__ Acc value(uint32_t v) const { return static_cast<uint32_t>(dist[v] + 1); }
+  __device__ Acc value_e(uint32_t, uint32_t u) const { return value(u); }
+  __device__ static Acc combine(Acc a, Acc b) { return a < b ? a : b; }
+  __device__ static Acc wave_reduce(Acc a) { return wave_min(a); }
+  __device__ static Acc reduce16(Acc a, uint64_t) { return group16_min(a); }
+  __device__ static bool found(Acc) { return false; }
+  __device__ void commit(uint32_t i, Acc a, Mode m) const {
+    if (a == 0xFFFFFFFFu) return;
+    int32_t nd = static_cast<int32_t>(a);
+    bool lowered;
+    if (m == kShared) {
+      lowered = atomicMin(&dist[i], nd) > nd;
+    } else {
+      lowered = nd < dist[i];
+      if (lowered) dist[i] = nd;
+    }
+    if (lowered && obits && claim_byte(visited, i)) bit_set(obits, i);
+  }
+};
+
+// Weighted SSSP_BF pull (BF_F::update, include/BellmanFord.h:30-41): the
+// edge length is the weight stored for (i, s), the entry of i's own list,
+// as the reference's dense map reads it.  Sums saturate at INT_MAX / 2.
+struct PullBFW {
+  const uint32_t *fbits;
+  int32_t *dist;
+  uint8_t *visited;
+  uint32_t *obits;
+  WView ws;
+  using Acc = uint32_t;
+  static constexpr bool kEarly = false;
+  static constexpr bool kOrdered = false;
+  static constexpr const char *kName = "bfw";
+  __device__ static Acc ident() { return 0xFFFFFFFFu; }
+  __device__ bool skip(uint32_t) const { return false; }
+  __device__ bool member(uint32_t v) const { return fbits == nullptr || bit_get(fbits, v); }
+  __device__ Acc value(uint32_t) const { return 0xFFFFFFFFu; }
+  __device__ Acc value_e(uint32_t i, uint32_t u) const {
+    uint32_t w = ws.lookup(i, u);
+    if (w == kWsMissing) return 0xFFFFFFFFu;
+    int64_t nd = static_cast<int64_t>(dist[u]) + w;
+    return nd >= 0x3FFFFFFF ? 0x3FFFFFFFu : static_cast<uint32_t>(nd);
+  }
   __device__ static Acc combine(Acc a, Acc b) { return a < b ? a : b; }
   __device__ static Acc wave_reduce(Acc a) { return wave_min(a); }
   __device__ static Acc reduce16(Acc a, uint64_t) { return group16_min(a); }
@@ -192,6 +236,7 @@ struct PullSum {
   __device__ bool skip(uint32_t i) const { return visited != nullptr && visited[i]; }
   __device__ bool member(uint32_t v) const { return fbits == nullptr || bit_get(fbits, v); }
   __device__ Acc value(uint32_t v) const { return SumAcc{src[v], 1}; }
+  __device__ Acc value_e(uint32_t, uint32_t u) const { return value(u); }
   __device__ static Acc combine(Acc a, Acc b) { return SumAcc{a.s + b.s, a.any | b.any}; }
   __device__ static Acc wave_reduce(Acc a) {
     a.s = wave_sum_d(a.s);
@@ -242,7 +287,7 @@ __global__ void __launch_bounds__(512) k_pull_inline(GView g, Op op) {
     for (uint32_t k = 0; k < kInline; k++) {
       uint32_t u = w[1 + k];
       if (!done && k < lim && op.member(u)) {
-        a = Op::combine(a, op.value(u));
+        a = Op::combine(a, op.value_e(v, u));
         if (Op::kEarly && Op::found(a)) done = true;
       }
     }
@@ -285,7 +330,7 @@ __global__ void __launch_bounds__(512) k_pull_pma(GView g, Op op, const uint32_t
       for (int j = 0; j < 4; j++) {
         if (val[j]) {
           if (got + r < need && op.member(d[j])) {
-            a = Op::combine(a, op.value(d[j]));
+            a = Op::combine(a, op.value_e(v, d[j]));
             hit = true;
           }
           r++;
@@ -326,7 +371,7 @@ __global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_
     typename Op::Acc a = Op::ident();
     if (threadIdx.x < c) {
       uint32_t d = g.bkeys[static_cast<uint64_t>(blk) * kBlk + threadIdx.x];
-      if (op.member(d)) a = op.value(d);
+      if (op.member(d)) a = op.value_e(v, d);
     }
     a = Op::wave_reduce(a);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = a;
@@ -383,6 +428,22 @@ struct PushBF {
   uint8_t *visited;
   __device__ bool apply(uint32_t s, uint32_t d) const {
     int32_t nd = dist[s] + 1;
+    if (nd >= dist[d]) return false;
+    if (atomicMin(&dist[d], nd) <= nd) return false;
+    return claim_byte(visited, d);
+  }
+};
+// Weighted SSSP_BF push (BF_F::updateAtomic, include/BellmanFord.h:42-47):
+// edge length = weight of (s, d); sums saturate at INT_MAX / 2.
+struct PushBFW {
+  int32_t *dist;
+  uint8_t *visited;
+  WView ws;
+  __device__ bool apply(uint32_t s, uint32_t d) const {
+    uint32_t w = ws.lookup(s, d);
+    if (w == kWsMissing) return false;
+    int64_t nd64 = static_cast<int64_t>(dist[s]) + w;
+    int32_t nd = nd64 >= 0x3FFFFFFF ? 0x3FFFFFFF : static_cast<int32_t>(nd64);
     if (nd >= dist[d]) return false;
     if (atomicMin(&dist[d], nd) <= nd) return false;
     return claim_byte(visited, d);
@@ -689,6 +750,7 @@ struct PullMS {
   __device__ bool skip(uint32_t i) const { return (vis[i] & full) == full; }
   __device__ bool member(uint32_t) const { return true; }
   __device__ Acc value(uint32_t v) const { return fr[v]; }
+  __device__ Acc value_e(uint32_t, uint32_t u) const { return value(u); }
   __device__ static Acc combine(Acc a, Acc b) { return a | b; }
   __device__ static Acc wave_reduce(Acc a) {
 #pragma unroll
@@ -1317,8 +1379,13 @@ void Traversal::pull(Frontier &in, Op op, const OpState &st, Frontier *out) {
     }
     case Op::BF: {
       if (!st.dist || !st.visited) throw Error(-1, "BF needs dist and visited");
-      PullBF p{fb, st.dist, const_cast<uint8_t *>(st.visited), ob};
-      launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+      if (g_.weighted()) {
+        PullBFW p{fb, st.dist, const_cast<uint8_t *>(st.visited), ob, g_.weight_view()};
+        launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+      } else {
+        PullBF p{fb, st.dist, const_cast<uint8_t *>(st.visited), ob};
+        launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+      }
       break;
     }
     default:
@@ -1393,8 +1460,13 @@ void Traversal::push(Frontier &in, Op op, const OpState &st, Frontier *out) {
     }
     case Op::BF: {
       if (!st.dist || !st.visited) throw Error(-1, "BF needs dist and visited");
-      PushBF p{st.dist, const_cast<uint8_t *>(st.visited)};
-      launch_push(*this, g_, p, in, out, cnt_, sec_pma_, sec_tree_);
+      if (g_.weighted()) {
+        PushBFW p{st.dist, const_cast<uint8_t *>(st.visited), g_.weight_view()};
+        launch_push(*this, g_, p, in, out, cnt_, sec_pma_, sec_tree_);
+      } else {
+        PushBF p{st.dist, const_cast<uint8_t *>(st.visited)};
+        launch_push(*this, g_, p, in, out, cnt_, sec_pma_, sec_tree_);
+      }
       break;
     }
     default:
@@ -1620,7 +1692,7 @@ uint32_t Traversal::sssp_bf(uint32_t src, int32_t *dist, uint32_t threshold) {
   Frontier *cur = &a, *nxt = &b;
   uint32_t round = 0;
   while (cur->count > 0) {
-    if (round == n) {  // a negative cycle in the reference's rule; unreachable with unit lengths
+    if (round == n) {  // a negative cycle in the reference's rule; lengths here are never negative
       k_fill_i32<<<grid_for(n, 256), 256, 0, s>>>(dist, -inf, n);
       TG_CHECK_LAUNCH();
       break;

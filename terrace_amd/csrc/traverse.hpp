@@ -73,8 +73,9 @@ class Traversal {
   void pagerank(uint32_t iters, double *out);
   uint32_t cc(uint32_t *ids, uint32_t threshold = 0);
   void bc(uint32_t src, double *out, uint32_t threshold = 0);
-  // SSSP_BF with unit lengths (include/BellmanFordUnweighted.h:60-130):
-  // dist = hop distance, INT_MAX / 2 unreachable; returns the rounds run.
+  // SSSP_BF (include/BellmanFordUnweighted.h:60-130; on a weighted graph
+  // include/BellmanFord.h:67-143 with the stored weights as lengths):
+  // dist = shortest distance, INT_MAX / 2 unreachable; returns the rounds.
   uint32_t sssp_bf(uint32_t src, int32_t *dist, uint32_t threshold = 0);
   uint64_t tc();
   // 64 BFS in one traversal: levels[j*n + v] for source j (k <= 64).

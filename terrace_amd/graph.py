@@ -26,11 +26,33 @@ def _u32(a) -> np.ndarray:
 class Graph:
     """graphstore::Graph (include/graph.h:73-230) resident in HBM."""
 
-    def __init__(self, size: int):
+    def __init__(self, size: int, weighted: bool = False):
         h = ctypes.c_void_p()
-        check(lib().tg_create(int(size), ctypes.byref(h)))
+        create = lib().tg_create_weighted if weighted else lib().tg_create
+        check(create(int(size), ctypes.byref(h)))
         self._h = h
         self._n = int(size)
+
+    @property
+    def weighted(self) -> bool:
+        out = ctypes.c_int()
+        check(lib().tg_is_weighted(self._h, ctypes.byref(out)))
+        return bool(out.value)
+
+    def add_edge_batch_weighted(self, srcs, dests, weights) -> None:
+        """Weighted add_edge_batch (include/graph.h:430-436): the first
+        weight given for an edge is kept; existing edges keep theirs."""
+        s, d, w = _u32(srcs), _u32(dests), _u32(weights)
+        if not (len(s) == len(d) == len(w)):
+            raise ValueError("srcs, dests and weights differ in length")
+        check(lib().tg_insert_batch_weighted(self._h, ptr(s), ptr(d), ptr(w), len(s)))
+
+    def edge_weights(self, srcs, dests) -> np.ndarray:
+        """Weight of each edge, 0xFFFFFFFF where absent."""
+        s, d = _u32(srcs), _u32(dests)
+        out = np.empty(len(s), np.uint32)
+        check(lib().tg_edge_weights(self._h, ptr(s), ptr(d), len(s), ptr(out)))
+        return out
 
     @classmethod
     def from_adjacency_graph(cls, path: str) -> "Graph":

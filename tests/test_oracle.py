@@ -299,3 +299,47 @@ def test_planted_mistakes_are_caught():
     bc = oracle.bc(g, int(z["src"][0]))
     reach = np.isfinite(bc)
     assert oracle.rel_l1(bc[reach] / 2.0, z["bc"][reach]) > 1e-3
+
+
+# --------------------------------------------------------------- weighted SSSP
+def dijkstra(g, w, src):
+    import heapq
+    inf = (2**31 - 1) // 2
+    dist = np.full(g.n, inf, np.int64)
+    dist[src] = 0
+    pq = [(0, src)]
+    while pq:
+        d, u = heapq.heappop(pq)
+        if d > dist[u]:
+            continue
+        for e in range(int(g.off[u]), int(g.off[u + 1])):
+            v = int(g.nbr[e])
+            nd = d + int(w[e])
+            if nd < dist[v]:
+                dist[v] = nd
+                heapq.heappush(pq, (nd, v))
+    return np.minimum(dist, inf).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_sssp_bf_weighted_equals_dijkstra(seed):
+    # positive lengths: the Bellman-Ford fixed point is the shortest distance
+    n = 1 << 9
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 90 + seed, 0.57, 0.19, 0.19, 3000))
+    g = oracle.CSR(keys, n)
+    rng = np.random.default_rng(seed)
+    w = rng.integers(1, 1000, g.m).astype(np.uint32)
+    for src in (0, int(np.argmax(g.degrees())), 7):
+        assert np.array_equal(oracle.sssp_bf_weighted(g, w, src), dijkstra(g, w, src))
+    # unit lengths reduce to the unweighted form
+    assert np.array_equal(oracle.sssp_bf_weighted(g, np.ones(g.m, np.uint32), 0), oracle.sssp_bf(g, 0))
+
+
+def test_weighted_set_model():
+    k0 = np.array([5, 9], np.uint64)
+    w0 = np.array([50, 90], np.uint32)
+    # duplicate key 7 in one batch: the first weight counts; 9 exists: kept
+    k, w = oracle.weighted_insert(k0, w0, [7, 9, 7, 1], [70, 99, 71, 10])
+    assert k.tolist() == [1, 5, 7, 9] and w.tolist() == [10, 50, 70, 90]
+    k, w = oracle.weighted_delete(k, w, [5, 8])
+    assert k.tolist() == [1, 7, 9] and w.tolist() == [10, 70, 90]
