@@ -85,6 +85,10 @@ class Graph {
   };
 
   explicit Graph(uint32_t size) : n_(size) { tg_check(tg_create(size, &h_)); }
+  // The WEIGHTED build's graph (include/graph.h:34-38) as a run-time choice.
+  Graph(uint32_t size, bool weighted) : n_(size) {
+    tg_check(weighted ? tg_create_weighted(size, &h_) : tg_create(size, &h_));
+  }
   ~Graph() {
     if (h_) tg_destroy(h_);
   }
@@ -107,6 +111,19 @@ class Graph {
   void add_edge_batch(vertex *srcs, vertex *dests, uint32_t edge_count, std::vector<uint32_t> &perm) {
     (void)perm;
     tg_check(tg_insert_batch(h_, srcs, dests, edge_count));
+  }
+  // graph.h:430-436 (WEIGHTED): the first weight of an edge in the batch
+  // counts; existing edges keep their weight.
+  void add_edge_batch(vertex *srcs, vertex *dests, weight *wghts, uint32_t edge_count, std::vector<uint32_t> &perm) {
+    (void)perm;
+    tg_check(tg_insert_batch_weighted(h_, srcs, dests, wghts, edge_count));
+  }
+  // is_edge of the weighted build returns the weight (0xFFFFFFFF here when
+  // absent).
+  weight edge_weight(const vertex s, const vertex d) {
+    weight w = 0;
+    tg_check(tg_edge_weights(h_, &s, &d, 1, &w));
+    return w;
   }
   void remove_edge_batch(const vertex *srcs, const vertex *dests, uint32_t edge_count) {
     tg_check(tg_delete_batch(h_, srcs, dests, edge_count));

@@ -316,6 +316,8 @@ struct vertex_op_traits<BF_Vertex_F> {
   }
 };
 
+// On a weighted graph the same loop is BellmanFord.h's SSSP_BF (:67-143):
+// the device op takes the stored weights as edge lengths.
 inline int32_t *SSSP_BF(Graph &G, long start) {
   const uint64_t n = G.get_num_vertices();
   const int32_t inf = 0x7FFFFFFF / 2;

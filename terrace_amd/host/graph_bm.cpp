@@ -2,7 +2,7 @@
 // the GPU container, written against the graphstore shim.
 //
 //   graph_bm -s SCALE -m EDGES [-a A -b B -c C] [-src V] [-rounds R]
-//            [-maxiters I] [-TC 1] [-SSSP 1] [-batches 10,100,...] [-trials T]
+//            [-maxiters I] [-TC 1] [-SSSP 1] [-weighted 1] [-batches 10,100,...] [-trials T]
 //            [-o PREFIX]
 //
 // The static graph is rMat<uint32_t>(2^SCALE, hash64(100), a, b, c) edges
@@ -68,6 +68,13 @@ void dump(const std::string &path, const T *p, size_t n) {
 
 }  // namespace
 
+// Edge length for the weighted run: symmetric, in [1, 1000].
+static uint32_t sym_weight(uint32_t u, uint32_t v) {
+  uint64_t a = u < v ? u : v, b = u < v ? v : u;
+  uint64_t h = (a * 2654435761ull + b * 40503ull) >> 7;
+  return static_cast<uint32_t>(h % 1000 + 1);
+}
+
 int main(int argc, char **argv) {
   Args P;
   for (int i = 1; i + 1 < argc; i += 2) P.kv[argv[i]] = argv[i + 1];
@@ -83,14 +90,26 @@ int main(int argc, char **argv) {
   const std::string out = P.get("-o", "");
   const uint32_t n = 1u << scale;
 
-  Graph graph(n);
+  const bool weighted = P.num("-weighted", 0) != 0;  // the WEIGHTED build (src/graph_bm_weighted.cc)
+  Graph graph(n, weighted);
   {
     DeviceArray<uint32_t> s(2 * m0), d(2 * m0);
     uint64_t w = 0;
     tg_check(tg_rmat_dev(n, static_cast<uint32_t>(hash64(100)), a, b, c, 1, 0, m0, 3, s.data(), d.data(), &w));
     tg_check(tg_device_sync());
     double t0 = now();
-    tg_check(tg_insert_batch_dev(graph.handle(), s.data(), d.data(), w));
+    if (!weighted) {
+      tg_check(tg_insert_batch_dev(graph.handle(), s.data(), d.data(), w));
+    } else {
+      // symmetric lengths in [1, 1000]: w(u, v) = w(v, u)
+      std::vector<uint32_t> hs(w), hd(w), hw(w);
+      s.download(hs.data(), w);
+      d.download(hd.data(), w);
+      for (uint64_t i = 0; i < w; i++) hw[i] = sym_weight(hs[i], hd[i]);
+      t0 = now();
+      std::vector<uint32_t> perm;
+      graph.add_edge_batch(hs.data(), hd.data(), hw.data(), static_cast<uint32_t>(w), perm);
+    }
     tg_check(tg_device_sync());
     double t = now() - t0;
     printf("Construction finished. Nodes: %u Edges: %llu Size: %.4f GB\n", graph.get_num_vertices(),
@@ -140,6 +159,7 @@ int main(int argc, char **argv) {
     printf("AVG\ttest=%s\ttime=%f\tgraph=rmat%u\n", test.c_str(), total / rounds, scale);
   }
 
+  if (weighted) return 0;  // the update batches below carry no weights
   // update batches (src/graph_bm.cc:316-390): directed rMAT edges over half
   // the id space, insert then delete the same edges
   std::vector<uint64_t> sizes;
