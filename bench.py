@@ -475,8 +475,8 @@ def batching_bench(args):
 def skew_bench(args):
     """Skew stress across all three tiers: rMAT scale 22 with a=.57 b=.19
     c=.19 (max degree above 1e5; inline, PMA and B-tree tiers populated).
-    BFS, PageRank (10 iterations), CC, BC and TC from the highest-degree
-    vertex, then mixed update batches of 1e6 edges whose sources are the
+    BFS, PageRank (10 iterations), CC, BC, SSSP_BF and TC from the
+    highest-degree vertex (results left in HBM), then mixed update batches of 1e6 edges whose sources are the
     top-1% degree vertices (90% inserts to random active vertices, 10%
     deletes of existing edges of those sources; both directions).  The
     adjacency after the first batch is checked against a host set model."""
@@ -502,23 +502,35 @@ def skew_bench(args):
     log(f"skew: {m} directed edges in {t_build:.3f} s, max degree {int(deg.max())}, tiers {tiers}")
     res = {"static_build_s": t_build, "directed_edges": m, "max_degree": int(deg.max()),
            "active_vertices": int((deg > 0).sum()), "tiers": tiers, "source": src}
+    from terrace_amd._lib import DeviceBuffer as _DB, check as _check
+    par_d, lev_d, f64_d, i32_d = _DB(4 * n), _DB(4 * n), _DB(8 * n), _DB(4 * n)
+    depth = ctypes.c_uint32()
+    L = tg.lib()
+    L.tg_device_sync()
     t0 = time.perf_counter()
-    parents, levels = tg.BFS_with_edge_map(G, src, with_levels=True)
+    _check(L.tg_bfs_dev(G.handle, src, par_d.ptr, lev_d.ptr, ctypes.byref(depth)))
     t = time.perf_counter() - t0
+    levels = lev_d.download(np.uint32, n)
     reached = levels != np.iinfo(np.uint32).max
     res["bfs_ms"], res["bfs_gteps"] = t * 1e3, float(deg[reached].astype(np.int64).sum()) / t / 1e9
     t0 = time.perf_counter()
-    tg.PR_S(G, 10)
+    _check(L.tg_pagerank_dev(G.handle, 10, f64_d.ptr))
     t = time.perf_counter() - t0
     res["pr_ms"], res["pr_gteps"] = t * 1e3, 10 * m / t / 1e9
+    rounds = ctypes.c_uint32()
     t0 = time.perf_counter()
-    tg.CC(G)
+    _check(L.tg_cc_dev(G.handle, i32_d.ptr, ctypes.byref(rounds)))
     res["cc_ms"] = (time.perf_counter() - t0) * 1e3
     t0 = time.perf_counter()
-    tg.BC(G, src)
+    _check(L.tg_bc_dev(G.handle, src, f64_d.ptr))
     res["bc_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    _check(L.tg_sssp_bf_dev(G.handle, src, i32_d.ptr, ctypes.byref(rounds)))
+    res["sssp_bf_ms"] = (time.perf_counter() - t0) * 1e3
+    for b_ in (par_d, lev_d, f64_d, i32_d):
+        b_.close()
     log(f"skew: BFS {res['bfs_ms']:.1f} ms, PR {res['pr_ms']:.1f} ms, CC {res['cc_ms']:.1f} ms, "
-        f"BC {res['bc_ms']:.1f} ms")
+        f"BC {res['bc_ms']:.1f} ms, SSSP_BF {res['sssp_bf_ms']:.1f} ms")
     t0 = time.perf_counter()
     res["tc"] = tg.TC(G)
     res["tc_ms"] = (time.perf_counter() - t0) * 1e3
@@ -689,20 +701,27 @@ def main():
     if not args.no_extra:
         src = 9
         deg = G.degrees()
-        # BFS
+        # BFS / PR / CC through the device-output entry points: results stay in
+        # HBM (as the reference's stay in host memory); the copy back is not timed
+        from terrace_amd._lib import DeviceBuffer as _DB, check as _check
+        par_d, lev_d, pr_d, cc_d = _DB(4 * n), _DB(4 * n), _DB(8 * n), _DB(4 * n)
+        depth = ctypes.c_uint32()
         G.profile(True)
         G.profile_reset()
+        tg.lib().tg_device_sync()
         t0 = time.perf_counter()
-        parents = tg.BFS_with_edge_map(G, src)
+        _check(tg.lib().tg_bfs_dev(G.handle, src, par_d.ptr, lev_d.ptr, ctypes.byref(depth)))
         t_bfs = time.perf_counter() - t0
-        reached = parents >= 0
+        levels = lev_d.download(np.uint32, n)
+        reached = levels != np.iinfo(np.uint32).max
         bfs_edges = int(deg[reached].astype(np.int64).sum())
         extra["bfs_ms"] = t_bfs * 1e3
         extra["bfs_gteps"] = bfs_edges / t_bfs / 1e9
+        extra["bfs_levels"] = int(depth.value)
         # PR
         G.profile_reset()
         t0 = time.perf_counter()
-        tg.PR_S(G, 10)
+        _check(tg.lib().tg_pagerank_dev(G.handle, 10, pr_d.ptr))
         t_pr = time.perf_counter() - t0
         m_now = G.get_num_edges()
         extra["pr_ms"] = t_pr * 1e3
@@ -730,10 +749,18 @@ def main():
             ms_, n_ = G.profile_get(kn)
             extra[kn + "_avg_ms"] = ms_ / max(n_, 1)
         # CC
-        t0 = time.perf_counter()
-        tg.CC(G)
-        extra["cc_ms"] = (time.perf_counter() - t0) * 1e3
         G.profile(False)
+        rounds = ctypes.c_uint32()
+        t0 = time.perf_counter()
+        _check(tg.lib().tg_cc_dev(G.handle, cc_d.ptr, ctypes.byref(rounds)))
+        extra["cc_ms"] = (time.perf_counter() - t0) * 1e3
+        # PageRank with its iterations replayed from a captured HIP graph
+        # (the profiled run above launches them one by one)
+        t0 = time.perf_counter()
+        _check(tg.lib().tg_pagerank_dev(G.handle, 10, pr_d.ptr))
+        extra["pr_graph_replay_ms"] = (time.perf_counter() - t0) * 1e3
+        for b_ in (par_d, lev_d, pr_d, cc_d):
+            b_.close()
         for bsz in (() if args.no_extra_batches else (1000, 10_000_000)):
             bt = make_batches(tg, G, n, probs, bsz, 2, 5000 + bsz % 97, rng, off, nbr)
             apply_batch(G, bt[0])

@@ -539,18 +539,25 @@ __global__ void __launch_bounds__(256) k_push_inline(GView g, Op op, PushOut o, 
 
 // Four frontier vertices per wave (16-lane groups, 4 slots per lane); the
 // groups step in lockstep so the wave's output buffer stays uniform.
-template <class Op>
+constexpr uint32_t kPushWaveFrontier = 16384;  // frontiers up to this size: a wave per PMA vertex
+
+// GS lanes per frontier PMA vertex (16 or 64), 4 slots per lane per step.
+// Large frontiers use groups of 16 (four vertices per wave); small ones,
+// whose vertices are few and heavy, a whole wave (256 slots per step).
+template <class Op, int GS>
 __global__ void __launch_bounds__(256) k_push_pma(GView g, Op op, PushOut o, const uint32_t *list,
                                                   const uint32_t *cnt_ptr) {
+  static_assert(GS == 16 || GS == 64, "group of 16 lanes or a wave");
+  constexpr int GPW = 64 / GS;  // groups per wave
   __shared__ uint32_t lq[4][kWQ];
   const uint32_t cnt = *cnt_ptr;
-  const uint32_t lane = threadIdx.x & 63, gl = lane & 15, gi = lane >> 4;
-  const uint64_t gmask = 0xFFFFull << (lane & 48);
+  const uint32_t lane = threadIdx.x & 63, gl = lane & (GS - 1), gi = lane / GS;
+  const uint64_t gmask = GS == 64 ? ~0ull : (0xFFFFull << (lane & 48));
   const uint64_t lt = lanemask_lt() & gmask;
   WaveQueue out{lq[threadIdx.x >> 6], o.queue, o.count, 0};
   uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
   uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
-  for (uint64_t base = wave * 4; base < cnt; base += nwaves * 4) {
+  for (uint64_t base = wave * GPW; base < cnt; base += nwaves * GPW) {
     uint64_t w = base + gi;
     bool have = w < cnt;
     uint32_t s = have ? list[w] : 0;
@@ -578,7 +585,7 @@ __global__ void __launch_bounds__(256) k_push_pma(GView g, Op op, PushOut o, con
       }
       if (act) {
         got += __popcll(B0) + __popcll(B1) + __popcll(B2) + __popcll(B3);
-        p += 64;
+        p += GS * 4;
       }
     }
   }
@@ -1421,7 +1428,11 @@ static void launch_push(Traversal &T, Graph &g, const Op &op, Frontier &in, Fron
   uint32_t fc = static_cast<uint32_t>(in.count);
   k_push_inline<Op><<<grid_for(fc, 256), 256, 0, s>>>(v, op, o, in.queue.get(), fc, sp.get(), st.get(), cnt.get());
   TG_CHECK_LAUNCH();
-  k_push_pma<Op><<<grid_for(static_cast<uint64_t>(fc) * 16, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
+  if (fc <= kPushWaveFrontier) {
+    k_push_pma<Op, 64><<<grid_for(static_cast<uint64_t>(fc) * 64, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
+  } else {
+    k_push_pma<Op, 16><<<grid_for(static_cast<uint64_t>(fc) * 16, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
+  }
   TG_CHECK_LAUNCH();
   k_push_tree<Op><<<grid_for(static_cast<uint64_t>(fc) * 256, 256, 1u << 16), 256, 0, s>>>(v, op, o, st.get(),
                                                                                          cnt.get() + 1);
