@@ -29,10 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_pull_pma<PullSum> launch at the default workload (scale 22,
-# 69M generated edges), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
-# scaled by a 1 GiB copy kernel (FETCH x2.000, WRITE x1.000).
-PMA_PULL_TRAFFIC_DEFAULT = 6.144e9
+# HBM bytes per PageRank iteration (k_prs_pull + k_prs_fix + k_prs_reduce)
+# at the default workload (scale 22, 69M generated edges), from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy kernel
+# (FETCH_SIZE x 2, WRITE_SIZE x 1).
+PR_ITER_TRAFFIC_DEFAULT = 2.228e9
 
 
 def log(*a):
@@ -718,36 +719,61 @@ def main():
         extra["bfs_ms"] = t_bfs * 1e3
         extra["bfs_gteps"] = bfs_edges / t_bfs / 1e9
         extra["bfs_levels"] = int(depth.value)
-        # PR
+        # PR: the first call after the updates also builds the sliced
+        # adjacency copy (pr_ms includes it); the replay below reuses it
         G.profile_reset()
         t0 = time.perf_counter()
         _check(tg.lib().tg_pagerank_dev(G.handle, 10, pr_d.ptr))
         t_pr = time.perf_counter() - t0
         m_now = G.get_num_edges()
         extra["pr_ms"] = t_pr * 1e3
+        extra["pr_ms_includes_slice_build"] = n <= (6 << 20)
         extra["pr_gteps"] = 10 * m_now / t_pr / 1e9
-        # Roofline of the dominant kernel, k_pull_pma<PullSum> (one launch per
-        # PageRank iteration), timed with HIP events on the traversal's stream.
-        # Algorithmic bytes per launch: the PageRank per-iteration figure
-        # (4 B per edge + 8 B metadata + 16 B p/p_next per vertex) over the
-        # units one launch processes: the PMA-tier vertices and their edges
-        # beyond the 13 inline slots.
-        tc_ = G.tier_counts()
-        pma_ms, pma_n = G.profile_get("pull_pma_sum")
-        pma_bytes = 4 * tc_["pma_edges"] + 24 * tc_["pma_vertices"]
-        pma_avg = pma_ms / max(pma_n, 1)
-        achieved = pma_bytes / (pma_avg * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS,
-                    "traffic": PMA_PULL_TRAFFIC_DEFAULT if default_workload else None,
-                    "kernel": "k_pull_pma<PullSum> (PageRank dense pull over PMA-tier vertices)",
-                    "avg_ms": pma_avg, "launches": pma_n, "algorithmic_bytes": pma_bytes,
-                    "units": tc_}
-        pull_ms, pull_n = G.profile_get("edge_map_pull")
-        extra["pr_pull_all_tiers_avg_ms"] = pull_ms / max(pull_n, 1)
-        for kn in ("pull_inline_sum", "pull_tree_sum"):
+        # Roofline of the dominant kernel work, one PageRank iteration, timed
+        # with HIP events on the traversal's stream.  Graphs up to 6M vertices
+        # run the XCD-sliced pull (k_prs_pull + k_prs_fix + k_prs_reduce per
+        # iteration):
+        # algorithmic bytes = the PageRank per-iteration figure 4m + 24n
+        # (4 B per edge id, 8 B metadata and 16 B p read / p_next write per
+        # vertex).  Larger graphs run the tiered pull, whose dominant kernel is
+        # k_pull_pma<PullSum>: 4 B per PMA edge beyond the inline slots +
+        # 24 B per PMA-tier vertex.
+        it_ms, it_n = G.profile_get("pr_iteration")
+        if it_n:
+            it_avg = it_ms / it_n
+            it_bytes = 4 * m_now + 24 * n
+            achieved = it_bytes / (it_avg * 1e-3) / 1e9
+            pull_ms_, pull_n_ = G.profile_get("prs_pull")
+            red_ms_, red_n_ = G.profile_get("prs_reduce")
+            fix_ms_, fix_n_ = G.profile_get("prs_fix")
+            roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": PR_ITER_TRAFFIC_DEFAULT if default_workload else None,
+                        "kernel": "PageRank iteration: k_prs_pull + k_prs_fix + k_prs_reduce (XCD-sliced pull)",
+                        "avg_ms": it_avg, "launches": it_n, "algorithmic_bytes": it_bytes,
+                        "k_prs_pull_avg_ms": pull_ms_ / max(pull_n_, 1),
+                        "k_prs_fix_avg_ms": fix_ms_ / max(fix_n_, 1),
+                        "k_prs_reduce_avg_ms": red_ms_ / max(red_n_, 1)}
+        else:
+            tc_ = G.tier_counts()
+            pma_ms, pma_n = G.profile_get("pull_pma_sum")
+            pma_bytes = 4 * tc_["pma_edges"] + 24 * tc_["pma_vertices"]
+            pma_avg = pma_ms / max(pma_n, 1)
+            achieved = pma_bytes / (pma_avg * 1e-3) / 1e9
+            roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "k_pull_pma<PullSum> (PageRank dense pull over PMA-tier vertices)",
+                        "avg_ms": pma_avg, "launches": pma_n, "algorithmic_bytes": pma_bytes, "units": tc_}
+        # the tiered pull on the same graph, for comparison
+        G.set_tuning(pr_sliced=0)
+        G.profile_reset()
+        t0 = time.perf_counter()
+        _check(tg.lib().tg_pagerank_dev(G.handle, 10, pr_d.ptr))
+        extra["pr_tiered_pull_ms"] = (time.perf_counter() - t0) * 1e3
+        for kn in ("pull_inline_sum", "pull_pma_sum", "pull_tree_sum"):
             ms_, n_ = G.profile_get(kn)
             extra[kn + "_avg_ms"] = ms_ / max(n_, 1)
+        G.set_tuning(pr_sliced=1)
         # CC
         G.profile(False)
         rounds = ctypes.c_uint32()

@@ -132,10 +132,13 @@ class Graph {
   // workgroup size of the inline and PMA pull kernels.  Results do not
   // depend on them.
   // pr_graph: PageRank iterations replayed from a captured HIP graph.
+  // pr_sliced: PageRank over the XCD-sliced adjacency copy when the graph
+  // has at most 6M vertices (else the tiered pull).
   struct Tuning {
     uint32_t thr_bfs = 20, thr_ms = 20, thr_cc = 20;
     uint32_t pull_block = 256;
     uint32_t pr_graph = 1;
+    uint32_t pr_sliced = 1;
   };
   Tuning tune;
 

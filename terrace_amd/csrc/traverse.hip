@@ -740,6 +740,284 @@ __global__ void __launch_bounds__(256) k_tc(const uint64_t *off, const uint32_t 
   if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
 }
 
+// ---------------------------------------------------------------- PageRank, XCD-sliced pull
+// A graph with n <= kPrSlicedMaxN runs PageRank over a slice-major copy of
+// its adjacency.  The source ids are cut into eight ranges holding equal
+// edge counts (cut points from the CSR offsets, exact for a symmetric
+// graph); slice k lists, for every vertex v in order, v's neighbours that
+// fall in range k.  Workgroups with blockIdx % 8 == k (the dispatcher places
+// workgroups on the XCDs round-robin) work on slice k only, so the rank
+// gathers of one XCD stay within one eighth of the rank array and its L2.
+//
+// Each slice is a CSR of n rows; a workgroup takes a tile of kPrsTile
+// consecutive steps of the merge path of (row ends, edge positions)
+// (Merrill & Garland, "Merge-based parallel sparse matrix-vector
+// multiplication", SC'16), so hub rows and empty rows cost the same per
+// step.  The tile boundaries depend on the graph only: k_prs_coords finds
+// them once per graph version, and a tile starts with two loads instead of
+// a binary search.  A row that crosses tiles leaves one carry per tile it
+// crosses; k_prs_fix adds a row's carries in tile order, and k_prs_reduce
+// adds the eight slice partials in slice order: the result is the same on
+// every run.
+constexpr uint32_t kPrSlices = kPrsSlices;
+constexpr uint32_t kPrSlicedMaxN = 6u << 20;
+constexpr uint32_t kPrsItems = 8;  // merge-path steps per thread
+constexpr uint32_t kPrsBlock = 256;
+constexpr uint32_t kPrsTile = kPrsItems * kPrsBlock;
+
+// bnd[k] = first vertex whose CSR offset reaches k * m / 8 (bnd[0] = 0,
+// bnd[8] = n): range k = ids [bnd[k], bnd[k+1]).
+__global__ void k_prs_bounds(const uint64_t *off, uint32_t n, uint32_t *bnd) {
+  const uint32_t k = threadIdx.x;
+  if (k > kPrSlices) return;
+  const uint64_t m = off[n];
+  uint32_t r = n;
+  if (k == 0) {
+    r = 0;
+  } else if (k < kPrSlices) {
+    const uint64_t target = m * k / kPrSlices;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+      uint32_t mid = lo + (hi - lo) / 2;
+      if (off[mid] < target) lo = mid + 1;
+      else hi = mid;
+    }
+    r = lo;
+  }
+  bnd[k] = r;
+}
+
+// first position in nbr[b, e) (sorted) holding an id >= x
+__device__ inline uint64_t prs_first_ge(const uint32_t *nbr, uint64_t b, uint64_t e, uint32_t x) {
+  uint64_t lo = b, hi = e;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (nbr[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_prs_counts(const uint64_t *off, const uint32_t *nbr, uint32_t n, const uint32_t *bnd, uint32_t *cnt,
+                             uint32_t *deg) {
+  GRID_STRIDE(v, static_cast<uint64_t>(n) + 1) {
+    if (v == n) {
+      for (uint32_t k = 0; k < kPrSlices; k++) cnt[static_cast<uint64_t>(k) * (n + 1) + n] = 0;
+      continue;
+    }
+    uint64_t b = off[v], e = off[v + 1];
+    deg[v] = static_cast<uint32_t>(e - b);
+    uint64_t lo = b;
+    for (uint32_t k = 0; k < kPrSlices; k++) {
+      uint64_t hi = k + 1 == kPrSlices ? e : prs_first_ge(nbr, lo, e, bnd[k + 1]);
+      cnt[static_cast<uint64_t>(k) * (n + 1) + v] = static_cast<uint32_t>(hi - lo);
+      lo = hi;
+    }
+  }
+}
+
+__global__ void k_prs_fill(const uint64_t *off, const uint32_t *nbr, uint32_t n, const uint32_t *bnd,
+                           const uint32_t *soff, uint32_t *snbr) {
+  GRID_STRIDE(v, n) {
+    uint64_t b = off[v], e = off[v + 1];
+    uint64_t lo = b;
+    for (uint32_t k = 0; k < kPrSlices; k++) {
+      uint64_t hi = k + 1 == kPrSlices ? e : prs_first_ge(nbr, lo, e, bnd[k + 1]);
+      uint32_t dst = soff[static_cast<uint64_t>(k) * (n + 1) + v];
+      for (uint64_t j = lo; j < hi; j++) snbr[dst + (j - lo)] = nbr[j];
+      lo = hi;
+    }
+  }
+}
+
+// start of each slice's edges (slice k starts at soff[k * (n + 1)]; entry 8
+// is the total)
+__global__ void k_prs_starts(const uint32_t *soff, uint32_t n, uint64_t *out) {
+  const uint32_t k = threadIdx.x;
+  if (k <= kPrSlices) out[k] = soff[static_cast<uint64_t>(k) * (n + 1)];
+}
+
+// q = p / degree for the first iteration (p = 1/n everywhere)
+__global__ void k_prs_norm0(const uint32_t *deg, uint32_t n, double p0, double *p, double *q) {
+  GRID_STRIDE(v, n) {
+    p[v] = p0;
+    q[v] = p0 / static_cast<double>(deg[v]);
+  }
+}
+
+// Merge-path coordinates of every tile boundary: coord[base[k] + k + t] =
+// rows of slice k ended before step t * kPrsTile (t = 0 .. tiles[k], the
+// last clamped to the slice's end).  Depends on the graph only.
+__global__ void k_prs_coords(const uint32_t *soff, uint32_t n, PrsTiles pt, uint32_t *coord) {
+  const uint64_t entries = static_cast<uint64_t>(pt.base[kPrSlices - 1]) + pt.tiles[kPrSlices - 1] + kPrSlices;
+  GRID_STRIDE(g, entries) {
+    uint32_t k = 0;
+    while (k + 1 < kPrSlices && g >= static_cast<uint64_t>(pt.base[k + 1]) + k + 1) k++;
+    const uint64_t t = g - pt.base[k] - k;
+    const uint32_t *so = soff + static_cast<uint64_t>(k) * (n + 1);
+    const uint32_t base = so[0];
+    const uint64_t ek = so[n] - base;
+    const uint64_t total = static_cast<uint64_t>(n) + ek;
+    const uint64_t d = t * kPrsTile < total ? t * kPrsTile : total;
+    uint64_t lo = d > ek ? d - ek : 0, hi = d < n ? d : n;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (static_cast<uint64_t>(so[mid + 1] - base) <= d - mid - 1) lo = mid + 1;
+      else hi = mid;
+    }
+    coord[g] = static_cast<uint32_t>(lo);
+  }
+}
+
+// One merge-path tile: writes part[k * n + v] for the rows the tile ends,
+// and the tile's carry (the row still open at its end, with the sum it has
+// so far) to crow / cval.
+__global__ void __launch_bounds__(kPrsBlock) k_prs_pull(const uint32_t *soff, const uint32_t *snbr,
+                                                        const uint32_t *coord, const double *q, double *part,
+                                                        uint32_t *crow, double *cval, uint32_t n, PrsTiles pt) {
+  const uint32_t k = blockIdx.x % kPrSlices;  // workgroups go round-robin over the 8 XCDs
+  const uint32_t tile = blockIdx.x / kPrSlices;
+  if (tile >= pt.tiles[k]) return;
+  // [0, nr) row ends (u32) then the gathered ranks of the tile's edges
+  __shared__ double s_buf[kPrsTile + 2];
+  __shared__ double s_wv[kPrsBlock / 64];
+  __shared__ uint32_t s_wf[kPrsBlock / 64];
+  const uint32_t t = threadIdx.x;
+  const uint32_t *so = soff + static_cast<uint64_t>(k) * (n + 1);
+  const uint32_t base = so[0];
+  const uint32_t ek = so[n] - base;
+  const uint64_t total = static_cast<uint64_t>(n) + ek;
+  // tile start and end on the merge path (rows ended before them)
+  const uint64_t d0 = static_cast<uint64_t>(tile) * kPrsTile;
+  const uint64_t d1 = d0 + kPrsTile < total ? d0 + kPrsTile : total;
+  const uint32_t *cx = coord + pt.base[k] + k + tile;
+  const uint32_t i0 = cx[0], i1 = cx[1];
+  const uint32_t j0 = static_cast<uint32_t>(d0 - i0);
+  const uint32_t nr = i1 - i0, ne = static_cast<uint32_t>(d1 - i1) - j0;
+  uint32_t *s_end = reinterpret_cast<uint32_t *>(s_buf);
+  double *s_q = s_buf + (nr + 1) / 2;
+  // nr + ne <= kPrsTile: each thread loads at most kPrsItems row ends and
+  // kPrsItems ranks, all issued before the first is used
+  const uint32_t *nb = snbr + base + j0;
+  uint32_t idx[kPrsItems];
+  double val[kPrsItems];
+#pragma unroll
+  for (uint32_t r = 0; r < kPrsItems; r++) {
+    const uint32_t e = t + r * kPrsBlock;
+    idx[r] = e < ne ? nb[e] : 0;
+    if (e < nr) s_end[e] = so[i0 + e + 1] - base;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < kPrsItems; r++) val[r] = t + r * kPrsBlock < ne ? q[idx[r]] : 0.0;
+#pragma unroll
+  for (uint32_t r = 0; r < kPrsItems; r++)
+    if (t + r * kPrsBlock < ne) s_q[t + r * kPrsBlock] = val[r];
+  __syncthreads();
+  // this thread's start on the tile's path
+  const uint32_t steps = nr + ne;
+  const uint32_t dt = t * kPrsItems < steps ? t * kPrsItems : steps;
+  const uint32_t dend = dt + kPrsItems < steps ? dt + kPrsItems : steps;
+  uint32_t lo = dt > ne ? dt - ne : 0, hi = dt < nr ? dt : nr;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s_end[mid] <= j0 + (dt - mid - 1)) lo = mid + 1;
+    else hi = mid;
+  }
+  uint32_t i = lo, j = dt - lo;
+  double acc = 0.0;
+  // sum of the row a step ends (rows end in order: the first ended row is
+  // i_first, the next i_first + 1, ...)
+  double out[kPrsItems];
+  uint32_t ended = 0;  // bit d: step d ended a row
+  const uint32_t i_first = lo;
+#pragma unroll
+  for (uint32_t d = 0; d < kPrsItems; d++) {
+    out[d] = 0.0;
+    if (dt + d < dend) {
+      if (i < nr && s_end[i] <= j0 + j) {
+        out[d] = acc;
+        ended |= 1u << d;
+        acc = 0.0;
+        i++;
+      } else {
+        acc += s_q[j];
+        j++;
+      }
+    }
+  }
+  // Segmented inclusive scan of the open-row sums over the threads (a thread
+  // that ended a row starts a new segment): within each wave by shuffles,
+  // then across the block's waves in wave order.
+  const uint32_t lane = t & 63, w = t >> 6;
+  double v = acc;
+  uint32_t f = ended != 0;
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const double v2 = __shfl_up(v, o, 64);
+    const uint32_t f2 = __shfl_up(f, o, 64);
+    if (lane >= o && !f) {
+      v = v2 + v;
+      f = f2;
+    }
+  }
+  if (lane == 63) {
+    s_wv[w] = v;
+    s_wf[w] = f;
+  }
+  __syncthreads();  // also: every thread is done with s_end and s_q
+  double cin = 0.0;  // what the waves before this one leave open
+  for (uint32_t x = 0; x < w; x++) cin = s_wf[x] ? s_wv[x] : cin + s_wv[x];
+  const double c = f ? v : cin + v;
+  const double up = __shfl_up(c, 1, 64);
+  const double prev = lane ? up : cin;  // open sum of the threads before this one
+  // row sums through LDS (s_buf now holds row i at s_buf[i]) for a
+  // coalesced store; the first row a thread ends also gets prev
+  uint32_t row = i_first;
+  bool head = true;
+#pragma unroll
+  for (uint32_t d = 0; d < kPrsItems; d++) {
+    if (ended & (1u << d)) {
+      s_buf[row] = head && t ? prev + out[d] : out[d];
+      head = false;
+      row++;
+    }
+  }
+  __syncthreads();
+  double *po = part + static_cast<uint64_t>(k) * n + i0;
+  for (uint32_t e = t; e < nr; e += kPrsBlock) po[e] = s_buf[e];
+  if (t == kPrsBlock - 1) {
+    crow[pt.base[k] + tile] = i0 + nr;
+    cval[pt.base[k] + tile] = c;
+  }
+}
+
+// Adds each row's carries (a run of equal rows in one slice's tile order) to
+// the row's partial, in tile order.
+__global__ void k_prs_fix(const uint32_t *crow, const double *cval, double *part, uint32_t n, PrsTiles pt,
+                          uint32_t ntiles) {
+  GRID_STRIDE(g, ntiles) {
+    uint32_t k = 0;
+    while (k + 1 < kPrSlices && g >= pt.base[k + 1]) k++;
+    const uint64_t end = pt.base[k] + pt.tiles[k];
+    const uint32_t row = crow[g];
+    if (row >= n) continue;
+    if (g > pt.base[k] && crow[g - 1] == row) continue;  // not the head of its run
+    double s = 0.0;
+    for (uint64_t h = g; h < end && crow[h] == row; h++) s += cval[h];
+    part[static_cast<uint64_t>(k) * n + row] = s + part[static_cast<uint64_t>(k) * n + row];
+  }
+}
+
+// p_next = sum of the slice partials in slice order; q for the next
+// iteration = p_next / degree
+__global__ void k_prs_reduce(const double *part, const uint32_t *deg, uint32_t n, double *p, double *q) {
+  GRID_STRIDE(v, n) {
+    double s = part[v];
+    for (uint32_t k = 1; k < kPrSlices; k++) s += part[static_cast<uint64_t>(k) * n + v];
+    p[v] = s;
+    q[v] = s / static_cast<double>(deg[v]);
+  }
+}
+
 // ---------------------------------------------------------------- multi-source BFS
 // 64 BFS sources share one traversal: bit j of a vertex's mask is source j.
 // Pull: a vertex ORs its neighbours' frontier masks; push: a frontier vertex
@@ -1523,8 +1801,12 @@ uint32_t Traversal::bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t
 
 // ---------------------------------------------------------------- PR
 void Traversal::pagerank(uint32_t iters, double *out) {
-  hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
+  if (n <= kPrSlicedMaxN && g_.tune.pr_sliced) {
+    pagerank_sliced(iters, out);
+    return;
+  }
+  hipStream_t s = stream_;
   dtmp_.ensure(n, s);
   dtmp2_.ensure(n, s);
   double *p = out, *q = dtmp_.get(), *pn = dtmp2_.get();
@@ -1573,6 +1855,111 @@ void Traversal::pagerank(uint32_t iters, double *out) {
     std::swap(p, pn);
   }
   if (p != out) TG_HIP(hipMemcpyAsync(out, p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  TG_HIP(hipStreamSynchronize(s));
+}
+
+void Traversal::prepare_pr_slices() {
+  if (prs_epoch_ == g_.epoch()) return;  // unchanged graph: the slices still hold
+  hipStream_t s = stream_;
+  uint32_t n = g_.num_vertices();
+  uint64_t m = g_.num_edges();
+  if (m > 0xFFFFFFFFull) throw Error(-1, "sliced PageRank: more than 2^32 edges");
+  csr_off_.ensure(static_cast<uint64_t>(n) + 1, s);
+  csr_nbr_.ensure(m + 1, s);
+  g_.export_adjacency(csr_off_.get(), csr_nbr_.get());
+  const uint64_t nc = static_cast<uint64_t>(kPrSlices) * (n + 1);
+  prs_off_.ensure(nc + 1, s);
+  prs_nbr_.ensure(m + 1, s);
+  prs_deg_.ensure(n + 1, s);
+  prs_bnd_.ensure(kPrSlices + 1, s);
+  prs_meta_.ensure(kPrSlices + 1, s);
+  k_prs_bounds<<<1, 64, 0, s>>>(csr_off_.get(), n, prs_bnd_.get());
+  TG_CHECK_LAUNCH();
+  k_prs_counts<<<grid_for(static_cast<uint64_t>(n) + 1, 256), 256, 0, s>>>(csr_off_.get(), csr_nbr_.get(), n,
+                                                                          prs_bnd_.get(), prs_off_.get(),
+                                                                          prs_deg_.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u32(prs_off_.get(), prs_off_.get(), nc, prs_off_.get() + nc, scan_, s);
+  k_prs_fill<<<grid_for(n, 256), 256, 0, s>>>(csr_off_.get(), csr_nbr_.get(), n, prs_bnd_.get(), prs_off_.get(),
+                                              prs_nbr_.get());
+  TG_CHECK_LAUNCH();
+  k_prs_starts<<<1, 64, 0, s>>>(prs_off_.get(), n, prs_meta_.get());
+  TG_CHECK_LAUNCH();
+  uint64_t st[kPrSlices + 1];
+  read_scalars(prs_meta_.get(), st, kPrSlices + 1, s);
+  uint32_t acc = 0;
+  prs_max_tiles_ = 0;
+  for (uint32_t k = 0; k < kPrSlices; k++) {
+    const uint64_t steps = static_cast<uint64_t>(n) + (st[k + 1] - st[k]);
+    prs_tiles_.tiles[k] = static_cast<uint32_t>((steps + kPrsTile - 1) / kPrsTile);
+    prs_tiles_.base[k] = acc;
+    acc += prs_tiles_.tiles[k];
+    prs_max_tiles_ = prs_tiles_.tiles[k] > prs_max_tiles_ ? prs_tiles_.tiles[k] : prs_max_tiles_;
+  }
+  prs_ntiles_ = acc;
+  prs_crow_.ensure(acc + 1, s);
+  prs_cval_.ensure(acc + 1, s);
+  prs_coord_.ensure(acc + kPrSlices, s);
+  k_prs_coords<<<grid_for(acc + kPrSlices, 256), 256, 0, s>>>(prs_off_.get(), n, prs_tiles_, prs_coord_.get());
+  TG_CHECK_LAUNCH();
+  prs_epoch_ = g_.epoch();
+}
+
+void Traversal::pagerank_sliced(uint32_t iters, double *out) {
+  hipStream_t s = stream_;
+  uint32_t n = g_.num_vertices();
+  prepare_pr_slices();
+  dtmp_.ensure(n, s);
+  prs_part_.ensure(static_cast<uint64_t>(kPrSlices) * n, s);
+  double *q = dtmp_.get();
+  k_prs_norm0<<<grid_for(n, 256), 256, 0, s>>>(prs_deg_.get(), n, 1 / static_cast<double>(n), out, q);
+  TG_CHECK_LAUNCH();
+  const PrsTiles pt = prs_tiles_;
+  const unsigned blocks = kPrSlices * prs_max_tiles_;
+  auto iterate = [&] {
+    hipEvent_t e[4] = {};
+    if (profiling) {
+      for (auto &x : e) TG_HIP(hipEventCreate(&x));
+      TG_HIP(hipEventRecord(e[0], s));
+    }
+    k_prs_pull<<<blocks, kPrsBlock, 0, s>>>(prs_off_.get(), prs_nbr_.get(), prs_coord_.get(), q, prs_part_.get(),
+                                            prs_crow_.get(), prs_cval_.get(), n, pt);
+    TG_CHECK_LAUNCH();
+    if (profiling) TG_HIP(hipEventRecord(e[1], s));
+    k_prs_fix<<<grid_for(prs_ntiles_, 256), 256, 0, s>>>(prs_crow_.get(), prs_cval_.get(), prs_part_.get(), n, pt,
+                                                         prs_ntiles_);
+    TG_CHECK_LAUNCH();
+    if (profiling) TG_HIP(hipEventRecord(e[2], s));
+    k_prs_reduce<<<grid_for(n, 256), 256, 0, s>>>(prs_part_.get(), prs_deg_.get(), n, out, q);
+    TG_CHECK_LAUNCH();
+    if (profiling) {
+      TG_HIP(hipEventRecord(e[3], s));
+      record("prs_pull", e[0], e[1]);
+      record("prs_fix", e[1], e[2]);
+      record("prs_reduce", e[2], e[3]);
+      record("pr_iteration", e[0], e[3]);
+      for (auto &x : e) TG_HIP(hipEventDestroy(x));
+    }
+  };
+  if (g_.tune.pr_graph && !profiling && iters >= 2) {
+    // one iteration (pull, fix, reduce: kernel nodes only) captured and replayed
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    TG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    iterate();
+    TG_HIP(hipStreamEndCapture(s, &graph));
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      TG_HIP(e);
+    }
+    for (uint32_t r = 0; r < iters; r++) TG_HIP(hipGraphLaunch(exec, s));
+    TG_HIP(hipStreamSynchronize(s));
+    TG_HIP(hipGraphExecDestroy(exec));
+    TG_HIP(hipGraphDestroy(graph));
+  } else {
+    for (uint32_t r = 0; r < iters; r++) iterate();
+  }
   TG_HIP(hipStreamSynchronize(s));
 }
 

@@ -37,6 +37,14 @@ struct OpState {
   int32_t *dist = nullptr;           // BF ShortestPathLen
 };
 
+// Sliced PageRank: the adjacency in 8 slices (one per XCD), the merge-path
+// tiles of each slice and the offset of each slice's tiles.
+constexpr uint32_t kPrsSlices = 8;
+struct PrsTiles {
+  uint32_t tiles[kPrsSlices];
+  uint32_t base[kPrsSlices];
+};
+
 class Traversal {
  public:
   // s = null: use the graph's stream.  Traversals with their own streams
@@ -129,6 +137,15 @@ class Traversal {
   const uint32_t *tree_list_ = nullptr;
   uint32_t ntree_list_ = 0;
   void prepare_tree_tasks();
+  // PageRank over the XCD-sliced adjacency copy (graphs up to 6M vertices)
+  void prepare_pr_slices();
+  void pagerank_sliced(uint32_t iters, double *out);
+  uint64_t prs_epoch_ = ~0ull;
+  DevBuf<uint32_t> prs_off_, prs_nbr_, prs_deg_, prs_bnd_, prs_crow_, prs_coord_;
+  DevBuf<uint64_t> prs_meta_;
+  DevBuf<double> prs_part_, prs_cval_;
+  PrsTiles prs_tiles_{};
+  uint32_t prs_max_tiles_ = 0, prs_ntiles_ = 0;
   void pull(Frontier &in, Op op, const OpState &st, Frontier *out);
   void push(Frontier &in, Op op, const OpState &st, Frontier *out);
   uint64_t popcount_bits(const Frontier &f);
