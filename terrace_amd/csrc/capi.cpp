@@ -905,7 +905,7 @@ int tg_stats(tg_graph *G, uint64_t *out) {
     out[4] = s.promotions;
     out[5] = G->g.num_trees();
     out[6] = G->g.pma_slots();
-    out[7] = 0;
+    out[7] = s.tree_small_updates;
   });
 }
 

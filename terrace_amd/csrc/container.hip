@@ -1001,6 +1001,137 @@ __global__ void k_tree_commit(GView g, const TreeRun *tr, uint32_t ntr, const ui
   }
 }
 
+// ------------------------------------------------------------ small tree updates in place
+// A tree update of at most kTreeSmallKeys keys whose groups (the keys that
+// land in one leaf block) all fit their block -- an insert leaves it with at
+// most kBlk keys, a delete leaves at least one -- rewrites those blocks in
+// place: two launches and one read instead of the copy-on-write pass, whose
+// block and index counts need three reads.  The index keeps its blocks; only
+// a block's count and first key change.  When a group does not fit, nothing
+// is written and the copy-on-write pass runs.
+constexpr uint32_t kTreeSmallKeys = 4096;
+
+// One workgroup.  gl[0] = number of groups (0 when one does not fit), then
+// per group x: gl[1 + 3x] first key, gl[2 + 3x] end, gl[3 + 3x] index
+// position of its block (groups in any order).  *status = 1 when a group
+// does not fit, else 0.
+template <bool kInsert>
+__global__ void __launch_bounds__(1024) k_tree_small_groups(GView g, const uint64_t *keys, uint32_t k, uint32_t *gl,
+                                                            uint64_t *status) {
+  __shared__ uint64_t s_gj[kTreeSmallKeys];
+  __shared__ uint32_t s_ng, s_fail;
+  if (threadIdx.x == 0) {
+    s_ng = 0;
+    s_fail = 0;
+  }
+  for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+    const uint64_t key = keys[i];
+    const uint32_t t = static_cast<uint32_t>(g.vb[key_src(key)].aux);
+    const uint32_t j = g.tnblk[t] == 0 ? 0 : tree_block_pos(g, t, key_dst(key));
+    s_gj[i] = (static_cast<uint64_t>(t) << 32) | j;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+    if (i > 0 && s_gj[i] == s_gj[i - 1]) continue;
+    // a group longer than kBlk + 1 keys never fits
+    uint32_t e = i + 1;
+    while (e < k && e - i <= kBlk && s_gj[e] == s_gj[i]) e++;
+    const uint32_t t = static_cast<uint32_t>(s_gj[i] >> 32), j = static_cast<uint32_t>(s_gj[i]);
+    bool ok = g.tnblk[t] != 0 && e - i <= kBlk;
+    if (ok) {
+      const uint32_t c = g.bcnt[g.iblk[g.toff[t] + j]];
+      ok = kInsert ? c + (e - i) <= kBlk : c > e - i;
+    }
+    if (!ok) {
+      s_fail = 1;
+    } else {
+      const uint32_t x = atomicAdd(&s_ng, 1u);
+      gl[1 + 3 * x] = i;
+      gl[2 + 3 * x] = e;
+      gl[3 + 3 * x] = j;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    gl[0] = s_fail ? 0 : s_ng;
+    *status = s_fail;
+  }
+}
+
+// One 256-thread workgroup per group listed by k_tree_small_groups: the
+// group's block rewritten in place (old keys staged in LDS first).
+template <bool kInsert>
+__global__ void __launch_bounds__(256) k_tree_small_apply(GView g, const uint64_t *keys, const uint32_t *gl) {
+  __shared__ uint32_t s_old[kBlk];
+  __shared__ uint32_t s_wsum[4];
+  const uint32_t ng = gl[0];
+  for (uint32_t x = blockIdx.x; x < ng; x += gridDim.x) {
+    const uint32_t i0 = gl[1 + 3 * x], i1 = gl[2 + 3 * x], j = gl[3 + 3 * x];
+    const uint32_t t = static_cast<uint32_t>(g.vb[key_src(keys[i0])].aux);
+    const uint64_t io = g.toff[t] + j;
+    const uint32_t b = g.iblk[io];
+    const uint32_t c = g.bcnt[b];
+    const uint32_t cnt = i1 - i0;
+    uint32_t *blk = g.bkeys + static_cast<uint64_t>(b) * kBlk;
+    const uint32_t p = threadIdx.x;
+    if (p < c) s_old[p] = blk[p];
+    __syncthreads();
+    // position of v among the group's keys (sorted by destination)
+    auto rank_in_keys = [&](uint32_t v) {
+      uint32_t lo = i0, hi = i1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (key_dst(keys[mid]) < v) lo = mid + 1;
+        else hi = mid;
+      }
+      return lo;
+    };
+    if (kInsert) {
+      // merged order: an old key moves right by the new keys below it, a
+      // new key lands after the old keys below it (the sets are disjoint)
+      if (p < c) {
+        const uint32_t v = s_old[p];
+        blk[p + (rank_in_keys(v) - i0)] = v;
+      }
+      for (uint32_t q = p; q < cnt; q += blockDim.x) {
+        const uint32_t v = key_dst(keys[i0 + q]);
+        uint32_t lo = 0, hi = c;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_old[mid] < v) lo = mid + 1;
+          else hi = mid;
+        }
+        blk[q + lo] = v;
+      }
+      if (p == 0) {
+        g.bcnt[b] = c + cnt;
+        const uint32_t k0 = key_dst(keys[i0]);
+        if (k0 < s_old[0]) g.imin[io] = k0;
+      }
+    } else {
+      bool keep = false;
+      uint32_t v = 0;
+      if (p < c) {
+        v = s_old[p];
+        const uint32_t r = rank_in_keys(v);
+        keep = !(r < i1 && key_dst(keys[r]) == v);
+      }
+      const uint64_t m = __ballot(keep);
+      const uint32_t r = __popcll(m & lanemask_lt());
+      if ((p & 63) == 0) s_wsum[p >> 6] = __popcll(m);
+      __syncthreads();
+      uint32_t base = 0;
+      for (uint32_t w = 0; w < (p >> 6); w++) base += s_wsum[w];
+      if (keep) {
+        blk[base + r] = v;
+        if (base + r == 0) g.imin[io] = v;
+      }
+      if (p == 0) g.bcnt[b] = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------ delete batch
 // One thread per run of existing keys: remove inline hits, mark secondary
 // deletes, and record how many inline slots must be refilled.
@@ -1575,8 +1706,31 @@ void Graph::pma_delete(const uint64_t *keys, uint64_t k) {
 }
 
 // ------------------------------------------------------------ tree update
+bool Graph::tree_small(const uint64_t *keys, uint64_t k, bool insert) {
+  if (k == 0 || k > kTreeSmallKeys) return false;
+  GView g = view();
+  DevBuf<uint32_t> &gl = s32_[30];
+  gl.ensure(1 + 3 * k, stream_);
+  const uint32_t kk = static_cast<uint32_t>(k);
+  const unsigned grid = static_cast<unsigned>(k < 1024 ? k : 1024);
+  if (insert) {
+    k_tree_small_groups<true><<<1, 1024, 0, stream_>>>(g, keys, kk, gl.get(), dsc_.get() + 13);
+    TG_CHECK_LAUNCH();
+    k_tree_small_apply<true><<<grid, 256, 0, stream_>>>(g, keys, gl.get());
+  } else {
+    k_tree_small_groups<false><<<1, 1024, 0, stream_>>>(g, keys, kk, gl.get(), dsc_.get() + 13);
+    TG_CHECK_LAUNCH();
+    k_tree_small_apply<false><<<grid, 256, 0, stream_>>>(g, keys, gl.get());
+  }
+  TG_CHECK_LAUNCH();
+  if (read_scalar(dsc_.get() + 13, stream_) != 0) return false;
+  stats_.tree_small_updates++;
+  return true;
+}
+
 void Graph::tree_insert(const uint64_t *keys, uint64_t k) {
   if (k == 0) return;
+  if (tree_small(keys, k, true)) return;
   GView g = view();
   g3_.ensure(k + 1, stream_);
   u0_.ensure(k + 1, stream_);
@@ -1661,6 +1815,7 @@ void Graph::tree_insert(const uint64_t *keys, uint64_t k) {
 
 void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
   if (k == 0) return;
+  if (tree_small(keys, k, false)) return;
   GView g = view();
   g3_.ensure(k + 1, stream_);
   u0_.ensure(k + 1, stream_);

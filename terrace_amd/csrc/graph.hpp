@@ -55,6 +55,7 @@ struct Stats {
   uint64_t pma_window_leaves = 0;
   uint64_t tree_blocks_written = 0;
   uint64_t promotions = 0;
+  uint64_t tree_small_updates = 0;  // tree updates done in place
 };
 
 class Graph {
@@ -185,7 +186,8 @@ class Graph {
   DevBuf<uint32_t> kick_;
   DevBuf<uint8_t> route_;
   DevBuf<uint64_t> scal_;  // small device scalars
-  // batch scalars: [0] error bits, [1..] counts read back in one copy
+  // batch scalars: [0] error bits, [1..] counts read back in one copy,
+  // [13] status of the in-place tree update
   static constexpr int kScalars = 16;
   static constexpr int kErrSort = 14, kErrScan = 15;  // look-back error words (sort, scans/selects)
   DevBuf<uint64_t> dsc_;
@@ -213,6 +215,9 @@ class Graph {
   void pma_delete(const uint64_t *keys, uint64_t k);
   void tree_insert(const uint64_t *keys, uint64_t k);
   void tree_delete(const uint64_t *keys, uint64_t k);
+  // in-place update of up to 4096 tree keys when every touched block keeps
+  // between 1 and 256 keys; false (nothing written) otherwise
+  bool tree_small(const uint64_t *keys, uint64_t k, bool insert);
   void ensure_blocks(uint64_t need);
   void ensure_index(uint64_t need);
   void ensure_trees(uint64_t need);

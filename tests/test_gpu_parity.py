@@ -318,6 +318,71 @@ def test_skew_all_tiers_hub_targeted_batches():
     G.close()
 
 
+def test_small_tree_updates_in_place_and_fallback():
+    """Small batches into B-tree vertices take the in-place path (every
+    touched leaf block keeps 1..256 keys: block counts and first keys change,
+    the index does not); a batch that overfills a block or empties one takes
+    the copy-on-write pass.  The adjacency equals the set model after every
+    batch, both paths are seen in the counters, and every algorithm matches
+    the oracle at the end (batches are symmetric)."""
+    n = 1 << 12
+    say("small tree updates")
+    rng = np.random.default_rng(41)
+    hubs = np.array([3, 100, 2000], np.uint64)
+
+    def sym(k):
+        k = np.asarray(k, np.uint64)
+        rev = ((k & np.uint64(0xFFFFFFFF)) << np.uint64(32)) | (k >> np.uint64(32))
+        return oracle.sort_unique(np.concatenate([k, rev]))
+
+    parts = []
+    for h in hubs:
+        nb = rng.choice(np.arange(14, n, dtype=np.uint64), 2500, replace=False)
+        parts.append((h << np.uint64(32)) | nb[nb != h])
+    bg = oracle.build_keys(*oracle.rmat_edges(n, 5, 0.5, 0.1, 0.1, 20000))
+    keys = oracle.sort_unique(np.concatenate([sym(np.concatenate(parts)), bg]))
+    G = tg.Graph(n)
+    G.add_edge_batch(*split(keys))
+    assert np.all(G.tiers()[hubs.astype(np.int64)] == 2)
+    st0 = G.stats()
+    for rnd in range(24):
+        h = hubs[rnd % 3]
+        cnt = int(rng.integers(1, 120))
+        if rnd % 2 == 0:
+            nb = rng.choice(np.arange(0, n, dtype=np.uint64), cnt, replace=False)
+            ins = sym((h << np.uint64(32)) | nb[nb != h])
+            G.add_edge_batch(*split(ins))
+            keys = oracle.set_union(keys, ins)
+        else:
+            mine = keys[(keys >> np.uint64(32)) == h]
+            dels = sym(mine[rng.choice(len(mine), min(cnt, len(mine)), replace=False)])
+            G.remove_edge_batch(*split(dels))
+            keys = oracle.set_difference(keys, dels)
+        assert_adjacency(G, keys, n, f"small tree batch {rnd}")
+    st1 = G.stats()
+    assert st1["tree_small_updates"] > st0["tree_small_updates"]
+    # 300 new neighbours of one hub in one id range overfill a block
+    h = hubs[0]
+    have = set((keys[(keys >> np.uint64(32)) == h] & np.uint64(0xFFFFFFFF)).tolist())
+    fresh = np.array([v for v in range(n) if v not in have and v != h][:300], np.uint64)
+    ins = sym((h << np.uint64(32)) | fresh)
+    G.add_edge_batch(*split(ins))
+    keys = oracle.set_union(keys, ins)
+    assert_adjacency(G, keys, n, "overfilled block")
+    st2 = G.stats()
+    assert st2["tree_blocks_written"] > st1["tree_blocks_written"]
+    # deleting a hub's 400 largest neighbours empties its last block
+    mine = keys[(keys >> np.uint64(32)) == hubs[1]]
+    dels = sym(mine[-400:])
+    G.remove_edge_batch(*split(dels))
+    keys = oracle.set_difference(keys, dels)
+    assert_adjacency(G, keys, n, "emptied block")
+    assert G.stats()["tree_blocks_written"] >= st2["tree_blocks_written"]
+    g = oracle.CSR(keys, n)
+    check_algorithms(G, g, int(hubs[2]), what="small tree")
+    G.close()
+
+
 def test_cfg_cpu_small_full_size():
     """The reference's CPU-runnable workload at full size: rMAT scale 20,
     8M generated edges (a=.5 b=.1 c=.1, nominal generator, static seed
