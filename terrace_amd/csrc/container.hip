@@ -413,6 +413,128 @@ __global__ void k_clear_marks(GView g, const uint32_t *T, const uint64_t *ntp, c
   }
 }
 
+// ---- PMA inserts that fit their leaves
+// When every touched leaf can take its new keys and stay within the leaf
+// density bound (old + new <= up[0]), the window search picks each leaf as
+// its own window and the redistribution is a sorted merge inside the leaf.
+// These three kernels produce that same state directly: k_pma_leaf_fit
+// checks the bound, k_pma_leaf_reset drops begin pointers into the leaves
+// (as k_gather does) and k_pma_leaf_merge writes the merged leaf, its
+// count and fence and the run-start begin pointers (as k_fill does).  The
+// last two do nothing when a leaf does not fit; the window path then runs.
+
+// Tstart[nt] = k and a cleared fit flag (one thread)
+__global__ void k_touched_end(uint32_t *Tstart, const uint64_t *ntp, uint32_t k, uint64_t *flag) {
+  Tstart[*ntp] = k;
+  *flag = 0;
+}
+
+__global__ void k_pma_leaf_fit(GView g, uint64_t up0, const uint32_t *T, const uint32_t *Tstart, const uint64_t *ntp,
+                               uint64_t *flag) {
+  const uint32_t nt = static_cast<uint32_t>(*ntp);
+  GRID_STRIDE(t, nt) {
+    const uint64_t newc = Tstart[t + 1] - Tstart[t];
+    if (g.pcnt[T[t]] + newc > up0) atomicOr(reinterpret_cast<unsigned long long *>(flag), 1ull);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_pma_leaf_reset(GView g, const uint32_t *T, const uint64_t *ntp,
+                                                        const uint64_t *flag) {
+  if (*flag) return;
+  const uint32_t nt = static_cast<uint32_t>(*ntp);
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t t = wave; t < nt; t += nwaves) {
+    const uint64_t slot = (static_cast<uint64_t>(T[t]) << kLeafShift) + lane;
+    if (g.pdst[slot] != kEmpty) {
+      const uint32_t s = g.psrc[slot];
+      if (g.vb[s].aux == slot) g.vb[s].aux = kPmaNone;
+    }
+  }
+}
+
+// One wave per touched leaf, four leaves per workgroup pass: the leaf's
+// elements (a leaf may hold holes after deletes) are compacted by rank in
+// LDS, then merged with the leaf's new keys (both sorted and disjoint) and
+// written packed left, the rest of the leaf emptied.
+__global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t *T, const uint32_t *Tstart,
+                                                        const uint64_t *ntp, const uint64_t *keys,
+                                                        const uint64_t *flag, uint64_t *err) {
+  if (*flag) return;
+  __shared__ uint64_t s_old[4][kLeafSlots];
+  const uint32_t nt = static_cast<uint32_t>(*ntp);
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr uint64_t kNone = ~0ull;
+  for (uint64_t tb = static_cast<uint64_t>(blockIdx.x) * 4; tb < nt; tb += static_cast<uint64_t>(gridDim.x) * 4) {
+    const uint64_t t = tb + w;
+    const bool live = t < nt;
+    uint64_t leaf = 0, base = 0;
+    uint32_t kb = 0, nb = 0, c = 0;
+    if (live) {
+      leaf = T[t];
+      base = leaf << kLeafShift;
+      kb = Tstart[t];
+      nb = Tstart[t + 1] - kb;
+      const uint32_t d = g.pdst[base + lane];
+      const bool v = d != kEmpty;
+      const uint64_t m = __ballot(v);
+      c = static_cast<uint32_t>(__popcll(m));
+      if (lane == 0 && c != g.pcnt[leaf]) atomicOr(reinterpret_cast<unsigned long long *>(err), 2ull);
+      if (v) s_old[w][__popcll(m & lanemask_lt())] = make_key(g.psrc[base + lane], d);
+    }
+    __syncthreads();
+    if (live) {
+      // c + nb <= up[0] < 64: lane l holds old element l and new key l
+      const uint64_t a = lane < c ? s_old[w][lane] : kNone;
+      const uint64_t b = lane < nb ? keys[kb + lane] : kNone;
+      // new key b: old elements below it
+      uint32_t lo = 0, hi = c;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_old[w][mid] < b) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lane < c) {
+        // old element a: new keys below it
+        uint32_t l2 = 0, h2 = nb;
+        while (l2 < h2) {
+          const uint32_t mid = (l2 + h2) >> 1;
+          if (keys[kb + mid] < a) l2 = mid + 1;
+          else h2 = mid;
+        }
+        const uint64_t slot = base + lane + l2;
+        // the element before a: the larger of a[lane - 1] and the last new key below a
+        bool has_prev = lane > 0 || l2 > 0;
+        uint64_t prev = lane > 0 ? s_old[w][lane - 1] : 0;
+        if (l2 > 0 && (lane == 0 || keys[kb + l2 - 1] > prev)) prev = keys[kb + l2 - 1];
+        const uint32_t s = key_src(a);
+        g.psrc[slot] = s;
+        g.pdst[slot] = key_dst(a);
+        if (!has_prev || key_src(prev) != s)
+          atomicMin(reinterpret_cast<unsigned long long *>(&g.vb[s].aux), static_cast<unsigned long long>(slot));
+      }
+      if (lane < nb) {
+        const uint64_t slot = base + lane + lo;
+        bool has_prev = lane > 0 || lo > 0;
+        uint64_t prev = lane > 0 ? keys[kb + lane - 1] : 0;
+        if (lo > 0 && (lane == 0 || s_old[w][lo - 1] > prev)) prev = s_old[w][lo - 1];
+        const uint32_t s = key_src(b);
+        g.psrc[slot] = s;
+        g.pdst[slot] = key_dst(b);
+        if (!has_prev || key_src(prev) != s)
+          atomicMin(reinterpret_cast<unsigned long long *>(&g.vb[s].aux), static_cast<unsigned long long>(slot));
+      }
+      if (lane >= c + nb) g.pdst[base + lane] = kEmpty;
+      if (lane == 0) {
+        g.pcnt[leaf] = c + nb;
+        g.fence[leaf] = (c > 0 && a < b) ? a : b;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Tiny helpers that move device counts around without a host round trip.
 __global__ void k_put_u32(uint32_t *p, const uint64_t *idx, uint32_t v) { p[*idx] = v; }
 __global__ void k_u32_to_u64(const uint32_t *a, uint64_t *b) { *b = *a; }
@@ -1649,8 +1771,29 @@ void Graph::pma_insert(const uint64_t *keys, uint64_t k, uint64_t root_known) {
   k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get());
   TG_CHECK_LAUNCH();
   select(SelTouched{u0_.get(), T.get(), Tstart.get()}, k, ntp, scan_scratch, stream_);
-  k_put_u32<<<1, 1, 0, stream_>>>(Tstart.get(), ntp, static_cast<uint32_t>(k));
+  uint64_t *fitp = dsc_.get() + 7;
+  k_touched_end<<<1, 1, 0, stream_>>>(Tstart.get(), ntp, static_cast<uint32_t>(k), fitp);
   TG_CHECK_LAUNCH();
+  // inserts that fit their leaves: merged in place (nothing is written when
+  // a leaf does not fit)
+  {
+    const unsigned wg = grid_for(k * 64, 256, 4096);  // a wave per touched leaf (at most k)
+    k_pma_leaf_fit<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_.up[0], T.get(), Tstart.get(), ntp, fitp);
+    TG_CHECK_LAUNCH();
+    k_pma_leaf_reset<<<wg, 256, 0, stream_>>>(g, T.get(), ntp, fitp);
+    TG_CHECK_LAUNCH();
+    k_pma_leaf_merge<<<wg, 256, 0, stream_>>>(g, T.get(), Tstart.get(), ntp, keys, fitp, dsc_.get());
+    TG_CHECK_LAUNCH();
+    uint64_t h2[2];
+    read_scalars(fitp, h2, 2, stream_);  // fit flag, touched leaves
+    if (h2[0] == 0) {
+      tree_dirty_ = true;
+      stats_.pma_windows += h2[1];
+      stats_.pma_window_leaves += h2[1];
+      stats_.pma_leaf_merges++;
+      return;
+    }
+  }
   k_climb<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_, T.get(), Tstart.get(), ntp, mark_.get(), chosen.get());
   TG_CHECK_LAUNCH();
   k_cover<<<grid_for(k, 256), 256, 0, stream_>>>(g, T.get(), ntp, mark_.get(), chosen.get(), u3_.get());

@@ -56,6 +56,7 @@ struct Stats {
   uint64_t tree_blocks_written = 0;
   uint64_t promotions = 0;
   uint64_t tree_small_updates = 0;  // tree updates done in place
+  uint64_t pma_leaf_merges = 0;     // PMA inserts merged inside their leaves
 };
 
 class Graph {
@@ -187,7 +188,7 @@ class Graph {
   DevBuf<uint8_t> route_;
   DevBuf<uint64_t> scal_;  // small device scalars
   // batch scalars: [0] error bits, [1..] counts read back in one copy,
-  // [13] status of the in-place tree update
+  // [7] PMA leaf-fit flag, [13] status of the in-place tree update
   static constexpr int kScalars = 16;
   static constexpr int kErrSort = 14, kErrScan = 15;  // look-back error words (sort, scans/selects)
   DevBuf<uint64_t> dsc_;

@@ -173,10 +173,10 @@ class Graph:
         return out
 
     def stats(self) -> dict:
-        out = np.zeros(8, np.uint64)
+        out = np.zeros(9, np.uint64)
         check(lib().tg_stats(self._h, ptr(out)))
         keys = ["pma_rebuilds", "pma_windows", "pma_window_leaves", "tree_blocks_written",
-                "promotions", "trees", "pma_slots", "tree_small_updates"]
+                "promotions", "trees", "pma_slots", "tree_small_updates", "pma_leaf_merges"]
         return {k: int(v) for k, v in zip(keys, out)}
 
     def apply_ops(self, src, dst, ops) -> None:

@@ -322,7 +322,8 @@ def test_small_tree_updates_in_place_and_fallback():
     """Small batches into B-tree vertices take the in-place path (every
     touched leaf block keeps 1..256 keys: block counts and first keys change,
     the index does not); a batch that overfills a block or empties one takes
-    the copy-on-write pass.  The adjacency equals the set model after every
+    the copy-on-write pass.  Their reverse edges go to PMA leaves that take
+    them in place (in-leaf merge).  The adjacency equals the set model after every
     batch, both paths are seen in the counters, and every algorithm matches
     the oracle at the end (batches are symmetric)."""
     n = 1 << 12
@@ -361,6 +362,7 @@ def test_small_tree_updates_in_place_and_fallback():
         assert_adjacency(G, keys, n, f"small tree batch {rnd}")
     st1 = G.stats()
     assert st1["tree_small_updates"] > st0["tree_small_updates"]
+    assert st1["pma_leaf_merges"] > st0["pma_leaf_merges"]  # the reverse edges land in PMA leaves
     # 300 new neighbours of one hub in one id range overfill a block
     h = hubs[0]
     have = set((keys[(keys >> np.uint64(32)) == h] & np.uint64(0xFFFFFFFF)).tolist())
