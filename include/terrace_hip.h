@@ -301,8 +301,9 @@ int tg_profile_get(tg_graph *g, const char *name, double *ms, uint64_t *launches
 /* internal counters: [0] PMA rebuilds, [1] PMA windows, [2] window leaves,
  * [3] tree blocks written, [4] promotions, [5] trees, [6] PMA slots,
  * [7] tree updates rewritten in place (no block split or emptied),
- * [8] PMA inserts merged inside their leaves (no window redistribution) */
-int tg_stats(tg_graph *g, uint64_t *out /* [9] */);
+ * [8] PMA inserts merged inside their leaves (no window redistribution),
+ * [9] elements held in the PMA */
+int tg_stats(tg_graph *g, uint64_t *out /* [10] */);
 
 #ifdef __cplusplus
 }

@@ -907,6 +907,7 @@ int tg_stats(tg_graph *G, uint64_t *out) {
     out[6] = G->g.pma_slots();
     out[7] = s.tree_small_updates;
     out[8] = s.pma_leaf_merges;
+    out[9] = G->g.pma_elements();
   });
 }
 

@@ -161,6 +161,51 @@ __global__ void k_pack(const uint32_t *src, const uint32_t *dst, uint64_t n, uin
   }
 }
 
+// Batches of at most kSmallSortKeys pairs: pack, validate and sort in one
+// workgroup (bitonic sort over LDS, padded with ~0 to a power of two); the
+// result equals k_pack followed by the radix sort.
+constexpr uint32_t kSmallSortKeys = 4096;
+
+__global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, const uint32_t *dst, uint32_t count,
+                                                          uint32_t nv, uint64_t *keys, uint64_t *err) {
+  __shared__ uint64_t s[kSmallSortKeys];
+  const uint32_t t = threadIdx.x;
+  uint32_t P = 1;
+  while (P < count) P <<= 1;
+  bool bad = false;
+  for (uint32_t i = t; i < P; i += blockDim.x) {
+    uint64_t v = ~0ull;
+    if (i < count) {
+      const uint32_t a = src[i], b = dst[i];
+      if (a >= nv || b >= nv) {
+        bad = true;
+        v = 0;
+      } else {
+        v = make_key(a, b);
+      }
+    }
+    s[i] = v;
+  }
+  if (bad) atomicOr(reinterpret_cast<unsigned long long *>(err), 1ull);
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = t; i < P; i += blockDim.x) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = s[i], b = s[ixj];
+          if ((a > b) == ((i & k) == 0)) {
+            s[i] = b;
+            s[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = t; i < count; i += blockDim.x) keys[i] = s[i];
+}
+
 __global__ void k_flag_unique(const uint64_t *keys, uint64_t n, uint32_t *flags) {
   GRID_STRIDE(i, n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
@@ -1124,15 +1169,13 @@ __global__ void k_tree_commit(GView g, const TreeRun *tr, uint32_t ntr, const ui
 }
 
 // ------------------------------------------------------------ small tree updates in place
-// A tree update of at most kTreeSmallKeys keys whose groups (the keys that
+// A tree update of at most Graph::kTreeSmallKeys keys whose groups (the keys that
 // land in one leaf block) all fit their block -- an insert leaves it with at
 // most kBlk keys, a delete leaves at least one -- rewrites those blocks in
 // place: two launches and one read instead of the copy-on-write pass, whose
 // block and index counts need three reads.  The index keeps its blocks; only
 // a block's count and first key change.  When a group does not fit, nothing
 // is written and the copy-on-write pass runs.
-constexpr uint32_t kTreeSmallKeys = 4096;
-
 // One workgroup.  gl[0] = number of groups (0 when one does not fit), then
 // per group x: gl[1 + 3x] first key, gl[2 + 3x] end, gl[3 + 3x] index
 // position of its block (groups in any order).  *status = 1 when a group
@@ -1140,7 +1183,7 @@ constexpr uint32_t kTreeSmallKeys = 4096;
 template <bool kInsert>
 __global__ void __launch_bounds__(1024) k_tree_small_groups(GView g, const uint64_t *keys, uint32_t k, uint32_t *gl,
                                                             uint64_t *status) {
-  __shared__ uint64_t s_gj[kTreeSmallKeys];
+  __shared__ uint64_t s_gj[Graph::kTreeSmallKeys];
   __shared__ uint32_t s_ng, s_fail;
   if (threadIdx.x == 0) {
     s_ng = 0;
@@ -1662,11 +1705,16 @@ void Graph::prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_
   k0_.ensure(count, stream_);
   k1_.ensure(count + 1, stream_);
   fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
-  k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
-  TG_CHECK_LAUNCH();
-  uint32_t bits = 1;
-  while (bits < 32 && (1ull << bits) < n_) bits++;
-  radix_sort_keys(k0_.get(), count, bits, bits, sort_, stream_);
+  if (count <= kSmallSortKeys) {
+    k_pack_sort_small<<<1, 1024, 0, stream_>>>(src, dst, static_cast<uint32_t>(count), n_, k0_.get(), dsc_.get());
+    TG_CHECK_LAUNCH();
+  } else {
+    k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
+    TG_CHECK_LAUNCH();
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < n_) bits++;
+    radix_sort_keys(k0_.get(), count, bits, bits, sort_, stream_);
+  }
   select(SelUnique{k0_.get(), k1_.get()}, count, dsc_.get() + 1, scan_scratch, stream_);
 }
 
@@ -1741,59 +1789,70 @@ void Graph::pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total) {
                                                      nullptr);
   TG_CHECK_LAUNCH();
   tree_dirty_ = true;
+  pma_elems_ = total;
   TG_HIP(hipStreamSynchronize(stream_));
   stats_.pma_rebuilds++;
 }
 
-void Graph::pma_insert(const uint64_t *keys, uint64_t k, uint64_t root_known) {
+void Graph::pma_insert(const uint64_t *keys, uint64_t k) {
   if (k == 0) return;
   if (k > 0xFFFFFFFFull) throw Error(-1, "batch too large");
-  rebuild_count_tree();
-  GView g = view();
-  uint64_t nl = 1ull << pma_H_;
-  uint64_t root_old = root_known != kRootUnknown
-                          ? root_known
-                          : read_scalar(pcnt_.get() + tree_level_off(pma_H_, pma_H_), stream_);
-  uint64_t total = root_old + k;
-  if (total > bounds_.up[pma_H_]) {
-    pma_rebuild(keys, k, total);
+  if (pma_elems_ + k > bounds_.up[pma_H_]) {
+    rebuild_count_tree();
+    pma_rebuild(keys, k, pma_elems_ + k);
     return;
   }
-  // touched leaves and window selection; counts stay on the device and the
-  // kernels are sized by the upper bound k
-  DevBuf<uint32_t> &T = f1_, &Tstart = f2_, &chosen = f3_;
+  pma_fast_launch(keys, k);
+  uint64_t h2[2];
+  read_scalars(dsc_.get() + 7, h2, 2, stream_);  // fit flag, touched leaves
+  if (h2[0] == 0) pma_fast_done(k, h2[1]);
+  else pma_window_insert(keys, k);
+}
+
+// Touched leaves of the sorted keys (T, Tstart in f1_, f2_; their count in
+// dsc_[8]) and the in-leaf merge, which writes nothing unless every touched
+// leaf fits (fit flag in dsc_[7]).  Needs pma_elems_ + k within the root
+// bound.
+void Graph::pma_fast_launch(const uint64_t *keys, uint64_t k) {
+  GView g = view();
+  DevBuf<uint32_t> &T = f1_, &Tstart = f2_;
   u0_.ensure(k + 1, stream_);  // leaf of key
-  u3_.ensure(k + 1, stream_);  // cover, then task -> window map
   T.ensure(k + 1, stream_);
   Tstart.ensure(k + 1, stream_);
-  chosen.ensure(k + 1, stream_);
-  uint64_t *ntp = dsc_.get() + 8, *nwp = dsc_.get() + 9;
+  uint64_t *ntp = dsc_.get() + 8, *fitp = dsc_.get() + 7;
   k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get());
   TG_CHECK_LAUNCH();
   select(SelTouched{u0_.get(), T.get(), Tstart.get()}, k, ntp, scan_scratch, stream_);
-  uint64_t *fitp = dsc_.get() + 7;
   k_touched_end<<<1, 1, 0, stream_>>>(Tstart.get(), ntp, static_cast<uint32_t>(k), fitp);
   TG_CHECK_LAUNCH();
-  // inserts that fit their leaves: merged in place (nothing is written when
-  // a leaf does not fit)
-  {
-    const unsigned wg = grid_for(k * 64, 256, 4096);  // a wave per touched leaf (at most k)
-    k_pma_leaf_fit<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_.up[0], T.get(), Tstart.get(), ntp, fitp);
-    TG_CHECK_LAUNCH();
-    k_pma_leaf_reset<<<wg, 256, 0, stream_>>>(g, T.get(), ntp, fitp);
-    TG_CHECK_LAUNCH();
-    k_pma_leaf_merge<<<wg, 256, 0, stream_>>>(g, T.get(), Tstart.get(), ntp, keys, fitp, dsc_.get());
-    TG_CHECK_LAUNCH();
-    uint64_t h2[2];
-    read_scalars(fitp, h2, 2, stream_);  // fit flag, touched leaves
-    if (h2[0] == 0) {
-      tree_dirty_ = true;
-      stats_.pma_windows += h2[1];
-      stats_.pma_window_leaves += h2[1];
-      stats_.pma_leaf_merges++;
-      return;
-    }
-  }
+  const unsigned wg = grid_for(k * 64, 256, 4096);  // a wave per touched leaf (at most k)
+  k_pma_leaf_fit<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_.up[0], T.get(), Tstart.get(), ntp, fitp);
+  TG_CHECK_LAUNCH();
+  k_pma_leaf_reset<<<wg, 256, 0, stream_>>>(g, T.get(), ntp, fitp);
+  TG_CHECK_LAUNCH();
+  k_pma_leaf_merge<<<wg, 256, 0, stream_>>>(g, T.get(), Tstart.get(), ntp, keys, fitp, dsc_.get());
+  TG_CHECK_LAUNCH();
+}
+
+void Graph::pma_fast_done(uint64_t k, uint64_t nt) {
+  tree_dirty_ = true;
+  pma_elems_ += k;
+  stats_.pma_windows += nt;
+  stats_.pma_window_leaves += nt;
+  stats_.pma_leaf_merges++;
+}
+
+// The window pass over the touched leaves pma_fast_launch found (a leaf did
+// not fit): density windows, gather and redistribution.
+void Graph::pma_window_insert(const uint64_t *keys, uint64_t k) {
+  rebuild_count_tree();
+  GView g = view();
+  uint64_t nl = 1ull << pma_H_;
+  const uint64_t root_old = pma_elems_;
+  DevBuf<uint32_t> &T = f1_, &Tstart = f2_, &chosen = f3_;
+  u3_.ensure(k + 1, stream_);  // cover, then task -> window map
+  chosen.ensure(k + 1, stream_);
+  uint64_t *ntp = dsc_.get() + 8, *nwp = dsc_.get() + 9;
   k_climb<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_, T.get(), Tstart.get(), ntp, mark_.get(), chosen.get());
   TG_CHECK_LAUNCH();
   k_cover<<<grid_for(k, 256), 256, 0, stream_>>>(g, T.get(), ntp, mark_.get(), chosen.get(), u3_.get());
@@ -1833,6 +1892,7 @@ void Graph::pma_insert(const uint64_t *keys, uint64_t k, uint64_t root_known) {
                                                           keys, tw);
   TG_CHECK_LAUNCH();
   tree_dirty_ = true;
+  pma_elems_ += k;
   stats_.pma_windows += nw;
   stats_.pma_window_leaves += ntasks;
 }
@@ -1846,11 +1906,12 @@ void Graph::pma_delete(const uint64_t *keys, uint64_t k) {
   tree_dirty_ = true;
   k_pma_fix_begin<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g2_.get());
   TG_CHECK_LAUNCH();
+  pma_elems_ -= k;  // every key is present (a missing one raises an error flag)
 }
 
 // ------------------------------------------------------------ tree update
-bool Graph::tree_small(const uint64_t *keys, uint64_t k, bool insert) {
-  if (k == 0 || k > kTreeSmallKeys) return false;
+// Launch the in-place tree update (status in dsc_[13]: 0 = done).
+void Graph::tree_small_launch(const uint64_t *keys, uint64_t k, bool insert) {
   GView g = view();
   DevBuf<uint32_t> &gl = s32_[30];
   gl.ensure(1 + 3 * k, stream_);
@@ -1866,6 +1927,11 @@ bool Graph::tree_small(const uint64_t *keys, uint64_t k, bool insert) {
     k_tree_small_apply<false><<<grid, 256, 0, stream_>>>(g, keys, gl.get());
   }
   TG_CHECK_LAUNCH();
+}
+
+bool Graph::tree_small(const uint64_t *keys, uint64_t k, bool insert) {
+  if (k == 0 || k > kTreeSmallKeys) return false;
+  tree_small_launch(keys, k, insert);
   if (read_scalar(dsc_.get() + 13, stream_) != 0) return false;
   stats_.tree_small_updates++;
   return true;
@@ -1874,6 +1940,11 @@ bool Graph::tree_small(const uint64_t *keys, uint64_t k, bool insert) {
 void Graph::tree_insert(const uint64_t *keys, uint64_t k) {
   if (k == 0) return;
   if (tree_small(keys, k, true)) return;
+  tree_insert_cow(keys, k);
+}
+
+// The copy-on-write tree insert: groups, new blocks, rebuilt index segments.
+void Graph::tree_insert_cow(const uint64_t *keys, uint64_t k) {
   GView g = view();
   g3_.ensure(k + 1, stream_);
   u0_.ensure(k + 1, stream_);
@@ -2135,15 +2206,10 @@ void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_);
   k_u32_to_u64<<<1, 1, 0, stream_>>>(ppos.get() + nruns, dsc_.get() + 5);
   TG_CHECK_LAUNCH();
-  // the PMA's element count (count tree root) comes back with the counts
-  rebuild_count_tree();
-  k_u32_to_u64<<<1, 1, 0, stream_>>>(pcnt_.get() + tree_level_off(pma_H_, pma_H_), dsc_.get() + 6);
-  TG_CHECK_LAUNCH();
-  uint64_t h[4];
-  read_scalars(dsc_.get() + 3, h, 4, stream_);
+  uint64_t h[3];
+  read_scalars(dsc_.get() + 3, h, 3, stream_);
   uint64_t npma = h[0], ntree = h[1];
   uint32_t np = static_cast<uint32_t>(h[2]);
-  uint64_t pma_root = h[3];
   // promotions: new trees for vertices routed to the B-tier the first time
   uint64_t next = 0;
   DevBuf<uint64_t> &ext = s64_[0];
@@ -2185,8 +2251,24 @@ void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
     while (bits < 32 && (1ull << bits) < n_) bits++;
     radix_sort_keys(g0_.get(), ntree, bits, bits, sort_, stream_);
   }
-  tree_insert(g0_.get(), ntree);
-  pma_insert(k3_.get(), npma, np ? kRootUnknown : pma_root);  // promotions changed the PMA
+  // the in-place tree update and the in-leaf PMA merge (disjoint data) are
+  // launched together and their outcomes read back at once; a part that
+  // did not fit then takes its general pass
+  const bool tree_try = ntree > 0 && ntree <= kTreeSmallKeys;
+  const bool pma_try = npma > 0 && pma_elems_ + npma <= bounds_.up[pma_H_];
+  if (tree_try) tree_small_launch(g0_.get(), ntree, true);
+  if (pma_try) pma_fast_launch(k3_.get(), npma);
+  uint64_t st[7] = {1, 0, 0, 0, 0, 0, 1};
+  if (tree_try || pma_try) read_scalars(dsc_.get() + 7, st, 7, stream_);  // [0] fit, [1] leaves, [6] tree
+  if (ntree) {
+    if (tree_try && st[6] == 0) stats_.tree_small_updates++;
+    else tree_insert_cow(g0_.get(), ntree);
+  }
+  if (npma) {
+    if (pma_try && st[0] == 0) pma_fast_done(npma, st[1]);
+    else if (pma_try) pma_window_insert(k3_.get(), npma);
+    else pma_insert(k3_.get(), npma);  // beyond the root bound: rebuild
+  }
   check_batch_flags();
 }
 
@@ -2510,6 +2592,7 @@ void Graph::copy_from(Graph &src, bool query_only) {
   cp(tvtx_.get(), src.tvtx_.get(), src.ntrees_ * 4ull);
   blk_top_ = src.blk_top_;
   idx_top_ = src.idx_top_;
+  pma_elems_ = src.pma_elems_;
   ntrees_ = src.ntrees_;
   lists_dirty_ = true;
   ++epoch_;

@@ -107,6 +107,7 @@ class Graph {
   uint64_t memory_bytes() const;
   uint32_t num_trees() const { return ntrees_; }
   uint64_t pma_slots() const { return pma_N_; }
+  uint64_t pma_elements() const { return pma_elems_; }
   const Stats &stats() const { return stats_; }
 
   // Versioned snapshot: make this graph a read-only copy of src's current
@@ -124,6 +125,9 @@ class Graph {
     uint32_t ntree = 0;
   };
   const TierLists &tier_lists();
+
+  // Largest tree update rewritten in place (see tree_small).
+  static constexpr uint64_t kTreeSmallKeys = 4096;
 
   // Mutation counter: changes whenever the adjacency may have changed.
   uint64_t epoch() const { return epoch_; }
@@ -159,6 +163,7 @@ class Graph {
   DevBuf<uint8_t> mark_;
   Bounds bounds_{};
   bool tree_dirty_ = false;  // count tree levels above the leaves are stale
+  uint64_t pma_elems_ = 0;   // elements in the PMA (the count tree's root, kept on the host)
   bool read_only_ = false;   // query-only snapshot
   // B-tree tier
   DevBuf<uint32_t> bkeys_, bcnt_;
@@ -208,9 +213,12 @@ class Graph {
   void delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
   void check_batch_flags();
-  static constexpr uint64_t kRootUnknown = ~0ull;
-  // root_known: the PMA's element count when the caller has just read it
-  void pma_insert(const uint64_t *keys, uint64_t k, uint64_t root_known = kRootUnknown);
+  // PMA inserts: pma_insert = in-leaf merge when every touched leaf fits,
+  // else the window pass (or a rebuild beyond the root bound)
+  void pma_insert(const uint64_t *keys, uint64_t k);
+  void pma_fast_launch(const uint64_t *keys, uint64_t k);
+  void pma_fast_done(uint64_t k, uint64_t nt);
+  void pma_window_insert(const uint64_t *keys, uint64_t k);
   void pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total);
   void rebuild_count_tree();
   void pma_delete(const uint64_t *keys, uint64_t k);
@@ -219,6 +227,8 @@ class Graph {
   // in-place update of up to 4096 tree keys when every touched block keeps
   // between 1 and 256 keys; false (nothing written) otherwise
   bool tree_small(const uint64_t *keys, uint64_t k, bool insert);
+  void tree_small_launch(const uint64_t *keys, uint64_t k, bool insert);
+  void tree_insert_cow(const uint64_t *keys, uint64_t k);
   void ensure_blocks(uint64_t need);
   void ensure_index(uint64_t need);
   void ensure_trees(uint64_t need);

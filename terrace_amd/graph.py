@@ -173,10 +173,11 @@ class Graph:
         return out
 
     def stats(self) -> dict:
-        out = np.zeros(9, np.uint64)
+        out = np.zeros(10, np.uint64)
         check(lib().tg_stats(self._h, ptr(out)))
         keys = ["pma_rebuilds", "pma_windows", "pma_window_leaves", "tree_blocks_written",
-                "promotions", "trees", "pma_slots", "tree_small_updates", "pma_leaf_merges"]
+                "promotions", "trees", "pma_slots", "tree_small_updates", "pma_leaf_merges",
+                "pma_elements"]
         return {k: int(v) for k, v in zip(keys, out)}
 
     def apply_ops(self, src, dst, ops) -> None:

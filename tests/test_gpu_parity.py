@@ -42,6 +42,7 @@ def assert_adjacency(G, keys, n, what=""):
     assert np.array_equal(nbr, ref.nbr[:ref.m]), f"{what}: neighbour lists differ"
     assert np.array_equal(G.degrees(), ref.degrees()), what
     assert G.get_num_edges() == ref.m
+    assert G.stats()["pma_elements"] == G.tier_counts()["pma_edges"], what  # host-kept PMA count
 
 
 def check_algorithms(G, g, src, iters=10, what=""):
