@@ -1722,6 +1722,11 @@ void Graph::prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_
 void Graph::check_batch_flags() {
   uint64_t h[kScalars];
   read_scalars(dsc_.get(), h, kScalars, stream_);
+  check_flags(h);
+}
+
+// The same checks on batch scalars already read back.
+void Graph::check_flags(const uint64_t *h) {
   uint64_t f = h[0];
   if (f) fill_u64(dsc_.get(), 0, 1, stream_);  // leave the flags clear for the next batch
   if (h[kErrScan] | h[kErrSort]) {
@@ -2030,6 +2035,11 @@ void Graph::tree_insert_cow(const uint64_t *keys, uint64_t k) {
 void Graph::tree_delete(const uint64_t *keys, uint64_t k) {
   if (k == 0) return;
   if (tree_small(keys, k, false)) return;
+  tree_delete_cow(keys, k);
+}
+
+// The copy-on-write tree delete.
+void Graph::tree_delete_cow(const uint64_t *keys, uint64_t k) {
   GView g = view();
   g3_.ensure(k + 1, stream_);
   u0_.ensure(k + 1, stream_);
@@ -2258,15 +2268,21 @@ void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   const bool pma_try = npma > 0 && pma_elems_ + npma <= bounds_.up[pma_H_];
   if (tree_try) tree_small_launch(g0_.get(), ntree, true);
   if (pma_try) pma_fast_launch(k3_.get(), npma);
-  uint64_t st[7] = {1, 0, 0, 0, 0, 0, 1};
-  if (tree_try || pma_try) read_scalars(dsc_.get() + 7, st, 7, stream_);  // [0] fit, [1] leaves, [6] tree
-  if (ntree) {
-    if (tree_try && st[6] == 0) stats_.tree_small_updates++;
-    else tree_insert_cow(g0_.get(), ntree);
+  // one read: the batch's error flags with the fit flag ([7]), touched
+  // leaves ([8]) and the tree update status ([13])
+  uint64_t hs[kScalars];
+  read_scalars(dsc_.get(), hs, kScalars, stream_);
+  const bool tree_ok = ntree == 0 || (tree_try && hs[13] == 0);
+  const bool pma_ok = npma == 0 || (pma_try && hs[7] == 0);
+  if (ntree && tree_ok) stats_.tree_small_updates++;
+  if (npma && pma_ok) pma_fast_done(npma, hs[8]);
+  if (tree_ok && pma_ok) {
+    check_flags(hs);
+    return;
   }
-  if (npma) {
-    if (pma_try && st[0] == 0) pma_fast_done(npma, st[1]);
-    else if (pma_try) pma_window_insert(k3_.get(), npma);
+  if (!tree_ok) tree_insert_cow(g0_.get(), ntree);
+  if (!pma_ok) {
+    if (pma_try) pma_window_insert(k3_.get(), npma);
     else pma_insert(k3_.get(), npma);  // beyond the root bound: rebuild
   }
   check_batch_flags();
@@ -2345,27 +2361,71 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   read_scalars(dsc_.get() + 4, h, 3, stream_);
   uint64_t npma = h[0], ntree = h[1], nb = h[2];
   pma_delete(k3_.get(), npma);
-  tree_delete(g0_.get(), ntree);
-  // refill inline slots from the secondary tier
+  // the in-place tree delete is launched without waiting for its status:
+  // the status comes back with the refill counts (or the final flags), and
+  // the refill is redone after the copy-on-write pass when it did not fit
+  const bool tree_try = ntree > 0 && ntree <= kTreeSmallKeys;
+  if (tree_try) tree_small_launch(g0_.get(), ntree, false);
+  else tree_delete(g0_.get(), ntree);
+  bool tree_pending = tree_try;  // status not read yet
+  uint64_t hs[kScalars];
+  // refill inline slots from the secondary tier (positional writes: running
+  // it again after a fallback gives the same result)
   if (nb) {
-    DevBuf<uint64_t> &bkeys = s64_[1];
+    DevBuf<uint64_t> &bkeys = s64_[1], &bpma = s64_[2], &btree = s64_[3];
     DevBuf<uint8_t> &broute = s8_[6];
     bkeys.ensure(nb + 1, stream_);
     broute.ensure(nb + 1, stream_);
-    g = view();
-    k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, run_bump.get(), bump_off.get(),
-                                                      bkeys.get(), broute.get());
-    TG_CHECK_LAUNCH();
-    k3_.ensure(nb + 1, stream_);
-    g0_.ensure(nb + 1, stream_);
-    select(SelRoute{bkeys.get(), broute.get(), kRoutePma, k3_.get()}, nb, dsc_.get() + 4, scan_scratch, stream_);
-    select(SelRoute{bkeys.get(), broute.get(), kRouteTree, g0_.get()}, nb, dsc_.get() + 5, scan_scratch, stream_);
-    read_scalars(dsc_.get() + 4, h, 2, stream_);
-    pma_delete(k3_.get(), h[0]);
-    tree_delete(g0_.get(), h[1]);
+    bpma.ensure(nb + 1, stream_);
+    btree.ensure(nb + 1, stream_);
+    for (int pass = 0; pass < 2; pass++) {
+      g = view();
+      k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, run_bump.get(), bump_off.get(),
+                                                        bkeys.get(), broute.get());
+      TG_CHECK_LAUNCH();
+      select(SelRoute{bkeys.get(), broute.get(), kRoutePma, bpma.get()}, nb, dsc_.get() + 4, scan_scratch, stream_);
+      select(SelRoute{bkeys.get(), broute.get(), kRouteTree, btree.get()}, nb, dsc_.get() + 5, scan_scratch,
+             stream_);
+      read_scalars(dsc_.get(), hs, kScalars, stream_);
+      if (!tree_pending) break;
+      tree_pending = false;
+      if (hs[13] == 0) {
+        stats_.tree_small_updates++;
+        break;
+      }
+      tree_delete_cow(g0_.get(), ntree);  // then the refill again
+    }
+    pma_delete(bpma.get(), hs[4]);
+    const uint64_t nbt = hs[5];
+    const bool bt_try = nbt > 0 && nbt <= kTreeSmallKeys;
+    if (bt_try) tree_small_launch(btree.get(), nbt, false);
+    else if (nbt) tree_delete(btree.get(), nbt);
+    read_scalars(dsc_.get(), hs, kScalars, stream_);  // the batch's flags (and the tree status)
+    if (bt_try) {
+      if (hs[13] == 0) {
+        stats_.tree_small_updates++;
+      } else {
+        tree_delete_cow(btree.get(), nbt);
+        read_scalars(dsc_.get(), hs, kScalars, stream_);
+      }
+    }
+  } else {
+    read_scalars(dsc_.get(), hs, kScalars, stream_);
+    if (tree_pending) {
+      if (hs[13] == 0) {
+        stats_.tree_small_updates++;
+      } else {
+        tree_delete_cow(g0_.get(), ntree);
+        read_scalars(dsc_.get(), hs, kScalars, stream_);
+      }
+    }
   }
-  if (ws_) ws_->erase(R, nR, stream_);  // R holds exactly the deleted (existing) keys
-  check_batch_flags();
+  if (ws_) {
+    ws_->erase(R, nR, stream_);  // R holds exactly the deleted (existing) keys
+    check_batch_flags();
+  } else {
+    check_flags(hs);
+  }
 }
 
 // ------------------------------------------------------------ coalesced ops

@@ -213,6 +213,7 @@ class Graph {
   void delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
   void check_batch_flags();
+  void check_flags(const uint64_t *h);
   // PMA inserts: pma_insert = in-leaf merge when every touched leaf fits,
   // else the window pass (or a rebuild beyond the root bound)
   void pma_insert(const uint64_t *keys, uint64_t k);
@@ -229,6 +230,7 @@ class Graph {
   bool tree_small(const uint64_t *keys, uint64_t k, bool insert);
   void tree_small_launch(const uint64_t *keys, uint64_t k, bool insert);
   void tree_insert_cow(const uint64_t *keys, uint64_t k);
+  void tree_delete_cow(const uint64_t *keys, uint64_t k);
   void ensure_blocks(uint64_t need);
   void ensure_index(uint64_t need);
   void ensure_trees(uint64_t need);

@@ -381,6 +381,13 @@ def test_small_tree_updates_in_place_and_fallback():
     keys = oracle.set_difference(keys, dels)
     assert_adjacency(G, keys, n, "emptied block")
     assert G.stats()["tree_blocks_written"] >= st2["tree_blocks_written"]
+    # inline slots refilled from a tree whose in-place delete did not fit
+    # (an emptied block): the refill is redone after the copy-on-write pass
+    mine = keys[(keys >> np.uint64(32)) == hubs[2]]
+    dels = sym(np.concatenate([mine[:13], mine[-400:]]))
+    G.remove_edge_batch(*split(dels))
+    keys = oracle.set_difference(keys, dels)
+    assert_adjacency(G, keys, n, "refill after fallback")
     g = oracle.CSR(keys, n)
     check_algorithms(G, g, int(hubs[2]), what="small tree")
     G.close()
