@@ -798,34 +798,99 @@ __device__ inline uint64_t prs_first_ge(const uint32_t *nbr, uint64_t b, uint64_
   return lo;
 }
 
-__global__ void k_prs_counts(const uint64_t *off, const uint32_t *nbr, uint32_t n, const uint32_t *bnd, uint32_t *cnt,
-                             uint32_t *deg) {
+__global__ void k_prs_counts(const uint64_t *off, const uint32_t *nbr, uint32_t n, const uint32_t *bnd,
+                             uint32_t *cnt, uint32_t *deg) {
   GRID_STRIDE(v, static_cast<uint64_t>(n) + 1) {
     if (v == n) {
       for (uint32_t k = 0; k < kPrSlices; k++) cnt[static_cast<uint64_t>(k) * (n + 1) + n] = 0;
       continue;
     }
-    uint64_t b = off[v], e = off[v + 1];
+    const uint64_t b = off[v], e = off[v + 1];
     deg[v] = static_cast<uint32_t>(e - b);
     uint64_t lo = b;
     for (uint32_t k = 0; k < kPrSlices; k++) {
-      uint64_t hi = k + 1 == kPrSlices ? e : prs_first_ge(nbr, lo, e, bnd[k + 1]);
+      const uint64_t hi = k + 1 == kPrSlices ? e : prs_first_ge(nbr, lo, e, bnd[k + 1]);
       cnt[static_cast<uint64_t>(k) * (n + 1) + v] = static_cast<uint32_t>(hi - lo);
       lo = hi;
     }
   }
 }
 
-__global__ void k_prs_fill(const uint64_t *off, const uint32_t *nbr, uint32_t n, const uint32_t *bnd,
-                           const uint32_t *soff, uint32_t *snbr) {
-  GRID_STRIDE(v, n) {
-    uint64_t b = off[v], e = off[v + 1];
-    uint64_t lo = b;
-    for (uint32_t k = 0; k < kPrSlices; k++) {
-      uint64_t hi = k + 1 == kPrSlices ? e : prs_first_ge(nbr, lo, e, bnd[k + 1]);
-      uint32_t dst = soff[static_cast<uint64_t>(k) * (n + 1) + v];
-      for (uint64_t j = lo; j < hi; j++) snbr[dst + (j - lo)] = nbr[j];
-      lo = hi;
+// The slice-major copy is a stable partition of the CSR neighbour array by
+// slice (within a slice, CSR order is the (vertex, neighbour) order the
+// copy needs), done edge-parallel in tiles of kPrsFillTile edges: per-tile
+// slice counts, one scan over [slice][tile], then a scatter that ranks each
+// edge among the tile's edges of its slice with wave ballots.
+constexpr uint32_t kPrsFillTile = 4096;
+
+struct PrsBounds {
+  uint32_t b[kPrSlices + 1];
+};
+
+__device__ inline uint32_t prs_slice_of(const PrsBounds &bd, uint32_t id) {
+  uint32_t k = 0;
+#pragma unroll
+  for (uint32_t i = 1; i < kPrSlices; i++) k += id >= bd.b[i] ? 1u : 0u;
+  return k;
+}
+
+__global__ void __launch_bounds__(256) k_prs_tile_counts(const uint32_t *nbr, uint64_t m, const uint32_t *bnd,
+                                                         uint32_t ntiles, uint32_t *tc) {
+  __shared__ uint32_t s_c[kPrSlices];
+  PrsBounds bd;
+  for (uint32_t i = 0; i <= kPrSlices; i++) bd.b[i] = bnd[i];
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    if (threadIdx.x < kPrSlices) s_c[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t c[kPrSlices] = {};
+    const uint64_t j0 = static_cast<uint64_t>(tile) * kPrsFillTile;
+    for (uint32_t r = threadIdx.x; r < kPrsFillTile; r += blockDim.x) {
+      const uint64_t j = j0 + r;
+      if (j < m) c[prs_slice_of(bd, nbr[j])]++;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPrSlices; k++)
+      if (c[k]) atomicAdd(&s_c[k], c[k]);
+    __syncthreads();
+    if (threadIdx.x < kPrSlices) tc[static_cast<uint64_t>(threadIdx.x) * ntiles + tile] = s_c[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// tb = exclusive scan of tc over [slice][tile]: where the tile's edges of
+// each slice start in snbr.
+__global__ void __launch_bounds__(256) k_prs_scatter(const uint32_t *nbr, uint64_t m, const uint32_t *bnd,
+                                                     uint32_t ntiles, const uint32_t *tb, uint32_t *snbr) {
+  __shared__ uint32_t s_w[4][kPrSlices];
+  __shared__ uint32_t s_run[kPrSlices];
+  PrsBounds bd;
+  for (uint32_t i = 0; i <= kPrSlices; i++) bd.b[i] = bnd[i];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    if (threadIdx.x < kPrSlices) s_run[threadIdx.x] = tb[static_cast<uint64_t>(threadIdx.x) * ntiles + tile];
+    const uint64_t j0 = static_cast<uint64_t>(tile) * kPrsFillTile;
+    for (uint32_t r = 0; r < kPrsFillTile; r += 256) {
+      const uint64_t j = j0 + r + threadIdx.x;
+      const bool live = j < m;
+      const uint32_t v = live ? nbr[j] : 0;
+      const uint32_t k = live ? prs_slice_of(bd, v) : kPrSlices;
+      uint32_t rank = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < kPrSlices; q++) {
+        const uint64_t mq = __ballot(k == q);
+        if (k == q) rank = __popcll(mq & lanemask_lt());
+        if (lane == 0) s_w[w][q] = __popcll(mq);
+      }
+      __syncthreads();
+      if (live) {
+        uint32_t base = s_run[k];
+        for (uint32_t x = 0; x < w; x++) base += s_w[x][k];
+        snbr[base + rank] = v;
+      }
+      __syncthreads();
+      if (threadIdx.x < kPrSlices)
+        s_run[threadIdx.x] += s_w[0][threadIdx.x] + s_w[1][threadIdx.x] + s_w[2][threadIdx.x] + s_w[3][threadIdx.x];
+      __syncthreads();
     }
   }
 }
@@ -1880,8 +1945,17 @@ void Traversal::prepare_pr_slices() {
                                                                           prs_deg_.get());
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(prs_off_.get(), prs_off_.get(), nc, prs_off_.get() + nc, scan_, s);
-  k_prs_fill<<<grid_for(n, 256), 256, 0, s>>>(csr_off_.get(), csr_nbr_.get(), n, prs_bnd_.get(), prs_off_.get(),
-                                              prs_nbr_.get());
+  const uint32_t ntl = static_cast<uint32_t>((m + kPrsFillTile - 1) / kPrsFillTile);
+  if (ntl) {
+    const uint64_t ntc = static_cast<uint64_t>(kPrSlices) * ntl;
+    prs_tc_.ensure(ntc + 1, s);
+    k_prs_tile_counts<<<grid_for(ntl, 1, 1u << 16), 256, 0, s>>>(csr_nbr_.get(), m, prs_bnd_.get(), ntl,
+                                                                 prs_tc_.get());
+    TG_CHECK_LAUNCH();
+    exclusive_scan_u32(prs_tc_.get(), prs_tc_.get(), ntc, prs_tc_.get() + ntc, scan_, s);
+    k_prs_scatter<<<grid_for(ntl, 1, 1u << 16), 256, 0, s>>>(csr_nbr_.get(), m, prs_bnd_.get(), ntl, prs_tc_.get(),
+                                                             prs_nbr_.get());
+  }
   TG_CHECK_LAUNCH();
   k_prs_starts<<<1, 64, 0, s>>>(prs_off_.get(), n, prs_meta_.get());
   TG_CHECK_LAUNCH();

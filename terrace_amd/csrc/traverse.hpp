@@ -141,7 +141,7 @@ class Traversal {
   void prepare_pr_slices();
   void pagerank_sliced(uint32_t iters, double *out);
   uint64_t prs_epoch_ = ~0ull;
-  DevBuf<uint32_t> prs_off_, prs_nbr_, prs_deg_, prs_bnd_, prs_crow_, prs_coord_;
+  DevBuf<uint32_t> prs_off_, prs_nbr_, prs_deg_, prs_bnd_, prs_crow_, prs_coord_, prs_tc_;
   DevBuf<uint64_t> prs_meta_;
   DevBuf<double> prs_part_, prs_cval_;
   PrsTiles prs_tiles_{};
