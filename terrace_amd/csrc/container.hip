@@ -206,10 +206,6 @@ __global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, c
   for (uint32_t i = t; i < count; i += blockDim.x) keys[i] = s[i];
 }
 
-__global__ void k_flag_unique(const uint64_t *keys, uint64_t n, uint32_t *flags) {
-  GRID_STRIDE(i, n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
-}
-
 __global__ void k_scatter_flagged(const uint64_t *in, const uint32_t *flags, const uint32_t *pos, uint64_t n,
                                   uint64_t *out) {
   GRID_STRIDE(i, n) if (flags[i]) out[pos[i]] = in[i];
@@ -246,21 +242,6 @@ __global__ void k_exists2_n(GView g, const uint64_t *keys, const uint64_t *np, u
 __global__ void k_exists_n(GView g, const uint64_t *keys, const uint64_t *np, uint64_t cap, uint32_t *ex) {
   const uint64_t n = *np;
   GRID_STRIDE(i, cap) ex[i] = (i < n && edge_exists(g, key_src(keys[i]), key_dst(keys[i]))) ? 1u : 0u;
-}
-
-__global__ void k_not(const uint32_t *a, uint64_t n, uint32_t *out) {
-  GRID_STRIDE(i, n) out[i] = a[i] ? 0u : 1u;
-}
-
-// run boundaries over sorted keys (by src)
-__global__ void k_flag_runs(const uint64_t *keys, uint64_t n, uint32_t *flags) {
-  GRID_STRIDE(i, n) flags[i] = (i == 0 || key_src(keys[i]) != key_src(keys[i - 1])) ? 1u : 0u;
-}
-
-__global__ void k_run_starts(const uint32_t *flags, const uint32_t *pos, uint64_t n, uint32_t *run_beg,
-                             uint32_t nruns) {
-  GRID_STRIDE(i, n) if (flags[i]) run_beg[pos[i]] = static_cast<uint32_t>(i);
-  if (blockIdx.x == 0 && threadIdx.x == 0) run_beg[nruns] = static_cast<uint32_t>(n);
 }
 
 // ------------------------------------------------------------ inline merge
@@ -377,30 +358,9 @@ __global__ void k_place_kicked_old(const uint64_t *R, const uint32_t *nidx, cons
   }
 }
 
-__global__ void k_run_of(const uint32_t *flags, const uint32_t *pos, uint64_t n, uint32_t *run_of) {
-  GRID_STRIDE(i, n) run_of[i] = pos[i] + flags[i] - 1;
-}
-
-__global__ void k_flag_route(const uint8_t *kroute, uint64_t n, uint8_t want, uint32_t *flags) {
-  GRID_STRIDE(i, n) flags[i] = kroute[i] == want ? 1u : 0u;
-}
-
 // ------------------------------------------------------------ PMA kernels
 __global__ void k_leaf_of(GView g, const uint64_t *keys, uint64_t k, uint32_t *leaf) {
   GRID_STRIDE(i, k) leaf[i] = static_cast<uint32_t>(pma_find_leaf(g, keys[i]));
-}
-
-__global__ void k_flag_change_u32(const uint32_t *a, uint64_t n, uint32_t *flags) {
-  GRID_STRIDE(i, n) flags[i] = (i == 0 || a[i] != a[i - 1]) ? 1u : 0u;
-}
-
-__global__ void k_touched(const uint32_t *leaf, const uint32_t *flags, const uint32_t *pos, uint64_t k,
-                          uint32_t *T, uint32_t *Tstart, uint32_t nt) {
-  GRID_STRIDE(i, k) if (flags[i]) {
-    T[pos[i]] = leaf[i];
-    Tstart[pos[i]] = static_cast<uint32_t>(i);
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) Tstart[nt] = static_cast<uint32_t>(k);
 }
 
 __global__ void k_climb(GView g, Bounds bd, const uint32_t *T, const uint32_t *Tstart, const uint64_t *ntp,
@@ -2428,20 +2388,7 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   }
 }
 
-// ------------------------------------------------------------ coalesced ops
-__global__ void k_last_op_flags(const uint64_t *keys, const uint32_t *ops, uint64_t n, uint32_t *fins,
-                                uint32_t *fdel) {
-  GRID_STRIDE(i, n) {
-    bool last = (i + 1 == n) || keys[i] != keys[i + 1];
-    fins[i] = (last && ops[i] == 0) ? 1u : 0u;
-    fdel[i] = (last && ops[i] != 0) ? 1u : 0u;
-  }
-}
-
-// A coalesced run of update batches: ops[i] = 0 insert / 1 delete, in
-// arrival order.  The stable sort keeps arrival order inside each key, so
-// the last op of a key decides, which equals applying the batches one by
-// one (both ops are idempotent set updates).
+// ------------------------------------------------------------ weights
 void Graph::enable_weights() {
   if (ws_) return;
   if (num_edges() != 0) throw Error(-1, "weights must be enabled on an empty graph");
@@ -2489,6 +2436,11 @@ void Graph::edge_weights(const uint32_t *src, const uint32_t *dst, uint64_t coun
   ws_->lookup(src, dst, count, out, stream_);
 }
 
+// ------------------------------------------------------------ coalesced ops
+// A coalesced run of update batches: ops[i] = 0 insert / 1 delete, in
+// arrival order.  The stable sort keeps arrival order inside each key, so
+// the last op of a key decides, which equals applying the batches one by
+// one (both ops are idempotent set updates).
 void Graph::apply_ops(const uint32_t *src, const uint32_t *dst, const uint32_t *ops, uint64_t count) {
   if (count == 0) return;
   if (ws_) throw Error(-1, "weighted graph: coalesced ops carry no weights");
