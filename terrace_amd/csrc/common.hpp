@@ -63,7 +63,6 @@ inline unsigned grid_for(uint64_t n, unsigned block, uint64_t cap = 1u << 20) {
 }
 
 // ---- wave64 helpers ------------------------------------------------------
-__device__ inline uint32_t lane_id() { return __lane_id(); }
 __device__ inline uint64_t lanemask_lt() {
   uint32_t l = __lane_id();
   return l == 0 ? 0ull : ((~0ull) >> (64 - l));

@@ -132,16 +132,6 @@ __device__ inline uint64_t lower_bound_dev(const T *a, uint64_t n, T x) {
   }
   return lo;
 }
-template <typename T>
-__device__ inline uint64_t upper_bound_dev(const T *a, uint64_t n, T x) {
-  uint64_t lo = 0, hi = n;
-  while (lo < hi) {
-    uint64_t mid = (lo + hi) >> 1;
-    if (a[mid] <= x) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
 
 #define GRID_STRIDE(i, n)                                                                  \
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < (n); \

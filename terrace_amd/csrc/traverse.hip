@@ -29,18 +29,6 @@ __device__ inline bool bit_set(uint32_t *bits, uint32_t v) {
   return (atomicOr(&bits[v >> 5], m) & m) == 0;
 }
 
-// Wave-aggregated queue append (lanes with pred write v).
-__device__ inline void queue_push(uint32_t *q, uint32_t *cnt, bool pred, uint32_t v) {
-  uint64_t m = __ballot(pred);
-  if (m == 0) return;
-  uint32_t lane = __lane_id();
-  int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
-  uint32_t base = 0;
-  if (static_cast<int>(lane) == leader) base = atomicAdd(cnt, static_cast<uint32_t>(__popcll(m)));
-  base = __shfl(base, leader, 64);
-  if (pred) q[base + __popcll(m & lanemask_lt())] = v;
-}
-
 template <typename T>
 __device__ inline T wave_min(T v) {
 #pragma unroll
