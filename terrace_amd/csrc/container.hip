@@ -38,12 +38,20 @@ __device__ inline uint64_t pma_find_leaf(const GView &g, uint64_t key) {
   return lo == 0 ? 0 : lo - 1;
 }
 
-// Slot of (s,d) in the PMA, or ~0 when absent.
+// Slot of (s,d) in the PMA, or ~0 when absent: the leaf's destinations
+// are read 16 bytes (4 slots) at a time, the source only where the
+// destination matches.
 __device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d) {
-  uint64_t f = pma_find_leaf(g, make_key(s, d));
-  uint64_t base = f << kLeafShift;
-  for (uint32_t i = 0; i < kLeafSlots; i++) {
-    if (g.pdst[base + i] == d && g.psrc[base + i] == s) return base + i;
+  const uint64_t base = pma_find_leaf(g, make_key(s, d)) << kLeafShift;
+  const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + base);
+#pragma unroll 4
+  for (uint32_t i = 0; i < kLeafSlots / 4; i++) {
+    const uint4 v = pd[i];
+    const uint64_t p = base + 4 * i;
+    if (v.x == d && g.psrc[p] == s) return p;
+    if (v.y == d && g.psrc[p + 1] == s) return p + 1;
+    if (v.z == d && g.psrc[p + 2] == s) return p + 2;
+    if (v.w == d && g.psrc[p + 3] == s) return p + 3;
   }
   return ~0ull;
 }
