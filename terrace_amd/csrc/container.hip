@@ -420,11 +420,11 @@ __global__ void k_clear_marks(GView g, const uint32_t *T, const uint64_t *ntp, c
 // When every touched leaf can take its new keys and stay within the leaf
 // density bound (old + new <= up[0]), the window search picks each leaf as
 // its own window and the redistribution is a sorted merge inside the leaf.
-// These three kernels produce that same state directly: k_pma_leaf_fit
-// checks the bound, k_pma_leaf_reset drops begin pointers into the leaves
-// (as k_gather does) and k_pma_leaf_merge writes the merged leaf, its
-// count and fence and the run-start begin pointers (as k_fill does).  The
-// last two do nothing when a leaf does not fit; the window path then runs.
+// Two kernels produce that same state directly: k_pma_leaf_fit checks the
+// bound, and k_pma_leaf_merge writes the merged leaf, its count and fence
+// and the run-start begin pointers (as k_gather and k_fill do between
+// them).  The merge does nothing when a leaf does not fit; the window path
+// then runs.
 
 // Tstart[nt] = k and a cleared fit flag (one thread)
 __global__ void k_touched_end(uint32_t *Tstart, const uint64_t *ntp, uint32_t k, uint64_t *flag) {
@@ -441,20 +441,16 @@ __global__ void k_pma_leaf_fit(GView g, uint64_t up0, const uint32_t *T, const u
   }
 }
 
-__global__ void __launch_bounds__(256) k_pma_leaf_reset(GView g, const uint32_t *T, const uint64_t *ntp,
-                                                        const uint64_t *flag) {
-  if (*flag) return;
-  const uint32_t nt = static_cast<uint32_t>(*ntp);
-  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
-  const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
-  const uint32_t lane = threadIdx.x & 63;
-  for (uint64_t t = wave; t < nt; t += nwaves) {
-    const uint64_t slot = (static_cast<uint64_t>(T[t]) << kLeafShift) + lane;
-    if (g.pdst[slot] != kEmpty) {
-      const uint32_t s = g.psrc[slot];
-      if (g.vb[s].aux == slot) g.vb[s].aux = kPmaNone;
-    }
-  }
+// Begin pointer of vertex s whose first element in the leaf at base now
+// sits at slot.  A pointer into this leaf (the run started here) moves with
+// a CAS, which fails only when an earlier leaf's wave has already lowered
+// it to a new earlier element; any other pointer takes the minimum.  Every
+// interleaving leaves the slot of the vertex's first element.
+__device__ inline void leaf_begin(const GView &g, uint32_t s, uint64_t base, uint64_t slot) {
+  unsigned long long *aux = reinterpret_cast<unsigned long long *>(&g.vb[s].aux);
+  const unsigned long long a = __hip_atomic_load(aux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (a >= base && a < base + kLeafSlots) atomicCAS(aux, a, static_cast<unsigned long long>(slot));
+  else atomicMin(aux, static_cast<unsigned long long>(slot));
 }
 
 // One wave per touched leaf, four leaves per workgroup pass: the leaf's
@@ -514,8 +510,7 @@ __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t 
         const uint32_t s = key_src(a);
         g.psrc[slot] = s;
         g.pdst[slot] = key_dst(a);
-        if (!has_prev || key_src(prev) != s)
-          atomicMin(reinterpret_cast<unsigned long long *>(&g.vb[s].aux), static_cast<unsigned long long>(slot));
+        if (!has_prev || key_src(prev) != s) leaf_begin(g, s, base, slot);
       }
       if (lane < nb) {
         const uint64_t slot = base + lane + lo;
@@ -525,8 +520,7 @@ __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t 
         const uint32_t s = key_src(b);
         g.psrc[slot] = s;
         g.pdst[slot] = key_dst(b);
-        if (!has_prev || key_src(prev) != s)
-          atomicMin(reinterpret_cast<unsigned long long *>(&g.vb[s].aux), static_cast<unsigned long long>(slot));
+        if (!has_prev || key_src(prev) != s) leaf_begin(g, s, base, slot);
       }
       if (lane >= c + nb) g.pdst[base + lane] = kEmpty;
       if (lane == 0) {
@@ -1790,8 +1784,6 @@ void Graph::pma_fast_launch(const uint64_t *keys, uint64_t k) {
   TG_CHECK_LAUNCH();
   const unsigned wg = grid_for(k * 64, 256, 4096);  // a wave per touched leaf (at most k)
   k_pma_leaf_fit<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_.up[0], T.get(), Tstart.get(), ntp, fitp);
-  TG_CHECK_LAUNCH();
-  k_pma_leaf_reset<<<wg, 256, 0, stream_>>>(g, T.get(), ntp, fitp);
   TG_CHECK_LAUNCH();
   k_pma_leaf_merge<<<wg, 256, 0, stream_>>>(g, T.get(), Tstart.get(), ntp, keys, fitp, dsc_.get());
   TG_CHECK_LAUNCH();
