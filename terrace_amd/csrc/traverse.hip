@@ -337,19 +337,12 @@ __global__ void __launch_bounds__(512) k_pull_pma(GView g, Op op, const uint32_t
 // parts != null: the task's partial goes to parts[t] and k_tree_parts adds
 // the partials of each vertex in leaf order (deterministic FP64 sums).
 template <class Op>
-__global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_t *list, uint32_t cnt,
+__global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_t *list, const uint32_t *task_li,
                                                    const uint64_t *task_off, uint64_t ntasks,
                                                    typename Op::Acc *parts) {
   __shared__ typename Op::Acc part[4];
   for (uint64_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
-    // vertex of task t: last list index with task_off <= t
-    uint32_t lo = 0, hi = cnt;
-    while (lo < hi) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (task_off[mid] <= t) lo = mid + 1;
-      else hi = mid;
-    }
-    uint32_t li = lo - 1;
+    const uint32_t li = task_li[t];  // list index of the task's vertex
     uint32_t v = list[li];
     if (op.skip(v)) continue;  // uniform across the block
     uint32_t tr = static_cast<uint32_t>(g.vb[v].aux);
@@ -580,19 +573,24 @@ __global__ void __launch_bounds__(256) k_push_pma(GView g, Op op, PushOut o, con
   out.flush();
 }
 
-// One workgroup per frontier B-tree vertex, 256 ids (one leaf block) per step.
+// Frontier B-tree vertices, 256 ids (one leaf block) per workgroup step.
+// Up to kPushTreeSpread of them: every workgroup takes a stride of each
+// vertex's blocks (a hub's blocks spread over the grid); more: one
+// workgroup per vertex.  The count is read on the device.
+constexpr uint32_t kPushTreeSpread = 64;
 template <class Op>
 __global__ void __launch_bounds__(256) k_push_tree(GView g, Op op, PushOut o, const uint32_t *list,
                                                    const uint32_t *cnt_ptr) {
   __shared__ uint32_t lq[4][kWQ];
   const uint32_t cnt = *cnt_ptr;
   WaveQueue out{lq[threadIdx.x >> 6], o.queue, o.count, 0};
-  for (uint64_t w = blockIdx.x; w < cnt; w += gridDim.x) {
+  const bool spread = cnt <= kPushTreeSpread;
+  for (uint64_t w = spread ? 0 : blockIdx.x; w < cnt; w += spread ? 1 : gridDim.x) {
     uint32_t s = list[w];
     uint32_t tr = static_cast<uint32_t>(g.vb[s].aux);
     uint64_t off = g.toff[tr];
     uint32_t nb = g.tnblk[tr];
-    for (uint32_t j = 0; j < nb; j++) {
+    for (uint32_t j = spread ? blockIdx.x : 0; j < nb; j += spread ? gridDim.x : 1) {
       uint32_t blk = g.iblk[off + j];
       uint32_t c = g.bcnt[blk];
       bool valid = threadIdx.x < c;
@@ -674,6 +672,14 @@ __global__ void k_bc_final(double *dep, const double *inv, uint32_t n) {
 
 __global__ void k_tree_task_counts(GView g, const uint32_t *list, uint32_t cnt, uint64_t *out) {
   GRID_STRIDE(i, cnt) out[i] = g.tnblk[static_cast<uint32_t>(g.vb[list[i]].aux)];
+}
+
+// task -> list index of its vertex (tasks of vertex i are task_off[i] ..
+// task_off[i + 1] - 1)
+__global__ void k_tree_task_map(const uint64_t *task_off, uint32_t cnt, uint32_t *task_li) {
+  GRID_STRIDE(i, cnt) {
+    for (uint64_t t = task_off[i]; t < task_off[i + 1]; t++) task_li[t] = static_cast<uint32_t>(i);
+  }
 }
 
 // TC over an exported CSR: every edge (i, v) with v > i counts
@@ -1627,6 +1633,11 @@ void Traversal::prepare_tree_tasks() {
   }
   exclusive_scan_u64(ttask_off_.get(), ttask_off_.get(), L.ntree, ttask_off_.get() + L.ntree, scan_, s);
   ntree_tasks_ = read_scalar(ttask_off_.get() + L.ntree, s);
+  ttask_li_.ensure(ntree_tasks_ + 1, s);
+  if (L.ntree) {
+    k_tree_task_map<<<grid_for(L.ntree, 256), 256, 0, s>>>(ttask_off_.get(), L.ntree, ttask_li_.get());
+    TG_CHECK_LAUNCH();
+  }
 }
 
 template <class Op>
@@ -1656,8 +1667,8 @@ static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierL
       pb.ensure(ntasks * sizeof(typename Op::Acc), s);
       parts = reinterpret_cast<typename Op::Acc *>(pb.get());
     }
-    k_pull_tree<Op><<<grid_for(ntasks * 256, 256, 1u << 20), 256, 0, s>>>(v, op, tree_list, ntree, task_off, ntasks,
-                                                                          parts);
+    k_pull_tree<Op><<<grid_for(ntasks * 256, 256, 1u << 20), 256, 0, s>>>(v, op, tree_list, T.tree_task_map(),
+                                                                          task_off, ntasks, parts);
     TG_CHECK_LAUNCH();
     if (parts) {
       k_tree_parts<Op><<<grid_for(ntree, 256), 256, 0, s>>>(op, tree_list, ntree, task_off, parts);
@@ -1765,8 +1776,9 @@ static void launch_push(Traversal &T, Graph &g, const Op &op, Frontier &in, Fron
     k_push_pma<Op, 16><<<grid_for(static_cast<uint64_t>(fc) * 16, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
   }
   TG_CHECK_LAUNCH();
-  k_push_tree<Op><<<grid_for(static_cast<uint64_t>(fc) * 256, 256, 1u << 16), 256, 0, s>>>(v, op, o, st.get(),
-                                                                                         cnt.get() + 1);
+  // at least 1024 workgroups, so a few frontier hubs spread their blocks
+  const unsigned tg = grid_for(static_cast<uint64_t>(fc) * 256, 256, 1u << 16);
+  k_push_tree<Op><<<tg < 1024 ? 1024 : tg, 256, 0, s>>>(v, op, o, st.get(), cnt.get() + 1);
   TG_CHECK_LAUNCH();
   if (T.profiling) {
     TG_HIP(hipEventRecord(e1, s));
