@@ -1868,8 +1868,12 @@ uint32_t Traversal::bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t
 void Traversal::pagerank(uint32_t iters, double *out) {
   uint32_t n = g_.num_vertices();
   if (n <= kPrSlicedMaxN && g_.tune.pr_sliced) {
-    pagerank_sliced(iters, out);
-    return;
+    // the sliced copy indexes edges with 32 bits; denser graphs take the tiered pull
+    const uint64_t m = prs_epoch_ == g_.epoch() ? prs_m_ : g_.num_edges();
+    if (m <= 0xFFFFFFFFull) {
+      pagerank_sliced(iters, out, m);
+      return;
+    }
   }
   hipStream_t s = stream_;
   dtmp_.ensure(n, s);
@@ -1923,12 +1927,12 @@ void Traversal::pagerank(uint32_t iters, double *out) {
   TG_HIP(hipStreamSynchronize(s));
 }
 
-void Traversal::prepare_pr_slices() {
+void Traversal::prepare_pr_slices(uint64_t m) {
   if (prs_epoch_ == g_.epoch()) return;  // unchanged graph: the slices still hold
   hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
-  uint64_t m = g_.num_edges();
   if (m > 0xFFFFFFFFull) throw Error(-1, "sliced PageRank: more than 2^32 edges");
+  prs_m_ = m;
   csr_off_.ensure(static_cast<uint64_t>(n) + 1, s);
   csr_nbr_.ensure(m + 1, s);
   g_.export_adjacency(csr_off_.get(), csr_nbr_.get());
@@ -1979,10 +1983,10 @@ void Traversal::prepare_pr_slices() {
   prs_epoch_ = g_.epoch();
 }
 
-void Traversal::pagerank_sliced(uint32_t iters, double *out) {
+void Traversal::pagerank_sliced(uint32_t iters, double *out, uint64_t m) {
   hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
-  prepare_pr_slices();
+  prepare_pr_slices(m);
   dtmp_.ensure(n, s);
   prs_part_.ensure(static_cast<uint64_t>(kPrSlices) * n, s);
   double *q = dtmp_.get();

@@ -140,9 +140,10 @@ class Traversal {
   uint32_t ntree_list_ = 0;
   void prepare_tree_tasks();
   // PageRank over the XCD-sliced adjacency copy (graphs up to 6M vertices)
-  void prepare_pr_slices();
-  void pagerank_sliced(uint32_t iters, double *out);
-  uint64_t prs_epoch_ = ~0ull;
+  // m: the graph's edge count (at most 2^32 - 1)
+  void prepare_pr_slices(uint64_t m);
+  void pagerank_sliced(uint32_t iters, double *out, uint64_t m);
+  uint64_t prs_epoch_ = ~0ull, prs_m_ = 0;
   DevBuf<uint32_t> prs_off_, prs_nbr_, prs_deg_, prs_bnd_, prs_crow_, prs_coord_, prs_tc_;
   DevBuf<uint64_t> prs_meta_;
   DevBuf<double> prs_part_, prs_cval_;
