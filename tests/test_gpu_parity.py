@@ -150,6 +150,64 @@ def test_mixed_updates_match_set_model(seed, skew):
     G.close()
 
 
+def test_random_small_batches_all_tiers():
+    """40 symmetric batches of log-uniform size (1..3000 keys, inserts and
+    deletes of existing edges, a third of them aimed at hubs) on a graph
+    with all three tiers: the in-place tree and in-leaf PMA paths and their
+    fallbacks in random mixes.  The adjacency equals the set model after
+    every batch; a second graph given the same batches coalesced through
+    apply_ops ends equal; every algorithm matches the oracle."""
+    n = 1 << 14
+    say("random small batches")
+    rng = np.random.default_rng(77)
+
+    def sym(k):
+        k = np.asarray(k, np.uint64)
+        rev = ((k & np.uint64(0xFFFFFFFF)) << np.uint64(32)) | (k >> np.uint64(32))
+        k = oracle.sort_unique(np.concatenate([k, rev]))
+        return k[(k >> np.uint64(32)) != (k & np.uint64(0xFFFFFFFF))]
+
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 17, 0.57, 0.19, 0.19, 300000))
+    G, H = tg.Graph(n), tg.Graph(n)
+    G.add_edge_batch(*split(keys))
+    H.add_edge_batch(*split(keys))
+    tiers = G.tiers()
+    assert (tiers == 1).any() and (tiers == 2).any()
+    hubs = np.nonzero(tiers == 2)[0].astype(np.uint64)
+    ops_s, ops_d, ops_o = [], [], []
+    st0 = G.stats()
+    for rnd in range(40):
+        size = int(np.exp(rng.uniform(0, np.log(1500))))
+        if rnd % 3 == 0:
+            src = hubs[rng.integers(0, len(hubs), size)]
+        else:
+            src = rng.integers(0, n, size).astype(np.uint64)
+        if rnd % 2 == 0:
+            batch = sym((src << np.uint64(32)) | rng.integers(0, n, size).astype(np.uint64))
+            G.add_edge_batch(*split(batch))
+            keys = oracle.set_union(keys, batch)
+            op = 0
+        else:
+            batch = sym(keys[rng.choice(len(keys), min(size, len(keys)), replace=False)])
+            G.remove_edge_batch(*split(batch))
+            keys = oracle.set_difference(keys, batch)
+            op = 1
+        bs, bd = split(batch)
+        ops_s.append(bs)
+        ops_d.append(bd)
+        ops_o.append(np.full(len(batch), op, np.uint32))
+        assert_adjacency(G, keys, n, f"random batch {rnd} ({len(batch)} keys, op {op})")
+    st1 = G.stats()
+    assert st1["tree_small_updates"] > st0["tree_small_updates"]
+    assert st1["pma_leaf_merges"] > st0["pma_leaf_merges"]
+    H.apply_ops(np.concatenate(ops_s), np.concatenate(ops_d), np.concatenate(ops_o))
+    assert_adjacency(H, keys, n, "coalesced replay")
+    g = oracle.CSR(keys, n)
+    check_algorithms(G, g, int(hubs[0]), what="random small batches")
+    G.close()
+    H.close()
+
+
 def test_delete_everything_then_reinsert():
     n = 1 << 11
     say("delete-all")
