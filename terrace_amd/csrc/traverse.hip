@@ -333,37 +333,37 @@ __global__ void __launch_bounds__(512) k_pull_pma(GView g, Op op, const uint32_t
   }
 }
 
-// One workgroup per (tree vertex, leaf block) task.
+// One wave per (tree vertex, leaf block) task; the task's vertex and block
+// come from the per-version task table.  Lane l reads keys l, l + 64, ...
+// of the block (coalesced), then one wave reduction.
 // parts != null: the task's partial goes to parts[t] and k_tree_parts adds
 // the partials of each vertex in leaf order (deterministic FP64 sums).
 template <class Op>
-__global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_t *list, const uint32_t *task_li,
-                                                   const uint64_t *task_off, uint64_t ntasks,
-                                                   typename Op::Acc *parts) {
-  __shared__ typename Op::Acc part[4];
-  for (uint64_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
-    const uint32_t li = task_li[t];  // list index of the task's vertex
-    uint32_t v = list[li];
-    if (op.skip(v)) continue;  // uniform across the block
-    uint32_t tr = static_cast<uint32_t>(g.vb[v].aux);
-    uint64_t j = t - task_off[li];
-    uint32_t blk = g.iblk[g.toff[tr] + j];
-    uint32_t c = g.bcnt[blk];
+__global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_t *task_v, const uint32_t *task_b,
+                                                   uint64_t ntasks, typename Op::Acc *parts) {
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t t = wave; t < ntasks; t += nwaves) {
+    const uint32_t v = task_v[t];
+    if (op.skip(v)) continue;  // uniform across the wave
+    const uint32_t blk = task_b[t];
+    const uint32_t c = g.bcnt[blk];
+    const uint32_t *keys = g.bkeys + static_cast<uint64_t>(blk) * kBlk;
     typename Op::Acc a = Op::ident();
-    if (threadIdx.x < c) {
-      uint32_t d = g.bkeys[static_cast<uint64_t>(blk) * kBlk + threadIdx.x];
-      if (op.member(d)) a = op.value_e(v, d);
+#pragma unroll
+    for (uint32_t r = 0; r < kBlk / 64; r++) {
+      const uint32_t e = lane + 64 * r;
+      if (e < c) {
+        const uint32_t d = keys[e];
+        if (op.member(d)) a = Op::combine(a, op.value_e(v, d));
+      }
     }
     a = Op::wave_reduce(a);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = a;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      typename Op::Acc r = part[0];
-      for (int w = 1; w < 4; w++) r = Op::combine(r, part[w]);
-      if (parts) parts[t] = r;
-      else op.commit(v, r, kShared);
+    if (lane == 0) {
+      if (parts) parts[t] = a;
+      else op.commit(v, a, kShared);
     }
-    __syncthreads();
   }
 }
 
@@ -674,11 +674,17 @@ __global__ void k_tree_task_counts(GView g, const uint32_t *list, uint32_t cnt, 
   GRID_STRIDE(i, cnt) out[i] = g.tnblk[static_cast<uint32_t>(g.vb[list[i]].aux)];
 }
 
-// task -> list index of its vertex (tasks of vertex i are task_off[i] ..
-// task_off[i + 1] - 1)
-__global__ void k_tree_task_map(const uint64_t *task_off, uint32_t cnt, uint32_t *task_li) {
+// task table: the vertex and the leaf block of every task (the tasks of
+// listed vertex i are task_off[i] .. task_off[i + 1] - 1, in leaf order)
+__global__ void k_tree_task_map(GView g, const uint32_t *list, const uint64_t *task_off, uint32_t cnt,
+                                uint32_t *task_v, uint32_t *task_b) {
   GRID_STRIDE(i, cnt) {
-    for (uint64_t t = task_off[i]; t < task_off[i + 1]; t++) task_li[t] = static_cast<uint32_t>(i);
+    const uint32_t v = list[i];
+    const uint64_t off = g.toff[static_cast<uint32_t>(g.vb[v].aux)];
+    for (uint64_t t = task_off[i]; t < task_off[i + 1]; t++) {
+      task_v[t] = v;
+      task_b[t] = g.iblk[off + (t - task_off[i])];
+    }
   }
 }
 
@@ -1633,9 +1639,11 @@ void Traversal::prepare_tree_tasks() {
   }
   exclusive_scan_u64(ttask_off_.get(), ttask_off_.get(), L.ntree, ttask_off_.get() + L.ntree, scan_, s);
   ntree_tasks_ = read_scalar(ttask_off_.get() + L.ntree, s);
-  ttask_li_.ensure(ntree_tasks_ + 1, s);
+  ttask_v_.ensure(ntree_tasks_ + 1, s);
+  ttask_b_.ensure(ntree_tasks_ + 1, s);
   if (L.ntree) {
-    k_tree_task_map<<<grid_for(L.ntree, 256), 256, 0, s>>>(ttask_off_.get(), L.ntree, ttask_li_.get());
+    k_tree_task_map<<<grid_for(L.ntree, 256), 256, 0, s>>>(g_.view(), L.tree, ttask_off_.get(), L.ntree,
+                                                           ttask_v_.get(), ttask_b_.get());
     TG_CHECK_LAUNCH();
   }
 }
@@ -1667,8 +1675,8 @@ static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierL
       pb.ensure(ntasks * sizeof(typename Op::Acc), s);
       parts = reinterpret_cast<typename Op::Acc *>(pb.get());
     }
-    k_pull_tree<Op><<<grid_for(ntasks * 256, 256, 1u << 20), 256, 0, s>>>(v, op, tree_list, T.tree_task_map(),
-                                                                          task_off, ntasks, parts);
+    k_pull_tree<Op><<<grid_for(ntasks * 64, 256, 1u << 20), 256, 0, s>>>(v, op, T.tree_task_vertex(),
+                                                                         T.tree_task_block(), ntasks, parts);
     TG_CHECK_LAUNCH();
     if (parts) {
       k_tree_parts<Op><<<grid_for(ntree, 256), 256, 0, s>>>(op, tree_list, ntree, task_off, parts);

@@ -113,7 +113,8 @@ class Traversal {
 
   uint64_t last_edges_visited = 0;  // Σ degree over processed frontiers
   DevBuf<uint8_t> &tree_parts() { return tree_parts_; }
-  const uint32_t *tree_task_map() const { return ttask_li_.get(); }
+  const uint32_t *tree_task_vertex() const { return ttask_v_.get(); }
+  const uint32_t *tree_task_block() const { return ttask_b_.get(); }
 
  private:
   Graph &g_;
@@ -122,7 +123,7 @@ class Traversal {
   DevBuf<uint32_t> cnt_;        // device counters
   DevBuf<uint32_t> sec_pma_, sec_tree_;  // frontier vertices with secondary runs
   DevBuf<uint64_t> ttask_off_;  // tree task prefix over listed tree vertices
-  DevBuf<uint32_t> ttask_li_;   // tree task -> list index of its vertex
+  DevBuf<uint32_t> ttask_v_, ttask_b_;  // tree task -> its vertex, its leaf block
   DevBuf<double> dtmp_, dtmp2_;
   DevBuf<uint32_t> utmp_;
   DevBuf<uint8_t> btmp_;
