@@ -141,11 +141,14 @@ class Graph {
   // pr_graph: PageRank iterations replayed from a captured HIP graph.
   // pr_sliced: PageRank over the XCD-sliced adjacency copy when the graph
   // has at most 6M vertices (else the tiered pull).
+  // push_wave: push frontiers up to this size give each PMA vertex a wave
+  // (64 lanes), larger ones a 16-lane group.
   struct Tuning {
     uint32_t thr_bfs = 20, thr_ms = 20, thr_cc = 5;
     uint32_t pull_block = 256;
     uint32_t pr_graph = 1;
     uint32_t pr_sliced = 1;
+    uint32_t push_wave = 16384;
   };
   Tuning tune;
 

@@ -520,7 +520,8 @@ __global__ void __launch_bounds__(256) k_push_inline(GView g, Op op, PushOut o, 
 
 // Four frontier vertices per wave (16-lane groups, 4 slots per lane); the
 // groups step in lockstep so the wave's output buffer stays uniform.
-constexpr uint32_t kPushWaveFrontier = 16384;  // frontiers up to this size: a wave per PMA vertex
+// Graph::Tuning::push_wave: frontiers up to this size push a PMA vertex
+// with a whole wave, larger ones with a 16-lane group.
 
 // GS lanes per frontier PMA vertex (16 or 64), 4 slots per lane per step.
 // Large frontiers use groups of 16 (four vertices per wave); small ones,
@@ -1778,7 +1779,7 @@ static void launch_push(Traversal &T, Graph &g, const Op &op, Frontier &in, Fron
   uint32_t fc = static_cast<uint32_t>(in.count);
   k_push_inline<Op><<<grid_for(fc, 256), 256, 0, s>>>(v, op, o, in.queue.get(), fc, sp.get(), st.get(), cnt.get());
   TG_CHECK_LAUNCH();
-  if (fc <= kPushWaveFrontier) {
+  if (fc <= g.tune.push_wave) {
     k_push_pma<Op, 64><<<grid_for(static_cast<uint64_t>(fc) * 64, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
   } else {
     k_push_pma<Op, 16><<<grid_for(static_cast<uint64_t>(fc) * 16, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
@@ -2443,6 +2444,24 @@ void Traversal::autotune(double *report) {
   auto rb = sweep(T.thr_bfs, [&] {
     for (uint32_t v : srcs) bfs(v, par.get(), nullptr);
   });
+  // PMA push mapping: the frontier size up to which a vertex gets a wave
+  {
+    double best = best_of([&] {
+      for (uint32_t v : srcs) bfs(v, par.get(), nullptr);
+    });
+    uint32_t bw = T.push_wave;
+    for (uint32_t pw : {4096u, 65536u}) {
+      T.push_wave = pw;
+      double x = best_of([&] {
+        for (uint32_t v : srcs) bfs(v, par.get(), nullptr);
+      });
+      if (x < best * 0.97) {
+        best = x;
+        bw = pw;
+      }
+    }
+    T.push_wave = bw;
+  }
   auto rm = sweep(T.thr_ms, [&] { bfs_multi(ms.data(), 64, lev.get()); });
   auto rc = sweep(T.thr_cc, [&] { cc(ids.get()); });
   double out[12] = {static_cast<double>(T.thr_bfs), static_cast<double>(T.thr_ms), static_cast<double>(T.thr_cc),

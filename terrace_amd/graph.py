@@ -199,10 +199,10 @@ class Graph:
             out[k] = int(out[k])
         return out
 
-    TUNING_KEYS = ("thr_bfs", "thr_ms", "thr_cc", "pull_block", "pr_graph", "pr_sliced")
+    TUNING_KEYS = ("thr_bfs", "thr_ms", "thr_cc", "pull_block", "pr_graph", "pr_sliced", "push_wave")
 
     def get_tuning(self) -> dict:
-        t = np.zeros(6, np.uint32)
+        t = np.zeros(len(self.TUNING_KEYS), np.uint32)
         check(lib().tg_get_tuning(self._h, ptr(t)))
         return {k: int(v) for k, v in zip(self.TUNING_KEYS, t)}
 

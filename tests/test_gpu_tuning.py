@@ -39,12 +39,15 @@ def test_autotune_keeps_results():
     assert np.array_equal(tg.PR_S(G, 10), pr0)  # same sum order in every block size
     np.testing.assert_allclose(tg.BC(G, src), bc0, rtol=1e-9)
     assert np.array_equal(tg.bfs_multi(G, np.arange(0, n, n // 64, dtype=np.uint32)[:64]), ms0)
-    # forced extremes of the threshold give the same levels
+    assert tun["push_wave"] in (4096, 16384, 65536)
+    # forced extremes of the threshold and of the PMA push mapping (always
+    # 16 lanes, always a wave) give the same levels and labels
     for thr in (1, 100000):
-        G.set_tuning(thr_bfs=thr, thr_ms=thr, thr_cc=thr)
-        _, lv = tg.BFS_with_edge_map(G, src, with_levels=True)
-        assert np.array_equal(lv, lv0)
-        assert np.array_equal(tg.CC(G), cc0)
+        for pw in (0, 1 << 30):
+            G.set_tuning(thr_bfs=thr, thr_ms=thr, thr_cc=thr, push_wave=pw)
+            _, lv = tg.BFS_with_edge_map(G, src, with_levels=True)
+            assert np.array_equal(lv, lv0)
+            assert np.array_equal(tg.CC(G), cc0)
     with pytest.raises(Exception):
         G.set_tuning(pull_block=96)
     G.close()
