@@ -596,6 +596,18 @@ struct SelExisting {
   __device__ bool pred(uint64_t i) const { return edge_exists(g, key_src(k[i]), key_dst(k[i])); }
   __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = k[i]; }
 };
+// vertices of one secondary tier: B-tree (tree) or PMA (more than the
+// inline slots and no tree)
+struct SelTier {
+  const VBlock *vb;
+  bool tree;
+  uint32_t *out;
+  __device__ bool pred(uint64_t i) const {
+    const uint64_t aux = vb[i].aux;
+    return tree ? (aux & kAuxTree) != 0 : ((aux & kAuxTree) == 0 && vb[i].degree > kInline);
+  }
+  __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = static_cast<uint32_t>(i); }
+};
 struct SelTouched {
   const uint32_t *leaf;
   uint32_t *T, *Tstart;
@@ -1358,15 +1370,6 @@ __global__ void k_tiers(GView g, uint8_t *out) {
   }
 }
 
-__global__ void k_flag_tiers(GView g, uint32_t *fp, uint32_t *ft) {
-  GRID_STRIDE(v, g.n) {
-    const VBlock &b = g.vb[v];
-    bool t = (b.aux & kAuxTree) != 0;
-    ft[v] = t ? 1u : 0u;
-    fp[v] = (!t && b.degree > kInline) ? 1u : 0u;
-  }
-}
-
 // [0] PMA-tier vertices, [1] their secondary edges, [2] B-tree vertices,
 // [3] their secondary edges (device counters, zeroed by the caller)
 __global__ void k_tier_counts(GView g, unsigned long long *out) {
@@ -1389,10 +1392,6 @@ __global__ void k_tier_counts(GView g, unsigned long long *out) {
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     if ((threadIdx.x & 63) == 0 && x) atomicAdd(out + j, x);
   }
-}
-
-__global__ void k_scatter_ids(const uint32_t *flags, const uint32_t *pos, uint64_t n, uint32_t *out) {
-  GRID_STRIDE(i, n) if (flags[i]) out[pos[i]] = static_cast<uint32_t>(i);
 }
 
 __global__ void k_export_inline(GView g, const uint64_t *off, uint32_t *nbrs) {
@@ -2492,25 +2491,15 @@ void Graph::tier_counts(uint64_t out[4]) {
 
 const Graph::TierLists &Graph::tier_lists() {
   if (!lists_dirty_) return lists_;
-  GView g = view();
-  DevBuf<uint32_t> &fp = s32_[14], &ft = s32_[15], &pp = s32_[16], &pt = s32_[17];
-  fp.ensure(n_ + 1, stream_);
-  ft.ensure(n_ + 1, stream_);
-  pp.ensure(n_ + 1, stream_);
-  pt.ensure(n_ + 1, stream_);
-  k_flag_tiers<<<grid_for(n_, 256), 256, 0, stream_>>>(g, fp.get(), ft.get());
-  TG_CHECK_LAUNCH();
-  exclusive_scan_u32(fp.get(), pp.get(), n_, pp.get() + n_, scan_scratch, stream_);
-  exclusive_scan_u32(ft.get(), pt.get(), n_, pt.get() + n_, scan_scratch, stream_);
-  uint32_t np = read_scalar(pp.get() + n_, stream_);
-  uint32_t nt = read_scalar(pt.get() + n_, stream_);
-  list_pma_.ensure(np + 1, stream_);
-  list_tree_.ensure(nt + 1, stream_);
-  k_scatter_ids<<<grid_for(n_, 256), 256, 0, stream_>>>(fp.get(), pp.get(), n_, list_pma_.get());
-  TG_CHECK_LAUNCH();
-  k_scatter_ids<<<grid_for(n_, 256), 256, 0, stream_>>>(ft.get(), pt.get(), n_, list_tree_.get());
-  TG_CHECK_LAUNCH();
-  TG_HIP(hipStreamSynchronize(stream_));
+  // the two lists by predicate-fused selects over the vertex blocks (sized
+  // by n), both counts in one read
+  list_pma_.ensure(static_cast<uint64_t>(n_) + 1, stream_);
+  list_tree_.ensure(static_cast<uint64_t>(n_) + 1, stream_);
+  select(SelTier{vb_.get(), false, list_pma_.get()}, n_, dsc_.get() + 11, scan_scratch, stream_);
+  select(SelTier{vb_.get(), true, list_tree_.get()}, n_, dsc_.get() + 12, scan_scratch, stream_);
+  uint64_t h[2];
+  read_scalars(dsc_.get() + 11, h, 2, stream_);
+  const uint32_t np = static_cast<uint32_t>(h[0]), nt = static_cast<uint32_t>(h[1]);
   lists_.pma = list_pma_.get();
   lists_.npma = np;
   lists_.tree = list_tree_.get();
