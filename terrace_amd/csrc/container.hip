@@ -1394,35 +1394,55 @@ __global__ void k_tier_counts(GView g, unsigned long long *out) {
   }
 }
 
-__global__ void k_export_inline(GView g, const uint64_t *off, uint32_t *nbrs) {
-  GRID_STRIDE(v, g.n) {
+// 16 lanes per vertex: lane i writes inline slot i (a group's stores are
+// one run of consecutive ids).
+__global__ void __launch_bounds__(256) k_export_inline(GView g, const uint64_t *off, uint32_t *nbrs) {
+  const uint32_t gl = threadIdx.x & 15;
+  const uint64_t grp = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 4;
+  const uint64_t ngrp = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 4;
+  for (uint64_t v = grp; v < g.n; v += ngrp) {
     const VBlock &b = g.vb[v];
-    uint32_t lim = b.degree < kInline ? b.degree : kInline;
-    uint64_t o = off[v];
-    for (uint32_t i = 0; i < lim; i++) nbrs[o + i] = b.nbr[i];
+    const uint32_t lim = b.degree < kInline ? b.degree : kInline;
+    if (gl < lim) nbrs[off[v] + gl] = b.nbr[gl];
   }
 }
 
+// 16-lane group per PMA vertex, 4 slots per lane per 16-byte load (64 slots
+// per group step, four vertices per wave); ballot ranks keep slot order.
 __global__ void __launch_bounds__(256) k_export_pma(GView g, const uint32_t *list, uint32_t cnt, const uint64_t *off,
                                                     uint32_t *nbrs) {
-  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
-  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
-  uint32_t lane = threadIdx.x & 63;
-  for (uint64_t w = wave; w < cnt; w += nwaves) {
-    uint32_t v = list[w];
+  const uint32_t lane = threadIdx.x & 63, gl = lane & 15;
+  const uint64_t gmask = 0xFFFFull << (lane & 48);
+  const uint64_t lt = lanemask_lt() & gmask;
+  const uint64_t grp = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 4;
+  const uint64_t ngrp = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 4;
+  for (uint64_t w = grp; w < cnt; w += ngrp) {
+    const uint32_t v = list[w];
     const VBlock &b = g.vb[v];
-    uint32_t need = b.degree - kInline;
-    uint64_t p = b.aux;
-    uint64_t o = off[v] + kInline;
+    const uint32_t need = b.degree - kInline;
+    const uint64_t begin = b.aux;
+    const uint64_t o = off[v] + kInline;
+    uint64_t p = begin & ~3ull;
     uint32_t got = 0;
     while (got < need && p < g.N) {
-      uint64_t slot = p + lane;
-      uint32_t d = slot < g.N ? g.pdst[slot] : kEmpty;
-      bool ok = d != kEmpty;
-      uint64_t m = __ballot(ok);
-      uint32_t r = __popcll(m & lanemask_lt());
-      if (ok && got + r < need) nbrs[o + got + r] = d;
-      got += __popcll(m);
+      const uint64_t slot = p + gl * 4;
+      uint4 q = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+      if (slot < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + slot);
+      const uint32_t d[4] = {q.x, q.y, q.z, q.w};
+      bool val[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) val[j] = d[j] != kEmpty && slot + j >= begin;
+      const uint64_t B0 = __ballot(val[0]) & gmask, B1 = __ballot(val[1]) & gmask;
+      const uint64_t B2 = __ballot(val[2]) & gmask, B3 = __ballot(val[3]) & gmask;
+      uint32_t r = __popcll(B0 & lt) + __popcll(B1 & lt) + __popcll(B2 & lt) + __popcll(B3 & lt);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (val[j]) {
+          if (got + r < need) nbrs[o + got + r] = d[j];
+          r++;
+        }
+      }
+      got += __popcll(B0) + __popcll(B1) + __popcll(B2) + __popcll(B3);
       p += 64;
     }
   }
@@ -2514,10 +2534,10 @@ void Graph::export_adjacency(uint64_t *offsets, uint32_t *nbrs) {
   TG_CHECK_LAUNCH();
   exclusive_scan_u64(offsets, offsets, n_, offsets + n_, scan_scratch, stream_);
   const TierLists &L = tier_lists();
-  k_export_inline<<<grid_for(n_, 256), 256, 0, stream_>>>(g, offsets, nbrs);
+  k_export_inline<<<grid_for(static_cast<uint64_t>(n_) * 16, 256), 256, 0, stream_>>>(g, offsets, nbrs);
   TG_CHECK_LAUNCH();
   if (L.npma) {
-    k_export_pma<<<grid_for(static_cast<uint64_t>(L.npma) * 64, 256), 256, 0, stream_>>>(g, L.pma, L.npma, offsets,
+    k_export_pma<<<grid_for(static_cast<uint64_t>(L.npma) * 16, 256), 256, 0, stream_>>>(g, L.pma, L.npma, offsets,
                                                                                          nbrs);
     TG_CHECK_LAUNCH();
   }
