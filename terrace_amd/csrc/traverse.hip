@@ -799,20 +799,32 @@ __device__ inline uint64_t prs_first_ge(const uint32_t *nbr, uint64_t b, uint64_
   return lo;
 }
 
-__global__ void k_prs_counts(const uint64_t *off, const uint32_t *nbr, uint32_t n, const uint32_t *bnd,
-                             uint32_t *cnt, uint32_t *deg) {
-  GRID_STRIDE(v, static_cast<uint64_t>(n) + 1) {
-    if (v == n) {
-      for (uint32_t k = 0; k < kPrSlices; k++) cnt[static_cast<uint64_t>(k) * (n + 1) + n] = 0;
-      continue;
+// 8 lanes per vertex (8 vertices per wave): lane k finds where the vertex's
+// slice-k neighbours start in its sorted list; the next lane's start ends
+// them.
+__global__ void __launch_bounds__(256) k_prs_counts(const uint64_t *off, const uint32_t *nbr, uint32_t n,
+                                                    const uint32_t *bnd, uint32_t *cnt, uint32_t *deg) {
+  const uint32_t lane = threadIdx.x & 63, k = lane & 7;
+  const uint64_t grp = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 3;
+  const uint64_t ngrp = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 3;
+  const uint32_t x = k ? bnd[k] : 0;
+  for (uint64_t v0 = grp - (grp & 7); v0 <= n; v0 += ngrp) {
+    // the 8 groups of a wave take 8 consecutive vertices, so the loop trip
+    // count is uniform across the wave (shuffles below)
+    const uint64_t v = v0 + (grp & 7);
+    uint64_t b = 0, e = 0;
+    if (v < n) {
+      b = off[v];
+      e = off[v + 1];
     }
-    const uint64_t b = off[v], e = off[v + 1];
-    deg[v] = static_cast<uint32_t>(e - b);
-    uint64_t lo = b;
-    for (uint32_t k = 0; k < kPrSlices; k++) {
-      const uint64_t hi = k + 1 == kPrSlices ? e : prs_first_ge(nbr, lo, e, bnd[k + 1]);
-      cnt[static_cast<uint64_t>(k) * (n + 1) + v] = static_cast<uint32_t>(hi - lo);
-      lo = hi;
+    const uint64_t pos = k == 0 ? b : prs_first_ge(nbr, b, e, x);
+    uint64_t next = __shfl(pos, static_cast<int>((lane + 1) & 63), 64);
+    if (k == 7) next = e;
+    if (v < n) {
+      cnt[static_cast<uint64_t>(k) * (n + 1) + v] = static_cast<uint32_t>(next - pos);
+      if (k == 0) deg[v] = static_cast<uint32_t>(e - b);
+    } else if (v == n) {
+      cnt[static_cast<uint64_t>(k) * (n + 1) + n] = 0;
     }
   }
 }
@@ -1953,9 +1965,9 @@ void Traversal::prepare_pr_slices(uint64_t m) {
   prs_meta_.ensure(kPrSlices + 1, s);
   k_prs_bounds<<<1, 64, 0, s>>>(csr_off_.get(), n, prs_bnd_.get());
   TG_CHECK_LAUNCH();
-  k_prs_counts<<<grid_for(static_cast<uint64_t>(n) + 1, 256), 256, 0, s>>>(csr_off_.get(), csr_nbr_.get(), n,
-                                                                          prs_bnd_.get(), prs_off_.get(),
-                                                                          prs_deg_.get());
+  k_prs_counts<<<grid_for((static_cast<uint64_t>(n) + 8) * 8, 256), 256, 0, s>>>(csr_off_.get(), csr_nbr_.get(), n,
+                                                                                 prs_bnd_.get(), prs_off_.get(),
+                                                                                 prs_deg_.get());
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(prs_off_.get(), prs_off_.get(), nc, prs_off_.get() + nc, scan_, s);
   const uint32_t ntl = static_cast<uint32_t>((m + kPrsFillTile - 1) / kPrsFillTile);
