@@ -829,6 +829,33 @@ def main():
             tg.lib().tg_device_sync()
             dt = time.perf_counter() - t0
             extra[f"update_edges_per_s_b{bsz}"] = (bt[1][2] + bt[1][5]) / dt
+        # insert-only and delete-only batches, timed apart as the reference's
+        # graph_bm does (src/graph_bm.cc:325-388): b requested edges (rMAT,
+        # symmetrized) inserted, then the same b edges deleted; median of
+        # up to 5 batches per size
+        ins_rate, del_rate = {}, {}
+        for bsz in (() if args.no_extra_batches else (1000, 100_000, 10_000_000)):
+            reps = 5 if bsz <= 100_000 else 1
+            ti, td, cnt_ = [], [], 0
+            for r_ in range(reps + 1):
+                s_, d_, w_ = tg.rmat_device(n, aspen(20000 + 7 * r_ + bsz % 89) & 0xFFFFFFFF, *probs, bsz // 2,
+                                            nominal=True, symmetric=True, drop_loops=True)
+                tg.lib().tg_device_sync()
+                t0 = time.perf_counter()
+                G.add_edge_batch_dev(s_, d_, w_)
+                tg.lib().tg_device_sync()
+                t1 = time.perf_counter()
+                G.remove_edge_batch_dev(s_, d_, w_)
+                tg.lib().tg_device_sync()
+                t2 = time.perf_counter()
+                if r_:  # the first of each size warms its buffers
+                    ti.append(t1 - t0)
+                    td.append(t2 - t1)
+                    cnt_ = w_
+            ins_rate[bsz] = cnt_ / float(np.median(ti))
+            del_rate[bsz] = cnt_ / float(np.median(td))
+        extra["insert_only_edges_per_s"] = ins_rate
+        extra["delete_only_edges_per_s"] = del_rate
         # the same 1e5 mixed batches handed over as host arrays (PCIe-inclusive)
         bt = make_batches(tg, G, n, probs, args.batch, 3, 8000, rng, off, nbr)
         hb = [(s_.download(np.uint32, w_), d_.download(np.uint32, w_), bs_.download(np.uint32, nd_),
