@@ -159,15 +159,41 @@ __global__ void k_pack(const uint32_t *src, const uint32_t *dst, uint64_t n, uin
   }
 }
 
-// Batches of at most kSmallSortKeys pairs: pack, validate and sort in one
-// workgroup (bitonic sort over LDS, padded with ~0 to a power of two); the
-// result equals k_pack followed by the radix sort.
+// Batches of at most kSmallSortKeys pairs: pack, validate, sort and unique
+// in one workgroup (bitonic sort over LDS, padded with ~0 to a power of
+// two); the result equals k_pack, the radix sort and the unique select.
 constexpr uint32_t kSmallSortKeys = 4096;
 
+// Exclusive scan of one value per thread over a 1024-thread workgroup (wave
+// shuffles, then the 16 wave totals); *total gets the sum.  s_w: 16 words.
+__device__ inline uint32_t wg_excl_scan(uint32_t x, uint32_t *s_w, uint32_t *total) {
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t v = x;
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  if (lane == 63) s_w[w] = v;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (uint32_t i = 0; i < 16; i++) {
+    const uint32_t c = s_w[i];
+    if (i < w) base += c;
+    tot += c;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + v - x;
+}
+
 __global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, const uint32_t *dst, uint32_t count,
-                                                          uint32_t nv, uint64_t *keys, uint64_t *err) {
+                                                          uint32_t nv, uint64_t *keys, uint64_t *err,
+                                                          uint64_t *nunique) {
   __shared__ uint64_t s[kSmallSortKeys];
+  __shared__ uint32_t s_w[16];
   const uint32_t t = threadIdx.x;
+  if (t < 14) err[t] = 0;  // the batch scalars (the look-back error words [14], [15] stay)
+  __syncthreads();
   uint32_t P = 1;
   while (P < count) P <<= 1;
   bool bad = false;
@@ -201,7 +227,98 @@ __global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, c
       __syncthreads();
     }
   }
-  for (uint32_t i = t; i < count; i += blockDim.x) keys[i] = s[i];
+  // unique, in order: thread t takes keys 4t .. 4t+3
+  bool f[4];
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) {
+    const uint32_t i = 4 * t + j;
+    f[j] = i < count && (i == 0 || s[i] != s[i - 1]);
+    c += f[j] ? 1u : 0u;
+  }
+  uint32_t tot;
+  uint32_t pos = wg_excl_scan(c, s_w, &tot);
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++)
+    if (f[j]) keys[pos++] = s[4 * t + j];
+  if (t == 0) *nunique = tot;
+}
+
+// After the existence check of a small batch (at most kSmallSortKeys keys,
+// one workgroup).  Insert: nex (flags, 0 past the unique count) becomes its
+// exclusive prefix over [0, cap] and the source runs of R are listed, with
+// run_beg[nruns] = nR.  Delete: the existing keys of R (ex flags) are
+// compacted into E and the source runs of E listed.  Counts into sc[2]
+// (insert: runs; delete: existing) and sc[3] (delete: runs); nR is sc[1].
+template <bool kInsert>
+__global__ void __launch_bounds__(1024) k_small_runs(const uint64_t *R, uint32_t *flags, uint32_t cap, uint64_t *E,
+                                                     uint32_t *run_beg, uint64_t *sc) {
+  __shared__ uint64_t s_k[kSmallSortKeys];
+  __shared__ uint32_t s_w[16];
+  const uint32_t t = threadIdx.x;
+  const uint32_t nR = static_cast<uint32_t>(sc[1]);
+  uint32_t tot;
+  uint32_t m = nR;  // keys whose runs are listed (R, or E for deletes)
+  if (kInsert) {
+    // exclusive prefix of the new-key flags over [0, cap], total at cap
+    uint32_t x[4], c = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+      const uint32_t i = 4 * t + j;
+      x[j] = i < cap ? flags[i] : 0u;
+      c += x[j];
+    }
+    uint32_t pre = wg_excl_scan(c, s_w, &tot);
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+      const uint32_t i = 4 * t + j;
+      if (i < cap) flags[i] = pre;
+      pre += x[j];
+    }
+    if (t == 0) flags[cap] = tot;
+    for (uint32_t i = t; i < nR; i += blockDim.x) s_k[i] = R[i];
+  } else {
+    // compact the existing keys into E (and LDS)
+    bool f[4];
+    uint64_t k[4];
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+      const uint32_t i = 4 * t + j;
+      f[j] = i < nR && flags[i] != 0;
+      k[j] = f[j] ? R[i] : 0;
+      c += f[j] ? 1u : 0u;
+    }
+    uint32_t pos = wg_excl_scan(c, s_w, &tot);
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+      if (f[j]) {
+        E[pos] = k[j];
+        s_k[pos] = k[j];
+        pos++;
+      }
+    }
+    m = tot;
+    if (t == 0) sc[2] = tot;
+  }
+  __syncthreads();
+  // source runs of s_k[0, m)
+  bool f[4];
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) {
+    const uint32_t i = 4 * t + j;
+    f[j] = i < m && (i == 0 || key_src(s_k[i]) != key_src(s_k[i - 1]));
+    c += f[j] ? 1u : 0u;
+  }
+  uint32_t pos = wg_excl_scan(c, s_w, &tot);
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++)
+    if (f[j]) run_beg[pos++] = 4 * t + j;
+  if (t == 0) {
+    run_beg[tot] = m;
+    sc[kInsert ? 2 : 3] = tot;
+  }
 }
 
 __global__ void k_scatter_flagged(const uint64_t *in, const uint32_t *flags, const uint32_t *pos, uint64_t n,
@@ -1675,11 +1792,14 @@ uint64_t Graph::compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t
 void Graph::prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_t count) {
   k0_.ensure(count, stream_);
   k1_.ensure(count + 1, stream_);
-  fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
   if (count <= kSmallSortKeys) {
-    k_pack_sort_small<<<1, 1024, 0, stream_>>>(src, dst, static_cast<uint32_t>(count), n_, k0_.get(), dsc_.get());
+    // also clears the batch scalars
+    k_pack_sort_small<<<1, 1024, 0, stream_>>>(src, dst, static_cast<uint32_t>(count), n_, k1_.get(), dsc_.get(),
+                                               dsc_.get() + 1);
     TG_CHECK_LAUNCH();
+    return;  // unique keys in k1_, their count in dsc_[1]
   } else {
+    fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
     k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
     TG_CHECK_LAUNCH();
     uint32_t bits = 1;
@@ -2105,15 +2225,25 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   f1_.ensure(count + 1, stream_);
   k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, R, dsc_.get() + 1, count, f0_.get(), f1_.get());
   TG_CHECK_LAUNCH();
-  exclusive_scan_u32(f1_.get(), f1_.get(), count, f1_.get() + count, scan_scratch, stream_);
   s32_[0].ensure(count + 1, stream_);
-  select(SelRunStartN{R, s32_[0].get(), dsc_.get() + 1}, count, dsc_.get() + 2, scan_scratch, stream_);
+  const bool small = count <= kSmallSortKeys;
+  if (small) {
+    // new-key prefix, source runs and run_beg[nruns] = nR in one workgroup
+    k_small_runs<true><<<1, 1024, 0, stream_>>>(R, f1_.get(), static_cast<uint32_t>(count), nullptr,
+                                                s32_[0].get(), dsc_.get());
+    TG_CHECK_LAUNCH();
+  } else {
+    exclusive_scan_u32(f1_.get(), f1_.get(), count, f1_.get() + count, scan_scratch, stream_);
+    select(SelRunStartN{R, s32_[0].get(), dsc_.get() + 1}, count, dsc_.get() + 2, scan_scratch, stream_);
+  }
   uint64_t h[3];
   read_scalars(dsc_.get(), h, 3, stream_);
   if (h[0] & 1) throw Error(-1, "vertex id out of range");
   if (h[1] == 0) return;
-  k_put_u32_at<<<1, 1, 0, stream_>>>(s32_[0].get(), h[2], static_cast<uint32_t>(h[1]));
-  TG_CHECK_LAUNCH();
+  if (!small) {
+    k_put_u32_at<<<1, 1, 0, stream_>>>(s32_[0].get(), h[2], static_cast<uint32_t>(h[1]));
+    TG_CHECK_LAUNCH();
+  }
   insert_runs(R, h[1], static_cast<uint32_t>(h[2]));
 }
 
@@ -2269,15 +2399,25 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   f0_.ensure(count + 1, stream_);
   k_exists_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, k1_.get(), dsc_.get() + 1, count, f0_.get());
   TG_CHECK_LAUNCH();
-  select(SelFlagged{k1_.get(), f0_.get(), k2_.get()}, count, dsc_.get() + 2, scan_scratch, stream_);
   s32_[0].ensure(count + 1, stream_);
-  select(SelRunStartN{k2_.get(), s32_[0].get(), dsc_.get() + 2}, count, dsc_.get() + 3, scan_scratch, stream_);
+  const bool small = count <= kSmallSortKeys;
+  if (small) {
+    // existing keys, their source runs and run_beg[nruns] in one workgroup
+    k_small_runs<false><<<1, 1024, 0, stream_>>>(k1_.get(), f0_.get(), static_cast<uint32_t>(count), k2_.get(),
+                                                 s32_[0].get(), dsc_.get());
+    TG_CHECK_LAUNCH();
+  } else {
+    select(SelFlagged{k1_.get(), f0_.get(), k2_.get()}, count, dsc_.get() + 2, scan_scratch, stream_);
+    select(SelRunStartN{k2_.get(), s32_[0].get(), dsc_.get() + 2}, count, dsc_.get() + 3, scan_scratch, stream_);
+  }
   uint64_t h[4];
   read_scalars(dsc_.get(), h, 4, stream_);
   if (h[0] & 1) throw Error(-1, "vertex id out of range");
   if (h[2] == 0) return;
-  k_put_u32_at<<<1, 1, 0, stream_>>>(s32_[0].get(), h[3], static_cast<uint32_t>(h[2]));
-  TG_CHECK_LAUNCH();
+  if (!small) {
+    k_put_u32_at<<<1, 1, 0, stream_>>>(s32_[0].get(), h[3], static_cast<uint32_t>(h[2]));
+    TG_CHECK_LAUNCH();
+  }
   delete_runs(k2_.get(), h[2], static_cast<uint32_t>(h[3]));
 }
 
