@@ -244,6 +244,56 @@ __global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, c
   if (t == 0) *nunique = tot;
 }
 
+// The tier split of a small batch's keys (at most kSmallSortKeys, one
+// workgroup, order kept): PMA keys into P (count sc[0]), B-tree keys into T
+// (sc[1]); and the exclusive prefix of per-run counts (promotions, inline
+// refills) over the runs (total at ppos[nruns] and sc[2]).
+__global__ void __launch_bounds__(1024) k_small_route(const uint64_t *K, const uint8_t *route, uint32_t nk,
+                                                      uint64_t *P, uint64_t *T, const uint32_t *promote,
+                                                      uint32_t nruns, uint32_t *ppos, uint64_t *sc) {
+  __shared__ uint32_t s_w[16];
+  const uint32_t t = threadIdx.x;
+  uint32_t rt[4], cp = 0, ct = 0;
+  uint64_t k[4];
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) {
+    const uint32_t i = 4 * t + j;
+    rt[j] = i < nk ? route[i] : 0u;
+    k[j] = rt[j] ? K[i] : 0;
+    cp += rt[j] == kRoutePma ? 1u : 0u;
+    ct += rt[j] == kRouteTree ? 1u : 0u;
+  }
+  uint32_t tp, tt;
+  uint32_t pp = wg_excl_scan(cp, s_w, &tp);
+  uint32_t pt = wg_excl_scan(ct, s_w, &tt);
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) {
+    if (rt[j] == kRoutePma) P[pp++] = k[j];
+    if (rt[j] == kRouteTree) T[pt++] = k[j];
+  }
+  uint32_t x[4], c = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) {
+    const uint32_t i = 4 * t + j;
+    x[j] = i < nruns ? promote[i] : 0u;
+    c += x[j];
+  }
+  uint32_t tq;
+  uint32_t pre = wg_excl_scan(c, s_w, &tq);
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) {
+    const uint32_t i = 4 * t + j;
+    if (i < nruns) ppos[i] = pre;
+    pre += x[j];
+  }
+  if (t == 0) {
+    ppos[nruns] = tq;
+    sc[0] = tp;
+    sc[1] = tt;
+    sc[2] = tq;
+  }
+}
+
 // After the existence check of a small batch (at most kSmallSortKeys keys,
 // one workgroup).  Insert: nex (flags, 0 past the unique count) becomes its
 // exclusive prefix over [0, cap] and the source runs of R are listed, with
@@ -367,12 +417,14 @@ __global__ void k_exists_n(GView g, const uint64_t *keys, const uint64_t *np, ui
 // run_nko).
 __global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
                                const uint32_t *run_beg, uint32_t nruns, uint8_t *run_route,
-                               uint32_t *run_c, uint32_t *run_nko, uint32_t *kick_old, uint32_t *promote) {
+                               uint32_t *run_c, uint32_t *run_nko, uint32_t *kick_old, uint32_t *promote,
+                               uint32_t *old_deg) {
   GRID_STRIDE(r, nruns) {
     uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
     uint32_t s = key_src(R[b0]);
     VBlock &vb = g.vb[s];
     uint32_t D = vb.degree;
+    old_deg[r] = D;  // promotions need the degree before the merge
     uint32_t lim = D < kInline ? D : kInline;
     uint32_t inl[kInline];
     for (uint32_t i = 0; i < kInline; i++) inl[i] = i < lim ? vb.nbr[i] : 0;
@@ -1624,10 +1676,6 @@ __global__ void k_init_aux(VBlock *vb, uint32_t n) {
   GRID_STRIDE(v, n) vb[v].aux = kPmaNone;
 }
 
-__global__ void k_old_degrees(GView g, const uint64_t *R, const uint32_t *run_beg, uint32_t nruns, uint32_t *od) {
-  GRID_STRIDE(r, nruns) od[r] = g.vb[key_src(R[run_beg[r]])].degree;
-}
-
 __global__ void k_gather_u32(const uint32_t *a, const uint32_t *flags, const uint32_t *pos, uint32_t n,
                              uint32_t *out) {
   GRID_STRIDE(i, n) if (flags[i]) out[pos[i]] = a[i];
@@ -2289,11 +2337,9 @@ void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   // the old degrees of promoted vertices are needed after the merge
   DevBuf<uint32_t> &old_deg = s32_[7];
   old_deg.ensure(nruns + 1, stream_);
-  k_old_degrees<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, old_deg.get());
-  TG_CHECK_LAUNCH();
   k_inline_merge<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, ex, nidx, run_beg.get(), nruns, run_route.get(),
                                                             run_c.get(), run_nko.get(), kick_old.get(),
-                                                            promote.get());
+                                                            promote.get(), old_deg.get());
   TG_CHECK_LAUNCH();
   // kicked keys, sorted per run, placed in kick buffer k2_
   k2_.ensure(nR + 1, stream_);
@@ -2310,11 +2356,17 @@ void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   // split kicked keys by tier and count promotions: one host read
   k3_.ensure(nR + 1, stream_);
   g0_.ensure(nR + 1, stream_);
-  select(SelRoute{k2_.get(), kroute.get(), kRoutePma, k3_.get()}, nR, dsc_.get() + 3, scan_scratch, stream_);
-  select(SelRoute{k2_.get(), kroute.get(), kRouteTree, g0_.get()}, nR, dsc_.get() + 4, scan_scratch, stream_);
-  exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_);
-  k_u32_to_u64<<<1, 1, 0, stream_>>>(ppos.get() + nruns, dsc_.get() + 5);
-  TG_CHECK_LAUNCH();
+  if (nR <= kSmallSortKeys) {
+    k_small_route<<<1, 1024, 0, stream_>>>(k2_.get(), kroute.get(), static_cast<uint32_t>(nR), k3_.get(), g0_.get(),
+                                           promote.get(), nruns, ppos.get(), dsc_.get() + 3);
+    TG_CHECK_LAUNCH();
+  } else {
+    select(SelRoute{k2_.get(), kroute.get(), kRoutePma, k3_.get()}, nR, dsc_.get() + 3, scan_scratch, stream_);
+    select(SelRoute{k2_.get(), kroute.get(), kRouteTree, g0_.get()}, nR, dsc_.get() + 4, scan_scratch, stream_);
+    exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_);
+    k_u32_to_u64<<<1, 1, 0, stream_>>>(ppos.get() + nruns, dsc_.get() + 5);
+    TG_CHECK_LAUNCH();
+  }
   uint64_t h[3];
   read_scalars(dsc_.get() + 3, h, 3, stream_);
   uint64_t npma = h[0], ntree = h[1];
@@ -2461,11 +2513,17 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   // secondary deletes by tier, and how many inline slots to refill
   k3_.ensure(nR + 1, stream_);
   g0_.ensure(nR + 1, stream_);
-  select(SelRoute{R, sec_route.get(), kRoutePma, k3_.get()}, nR, dsc_.get() + 4, scan_scratch, stream_);
-  select(SelRoute{R, sec_route.get(), kRouteTree, g0_.get()}, nR, dsc_.get() + 5, scan_scratch, stream_);
-  exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_);
-  k_u32_to_u64<<<1, 1, 0, stream_>>>(bump_off.get() + nruns, dsc_.get() + 6);
-  TG_CHECK_LAUNCH();
+  if (nR <= kSmallSortKeys) {
+    k_small_route<<<1, 1024, 0, stream_>>>(R, sec_route.get(), static_cast<uint32_t>(nR), k3_.get(), g0_.get(),
+                                           run_bump.get(), nruns, bump_off.get(), dsc_.get() + 4);
+    TG_CHECK_LAUNCH();
+  } else {
+    select(SelRoute{R, sec_route.get(), kRoutePma, k3_.get()}, nR, dsc_.get() + 4, scan_scratch, stream_);
+    select(SelRoute{R, sec_route.get(), kRouteTree, g0_.get()}, nR, dsc_.get() + 5, scan_scratch, stream_);
+    exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_);
+    k_u32_to_u64<<<1, 1, 0, stream_>>>(bump_off.get() + nruns, dsc_.get() + 6);
+    TG_CHECK_LAUNCH();
+  }
   uint64_t h[3];
   read_scalars(dsc_.get() + 4, h, 3, stream_);
   uint64_t npma = h[0], ntree = h[1], nb = h[2];
@@ -2492,9 +2550,18 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
       k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, run_bump.get(), bump_off.get(),
                                                         bkeys.get(), broute.get());
       TG_CHECK_LAUNCH();
-      select(SelRoute{bkeys.get(), broute.get(), kRoutePma, bpma.get()}, nb, dsc_.get() + 4, scan_scratch, stream_);
-      select(SelRoute{bkeys.get(), broute.get(), kRouteTree, btree.get()}, nb, dsc_.get() + 5, scan_scratch,
-             stream_);
+      if (nb <= kSmallSortKeys) {
+        DevBuf<uint32_t> &none = s32_[31];  // no per-run counts here
+        none.ensure(1, stream_);
+        k_small_route<<<1, 1024, 0, stream_>>>(bkeys.get(), broute.get(), static_cast<uint32_t>(nb), bpma.get(),
+                                               btree.get(), nullptr, 0, none.get(), dsc_.get() + 4);
+        TG_CHECK_LAUNCH();
+      } else {
+        select(SelRoute{bkeys.get(), broute.get(), kRoutePma, bpma.get()}, nb, dsc_.get() + 4, scan_scratch,
+               stream_);
+        select(SelRoute{bkeys.get(), broute.get(), kRouteTree, btree.get()}, nb, dsc_.get() + 5, scan_scratch,
+               stream_);
+      }
       read_scalars(dsc_.get(), hs, kScalars, stream_);
       if (!tree_pending) break;
       tree_pending = false;
