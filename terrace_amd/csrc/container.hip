@@ -38,11 +38,39 @@ __device__ inline uint64_t pma_find_leaf(const GView &g, uint64_t key) {
   return lo == 0 ? 0 : lo - 1;
 }
 
-// Slot of (s,d) in the PMA, or ~0 when absent: the leaf's destinations
-// are read 16 bytes (4 slots) at a time, the source only where the
-// destination matches.
-__device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d) {
-  const uint64_t base = pma_find_leaf(g, make_key(s, d)) << kLeafShift;
+// The same leaf, searched from the leaf of begin_slot (the vertex's first
+// PMA slot) onwards.  Fences never decrease, so when fence[start] <= key the
+// answer is at or after the start, and a galloping search finds it in about
+// 2 log2(distance) fence reads instead of H; the vertex's keys span a few
+// leaves.  Any other start (none, a tree id, or a leaf past the key) takes
+// the full binary search, so a stale begin slot costs time, never the result.
+__device__ inline uint64_t pma_find_leaf_from(const GView &g, uint64_t key, uint64_t begin_slot) {
+  const uint64_t nl = 1ull << g.H;
+  uint64_t lo = begin_slot >> kLeafShift;
+  if (lo >= nl || g.fence[lo] > key) return pma_find_leaf(g, key);
+  uint64_t step = 1, hi = nl;
+  while (lo + step < nl) {
+    if (g.fence[lo + step] > key) {
+      hi = lo + step;
+      break;
+    }
+    lo += step;
+    step <<= 1;
+  }
+  // fence[lo] <= key, and fence[hi] > key unless hi == nl
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (g.fence[mid] <= key) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Slot of (s,d) in the PMA, or ~0 when absent (begin: the vertex's aux
+// word): the leaf's destinations are read 16 bytes (4 slots) at a time,
+// the source only where the destination matches.
+__device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d, uint64_t begin) {
+  const uint64_t base = pma_find_leaf_from(g, make_key(s, d), begin) << kLeafShift;
   const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + base);
 #pragma unroll 4
   for (uint32_t i = 0; i < kLeafSlots / 4; i++) {
@@ -100,7 +128,7 @@ __device__ inline bool edge_exists(const GView &g, uint32_t s, uint32_t d) {
   if (D <= kInline) return false;
   uint64_t aux = b.aux;
   if (aux & kAuxTree) return tree_has(g, static_cast<uint32_t>(aux), d);
-  return pma_slot_of(g, s, d) != ~0ull;
+  return pma_slot_of(g, s, d, aux) != ~0ull;
 }
 
 // Merge path: number of A elements among the first m of merge(A, B)
@@ -241,6 +269,117 @@ __global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, c
 #pragma unroll
   for (uint32_t j = 0; j < 4; j++)
     if (f[j]) keys[pos++] = s[4 * t + j];
+  if (t == 0) *nunique = tot;
+}
+
+// Batches of more than kSmallSortKeys and at most kMidSortKeys pairs: the
+// same pack, validate, sort and unique in one workgroup, by an LSD radix
+// sort held in LDS.  Each pass ranks 8-bit digits per wave with ballots in
+// index order (so the pass is stable), turns the per-wave digit counts into
+// start positions, and scatters the keys, held in registers, back into the
+// same buffer.  Digits cover bits [0, bits) and [32, 32 + bits) (vertex ids
+// below 2^bits); the result equals the radix sort and the unique select.
+constexpr uint32_t kMidSortKeys = 16384;
+constexpr uint32_t kMidItems = kMidSortKeys / 1024;
+
+__global__ void __launch_bounds__(1024) k_pack_sort_mid(const uint32_t *src, const uint32_t *dst, uint32_t count,
+                                                        uint32_t nv, uint32_t bits, uint64_t *keys, uint64_t *err,
+                                                        uint64_t *nunique) {
+  __shared__ uint64_t s[kMidSortKeys];
+  __shared__ uint32_t s_cnt[16][256];
+  __shared__ uint32_t s_w[16];
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t < 14) err[t] = 0;  // the batch scalars (the look-back error words [14], [15] stay)
+  __syncthreads();
+  bool bad = false;
+  for (uint32_t i = t; i < count; i += 1024) {
+    const uint32_t a = src[i], b = dst[i];
+    uint64_t v = 0;
+    if (a >= nv || b >= nv) bad = true;
+    else v = make_key(a, b);
+    s[i] = v;
+  }
+  if (bad) atomicOr(reinterpret_cast<unsigned long long *>(err), 1ull);
+  __syncthreads();
+  const uint64_t lt = lanemask_lt();
+  const uint32_t half = (bits + 7) / 8;
+  for (uint32_t p = 0; p < 2 * half; p++) {
+    const uint32_t shift = p < half ? 8 * p : 32 + 8 * (p - half);
+    for (uint32_t i = t; i < 16 * 256; i += 1024) (&s_cnt[0][0])[i] = 0;
+    __syncthreads();
+    // wave w holds keys [w * 1024, w * 1024 + 1024), 64 per step in order
+    uint64_t k[kMidItems];
+    uint32_t rank[kMidItems];
+#pragma unroll
+    for (uint32_t j = 0; j < kMidItems; j++) {
+      const uint32_t idx = w * (kMidItems * 64) + j * 64 + lane;
+      k[j] = 0;
+      rank[j] = 0;
+      if (w * (kMidItems * 64) + j * 64 >= count) continue;  // no key in this step (wave-uniform)
+      const bool valid = idx < count;
+      k[j] = valid ? s[idx] : 0;
+      const uint32_t d = static_cast<uint32_t>(k[j] >> shift) & 0xFFu;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (uint32_t bb = 0; bb < 8; bb++) {
+        const uint64_t m = __ballot((d >> bb) & 1u);
+        peers &= ((d >> bb) & 1u) ? m : ~m;
+      }
+      const uint32_t r = static_cast<uint32_t>(__popcll(peers & lt));
+      const int leader = valid ? __ffsll(static_cast<unsigned long long>(peers)) - 1 : 0;
+      uint32_t base = 0;
+      if (valid && lane == static_cast<uint32_t>(leader)) {
+        base = s_cnt[w][d];
+        s_cnt[w][d] = base + static_cast<uint32_t>(__popcll(peers));
+      }
+      base = __shfl(base, leader, 64);
+      rank[j] = base + r;
+    }
+    __syncthreads();
+    // digit totals, their exclusive scan (threads 0-255, one digit each),
+    // then each wave's start within its digit
+    uint32_t c = 0;
+    if (t < 256)
+      for (uint32_t x = 0; x < 16; x++) c += s_cnt[x][t];
+    uint32_t v = c;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(v, o, 64);
+      if (lane >= o) v += y;
+    }
+    if (t < 256 && lane == 63) s_w[w] = v;
+    __syncthreads();
+    if (t < 256) {
+      uint32_t run = v - c;
+      for (uint32_t x = 0; x < w; x++) run += s_w[x];
+      for (uint32_t x = 0; x < 16; x++) {
+        const uint32_t y = s_cnt[x][t];
+        s_cnt[x][t] = run;
+        run += y;
+      }
+    }
+    __syncthreads();
+    // every key of the pass is in registers: scatter in place
+#pragma unroll
+    for (uint32_t j = 0; j < kMidItems; j++) {
+      const uint32_t idx = w * (kMidItems * 64) + j * 64 + lane;
+      if (idx < count) s[s_cnt[w][static_cast<uint32_t>(k[j] >> shift) & 0xFFu] + rank[j]] = k[j];
+    }
+    __syncthreads();
+  }
+  // unique, in order: thread t takes keys kMidItems * t ...
+  bool f[kMidItems];
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kMidItems; j++) {
+    const uint32_t i = kMidItems * t + j;
+    f[j] = i < count && (i == 0 || s[i] != s[i - 1]);
+    c += f[j] ? 1u : 0u;
+  }
+  uint32_t tot;
+  uint32_t pos = wg_excl_scan(c, s_w, &tot);
+#pragma unroll
+  for (uint32_t j = 0; j < kMidItems; j++)
+    if (f[j]) keys[pos++] = s[kMidItems * t + j];
   if (t == 0) *nunique = tot;
 }
 
@@ -527,7 +666,10 @@ __global__ void k_place_kicked_old(const uint64_t *R, const uint32_t *nidx, cons
 
 // ------------------------------------------------------------ PMA kernels
 __global__ void k_leaf_of(GView g, const uint64_t *keys, uint64_t k, uint32_t *leaf) {
-  GRID_STRIDE(i, k) leaf[i] = static_cast<uint32_t>(pma_find_leaf(g, keys[i]));
+  GRID_STRIDE(i, k) {
+    const uint64_t key = keys[i];
+    leaf[i] = static_cast<uint32_t>(pma_find_leaf_from(g, key, g.vb[key_src(key)].aux));
+  }
 }
 
 __global__ void k_climb(GView g, Bounds bd, const uint32_t *T, const uint32_t *Tstart, const uint64_t *ntp,
@@ -622,82 +764,87 @@ __device__ inline void leaf_begin(const GView &g, uint32_t s, uint64_t base, uin
   else atomicMin(aux, static_cast<unsigned long long>(slot));
 }
 
-// One wave per touched leaf, four leaves per workgroup pass: the leaf's
-// elements (a leaf may hold holes after deletes) are compacted by rank in
-// LDS, then merged with the leaf's new keys (both sorted and disjoint) and
-// written packed left, the rest of the leaf emptied.
+// One wave per touched leaf (the grid covers every possible leaf; waves
+// past the touched count exit): the leaf's elements (a leaf may hold holes
+// after deletes) are compacted by rank in LDS next to the leaf's new keys,
+// then merged (both sorted and disjoint) and written packed left, the rest
+// of the leaf emptied.
 __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t *T, const uint32_t *Tstart,
                                                         const uint64_t *ntp, const uint64_t *keys,
                                                         const uint64_t *flag, uint64_t *err) {
   if (*flag) return;
-  __shared__ uint64_t s_old[4][kLeafSlots];
   const uint32_t nt = static_cast<uint32_t>(*ntp);
+  if (static_cast<uint64_t>(blockIdx.x) * 4 >= nt) return;  // the whole workgroup is idle
+  __shared__ uint64_t s_old[4][kLeafSlots];
+  __shared__ uint64_t s_new[4][kLeafSlots];
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr uint64_t kNone = ~0ull;
-  for (uint64_t tb = static_cast<uint64_t>(blockIdx.x) * 4; tb < nt; tb += static_cast<uint64_t>(gridDim.x) * 4) {
-    const uint64_t t = tb + w;
-    const bool live = t < nt;
-    uint64_t leaf = 0, base = 0;
-    uint32_t kb = 0, nb = 0, c = 0;
-    if (live) {
-      leaf = T[t];
-      base = leaf << kLeafShift;
-      kb = Tstart[t];
-      nb = Tstart[t + 1] - kb;
-      const uint32_t d = g.pdst[base + lane];
-      const bool v = d != kEmpty;
-      const uint64_t m = __ballot(v);
-      c = static_cast<uint32_t>(__popcll(m));
-      if (lane == 0 && c != g.pcnt[leaf]) atomicOr(reinterpret_cast<unsigned long long *>(err), 2ull);
-      if (v) s_old[w][__popcll(m & lanemask_lt())] = make_key(g.psrc[base + lane], d);
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 4 + w;
+  const bool live = t < nt;
+  uint64_t leaf = 0, base = 0;
+  uint32_t nb = 0, c = 0;
+  uint64_t b = kNone;
+  if (live) {
+    leaf = T[t];
+    base = leaf << kLeafShift;
+    const uint32_t kb = Tstart[t];
+    nb = Tstart[t + 1] - kb;
+    const uint32_t d = g.pdst[base + lane];
+    const bool v = d != kEmpty;
+    const uint64_t m = __ballot(v);
+    c = static_cast<uint32_t>(__popcll(m));
+    if (lane == 0 && c != g.pcnt[leaf]) atomicOr(reinterpret_cast<unsigned long long *>(err), 2ull);
+    if (v) s_old[w][__popcll(m & lanemask_lt())] = make_key(g.psrc[base + lane], d);
+    if (lane < nb) {
+      b = keys[kb + lane];
+      s_new[w][lane] = b;
     }
-    __syncthreads();
-    if (live) {
-      // c + nb <= up[0] < 64: lane l holds old element l and new key l
-      const uint64_t a = lane < c ? s_old[w][lane] : kNone;
-      const uint64_t b = lane < nb ? keys[kb + lane] : kNone;
-      // new key b: old elements below it
-      uint32_t lo = 0, hi = c;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_old[w][mid] < b) lo = mid + 1;
-        else hi = mid;
-      }
-      if (lane < c) {
-        // old element a: new keys below it
-        uint32_t l2 = 0, h2 = nb;
-        while (l2 < h2) {
-          const uint32_t mid = (l2 + h2) >> 1;
-          if (keys[kb + mid] < a) l2 = mid + 1;
-          else h2 = mid;
-        }
-        const uint64_t slot = base + lane + l2;
-        // the element before a: the larger of a[lane - 1] and the last new key below a
-        bool has_prev = lane > 0 || l2 > 0;
-        uint64_t prev = lane > 0 ? s_old[w][lane - 1] : 0;
-        if (l2 > 0 && (lane == 0 || keys[kb + l2 - 1] > prev)) prev = keys[kb + l2 - 1];
-        const uint32_t s = key_src(a);
-        g.psrc[slot] = s;
-        g.pdst[slot] = key_dst(a);
-        if (!has_prev || key_src(prev) != s) leaf_begin(g, s, base, slot);
-      }
-      if (lane < nb) {
-        const uint64_t slot = base + lane + lo;
-        bool has_prev = lane > 0 || lo > 0;
-        uint64_t prev = lane > 0 ? keys[kb + lane - 1] : 0;
-        if (lo > 0 && (lane == 0 || s_old[w][lo - 1] > prev)) prev = s_old[w][lo - 1];
-        const uint32_t s = key_src(b);
-        g.psrc[slot] = s;
-        g.pdst[slot] = key_dst(b);
-        if (!has_prev || key_src(prev) != s) leaf_begin(g, s, base, slot);
-      }
-      if (lane >= c + nb) g.pdst[base + lane] = kEmpty;
-      if (lane == 0) {
-        g.pcnt[leaf] = c + nb;
-        g.fence[leaf] = (c > 0 && a < b) ? a : b;
-      }
+  }
+  __syncthreads();
+  if (!live) return;
+  // c + nb <= up[0] < 64: lane l holds old element l and new key l
+  const uint64_t a = lane < c ? s_old[w][lane] : kNone;
+  // new key b: old elements below it
+  uint32_t lo = 0, hi = c;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s_old[w][mid] < b) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lane < c) {
+    // old element a: new keys below it
+    uint32_t l2 = 0, h2 = nb;
+    while (l2 < h2) {
+      const uint32_t mid = (l2 + h2) >> 1;
+      if (s_new[w][mid] < a) l2 = mid + 1;
+      else h2 = mid;
     }
-    __syncthreads();
+    const uint64_t slot = base + lane + l2;
+    // the element before a: the larger of a[lane - 1] and the last new key below a
+    const bool has_prev = lane > 0 || l2 > 0;
+    uint64_t prev = lane > 0 ? s_old[w][lane - 1] : 0;
+    if (l2 > 0 && (lane == 0 || s_new[w][l2 - 1] > prev)) prev = s_new[w][l2 - 1];
+    const uint32_t s = key_src(a);
+    g.psrc[slot] = s;
+    g.pdst[slot] = key_dst(a);
+    if (!has_prev || key_src(prev) != s) leaf_begin(g, s, base, slot);
+  }
+  if (lane < nb) {
+    const uint64_t slot = base + lane + lo;
+    const bool has_prev = lane > 0 || lo > 0;
+    uint64_t prev = lane > 0 ? s_new[w][lane - 1] : 0;
+    if (lo > 0 && (lane == 0 || s_old[w][lo - 1] > prev)) prev = s_old[w][lo - 1];
+    const uint32_t s = key_src(b);
+    g.psrc[slot] = s;
+    g.pdst[slot] = key_dst(b);
+    if (!has_prev || key_src(prev) != s) leaf_begin(g, s, base, slot);
+  }
+  if (lane >= c + nb) g.pdst[base + lane] = kEmpty;
+  if (lane == 0) {
+    g.pcnt[leaf] = c + nb;
+    // fence: the smaller of the first old element and the first new key
+    const uint64_t a0 = c > 0 ? s_old[w][0] : kNone, b0 = nb > 0 ? s_new[w][0] : kNone;
+    g.fence[leaf] = a0 < b0 ? a0 : b0;
   }
 }
 
@@ -980,7 +1127,8 @@ __global__ void __launch_bounds__(256) k_tree_up(uint32_t *pcnt, uint32_t H, uin
 __global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t k, uint64_t *fix_slot, uint64_t *err) {
   GRID_STRIDE(i, k) {
     uint32_t s = key_src(keys[i]), d = key_dst(keys[i]);
-    uint64_t slot = pma_slot_of(g, s, d);
+    const uint64_t begin = g.vb[s].aux;
+    uint64_t slot = pma_slot_of(g, s, d, begin);
     fix_slot[i] = ~0ull;
     if (slot == ~0ull) {
       atomicOr(reinterpret_cast<unsigned long long *>(err), 4ull);
@@ -989,7 +1137,7 @@ __global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t k, uint64_t
     g.pdst[slot] = kEmpty;
     uint64_t leaf = slot >> kLeafShift;
     atomicSub(&g.pcnt[leaf], 1u);
-    if (g.vb[s].aux == slot) fix_slot[i] = slot;
+    if (begin == slot) fix_slot[i] = slot;
   }
 }
 
@@ -1846,6 +1994,13 @@ void Graph::prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_
                                                dsc_.get() + 1);
     TG_CHECK_LAUNCH();
     return;  // unique keys in k1_, their count in dsc_[1]
+  } else if (count <= kMidSortKeys) {
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < n_) bits++;
+    k_pack_sort_mid<<<1, 1024, 0, stream_>>>(src, dst, static_cast<uint32_t>(count), n_, bits, k1_.get(),
+                                             dsc_.get(), dsc_.get() + 1);
+    TG_CHECK_LAUNCH();
+    return;
   } else {
     fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
     k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
@@ -1969,7 +2124,11 @@ void Graph::pma_fast_launch(const uint64_t *keys, uint64_t k) {
   select(SelTouched{u0_.get(), T.get(), Tstart.get()}, k, ntp, scan_scratch, stream_);
   k_touched_end<<<1, 1, 0, stream_>>>(Tstart.get(), ntp, static_cast<uint32_t>(k), fitp);
   TG_CHECK_LAUNCH();
-  const unsigned wg = grid_for(k * 64, 256, 4096);  // a wave per touched leaf (at most k)
+  // a wave per touched leaf (at most k of them), four per workgroup
+  const uint64_t most = k < (1ull << pma_H_) ? k : (1ull << pma_H_);
+  const uint64_t wg64 = (most + 3) / 4;
+  if (wg64 > 0x7FFFFFFFull) throw Error(-1, "PMA merge: too many keys");
+  const unsigned wg = static_cast<unsigned>(wg64);
   k_pma_leaf_fit<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_.up[0], T.get(), Tstart.get(), ntp, fitp);
   TG_CHECK_LAUNCH();
   k_pma_leaf_merge<<<wg, 256, 0, stream_>>>(g, T.get(), Tstart.get(), ntp, keys, fitp, dsc_.get());

@@ -283,6 +283,44 @@ def test_api_edge_cases():
     G.close()
 
 
+def test_mid_size_batches_sorted_in_one_workgroup():
+    """Batches of 4097..16384 raw pairs are packed, sorted and de-duplicated
+    by one workgroup (LDS radix sort); 16385 takes the multi-workgroup sort.
+    Unsorted pairs with repeats, ids up to n - 1 on a vertex count that is
+    not a power of two (17-bit digits), inserts and deletes (some of absent
+    edges) match the set model after every batch, and a mid-size batch
+    holding one bad id is rejected without changing the graph."""
+    n = 100_003
+    say("mid-size batches")
+    rng = np.random.default_rng(41)
+    keys = oracle.build_keys(*oracle.rmat_edges(1 << 16, 23, 0.57, 0.19, 0.19, 200000))
+    G = tg.Graph(n)
+    G.add_edge_batch(*split(keys))
+    for rnd, size in enumerate([4097, 9000, 16384, 16385, 12000, 5000]):
+        hi = np.uint64(n)
+        s = rng.integers(0, n, size).astype(np.uint64)
+        d = rng.integers(0, n, size).astype(np.uint64)
+        s[:50], d[:50] = hi - np.uint64(1), rng.integers(0, n, 50).astype(np.uint64)
+        raw = (s << np.uint64(32)) | d
+        raw[size // 2:size // 2 + 200] = raw[:200]  # repeats inside the batch
+        if rnd % 2 == 0:
+            G.add_edge_batch(*split(raw))
+            keys = oracle.set_union(keys, oracle.sort_unique(raw))
+        else:
+            raw[:size // 2] = keys[rng.choice(len(keys), size // 2, replace=False)]  # existing edges
+            G.remove_edge_batch(*split(raw))
+            keys = oracle.set_difference(keys, oracle.sort_unique(raw))
+        assert_adjacency(G, keys, n, f"mid-size batch {rnd} ({size})")
+    m = G.get_num_edges()
+    bs, bd = split(keys[:10000])
+    bs = bs.copy()
+    bs[7000] = n  # out of range
+    with pytest.raises(tg.TerraceError) as ei:
+        G.remove_edge_batch(bs, bd)
+    assert ei.value.code == -1 and G.get_num_edges() == m
+    G.close()
+
+
 def test_edge_map_frontier_api():
     n = 1 << 10
     say("edge_map api")
