@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -32,6 +33,8 @@ struct alignas(64) VBlock {
   uint64_t aux;
 };
 static_assert(sizeof(VBlock) == 64, "vertex block must be 64 bytes");
+static_assert(offsetof(VBlock, nbr) == 4 && offsetof(VBlock, aux) == 56,
+              "device code reads the block as 16 words: degree, 13 neighbours, aux");
 
 __host__ __device__ inline uint64_t make_key(uint32_t s, uint32_t d) {
   return (static_cast<uint64_t>(s) << 32) | d;

@@ -66,29 +66,67 @@ __device__ inline uint64_t pma_find_leaf_from(const GView &g, uint64_t key, uint
   return lo;
 }
 
-// Slot of (s,d) in the PMA, or ~0 when absent (begin: the vertex's aux
-// word): the leaf's destinations are read 16 bytes (4 slots) at a time,
-// the source only where the destination matches.
-__device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d, uint64_t begin) {
-  const uint64_t base = pma_find_leaf_from(g, make_key(s, d), begin) << kLeafShift;
-  const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + base);
-#pragma unroll 4
+// Leaf of key when it is the begin leaf L0 (fence[L0] <= key < fence[L0 + 1],
+// the usual case for a vertex of the PMA tier), else the search above.
+__device__ inline uint64_t pma_leaf_of(const GView &g, uint64_t key, uint64_t begin) {
+  const uint64_t L0 = begin >> kLeafShift;
+  if (L0 + 1 < (1ull << g.H)) {
+    const uint64_t f0 = g.fence[L0], f1 = g.fence[L0 + 1];
+    if (f0 <= key && key < f1) return L0;
+  }
+  return pma_find_leaf_from(g, key, begin);
+}
+
+// The slot among a leaf's destinations v (16 groups of 4 slots from base)
+// whose destination is d and source s, or ~0.
+__device__ inline uint64_t leaf_match(const GView &g, const uint4 *v, uint64_t base, uint32_t s, uint32_t d) {
+#pragma unroll
   for (uint32_t i = 0; i < kLeafSlots / 4; i++) {
-    const uint4 v = pd[i];
     const uint64_t p = base + 4 * i;
-    if (v.x == d && g.psrc[p] == s) return p;
-    if (v.y == d && g.psrc[p + 1] == s) return p + 1;
-    if (v.z == d && g.psrc[p + 2] == s) return p + 2;
-    if (v.w == d && g.psrc[p + 3] == s) return p + 3;
+    if (v[i].x == d && g.psrc[p] == s) return p;
+    if (v[i].y == d && g.psrc[p + 1] == s) return p + 1;
+    if (v[i].z == d && g.psrc[p + 2] == s) return p + 2;
+    if (v[i].w == d && g.psrc[p + 3] == s) return p + 3;
   }
   return ~0ull;
 }
 
+// Slot of (s,d) in the PMA, or ~0 when absent (begin: the vertex's aux
+// word).  The begin leaf's destinations are loaded with its two fences, so
+// a key inside the begin leaf costs one round trip (and the source check
+// where a destination matches); other keys take the fence search.
+__device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d, uint64_t begin) {
+  const uint64_t key = make_key(s, d);
+  const uint64_t L0 = begin >> kLeafShift;
+  uint4 v[kLeafSlots / 4];
+  if (L0 + 1 < (1ull << g.H)) {
+    const uint64_t f0 = g.fence[L0], f1 = g.fence[L0 + 1];
+    const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + (L0 << kLeafShift));
+#pragma unroll
+    for (uint32_t i = 0; i < kLeafSlots / 4; i++) v[i] = pd[i];
+    if (f0 <= key && key < f1) return leaf_match(g, v, L0 << kLeafShift, s, d);
+  }
+  const uint64_t base = pma_find_leaf_from(g, key, begin) << kLeafShift;
+  const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + base);
+#pragma unroll
+  for (uint32_t i = 0; i < kLeafSlots / 4; i++) v[i] = pd[i];
+  return leaf_match(g, v, base, s, d);
+}
+
 // Index position j of the leaf block of tree t that may hold d
-// (last block whose first key <= d, or 0).
+// (last block whose first key <= d, or 0).  Trees of at most 16 blocks
+// (up to 4096 neighbours) load every first key at once.
 __device__ inline uint32_t tree_block_pos(const GView &g, uint32_t t, uint32_t d) {
-  uint64_t off = g.toff[t];
-  uint32_t lo = 0, hi = g.tnblk[t];
+  const uint64_t off = g.toff[t];
+  const uint32_t nb = g.tnblk[t];
+  if (nb <= 16) {
+    uint32_t j = 0;
+#pragma unroll
+    for (uint32_t x = 1; x < 16; x++)
+      if (x < nb && g.imin[off + x] <= d) j = x;
+    return j;
+  }
+  uint32_t lo = 0, hi = nb;
   while (lo < hi) {
     uint32_t mid = (lo + hi) >> 1;
     if (g.imin[off + mid] <= d) lo = mid + 1;
@@ -97,14 +135,25 @@ __device__ inline uint32_t tree_block_pos(const GView &g, uint32_t t, uint32_t d
   return lo == 0 ? 0 : lo - 1;
 }
 
+// d among the first cnt keys of a leaf block (kBlk = 256 slots, sorted):
+// 16 samples at stride 16, then the 16 keys after the last sample <= d,
+// each level loaded at once (slots past cnt are read but never used).
 __device__ inline bool block_has(const uint32_t *keys, uint32_t cnt, uint32_t d) {
-  uint32_t lo = 0, hi = cnt;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (keys[mid] < d) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo < cnt && keys[lo] == d;
+  static_assert(kBlk == 256, "two 16-way levels cover a block");
+  uint32_t smp[16];
+#pragma unroll
+  for (uint32_t x = 1; x < 16; x++) smp[x] = keys[16 * x];
+  uint32_t lo = 0;
+#pragma unroll
+  for (uint32_t x = 1; x < 16; x++)
+    if (16 * x < cnt && smp[x] <= d) lo = 16 * x;
+  uint32_t v[16];
+#pragma unroll
+  for (uint32_t x = 0; x < 16; x++) v[x] = keys[lo + x];
+  bool hit = false;
+#pragma unroll
+  for (uint32_t x = 0; x < 16; x++) hit |= lo + x < cnt && v[x] == d;
+  return hit;
 }
 
 __device__ inline bool tree_has(const GView &g, uint32_t t, uint32_t d) {
@@ -116,17 +165,22 @@ __device__ inline bool tree_has(const GView &g, uint32_t t, uint32_t d) {
 }
 
 // Graph::is_edge (graph.h:643-673): inline first, then the secondary tier.
+// The 64-byte block is loaded whole (four 16-byte loads issued together).
 __device__ inline bool edge_exists(const GView &g, uint32_t s, uint32_t d) {
-  const VBlock &b = g.vb[s];
-  uint32_t D = b.degree;
-  uint32_t lim = D < kInline ? D : kInline;
-  for (uint32_t i = 0; i < lim; i++) {
-    uint32_t x = b.nbr[i];
-    if (x == d) return true;
-    if (x > d) return false;
-  }
-  if (D <= kInline) return false;
-  uint64_t aux = b.aux;
+  const uint4 *vp = reinterpret_cast<const uint4 *>(g.vb + s);
+  const uint4 q0 = vp[0], q1 = vp[1], q2 = vp[2], q3 = vp[3];
+  // degree, nbr[0..12], aux (low, high)
+  const uint32_t w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+  const uint32_t D = w[0];
+  const uint32_t lim = D < kInline ? D : kInline;
+  bool hit = false;
+#pragma unroll
+  for (uint32_t i = 0; i < kInline; i++) hit |= i < lim && w[1 + i] == d;
+  if (hit) return true;
+  // inline slots hold the 13 smallest neighbours: past them only if d is larger
+  if (D <= kInline || d < w[kInline]) return false;
+  const uint64_t aux = static_cast<uint64_t>(w[14]) | (static_cast<uint64_t>(w[15]) << 32);
   if (aux & kAuxTree) return tree_has(g, static_cast<uint32_t>(aux), d);
   return pma_slot_of(g, s, d, aux) != ~0ull;
 }
@@ -272,115 +326,161 @@ __global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, c
   if (t == 0) *nunique = tot;
 }
 
-// Batches of more than kSmallSortKeys and at most kMidSortKeys pairs: the
-// same pack, validate, sort and unique in one workgroup, by an LSD radix
-// sort held in LDS.  Each pass ranks 8-bit digits per wave with ballots in
-// index order (so the pass is stable), turns the per-wave digit counts into
-// start positions, and scatters the keys, held in registers, back into the
-// same buffer.  Digits cover bits [0, bits) and [32, 32 + bits) (vertex ids
-// below 2^bits); the result equals the radix sort and the unique select.
-constexpr uint32_t kMidSortKeys = 16384;
-constexpr uint32_t kMidItems = kMidSortKeys / 1024;
+// Batches of more than kSmallSortKeys and at most kBucketSortMax pairs: a
+// bucket sort.  The compressed key c = src << bits | dst (vertex ids below
+// 2^bits) orders like the key; its top 12 bits pick one of 4096 buckets,
+// so buckets follow key order.  Pack, validate and count per bucket;
+// scatter by bucket (each block scans the counts itself; order inside a
+// bucket is arbitrary); one workgroup per bucket sorts it in LDS (bitonic);
+// then the unique select as after the radix sort.  A bucket beyond kBucketCap
+// keys sets kSortOverflow in the error word and the caller sorts again by
+// radix.  The sort kernel clears the counts and cursors for the next call.
+constexpr uint32_t kBucketBits = 12, kBuckets = 1u << kBucketBits;
+constexpr uint32_t kBucketCap = 2048;
+constexpr uint64_t kBucketSortMax = 1u << 18;
+constexpr uint64_t kSortOverflow = 256;
+constexpr uint32_t kBucketBlock = 256, kBucketItems = 8, kBucketTile = kBucketBlock * kBucketItems;
 
-__global__ void __launch_bounds__(1024) k_pack_sort_mid(const uint32_t *src, const uint32_t *dst, uint32_t count,
-                                                        uint32_t nv, uint32_t bits, uint64_t *keys, uint64_t *err,
-                                                        uint64_t *nunique) {
-  __shared__ uint64_t s[kMidSortKeys];
-  __shared__ uint32_t s_cnt[16][256];
-  __shared__ uint32_t s_w[16];
-  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (t < 14) err[t] = 0;  // the batch scalars (the look-back error words [14], [15] stay)
+__device__ inline uint32_t bucket_of(uint64_t key, uint32_t bits, uint32_t shift) {
+  const uint64_t c = (static_cast<uint64_t>(key_src(key)) << bits) | key_dst(key);
+  return static_cast<uint32_t>(c >> shift);
+}
+
+__global__ void __launch_bounds__(kBucketBlock) k_bucket_count(const uint32_t *src, const uint32_t *dst,
+                                                               uint32_t count, uint32_t nv, uint32_t bits,
+                                                               uint32_t shift, uint64_t *packed, uint32_t *bcnt,
+                                                               uint64_t *err) {
+  __shared__ uint32_t s_c[kBuckets];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < kBuckets; i += kBucketBlock) s_c[i] = 0;
   __syncthreads();
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBucketTile;
+  uint32_t ra[kBucketItems], rb[kBucketItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kBucketItems; j++) {
+    const uint64_t i = base + j * kBucketBlock + t;
+    ra[j] = i < count ? src[i] : 0;
+    rb[j] = i < count ? dst[i] : 0;
+  }
   bool bad = false;
-  for (uint32_t i = t; i < count; i += 1024) {
-    const uint32_t a = src[i], b = dst[i];
-    uint64_t v = 0;
-    if (a >= nv || b >= nv) bad = true;
-    else v = make_key(a, b);
-    s[i] = v;
+#pragma unroll
+  for (uint32_t j = 0; j < kBucketItems; j++) {
+    const uint64_t i = base + j * kBucketBlock + t;
+    if (i >= count) continue;
+    uint64_t key = 0;
+    if (ra[j] >= nv || rb[j] >= nv) bad = true;
+    else key = make_key(ra[j], rb[j]);
+    packed[i] = key;
+    atomicAdd(&s_c[bucket_of(key, bits, shift)], 1u);
   }
   if (bad) atomicOr(reinterpret_cast<unsigned long long *>(err), 1ull);
   __syncthreads();
-  const uint64_t lt = lanemask_lt();
-  const uint32_t half = (bits + 7) / 8;
-  for (uint32_t p = 0; p < 2 * half; p++) {
-    const uint32_t shift = p < half ? 8 * p : 32 + 8 * (p - half);
-    for (uint32_t i = t; i < 16 * 256; i += 1024) (&s_cnt[0][0])[i] = 0;
-    __syncthreads();
-    // wave w holds keys [w * 1024, w * 1024 + 1024), 64 per step in order
-    uint64_t k[kMidItems];
-    uint32_t rank[kMidItems];
+  for (uint32_t i = t; i < kBuckets; i += kBucketBlock)
+    if (s_c[i]) atomicAdd(&bcnt[i], s_c[i]);
+}
+
+// Exclusive scan of the 4096 bucket counts into s_off (256 threads, 16
+// buckets each); block 0 also stores the starts for the sort kernel.
+__device__ inline void bucket_starts(const uint32_t *bcnt, uint32_t *s_off, uint32_t *s_w, uint32_t *boff) {
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t x[16], sum = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < kMidItems; j++) {
-      const uint32_t idx = w * (kMidItems * 64) + j * 64 + lane;
-      k[j] = 0;
-      rank[j] = 0;
-      if (w * (kMidItems * 64) + j * 64 >= count) continue;  // no key in this step (wave-uniform)
-      const bool valid = idx < count;
-      k[j] = valid ? s[idx] : 0;
-      const uint32_t d = static_cast<uint32_t>(k[j] >> shift) & 0xFFu;
-      uint64_t peers = __ballot(valid);
-#pragma unroll
-      for (uint32_t bb = 0; bb < 8; bb++) {
-        const uint64_t m = __ballot((d >> bb) & 1u);
-        peers &= ((d >> bb) & 1u) ? m : ~m;
-      }
-      const uint32_t r = static_cast<uint32_t>(__popcll(peers & lt));
-      const int leader = valid ? __ffsll(static_cast<unsigned long long>(peers)) - 1 : 0;
-      uint32_t base = 0;
-      if (valid && lane == static_cast<uint32_t>(leader)) {
-        base = s_cnt[w][d];
-        s_cnt[w][d] = base + static_cast<uint32_t>(__popcll(peers));
-      }
-      base = __shfl(base, leader, 64);
-      rank[j] = base + r;
-    }
-    __syncthreads();
-    // digit totals, their exclusive scan (threads 0-255, one digit each),
-    // then each wave's start within its digit
-    uint32_t c = 0;
-    if (t < 256)
-      for (uint32_t x = 0; x < 16; x++) c += s_cnt[x][t];
-    uint32_t v = c;
-    for (uint32_t o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(v, o, 64);
-      if (lane >= o) v += y;
-    }
-    if (t < 256 && lane == 63) s_w[w] = v;
-    __syncthreads();
-    if (t < 256) {
-      uint32_t run = v - c;
-      for (uint32_t x = 0; x < w; x++) run += s_w[x];
-      for (uint32_t x = 0; x < 16; x++) {
-        const uint32_t y = s_cnt[x][t];
-        s_cnt[x][t] = run;
-        run += y;
-      }
-    }
-    __syncthreads();
-    // every key of the pass is in registers: scatter in place
-#pragma unroll
-    for (uint32_t j = 0; j < kMidItems; j++) {
-      const uint32_t idx = w * (kMidItems * 64) + j * 64 + lane;
-      if (idx < count) s[s_cnt[w][static_cast<uint32_t>(k[j] >> shift) & 0xFFu] + rank[j]] = k[j];
-    }
-    __syncthreads();
+  for (uint32_t q = 0; q < 16; q++) {
+    x[q] = bcnt[16 * t + q];
+    sum += x[q];
   }
-  // unique, in order: thread t takes keys kMidItems * t ...
-  bool f[kMidItems];
-  uint32_t c = 0;
-#pragma unroll
-  for (uint32_t j = 0; j < kMidItems; j++) {
-    const uint32_t i = kMidItems * t + j;
-    f[j] = i < count && (i == 0 || s[i] != s[i - 1]);
-    c += f[j] ? 1u : 0u;
+  uint32_t inc = sum;
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
   }
-  uint32_t tot;
-  uint32_t pos = wg_excl_scan(c, s_w, &tot);
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  uint32_t run = inc - sum;
+  for (uint32_t v = 0; v < w; v++) run += s_w[v];
 #pragma unroll
-  for (uint32_t j = 0; j < kMidItems; j++)
-    if (f[j]) keys[pos++] = s[kMidItems * t + j];
-  if (t == 0) *nunique = tot;
+  for (uint32_t q = 0; q < 16; q++) {
+    s_off[16 * t + q] = run;
+    if (blockIdx.x == 0) boff[16 * t + q] = run;
+    run += x[q];
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBucketBlock) k_bucket_scatter(const uint64_t *packed, uint32_t count,
+                                                                 uint32_t bits, uint32_t shift,
+                                                                 const uint32_t *bcnt, uint32_t *bcur,
+                                                                 uint32_t *boff, uint64_t *out) {
+  __shared__ uint32_t s_off[kBuckets];
+  __shared__ uint32_t s_c[kBuckets];
+  __shared__ uint32_t s_w[kBucketBlock / 64];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < kBuckets; i += kBucketBlock) s_c[i] = 0;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBucketTile;
+  uint64_t k[kBucketItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kBucketItems; j++) {
+    const uint64_t i = base + j * kBucketBlock + t;
+    k[j] = i < count ? packed[i] : 0;
+  }
+  bucket_starts(bcnt, s_off, s_w, boff);  // also orders the s_c clearing before its use
+  uint32_t r[kBucketItems];
+#pragma unroll
+  for (uint32_t j = 0; j < kBucketItems; j++) {
+    const uint64_t i = base + j * kBucketBlock + t;
+    r[j] = i < count ? atomicAdd(&s_c[bucket_of(k[j], bits, shift)], 1u) : 0u;
+  }
+  __syncthreads();
+  // one global reservation per bucket this block holds keys of
+  for (uint32_t i = t; i < kBuckets; i += kBucketBlock)
+    if (s_c[i]) s_off[i] += atomicAdd(&bcur[i], s_c[i]);
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < kBucketItems; j++) {
+    const uint64_t i = base + j * kBucketBlock + t;
+    if (i >= count) continue;
+    const uint32_t pos = s_off[bucket_of(k[j], bits, shift)] + r[j];
+    if (pos < count) out[pos] = k[j];  // always, unless the counters were stale
+  }
+}
+
+// One workgroup per bucket sorts it in LDS (bitonic) and writes it back in
+// place; no workgroup waits on another.
+__global__ void __launch_bounds__(kBucketBlock) k_bucket_sort(uint64_t *keys, uint32_t count, const uint32_t *boff,
+                                                              uint32_t *bcnt, uint32_t *bcur, uint64_t *err) {
+  __shared__ uint64_t s[kBucketCap];
+  const uint32_t b = blockIdx.x, t = threadIdx.x;
+  const uint32_t start = boff[b], size = bcnt[b];
+  __syncthreads();  // every thread has read the count before it is cleared
+  if (t == 0) {
+    bcnt[b] = 0;
+    bcur[b] = 0;
+  }
+  // a bucket past the batch (stale counters) is treated as an overflow too
+  const bool fits = size <= kBucketCap && static_cast<uint64_t>(start) + size <= count;
+  if (!fits && t == 0) atomicOr(reinterpret_cast<unsigned long long *>(err), kSortOverflow);
+  const uint32_t n = fits ? size : 0;
+  if (n < 2) return;  // uniform over the workgroup
+  uint32_t P = 2;
+  while (P < n) P <<= 1;
+  for (uint32_t i = t; i < P; i += kBucketBlock) s[i] = i < n ? keys[start + i] : ~0ull;
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = t; i < P; i += kBucketBlock) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t x = s[i], y = s[ixj];
+          if ((x > y) == ((i & k) == 0)) {
+            s[i] = y;
+            s[ixj] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = t; i < n; i += kBucketBlock) keys[start + i] = s[i];
 }
 
 // The tier split of a small batch's keys (at most kSmallSortKeys, one
@@ -668,7 +768,7 @@ __global__ void k_place_kicked_old(const uint64_t *R, const uint32_t *nidx, cons
 __global__ void k_leaf_of(GView g, const uint64_t *keys, uint64_t k, uint32_t *leaf) {
   GRID_STRIDE(i, k) {
     const uint64_t key = keys[i];
-    leaf[i] = static_cast<uint32_t>(pma_find_leaf_from(g, key, g.vb[key_src(key)].aux));
+    leaf[i] = static_cast<uint32_t>(pma_leaf_of(g, key, g.vb[key_src(key)].aux));
   }
 }
 
@@ -764,87 +864,104 @@ __device__ inline void leaf_begin(const GView &g, uint32_t s, uint64_t base, uin
   else atomicMin(aux, static_cast<unsigned long long>(slot));
 }
 
-// One wave per touched leaf (the grid covers every possible leaf; waves
-// past the touched count exit): the leaf's elements (a leaf may hold holes
-// after deletes) are compacted by rank in LDS next to the leaf's new keys,
-// then merged (both sorted and disjoint) and written packed left, the rest
-// of the leaf emptied.
+// Sixteen lanes per touched leaf, four leaves per wave (the grid covers
+// every possible leaf; groups past the touched count exit).  Lane l of a
+// group reads slots 4l .. 4l+3; the leaf's elements (a leaf may hold holes
+// after deletes) are compacted by a 16-lane prefix into LDS next to the
+// leaf's new keys; then lane l finds the merge-path split for output slots
+// 4l .. 4l+3, merges its four outputs and writes them back as one 16-byte
+// store per array, packed left, the rest of the leaf emptied.
+constexpr uint32_t kMergeGroups = 16;  // leaves per 256-thread workgroup
+
 __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t *T, const uint32_t *Tstart,
                                                         const uint64_t *ntp, const uint64_t *keys,
                                                         const uint64_t *flag, uint64_t *err) {
   if (*flag) return;
   const uint32_t nt = static_cast<uint32_t>(*ntp);
-  if (static_cast<uint64_t>(blockIdx.x) * 4 >= nt) return;  // the whole workgroup is idle
-  __shared__ uint64_t s_old[4][kLeafSlots];
-  __shared__ uint64_t s_new[4][kLeafSlots];
-  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (static_cast<uint64_t>(blockIdx.x) * kMergeGroups >= nt) return;  // the whole workgroup is idle
+  __shared__ uint64_t s_old[kMergeGroups][kLeafSlots];
+  __shared__ uint64_t s_new[kMergeGroups][kLeafSlots];
   constexpr uint64_t kNone = ~0ull;
-  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 4 + w;
+  const uint32_t q = threadIdx.x >> 4, l = threadIdx.x & 15;
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kMergeGroups + q;
   const bool live = t < nt;
   uint64_t leaf = 0, base = 0;
   uint32_t nb = 0, c = 0;
-  uint64_t b = kNone;
+  uint4 sv = make_uint4(0, 0, 0, 0);
   if (live) {
     leaf = T[t];
     base = leaf << kLeafShift;
     const uint32_t kb = Tstart[t];
     nb = Tstart[t + 1] - kb;
-    const uint32_t d = g.pdst[base + lane];
-    const bool v = d != kEmpty;
-    const uint64_t m = __ballot(v);
-    c = static_cast<uint32_t>(__popcll(m));
-    if (lane == 0 && c != g.pcnt[leaf]) atomicOr(reinterpret_cast<unsigned long long *>(err), 2ull);
-    if (v) s_old[w][__popcll(m & lanemask_lt())] = make_key(g.psrc[base + lane], d);
-    if (lane < nb) {
-      b = keys[kb + lane];
-      s_new[w][lane] = b;
+    const uint4 dv = reinterpret_cast<const uint4 *>(g.pdst + base)[l];
+    sv = reinterpret_cast<const uint4 *>(g.psrc + base)[l];
+    const uint32_t dd[4] = {dv.x, dv.y, dv.z, dv.w}, ss[4] = {sv.x, sv.y, sv.z, sv.w};
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) cnt += dd[j] != kEmpty ? 1u : 0u;
+    uint32_t inc = cnt;
+    for (uint32_t o = 1; o < 16; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o, 16);
+      if (l >= o) inc += y;
     }
+    c = __shfl(inc, 15, 16);
+    uint32_t pos = inc - cnt;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++)
+      if (dd[j] != kEmpty) s_old[q][pos++] = make_key(ss[j], dd[j]);
+    if (l == 0 && c != g.pcnt[leaf]) atomicOr(reinterpret_cast<unsigned long long *>(err), 2ull);
+    for (uint32_t j = l; j < nb; j += 16) s_new[q][j] = keys[kb + j];
   }
   __syncthreads();
   if (!live) return;
-  // c + nb <= up[0] < 64: lane l holds old element l and new key l
-  const uint64_t a = lane < c ? s_old[w][lane] : kNone;
-  // new key b: old elements below it
-  uint32_t lo = 0, hi = c;
+  // c + nb <= up[0] < 64.  Merge-path split for output o0: i old elements
+  // and o0 - i new keys come before it (the two lists are disjoint).
+  const uint64_t *A = s_old[q], *B = s_new[q];
+  const uint32_t total = c + nb, o0 = 4 * l;
+  uint32_t lo = o0 > nb ? o0 - nb : 0, hi = o0 < c ? o0 : c;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (s_old[w][mid] < b) lo = mid + 1;
+    if (A[mid] < B[o0 - mid - 1]) lo = mid + 1;
     else hi = mid;
   }
-  if (lane < c) {
-    // old element a: new keys below it
-    uint32_t l2 = 0, h2 = nb;
-    while (l2 < h2) {
-      const uint32_t mid = (l2 + h2) >> 1;
-      if (s_new[w][mid] < a) l2 = mid + 1;
-      else h2 = mid;
+  uint32_t i = lo, j = o0 - lo;
+  // the output before o0: the larger of the last old and last new taken
+  uint64_t prev = kNone;
+  if (o0 > 0 && o0 <= total) {
+    const uint64_t pa = i > 0 ? A[i - 1] : 0, pb = j > 0 ? B[j - 1] : 0;
+    prev = pa > pb ? pa : pb;
+  }
+  uint64_t out[4];
+#pragma unroll
+  for (uint32_t r = 0; r < 4; r++) {
+    out[r] = kNone;
+    if (o0 + r < total) {
+      if (j >= nb || (i < c && A[i] < B[j])) out[r] = A[i++];
+      else out[r] = B[j++];
     }
-    const uint64_t slot = base + lane + l2;
-    // the element before a: the larger of a[lane - 1] and the last new key below a
-    const bool has_prev = lane > 0 || l2 > 0;
-    uint64_t prev = lane > 0 ? s_old[w][lane - 1] : 0;
-    if (l2 > 0 && (lane == 0 || s_new[w][l2 - 1] > prev)) prev = s_new[w][l2 - 1];
-    const uint32_t s = key_src(a);
-    g.psrc[slot] = s;
-    g.pdst[slot] = key_dst(a);
-    if (!has_prev || key_src(prev) != s) leaf_begin(g, s, base, slot);
   }
-  if (lane < nb) {
-    const uint64_t slot = base + lane + lo;
-    const bool has_prev = lane > 0 || lo > 0;
-    uint64_t prev = lane > 0 ? s_new[w][lane - 1] : 0;
-    if (lo > 0 && (lane == 0 || s_old[w][lo - 1] > prev)) prev = s_old[w][lo - 1];
-    const uint32_t s = key_src(b);
-    g.psrc[slot] = s;
-    g.pdst[slot] = key_dst(b);
-    if (!has_prev || key_src(prev) != s) leaf_begin(g, s, base, slot);
+  uint4 nd, ns;
+  nd.x = o0 + 0 < total ? key_dst(out[0]) : kEmpty;
+  nd.y = o0 + 1 < total ? key_dst(out[1]) : kEmpty;
+  nd.z = o0 + 2 < total ? key_dst(out[2]) : kEmpty;
+  nd.w = o0 + 3 < total ? key_dst(out[3]) : kEmpty;
+  ns.x = o0 + 0 < total ? key_src(out[0]) : sv.x;
+  ns.y = o0 + 1 < total ? key_src(out[1]) : sv.y;
+  ns.z = o0 + 2 < total ? key_src(out[2]) : sv.z;
+  ns.w = o0 + 3 < total ? key_src(out[3]) : sv.w;
+  reinterpret_cast<uint4 *>(g.pdst + base)[l] = nd;
+  reinterpret_cast<uint4 *>(g.psrc + base)[l] = ns;
+  // run starts in this leaf move the vertex's begin pointer
+#pragma unroll
+  for (uint32_t r = 0; r < 4; r++) {
+    if (o0 + r >= total) break;
+    const uint64_t before = r == 0 ? prev : out[r - 1];
+    const uint32_t sx = key_src(out[r]);
+    if (before == kNone || key_src(before) != sx) leaf_begin(g, sx, base, base + o0 + r);
   }
-  if (lane >= c + nb) g.pdst[base + lane] = kEmpty;
-  if (lane == 0) {
-    g.pcnt[leaf] = c + nb;
-    // fence: the smaller of the first old element and the first new key
-    const uint64_t a0 = c > 0 ? s_old[w][0] : kNone, b0 = nb > 0 ? s_new[w][0] : kNone;
-    g.fence[leaf] = a0 < b0 ? a0 : b0;
+  if (l == 0) {
+    g.pcnt[leaf] = total;
+    g.fence[leaf] = out[0];  // total >= 1: the leaf's first key
   }
 }
 
@@ -1984,8 +2101,10 @@ uint64_t Graph::compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t
 }
 
 // Pack, validate, sort and unique the batch into k1_ without a read back:
-// error bits in dsc_[0], the unique count in dsc_[1].
-void Graph::prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_t count) {
+// error bits in dsc_[0], the unique count in dsc_[1].  Returns true when
+// the bucket sort ran: a kSortOverflow bit in dsc_[0] then means the
+// caller must prepare again with radix = true.
+bool Graph::prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_t count, bool radix) {
   k0_.ensure(count, stream_);
   k1_.ensure(count + 1, stream_);
   if (count <= kSmallSortKeys) {
@@ -1993,23 +2112,37 @@ void Graph::prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_
     k_pack_sort_small<<<1, 1024, 0, stream_>>>(src, dst, static_cast<uint32_t>(count), n_, k1_.get(), dsc_.get(),
                                                dsc_.get() + 1);
     TG_CHECK_LAUNCH();
-    return;  // unique keys in k1_, their count in dsc_[1]
-  } else if (count <= kMidSortKeys) {
-    uint32_t bits = 1;
-    while (bits < 32 && (1ull << bits) < n_) bits++;
-    k_pack_sort_mid<<<1, 1024, 0, stream_>>>(src, dst, static_cast<uint32_t>(count), n_, bits, k1_.get(),
-                                             dsc_.get(), dsc_.get() + 1);
+    return false;  // unique keys in k1_, their count in dsc_[1]
+  }
+  uint32_t bits = 1;
+  while (bits < 32 && (1ull << bits) < n_) bits++;
+  fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
+  if (count <= kBucketSortMax && !radix) {
+    if (sbcnt_.ensure(kBuckets, stream_)) fill_u32(sbcnt_.get(), 0, kBuckets, stream_);
+    if (sbcur_.ensure(kBuckets, stream_)) fill_u32(sbcur_.get(), 0, kBuckets, stream_);
+    sboff_.ensure(kBuckets, stream_);
+    sbkeys_.ensure(count, stream_);
+    const uint32_t shift = 2 * bits > kBucketBits ? 2 * bits - kBucketBits : 0;
+    const unsigned nb = static_cast<unsigned>((count + kBucketTile - 1) / kBucketTile);
+    k_bucket_count<<<nb, kBucketBlock, 0, stream_>>>(src, dst, static_cast<uint32_t>(count), n_, bits, shift,
+                                                     k0_.get(), sbcnt_.get(), dsc_.get());
     TG_CHECK_LAUNCH();
-    return;
-  } else {
-    fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
+    k_bucket_scatter<<<nb, kBucketBlock, 0, stream_>>>(k0_.get(), static_cast<uint32_t>(count), bits, shift,
+                                                       sbcnt_.get(), sbcur_.get(), sboff_.get(), sbkeys_.get());
+    TG_CHECK_LAUNCH();
+    k_bucket_sort<<<kBuckets, kBucketBlock, 0, stream_>>>(sbkeys_.get(), static_cast<uint32_t>(count),
+                                                          sboff_.get(), sbcnt_.get(), sbcur_.get(), dsc_.get());
+    TG_CHECK_LAUNCH();
+    select(SelUnique{sbkeys_.get(), k1_.get()}, count, dsc_.get() + 1, scan_scratch, stream_);
+    return true;
+  }
+  {
     k_pack<<<grid_for(count, 256), 256, 0, stream_>>>(src, dst, count, n_, k0_.get(), dsc_.get());
     TG_CHECK_LAUNCH();
-    uint32_t bits = 1;
-    while (bits < 32 && (1ull << bits) < n_) bits++;
     radix_sort_keys(k0_.get(), count, bits, bits, sort_, stream_);
   }
   select(SelUnique{k0_.get(), k1_.get()}, count, dsc_.get() + 1, scan_scratch, stream_);
+  return false;
 }
 
 // Internal consistency flags raised by kernels during the batch.
@@ -2124,9 +2257,9 @@ void Graph::pma_fast_launch(const uint64_t *keys, uint64_t k) {
   select(SelTouched{u0_.get(), T.get(), Tstart.get()}, k, ntp, scan_scratch, stream_);
   k_touched_end<<<1, 1, 0, stream_>>>(Tstart.get(), ntp, static_cast<uint32_t>(k), fitp);
   TG_CHECK_LAUNCH();
-  // a wave per touched leaf (at most k of them), four per workgroup
+  // sixteen lanes per touched leaf (at most min(k, leaves) of them)
   const uint64_t most = k < (1ull << pma_H_) ? k : (1ull << pma_H_);
-  const uint64_t wg64 = (most + 3) / 4;
+  const uint64_t wg64 = (most + kMergeGroups - 1) / kMergeGroups;
   if (wg64 > 0x7FFFFFFFull) throw Error(-1, "PMA merge: too many keys");
   const unsigned wg = static_cast<unsigned>(wg64);
   k_pma_leaf_fit<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_.up[0], T.get(), Tstart.get(), ntp, fitp);
@@ -2425,26 +2558,31 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
   // sort, unique, existence and source runs sized by count; the unique count
   // stays on the device until the one read below
-  prepare_keys_async(src, dst, count);
-  const uint64_t *R = k1_.get();
-  GView g = view();
-  f0_.ensure(count + 1, stream_);
-  f1_.ensure(count + 1, stream_);
-  k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, R, dsc_.get() + 1, count, f0_.get(), f1_.get());
-  TG_CHECK_LAUNCH();
-  s32_[0].ensure(count + 1, stream_);
+  const uint64_t *R = nullptr;
   const bool small = count <= kSmallSortKeys;
-  if (small) {
-    // new-key prefix, source runs and run_beg[nruns] = nR in one workgroup
-    k_small_runs<true><<<1, 1024, 0, stream_>>>(R, f1_.get(), static_cast<uint32_t>(count), nullptr,
-                                                s32_[0].get(), dsc_.get());
-    TG_CHECK_LAUNCH();
-  } else {
-    exclusive_scan_u32(f1_.get(), f1_.get(), count, f1_.get() + count, scan_scratch, stream_);
-    select(SelRunStartN{R, s32_[0].get(), dsc_.get() + 1}, count, dsc_.get() + 2, scan_scratch, stream_);
-  }
   uint64_t h[3];
-  read_scalars(dsc_.get(), h, 3, stream_);
+  // a bucket sort that overflowed is redone by radix (second pass)
+  for (int pass = 0; pass < 2; pass++) {
+    const bool bucket = prepare_keys_async(src, dst, count, pass == 1);
+    R = k1_.get();  // after the prepare, which sizes k1_
+    GView g = view();
+    f0_.ensure(count + 1, stream_);
+    f1_.ensure(count + 1, stream_);
+    k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, R, dsc_.get() + 1, count, f0_.get(), f1_.get());
+    TG_CHECK_LAUNCH();
+    s32_[0].ensure(count + 1, stream_);
+    if (small) {
+      // new-key prefix, source runs and run_beg[nruns] = nR in one workgroup
+      k_small_runs<true><<<1, 1024, 0, stream_>>>(R, f1_.get(), static_cast<uint32_t>(count), nullptr,
+                                                  s32_[0].get(), dsc_.get());
+      TG_CHECK_LAUNCH();
+    } else {
+      exclusive_scan_u32(f1_.get(), f1_.get(), count, f1_.get() + count, scan_scratch, stream_);
+      select(SelRunStartN{R, s32_[0].get(), dsc_.get() + 1}, count, dsc_.get() + 2, scan_scratch, stream_);
+    }
+    read_scalars(dsc_.get(), h, 3, stream_);
+    if (!bucket || !(h[0] & kSortOverflow)) break;
+  }
   if (h[0] & 1) throw Error(-1, "vertex id out of range");
   if (h[1] == 0) return;
   if (!small) {
@@ -2604,25 +2742,30 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
   // sort, unique, the existing keys and their source runs sized by count;
   // the three counts come back in one read
-  prepare_keys_async(src, dst, count);
-  GView g = view();
-  k2_.ensure(count + 1, stream_);
-  f0_.ensure(count + 1, stream_);
-  k_exists_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, k1_.get(), dsc_.get() + 1, count, f0_.get());
-  TG_CHECK_LAUNCH();
-  s32_[0].ensure(count + 1, stream_);
   const bool small = count <= kSmallSortKeys;
-  if (small) {
-    // existing keys, their source runs and run_beg[nruns] in one workgroup
-    k_small_runs<false><<<1, 1024, 0, stream_>>>(k1_.get(), f0_.get(), static_cast<uint32_t>(count), k2_.get(),
-                                                 s32_[0].get(), dsc_.get());
-    TG_CHECK_LAUNCH();
-  } else {
-    select(SelFlagged{k1_.get(), f0_.get(), k2_.get()}, count, dsc_.get() + 2, scan_scratch, stream_);
-    select(SelRunStartN{k2_.get(), s32_[0].get(), dsc_.get() + 2}, count, dsc_.get() + 3, scan_scratch, stream_);
-  }
   uint64_t h[4];
-  read_scalars(dsc_.get(), h, 4, stream_);
+  // a bucket sort that overflowed is redone by radix (second pass)
+  for (int pass = 0; pass < 2; pass++) {
+    const bool bucket = prepare_keys_async(src, dst, count, pass == 1);
+    GView g = view();
+    k2_.ensure(count + 1, stream_);
+    f0_.ensure(count + 1, stream_);
+    k_exists_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, k1_.get(), dsc_.get() + 1, count, f0_.get());
+    TG_CHECK_LAUNCH();
+    s32_[0].ensure(count + 1, stream_);
+    if (small) {
+      // existing keys, their source runs and run_beg[nruns] in one workgroup
+      k_small_runs<false><<<1, 1024, 0, stream_>>>(k1_.get(), f0_.get(), static_cast<uint32_t>(count), k2_.get(),
+                                                   s32_[0].get(), dsc_.get());
+      TG_CHECK_LAUNCH();
+    } else {
+      select(SelFlagged{k1_.get(), f0_.get(), k2_.get()}, count, dsc_.get() + 2, scan_scratch, stream_);
+      select(SelRunStartN{k2_.get(), s32_[0].get(), dsc_.get() + 2}, count, dsc_.get() + 3, scan_scratch,
+             stream_);
+    }
+    read_scalars(dsc_.get(), h, 4, stream_);
+    if (!bucket || !(h[0] & kSortOverflow)) break;
+  }
   if (h[0] & 1) throw Error(-1, "vertex id out of range");
   if (h[2] == 0) return;
   if (!small) {

@@ -209,10 +209,15 @@ class Graph {
   DevBuf<uint32_t> s32_[32];
   DevBuf<uint8_t> s8_[8];
   DevBuf<uint64_t> s64_[4];
+  // bucket sort of mid-size batches: per-bucket counts and cursors (zero
+  // between calls), bucket starts, the bucketed keys
+  DevBuf<uint32_t> sbcnt_, sbcur_, sboff_;
+  DevBuf<uint64_t> sbkeys_;
 
   void init_pma(uint64_t nleaves);
   void compute_bounds();
-  void prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_t count);
+  // true when the bucket sort ran (its overflow bit may then be set)
+  bool prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_t count, bool radix = false);
   void insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
   void delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);

@@ -332,29 +332,39 @@ __global__ void __launch_bounds__(kSortBlock) k_onesweep(const uint64_t *keys, u
   uint64_t tile = static_cast<uint64_t>(b) * kSortTile;
   uint64_t k[kSortItems];
   uint32_t rank[kSortItems];
+  int lead[kSortItems];
   const uint64_t lt = lanemask_lt();
+  // all key loads issued first
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    uint64_t idx = sort_index(tile, w, j, lane);
+    k[j] = idx < n ? keys[idx] : 0;
+  }
+  // each group of equal digits takes its place with one LDS atomic by its
+  // leader: a wave's atomics complete in issue order, so ranks follow the
+  // index order without one step waiting on the one before
 #pragma unroll
   for (int j = 0; j < kSortItems; j++) {
     uint64_t idx = sort_index(tile, w, j, lane);
     bool valid = idx < n;
-    k[j] = valid ? keys[idx] : 0;
     uint32_t d = static_cast<uint32_t>((k[j] >> shift) & 0xFF);
-    uint64_t active = __ballot(valid);
-    uint64_t peers = active;
+    uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int bb = 0; bb < 8; bb++) {
       uint64_t m = __ballot((d >> bb) & 1u);
       peers &= ((d >> bb) & 1u) ? m : ~m;
     }
-    uint32_t r = __popcll(peers & lt);
-    int leader = valid ? __ffsll(static_cast<unsigned long long>(peers)) - 1 : 0;
-    uint32_t base = 0;
-    if (valid && lane == leader) {
-      base = wave_cnt[w][d];
-      wave_cnt[w][d] = base + static_cast<uint32_t>(__popcll(peers));
-    }
-    base = __shfl(base, leader, 64);
-    rank[j] = base + r;
+    lead[j] = valid ? __ffsll(static_cast<unsigned long long>(peers)) - 1 : 0;
+    uint32_t old = 0;
+    if (valid && lane == static_cast<uint32_t>(lead[j]))
+      old = atomicAdd(&wave_cnt[w][d], static_cast<uint32_t>(__popcll(peers)));
+    rank[j] = old + static_cast<uint32_t>(__popcll(peers & lt));  // the leader's lt-count is 0
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    const uint32_t r = rank[j];
+    const uint32_t base = __shfl(r, lead[j], 64);
+    rank[j] = lane == static_cast<uint32_t>(lead[j]) ? r : base + r;
   }
   __syncthreads();
   const uint32_t d = threadIdx.x;  // 256 threads == 256 digits

@@ -83,11 +83,17 @@ __device__ inline uint64_t word(uint32_t epoch, uint32_t status, uint32_t value)
   return (static_cast<uint64_t>(epoch) << 34) | (static_cast<uint64_t>(status) << 32) | value;
 }
 
+// Launches of at most this many blocks use the block index as their order:
+// the dispatcher starts workgroups in index order, so a block only waits on
+// blocks already started, and the ticket's atomic round trip is saved.
+constexpr uint32_t kOrderedBlocks = 1024;
+
 // Block ticket in start order (one thread takes it, the block shares it).
 // The block that takes the last ticket resets the counter for the next
 // launch on the stream.  Returns nblocks when the counter was inconsistent
 // (the caller skips its work and the error word is set).
 __device__ inline uint32_t ticket(uint32_t *misc, uint32_t *err, uint32_t nblocks, uint32_t *lds_slot) {
+  if (nblocks <= kOrderedBlocks) return blockIdx.x;
   if (threadIdx.x == 0) {
     uint32_t v = atomicAdd(&misc[0], 1u);
     if (v == nblocks - 1) atomicExch(&misc[0], 0u);

@@ -283,13 +283,15 @@ def test_api_edge_cases():
     G.close()
 
 
-def test_mid_size_batches_sorted_in_one_workgroup():
-    """Batches of 4097..16384 raw pairs are packed, sorted and de-duplicated
-    by one workgroup (LDS radix sort); 16385 takes the multi-workgroup sort.
-    Unsorted pairs with repeats, ids up to n - 1 on a vertex count that is
-    not a power of two (17-bit digits), inserts and deletes (some of absent
-    edges) match the set model after every batch, and a mid-size batch
-    holding one bad id is rejected without changing the graph."""
+def test_mid_size_batches_bucket_sort():
+    """Batches of 4097..2^18 raw pairs are sorted by 4096 buckets of the
+    key's top bits, each bucket sorted and de-duplicated in LDS.  Unsorted
+    pairs with repeats, ids up to n - 1 on a vertex count that is not a
+    power of two, inserts and deletes (some of absent edges) match the set
+    model after every batch; batches whose keys share one source (one
+    bucket beyond its LDS capacity) are sorted again by radix and match
+    too; a mid-size batch holding one bad id is rejected without changing
+    the graph."""
     n = 100_003
     say("mid-size batches")
     rng = np.random.default_rng(41)
@@ -311,6 +313,15 @@ def test_mid_size_batches_sorted_in_one_workgroup():
             G.remove_edge_batch(*split(raw))
             keys = oracle.set_difference(keys, oracle.sort_unique(raw))
         assert_adjacency(G, keys, n, f"mid-size batch {rnd} ({size})")
+    for rnd, hub in enumerate([7, n - 1]):  # one bucket holds the whole batch
+        d = rng.permutation(n)[:12000].astype(np.uint64)
+        raw = (np.uint64(hub) << np.uint64(32)) | d
+        G.add_edge_batch(*split(raw))
+        keys = oracle.set_union(keys, oracle.sort_unique(raw))
+        assert_adjacency(G, keys, n, f"one-source insert {rnd}")
+        G.remove_edge_batch(*split(raw[::2]))
+        keys = oracle.set_difference(keys, oracle.sort_unique(raw[::2]))
+        assert_adjacency(G, keys, n, f"one-source delete {rnd}")
     m = G.get_num_edges()
     bs, bd = split(keys[:10000])
     bs = bs.copy()
