@@ -657,7 +657,8 @@ __global__ void k_exists_n(GView g, const uint64_t *keys, const uint64_t *np, ui
 __global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
                                const uint32_t *run_beg, uint32_t nruns, uint8_t *run_route,
                                uint32_t *run_c, uint32_t *run_nko, uint32_t *kick_old, uint32_t *promote,
-                               uint32_t *old_deg) {
+                               uint32_t *old_deg, uint8_t *kroute, uint64_t nR) {
+  GRID_STRIDE(i, nR) kroute[i] = kRouteNone;  // the kicked keys' routes start empty
   GRID_STRIDE(r, nruns) {
     uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
     uint32_t s = key_src(R[b0]);
@@ -708,6 +709,27 @@ __global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, c
   }
 }
 
+// The old inline entries a run pushed out, placed among its kicked keys.
+__device__ inline void place_kicked_old(uint32_t r, const uint64_t *R, const uint32_t *nidx,
+                                        const uint32_t *run_beg, const uint8_t *run_route, const uint32_t *run_c,
+                                        const uint32_t *run_nko, const uint32_t *kick_old, uint64_t *kick,
+                                        uint8_t *kroute) {
+  uint8_t route = run_route[r];
+  uint32_t nko = run_nko[r];
+  if (route == kRouteNone || nko == 0) return;
+  uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
+  uint32_t s = key_src(R[b0]);
+  uint32_t c = run_c[r];
+  for (uint32_t e = 0; e < nko; e++) {
+    uint32_t x = kick_old[static_cast<uint64_t>(r) * kInline + e];
+    uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, make_key(s, x));
+    uint32_t new_less = nidx[lb] - nidx[b0] - c;  // new tail keys below x
+    uint64_t pos = b0 + e + new_less;
+    kick[pos] = make_key(s, x);
+    kroute[pos] = route;
+  }
+}
+
 // Place the kicked keys of every run, sorted, into kick[run_beg ..]:
 // new keys beyond the first c of the run, merged with the old inline
 // entries that were pushed out.
@@ -740,32 +762,14 @@ __global__ void k_place_kicked(const uint64_t *R, const uint32_t *ex, const uint
     kick[pos] = R[i];
     kroute[pos] = route;
   }
-}
-
-__global__ void k_place_kicked_old(const uint64_t *R, const uint32_t *nidx, const uint32_t *run_beg,
-                                   uint32_t nruns, const uint8_t *run_route, const uint32_t *run_c,
-                                   const uint32_t *run_nko, const uint32_t *kick_old, uint64_t *kick,
-                                   uint8_t *kroute) {
-  GRID_STRIDE(r, nruns) {
-    uint8_t route = run_route[r];
-    uint32_t nko = run_nko[r];
-    if (route == kRouteNone || nko == 0) continue;
-    uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
-    uint32_t s = key_src(R[b0]);
-    uint32_t c = run_c[r];
-    for (uint32_t e = 0; e < nko; e++) {
-      uint32_t x = kick_old[static_cast<uint64_t>(r) * kInline + e];
-      uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, make_key(s, x));
-      uint32_t new_less = nidx[lb] - nidx[b0] - c;  // new tail keys below x
-      uint64_t pos = b0 + e + new_less;
-      kick[pos] = make_key(s, x);
-      kroute[pos] = route;
-    }
-  }
+  // and the old inline entries each run pushed out (their own positions)
+  GRID_STRIDE(r, nruns) place_kicked_old(static_cast<uint32_t>(r), R, nidx, run_beg, run_route, run_c, run_nko,
+                                         kick_old, kick, kroute);
 }
 
 // ------------------------------------------------------------ PMA kernels
-__global__ void k_leaf_of(GView g, const uint64_t *keys, uint64_t k, uint32_t *leaf) {
+__global__ void k_leaf_of(GView g, const uint64_t *keys, uint64_t k, uint32_t *leaf, uint64_t *fit_flag) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *fit_flag = 0;  // set later by k_pma_leaf_fit
   GRID_STRIDE(i, k) {
     const uint64_t key = keys[i];
     leaf[i] = static_cast<uint32_t>(pma_leaf_of(g, key, g.vb[key_src(key)].aux));
@@ -837,17 +841,13 @@ __global__ void k_clear_marks(GView g, const uint32_t *T, const uint64_t *ntp, c
 // them).  The merge does nothing when a leaf does not fit; the window path
 // then runs.
 
-// Tstart[nt] = k and a cleared fit flag (one thread)
-__global__ void k_touched_end(uint32_t *Tstart, const uint64_t *ntp, uint32_t k, uint64_t *flag) {
-  Tstart[*ntp] = k;
-  *flag = 0;
-}
-
-__global__ void k_pma_leaf_fit(GView g, uint64_t up0, const uint32_t *T, const uint32_t *Tstart, const uint64_t *ntp,
-                               uint64_t *flag) {
+// Also closes Tstart (Tstart[nt] = k) for the kernels after it.
+__global__ void k_pma_leaf_fit(GView g, uint64_t up0, const uint32_t *T, uint32_t *Tstart, const uint64_t *ntp,
+                               uint32_t k, uint64_t *flag) {
   const uint32_t nt = static_cast<uint32_t>(*ntp);
+  if (blockIdx.x == 0 && threadIdx.x == 0) Tstart[nt] = k;
   GRID_STRIDE(t, nt) {
-    const uint64_t newc = Tstart[t + 1] - Tstart[t];
+    const uint64_t newc = (t + 1 < nt ? Tstart[t + 1] : k) - Tstart[t];
     if (g.pcnt[T[t]] + newc > up0) atomicOr(reinterpret_cast<unsigned long long *>(flag), 1ull);
   }
 }
@@ -2252,17 +2252,16 @@ void Graph::pma_fast_launch(const uint64_t *keys, uint64_t k) {
   T.ensure(k + 1, stream_);
   Tstart.ensure(k + 1, stream_);
   uint64_t *ntp = dsc_.get() + 8, *fitp = dsc_.get() + 7;
-  k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get());
+  k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get(), fitp);
   TG_CHECK_LAUNCH();
   select(SelTouched{u0_.get(), T.get(), Tstart.get()}, k, ntp, scan_scratch, stream_);
-  k_touched_end<<<1, 1, 0, stream_>>>(Tstart.get(), ntp, static_cast<uint32_t>(k), fitp);
-  TG_CHECK_LAUNCH();
   // sixteen lanes per touched leaf (at most min(k, leaves) of them)
   const uint64_t most = k < (1ull << pma_H_) ? k : (1ull << pma_H_);
   const uint64_t wg64 = (most + kMergeGroups - 1) / kMergeGroups;
   if (wg64 > 0x7FFFFFFFull) throw Error(-1, "PMA merge: too many keys");
   const unsigned wg = static_cast<unsigned>(wg64);
-  k_pma_leaf_fit<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_.up[0], T.get(), Tstart.get(), ntp, fitp);
+  k_pma_leaf_fit<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_.up[0], T.get(), Tstart.get(), ntp,
+                                                         static_cast<uint32_t>(k), fitp);
   TG_CHECK_LAUNCH();
   k_pma_leaf_merge<<<wg, 256, 0, stream_>>>(g, T.get(), Tstart.get(), ntp, keys, fitp, dsc_.get());
   TG_CHECK_LAUNCH();
@@ -2634,21 +2633,17 @@ void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   // the old degrees of promoted vertices are needed after the merge
   DevBuf<uint32_t> &old_deg = s32_[7];
   old_deg.ensure(nruns + 1, stream_);
-  k_inline_merge<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, ex, nidx, run_beg.get(), nruns, run_route.get(),
-                                                            run_c.get(), run_nko.get(), kick_old.get(),
-                                                            promote.get(), old_deg.get());
+  // (also clears the kicked keys' routes)
+  k_inline_merge<<<grid_for(nR, 256), 256, 0, stream_>>>(g, R, ex, nidx, run_beg.get(), nruns, run_route.get(),
+                                                         run_c.get(), run_nko.get(), kick_old.get(), promote.get(),
+                                                         old_deg.get(), kroute.get(), nR);
   TG_CHECK_LAUNCH();
-  // kicked keys, sorted per run, placed in kick buffer k2_
+  // kicked keys, sorted per run, placed in kick buffer k2_ (only routed
+  // entries are read back)
   k2_.ensure(nR + 1, stream_);
-  fill_u64(k2_.get(), kInvalidKey, nR, stream_);
-  TG_HIP(hipMemsetAsync(kroute.get(), 0, nR, stream_));
   k_place_kicked<<<grid_for(nR, 256), 256, 0, stream_>>>(R, ex, nidx, nR, run_beg.get(), nruns, run_route.get(),
                                                          run_c.get(), run_nko.get(), kick_old.get(), k2_.get(),
                                                          kroute.get());
-  TG_CHECK_LAUNCH();
-  k_place_kicked_old<<<grid_for(nruns, 256), 256, 0, stream_>>>(R, nidx, run_beg.get(), nruns, run_route.get(),
-                                                                run_c.get(), run_nko.get(), kick_old.get(), k2_.get(),
-                                                                kroute.get());
   TG_CHECK_LAUNCH();
   // split kicked keys by tier and count promotions: one host read
   k3_.ensure(nR + 1, stream_);
