@@ -1007,14 +1007,6 @@ struct SelRunStartN {
   __device__ bool pred(uint64_t i) const { return i < *np && (i == 0 || key_src(k[i]) != key_src(k[i - 1])); }
   __device__ void emit(uint64_t i, uint64_t pos) const { beg[pos] = static_cast<uint32_t>(i); }
 };
-struct SelRoute {
-  const uint64_t *k;
-  const uint8_t *route;
-  uint8_t want;
-  uint64_t *out;
-  __device__ bool pred(uint64_t i) const { return route[i] == want; }
-  __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = k[i]; }
-};
 struct SelFlagged {
   const uint64_t *k;
   const uint32_t *flag;
@@ -1029,17 +1021,30 @@ struct SelExisting {
   __device__ bool pred(uint64_t i) const { return edge_exists(g, key_src(k[i]), key_dst(k[i])); }
   __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = k[i]; }
 };
-// vertices of one secondary tier: B-tree (tree) or PMA (more than the
-// inline slots and no tree)
-struct SelTier {
+// Both tier lists in one pass over the vertex blocks: 1 = PMA (more than
+// the inline slots and no tree), 2 = B-tree.
+struct SelTiers {
   const VBlock *vb;
-  bool tree;
-  uint32_t *out;
-  __device__ bool pred(uint64_t i) const {
+  uint32_t *pma, *tree;
+  __device__ uint32_t pick(uint64_t i) const {
     const uint64_t aux = vb[i].aux;
-    return tree ? (aux & kAuxTree) != 0 : ((aux & kAuxTree) == 0 && vb[i].degree > kInline);
+    if (aux & kAuxTree) return 2u;
+    return vb[i].degree > kInline ? 1u : 0u;
   }
-  __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = static_cast<uint32_t>(i); }
+  __device__ void emit(uint64_t i, uint32_t which, uint64_t pos) const {
+    (which == 1 ? pma : tree)[pos] = static_cast<uint32_t>(i);
+  }
+};
+// Keys of the two secondary tiers: 1 = PMA (into P), 2 = B-tree (into T).
+struct SelRoutes {
+  const uint64_t *k;
+  const uint8_t *route;
+  uint64_t *P, *T;
+  __device__ uint32_t pick(uint64_t i) const {
+    const uint8_t r = route[i];
+    return r == kRoutePma ? 1u : (r == kRouteTree ? 2u : 0u);
+  }
+  __device__ void emit(uint64_t i, uint32_t which, uint64_t pos) const { (which == 1 ? P : T)[pos] = k[i]; }
 };
 struct SelTouched {
   const uint32_t *leaf;
@@ -2653,8 +2658,8 @@ void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
                                            promote.get(), nruns, ppos.get(), dsc_.get() + 3);
     TG_CHECK_LAUNCH();
   } else {
-    select(SelRoute{k2_.get(), kroute.get(), kRoutePma, k3_.get()}, nR, dsc_.get() + 3, scan_scratch, stream_);
-    select(SelRoute{k2_.get(), kroute.get(), kRouteTree, g0_.get()}, nR, dsc_.get() + 4, scan_scratch, stream_);
+    select2(SelRoutes{k2_.get(), kroute.get(), k3_.get(), g0_.get()}, nR, dsc_.get() + 3, dsc_.get() + 4,
+            scan_scratch, stream_);
     exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_);
     k_u32_to_u64<<<1, 1, 0, stream_>>>(ppos.get() + nruns, dsc_.get() + 5);
     TG_CHECK_LAUNCH();
@@ -2815,8 +2820,8 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
                                            run_bump.get(), nruns, bump_off.get(), dsc_.get() + 4);
     TG_CHECK_LAUNCH();
   } else {
-    select(SelRoute{R, sec_route.get(), kRoutePma, k3_.get()}, nR, dsc_.get() + 4, scan_scratch, stream_);
-    select(SelRoute{R, sec_route.get(), kRouteTree, g0_.get()}, nR, dsc_.get() + 5, scan_scratch, stream_);
+    select2(SelRoutes{R, sec_route.get(), k3_.get(), g0_.get()}, nR, dsc_.get() + 4, dsc_.get() + 5, scan_scratch,
+            stream_);
     exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_);
     k_u32_to_u64<<<1, 1, 0, stream_>>>(bump_off.get() + nruns, dsc_.get() + 6);
     TG_CHECK_LAUNCH();
@@ -2854,10 +2859,8 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
                                                btree.get(), nullptr, 0, none.get(), dsc_.get() + 4);
         TG_CHECK_LAUNCH();
       } else {
-        select(SelRoute{bkeys.get(), broute.get(), kRoutePma, bpma.get()}, nb, dsc_.get() + 4, scan_scratch,
-               stream_);
-        select(SelRoute{bkeys.get(), broute.get(), kRouteTree, btree.get()}, nb, dsc_.get() + 5, scan_scratch,
-               stream_);
+        select2(SelRoutes{bkeys.get(), broute.get(), bpma.get(), btree.get()}, nb, dsc_.get() + 4, dsc_.get() + 5,
+                scan_scratch, stream_);
       }
       read_scalars(dsc_.get(), hs, kScalars, stream_);
       if (!tree_pending) break;
@@ -3019,8 +3022,8 @@ const Graph::TierLists &Graph::tier_lists() {
   // by n), both counts in one read
   list_pma_.ensure(static_cast<uint64_t>(n_) + 1, stream_);
   list_tree_.ensure(static_cast<uint64_t>(n_) + 1, stream_);
-  select(SelTier{vb_.get(), false, list_pma_.get()}, n_, dsc_.get() + 11, scan_scratch, stream_);
-  select(SelTier{vb_.get(), true, list_tree_.get()}, n_, dsc_.get() + 12, scan_scratch, stream_);
+  select2(SelTiers{vb_.get(), list_pma_.get(), list_tree_.get()}, n_, dsc_.get() + 11, dsc_.get() + 12, scan_scratch,
+          stream_);
   uint64_t h[2];
   read_scalars(dsc_.get() + 11, h, 2, stream_);
   const uint32_t np = static_cast<uint32_t>(h[0]), nt = static_cast<uint32_t>(h[1]);

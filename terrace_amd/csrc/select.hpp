@@ -126,6 +126,66 @@ __global__ void __launch_bounds__(kBlock) k_select_lb(F f, uint64_t n, uint64_t 
   }
 }
 
+// Two outputs in one pass: F::pick(i) gives 0 (neither), 1 or 2, and
+// F::emit(i, which, pos) writes to output `which` (1 or 2).  Each output
+// has its own look-back column (state stride 2); wave 0 walks column 0 and
+// wave 1 column 1 at the same time.
+template <class F>
+__global__ void __launch_bounds__(kBlock) k_select2_lb(F f, uint64_t n, uint64_t *total1, uint64_t *total2,
+                                                       uint64_t *state, uint32_t *misc, uint32_t *err,
+                                                       uint32_t nblocks, uint32_t epoch) {
+  __shared__ uint32_t wcnt[2][kItems][kBlock / 64];
+  __shared__ uint32_t wpre[2][kItems][kBlock / 64];
+  __shared__ uint32_t s_ticket, s_run[2];
+  __shared__ uint64_t s_off[2];
+  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
+  if (b >= nblocks) return;
+  const uint32_t w = threadIdx.x >> 6;
+  const uint64_t lt = lanemask_lt();
+  uint64_t base = static_cast<uint64_t>(b) * kTile;
+  uint32_t c[kItems], r[kItems];
+#pragma unroll
+  for (int i = 0; i < kItems; i++) {
+    uint64_t idx = base + static_cast<uint64_t>(i) * kBlock + threadIdx.x;
+    c[i] = idx < n ? f.pick(idx) : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; i++) {
+    const uint64_t m1 = __ballot(c[i] == 1), m2 = __ballot(c[i] == 2);
+    r[i] = static_cast<uint32_t>(__popcll((c[i] == 2 ? m2 : m1) & lt));
+    if ((threadIdx.x & 63) == 0) {
+      wcnt[0][i][w] = static_cast<uint32_t>(__popcll(m1));
+      wcnt[1][i][w] = static_cast<uint32_t>(__popcll(m2));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 || threadIdx.x == 64) {
+    const uint32_t col = threadIdx.x >> 6;
+    uint32_t run = 0;
+    for (int i = 0; i < kItems; i++)
+      for (int ww = 0; ww < kBlock / 64; ww++) {
+        wpre[col][i][ww] = run;
+        run += wcnt[col][i][ww];
+      }
+    const uint64_t off = lbk::lookback(state, 2, b, col, err, epoch, run);
+    s_off[col] = off;
+    s_run[col] = run;
+  }
+  __syncthreads();
+  if (b == nblocks - 1 && threadIdx.x == 0) {
+    if (total1) *total1 = s_off[0] + s_run[0];
+    if (total2) *total2 = s_off[1] + s_run[1];
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; i++) {
+    if (c[i]) {
+      const uint32_t col = c[i] - 1;
+      uint64_t idx = base + static_cast<uint64_t>(i) * kBlock + threadIdx.x;
+      f.emit(idx, c[i], s_off[col] + wpre[col][i][w] + r[i]);
+    }
+  }
+}
+
 }  // namespace sel
 
 // Compacts the indices with f.pred into f.emit positions; *total_dev gets
@@ -156,6 +216,23 @@ void select(const F &f, uint64_t n, uint64_t *total_dev, ScanScratch &ss, hipStr
   TG_CHECK_LAUNCH();
   exclusive_scan_u64(sums.get(), sums.get(), nblocks, nullptr, ss, s);
   sel::k_emit<F><<<static_cast<unsigned>(nblocks), sel::kBlock, 0, s>>>(f, n, sums.get(), total_dev);
+  TG_CHECK_LAUNCH();
+}
+
+// Two compactions in one pass (see k_select2_lb); the totals go to two
+// device scalars.  Needs the look-back scratch.
+template <class F>
+void select2(const F &f, uint64_t n, uint64_t *total1, uint64_t *total2, ScanScratch &ss, hipStream_t s) {
+  if (!ss.lookback || n > 0xFFFFFFFFull) throw Error(-1, "select2: needs the look-back scratch and n < 2^32");
+  if (n == 0) {
+    fill_u64(total1, 0, 1, s);
+    fill_u64(total2, 0, 1, s);
+    return;
+  }
+  const uint32_t nb = static_cast<uint32_t>((n + sel::kTile - 1) / sel::kTile);
+  const uint32_t epoch = ss.lb.begin(2ull * nb, s);
+  sel::k_select2_lb<F><<<nb, sel::kBlock, 0, s>>>(f, n, total1, total2, ss.lb.state.get(), ss.lb.misc.get(),
+                                                  ss.lb.err_word(), nb, epoch);
   TG_CHECK_LAUNCH();
 }
 
