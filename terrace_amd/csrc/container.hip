@@ -654,6 +654,92 @@ __global__ void k_exists_n(GView g, const uint64_t *keys, const uint64_t *np, ui
 // Writes the new inline slots and degree, the run's route, and the old
 // inline entries that are kicked out (kick_old[run*13 ..], count in
 // run_nko).
+// k_inline_merge for a run of at most kMergeRun keys, in registers: the
+// run's keys and new-key flags are loaded at once, and every inline entry
+// and new key computes its rank in the merged order (inline and new keys
+// are disjoint); ranks below kInline are the new inline slots, the old
+// entries ranked beyond are kicked out (they are the largest old ones).
+// Same outputs as the general loop below.
+constexpr uint32_t kMergeRun = 16;
+
+__device__ inline void inline_merge_short(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
+                                          uint32_t b0, uint32_t b1, uint32_t s, uint64_t r, uint8_t *run_route,
+                                          uint32_t *run_c, uint32_t *run_nko, uint32_t *kick_old,
+                                          uint32_t *promote, uint32_t *old_deg) {
+  const uint4 *vp = reinterpret_cast<const uint4 *>(g.vb + s);
+  const uint4 q0 = vp[0], q1 = vp[1], q2 = vp[2], q3 = vp[3];
+  const uint32_t w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+  const uint32_t L = b1 - b0;
+  uint32_t rd[kMergeRun];
+  bool rnew[kMergeRun];
+#pragma unroll
+  for (uint32_t j = 0; j < kMergeRun; j++) {
+    rd[j] = j < L ? key_dst(R[b0 + j]) : 0u;
+    rnew[j] = j < L && ex[b0 + j] == 0;
+  }
+  const uint32_t D = w[0];
+  old_deg[r] = D;  // promotions need the degree before the merge
+  const uint32_t lim = D < kInline ? D : kInline;
+  const bool is_tree = (w[15] & 0x80000000u) != 0;  // aux bit 63
+  uint32_t nnew = 0, dup_inline = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kMergeRun; j++) {
+    nnew += rnew[j] ? 1u : 0u;
+    bool in = false;
+#pragma unroll
+    for (uint32_t p = 0; p < kInline; p++) in |= p < lim && w[1 + p] == rd[j];
+    dup_inline += (j < L && !rnew[j] && in) ? 1u : 0u;
+  }
+  // k = |inline ∪ run| as the reference counts it (graph.h:290-330)
+  const uint64_t size = L;
+  const uint64_t kk = static_cast<uint64_t>(lim) + size - dup_inline;
+  const bool overflow = (D == 0) ? (size > kInline) : (kk > kInline);
+  uint8_t route = kRouteNone;
+  if (overflow) route = (is_tree || static_cast<uint64_t>(D) + size > kMediumDegree) ? kRouteTree : kRoutePma;
+  VBlock &vb = g.vb[s];
+  // old inline entries: rank = p + new keys below; the kept ones are a prefix
+  uint32_t rank_old[kInline];
+  uint32_t kept_old = 0;
+#pragma unroll
+  for (uint32_t p = 0; p < kInline; p++) {
+    uint32_t rk = p;
+#pragma unroll
+    for (uint32_t j = 0; j < kMergeRun; j++) rk += (rnew[j] && rd[j] < w[1 + p]) ? 1u : 0u;
+    rank_old[p] = rk;
+    kept_old += (p < lim && rk < kInline) ? 1u : 0u;
+  }
+#pragma unroll
+  for (uint32_t p = 0; p < kInline; p++) {
+    if (p >= lim) continue;
+    if (rank_old[p] < kInline) {
+      if (rank_old[p] != p) vb.nbr[rank_old[p]] = w[1 + p];
+    } else {
+      kick_old[r * kInline + (p - kept_old)] = w[1 + p];
+    }
+  }
+  // new keys: rank = inline entries below + new keys before it in the run
+  uint32_t c = 0, before = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kMergeRun; j++) {
+    if (!rnew[j]) continue;
+    uint32_t rk = before;
+#pragma unroll
+    for (uint32_t p = 0; p < kInline; p++) rk += (p < lim && w[1 + p] < rd[j]) ? 1u : 0u;
+    if (rk < kInline) {
+      vb.nbr[rk] = rd[j];
+      c++;
+    }
+    before++;
+  }
+  vb.degree = D + nnew;
+  run_route[r] = route;
+  run_c[r] = c;
+  run_nko[r] = lim - kept_old;
+  promote[r] = (route == kRouteTree && !is_tree) ? 1u : 0u;
+  (void)nidx;
+}
+
 __global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
                                const uint32_t *run_beg, uint32_t nruns, uint8_t *run_route,
                                uint32_t *run_c, uint32_t *run_nko, uint32_t *kick_old, uint32_t *promote,
@@ -662,6 +748,10 @@ __global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, c
   GRID_STRIDE(r, nruns) {
     uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
     uint32_t s = key_src(R[b0]);
+    if (b1 - b0 <= kMergeRun) {
+      inline_merge_short(g, R, ex, nidx, b0, b1, s, r, run_route, run_c, run_nko, kick_old, promote, old_deg);
+      continue;
+    }
     VBlock &vb = g.vb[s];
     uint32_t D = vb.degree;
     old_deg[r] = D;  // promotions need the degree before the merge
@@ -1708,33 +1798,39 @@ __global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_be
   GRID_STRIDE(r, nruns) {
     uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
     uint32_t s = key_src(R[b0]);
-    VBlock &vb = g.vb[s];
-    uint32_t D = vb.degree;
-    uint32_t lim = D < kInline ? D : kInline;
-    uint32_t inl[kInline];
-    for (uint32_t i = 0; i < lim; i++) inl[i] = vb.nbr[i];
-    uint8_t tier = (vb.aux & kAuxTree) ? kRouteTree : kRoutePma;
-    bool hit[kInline];
-    for (uint32_t p = 0; p < kInline; p++) hit[p] = false;
+    // the block as 16 words (degree, nbr[0..12], aux): registers only, the
+    // inline hits as a 13-bit mask (no run-time indexed private array)
+    const uint4 *vp = reinterpret_cast<const uint4 *>(g.vb + s);
+    const uint4 q0 = vp[0], q1 = vp[1], q2 = vp[2], q3 = vp[3];
+    const uint32_t w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                            q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+    const uint32_t D = w[0];
+    const uint32_t lim = D < kInline ? D : kInline;
+    const uint8_t tier = (w[15] & 0x80000000u) ? kRouteTree : kRoutePma;  // aux bit 63: B-tree
+    uint32_t hit = 0;
     for (uint32_t q = b0; q < b1; q++) {
-      uint32_t d = key_dst(R[q]);
-      uint8_t rt = tier;
-      for (uint32_t p = 0; p < lim; p++)
-        if (inl[p] == d) {
-          hit[p] = true;
-          rt = kRouteNone;
-        }
-      sec_route[q] = rt;
+      const uint32_t d = key_dst(R[q]);
+      uint32_t m = 0;
+#pragma unroll
+      for (uint32_t p = 0; p < kInline; p++) m |= (p < lim && w[1 + p] == d) ? (1u << p) : 0u;
+      hit |= m;
+      sec_route[q] = m ? kRouteNone : tier;
     }
+    // kept inline entries move down over the deleted ones, in order
+    VBlock &vb = g.vb[s];
     uint32_t keep = 0;
-    for (uint32_t p = 0; p < lim; p++)
-      if (!hit[p]) inl[keep++] = inl[p];
-    uint32_t x = lim - keep;        // inline entries deleted
+#pragma unroll
+    for (uint32_t p = 0; p < kInline; p++) {
+      if (p < lim && !((hit >> p) & 1u)) {
+        if (keep != p) vb.nbr[keep] = w[1 + p];
+        keep++;
+      }
+    }
+    uint32_t x = lim - keep;  // inline entries deleted
     uint32_t len = b1 - b0;
     uint32_t sec_before = D > kInline ? D - kInline : 0;
     uint32_t sec_after = sec_before - (len - x);
     uint32_t y = x < sec_after ? x : sec_after;
-    for (uint32_t i = 0; i < keep; i++) vb.nbr[i] = inl[i];
     vb.degree = D - len;
     run_bump[r] = y;
   }
