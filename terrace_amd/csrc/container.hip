@@ -654,22 +654,21 @@ __global__ void k_exists_n(GView g, const uint64_t *keys, const uint64_t *np, ui
 // Writes the new inline slots and degree, the run's route, and the old
 // inline entries that are kicked out (kick_old[run*13 ..], count in
 // run_nko).
-// k_inline_merge for a run of at most kMergeRun keys, in registers: the
-// run's keys and new-key flags are loaded at once, and every inline entry
-// and new key computes its rank in the merged order (inline and new keys
-// are disjoint); ranks below kInline are the new inline slots, the old
-// entries ranked beyond are kicked out (they are the largest old ones).
-// Same outputs as the general loop below.
+// k_inline_merge in registers for a run of at most kMergeRun keys, or a
+// longer run whose first kMergeRun keys hold at least kInline new keys
+// (then only they can enter the inline slots; any later key is larger).
+// The window's keys and new-key flags are loaded at once, and every inline
+// entry and new key computes its rank in the merged order (inline and new
+// keys are disjoint); ranks below kInline are the new inline slots, the
+// old entries ranked beyond are kicked out (the largest old ones).  Same
+// outputs as the general loop; returns false, having written nothing, for
+// a run it does not cover.
 constexpr uint32_t kMergeRun = 16;
 
-__device__ inline void inline_merge_short(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
-                                          uint32_t b0, uint32_t b1, uint32_t s, uint64_t r, uint8_t *run_route,
-                                          uint32_t *run_c, uint32_t *run_nko, uint32_t *kick_old,
-                                          uint32_t *promote, uint32_t *old_deg) {
-  const uint4 *vp = reinterpret_cast<const uint4 *>(g.vb + s);
-  const uint4 q0 = vp[0], q1 = vp[1], q2 = vp[2], q3 = vp[3];
-  const uint32_t w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+__device__ inline bool inline_merge_window(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
+                                           uint32_t b0, uint32_t b1, uint32_t s, uint64_t r, uint8_t *run_route,
+                                           uint32_t *run_c, uint32_t *run_nko, uint32_t *kick_old,
+                                           uint32_t *promote, uint32_t *old_deg) {
   const uint32_t L = b1 - b0;
   uint32_t rd[kMergeRun];
   bool rnew[kMergeRun];
@@ -678,23 +677,32 @@ __device__ inline void inline_merge_short(GView g, const uint64_t *R, const uint
     rd[j] = j < L ? key_dst(R[b0 + j]) : 0u;
     rnew[j] = j < L && ex[b0 + j] == 0;
   }
+  uint32_t wnew = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kMergeRun; j++) wnew += rnew[j] ? 1u : 0u;
+  if (L > kMergeRun && wnew < kInline) return false;
+  const uint4 *vp = reinterpret_cast<const uint4 *>(g.vb + s);
+  const uint4 q0 = vp[0], q1 = vp[1], q2 = vp[2], q3 = vp[3];
+  const uint32_t w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
   const uint32_t D = w[0];
   old_deg[r] = D;  // promotions need the degree before the merge
   const uint32_t lim = D < kInline ? D : kInline;
   const bool is_tree = (w[15] & 0x80000000u) != 0;  // aux bit 63
-  uint32_t nnew = 0, dup_inline = 0;
+  const uint32_t nnew = L > kMergeRun ? nidx[b1] - nidx[b0] : wnew;
+  uint32_t dup_inline = 0;
 #pragma unroll
   for (uint32_t j = 0; j < kMergeRun; j++) {
-    nnew += rnew[j] ? 1u : 0u;
     bool in = false;
 #pragma unroll
     for (uint32_t p = 0; p < kInline; p++) in |= p < lim && w[1 + p] == rd[j];
     dup_inline += (j < L && !rnew[j] && in) ? 1u : 0u;
   }
-  // k = |inline ∪ run| as the reference counts it (graph.h:290-330)
+  // k = |inline ∪ run| as the reference counts it (graph.h:290-330); a run
+  // longer than kMergeRun overflows whatever the duplicates (k >= |run|)
   const uint64_t size = L;
   const uint64_t kk = static_cast<uint64_t>(lim) + size - dup_inline;
-  const bool overflow = (D == 0) ? (size > kInline) : (kk > kInline);
+  const bool overflow = L > kMergeRun || ((D == 0) ? (size > kInline) : (kk > kInline));
   uint8_t route = kRouteNone;
   if (overflow) route = (is_tree || static_cast<uint64_t>(D) + size > kMediumDegree) ? kRouteTree : kRoutePma;
   VBlock &vb = g.vb[s];
@@ -737,7 +745,7 @@ __device__ inline void inline_merge_short(GView g, const uint64_t *R, const uint
   run_c[r] = c;
   run_nko[r] = lim - kept_old;
   promote[r] = (route == kRouteTree && !is_tree) ? 1u : 0u;
-  (void)nidx;
+  return true;
 }
 
 __global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
@@ -748,10 +756,8 @@ __global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, c
   GRID_STRIDE(r, nruns) {
     uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
     uint32_t s = key_src(R[b0]);
-    if (b1 - b0 <= kMergeRun) {
-      inline_merge_short(g, R, ex, nidx, b0, b1, s, r, run_route, run_c, run_nko, kick_old, promote, old_deg);
+    if (inline_merge_window(g, R, ex, nidx, b0, b1, s, r, run_route, run_c, run_nko, kick_old, promote, old_deg))
       continue;
-    }
     VBlock &vb = g.vb[s];
     uint32_t D = vb.degree;
     old_deg[r] = D;  // promotions need the degree before the merge
