@@ -1368,16 +1368,31 @@ __global__ void k_pma_fix_begin(GView g, const uint64_t *keys, uint64_t k, const
     uint32_t s = key_src(keys[i]);
     uint64_t p = slot + 1;
     uint64_t nb = kPmaNone;
+    // a leaf at a time: its count and all 64 destinations loaded together,
+    // the first occupied slot at or after p found in registers
     while (p < g.N) {
-      if ((p & (kLeafSlots - 1)) == 0 && g.pcnt[p >> kLeafShift] == 0) {
-        p += kLeafSlots;
-        continue;
+      const uint64_t leaf = p >> kLeafShift, base = leaf << kLeafShift;
+      const uint32_t cnt = g.pcnt[leaf];
+      const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + base);
+      uint4 v[kLeafSlots / 4];
+#pragma unroll
+      for (uint32_t j = 0; j < kLeafSlots / 4; j++) v[j] = pd[j];
+      uint64_t found = ~0ull;
+      if (cnt != 0) {
+        const uint32_t from = static_cast<uint32_t>(p - base);
+#pragma unroll
+        for (uint32_t j = kLeafSlots / 4; j-- > 0;) {  // lowest index wins
+          if (4 * j + 3 >= from && v[j].w != kEmpty) found = base + 4 * j + 3;
+          if (4 * j + 2 >= from && v[j].z != kEmpty) found = base + 4 * j + 2;
+          if (4 * j + 1 >= from && v[j].y != kEmpty) found = base + 4 * j + 1;
+          if (4 * j >= from && v[j].x != kEmpty) found = base + 4 * j;
+        }
       }
-      if (g.pdst[p] != kEmpty) {
-        if (g.psrc[p] == s) nb = p;
+      if (found != ~0ull) {
+        if (g.psrc[found] == s) nb = found;
         break;
       }
-      p++;
+      p = base + kLeafSlots;
     }
     g.vb[s].aux = nb;
   }
