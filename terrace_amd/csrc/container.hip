@@ -648,6 +648,82 @@ __global__ void k_exists_n(GView g, const uint64_t *keys, const uint64_t *np, ui
   GRID_STRIDE(i, cap) ex[i] = (i < n && edge_exists(g, key_src(keys[i]), key_dst(keys[i]))) ? 1u : 0u;
 }
 
+// Insert preparation of a batch past kSmallSortKeys, after the existence
+// flags ex (k_exists2_n, one key per thread: the lookups need the whole
+// GPU's memory parallelism), in one single-pass launch over the first *np
+// unique keys R (of cap): nidx[i] = new keys before i, for every i <= cap
+// (nidx[cap] = their total); the source run starts into run_beg, their
+// count into *nruns and run_beg[nruns] = *np.  Two look-back columns (new
+// keys, run starts), each walked by its own wave; ranks within the tile
+// come from ballots, so both outputs follow index order.
+constexpr int kPrepBlock = 256, kPrepItems = 8, kPrepTile = kPrepBlock * kPrepItems;
+
+__global__ void __launch_bounds__(kPrepBlock) k_insert_prep(const uint64_t *R, const uint64_t *np,
+                                                            uint64_t cap, const uint32_t *ex, uint32_t *nidx,
+                                                            uint32_t *run_beg, uint64_t *nruns, uint64_t *state,
+                                                            uint32_t *misc, uint32_t *err, uint32_t nblocks,
+                                                            uint32_t epoch) {
+  __shared__ uint32_t wcnt[2][kPrepItems][kPrepBlock / 64];
+  __shared__ uint32_t wpre[2][kPrepItems][kPrepBlock / 64];
+  __shared__ uint32_t s_ticket, s_run[2];
+  __shared__ uint64_t s_off[2];
+  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
+  if (b >= nblocks) return;
+  const uint64_t n = *np;
+  const uint32_t t = threadIdx.x, w = t >> 6;
+  const uint64_t lt = lanemask_lt();
+  const uint64_t base = static_cast<uint64_t>(b) * kPrepTile;
+  bool e[kPrepItems], st[kPrepItems];
+#pragma unroll
+  for (int i = 0; i < kPrepItems; i++) {
+    const uint64_t idx = base + static_cast<uint64_t>(i) * kPrepBlock + t;
+    e[i] = false;
+    st[i] = false;
+    if (idx < n) {
+      e[i] = ex[idx] != 0;
+      st[i] = idx == 0 || key_src(R[idx - 1]) != key_src(R[idx]);
+    }
+  }
+  uint32_t rn[kPrepItems], rs[kPrepItems];
+#pragma unroll
+  for (int i = 0; i < kPrepItems; i++) {
+    const uint64_t idx = base + static_cast<uint64_t>(i) * kPrepBlock + t;
+    const uint64_t mn = __ballot(idx < n && !e[i]), ms = __ballot(st[i]);
+    rn[i] = static_cast<uint32_t>(__popcll(mn & lt));
+    rs[i] = static_cast<uint32_t>(__popcll(ms & lt));
+    if ((t & 63) == 0) {
+      wcnt[0][i][w] = static_cast<uint32_t>(__popcll(mn));
+      wcnt[1][i][w] = static_cast<uint32_t>(__popcll(ms));
+    }
+  }
+  __syncthreads();
+  if (t == 0 || t == 64) {
+    const uint32_t col = t >> 6;
+    uint32_t run = 0;
+    for (int i = 0; i < kPrepItems; i++)
+      for (int v = 0; v < kPrepBlock / 64; v++) {
+        wpre[col][i][v] = run;
+        run += wcnt[col][i][v];
+      }
+    s_off[col] = lbk::lookback(state, 2, b, col, err, epoch, run);
+    s_run[col] = run;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kPrepItems; i++) {
+    const uint64_t idx = base + static_cast<uint64_t>(i) * kPrepBlock + t;
+    if (idx >= cap) continue;
+    nidx[idx] = static_cast<uint32_t>(s_off[0] + wpre[0][i][w] + rn[i]);
+    if (st[i]) run_beg[s_off[1] + wpre[1][i][w] + rs[i]] = static_cast<uint32_t>(idx);
+  }
+  if (b == nblocks - 1 && t == 0) {
+    const uint64_t runs = s_off[1] + s_run[1];
+    nidx[cap] = static_cast<uint32_t>(s_off[0] + s_run[0]);
+    *nruns = runs;
+    run_beg[runs] = static_cast<uint32_t>(n);
+  }
+}
+
 // ------------------------------------------------------------ inline merge
 // One thread per source run of the unique sorted batch R.  ex[] marks keys
 // already present; nidx[] is the exclusive count of new keys (size |R|+1).
@@ -1068,7 +1144,6 @@ __global__ void k_u32_pair(const uint32_t *a, const uint32_t *b, uint64_t *out) 
   out[0] = *a;
   out[1] = *b;
 }
-__global__ void k_put_u32_at(uint32_t *p, uint64_t i, uint32_t v) { p[i] = v; }
 
 // ---- selectors (select.hpp): predicate + emit, no flag arrays
 // first key of each run of equal keys, with its weight
@@ -1102,6 +1177,7 @@ struct SelRunStartN {
   const uint64_t *np;
   __device__ bool pred(uint64_t i) const { return i < *np && (i == 0 || key_src(k[i]) != key_src(k[i - 1])); }
   __device__ void emit(uint64_t i, uint64_t pos) const { beg[pos] = static_cast<uint32_t>(i); }
+  __device__ void finish(uint64_t runs) const { beg[runs] = static_cast<uint32_t>(*np); }  // beg[nruns] = n
 };
 struct SelFlagged {
   const uint64_t *k;
@@ -2689,28 +2765,30 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
     GView g = view();
     f0_.ensure(count + 1, stream_);
     f1_.ensure(count + 1, stream_);
+    s32_[0].ensure(count + 1, stream_);
     k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, R, dsc_.get() + 1, count, f0_.get(), f1_.get());
     TG_CHECK_LAUNCH();
-    s32_[0].ensure(count + 1, stream_);
     if (small) {
       // new-key prefix, source runs and run_beg[nruns] = nR in one workgroup
       k_small_runs<true><<<1, 1024, 0, stream_>>>(R, f1_.get(), static_cast<uint32_t>(count), nullptr,
                                                   s32_[0].get(), dsc_.get());
       TG_CHECK_LAUNCH();
     } else {
-      exclusive_scan_u32(f1_.get(), f1_.get(), count, f1_.get() + count, scan_scratch, stream_);
-      select(SelRunStartN{R, s32_[0].get(), dsc_.get() + 1}, count, dsc_.get() + 2, scan_scratch, stream_);
+      // new-key prefix and source runs in one launch
+      const uint32_t nb = static_cast<uint32_t>((count + kPrepTile - 1) / kPrepTile);
+      const uint32_t epoch = scan_scratch.lb.begin(2ull * nb, stream_);
+      k_insert_prep<<<nb, kPrepBlock, 0, stream_>>>(R, dsc_.get() + 1, count, f0_.get(), f1_.get(),
+                                                    s32_[0].get(), dsc_.get() + 2, scan_scratch.lb.state.get(),
+                                                    scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb,
+                                                    epoch);
+      TG_CHECK_LAUNCH();
     }
     read_scalars(dsc_.get(), h, 3, stream_);
     if (!bucket || !(h[0] & kSortOverflow)) break;
   }
   if (h[0] & 1) throw Error(-1, "vertex id out of range");
   if (h[1] == 0) return;
-  if (!small) {
-    k_put_u32_at<<<1, 1, 0, stream_>>>(s32_[0].get(), h[2], static_cast<uint32_t>(h[1]));
-    TG_CHECK_LAUNCH();
-  }
-  insert_runs(R, h[1], static_cast<uint32_t>(h[2]));
+  insert_runs(R, h[1], static_cast<uint32_t>(h[2]));  // run_beg[nruns] = nR is written
 }
 
 // Insert sorted unique keys (R must not alias the batch scratch buffers).
@@ -2885,11 +2963,7 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   }
   if (h[0] & 1) throw Error(-1, "vertex id out of range");
   if (h[2] == 0) return;
-  if (!small) {
-    k_put_u32_at<<<1, 1, 0, stream_>>>(s32_[0].get(), h[3], static_cast<uint32_t>(h[2]));
-    TG_CHECK_LAUNCH();
-  }
-  delete_runs(k2_.get(), h[2], static_cast<uint32_t>(h[3]));
+  delete_runs(k2_.get(), h[2], static_cast<uint32_t>(h[3]));  // run_beg[nruns] = nR is written
 }
 
 // Delete sorted unique keys (absent keys are ignored).

@@ -7,11 +7,21 @@
 //   select(f, n, total_dev, scratch, stream)   // *total_dev = count
 #pragma once
 
+#include <type_traits>
+#include <utility>
+
 #include "prims.hpp"
 
 namespace tg {
 
 namespace sel {
+
+// F::finish(total), when F has it, runs once after the last position is
+// known (last block, one thread): e.g. a terminator after the output.
+template <class F, class = void>
+struct has_finish : std::false_type {};
+template <class F>
+struct has_finish<F, std::void_t<decltype(std::declval<const F &>().finish(uint64_t{}))>> : std::true_type {};
 
 constexpr int kBlock = 256;
 constexpr int kItems = 8;
@@ -61,7 +71,11 @@ __global__ void __launch_bounds__(kBlock) k_emit(F f, uint64_t n, const uint64_t
         wpre[i][ww] = run;
         run += wcnt[i][ww];
       }
-    if (total && blockIdx.x == gridDim.x - 1) *total = (offs ? offs[blockIdx.x] : 0) + run;
+    if (blockIdx.x == gridDim.x - 1) {
+      const uint64_t t = (offs ? offs[blockIdx.x] : 0) + run;
+      if (total) *total = t;
+      if constexpr (has_finish<F>::value) f.finish(t);
+    }
   }
   __syncthreads();
   uint64_t boff = offs ? offs[blockIdx.x] : 0;
@@ -113,7 +127,10 @@ __global__ void __launch_bounds__(kBlock) k_select_lb(F f, uint64_t n, uint64_t 
       }
     uint64_t off = lbk::lookback(state, 1, b, 0, err, epoch, run);
     s_off = off;
-    if (total && b == nblocks - 1) *total = off + run;
+    if (b == nblocks - 1) {
+      if (total) *total = off + run;
+      if constexpr (has_finish<F>::value) f.finish(off + run);
+    }
   }
   __syncthreads();
   const uint64_t boff = s_off;
