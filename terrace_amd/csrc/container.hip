@@ -697,16 +697,20 @@ __global__ void __launch_bounds__(kPrepBlock) k_insert_prep(const uint64_t *R, c
     }
   }
   __syncthreads();
-  if (t == 0 || t == 64) {
+  if (t < 128) {  // wave 0 walks column 0, wave 1 column 1
     const uint32_t col = t >> 6;
+    const bool lead = (t & 63) == 0;
     uint32_t run = 0;
     for (int i = 0; i < kPrepItems; i++)
       for (int v = 0; v < kPrepBlock / 64; v++) {
-        wpre[col][i][v] = run;
+        if (lead) wpre[col][i][v] = run;
         run += wcnt[col][i][v];
       }
-    s_off[col] = lbk::lookback(state, 2, b, col, err, epoch, run);
-    s_run[col] = run;
+    const uint32_t off = lbk::lookback_wave(state, 2, b, col, err, epoch, run);
+    if (lead) {
+      s_off[col] = off;
+      s_run[col] = run;
+    }
   }
   __syncthreads();
 #pragma unroll

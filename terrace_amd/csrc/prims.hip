@@ -139,7 +139,10 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_lb(const InT *in, InT *out,
   }
   uint64_t tot;
   uint64_t pre = block_excl_scan(sum, lds, &tot);
-  if (threadIdx.x == 0) s_excl = lbk::lookback(state, 1, b, 0, err, epoch, static_cast<uint32_t>(tot));
+  if (threadIdx.x < 64) {  // wave 0 walks the look-back
+    const uint32_t x = lbk::lookback_wave(state, 1, b, 0, err, epoch, static_cast<uint32_t>(tot));
+    if (threadIdx.x == 0) s_excl = x;
+  }
   __syncthreads();
   const uint64_t excl = s_excl;
   pre += excl;
@@ -374,7 +377,7 @@ __global__ void __launch_bounds__(kSortBlock) k_onesweep(const uint64_t *keys, u
   uint64_t gtot;
   uint64_t gbase = block_excl_scan(static_cast<uint64_t>(ghist[d]), lds, &gtot);
   // block prefix of digit d over the blocks before b
-  uint64_t excl = lbk::lookback(state, 256, b, d, err, epoch, c);
+  uint64_t excl = lbk::lookback_window(state, 256, b, d, err, epoch, c);
   uint64_t run = gbase + excl;
   for (int ww = 0; ww < kWaves; ww++) {
     uint32_t x = wave_cnt[ww][d];

@@ -107,11 +107,15 @@ __device__ inline uint32_t ticket(uint32_t *misc, uint32_t *err, uint32_t nblock
   return *lds_slot;
 }
 
-// Publishes agg for slot `me` and returns the sum of the values before it
-// (slots me - stride, me - 2 stride, ... down to index 0).  Totals must fit
-// 32 bits.  Run by one thread per slot.
-__device__ inline uint32_t lookback(uint64_t *state, uint64_t stride, uint32_t b, uint64_t col, uint32_t *err,
-                                    uint32_t epoch, uint32_t agg) {
+// Publishes agg for slot (b, col) and returns the sum of the values before
+// it (slots (b - 1, col), (b - 2, col), ... down to block 0).  Totals must
+// fit 32 bits.  Run by one thread per slot, reading kLbWindow predecessors
+// with independent loads per round trip (for launches with many slots per
+// block, e.g. one per radix digit).
+constexpr int kLbWindow = 4;
+
+__device__ inline uint32_t lookback_window(uint64_t *state, uint64_t stride, uint32_t b, uint64_t col,
+                                           uint32_t *err, uint32_t epoch, uint32_t agg) {
   uint64_t *me = state + static_cast<uint64_t>(b) * stride + col;
   if (b == 0) {
     store(me, word(epoch, 2, agg));
@@ -123,21 +127,87 @@ __device__ inline uint32_t lookback(uint64_t *state, uint64_t stride, uint32_t b
   uint32_t spins = 0;
   const uint64_t tag = static_cast<uint64_t>(epoch) << 34;
   while (p >= 0) {
-    uint64_t f = load(state + static_cast<uint64_t>(p) * stride + col);
-    uint32_t st = static_cast<uint32_t>(f >> 32) & 3u;
-    if ((f & ~((1ull << 34) - 1)) != tag || st == 0) {
+    uint64_t f[kLbWindow];
+#pragma unroll
+    for (int j = 0; j < kLbWindow; j++)
+      f[j] = p - j >= 0 ? load(state + static_cast<uint64_t>(p - j) * stride + col) : 0;
+    // consume in order up to the first word not ready or the first inclusive
+    bool done = false, stalled = false;
+    int used = 0;
+#pragma unroll
+    for (int j = 0; j < kLbWindow; j++) {
+      if (done || stalled || p - j < 0) continue;
+      const uint32_t st = static_cast<uint32_t>(f[j] >> 32) & 3u;
+      if ((f[j] & ~((1ull << 34) - 1)) != tag || st == 0) {
+        stalled = true;
+        continue;
+      }
+      excl += static_cast<uint32_t>(f[j]);
+      used++;
+      if (st == 2) done = true;
+    }
+    if (done) break;
+    p -= used;
+    if (stalled) {
       if (++spins > kSpinLimit) {
         atomicOr(err, 1u);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
-      continue;
     }
-    excl += static_cast<uint32_t>(f);
-    if (st == 2) break;
-    p--;
   }
   store(me, word(epoch, 2, excl + agg));
+  return excl;
+}
+
+// The same for one slot per block, run by all 64 lanes of one wave (same
+// arguments in every lane; the result is the same in every lane): lane l
+// reads predecessor top - l, so one round trip covers 64 predecessors; a
+// ballot finds the nearest inclusive prefix and a wave sum adds the words
+// up to it.
+__device__ inline uint32_t lookback_wave(uint64_t *state, uint64_t stride, uint32_t b, uint64_t col,
+                                         uint32_t *err, uint32_t epoch, uint32_t agg) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint64_t *me = state + static_cast<uint64_t>(b) * stride + col;
+  if (b == 0) {
+    if (lane == 0) store(me, word(epoch, 2, agg));
+    return 0;
+  }
+  if (lane == 0) store(me, word(epoch, 1, agg));
+  uint32_t excl = 0;
+  int64_t top = static_cast<int64_t>(b) - 1;
+  uint32_t spins = 0;
+  const uint64_t tag = static_cast<uint64_t>(epoch) << 34;
+  while (true) {
+    const int64_t p = top - static_cast<int64_t>(lane);
+    bool ready = true;
+    uint32_t st = 2, val = 0;  // past block 0: never consumed (block 0 is inclusive)
+    if (p >= 0) {
+      const uint64_t f = load(state + static_cast<uint64_t>(p) * stride + col);
+      st = static_cast<uint32_t>(f >> 32) & 3u;
+      ready = (f & ~((1ull << 34) - 1)) == tag && st != 0;
+      val = static_cast<uint32_t>(f);
+    }
+    const uint64_t inc = __ballot(ready && st == 2);
+    const uint64_t stalled = __ballot(!ready);
+    const int first = inc ? __ffsll(static_cast<unsigned long long>(inc)) - 1 : 63;
+    const uint64_t need = first >= 63 ? ~0ull : ((1ull << (first + 1)) - 1);  // lanes 0..first
+    if (stalled & need) {
+      if (++spins > kSpinLimit) {
+        if (lane == 0) atomicOr(err, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    uint32_t v = static_cast<int>(lane) <= first ? val : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    excl += v;
+    if (inc) break;
+    top -= 64;
+  }
+  if (lane == 0) store(me, word(epoch, 2, excl + agg));
   return excl;
 }
 

@@ -118,18 +118,20 @@ __global__ void __launch_bounds__(kBlock) k_select_lb(F f, uint64_t n, uint64_t 
     if ((threadIdx.x & 63) == 0) wcnt[i][w] = __popcll(m);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {  // wave 0: the tile's prefixes, then the look-back
     uint32_t run = 0;
     for (int i = 0; i < kItems; i++)
       for (int ww = 0; ww < kBlock / 64; ww++) {
-        wpre[i][ww] = run;
+        if (threadIdx.x == 0) wpre[i][ww] = run;
         run += wcnt[i][ww];
       }
-    uint64_t off = lbk::lookback(state, 1, b, 0, err, epoch, run);
-    s_off = off;
-    if (b == nblocks - 1) {
-      if (total) *total = off + run;
-      if constexpr (has_finish<F>::value) f.finish(off + run);
+    const uint64_t off = lbk::lookback_wave(state, 1, b, 0, err, epoch, run);
+    if (threadIdx.x == 0) {
+      s_off = off;
+      if (b == nblocks - 1) {
+        if (total) *total = off + run;
+        if constexpr (has_finish<F>::value) f.finish(off + run);
+      }
     }
   }
   __syncthreads();
@@ -176,17 +178,20 @@ __global__ void __launch_bounds__(kBlock) k_select2_lb(F f, uint64_t n, uint64_t
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0 || threadIdx.x == 64) {
+  if (threadIdx.x < 128) {  // wave 0 walks column 0, wave 1 column 1
     const uint32_t col = threadIdx.x >> 6;
+    const bool lead = (threadIdx.x & 63) == 0;
     uint32_t run = 0;
     for (int i = 0; i < kItems; i++)
       for (int ww = 0; ww < kBlock / 64; ww++) {
-        wpre[col][i][ww] = run;
+        if (lead) wpre[col][i][ww] = run;
         run += wcnt[col][i][ww];
       }
-    const uint64_t off = lbk::lookback(state, 2, b, col, err, epoch, run);
-    s_off[col] = off;
-    s_run[col] = run;
+    const uint64_t off = lbk::lookback_wave(state, 2, b, col, err, epoch, run);
+    if (lead) {
+      s_off[col] = off;
+      s_run[col] = run;
+    }
   }
   __syncthreads();
   if (b == nblocks - 1 && threadIdx.x == 0) {
