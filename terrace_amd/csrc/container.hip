@@ -2859,9 +2859,8 @@ void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   } else {
     select2(SelRoutes{k2_.get(), kroute.get(), k3_.get(), g0_.get()}, nR, dsc_.get() + 3, dsc_.get() + 4,
             scan_scratch, stream_);
-    exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_);
-    k_u32_to_u64<<<1, 1, 0, stream_>>>(ppos.get() + nruns, dsc_.get() + 5);
-    TG_CHECK_LAUNCH();
+    exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_,
+                       dsc_.get() + 5);
   }
   uint64_t h[3];
   read_scalars(dsc_.get() + 3, h, 3, stream_);
@@ -3017,9 +3016,8 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   } else {
     select2(SelRoutes{R, sec_route.get(), k3_.get(), g0_.get()}, nR, dsc_.get() + 4, dsc_.get() + 5, scan_scratch,
             stream_);
-    exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_);
-    k_u32_to_u64<<<1, 1, 0, stream_>>>(bump_off.get() + nruns, dsc_.get() + 6);
-    TG_CHECK_LAUNCH();
+    exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_,
+                       dsc_.get() + 6);
   }
   uint64_t h[3];
   read_scalars(dsc_.get() + 4, h, 3, stream_);

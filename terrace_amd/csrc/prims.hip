@@ -76,7 +76,8 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const InT *in, uint6
 // tile is staged through LDS so global loads and stores stay coalesced.
 template <typename InT, typename AccT>
 __global__ void __launch_bounds__(kScanBlock) k_scan_tile(const InT *in, InT *out, uint64_t n,
-                                                          const AccT *offsets, InT *total_dev) {
+                                                          const AccT *offsets, InT *total_dev,
+                                                          uint64_t *total64) {
   __shared__ AccT lds[kScanBlock / 64];
   __shared__ InT tile[kScanTile];
   const uint64_t tbase = static_cast<uint64_t>(blockIdx.x) * kScanTile;
@@ -108,13 +109,17 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_tile(const InT *in, InT *ou
     uint64_t idx = tbase + static_cast<uint64_t>(i) * kScanBlock + threadIdx.x;
     if (idx < n) out[idx] = tile[i * kScanBlock + threadIdx.x];
   }
-  if (total_dev && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_dev = static_cast<InT>(off + tot);
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    if (total_dev) *total_dev = static_cast<InT>(off + tot);
+    if (total64) *total64 = static_cast<uint64_t>(off + tot);
+  }
 }
 
 // Single-pass scan: tile scan plus the decoupled look-back for the block's
 // prefix (one launch for any n).
 template <typename InT>
 __global__ void __launch_bounds__(kScanBlock) k_scan_lb(const InT *in, InT *out, uint64_t n, InT *total_dev,
+                                                        uint64_t *total64,
                                                         uint64_t *state, uint32_t *misc, uint32_t *err,
                                                         uint32_t nblocks, uint32_t epoch) {
   __shared__ uint64_t lds[kScanBlock / 64];
@@ -157,7 +162,10 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_lb(const InT *in, InT *out,
     uint64_t idx = tbase + static_cast<uint64_t>(i) * kScanBlock + threadIdx.x;
     if (idx < n) out[idx] = tile[i * kScanBlock + threadIdx.x];
   }
-  if (total_dev && b == nblocks - 1 && threadIdx.x == 0) *total_dev = static_cast<InT>(excl + tot);
+  if (b == nblocks - 1 && threadIdx.x == 0) {
+    if (total_dev) *total_dev = static_cast<InT>(excl + tot);
+    if (total64) *total64 = excl + tot;
+  }
 }
 
 template <typename T>
@@ -165,17 +173,21 @@ __global__ void k_set_zero(T *p) { *p = 0; }
 
 template <typename InT>
 void scan_impl(const InT *in, InT *out, uint64_t n, InT *total_dev, ScanScratch &ss, int level,
-               hipStream_t s) {
+               hipStream_t s, uint64_t *total64 = nullptr) {
   if (n == 0) {
     if (total_dev) {
       k_set_zero<InT><<<1, 1, 0, s>>>(total_dev);
+      TG_CHECK_LAUNCH();
+    }
+    if (total64) {
+      k_set_zero<uint64_t><<<1, 1, 0, s>>>(total64);
       TG_CHECK_LAUNCH();
     }
     return;
   }
   uint64_t nblocks = (n + kScanTile - 1) / kScanTile;
   if (nblocks == 1) {
-    k_scan_tile<InT, uint64_t><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, total_dev);
+    k_scan_tile<InT, uint64_t><<<1, kScanBlock, 0, s>>>(in, out, n, nullptr, total_dev, total64);
     TG_CHECK_LAUNCH();
     return;
   }
@@ -183,7 +195,7 @@ void scan_impl(const InT *in, InT *out, uint64_t n, InT *total_dev, ScanScratch 
   if (ss.lookback && sizeof(InT) == 4 && nblocks <= 0xFFFFFFFFull) {
     uint32_t nb = static_cast<uint32_t>(nblocks);
     uint32_t epoch = ss.lb.begin(nb, s);
-    k_scan_lb<InT><<<nb, kScanBlock, 0, s>>>(in, out, n, total_dev, ss.lb.state.get(), ss.lb.misc.get(),
+    k_scan_lb<InT><<<nb, kScanBlock, 0, s>>>(in, out, n, total_dev, total64, ss.lb.state.get(), ss.lb.misc.get(),
                                              ss.lb.err_word(), nb, epoch);
     TG_CHECK_LAUNCH();
     return;
@@ -195,7 +207,7 @@ void scan_impl(const InT *in, InT *out, uint64_t n, InT *total_dev, ScanScratch 
   TG_CHECK_LAUNCH();
   scan_impl<uint64_t>(sums.get(), sums.get(), nblocks, nullptr, ss, level + 1, s);
   k_scan_tile<InT, uint64_t><<<static_cast<unsigned>(nblocks), kScanBlock, 0, s>>>(in, out, n, sums.get(),
-                                                                                  total_dev);
+                                                                                  total_dev, total64);
   TG_CHECK_LAUNCH();
 }
 
@@ -416,8 +428,8 @@ __global__ void k_fill_f64(double *p, double v, uint64_t n) {
 }  // namespace
 
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *total_dev,
-                        ScanScratch &ss, hipStream_t s) {
-  scan_impl<uint32_t>(in, out, n, total_dev, ss, 0, s);
+                        ScanScratch &ss, hipStream_t s, uint64_t *total64) {
+  scan_impl<uint32_t>(in, out, n, total_dev, ss, 0, s, total64);
 }
 
 void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *total_dev,

@@ -217,10 +217,11 @@ __device__ inline uint32_t lookback_wave(uint64_t *state, uint64_t stride, uint3
 // (synchronises the stream); clears the flag.
 uint32_t check_lookback(ScanScratch &ss, hipStream_t s);
 
-// out[i] = sum(in[0..i)), returns nothing; *total_dev (optional) gets the sum.
-// in and out may alias.  n may be 0.
+// out[i] = sum(in[0..i)), returns nothing; *total_dev (optional) gets the sum,
+// and so does *total64 (optional, as a 64-bit word).  in and out may alias.
+// n may be 0.
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *total_dev,
-                        ScanScratch &ss, hipStream_t s);
+                        ScanScratch &ss, hipStream_t s, uint64_t *total64 = nullptr);
 void exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *total_dev,
                         ScanScratch &ss, hipStream_t s);
 
