@@ -332,6 +332,49 @@ def test_mid_size_batches_bucket_sort():
     G.close()
 
 
+def test_delete_prefix_moves_begin_across_leaves():
+    """Vertices of degree ~500 (PMA tier, their runs spanning several
+    64-slot leaves) lose their smallest neighbours in one batch: the 13
+    inline ones and the next 100, then the next 200, then all but 10.  The
+    begin pointer must skip whole emptied leaves; the adjacency (exported
+    through the begin pointers) equals the set model after every batch, and
+    BFS, PageRank and CC match the oracle."""
+    n = 1 << 14
+    say("delete prefix across leaves")
+    rng = np.random.default_rng(13)
+    hubs = np.array([3, 777, 9000, 16000], np.uint64)
+
+    def sym(k):  # both directions (the traversals expect an undirected graph)
+        k = np.asarray(k, np.uint64)
+        rev = ((k & np.uint64(0xFFFFFFFF)) << np.uint64(32)) | (k >> np.uint64(32))
+        return oracle.sort_unique(np.concatenate([k, rev]))
+
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 29, 0.5, 0.1, 0.1, 60000))
+    extra = []
+    for h in hubs:
+        nb = np.sort(rng.choice(n, 520, replace=False)).astype(np.uint64)
+        nb = nb[np.isin(nb, hubs, invert=True)]
+        extra.append((h << np.uint64(32)) | nb)
+    keys = oracle.set_union(keys, sym(np.concatenate(extra)))
+    G = tg.Graph(n)
+    G.add_edge_batch(*split(keys))
+    assert (G.tiers()[hubs.astype(np.int64)] == 1).all()
+    assert_adjacency(G, keys, n, "built")
+    for cut in (113, 200, -10):  # from the front of each current row
+        dels = []
+        for h in hubs:
+            row = keys[(keys >> np.uint64(32)) == h]
+            dels.append(row[:cut])
+        dels = sym(np.concatenate(dels))
+        G.remove_edge_batch(*split(dels))
+        keys = oracle.set_difference(keys, dels)
+        assert_adjacency(G, keys, n, f"prefix delete {cut}")
+    assert (G.degrees()[hubs.astype(np.int64)] == 10).all()
+    g = oracle.CSR(keys, n)
+    check_algorithms(G, g, int(hubs[0]), what="prefix deletes")
+    G.close()
+
+
 def test_edge_map_frontier_api():
     n = 1 << 10
     say("edge_map api")
