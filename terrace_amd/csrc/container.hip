@@ -1028,17 +1028,6 @@ __global__ void k_pma_leaf_fit(GView g, uint64_t up0, const uint32_t *T, uint32_
   }
 }
 
-// Begin pointer of vertex s whose first element in the leaf at base now
-// sits at slot.  A pointer into this leaf (the run started here) moves with
-// a CAS, which fails only when an earlier leaf's wave has already lowered
-// it to a new earlier element; any other pointer takes the minimum.  Every
-// interleaving leaves the slot of the vertex's first element.
-__device__ inline void leaf_begin(const GView &g, uint32_t s, uint64_t base, uint64_t slot) {
-  unsigned long long *aux = reinterpret_cast<unsigned long long *>(&g.vb[s].aux);
-  const unsigned long long a = __hip_atomic_load(aux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (a >= base && a < base + kLeafSlots) atomicCAS(aux, a, static_cast<unsigned long long>(slot));
-  else atomicMin(aux, static_cast<unsigned long long>(slot));
-}
 
 // Sixteen lanes per touched leaf, four leaves per wave (the grid covers
 // every possible leaf; groups past the touched count exit).  Lane l of a
@@ -1127,13 +1116,30 @@ __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t 
   ns.w = o0 + 3 < total ? key_src(out[3]) : sv.w;
   reinterpret_cast<uint4 *>(g.pdst + base)[l] = nd;
   reinterpret_cast<uint4 *>(g.psrc + base)[l] = ns;
-  // run starts in this leaf move the vertex's begin pointer
+  // Run starts in this leaf move the vertex's begin pointer.  A pointer
+  // into this leaf (the run started here) moves with a CAS, which fails
+  // only when an earlier leaf's wave has already lowered it to a new
+  // earlier element; any other pointer takes the minimum.  Every
+  // interleaving leaves the slot of the vertex's first element.  The four
+  // pointer loads of a lane are issued together: each of its outputs that
+  // starts a run belongs to a different vertex.
+  bool start[4];
+  unsigned long long cur[4];
 #pragma unroll
   for (uint32_t r = 0; r < 4; r++) {
-    if (o0 + r >= total) break;
     const uint64_t before = r == 0 ? prev : out[r - 1];
-    const uint32_t sx = key_src(out[r]);
-    if (before == kNone || key_src(before) != sx) leaf_begin(g, sx, base, base + o0 + r);
+    start[r] = o0 + r < total && (before == kNone || key_src(before) != key_src(out[r]));
+    cur[r] = start[r] ? __hip_atomic_load(reinterpret_cast<unsigned long long *>(&g.vb[key_src(out[r])].aux),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                      : 0ull;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < 4; r++) {
+    if (!start[r]) continue;
+    unsigned long long *aux = reinterpret_cast<unsigned long long *>(&g.vb[key_src(out[r])].aux);
+    const unsigned long long slot = base + o0 + r;
+    if (cur[r] >= base && cur[r] < base + kLeafSlots) atomicCAS(aux, cur[r], slot);
+    else atomicMin(aux, slot);
   }
   if (l == 0) {
     g.pcnt[leaf] = total;
