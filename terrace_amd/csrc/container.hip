@@ -1445,42 +1445,44 @@ __global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t k, uint64_t
   }
 }
 
+// Begin slot of vertex s from slot p on: the first occupied slot at or
+// after p if it holds an element of s, else kPmaNone (the PMA is sorted, so
+// the vertex has no element later).  A leaf at a time: its count and all
+// 64 destinations loaded together, the first occupied slot found in
+// registers.
+__device__ inline uint64_t next_begin(const GView &g, uint32_t s, uint64_t p) {
+  while (p < g.N) {
+    const uint64_t leaf = p >> kLeafShift, base = leaf << kLeafShift;
+    const uint32_t cnt = g.pcnt[leaf];
+    const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + base);
+    uint4 v[kLeafSlots / 4];
+#pragma unroll
+    for (uint32_t j = 0; j < kLeafSlots / 4; j++) v[j] = pd[j];
+    uint64_t found = ~0ull;
+    if (cnt != 0) {
+      const uint32_t from = static_cast<uint32_t>(p - base);
+#pragma unroll
+      for (uint32_t j = kLeafSlots / 4; j-- > 0;) {  // lowest index wins
+        if (4 * j + 3 >= from && v[j].w != kEmpty) found = base + 4 * j + 3;
+        if (4 * j + 2 >= from && v[j].z != kEmpty) found = base + 4 * j + 2;
+        if (4 * j + 1 >= from && v[j].y != kEmpty) found = base + 4 * j + 1;
+        if (4 * j >= from && v[j].x != kEmpty) found = base + 4 * j;
+      }
+    }
+    if (found != ~0ull) return g.psrc[found] == s ? found : kPmaNone;
+    p = base + kLeafSlots;
+  }
+  return kPmaNone;
+}
+
 // New begin for vertices whose first PMA element was deleted: the next
 // occupied slot if it still belongs to the vertex.
 __global__ void k_pma_fix_begin(GView g, const uint64_t *keys, uint64_t k, const uint64_t *fix_slot) {
   GRID_STRIDE(i, k) {
     uint64_t slot = fix_slot[i];
     if (slot == ~0ull) continue;
-    uint32_t s = key_src(keys[i]);
-    uint64_t p = slot + 1;
-    uint64_t nb = kPmaNone;
-    // a leaf at a time: its count and all 64 destinations loaded together,
-    // the first occupied slot at or after p found in registers
-    while (p < g.N) {
-      const uint64_t leaf = p >> kLeafShift, base = leaf << kLeafShift;
-      const uint32_t cnt = g.pcnt[leaf];
-      const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + base);
-      uint4 v[kLeafSlots / 4];
-#pragma unroll
-      for (uint32_t j = 0; j < kLeafSlots / 4; j++) v[j] = pd[j];
-      uint64_t found = ~0ull;
-      if (cnt != 0) {
-        const uint32_t from = static_cast<uint32_t>(p - base);
-#pragma unroll
-        for (uint32_t j = kLeafSlots / 4; j-- > 0;) {  // lowest index wins
-          if (4 * j + 3 >= from && v[j].w != kEmpty) found = base + 4 * j + 3;
-          if (4 * j + 2 >= from && v[j].z != kEmpty) found = base + 4 * j + 2;
-          if (4 * j + 1 >= from && v[j].y != kEmpty) found = base + 4 * j + 1;
-          if (4 * j >= from && v[j].x != kEmpty) found = base + 4 * j;
-        }
-      }
-      if (found != ~0ull) {
-        if (g.psrc[found] == s) nb = found;
-        break;
-      }
-      p = base + kLeafSlots;
-    }
-    g.vb[s].aux = nb;
+    const uint32_t s = key_src(keys[i]);
+    g.vb[s].aux = next_begin(g, s, slot + 1);
   }
 }
 
@@ -1946,14 +1948,21 @@ __global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_be
 // Refill inline slots from the smallest secondary neighbours (graph.h:729-780,
 // batched): thread per run with y > 0; emits the moved keys for deletion
 // from the secondary tier.
+// PMA vertices: the moved elements are removed from the PMA here (slot
+// emptied, leaf count lowered, begin pointer moved to the next element of
+// the vertex; their count added to *pma_taken) and routed to no tier.
+// B-tree vertices: the moved keys are routed to the tree delete.  With
+// trees_only, PMA vertices are skipped (a second pass after the B-tree
+// fallback; their refill is already done).
 __global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, uint32_t nruns,
                        const uint32_t *run_bump, const uint32_t *bump_off, uint64_t *bump_keys,
-                       uint8_t *bump_route) {
+                       uint8_t *bump_route, bool trees_only, uint64_t *pma_taken) {
   GRID_STRIDE(r, nruns) {
     uint32_t y = run_bump[r];
     if (y == 0) continue;
     uint32_t s = key_src(R[run_beg[r]]);
     VBlock &vb = g.vb[s];
+    if (trees_only && !(vb.aux & kAuxTree)) continue;
     uint32_t D = vb.degree;  // already the post-delete degree
     // inline currently holds min(D,13) - y entries
     uint32_t have = (D < kInline ? D : kInline) - y;
@@ -1981,11 +1990,15 @@ __global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, uint
         if (d != kEmpty) {
           vb.nbr[have + got] = d;
           bump_keys[out + got] = make_key(s, d);
-          bump_route[out + got] = kRoutePma;
+          bump_route[out + got] = kRouteNone;  // removed here
+          g.pdst[p] = kEmpty;
+          atomicSub(&g.pcnt[p >> kLeafShift], 1u);
           got++;
         }
         p++;
       }
+      vb.aux = next_begin(g, s, p);  // elements after the vertex's belong to later vertices
+      atomicAdd(reinterpret_cast<unsigned long long *>(pma_taken), static_cast<unsigned long long>(got));
     }
   }
 }
@@ -3048,9 +3061,13 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
     btree.ensure(nb + 1, stream_);
     for (int pass = 0; pass < 2; pass++) {
       g = view();
+      // PMA refills are removed from the PMA by k_bump itself (count in
+      // dsc_[9]); the second pass, after a B-tree fallback, redoes only
+      // the B-tree vertices
       k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, run_bump.get(), bump_off.get(),
-                                                        bkeys.get(), broute.get());
+                                                        bkeys.get(), broute.get(), pass == 1, dsc_.get() + 9);
       TG_CHECK_LAUNCH();
+      tree_dirty_ = true;
       if (nb <= kSmallSortKeys) {
         DevBuf<uint32_t> &none = s32_[31];  // no per-run counts here
         none.ensure(1, stream_);
@@ -3070,7 +3087,8 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
       }
       tree_delete_cow(g0_.get(), ntree);  // then the refill again
     }
-    pma_delete(bpma.get(), hs[4]);
+    pma_elems_ -= hs[9];          // PMA elements the refill moved inline
+    pma_delete(bpma.get(), hs[4]);  // none: k_bump removed them
     const uint64_t nbt = hs[5];
     const bool bt_try = nbt > 0 && nbt <= kTreeSmallKeys;
     if (bt_try) tree_small_launch(btree.get(), nbt, false);
