@@ -3093,7 +3093,9 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
     const bool bt_try = nbt > 0 && nbt <= kTreeSmallKeys;
     if (bt_try) tree_small_launch(btree.get(), nbt, false);
     else if (nbt) tree_delete(btree.get(), nbt);
-    read_scalars(dsc_.get(), hs, kScalars, stream_);  // the batch's flags (and the tree status)
+    // the batch's flags (and the tree status); the refill's read already
+    // holds them when nothing was launched after it
+    if (hs[4] || nbt) read_scalars(dsc_.get(), hs, kScalars, stream_);
     if (bt_try) {
       if (hs[13] == 0) {
         stats_.tree_small_updates++;
