@@ -1029,14 +1029,18 @@ __global__ void k_pma_leaf_fit(GView g, uint64_t up0, const uint32_t *T, uint32_
 }
 
 
-// Sixteen lanes per touched leaf, four leaves per wave (the grid covers
-// every possible leaf; groups past the touched count exit).  Lane l of a
-// group reads slots 4l .. 4l+3; the leaf's elements (a leaf may hold holes
-// after deletes) are compacted by a 16-lane prefix into LDS next to the
-// leaf's new keys; then lane l finds the merge-path split for output slots
-// 4l .. 4l+3, merges its four outputs and writes them back as one 16-byte
-// store per array, packed left, the rest of the leaf emptied.
-constexpr uint32_t kMergeGroups = 16;  // leaves per 256-thread workgroup
+// kLeafLanes lanes per touched leaf (the grid covers every possible leaf;
+// groups past the touched count exit).  Lane l of a group reads slots
+// kLaneSlots * l .. + kLaneSlots - 1 (16-byte loads); the leaf's elements
+// (a leaf may hold holes after deletes) are compacted by a group prefix
+// into LDS next to the leaf's new keys; then lane l finds the merge-path
+// split for its output slots, merges its outputs and writes them back with
+// 16-byte stores, packed left, the rest of the leaf emptied.  16 lanes (four
+// leaves per wave) measured fastest: one wave per leaf needs four times the
+// waves, and 8 lanes per leaf doubles the LDS per workgroup (lower
+// occupancy) and the serial merge per lane.
+constexpr uint32_t kLeafLanes = 16, kLaneSlots = kLeafSlots / kLeafLanes, kLaneQuads = kLaneSlots / 4;
+constexpr uint32_t kMergeGroups = 256 / kLeafLanes;  // leaves per 256-thread workgroup
 
 __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t *T, const uint32_t *Tstart,
                                                         const uint64_t *ntp, const uint64_t *keys,
@@ -1047,42 +1051,47 @@ __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t 
   __shared__ uint64_t s_old[kMergeGroups][kLeafSlots];
   __shared__ uint64_t s_new[kMergeGroups][kLeafSlots];
   constexpr uint64_t kNone = ~0ull;
-  const uint32_t q = threadIdx.x >> 4, l = threadIdx.x & 15;
+  const uint32_t q = threadIdx.x / kLeafLanes, l = threadIdx.x % kLeafLanes;
   const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kMergeGroups + q;
   const bool live = t < nt;
   uint64_t leaf = 0, base = 0;
   uint32_t nb = 0, c = 0;
-  uint4 sv = make_uint4(0, 0, 0, 0);
+  uint32_t sv[kLaneSlots] = {};  // old sources of this lane's slots (kept in emptied slots)
   if (live) {
     leaf = T[t];
     base = leaf << kLeafShift;
     const uint32_t kb = Tstart[t];
     nb = Tstart[t + 1] - kb;
-    const uint4 dv = reinterpret_cast<const uint4 *>(g.pdst + base)[l];
-    sv = reinterpret_cast<const uint4 *>(g.psrc + base)[l];
-    const uint32_t dd[4] = {dv.x, dv.y, dv.z, dv.w}, ss[4] = {sv.x, sv.y, sv.z, sv.w};
+    uint32_t dd[kLaneSlots];
+#pragma unroll
+    for (uint32_t h = 0; h < kLaneQuads; h++) {
+      const uint4 dv = reinterpret_cast<const uint4 *>(g.pdst + base)[kLaneQuads * l + h];
+      const uint4 sq = reinterpret_cast<const uint4 *>(g.psrc + base)[kLaneQuads * l + h];
+      dd[4 * h] = dv.x, dd[4 * h + 1] = dv.y, dd[4 * h + 2] = dv.z, dd[4 * h + 3] = dv.w;
+      sv[4 * h] = sq.x, sv[4 * h + 1] = sq.y, sv[4 * h + 2] = sq.z, sv[4 * h + 3] = sq.w;
+    }
     uint32_t cnt = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < 4; j++) cnt += dd[j] != kEmpty ? 1u : 0u;
+    for (uint32_t j = 0; j < kLaneSlots; j++) cnt += dd[j] != kEmpty ? 1u : 0u;
     uint32_t inc = cnt;
-    for (uint32_t o = 1; o < 16; o <<= 1) {
-      const uint32_t y = __shfl_up(inc, o, 16);
+    for (uint32_t o = 1; o < kLeafLanes; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o, kLeafLanes);
       if (l >= o) inc += y;
     }
-    c = __shfl(inc, 15, 16);
+    c = __shfl(inc, kLeafLanes - 1, kLeafLanes);
     uint32_t pos = inc - cnt;
 #pragma unroll
-    for (uint32_t j = 0; j < 4; j++)
-      if (dd[j] != kEmpty) s_old[q][pos++] = make_key(ss[j], dd[j]);
+    for (uint32_t j = 0; j < kLaneSlots; j++)
+      if (dd[j] != kEmpty) s_old[q][pos++] = make_key(sv[j], dd[j]);
     if (l == 0 && c != g.pcnt[leaf]) atomicOr(reinterpret_cast<unsigned long long *>(err), 2ull);
-    for (uint32_t j = l; j < nb; j += 16) s_new[q][j] = keys[kb + j];
+    for (uint32_t j = l; j < nb; j += kLeafLanes) s_new[q][j] = keys[kb + j];
   }
   __syncthreads();
   if (!live) return;
   // c + nb <= up[0] < 64.  Merge-path split for output o0: i old elements
   // and o0 - i new keys come before it (the two lists are disjoint).
   const uint64_t *A = s_old[q], *B = s_new[q];
-  const uint32_t total = c + nb, o0 = 4 * l;
+  const uint32_t total = c + nb, o0 = kLaneSlots * l;
   uint32_t lo = o0 > nb ? o0 - nb : 0, hi = o0 < c ? o0 : c;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -1096,37 +1105,41 @@ __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t 
     const uint64_t pa = i > 0 ? A[i - 1] : 0, pb = j > 0 ? B[j - 1] : 0;
     prev = pa > pb ? pa : pb;
   }
-  uint64_t out[4];
+  uint64_t out[kLaneSlots];
 #pragma unroll
-  for (uint32_t r = 0; r < 4; r++) {
+  for (uint32_t r = 0; r < kLaneSlots; r++) {
     out[r] = kNone;
     if (o0 + r < total) {
       if (j >= nb || (i < c && A[i] < B[j])) out[r] = A[i++];
       else out[r] = B[j++];
     }
   }
-  uint4 nd, ns;
-  nd.x = o0 + 0 < total ? key_dst(out[0]) : kEmpty;
-  nd.y = o0 + 1 < total ? key_dst(out[1]) : kEmpty;
-  nd.z = o0 + 2 < total ? key_dst(out[2]) : kEmpty;
-  nd.w = o0 + 3 < total ? key_dst(out[3]) : kEmpty;
-  ns.x = o0 + 0 < total ? key_src(out[0]) : sv.x;
-  ns.y = o0 + 1 < total ? key_src(out[1]) : sv.y;
-  ns.z = o0 + 2 < total ? key_src(out[2]) : sv.z;
-  ns.w = o0 + 3 < total ? key_src(out[3]) : sv.w;
-  reinterpret_cast<uint4 *>(g.pdst + base)[l] = nd;
-  reinterpret_cast<uint4 *>(g.psrc + base)[l] = ns;
+#pragma unroll
+  for (uint32_t h = 0; h < kLaneQuads; h++) {
+    uint4 nd, ns;
+    const uint32_t r = 4 * h;
+    nd.x = o0 + r < total ? key_dst(out[r]) : kEmpty;
+    nd.y = o0 + r + 1 < total ? key_dst(out[r + 1]) : kEmpty;
+    nd.z = o0 + r + 2 < total ? key_dst(out[r + 2]) : kEmpty;
+    nd.w = o0 + r + 3 < total ? key_dst(out[r + 3]) : kEmpty;
+    ns.x = o0 + r < total ? key_src(out[r]) : sv[r];
+    ns.y = o0 + r + 1 < total ? key_src(out[r + 1]) : sv[r + 1];
+    ns.z = o0 + r + 2 < total ? key_src(out[r + 2]) : sv[r + 2];
+    ns.w = o0 + r + 3 < total ? key_src(out[r + 3]) : sv[r + 3];
+    reinterpret_cast<uint4 *>(g.pdst + base)[kLaneQuads * l + h] = nd;
+    reinterpret_cast<uint4 *>(g.psrc + base)[kLaneQuads * l + h] = ns;
+  }
   // Run starts in this leaf move the vertex's begin pointer.  A pointer
   // into this leaf (the run started here) moves with a CAS, which fails
   // only when an earlier leaf's wave has already lowered it to a new
   // earlier element; any other pointer takes the minimum.  Every
-  // interleaving leaves the slot of the vertex's first element.  The four
-  // pointer loads of a lane are issued together: each of its outputs that
-  // starts a run belongs to a different vertex.
-  bool start[4];
-  unsigned long long cur[4];
+  // interleaving leaves the slot of the vertex's first element.  A lane's
+  // pointer loads are issued together: each of its outputs that starts a
+  // run belongs to a different vertex.
+  bool start[kLaneSlots];
+  unsigned long long cur[kLaneSlots];
 #pragma unroll
-  for (uint32_t r = 0; r < 4; r++) {
+  for (uint32_t r = 0; r < kLaneSlots; r++) {
     const uint64_t before = r == 0 ? prev : out[r - 1];
     start[r] = o0 + r < total && (before == kNone || key_src(before) != key_src(out[r]));
     cur[r] = start[r] ? __hip_atomic_load(reinterpret_cast<unsigned long long *>(&g.vb[key_src(out[r])].aux),
@@ -1134,7 +1147,7 @@ __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t 
                       : 0ull;
   }
 #pragma unroll
-  for (uint32_t r = 0; r < 4; r++) {
+  for (uint32_t r = 0; r < kLaneSlots; r++) {
     if (!start[r]) continue;
     unsigned long long *aux = reinterpret_cast<unsigned long long *>(&g.vb[key_src(out[r])].aux);
     const unsigned long long slot = base + o0 + r;
@@ -2476,7 +2489,7 @@ void Graph::pma_fast_launch(const uint64_t *keys, uint64_t k) {
   k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get(), fitp);
   TG_CHECK_LAUNCH();
   select(SelTouched{u0_.get(), T.get(), Tstart.get()}, k, ntp, scan_scratch, stream_);
-  // sixteen lanes per touched leaf (at most min(k, leaves) of them)
+  // kLeafLanes lanes per touched leaf (at most min(k, leaves) of them)
   const uint64_t most = k < (1ull << pma_H_) ? k : (1ull << pma_H_);
   const uint64_t wg64 = (most + kMergeGroups - 1) / kMergeGroups;
   if (wg64 > 0x7FFFFFFFull) throw Error(-1, "PMA merge: too many keys");
