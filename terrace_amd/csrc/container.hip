@@ -656,13 +656,18 @@ __global__ void k_exists_n(GView g, const uint64_t *keys, const uint64_t *np, ui
 // count into *nruns and run_beg[nruns] = *np.  Two look-back columns (new
 // keys, run starts), each walked by its own wave; ranks within the tile
 // come from ballots, so both outputs follow index order.
-constexpr int kPrepBlock = 256, kPrepItems = 8, kPrepTile = kPrepBlock * kPrepItems;
+constexpr int kPrepBlock = 256;
 
-__global__ void __launch_bounds__(kPrepBlock) k_insert_prep(const uint64_t *R, const uint64_t *np,
-                                                            uint64_t cap, const uint32_t *ex, uint32_t *nidx,
+// kItems keys per thread.  kLookup: the existence flags are computed here
+// (one key per thread keeps the lookups' memory parallelism) and written
+// to ex; otherwise ex holds them already.
+template <int kItems, bool kLookup>
+__global__ void __launch_bounds__(kPrepBlock) k_insert_prep(GView g, const uint64_t *R, const uint64_t *np,
+                                                            uint64_t cap, uint32_t *ex, uint32_t *nidx,
                                                             uint32_t *run_beg, uint64_t *nruns, uint64_t *state,
                                                             uint32_t *misc, uint32_t *err, uint32_t nblocks,
                                                             uint32_t epoch) {
+  constexpr int kPrepItems = kItems, kPrepTile = kPrepBlock * kItems;
   __shared__ uint32_t wcnt[2][kPrepItems][kPrepBlock / 64];
   __shared__ uint32_t wpre[2][kPrepItems][kPrepBlock / 64];
   __shared__ uint32_t s_ticket, s_run[2];
@@ -680,8 +685,9 @@ __global__ void __launch_bounds__(kPrepBlock) k_insert_prep(const uint64_t *R, c
     e[i] = false;
     st[i] = false;
     if (idx < n) {
-      e[i] = ex[idx] != 0;
-      st[i] = idx == 0 || key_src(R[idx - 1]) != key_src(R[idx]);
+      const uint64_t key = R[idx];
+      e[i] = kLookup ? edge_exists(g, key_src(key), key_dst(key)) : ex[idx] != 0;
+      st[i] = idx == 0 || key_src(R[idx - 1]) != key_src(key);
     }
   }
   uint32_t rn[kPrepItems], rs[kPrepItems];
@@ -717,6 +723,7 @@ __global__ void __launch_bounds__(kPrepBlock) k_insert_prep(const uint64_t *R, c
   for (int i = 0; i < kPrepItems; i++) {
     const uint64_t idx = base + static_cast<uint64_t>(i) * kPrepBlock + t;
     if (idx >= cap) continue;
+    if (kLookup) ex[idx] = e[i] ? 1u : 0u;
     nidx[idx] = static_cast<uint32_t>(s_off[0] + wpre[0][i][w] + rn[i]);
     if (st[i]) run_beg[s_off[1] + wpre[1][i][w] + rs[i]] = static_cast<uint32_t>(idx);
   }
@@ -2802,21 +2809,31 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
     f0_.ensure(count + 1, stream_);
     f1_.ensure(count + 1, stream_);
     s32_[0].ensure(count + 1, stream_);
-    k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, R, dsc_.get() + 1, count, f0_.get(), f1_.get());
-    TG_CHECK_LAUNCH();
+    if (small || count > kBucketSortMax) {
+      k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, R, dsc_.get() + 1, count, f0_.get(), f1_.get());
+      TG_CHECK_LAUNCH();
+    }
     if (small) {
       // new-key prefix, source runs and run_beg[nruns] = nR in one workgroup
       k_small_runs<true><<<1, 1024, 0, stream_>>>(R, f1_.get(), static_cast<uint32_t>(count), nullptr,
                                                   s32_[0].get(), dsc_.get());
       TG_CHECK_LAUNCH();
     } else {
-      // new-key prefix and source runs in one launch
-      const uint32_t nb = static_cast<uint32_t>((count + kPrepTile - 1) / kPrepTile);
-      const uint32_t epoch = scan_scratch.lb.begin(2ull * nb, stream_);
-      k_insert_prep<<<nb, kPrepBlock, 0, stream_>>>(R, dsc_.get() + 1, count, f0_.get(), f1_.get(),
-                                                    s32_[0].get(), dsc_.get() + 2, scan_scratch.lb.state.get(),
-                                                    scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb,
-                                                    epoch);
+      // new-key prefix and source runs in one launch; up to kBucketSortMax
+      // keys it also does the lookups, one key per thread
+      if (count <= kBucketSortMax) {
+        const uint32_t nb = static_cast<uint32_t>((count + kPrepBlock - 1) / kPrepBlock);
+        const uint32_t epoch = scan_scratch.lb.begin(2ull * nb, stream_);
+        k_insert_prep<1, true><<<nb, kPrepBlock, 0, stream_>>>(
+            g, R, dsc_.get() + 1, count, f0_.get(), f1_.get(), s32_[0].get(), dsc_.get() + 2,
+            scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb, epoch);
+      } else {
+        const uint32_t nb = static_cast<uint32_t>((count + 8 * kPrepBlock - 1) / (8 * kPrepBlock));
+        const uint32_t epoch = scan_scratch.lb.begin(2ull * nb, stream_);
+        k_insert_prep<8, false><<<nb, kPrepBlock, 0, stream_>>>(
+            g, R, dsc_.get() + 1, count, f0_.get(), f1_.get(), s32_[0].get(), dsc_.get() + 2,
+            scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb, epoch);
+      }
       TG_CHECK_LAUNCH();
     }
     read_scalars(dsc_.get(), h, 3, stream_);
