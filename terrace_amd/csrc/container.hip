@@ -735,6 +735,48 @@ __global__ void __launch_bounds__(kPrepBlock) k_insert_prep(GView g, const uint6
   }
 }
 
+// Delete preparation of a batch of kSmallSortKeys to kBucketSortMax keys in
+// one single-pass launch: the existing keys among the first *np of R
+// compacted into out in order (count into *nout), one key per thread (the
+// lookups keep their memory parallelism).
+__global__ void __launch_bounds__(kPrepBlock) k_delete_prep(GView g, const uint64_t *R, const uint64_t *np,
+                                                            uint64_t *out, uint64_t *nout, uint64_t *state,
+                                                            uint32_t *misc, uint32_t *err, uint32_t nblocks,
+                                                            uint32_t epoch) {
+  __shared__ uint32_t wcnt[kPrepBlock / 64];
+  __shared__ uint32_t s_ticket;
+  __shared__ uint64_t s_off;
+  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
+  if (b >= nblocks) return;
+  const uint64_t n = *np;
+  const uint32_t t = threadIdx.x, w = t >> 6;
+  const uint64_t idx = static_cast<uint64_t>(b) * kPrepBlock + t;
+  uint64_t key = 0;
+  bool e = false;
+  if (idx < n) {
+    key = R[idx];
+    e = edge_exists(g, key_src(key), key_dst(key));
+  }
+  const uint64_t m = __ballot(e);
+  const uint32_t r = static_cast<uint32_t>(__popcll(m & lanemask_lt()));
+  if ((t & 63) == 0) wcnt[w] = static_cast<uint32_t>(__popcll(m));
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+  for (uint32_t v = 0; v < kPrepBlock / 64; v++) {
+    before += v < w ? wcnt[v] : 0u;
+    tot += wcnt[v];
+  }
+  if (t < 64) {  // wave 0 walks the look-back
+    const uint32_t off = lbk::lookback_wave(state, 1, b, 0, err, epoch, tot);
+    if (t == 0) {
+      s_off = off;
+      if (b == nblocks - 1) *nout = off + tot;
+    }
+  }
+  __syncthreads();
+  if (e) out[s_off + before + r] = key;
+}
+
 // ------------------------------------------------------------ inline merge
 // One thread per source run of the unique sorted batch R.  ex[] marks keys
 // already present; nidx[] is the exclusive count of new keys (size |R|+1).
@@ -2997,8 +3039,10 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
     GView g = view();
     k2_.ensure(count + 1, stream_);
     f0_.ensure(count + 1, stream_);
-    k_exists_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, k1_.get(), dsc_.get() + 1, count, f0_.get());
-    TG_CHECK_LAUNCH();
+    if (small || count > kBucketSortMax) {
+      k_exists_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, k1_.get(), dsc_.get() + 1, count, f0_.get());
+      TG_CHECK_LAUNCH();
+    }
     s32_[0].ensure(count + 1, stream_);
     if (small) {
       // existing keys, their source runs and run_beg[nruns] in one workgroup
@@ -3006,7 +3050,17 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
                                                    s32_[0].get(), dsc_.get());
       TG_CHECK_LAUNCH();
     } else {
-      select(SelFlagged{k1_.get(), f0_.get(), k2_.get()}, count, dsc_.get() + 2, scan_scratch, stream_);
+      if (count <= kBucketSortMax) {
+        // existence and compaction of the existing keys in one launch
+        const uint32_t nb = static_cast<uint32_t>((count + kPrepBlock - 1) / kPrepBlock);
+        const uint32_t epoch = scan_scratch.lb.begin(nb, stream_);
+        k_delete_prep<<<nb, kPrepBlock, 0, stream_>>>(g, k1_.get(), dsc_.get() + 1, k2_.get(), dsc_.get() + 2,
+                                                      scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(),
+                                                      scan_scratch.lb.err_word(), nb, epoch);
+        TG_CHECK_LAUNCH();
+      } else {
+        select(SelFlagged{k1_.get(), f0_.get(), k2_.get()}, count, dsc_.get() + 2, scan_scratch, stream_);
+      }
       select(SelRunStartN{k2_.get(), s32_[0].get(), dsc_.get() + 2}, count, dsc_.get() + 3, scan_scratch,
              stream_);
     }
