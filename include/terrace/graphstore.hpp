@@ -5,14 +5,18 @@
 // -lterrace_hip.
 //
 // Algorithm code keeps the reference's shape: functor structs passed to
-// edgeMap / vertexMap.  Device functors cannot cross a C ABI, so each
-// functor type maps to a device op through edge_op_traits / vertex_op_traits;
-// a functor without a trait is rejected at compile time.  Functor state
-// lives in HBM (DeviceArray<T>), not in host arrays.
+// edgeMap / vertexMap, built with the reference's constructor arguments.
+// Device functors cannot cross a C ABI, so each functor type maps to a
+// device op through edge_op_traits / vertex_op_traits; a functor without a
+// trait is rejected at compile time.  Functor state lives in HBM
+// (DeviceArray<T>, which converts to the T* the functors hold), not in
+// host arrays.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -45,17 +49,19 @@ class DeviceArray {
     if (p_) tg_dev_free(p_);
   }
   T *data() const { return p_; }
+  operator T *() const { return p_; }  // functors hold the device pointer
   uint64_t size() const { return n_; }
   void upload(const T *h, uint64_t count) { tg_check(tg_memcpy_h2d(p_, h, count * sizeof(T))); }
   void download(T *h, uint64_t count) const { tg_check(tg_memcpy_d2h(h, p_, count * sizeof(T))); }
   void fill(const T &v) {
-    std::vector<T> h(n_, v);
-    upload(h.data(), n_);
+    std::unique_ptr<T[]> h(new T[n_ ? n_ : 1]);
+    std::fill(h.get(), h.get() + n_, v);
+    upload(h.get(), n_);
   }
   void set(uint64_t i, const T &v) { tg_check(tg_memcpy_h2d(p_ + i, &v, sizeof(T))); }
-  std::vector<T> to_host() const {
-    std::vector<T> h(n_);
-    download(h.data(), n_);
+  std::unique_ptr<T[]> to_host() const {
+    std::unique_ptr<T[]> h(new T[n_ ? n_ : 1]);
+    download(h.get(), n_);
     return h;
   }
 
@@ -152,17 +158,16 @@ class Graph {
     return r;
   }
   uint32_t get_num_vertices(void) const { return n_; }
-  // graph.h:1085 (TC.h:31-47): common neighbours below min(s, d).
+  // graph.h:1085-1255: common neighbours of s and d below min(s, d), on
+  // the device (tg_count_common).
   uint32_t count_common(const vertex s, const vertex d) const {
-    std::vector<vertex> a = neighbor_list(s), b = neighbor_list(d);
     uint32_t c = 0;
-    size_t i = 0, j = 0;
-    while (i < a.size() && j < b.size() && a[i] < s && b[j] < d) {
-      if (a[i] == b[j]) { c++; i++; j++; }
-      else if (a[i] < b[j]) i++;
-      else j++;
-    }
+    tg_check(tg_count_common(h_, &s, &d, 1, &c));
     return c;
+  }
+  // Many pairs in one call: out[i] = count_common(srcs[i], dests[i]).
+  void count_common(const vertex *srcs, const vertex *dests, uint64_t q, uint32_t *out) const {
+    tg_check(tg_count_common(h_, srcs, dests, q, out));
   }
   std::vector<vertex> neighbor_list(vertex v) const {
     uint32_t deg = 0;
@@ -178,22 +183,37 @@ class Graph {
   uint32_t n_;
 };
 
-// VertexSubset (include/VertexSubset.cpp): VertexSubset(e, n) holds one
-// vertex, VertexSubset(0, n, true) every vertex.  Copies share the device
-// frontier as the reference's shallow copies do; del() releases it.
+// VertexSubset (include/VertexSubset.cpp).  VertexSubset(e, n) holds one
+// vertex, VertexSubset(0, n, true) every vertex, VertexSubset(els, n) the
+// vertices whose els[i] is true; none of them names a graph, as in the
+// reference: the first edgeMap binds the subset to its graph.  Copies share
+// the device frontier as the reference's shallow copies do; del() releases
+// it.
 class VertexSubset {
  public:
   VertexSubset() = default;
-  VertexSubset(Graph &g, uint32_t e, uint64_t max_el, bool all = false) : g_(&g) {
-    (void)max_el;
+  // VertexSubset(el_t e, uint64_t max_el, bool all) (VertexSubset.cpp:109)
+  VertexSubset(uint32_t e, uint64_t max_el, bool all = false) : n_(max_el) {
+    tg_check(tg_frontier_create_n(static_cast<uint32_t>(max_el), &f_));
+    if (all) tg_check(tg_frontier_set_all(f_));
+    else tg_check(tg_frontier_set(f_, &e, 1));
+  }
+  // VertexSubset(bool *els, uint64_t len) (VertexSubset.cpp:119)
+  VertexSubset(bool *els, uint64_t len) : n_(len) {
+    static_assert(sizeof(bool) == 1, "dense subsets are passed as bytes");
+    tg_check(tg_frontier_create_n(static_cast<uint32_t>(len), &f_));
+    tg_check(tg_frontier_set_dense(f_, reinterpret_cast<const uint8_t *>(els), len));
+  }
+  // The same bound to g from the start.
+  VertexSubset(Graph &g, uint32_t e, uint64_t max_el, bool all = false) : n_(max_el) {
     tg_check(tg_frontier_create(g.handle(), &f_));
     if (all) tg_check(tg_frontier_set_all(f_));
     else tg_check(tg_frontier_set(f_, &e, 1));
   }
-  static VertexSubset empty(Graph &g) {
+  static VertexSubset empty(uint64_t n) {
     VertexSubset v;
-    v.g_ = &g;
-    tg_check(tg_frontier_create(g.handle(), &v.f_));
+    v.n_ = n;
+    tg_check(tg_frontier_create_n(static_cast<uint32_t>(n), &v.f_));
     return v;
   }
   uint64_t get_n() const {
@@ -210,7 +230,7 @@ class VertexSubset {
     return out;
   }
   VertexSubset copy() const {
-    VertexSubset v = empty(*g_);
+    VertexSubset v = empty(n_);
     tg_check(tg_frontier_copy(v.f_, f_));
     return v;
   }
@@ -219,10 +239,10 @@ class VertexSubset {
     f_ = nullptr;
   }
   tg_frontier *handle() const { return f_; }
-  Graph *graph() const { return g_; }
+  uint64_t max_el() const { return n_; }
 
  private:
-  Graph *g_ = nullptr;
+  uint64_t n_ = 0;
   tg_frontier *f_ = nullptr;
 };
 
@@ -241,12 +261,22 @@ struct has_vertex_op : std::false_type {};
 template <class F>
 struct has_vertex_op<F, std::void_t<decltype(vertex_op_traits<F>::op)>> : std::true_type {};
 
+// The graph a vertex functor holds (PR_Vertex(p, G) does), else none.
+template <class F, class = void>
+struct functor_graph {
+  static tg_graph *get(const F &) { return nullptr; }
+};
+template <class F>
+struct functor_graph<F, std::void_t<decltype(std::declval<const F &>().G.handle())>> {
+  static tg_graph *get(const F &f) { return f.G.handle(); }
+};
+
 // edgeMap (include/Map.cpp:66-74).
 template <class F>
 VertexSubset edgeMap(Graph &G, VertexSubset &vs, F f, bool output = true, uint32_t threshold = 20) {
   static_assert(has_edge_op<F>::value, "edgeMap: functor has no device op (edge_op_traits)");
   tg_op_args a = edge_op_traits<F>::args(f);
-  VertexSubset out = output ? VertexSubset::empty(G) : VertexSubset();
+  VertexSubset out = output ? VertexSubset::empty(G.get_num_vertices()) : VertexSubset();
   tg_check(tg_edge_map(G.handle(), vs.handle(), edge_op_traits<F>::op, &a, output ? out.handle() : nullptr,
                        threshold));
   return out;
@@ -258,7 +288,7 @@ template <class F>
 VertexSubset vertexMap(VertexSubset &vs, F f, bool output = true) {
   static_assert(has_vertex_op<F>::value, "vertexMap: functor has no device op (vertex_op_traits)");
   tg_vmap_args a = vertex_op_traits<F>::args(f);
-  tg_check(tg_vertex_map(vs.graph()->handle(), vs.handle(), vertex_op_traits<F>::op, &a));
+  tg_check(tg_vertex_map(functor_graph<F>::get(f), vs.handle(), vertex_op_traits<F>::op, &a));
   return output ? vs.copy() : VertexSubset();
 }
 

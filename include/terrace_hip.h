@@ -27,8 +27,16 @@ extern "C" {
 
 #define TG_OK 0
 #define TG_EINVAL (-1)   /* bad argument (id out of range, null pointer) */
-#define TG_ENOMEM (-2)   /* device allocation failed; the graph is unchanged */
+#define TG_ENOMEM (-2)   /* device allocation failed before the call changed anything;
+                            the graph is unchanged */
 #define TG_EDEVICE (-3)  /* HIP error; message via tg_last_error() */
+/* Internal check failed.  When an update batch fails after it began changing
+ * the graph (an allocation for the graph's own growth: PMA, B-tree blocks or
+ * index; or an internal consistency check), the adjacency may be
+ * half-updated: the graph is marked invalid, the call returns TG_EINTERNAL,
+ * and every later call on that graph returns TG_EINTERNAL.  A failure in the
+ * batch's preparation (sort, existence check, source runs) is reported
+ * before any write and leaves the graph valid and unchanged. */
 #define TG_EINTERNAL (-4)
 
 typedef struct tg_graph tg_graph;
@@ -80,6 +88,12 @@ int tg_remove_edge(tg_graph *g, uint32_t s, uint32_t d, int *result);
 /* Graph::is_edge (include/graph.h:643-673): *out = 1 or 0 */
 int tg_is_edge(tg_graph *g, uint32_t s, uint32_t d, uint32_t *out);
 int tg_is_edge_batch(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t count, uint8_t *out);
+/* Graph::count_common (include/graph.h:1085-1255, declared :171-211) for q
+ * pairs at once: out[i] = |N(src[i]) ∩ N(dst[i]) ∩ [0, min(src[i], dst[i]))|,
+ * the common neighbours below both endpoints, over every pair of tiers
+ * (inline, PMA, B-tree).  _dev: device arrays. */
+int tg_count_common(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out);
+int tg_count_common_dev(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out);
 /* Graph::degree (include/graph.h:797-803) */
 int tg_degree(tg_graph *g, uint32_t v, uint32_t *out);
 int tg_degrees_all(tg_graph *g, uint32_t *out /* [n] */);
@@ -127,14 +141,18 @@ int tg_bc_dev(tg_graph *g, uint32_t src, double *out);
 int tg_sssp_bf_dev(tg_graph *g, uint32_t src, int32_t *dist, uint32_t *rounds);
 
 /* ---- frontiers and edgeMap (include/VertexSubset.cpp, include/Map.cpp) */
-enum tg_op { TG_OP_BFS = 1, TG_OP_CC = 2, TG_OP_BC_FWD = 3, TG_OP_BC_BACK = 4, TG_OP_PR = 5, TG_OP_BF = 7 };
+enum tg_op { TG_OP_BFS = 1, TG_OP_CC = 2, TG_OP_BC_FWD = 3, TG_OP_BC_BACK = 4, TG_OP_PR = 5, TG_OP_TC = 6,
+             TG_OP_BF = 7 };
 /* Device arrays an op reads and writes (all device pointers):
  *   BFS     parent (int32, -1 = unvisited), level (optional), depth
  *   CC      ids, prev (prevIDs; output = vertices whose id changed)
  *   BC_*    val (NumPaths / Dependencies), visited (uint8)
  *   PR      src_val (normalised p_curr), val (p_next, assigned)
  *   BF      dist (int32 ShortestPathLen), visited (claimed: the array is
- *           written and must be padded to a multiple of 4 bytes) */
+ *           written and must be padded to a multiple of 4 bytes)
+ *   TC      count (uint64 device counter): TC.h's countF (TC.h:51-70);
+ *           count += count_common(u, v) for every frontier vertex u and
+ *           neighbour v < u; no output frontier (out must be NULL) */
 typedef struct {
   int32_t *parent;
   uint32_t *level;
@@ -145,9 +163,18 @@ typedef struct {
   const double *src_val;
   const uint8_t *visited;
   int32_t *dist;
+  uint64_t *count;
 } tg_op_args;
-/* VertexSubset(el_t e, uint64_t n, bool all) */
+/* An empty frontier of graph g's vertex count. */
 int tg_frontier_create(tg_graph *g, tg_frontier **out);
+/* VertexSubset(el_t e, uint64_t max_el, bool all) and VertexSubset(bool *els,
+ * uint64_t len) (include/VertexSubset.cpp:109-128) carry no graph: an empty
+ * frontier over [0, n) that the first tg_edge_map (or tg_vertex_map with a
+ * graph) binds to its graph, whose vertex count must be n.  Until then
+ * tg_vertex_map accepts g = NULL for every vertex op but PR_VERTEX. */
+int tg_frontier_create_n(uint32_t n, tg_frontier **out);
+/* members = { i : els[i] != 0 }, len = the frontier's n */
+int tg_frontier_set_dense(tg_frontier *f, const uint8_t *els, uint64_t len);
 int tg_frontier_destroy(tg_frontier *f);
 int tg_frontier_set(tg_frontier *f, const uint32_t *verts, uint64_t count); /* host list */
 int tg_frontier_set_all(tg_frontier *f);
@@ -179,7 +206,7 @@ typedef struct {
   uint32_t *prev;
   uint8_t *visited;
 } tg_vmap_args;
-int tg_vertex_map(tg_graph *g, tg_frontier *f, int vop, const tg_vmap_args *args);
+int tg_vertex_map(tg_graph *g, tg_frontier *f, int vop, const tg_vmap_args *args); /* g: see create_n */
 /* Copy of a frontier (VertexSubset(other, true)). */
 int tg_frontier_copy(tg_frontier *dst, tg_frontier *src);
 
@@ -305,6 +332,13 @@ int tg_profile_get(tg_graph *g, const char *name, double *ms, uint64_t *launches
  * [8] PMA inserts merged inside their leaves (no window redistribution),
  * [9] elements held in the PMA */
 int tg_stats(tg_graph *g, uint64_t *out /* [10] */);
+/* Failure injection for tests of the error paths; no counterpart in the
+ * reference.  what = 1: the next update batch's preparation reports a
+ * look-back error (TG_EINTERNAL, graph unchanged and still valid); what = 2:
+ * the next batch fails as if an allocation failed after its first write
+ * (TG_EINTERNAL, graph marked invalid).  Every kernel still runs to its
+ * normal end. */
+int tg_debug_fail_next(tg_graph *g, int what);
 
 #ifdef __cplusplus
 }

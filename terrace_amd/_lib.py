@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG, "libterrace_hip.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "terrace_hip.h")
 
 TG_OK, TG_EINVAL, TG_ENOMEM, TG_EDEVICE, TG_EINTERNAL = 0, -1, -2, -3, -4
-OP_BFS, OP_CC, OP_BC_FWD, OP_BC_BACK, OP_PR, OP_BF = 1, 2, 3, 4, 5, 7
+OP_BFS, OP_CC, OP_BC_FWD, OP_BC_BACK, OP_PR, OP_TC, OP_BF = 1, 2, 3, 4, 5, 6, 7
 
 
 class TerraceError(RuntimeError):
@@ -36,6 +36,7 @@ class OpArgs(ctypes.Structure):
         ("src_val", ctypes.c_void_p),
         ("visited", ctypes.c_void_p),
         ("dist", ctypes.c_void_p),
+        ("count", ctypes.c_void_p),
     ]
 
 
@@ -151,6 +152,11 @@ _SIGS = {
                                       ctypes.POINTER(ctypes.c_uint64)]),
     "tg_stats": (ctypes.c_int, [_P, _P]),
     "tg_tier_counts": (ctypes.c_int, [_P, _P]),
+    "tg_debug_fail_next": (ctypes.c_int, [_P, ctypes.c_int]),
+    "tg_count_common": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64, _P]),
+    "tg_count_common_dev": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64, _P]),
+    "tg_frontier_create_n": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(_P)]),
+    "tg_frontier_set_dense": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
 }
 
 

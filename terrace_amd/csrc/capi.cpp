@@ -7,6 +7,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "gen.hpp"
 #include "io.hpp"
@@ -232,6 +233,28 @@ int tg_is_edge_batch(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint
     G->s_u8.ensure(count + 1, st);
     G->g.is_edge_batch(G->s_src.get(), G->s_dst.get(), count, G->s_u8.get());
     d2h(out, G->s_u8.get(), count, st);
+  });
+}
+
+int tg_count_common(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out) {
+  return guard([&] {
+    NEED(G && (q == 0 || (src && dst && out)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    if (!q) return;
+    hipStream_t st = G->g.stream();
+    stage_pair(G, src, dst, q);
+    G->s_u32.ensure(q + 1, st);
+    G->g.count_common(G->s_src.get(), G->s_dst.get(), q, G->s_u32.get());
+    d2h(out, G->s_u32.get(), q * 4, st);
+  });
+}
+
+int tg_count_common_dev(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out) {
+  return guard([&] {
+    NEED(G && (q == 0 || (src && dst && out)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->g.count_common(src, dst, q, out);
+    TG_HIP(hipStreamSynchronize(G->g.stream()));
   });
 }
 
@@ -473,27 +496,80 @@ int tg_frontier_create(tg_graph *G, tg_frontier **out) {
   });
 }
 
+int tg_frontier_create_n(uint32_t n, tg_frontier **out) {
+  return guard([&] {
+    NEED(out && n, "null argument or n = 0");
+    auto *f = new tg_frontier;
+    f->owner = nullptr;  // bound to a graph by its first edge map
+    f->f.n = n;
+    f->f.count = 0;
+    f->f.has_queue = true;
+    f->f.queue.ensure(n, nullptr);
+    *out = f;
+  });
+}
+
 int tg_frontier_destroy(tg_frontier *f) {
   return guard([&] { delete f; });
 }
 
+namespace {
+// The frontier's graph lock and stream; an unbound frontier uses the null
+// stream with synchronous copies.
+struct FrontierCtx {
+  std::unique_lock<std::mutex> lk;
+  hipStream_t st = nullptr;
+  explicit FrontierCtx(tg_frontier *F) {
+    if (F->owner) {
+      lk = std::unique_lock<std::mutex>(F->owner->g.mu);
+      st = F->owner->g.stream();
+    }
+  }
+};
+
+// Binds an unbound frontier to G (same vertex count); false when F belongs
+// to another graph.
+bool bind_frontier(tg_frontier *F, tg_graph *G) {
+  if (!F->owner) {
+    NEED(F->f.n == G->g.num_vertices(), "frontier size differs from the graph's vertex count");
+    F->owner = G;
+  }
+  return F->owner == G;
+}
+
+void frontier_upload(tg_frontier *F, std::vector<uint32_t> &v, hipStream_t st) {
+  F->f.queue.ensure(F->f.n, st);
+  h2d(F->f.queue.get(), v.data(), v.size() * 4, st);
+  F->f.count = v.size();
+  F->f.all = false;
+  F->f.has_queue = true;
+  F->f.has_bits = false;
+  TG_HIP(hipStreamSynchronize(st));
+}
+}  // namespace
+
 int tg_frontier_set(tg_frontier *F, const uint32_t *verts, uint64_t count) {
   return guard([&] {
     NEED(F && (count == 0 || verts), "null argument");
-    std::lock_guard<std::mutex> l(F->owner->g.mu);
+    FrontierCtx c(F);
     uint32_t n = F->f.n;
     std::vector<uint32_t> v(verts, verts + count);
     std::sort(v.begin(), v.end());
     v.erase(std::unique(v.begin(), v.end()), v.end());
     NEED(v.empty() || v.back() < n, "vertex id out of range");
-    hipStream_t st = F->owner->g.stream();
-    F->f.queue.ensure(n, st);
-    h2d(F->f.queue.get(), v.data(), v.size() * 4, st);
-    F->f.count = v.size();
-    F->f.all = false;
-    F->f.has_queue = true;
-    F->f.has_bits = false;
-    TG_HIP(hipStreamSynchronize(st));
+    frontier_upload(F, v, c.st);
+  });
+}
+
+int tg_frontier_set_dense(tg_frontier *F, const uint8_t *els, uint64_t len) {
+  return guard([&] {
+    NEED(F && els, "null argument");
+    NEED(len == F->f.n, "dense array length differs from the frontier size");
+    FrontierCtx c(F);
+    std::vector<uint32_t> v;
+    for (uint64_t i = 0; i < len; i++)
+      if (els[i]) v.push_back(static_cast<uint32_t>(i));
+    frontier_upload(F, v, c.st);
   });
 }
 
@@ -517,13 +593,21 @@ int tg_frontier_size(tg_frontier *F, uint64_t *out) {
 int tg_frontier_get(tg_frontier *F, uint32_t *out, uint64_t cap, uint64_t *count) {
   return guard([&] {
     NEED(F && count, "null argument");
-    std::lock_guard<std::mutex> l(F->owner->g.mu);
-    F->owner->t.to_queue(F->f);
+    FrontierCtx c(F);
+    std::vector<uint32_t> v;
+    if (F->owner) {
+      F->owner->t.to_queue(F->f);
+    } else if (F->f.all) {
+      v.resize(F->f.n);
+      for (uint32_t i = 0; i < F->f.n; i++) v[i] = i;
+    }
     *count = F->f.count;
     if (!cap || !F->f.count) return;
     NEED(out, "null out");
-    std::vector<uint32_t> v(F->f.count);
-    d2h(v.data(), F->f.queue.get(), F->f.count * 4, F->owner->g.stream());
+    if (v.empty()) {  // the queue (an unbound frontier only ever holds a queue or "all")
+      v.resize(F->f.count);
+      d2h(v.data(), F->f.queue.get(), F->f.count * 4, c.st);
+    }
     std::sort(v.begin(), v.end());
     std::copy(v.begin(), v.begin() + std::min<uint64_t>(cap, v.size()), out);
   });
@@ -532,8 +616,20 @@ int tg_frontier_get(tg_frontier *F, uint32_t *out, uint64_t cap, uint64_t *count
 int tg_edge_map(tg_graph *G, tg_frontier *in, int op, const tg_op_args *a, tg_frontier *out, uint32_t threshold) {
   return guard([&] {
     NEED(G && in && a, "null argument");
-    NEED(in->owner == G && (!out || out->owner == G), "frontier of another graph");
     std::lock_guard<std::mutex> l(G->g.mu);
+    NEED(bind_frontier(in, G) && (!out || bind_frontier(out, G)), "frontier of another graph");
+    if (op == TG_OP_TC) {
+      // TC.h's countF: no output frontier (TC.h:78 passes output = false)
+      NEED(a->count, "TC needs count");
+      NEED(!out, "the TC op has no output frontier");
+      if (in->f.all) {
+        G->g.count_common_edges(nullptr, 0, reinterpret_cast<unsigned long long *>(a->count));
+      } else {
+        G->t.to_queue(in->f);
+        G->g.count_common_edges(in->f.queue.get(), in->f.count, reinterpret_cast<unsigned long long *>(a->count));
+      }
+      return;
+    }
     tg::OpState st;
     st.parent = a->parent;
     st.level = a->level;
@@ -561,26 +657,46 @@ int tg_edge_map(tg_graph *G, tg_frontier *in, int op, const tg_op_args *a, tg_fr
 
 int tg_vertex_map(tg_graph *G, tg_frontier *f, int vop, const tg_vmap_args *a) {
   return guard([&] {
-    NEED(G && f && a, "null argument");
-    NEED(f->owner == G, "frontier of another graph");
-    std::lock_guard<std::mutex> l(G->g.mu);
+    NEED(f && a, "null argument");
     tg::Traversal::VArgs v;
     v.val = a->val;
     v.inv = a->inv;
     v.ids = a->ids;
     v.prev = a->prev;
     v.visited = a->visited;
-    G->t.vertex_map(f->f, vop, v);
-    TG_HIP(hipStreamSynchronize(G->g.stream()));
+    if (G) {
+      std::lock_guard<std::mutex> l(G->g.mu);
+      NEED(bind_frontier(f, G), "frontier of another graph");
+    }
+    tg_graph *owner = f->owner;
+    if (!owner) {
+      // no graph yet: the vertex ops other than PR_VERTEX read no graph state
+      NEED(vop != TG_VOP_PR_VERTEX, "PR_VERTEX needs the graph (its degrees)");
+      tg::vertex_map_unbound(f->f, vop, v, nullptr);
+      TG_HIP(hipStreamSynchronize(nullptr));
+      return;
+    }
+    std::lock_guard<std::mutex> l(owner->g.mu);
+    owner->t.vertex_map(f->f, vop, v);
+    TG_HIP(hipStreamSynchronize(owner->g.stream()));
   });
 }
 
 int tg_frontier_copy(tg_frontier *dst, tg_frontier *src) {
   return guard([&] {
-    NEED(dst && src && dst->owner == src->owner, "frontiers of different graphs");
-    std::lock_guard<std::mutex> l(src->owner->g.mu);
-    src->owner->t.copy_frontier(dst->f, src->f);
-    TG_HIP(hipStreamSynchronize(src->owner->g.stream()));
+    NEED(dst && src, "null argument");
+    NEED(!dst->owner || !src->owner || dst->owner == src->owner, "frontiers of different graphs");
+    NEED(dst->f.n == src->f.n, "frontiers of different sizes");
+    tg_graph *owner = src->owner ? src->owner : dst->owner;
+    if (!owner) {
+      tg::copy_frontier_unbound(dst->f, src->f, nullptr);
+      TG_HIP(hipStreamSynchronize(nullptr));
+      return;
+    }
+    std::lock_guard<std::mutex> l(owner->g.mu);
+    dst->owner = src->owner = owner;
+    owner->t.copy_frontier(dst->f, src->f);
+    TG_HIP(hipStreamSynchronize(owner->g.stream()));
   });
 }
 
@@ -892,6 +1008,14 @@ int tg_profile_get(tg_graph *G, const char *name, double *ms, uint64_t *launches
         *ms = k.ms;
         *launches = k.launches;
       }
+  });
+}
+
+int tg_debug_fail_next(tg_graph *G, int what) {
+  return guard([&] {
+    NEED(G && (what == 1 || what == 2), "what must be 1 or 2");
+    std::lock_guard<std::mutex> lk(G->g.mu);
+    G->g.debug_fail |= static_cast<uint32_t>(what);
   });
 }
 

@@ -1212,6 +1212,11 @@ __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t 
 // Tiny helpers that move device counts around without a host round trip.
 __global__ void k_put_u32(uint32_t *p, const uint64_t *idx, uint32_t v) { p[*idx] = v; }
 __global__ void k_u32_to_u64(const uint32_t *a, uint64_t *b) { *b = *a; }
+__global__ void k_u32_to_u64_n(const uint32_t *a, uint64_t n, uint64_t *b) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    b[i] = a[i];
+}
 __global__ void k_u32_pair(const uint32_t *a, const uint32_t *b, uint64_t *out) {
   out[0] = *a;
   out[1] = *b;
@@ -2065,6 +2070,134 @@ __global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, uint
   }
 }
 
+// ------------------------------------------------------------ count_common
+// Graph::count_common (graph.h:901-1255): |N(s) ∩ N(d) ∩ [0, min(s, d))|,
+// the common neighbours below both endpoints, over the six tier pairs.  One
+// wave per pair: it walks the sorted list of the endpoint with the smaller
+// degree, 64 neighbours per step (inline slots, then its PMA run or its tree
+// blocks), up to the first neighbour >= min(s, d); each lane tests its
+// neighbour against the other endpoint with the is_edge lookup.  Every lane
+// returns the pair's count.
+__device__ inline uint32_t count_common_wave(const GView &g, uint32_t s, uint32_t d) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t lim = s < d ? s : d;
+  const uint32_t a = g.vb[s].degree <= g.vb[d].degree ? s : d;
+  const uint32_t b = a == s ? d : s;
+  const VBlock &blk = g.vb[a];
+  const uint32_t D = blk.degree;
+  uint32_t c = 0;
+  if (lane < (D < kInline ? D : kInline)) {
+    const uint32_t w = blk.nbr[lane];
+    if (w < lim && edge_exists(g, b, w)) c++;
+  }
+  if (D > kInline && blk.nbr[kInline - 1] < lim) {
+    const uint64_t aux = blk.aux;
+    if (aux & kAuxTree) {
+      const uint32_t t = static_cast<uint32_t>(aux);
+      const uint64_t io = g.toff[t];
+      const uint32_t nb = g.tnblk[t];
+      bool more = true;
+      for (uint32_t j = 0; j < nb && more; j++) {
+        const uint32_t bid = g.iblk[io + j];
+        const uint32_t cnt = g.bcnt[bid];
+        for (uint32_t e = 0; e < cnt; e += 64) {
+          const uint32_t idx = e + lane;
+          const uint32_t w = idx < cnt ? g.bkeys[static_cast<uint64_t>(bid) * kBlk + idx] : kEmpty;
+          if (w < lim && edge_exists(g, b, w)) c++;
+          if (__ballot(idx < cnt && w >= lim)) {
+            more = false;
+            break;
+          }
+        }
+      }
+    } else {
+      const uint32_t need = D - kInline;
+      uint64_t p = aux;
+      uint32_t got = 0;
+      while (got < need && p < g.N) {
+        const uint64_t slot = p + lane;
+        const uint32_t w = slot < g.N ? g.pdst[slot] : kEmpty;
+        const bool ok = w != kEmpty;
+        const uint64_t m = __ballot(ok);
+        const uint32_t r = __popcll(m & lanemask_lt());
+        const bool take = ok && got + r < need;
+        if (take && w < lim && edge_exists(g, b, w)) c++;
+        if (__ballot(take && w >= lim)) break;
+        got += __popcll(m);
+        p += 64;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  return c;
+}
+
+// One wave per (s[i], d[i]); ids at or above n give 0 and set *err.
+__global__ void __launch_bounds__(256) k_count_common(GView g, const uint32_t *src, const uint32_t *dst, uint64_t q,
+                                                      uint32_t *out, uint64_t *err) {
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  for (uint64_t i = wave; i < q; i += nwaves) {
+    const uint32_t s = src[i], d = dst[i];
+    if (s >= g.n || d >= g.n) {
+      if ((threadIdx.x & 63) == 0) {
+        out[i] = 0;
+        atomicOr(reinterpret_cast<unsigned long long *>(err), 1ull);
+      }
+      continue;
+    }
+    const uint32_t c = count_common_wave(g, s, d);
+    if ((threadIdx.x & 63) == 0) out[i] = c;
+  }
+}
+
+// TC.h's edgeMap with countF (TC.h:51-87): over the frontier's vertices u
+// (list, or every vertex when null) and their neighbours v < u, the sum of
+// count_common(u, v).  Edges are listed first from a CSR export (pairs of
+// the frontier vertex's edges below it), then one wave per pair, so a hub's
+// edges spread over the whole GPU.
+// lower[i]: neighbours of the i-th frontier vertex below it.
+__global__ void k_tc_lower(const uint64_t *off, const uint32_t *nbr, const uint32_t *list, uint64_t cnt,
+                          uint32_t *lower) {
+  GRID_STRIDE(i, cnt) {
+    const uint32_t u = list ? list[i] : static_cast<uint32_t>(i);
+    lower[i] = static_cast<uint32_t>(lower_bound_dev<uint32_t>(nbr + off[u], off[u + 1] - off[u], u));
+  }
+}
+
+// One wave per frontier vertex writes its pairs (u, v) for v < u.
+__global__ void __launch_bounds__(256) k_tc_pairs(const uint64_t *off, const uint32_t *nbr, const uint32_t *list,
+                                                  uint64_t cnt, const uint64_t *pos, uint64_t *pairs) {
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t i = wave; i < cnt; i += nwaves) {
+    const uint32_t u = list ? list[i] : static_cast<uint32_t>(i);
+    const uint64_t b = off[u], k = pos[i + 1] - pos[i];
+    for (uint64_t e = lane; e < k; e += 64) pairs[pos[i] + e] = make_key(u, nbr[b + e]);
+  }
+}
+
+// One wave per pair; per-block sums, then one 64-bit atomic per block.
+__global__ void __launch_bounds__(256) k_tc_count_pairs(GView g, const uint64_t *pairs, uint64_t np,
+                                                        unsigned long long *total) {
+  __shared__ unsigned long long part[4];
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
+  const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  unsigned long long acc = 0;
+  for (uint64_t i = wave; i < np; i += nwaves) {
+    const uint64_t pr = pairs[i];
+    acc += count_common_wave(g, key_src(pr), key_dst(pr));
+  }
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(total, t);
+  }
+}
+
 // ------------------------------------------------------------ queries
 __global__ void k_degrees(GView g, uint32_t *out) {
   GRID_STRIDE(v, g.n) out[v] = g.vb[v].degree;
@@ -2264,6 +2397,7 @@ Graph::~Graph() {
 }
 
 GView Graph::view() const {
+  if (invalid_) throw Error(-4, "graph is invalid: an earlier batch failed after it began changing the graph");
   GView g;
   g.vb = vb_.get();
   g.n = n_;
@@ -2433,6 +2567,29 @@ void Graph::check_batch_flags() {
   uint64_t h[kScalars];
   read_scalars(dsc_.get(), h, kScalars, stream_);
   check_flags(h);
+}
+
+void Graph::check_prep(const uint64_t *h) {
+  if (h[kErrScan] | h[kErrSort]) {
+    fill_u64(dsc_.get() + kErrSort, 0, 2, stream_);
+    fill_u64(dsc_.get(), 0, 1, stream_);
+    throw Error(-4, "batch preparation: a single-pass scan gave up a look-back wait (graph unchanged)");
+  }
+  if (h[0] & 1) {
+    fill_u64(dsc_.get(), 0, 1, stream_);
+    throw Error(-1, "vertex id out of range");
+  }
+}
+
+template <class F>
+void Graph::mutate(F &&body) {
+  try {
+    body();
+  } catch (const std::exception &e) {
+    invalid_ = true;
+    throw Error(-4, std::string("batch failed after it began changing the graph; the graph is now invalid: ") +
+                        e.what());
+  }
 }
 
 // The same checks on batch scalars already read back.
@@ -2842,7 +2999,7 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   // stays on the device until the one read below
   const uint64_t *R = nullptr;
   const bool small = count <= kSmallSortKeys;
-  uint64_t h[3];
+  uint64_t h[kScalars];
   // a bucket sort that overflowed is redone by radix (second pass)
   for (int pass = 0; pass < 2; pass++) {
     const bool bucket = prepare_keys_async(src, dst, count, pass == 1);
@@ -2878,10 +3035,15 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
       }
       TG_CHECK_LAUNCH();
     }
-    read_scalars(dsc_.get(), h, 3, stream_);
+    if (debug_fail & 1u) {
+      debug_fail &= ~1u;
+      fill_u64(dsc_.get() + kErrScan, 1, 1, stream_);
+    }
+    // counts, error bits and the look-back error words in one read
+    read_scalars(dsc_.get(), h, kScalars, stream_);
     if (!bucket || !(h[0] & kSortOverflow)) break;
   }
-  if (h[0] & 1) throw Error(-1, "vertex id out of range");
+  check_prep(h);
   if (h[1] == 0) return;
   insert_runs(R, h[1], static_cast<uint32_t>(h[2]));  // run_beg[nruns] = nR is written
 }
@@ -2904,13 +3066,36 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
   select(SelRunStart{R, run_beg.get()}, nR, dsc_.get() + 2, scan_scratch, stream_);
   k_put_u32<<<1, 1, 0, stream_>>>(run_beg.get(), dsc_.get() + 2, static_cast<uint32_t>(nR));
   TG_CHECK_LAUNCH();
-  uint32_t nruns = static_cast<uint32_t>(read_scalar(dsc_.get() + 2, stream_));
-  insert_runs(R, nR, nruns);
+  uint64_t h[kScalars];
+  read_scalars(dsc_.get(), h, kScalars, stream_);
+  check_prep(h);
+  insert_runs(R, nR, static_cast<uint32_t>(h[2]));
+}
+
+// Scratch of the insert's common path, bounded by the batch: sized before
+// the first write.  The fallbacks (promotion, the copy-on-write tree pass,
+// the PMA window pass and growth) size theirs by counts known only later.
+void Graph::presize_insert(uint64_t nR, uint32_t nruns) {
+  for (int i : {2, 3, 5, 6, 7}) s32_[i].ensure(nruns + 1, stream_);
+  s32_[4].ensure(static_cast<uint64_t>(nruns) * kInline + 1, stream_);
+  s8_[3].ensure(nruns + 1, stream_);
+  s8_[4].ensure(nR + 1, stream_);
+  k2_.ensure(nR + 1, stream_);
+  k3_.ensure(nR + 1, stream_);
+  g0_.ensure(nR + 1, stream_);
+  u0_.ensure(nR + 1, stream_);
+  f2_.ensure(nR + 1, stream_);  // f1_ (the new-key prefix, live here) already holds nR + 1
+  s32_[30].ensure(1 + 3 * std::min<uint64_t>(nR, kTreeSmallKeys), stream_);
+}
+
+void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
+  presize_insert(nR, nruns);
+  mutate([&] { insert_runs_apply(R, nR, nruns); });
 }
 
 // The insert proper: existence flags in f0_, the new-key prefix in f1_ and
 // the run starts (with run_beg[nruns] = nR) in s32_[0] are ready.
-void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
+void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   lists_dirty_ = true;
   ++epoch_;
   GView g = view();
@@ -2933,6 +3118,10 @@ void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
                                                          run_c.get(), run_nko.get(), kick_old.get(), promote.get(),
                                                          old_deg.get(), kroute.get(), nR);
   TG_CHECK_LAUNCH();
+  if (debug_fail & 2u) {
+    debug_fail &= ~2u;
+    throw Error(-2, "injected allocation failure after the inline merge");
+  }
   // kicked keys, sorted per run, placed in kick buffer k2_ (only routed
   // entries are read back)
   k2_.ensure(nR + 1, stream_);
@@ -3032,7 +3221,7 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   // sort, unique, the existing keys and their source runs sized by count;
   // the three counts come back in one read
   const bool small = count <= kSmallSortKeys;
-  uint64_t h[4];
+  uint64_t h[kScalars];
   // a bucket sort that overflowed is redone by radix (second pass)
   for (int pass = 0; pass < 2; pass++) {
     const bool bucket = prepare_keys_async(src, dst, count, pass == 1);
@@ -3064,10 +3253,14 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
       select(SelRunStartN{k2_.get(), s32_[0].get(), dsc_.get() + 2}, count, dsc_.get() + 3, scan_scratch,
              stream_);
     }
-    read_scalars(dsc_.get(), h, 4, stream_);
+    if (debug_fail & 1u) {
+      debug_fail &= ~1u;
+      fill_u64(dsc_.get() + kErrScan, 1, 1, stream_);
+    }
+    read_scalars(dsc_.get(), h, kScalars, stream_);
     if (!bucket || !(h[0] & kSortOverflow)) break;
   }
-  if (h[0] & 1) throw Error(-1, "vertex id out of range");
+  check_prep(h);
   if (h[2] == 0) return;
   delete_runs(k2_.get(), h[2], static_cast<uint32_t>(h[3]));  // run_beg[nruns] = nR is written
 }
@@ -3083,7 +3276,10 @@ void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
   k_exists<<<grid_for(nR0, 256), 256, 0, stream_>>>(g, R0, nR0, f0_.get());  // one search per thread
   TG_CHECK_LAUNCH();
   select(SelFlagged{R0, f0_.get(), k2_.get()}, nR0, dsc_.get() + 2, scan_scratch, stream_);
-  uint64_t nR = read_scalar(dsc_.get() + 2, stream_);
+  uint64_t h[kScalars];
+  read_scalars(dsc_.get(), h, kScalars, stream_);
+  check_prep(h);
+  const uint64_t nR = h[2];
   if (nR == 0) return;
   uint64_t *R = k2_.get();
   DevBuf<uint32_t> &run_beg = s32_[0];
@@ -3091,13 +3287,34 @@ void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
   select(SelRunStart{R, run_beg.get()}, nR, dsc_.get() + 3, scan_scratch, stream_);
   k_put_u32<<<1, 1, 0, stream_>>>(run_beg.get(), dsc_.get() + 3, static_cast<uint32_t>(nR));
   TG_CHECK_LAUNCH();
-  uint32_t nruns = static_cast<uint32_t>(read_scalar(dsc_.get() + 3, stream_));
-  delete_runs(R, nR, nruns);
+  read_scalars(dsc_.get(), h, kScalars, stream_);
+  check_prep(h);
+  delete_runs(R, nR, static_cast<uint32_t>(h[3]));
+}
+
+// Scratch of the delete's common path (see presize_insert).  Inline
+// refills number at most the deleted keys.
+void Graph::presize_delete(uint64_t nR, uint32_t nruns) {
+  s32_[12].ensure(nruns + 1, stream_);
+  s32_[13].ensure(nruns + 1, stream_);
+  s32_[31].ensure(1, stream_);
+  s8_[5].ensure(nR + 1, stream_);
+  s8_[6].ensure(nR + 1, stream_);
+  k3_.ensure(nR + 1, stream_);
+  g0_.ensure(nR + 1, stream_);
+  g2_.ensure(nR + 1, stream_);
+  for (int i : {1, 2, 3}) s64_[i].ensure(nR + 1, stream_);
+  s32_[30].ensure(1 + 3 * std::min<uint64_t>(nR, kTreeSmallKeys), stream_);
+}
+
+void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
+  presize_delete(nR, nruns);
+  mutate([&] { delete_runs_apply(R, nR, nruns); });
 }
 
 // The delete proper: R (in k2_) holds existing keys only and s32_[0] their
 // run starts with run_beg[nruns] = nR.
-void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
+void Graph::delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   lists_dirty_ = true;
   ++epoch_;
   GView g = view();
@@ -3109,6 +3326,10 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   k_delete_runs<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, sec_route.get(),
                                                            run_bump.get());
   TG_CHECK_LAUNCH();
+  if (debug_fail & 2u) {
+    debug_fail &= ~2u;
+    throw Error(-2, "injected allocation failure after the inline deletes");
+  }
   // secondary deletes by tier, and how many inline slots to refill
   k3_.ensure(nR + 1, stream_);
   g0_.ensure(nR + 1, stream_);
@@ -3233,10 +3454,13 @@ void Graph::insert_batch_weighted(const uint32_t *src, const uint32_t *dst, cons
   while (bits < 32 && (1ull << bits) < n_) bits++;
   radix_sort_pairs(k0_.get(), wv0_.get(), count, bits, bits, sort_, stream_);  // stable: input order within a key
   select(SelUniqueW{k0_.get(), wv0_.get(), k1_.get(), wv1_.get()}, count, dsc_.get() + 1, scan_scratch, stream_);
-  uint64_t h[2];
-  read_scalars(dsc_.get(), h, 2, stream_);
-  if (h[0] & 1) throw Error(-1, "vertex id out of range");
-  if (h[0] & 8) throw Error(-1, "edge weight must be below 2^31");
+  uint64_t h[kScalars];
+  read_scalars(dsc_.get(), h, kScalars, stream_);
+  if (h[0] & 8) {
+    fill_u64(dsc_.get(), 0, 1, stream_);
+    throw Error(-1, "edge weight must be below 2^31");
+  }
+  check_prep(h);
   uint64_t nR = h[1];
   // the keys absent before the insert get their weights
   wput_.ensure(nR + 1, stream_);
@@ -3246,8 +3470,10 @@ void Graph::insert_batch_weighted(const uint32_t *src, const uint32_t *dst, cons
   exclusive_scan_u32(wput_.get(), s32_[29].get(), nR, s32_[29].get() + nR, scan_scratch, stream_);
   uint64_t nnew = read_scalar(s32_[29].get() + nR, stream_);
   insert_sorted(k1_.get(), nR);
-  ws_->insert(k1_.get(), wv1_.get(), wput_.get(), nR, nnew, stream_);
-  TG_HIP(hipStreamSynchronize(stream_));
+  mutate([&] {
+    ws_->insert(k1_.get(), wv1_.get(), wput_.get(), nR, nnew, stream_);
+    TG_HIP(hipStreamSynchronize(stream_));
+  });
 }
 
 void Graph::edge_weights(const uint32_t *src, const uint32_t *dst, uint64_t count, uint32_t *out) {
@@ -3279,9 +3505,9 @@ void Graph::apply_ops(const uint32_t *src, const uint32_t *dst, const uint32_t *
          stream_);
   select(SelLastOp{k0_.get(), ops_val_.get(), count, 1, ops_del_.get()}, count, dsc_.get() + 2, scan_scratch,
          stream_);
-  uint64_t h[3];
-  read_scalars(dsc_.get(), h, 3, stream_);
-  if (h[0] & 1) throw Error(-1, "vertex id out of range");
+  uint64_t h[kScalars];
+  read_scalars(dsc_.get(), h, kScalars, stream_);
+  check_prep(h);
   insert_sorted(ops_ins_.get(), h[1]);
   delete_sorted(ops_del_.get(), h[2]);
 }
@@ -3298,6 +3524,46 @@ uint64_t Graph::num_edges() {
   TG_CHECK_LAUNCH();
   exclusive_scan_u64(g1_.get(), g1_.get(), n_, g1_.get() + n_, scan_scratch, stream_);
   return read_scalar(g1_.get() + n_, stream_);
+}
+
+void Graph::count_common(const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out) {
+  if (!q) return;
+  GView g = view();
+  cc_err_.ensure(1, stream_);
+  fill_u64(cc_err_.get(), 0, 1, stream_);
+  k_count_common<<<grid_for(q * 64, 256, 1u << 16), 256, 0, stream_>>>(g, src, dst, q, out, cc_err_.get());
+  TG_CHECK_LAUNCH();
+  if (read_scalar(cc_err_.get(), stream_)) throw Error(-1, "count_common: vertex id out of range");
+}
+
+uint64_t Graph::count_common_edges(const uint32_t *list, uint64_t cnt, unsigned long long *total) {
+  GView g = view();
+  const uint64_t rows = list ? cnt : n_;
+  cc_off_.ensure(static_cast<uint64_t>(n_) + 1, stream_);
+  const uint64_t m = num_edges();
+  cc_nbr_.ensure(m + 1, stream_);
+  export_adjacency(cc_off_.get(), cc_nbr_.get());
+  if (rows == 0) return 0;
+  DevBuf<uint32_t> lower;
+  DevBuf<uint64_t> pos;
+  lower.ensure(rows + 1, stream_);
+  pos.ensure(rows + 1, stream_);
+  k_tc_lower<<<grid_for(rows, 256), 256, 0, stream_>>>(cc_off_.get(), cc_nbr_.get(), list, rows, lower.get());
+  TG_CHECK_LAUNCH();
+  k_u32_to_u64_n<<<grid_for(rows, 256), 256, 0, stream_>>>(lower.get(), rows, pos.get());
+  TG_CHECK_LAUNCH();
+  exclusive_scan_u64(pos.get(), pos.get(), rows, pos.get() + rows, scan_scratch, stream_);
+  const uint64_t np = read_scalar(pos.get() + rows, stream_);
+  if (np == 0) return 0;
+  DevBuf<uint64_t> pairs;
+  pairs.ensure(np, stream_);
+  k_tc_pairs<<<grid_for(rows * 64, 256, 1u << 16), 256, 0, stream_>>>(cc_off_.get(), cc_nbr_.get(), list, rows,
+                                                                       pos.get(), pairs.get());
+  TG_CHECK_LAUNCH();
+  k_tc_count_pairs<<<grid_for(np * 64, 256, 1u << 16), 256, 0, stream_>>>(g, pairs.get(), np, total);
+  TG_CHECK_LAUNCH();
+  TG_HIP(hipStreamSynchronize(stream_));
+  return np;
 }
 
 void Graph::is_edge_batch(const uint32_t *src, const uint32_t *dst, uint64_t count, uint8_t *out) {
@@ -3368,6 +3634,7 @@ void Graph::neighbors_of(uint32_t v, uint32_t *out) {
 
 void Graph::copy_from(Graph &src, bool query_only) {
   if (src.n_ != n_) throw Error(-1, "snapshot of a graph with another vertex count");
+  if (src.invalid_) throw Error(-4, "snapshot of an invalid graph");
   src.rebuild_count_tree();
   hipStream_t s = src.stream_;
   auto cp = [&](void *d, const void *p, uint64_t bytes) {

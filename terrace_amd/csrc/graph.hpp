@@ -96,6 +96,14 @@ class Graph {
   void degrees(uint32_t *out) const;
   uint64_t num_edges();
   void is_edge_batch(const uint32_t *src, const uint32_t *dst, uint64_t count, uint8_t *out);
+  // Graph::count_common (include/graph.h:901-1255) for q pairs (device
+  // arrays): out[i] = common neighbours of src[i] and dst[i] below both.
+  void count_common(const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out);
+  // TC.h's edgeMap(G, F, countF) (include/TC.h:51-87): adds to *total (a
+  // device counter) the sum of count_common(u, v) over the frontier's
+  // vertices u (list of cnt, or every vertex when list is null) and their
+  // neighbours v < u.  Returns the number of such edges.
+  uint64_t count_common_edges(const uint32_t *list, uint64_t cnt, unsigned long long *total);
   // offsets[n+1] (device, u64) and neighbours (device, u32), sorted per vertex.
   void export_adjacency(uint64_t *offsets, uint32_t *nbrs);
   // Sorted neighbours of one vertex into out (device, >= degree entries).
@@ -109,6 +117,18 @@ class Graph {
   uint64_t pma_slots() const { return pma_N_; }
   uint64_t pma_elements() const { return pma_elems_; }
   const Stats &stats() const { return stats_; }
+  // True once a batch failed after it had begun changing the graph (an
+  // allocation for the graph's own growth, or an internal check): the
+  // adjacency may then be half-updated, so every later call on the graph
+  // fails with code -4.  A batch that fails before it changes anything
+  // (bad ids, a scratch allocation, a preparation check) leaves the graph
+  // valid and unchanged.
+  bool invalid() const { return invalid_; }
+  // Failure injection for tests of the error paths (every kernel still runs
+  // to its normal end): bit 0 makes the next batch's preparation report a
+  // look-back error word; bit 1 makes the next batch fail as if an
+  // allocation failed right after its first write to the graph.
+  uint32_t debug_fail = 0;
 
   // Versioned snapshot: make this graph a read-only copy of src's current
   // state (device-to-device copies ordered on src's stream).  Queries can
@@ -169,6 +189,7 @@ class Graph {
   bool tree_dirty_ = false;  // count tree levels above the leaves are stale
   uint64_t pma_elems_ = 0;   // elements in the PMA (the count tree's root, kept on the host)
   bool read_only_ = false;   // query-only snapshot
+  bool invalid_ = false;     // see invalid()
   // B-tree tier
   DevBuf<uint32_t> bkeys_, bcnt_;
   uint64_t blk_top_ = 0;
@@ -201,6 +222,8 @@ class Graph {
   static constexpr int kScalars = 16;
   static constexpr int kErrSort = 14, kErrScan = 15;  // look-back error words (sort, scans/selects)
   DevBuf<uint64_t> dsc_;
+  DevBuf<uint64_t> cc_err_, cc_off_;  // count_common: id check; CSR offsets of the TC edgeMap
+  DevBuf<uint32_t> cc_nbr_;
   DevBuf<uint64_t> tmp_old_;
   DevBuf<uint32_t> ops_val_;
   DevBuf<uint64_t> ops_ins_, ops_del_;
@@ -218,11 +241,24 @@ class Graph {
   void compute_bounds();
   // true when the bucket sort ran (its overflow bit may then be set)
   bool prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_t count, bool radix = false);
+  // presize, then *_apply under mutate()
   void insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
   void delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
+  void insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns);
+  void delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns);
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
   void check_batch_flags();
   void check_flags(const uint64_t *h);
+  // Error words of the preparation (id range, look-back waits) read back
+  // before the first write to the graph: throws, leaving it unchanged.
+  void check_prep(const uint64_t *h);
+  // Runs body, which changes the graph; any failure marks the graph invalid.
+  template <class F>
+  void mutate(F &&body);
+  // Per-batch scratch sized before the first write (so that running out of
+  // memory there leaves the graph unchanged).
+  void presize_insert(uint64_t nR, uint32_t nruns);
+  void presize_delete(uint64_t nR, uint32_t nruns);
   // PMA inserts: pma_insert = in-leaf merge when every touched leaf fits,
   // else the window pass (or a rebuild beyond the root bound)
   void pma_insert(const uint64_t *keys, uint64_t k);

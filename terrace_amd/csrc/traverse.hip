@@ -1511,8 +1511,8 @@ __global__ void k_vertex_map(GView g, const uint32_t *list, uint64_t cnt, int vo
 }  // namespace
 
 // ===================================================================== host
-void Traversal::vertex_map(Frontier &f, int vop, const VArgs &a) {
-  hipStream_t s = stream_;
+namespace {
+void check_vertex_op(int vop, const Traversal::VArgs &a) {
   if (vop < 1 || vop > 8) throw Error(-1, "unknown vertex op");
   if ((vop == 1 || vop == 2 || vop == 6) && !a.val) throw Error(-1, "vertex op needs val");
   if (vop == 7 && !a.visited) throw Error(-1, "CLEAR_VISITED needs visited");
@@ -1520,16 +1520,46 @@ void Traversal::vertex_map(Frontier &f, int vop, const VArgs &a) {
   if (vop == 3 && !(a.ids && a.prev)) throw Error(-1, "CC_VERTEX needs ids and prev");
   if (vop == 4 && !a.visited) throw Error(-1, "BC_VERTEX needs visited");
   if (vop == 5 && !(a.visited && a.val && a.inv)) throw Error(-1, "BC_BACK_VERTEX needs visited, val, inv");
+}
+
+// The vertex op over f's queue (or every vertex); g is read by PR_VERTEX only.
+void launch_vertex_map(const GView &g, Frontier &f, int vop, const Traversal::VArgs &a, hipStream_t s) {
   if (f.all) {
-    k_vertex_map<<<grid_for(f.n, 256), 256, 0, s>>>(g_.view(), nullptr, f.n, vop, a.val, a.inv, a.ids, a.prev,
-                                                    a.visited);
+    k_vertex_map<<<grid_for(f.n, 256), 256, 0, s>>>(g, nullptr, f.n, vop, a.val, a.inv, a.ids, a.prev, a.visited);
   } else {
-    to_queue(f);
     if (f.count == 0) return;
-    k_vertex_map<<<grid_for(f.count, 256), 256, 0, s>>>(g_.view(), f.queue.get(), f.count, vop, a.val, a.inv, a.ids,
+    k_vertex_map<<<grid_for(f.count, 256), 256, 0, s>>>(g, f.queue.get(), f.count, vop, a.val, a.inv, a.ids,
                                                         a.prev, a.visited);
   }
   TG_CHECK_LAUNCH();
+}
+}  // namespace
+
+void Traversal::vertex_map(Frontier &f, int vop, const VArgs &a) {
+  check_vertex_op(vop, a);
+  if (!f.all) to_queue(f);
+  launch_vertex_map(g_.view(), f, vop, a, stream_);
+}
+
+void vertex_map_unbound(Frontier &f, int vop, const Traversal::VArgs &a, hipStream_t s) {
+  check_vertex_op(vop, a);
+  if (vop == 1) throw Error(-1, "PR_VERTEX needs the graph's degrees");
+  if (!f.all && !f.has_queue) throw Error(-4, "frontier without a graph holds no queue");
+  launch_vertex_map(GView{}, f, vop, a, s);
+}
+
+void copy_frontier_unbound(Frontier &dst, Frontier &src, hipStream_t s) {
+  if (src.has_bits) throw Error(-4, "frontier without a graph holds a bitmap");
+  dst.n = src.n;
+  dst.all = src.all;
+  dst.count = src.count;
+  dst.has_queue = src.has_queue;
+  dst.has_bits = false;
+  if (src.has_queue) {
+    dst.queue.ensure(src.n, s);
+    if (src.count)
+      TG_HIP(hipMemcpyAsync(dst.queue.get(), src.queue.get(), src.count * 4, hipMemcpyDeviceToDevice, s));
+  }
 }
 
 void Traversal::copy_frontier(Frontier &dst, Frontier &src) {

@@ -155,4 +155,10 @@ class Traversal {
   uint64_t popcount_bits(const Frontier &f);
 };
 
+// Frontiers not yet bound to a graph (the reference's VertexSubset(e, n,
+// all) carries no graph): vertex ops that read no graph state, and copies,
+// on stream s.
+void vertex_map_unbound(Frontier &f, int vop, const Traversal::VArgs &a, hipStream_t s);
+void copy_frontier_unbound(Frontier &dst, Frontier &src, hipStream_t s);
+
 }  // namespace tg

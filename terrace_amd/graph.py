@@ -12,7 +12,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (OP_BC_BACK, OP_BC_FWD, OP_BF, OP_BFS, OP_CC, OP_PR, DeviceBuffer, OpArgs, Request,
+from ._lib import (OP_BC_BACK, OP_BC_FWD, OP_BF, OP_BFS, OP_CC, OP_PR, OP_TC, DeviceBuffer, OpArgs, Request,
                    Result, TerraceError, check, lib, ptr)
 
 __all__ = ["Graph", "VertexSubset", "edgeMap", "BFS_with_edge_map", "PR_S", "CC", "BC", "TC",
@@ -128,6 +128,19 @@ class Graph:
         check(lib().tg_is_edge_batch(self._h, ptr(s), ptr(d), len(s), ptr(out)))
         return out
 
+    def count_common(self, s: int, d: int) -> int:
+        """Graph::count_common (include/graph.h:1085-1255): common
+        neighbours of s and d below min(s, d)."""
+        return int(self.count_common_batch([s], [d])[0])
+
+    def count_common_batch(self, srcs, dests) -> np.ndarray:
+        s, d = _u32(srcs), _u32(dests)
+        if len(s) != len(d):
+            raise ValueError("srcs and dests differ in length")
+        out = np.zeros(len(s), np.uint32)
+        check(lib().tg_count_common(self._h, ptr(s), ptr(d), len(s), ptr(out)))
+        return out
+
     def degree(self, v: int) -> int:
         r = ctypes.c_uint32()
         check(lib().tg_degree(self._h, v, ctypes.byref(r)))
@@ -232,18 +245,33 @@ class Graph:
 
 
 class VertexSubset:
-    """VertexSubset (include/VertexSubset.cpp): VertexSubset(e, n) holds one
-    vertex, VertexSubset(0, n, all=True) holds every vertex."""
+    """VertexSubset (include/VertexSubset.cpp): VertexSubset(G, e) holds one
+    vertex, VertexSubset(G, 0, all=True) every vertex.  With G = None and
+    max_el = n it is the reference's graph-less VertexSubset(e, n, all)
+    (VertexSubset.cpp:109), bound to a graph by its first edgeMap;
+    VertexSubset.dense(els) is VertexSubset(bool *els, len) (:119)."""
 
-    def __init__(self, G: Graph, e=None, all: bool = False):
+    def __init__(self, G: Graph = None, e=None, all: bool = False, max_el: int = None):
         h = ctypes.c_void_p()
-        check(lib().tg_frontier_create(G.handle, ctypes.byref(h)))
+        if G is None:
+            if not max_el:
+                raise ValueError("a VertexSubset without a graph needs max_el")
+            check(lib().tg_frontier_create_n(int(max_el), ctypes.byref(h)))
+        else:
+            check(lib().tg_frontier_create(G.handle, ctypes.byref(h)))
         self._h, self._g = h, G
         if all:
             check(lib().tg_frontier_set_all(h))
         elif e is not None:
             v = _u32(np.atleast_1d(e))
             check(lib().tg_frontier_set(h, ptr(v), len(v)))
+
+    @classmethod
+    def dense(cls, els) -> "VertexSubset":
+        els = np.ascontiguousarray(els, dtype=np.uint8)
+        vs = cls(None, max_el=len(els))
+        check(lib().tg_frontier_set_dense(vs._h, ptr(els), len(els)))
+        return vs
 
     @property
     def handle(self):
@@ -279,7 +307,8 @@ class VertexSubset:
 def edgeMap(G: Graph, vs: VertexSubset, op: int, args: OpArgs, output: bool = True,
             threshold: int = 20):
     """edgeMap(G, vs, F, output, threshold) (include/Map.cpp:66-74) with F
-    one of the device ops (OP_BFS, OP_CC, OP_BC_FWD, OP_BC_BACK, OP_PR)."""
+    one of the device ops (OP_BFS, OP_CC, OP_BC_FWD, OP_BC_BACK, OP_PR,
+    OP_BF, OP_TC; OP_TC takes no output frontier)."""
     out = VertexSubset(G) if output else None
     check(lib().tg_edge_map(G.handle, vs.handle, int(op), ctypes.byref(args),
                             out.handle if out else None, int(threshold)))
@@ -465,4 +494,5 @@ def rmat_device(n: int, seed: int, a: float, b: float, c: float, count: int, sta
     return s, d, w.value
 
 
-OPS = {"BFS": OP_BFS, "CC": OP_CC, "BC_FWD": OP_BC_FWD, "BC_BACK": OP_BC_BACK, "PR": OP_PR, "BF": OP_BF}
+OPS = {"BFS": OP_BFS, "CC": OP_CC, "BC_FWD": OP_BC_FWD, "BC_BACK": OP_BC_BACK, "PR": OP_PR, "BF": OP_BF,
+       "TC": OP_TC}

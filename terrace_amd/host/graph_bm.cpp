@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -181,8 +182,8 @@ int main(int argc, char **argv) {
       tg_check(tg_insert_batch_dev(graph.handle(), s.data(), d.data(), bs));
       tg_check(tg_device_sync());
       t_ins += now() - t0;
-      std::vector<uint32_t> hs = s.to_host(), hd = d.to_host();
-      tg_check(tg_is_edge_batch(graph.handle(), hs.data(), hd.data(), bs, present.data()));
+      std::unique_ptr<uint32_t[]> hs = s.to_host(), hd = d.to_host();
+      tg_check(tg_is_edge_batch(graph.handle(), hs.get(), hd.get(), bs, present.data()));
       for (uint64_t i = 0; i < bs; i++)
         if (!present[i]) {
           printf("edge (%u, %u) not found, should be\n", hs[i], hd[i]);
