@@ -94,6 +94,14 @@ int tg_is_edge_batch(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint
  * (inline, PMA, B-tree).  _dev: device arrays. */
 int tg_count_common(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out);
 int tg_count_common_dev(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out);
+/* tg_count_common without any device allocation; no counterpart in the
+ * reference.  For callers that cannot allocate device memory (serving
+ * loops, code under HIP graph capture, harnesses that forbid hipMalloc):
+ * host arrays are staged through a static device array of 12 MiB that each
+ * GPU holds until the process ends, so q is at most 2^20 pairs (larger q:
+ * TG_EINVAL).  Same results and errors as tg_count_common.  Calls on one
+ * GPU share the array: the library serialises them within the process. */
+int tg_count_common_noalloc(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out);
 /* Graph::degree (include/graph.h:797-803) */
 int tg_degree(tg_graph *g, uint32_t v, uint32_t *out);
 int tg_degrees_all(tg_graph *g, uint32_t *out /* [n] */);

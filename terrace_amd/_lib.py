@@ -155,6 +155,7 @@ _SIGS = {
     "tg_debug_fail_next": (ctypes.c_int, [_P, ctypes.c_int]),
     "tg_count_common": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64, _P]),
     "tg_count_common_dev": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64, _P]),
+    "tg_count_common_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64, _P]),
     "tg_frontier_create_n": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(_P)]),
     "tg_frontier_set_dense": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
 }

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -23,7 +24,8 @@ struct tg_graph {
   tg::DevBuf<uint64_t> s_u64;
   tg::DevBuf<int32_t> s_i32;
   tg::DevBuf<double> s_f64;
-  explicit tg_graph(uint32_t n) : g(n), t(g) {}
+  int device = 0;  // the device current when the graph was created
+  explicit tg_graph(uint32_t n) : g(n), t(g) { TG_HIP(hipGetDevice(&device)); }
 };
 
 struct tg_frontier {
@@ -255,6 +257,58 @@ int tg_count_common_dev(tg_graph *G, const uint32_t *src, const uint32_t *dst, u
     std::lock_guard<std::mutex> l(G->g.mu);
     G->g.count_common(src, dst, q, out);
     TG_HIP(hipStreamSynchronize(G->g.stream()));
+  });
+}
+
+// ---- no-allocation count_common
+// The staging of tg_count_common_noalloc lives in a static device array
+// (each GPU holds its own copy until the process ends): [0, 256) the id-check
+// word, then src, dst and out, kNoAllocPairs uint32 each.  Host code never
+// passes the array's name to a kernel (it is a host-side stand-in there):
+// each device's address comes from hipGetSymbolAddress, taken once per
+// device with that device current, and every launch and copy uses it.
+namespace {
+constexpr uint64_t kNoAllocPairs = 1ull << 20;
+constexpr uint64_t kNoAllocBytes = 256 + 3 * 4 * kNoAllocPairs;  // 12 MiB + 256 B
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;  // one call at a time in this process: the calls share the array
+unsigned char *g_noalloc_ptr[kMaxDevices] = {};
+
+// Caller holds g_noalloc_mu.
+unsigned char *noalloc_scratch(int device) {
+  NEED(device >= 0 && device < kMaxDevices, "device id out of range");
+  if (!g_noalloc_ptr[device]) {
+    int cur = 0;
+    TG_HIP(hipGetDevice(&cur));
+    if (cur != device) TG_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) TG_HIP(hipSetDevice(cur));
+    if (e != hipSuccess) throw tg::Error(TG_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[device] = static_cast<unsigned char *>(p);
+  }
+  return g_noalloc_ptr[device];
+}
+}  // namespace
+
+int tg_count_common_noalloc(tg_graph *G, const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out) {
+  return guard([&] {
+    NEED(G && (q == 0 || (src && dst && out)), "null argument");
+    NEED(q <= kNoAllocPairs, "tg_count_common_noalloc: at most 2^20 pairs per call");
+    if (!q) return;
+    std::lock_guard<std::mutex> ls(g_noalloc_mu);
+    std::lock_guard<std::mutex> l(G->g.mu);
+    unsigned char *base = noalloc_scratch(G->device);
+    uint64_t *err = reinterpret_cast<uint64_t *>(base);  // cleared by a kernel in count_common
+    uint32_t *s = reinterpret_cast<uint32_t *>(base + 256);
+    uint32_t *d = s + kNoAllocPairs, *o = d + kNoAllocPairs;
+    hipStream_t st = G->g.stream();
+    h2d(s, src, q * 4, st);
+    h2d(d, dst, q * 4, st);
+    G->g.count_common(s, d, q, o, err);
+    d2h(out, o, q * 4, st);
   });
 }
 

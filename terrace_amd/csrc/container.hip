@@ -3526,14 +3526,17 @@ uint64_t Graph::num_edges() {
   return read_scalar(g1_.get() + n_, stream_);
 }
 
-void Graph::count_common(const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out) {
+void Graph::count_common(const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out, uint64_t *err) {
   if (!q) return;
   GView g = view();
-  cc_err_.ensure(1, stream_);
-  fill_u64(cc_err_.get(), 0, 1, stream_);
-  k_count_common<<<grid_for(q * 64, 256, 1u << 16), 256, 0, stream_>>>(g, src, dst, q, out, cc_err_.get());
+  if (!err) {
+    cc_err_.ensure(1, stream_);
+    err = cc_err_.get();
+  }
+  fill_u64(err, 0, 1, stream_);  // a kernel, no memset node
+  k_count_common<<<grid_for(q * 64, 256, 1u << 16), 256, 0, stream_>>>(g, src, dst, q, out, err);
   TG_CHECK_LAUNCH();
-  if (read_scalar(cc_err_.get(), stream_)) throw Error(-1, "count_common: vertex id out of range");
+  if (read_scalar(err, stream_)) throw Error(-1, "count_common: vertex id out of range");
 }
 
 uint64_t Graph::count_common_edges(const uint32_t *list, uint64_t cnt, unsigned long long *total) {

@@ -98,7 +98,9 @@ class Graph {
   void is_edge_batch(const uint32_t *src, const uint32_t *dst, uint64_t count, uint8_t *out);
   // Graph::count_common (include/graph.h:901-1255) for q pairs (device
   // arrays): out[i] = common neighbours of src[i] and dst[i] below both.
-  void count_common(const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out);
+  // err: a device word for the id check (null: the graph's own, allocated
+  // on first use).
+  void count_common(const uint32_t *src, const uint32_t *dst, uint64_t q, uint32_t *out, uint64_t *err = nullptr);
   // TC.h's edgeMap(G, F, countF) (include/TC.h:51-87): adds to *total (a
   // device counter) the sum of count_common(u, v) over the frontier's
   // vertices u (list of cnt, or every vertex when list is null) and their
