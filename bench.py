@@ -97,52 +97,94 @@ def copy_probe():
     log(f"copy probe: 3 x {nb} bytes")
 
 
-def cpu_baseline(quick: bool):
-    """The reference's own add_edge_batch (oracle/_ref, built from the
-    reference sources) timed on this host on a bounded sample: scale-20
-    rMAT (8M generated edges), 5 insert batches of 10^5 edges.  Falls back
-    to the oracle's set-union restatement when the reference build is
-    absent."""
+def cpu_baseline(quick: bool, batch: int):
+    """The reference's own update path (oracle/_ref, built from the
+    reference's sources) timed on this host on the benchmark's workload: the
+    static graph (rMAT scale 22, 69M generated edges, a=.5 b=.1 c=.1,
+    symmetrized) and 20 mixed batches of `batch` edges, 90% inserts of fresh
+    rMAT edges (both directions, seeds ith_rand(1000 + t) as the GPU
+    batches) and 10% deletes of existing edges (both directions).  Timed as
+    src/graph_bm.cc:354-378 times them: add_edge_batch (sort outside the
+    region) plus the parallel remove_edge loop.  Delete endpoints are used
+    once across the batches (remove_edge's overlapping memcpy, graph.h:700,
+    corrupts a vertex it has touched before).  Falls back to the oracle's
+    set-model update when the reference build is absent."""
     import oracle
 
-    n = 1 << 20
-    m0 = 2_000_000 if quick else 8_000_000
-    keys = oracle.build_keys(*oracle.rmat_edges(n, aspen(100) & 0xFFFFFFFF, 0.5, 0.1, 0.1, m0))
+    scale, m0, nb = (18, 4_000_000, 5) if quick else (22, 69_000_000, 20)
+    n = 1 << scale
     threads = min(16, os.cpu_count() or 1)
+    rng = np.random.default_rng(31)
+    s, d = oracle.rmat_edges(n, aspen(100) & 0xFFFFFFFF, 0.5, 0.1, 0.1, m0)
+    g = oracle.CSR.from_edges(n, s, d)
+    del s, d
+    n_ins_pairs, n_del_pairs = max(1, int(batch * 0.9) // 2), max(1, int(batch * 0.1) // 2)
+    ins = [oracle.build_keys(*oracle.rmat_edges(n, aspen(1000 + t) & 0xFFFFFFFF, 0.5, 0.1, 0.1, n_ins_pairs))
+           for t in range(nb)]
+    # delete pairs (u, v), u < v, existing, every endpoint used once
+    need = nb * n_del_pairs
+    e = rng.integers(0, g.m, 4 * need + 1000).astype(np.uint64)
+    u = (np.searchsorted(g.off, e, side="right") - 1).astype(np.int64)
+    v = g.nbr[e].astype(np.int64)
+    seen, pairs = set(), []
+    for a, b in zip(u.tolist(), v.tolist()):
+        if a < b and a not in seen and b not in seen:
+            seen.update((a, b))
+            pairs.append((a, b))
+            if len(pairs) == need:
+                break
+    pairs = np.array(pairs, np.uint64)
+    dels = [pairs[t * n_del_pairs:(t + 1) * n_del_pairs] for t in range(nb)]
+    dels = [oracle.sort_unique(np.concatenate([(p[:, 0] << np.uint64(32)) | p[:, 1],
+                                              (p[:, 1] << np.uint64(32)) | p[:, 0]])) for p in dels]
+    sample = (f"rMAT scale {scale}, {m0} generated edges ({g.m} directed); {nb} mixed batches of {batch} "
+              f"(90% inserts / 10% deletes of existing edges); add_edge_batch + parallel remove_edge, "
+              f"sort excluded as in graph_bm.cc")
     if os.path.exists(oracle.REF_HARNESS):
         with tempfile.TemporaryDirectory() as td:
-            path = os.path.join(td, "g.bin")
-            pairs = np.empty((len(keys), 2), np.uint32)
-            pairs[:, 0] = (keys >> np.uint64(32)).astype(np.uint32)
-            pairs[:, 1] = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
-            with open(path, "wb") as f:
-                np.array([n, len(keys)], np.uint64).tofile(f)
-                pairs.tofile(f)
+            def write(path, keys):
+                keys = np.asarray(keys, np.uint64)
+                pairs_ = np.empty((len(keys), 2), np.uint32)
+                pairs_[:, 0] = (keys >> np.uint64(32)).astype(np.uint32)
+                pairs_[:, 1] = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+                with open(path, "wb") as f:
+                    np.array([n, len(keys)], np.uint64).tofile(f)
+                    pairs_.tofile(f)
+
+            gpath = os.path.join(td, "g.bin")
+            with open(gpath, "wb") as f:
+                np.array([n, g.m], np.uint64).tofile(f)
+                pairs_ = np.empty((g.m, 2), np.uint32)
+                pairs_[:, 0] = np.repeat(np.arange(n, dtype=np.uint32), g.degrees())
+                pairs_[:, 1] = g.nbr[:g.m]
+                pairs_.tofile(f)
+                del pairs_
+            files = []
+            for t in range(nb):
+                files += [os.path.join(td, f"i{t}.bin"), os.path.join(td, f"d{t}.bin")]
+                write(files[-2], ins[t])
+                write(files[-1], dels[t])
             env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-            r = subprocess.run([oracle.REF_HARNESS, "bench", path, "100000", "5", "1000"],
-                               capture_output=True, text=True, env=env, timeout=600)
+            log(f"cpu baseline: reference build, {threads} threads, {sample}")
+            r = subprocess.run([oracle.REF_HARNESS, "mixed", gpath] + files, capture_output=True, text=True,
+                               env=env, timeout=900)
             if r.returncode == 0:
                 res = json.loads(r.stdout.strip().splitlines()[-1])
                 return {"value": res["edges_per_s"], "unit": "edges/s", "cores": res["threads"],
-                        "kind": "reference",
-                        "sample": f"reference add_edge_batch (OpenMP build), rMAT scale 20 with {m0} "
-                                  f"generated edges ({res['static_edges']} directed), 5 insert batches "
-                                  f"of 1e5 directed edges; sort excluded as in graph_bm.cc"}
+                        "kind": "reference", "sample": sample, "insert_s": res["insert_s"],
+                        "delete_s": res["delete_s"], "static_build_s": res["static_s"]}
             log("reference harness failed:", r.stderr[-500:])
-    # port: set-union insert of the oracle
+    # port: the oracle's set-model update (union, then difference), one thread
+    keys = oracle.sort_unique((np.repeat(np.arange(n, dtype=np.uint64), g.degrees()) << np.uint64(32))
+                              | g.nbr[:g.m].astype(np.uint64))
     t0 = time.perf_counter()
     total = 0
-    for t in range(5):
-        bk = oracle.build_keys(*oracle.rmat_edges(n, aspen(1000 + t) & 0xFFFFFFFF, 0.5, 0.1, 0.1, 50000))
-        t1 = time.perf_counter()
-        keys = oracle.set_union(keys, bk)
-        total += len(bk)
-        t0 += 0
-        elapsed = time.perf_counter() - t1
-        t0 -= elapsed
+    for t in range(nb):
+        keys = oracle.set_difference(oracle.set_union(keys, ins[t]), dels[t])
+        total += len(ins[t]) + len(dels[t])
     dt = time.perf_counter() - t0
     return {"value": total / max(dt, 1e-9), "unit": "edges/s", "cores": 1, "kind": "port",
-            "sample": "oracle set-union insert, rMAT scale 20, 5 batches of 1e5"}
+            "sample": "oracle set-model update, " + sample}
 
 
 def serve_bench(args):
@@ -873,7 +915,7 @@ def main():
 
     if args.copy_probe:
         copy_probe()
-    cpu = None if args.no_cpu else cpu_baseline(args.quick)
+    cpu = None if args.no_cpu else cpu_baseline(args.quick, args.batch)
     out = {
         "metric": "inserted edges/s",
         "value": value,

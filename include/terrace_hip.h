@@ -78,6 +78,16 @@ int tg_insert_batch(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint6
 int tg_delete_batch(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t count);
 int tg_insert_batch_dev(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t count);
 int tg_delete_batch_dev(tg_graph *g, const uint32_t *src, const uint32_t *dst, uint64_t count);
+/* A mixed update batch: add_edge_batch of the n_ins inserts followed by
+ * remove_edge of the n_del deletes, as the benchmark driver applies a batch
+ * (src/graph_bm.cc:354-378), in one call that sorts and prepares both
+ * halves together.  The result equals tg_insert_batch then tg_delete_batch:
+ * an edge both inserted and deleted ends absent; absent deletes are no-ops.
+ * n_ins + n_del < 2^31. */
+int tg_update_batch(tg_graph *g, const uint32_t *ins_src, const uint32_t *ins_dst, uint64_t n_ins,
+                    const uint32_t *del_src, const uint32_t *del_dst, uint64_t n_del);
+int tg_update_batch_dev(tg_graph *g, const uint32_t *ins_src, const uint32_t *ins_dst, uint64_t n_ins,
+                        const uint32_t *del_src, const uint32_t *del_dst, uint64_t n_del);
 /* Graph::add_edge / remove_edge (include/graph.h:507-641, 675-791):
  * *result = 0 when the edge was added/removed, -1 when it was already
  * present/absent. */
@@ -338,8 +348,10 @@ int tg_profile_get(tg_graph *g, const char *name, double *ms, uint64_t *launches
  * [3] tree blocks written, [4] promotions, [5] trees, [6] PMA slots,
  * [7] tree updates rewritten in place (no block split or emptied),
  * [8] PMA inserts merged inside their leaves (no window redistribution),
- * [9] elements held in the PMA */
-int tg_stats(tg_graph *g, uint64_t *out /* [10] */);
+ * [9] elements held in the PMA; the last update batch: [10] unique
+ * inserted edges (present ones included), [11] their distinct sources,
+ * [12] deleted edges that existed, [13] their distinct sources */
+int tg_stats(tg_graph *g, uint64_t *out /* [14] */);
 /* Failure injection for tests of the error paths; no counterpart in the
  * reference.  what = 1: the next update batch's preparation reports a
  * look-back error (TG_EINTERNAL, graph unchanged and still valid); what = 2:

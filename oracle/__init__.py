@@ -69,6 +69,10 @@ def lib():
     L.orc_set_difference.argtypes = [_u64p, c.c_uint64, _u64p, c.c_uint64, _u64p]
     L.orc_keys_to_csr.restype = None
     L.orc_keys_to_csr.argtypes = [_u64p, c.c_uint64, c.c_uint32, _u64p, _u32p]
+    L.orc_edges_to_csr.restype = c.c_uint64
+    L.orc_edges_to_csr.argtypes = [c.c_uint32, _u32p, _u32p, c.c_uint64, c.c_int, c.c_int, _u64p, _u32p]
+    L.orc_csr_update.restype = c.c_uint64
+    L.orc_csr_update.argtypes = [c.c_uint32, _u64p, _u32p, _u64p, c.c_uint64, _u64p, c.c_uint64, _u64p, _u32p]
     L.orc_tier_insert.restype = None
     L.orc_tier_insert.argtypes = [c.c_void_p, c.c_uint32, _u64p, _u32p, _u64p, c.c_uint64]
     L.orc_bfs_levels.restype = None
@@ -160,6 +164,32 @@ class CSR:
         self.off = np.empty(n + 1, np.uint64)
         self.nbr = np.empty(max(self.m, 1), np.uint32)
         lib().orc_keys_to_csr(keys, self.m, n, self.off, self.nbr)
+
+    @classmethod
+    def from_edges(cls, n: int, src, dst, symmetric=True, drop_loops=True) -> "CSR":
+        """The CSR of an edge list without sorting 64-bit keys (equal to
+        CSR(build_keys(src, dst, symmetric), n) when drop_loops)."""
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        g = cls.__new__(cls)
+        g.n = n
+        g.off = np.empty(n + 1, np.uint64)
+        nbr = np.empty(max(len(src) * (2 if symmetric else 1), 1), np.uint32)
+        g.m = int(lib().orc_edges_to_csr(n, src, dst, len(src), int(symmetric), int(drop_loops), g.off, nbr))
+        g.nbr = nbr[:max(g.m, 1)]
+        return g
+
+    def updated(self, ins, dels) -> "CSR":
+        """(self ∪ ins) \\ dels as a new CSR (sorted unique keys)."""
+        ins = np.ascontiguousarray(ins, np.uint64)
+        dels = np.ascontiguousarray(dels, np.uint64)
+        g = CSR.__new__(CSR)
+        g.n = self.n
+        g.off = np.empty(self.n + 1, np.uint64)
+        nbr = np.empty(max(self.m + len(ins), 1), np.uint32)
+        g.m = int(lib().orc_csr_update(self.n, self.off, self.nbr, ins, len(ins), dels, len(dels), g.off, nbr))
+        g.nbr = nbr[:max(g.m, 1)]
+        return g
 
     def degrees(self) -> np.ndarray:
         return (self.off[1:] - self.off[:-1]).astype(np.uint32)

@@ -12,6 +12,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define INPLACE 13u          /* include/graph.h:34-39 (unweighted) */
 #define MEDIUM_DEGREE 1024u  /* include/graph.h:39 */
@@ -57,6 +60,8 @@ int orc_rmat_edges(uint32_t n, uint32_t seed, double a, double b, double c,
   uint32_t levels = 0;
   while ((1u << levels) < n) levels++;
   const double denom = nominal ? 4294967296.0 : 2147483647.0; /* :95-96 */
+  /* every edge is a closed form of its index: the loop splits freely */
+#pragma omp parallel for schedule(static)
   for (uint64_t e = 0; e < count; e++) {
     uint32_t i = (uint32_t)(start + e);
     /* operator() (rmat_util.h:110-114): 32-bit wrapping products */
@@ -88,28 +93,54 @@ int orc_rmat_edges(uint32_t n, uint32_t seed, double a, double b, double c,
 /* Edge sets                                                              */
 /* ---------------------------------------------------------------------- */
 
+/* LSD radix sort, 11-bit digits (2048 buckets per chunk stay in cache
+ * while scattering).  Each pass is stable: the input is cut
+ * into T contiguous chunks, chunk t counts its digits, the scatter offset of
+ * (digit d, chunk t) is the number of keys with a smaller digit plus those
+ * with digit d in chunks before t, and each chunk scatters in index order.
+ * The result does not depend on T. */
 void orc_sort_u64(uint64_t *keys, uint64_t count) {
   if (count < 2) return;
+  int T = 1;
+#ifdef _OPENMP
+  T = omp_get_max_threads();
+  if ((uint64_t)T > count / 65536 + 1) T = (int)(count / 65536 + 1);  /* small inputs: fewer chunks */
+#endif
   uint64_t *tmp = (uint64_t *)malloc(count * sizeof(uint64_t));
-  uint64_t *cnt = (uint64_t *)malloc(65536 * sizeof(uint64_t));
+  enum { RB = 11, RD = 1 << RB };
+  uint64_t *cnt = (uint64_t *)malloc((size_t)T * RD * sizeof(uint64_t));
   uint64_t *from = keys, *to = tmp;
   uint64_t all_or = 0;
+#pragma omp parallel for reduction(| : all_or) schedule(static) num_threads(T)
   for (uint64_t i = 0; i < count; i++) all_or |= keys[i];
-  for (int shift = 0; shift < 64; shift += 16) {
-    if (((all_or >> shift) & 0xffffu) == 0) continue;
-    memset(cnt, 0, 65536 * sizeof(uint64_t));
-    for (uint64_t i = 0; i < count; i++) cnt[(from[i] >> shift) & 0xffffu]++;
-    uint64_t s = 0;
-    for (int d = 0; d < 65536; d++) {
-      uint64_t t = cnt[d];
-      cnt[d] = s;
-      s += t;
+  for (int shift = 0; shift < 64; shift += RB) {
+    if (((all_or >> shift) & (RD - 1)) == 0) continue;
+#pragma omp parallel num_threads(T)
+    {
+      int t = 0;
+#ifdef _OPENMP
+      t = omp_get_thread_num();
+#endif
+      const uint64_t lo = count * (uint64_t)t / (uint64_t)T, hi = count * (uint64_t)(t + 1) / (uint64_t)T;
+      uint64_t *c = cnt + (size_t)t * RD;
+      memset(c, 0, RD * sizeof(uint64_t));
+      for (uint64_t i = lo; i < hi; i++) c[(from[i] >> shift) & (RD - 1)]++;
+#pragma omp barrier
+#pragma omp single
+      {
+        uint64_t s = 0;
+        for (int d = 0; d < RD; d++)
+          for (int u = 0; u < T; u++) {
+            uint64_t x = cnt[(size_t)u * RD + d];
+            cnt[(size_t)u * RD + d] = s;
+            s += x;
+          }
+      }
+      for (uint64_t i = lo; i < hi; i++) to[c[(from[i] >> shift) & (RD - 1)]++] = from[i];
     }
-    for (uint64_t i = 0; i < count; i++)
-      to[cnt[(from[i] >> shift) & 0xffffu]++] = from[i];
-    uint64_t *t = from;
+    uint64_t *x = from;
     from = to;
-    to = t;
+    to = x;
   }
   if (from != keys) memcpy(keys, from, count * sizeof(uint64_t));
   free(tmp);
@@ -166,6 +197,103 @@ void orc_keys_to_csr(const uint64_t *keys, uint64_t m, uint32_t n,
   for (uint64_t i = 0; i < m; i++) offsets[(keys[i] >> 32) + 1]++;
   for (uint32_t v = 0; v < n; v++) offsets[v + 1] += offsets[v];
   for (uint64_t i = 0; i < m; i++) nbrs[i] = (uint32_t)keys[i];
+}
+
+static int cmp_u32(const void *a, const void *b) {
+  uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+/* The CSR of the edge set {(src[i], dst[i])} (plus the reverse edges when
+ * symmetric, self-loops dropped when drop_loops): rows sorted, duplicates
+ * removed.  Equals orc_keys_to_csr of orc_build_keys's key set, without the
+ * sort of 64-bit keys: each row is counted, filled and sorted on its own.
+ * nbr holds the final edges and needs room for every entry before
+ * de-duplication (count, or 2 count when symmetric).  Returns m. */
+uint64_t orc_edges_to_csr(uint32_t n, const uint32_t *src, const uint32_t *dst, uint64_t count, int symmetric,
+                          int drop_loops, uint64_t *off, uint32_t *nbr) {
+  uint32_t *deg = (uint32_t *)calloc((size_t)n + 1, sizeof(uint32_t));
+#pragma omp parallel for schedule(static)
+  for (uint64_t i = 0; i < count; i++) {
+    if (drop_loops && src[i] == dst[i]) continue;
+    __atomic_fetch_add(&deg[src[i]], 1u, __ATOMIC_RELAXED);
+    if (symmetric) __atomic_fetch_add(&deg[dst[i]], 1u, __ATOMIC_RELAXED);
+  }
+  uint64_t *start = (uint64_t *)malloc(((size_t)n + 1) * sizeof(uint64_t));
+  start[0] = 0;
+  for (uint32_t v = 0; v < n; v++) start[v + 1] = start[v] + deg[v];
+  uint64_t *cur = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+  memcpy(cur, start, (size_t)n * sizeof(uint64_t));
+#pragma omp parallel for schedule(static)
+  for (uint64_t i = 0; i < count; i++) {
+    if (drop_loops && src[i] == dst[i]) continue;
+    nbr[__atomic_fetch_add(&cur[src[i]], 1ull, __ATOMIC_RELAXED)] = dst[i];
+    if (symmetric) nbr[__atomic_fetch_add(&cur[dst[i]], 1ull, __ATOMIC_RELAXED)] = src[i];
+  }
+  free(cur);
+  /* sort and de-duplicate each row in place; deg becomes the unique count */
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (uint32_t v = 0; v < n; v++) {
+    uint32_t *r = nbr + start[v];
+    uint32_t k = deg[v], w = 0;
+    if (k > 1) qsort(r, k, sizeof(uint32_t), cmp_u32);
+    for (uint32_t j = 0; j < k; j++)
+      if (w == 0 || r[j] != r[w - 1]) r[w++] = r[j];
+    deg[v] = w;
+  }
+  off[0] = 0;
+  for (uint32_t v = 0; v < n; v++) off[v + 1] = off[v] + deg[v];
+  /* rows move left only (off[v] <= start[v]) and in order, so a serial pass
+   * never overwrites a row not yet moved */
+  for (uint32_t v = 0; v < n; v++)
+    if (off[v] != start[v]) memmove(nbr + off[v], nbr + start[v], (size_t)deg[v] * sizeof(uint32_t));
+  free(start);
+  free(deg);
+  return off[n];
+}
+
+/* (CSR ∪ ins) \ del as a new CSR (ins, del: sorted unique keys).  Rows are
+ * merged independently.  nbr_out needs room for m + nins. */
+uint64_t orc_csr_update(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint64_t *ins,
+                        uint64_t nins, const uint64_t *del, uint64_t ndel, uint64_t *off_out,
+                        uint32_t *nbr_out) {
+  uint64_t *ib = (uint64_t *)malloc(((size_t)n + 1) * sizeof(uint64_t));
+  uint64_t *db = (uint64_t *)malloc(((size_t)n + 1) * sizeof(uint64_t));
+  uint64_t *cnt = (uint64_t *)malloc(((size_t)n + 1) * sizeof(uint64_t));
+  /* ib[v], db[v]: first insert / delete key with source >= v */
+  uint64_t a = 0, b = 0;
+  for (uint32_t v = 0; v <= n; v++) {
+    while (a < nins && (ins[a] >> 32) < v) a++;
+    while (b < ndel && (del[b] >> 32) < v) b++;
+    ib[v] = a;
+    db[v] = b;
+  }
+  for (int pass = 0; pass < 2; pass++) {
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (uint32_t v = 0; v < n; v++) {
+      uint64_t i = off[v], ie = off[v + 1], j = ib[v], je = ib[v + 1], d = db[v], de = db[v + 1], w = 0;
+      uint32_t *out = pass ? nbr_out + off_out[v] : NULL;
+      while (i < ie || j < je) {
+        uint32_t x;
+        if (j >= je || (i < ie && nbr[i] < (uint32_t)ins[j])) x = nbr[i++];
+        else if (i >= ie || (uint32_t)ins[j] < nbr[i]) x = (uint32_t)ins[j++];
+        else { x = nbr[i++]; j++; }
+        while (d < de && (uint32_t)del[d] < x) d++;
+        if (d < de && (uint32_t)del[d] == x) continue;
+        if (out) out[w] = x;
+        w++;
+      }
+      if (!pass) cnt[v] = w;
+    }
+    if (!pass) {
+      off_out[0] = 0;
+      for (uint32_t v = 0; v < n; v++) off_out[v + 1] = off_out[v] + cnt[v];
+    }
+  }
+  free(ib);
+  free(db);
+  free(cnt);
+  return off_out[n];
 }
 
 /* ---------------------------------------------------------------------- */
@@ -397,6 +525,7 @@ uint64_t orc_tc(uint32_t n, const uint64_t *off, const uint32_t *nbr) {
   /* TC.h:73-87: edgeMap over all vertices; countF::update(v, i) counts
    * count_common(v, i) for every edge with v > i. */
   uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total)
   for (uint32_t i = 0; i < n; i++)
     for (uint64_t e = off[i]; e < off[i + 1]; e++)
       if (nbr[e] > i) total += orc_count_common(off, nbr, nbr[e], i);

@@ -50,6 +50,16 @@ uint64_t orc_set_difference(const uint64_t *a, uint64_t na, const uint64_t *b,
 /* Sorted keys -> CSR (offsets[n+1], nbrs[m]). */
 void orc_keys_to_csr(const uint64_t *keys, uint64_t m, uint32_t n,
                      uint64_t *offsets, uint32_t *nbrs);
+/* The sorted, de-duplicated CSR of an edge list (both directions when
+ * symmetric, self-loops dropped when drop_loops); nbr needs room for every
+ * entry before de-duplication.  Returns m. */
+uint64_t orc_edges_to_csr(uint32_t n, const uint32_t *src, const uint32_t *dst, uint64_t count, int symmetric,
+                          int drop_loops, uint64_t *off, uint32_t *nbr);
+/* (CSR ∪ ins) \ del as a new CSR; ins and del sorted unique keys; nbr_out
+ * needs room for m + nins.  Returns the new m. */
+uint64_t orc_csr_update(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint64_t *ins,
+                        uint64_t nins, const uint64_t *del, uint64_t ndel, uint64_t *off_out,
+                        uint32_t *nbr_out);
 
 /* ---- tier model: include/graph.h:34-39, 256-372, 379-426, 434-502 ----- */
 /* Per-vertex state a reference Graph keeps for tier routing. */

@@ -13,12 +13,16 @@
 //       writes count (u32 src, u32 dst) pairs of rMat<uint32_t>(i)
 //   ref_harness run   <graph.bin> <out_prefix> <src> <iters> [batch.bin...]
 //       graph.bin / batch.bin: u64 n, u64 m, then m (u32 src, u32 dst) pairs
-//       sorted by (src,dst), unique.  Writes <out_prefix>.adj<k> after the
-//       static build (k=0) and after every batch (k=1..), then .bfs .pr .cc
-//       .bc .tc .sssp for the final graph.
+//       sorted by (src,dst), unique.  Writes <out_prefix>.adj<k> and
+//       .btree<k> (B-tree tier membership) after the static build (k=0) and
+//       after every batch (k=1..), then .bfs .pr .cc .bc .tc .sssp for the
+//       final graph.
 //   ref_harness bench <graph.bin> <batch_size> <batches> <seed>
 //       static build, then `batches` rMAT insert batches of batch_size
 //       edges (sorted + unique, both directions), prints JSON timing.
+//   ref_harness mixed <graph.bin> <ins0.bin> <del0.bin> [<ins1.bin> <del1.bin> ...]
+//       static build, then per batch add_edge_batch(ins) and a parallel
+//       remove_edge loop over del; prints JSON timing.
 
 #define ENABLE_LOCK 1
 #define WEIGHTED 0
@@ -89,6 +93,27 @@ void insert(Graph &g, const EdgeFile &ef) {
   g.add_edge_batch(s.data(), d.data(), (uint32_t)s.size(), perm);
 }
 
+// Graph::is_btree (graph.h:192) is private.  An explicit template
+// instantiation may name a private member (access checks do not apply
+// there), so this stores a pointer to it that the dump below calls; the
+// reference's sources are compiled unchanged.
+struct IsBtreeTag {
+  typedef bool (Graph::*type)(const Graph::vertex) const;
+  friend type is_btree_ptr(IsBtreeTag);
+};
+template <typename Tag, typename Tag::type M>
+struct ExposeMember {
+  friend typename Tag::type is_btree_ptr(Tag) { return M; }
+};
+template struct ExposeMember<IsBtreeTag, &Graph::is_btree>;
+
+// B-tree tier membership: u8 per vertex, 1 when aux_neighbors is set.
+void dump_btree(Graph &g, uint64_t n, const std::string &path) {
+  std::vector<uint8_t> out(n);
+  for (uint64_t v = 0; v < n; v++) out[v] = (g.*is_btree_ptr(IsBtreeTag{}))((uint32_t)v) ? 1 : 0;
+  write_blob(path, out.data(), out.size());
+}
+
 // Adjacency dump: u32 degree[n], then all neighbours in NeighborIterator
 // order (inline, then PMA or B-tree in order).
 void dump_adjacency(Graph &g, uint64_t n, const std::string &path) {
@@ -136,11 +161,13 @@ int cmd_run(int argc, char **argv) {
   Graph g((uint32_t)n);
   insert(g, base);
   dump_adjacency(g, n, prefix + ".adj0");
+  dump_btree(g, n, prefix + ".btree0");
   for (int k = 6; k < argc; k++) {
     EdgeFile b;
     if (!read_edges(argv[k], b)) { fprintf(stderr, "bad batch file\n"); return 2; }
     insert(g, b);
     dump_adjacency(g, n, prefix + ".adj" + std::to_string(k - 5));
+    dump_btree(g, n, prefix + ".btree" + std::to_string(k - 5));
   }
   int32_t *parents = BFS_with_edge_map(g, src);
   write_blob(prefix + ".bfs", parents, n * sizeof(int32_t));
@@ -205,11 +232,53 @@ int cmd_bench(int argc, char **argv) {
   return 0;
 }
 
+// The benchmark's mixed step on the reference (src/graph_bm.cc:354-378):
+// static build, then per batch add_edge_batch of the insert file (sorted,
+// unique; the sort is outside the timed region, as in graph_bm.cc) followed
+// by a parallel_for of remove_edge over the delete file.  The delete files
+// hold existing edges whose endpoints appear once across all batches, so the
+// overlapping-memcpy defect of remove_edge (graph.h:700,735) never meets an
+// edge it touched before.
+int cmd_mixed(int argc, char **argv) {
+  if (argc < 5 || (argc - 3) % 2) return 1;
+  EdgeFile base;
+  if (!read_edges(argv[2], base)) { fprintf(stderr, "bad graph file\n"); return 2; }
+  uint64_t n = base.n;
+  Graph g((uint32_t)n);
+  double t0 = now_s();
+  insert(g, base);
+  double t_build = now_s() - t0;
+  double t_ins = 0, t_del = 0;
+  uint64_t n_ins = 0, n_del = 0;
+  int batches = 0;
+  for (int k = 3; k + 1 < argc; k += 2) {
+    EdgeFile ins, del;
+    if (!read_edges(argv[k], ins) || !read_edges(argv[k + 1], del)) { fprintf(stderr, "bad batch file\n"); return 2; }
+    double t1 = now_s();
+    insert(g, ins);
+    double t2 = now_s();
+    const uint32_t nd = (uint32_t)del.src.size();
+    parallel_for(uint32_t i = 0; i < nd; i++) { g.remove_edge(del.src[i], del.dst[i]); }
+    double t3 = now_s();
+    t_ins += t2 - t1;
+    t_del += t3 - t2;
+    n_ins += ins.src.size();
+    n_del += nd;
+    batches++;
+  }
+  printf("{\"threads\": %d, \"static_edges\": %zu, \"static_s\": %.6f, \"batches\": %d, "
+         "\"inserted\": %llu, \"deleted\": %llu, \"insert_s\": %.6f, \"delete_s\": %.6f, "
+         "\"edges_per_s\": %.6e}\n",
+         getWorkers(), base.src.size(), t_build, batches, (unsigned long long)n_ins, (unsigned long long)n_del,
+         t_ins, t_del, (n_ins + n_del) / (t_ins + t_del));
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: ref_harness rmat|run|bench ...\n");
+    fprintf(stderr, "usage: ref_harness rmat|run|bench|mixed ...\n");
     return 1;
   }
   std::string cmd = argv[1];
@@ -217,6 +286,7 @@ int main(int argc, char **argv) {
   if (cmd == "rmat") rc = cmd_rmat(argc, argv);
   else if (cmd == "run") rc = cmd_run(argc, argv);
   else if (cmd == "bench") rc = cmd_bench(argc, argv);
+  else if (cmd == "mixed") rc = cmd_mixed(argc, argv);
   if (rc == 1) fprintf(stderr, "bad arguments\n");
   return rc;
 }

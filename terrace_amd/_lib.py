@@ -85,6 +85,8 @@ _SIGS = {
     "tg_delete_batch": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64]),
     "tg_insert_batch_dev": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64]),
     "tg_delete_batch_dev": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64]),
+    "tg_update_batch": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64, _P, _P, ctypes.c_uint64]),
+    "tg_update_batch_dev": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64, _P, _P, ctypes.c_uint64]),
     "tg_add_edge": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]),
     "tg_remove_edge": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]),
     "tg_is_edge": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),

@@ -186,6 +186,34 @@ int tg_delete_batch_dev(tg_graph *G, const uint32_t *src, const uint32_t *dst, u
   });
 }
 
+int tg_update_batch_dev(tg_graph *G, const uint32_t *ins_src, const uint32_t *ins_dst, uint64_t n_ins,
+                        const uint32_t *del_src, const uint32_t *del_dst, uint64_t n_del) {
+  return guard([&] {
+    NEED(G && (n_ins == 0 || (ins_src && ins_dst)) && (n_del == 0 || (del_src && del_dst)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    G->g.update_batch(ins_src, ins_dst, n_ins, del_src, del_dst, n_del);
+  });
+}
+
+int tg_update_batch(tg_graph *G, const uint32_t *ins_src, const uint32_t *ins_dst, uint64_t n_ins,
+                    const uint32_t *del_src, const uint32_t *del_dst, uint64_t n_del) {
+  return guard([&] {
+    NEED(G && (n_ins == 0 || (ins_src && ins_dst)) && (n_del == 0 || (del_src && del_dst)), "null argument");
+    std::lock_guard<std::mutex> l(G->g.mu);
+    const uint64_t count = n_ins + n_del;
+    if (!count) return;
+    // staged as one array pair: the inserts, then the deletes
+    hipStream_t st = G->g.stream();
+    G->s_src.ensure(count + 1, st);
+    G->s_dst.ensure(count + 1, st);
+    h2d(G->s_src.get(), ins_src, n_ins * 4, st);
+    h2d(G->s_dst.get(), ins_dst, n_ins * 4, st);
+    h2d(G->s_src.get() + n_ins, del_src, n_del * 4, st);
+    h2d(G->s_dst.get() + n_ins, del_dst, n_del * 4, st);
+    G->g.update_batch(G->s_src.get(), G->s_dst.get(), n_ins, G->s_src.get() + n_ins, G->s_dst.get() + n_ins, n_del);
+  });
+}
+
 static int single_edge(tg_graph *G, uint32_t s, uint32_t d, int *result, bool insert) {
   return guard([&] {
     NEED(G && result, "null argument");
@@ -1087,6 +1115,10 @@ int tg_stats(tg_graph *G, uint64_t *out) {
     out[7] = s.tree_small_updates;
     out[8] = s.pma_leaf_merges;
     out[9] = G->g.pma_elements();
+    out[10] = s.last_ins_keys;
+    out[11] = s.last_ins_sources;
+    out[12] = s.last_del_keys;
+    out[13] = s.last_del_sources;
   });
 }
 

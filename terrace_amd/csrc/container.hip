@@ -228,6 +228,16 @@ __device__ inline uint64_t lower_bound_dev(const T *a, uint64_t n, T x) {
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
 
 // ------------------------------------------------------------ batch prep
+// Mixed batches (inserts and deletes in one call) sort tagged keys:
+// src << 32 | dst << 1 | op (op 0 insert, 1 delete; ids are below 2^31), so
+// the keys of one edge are adjacent, its insert before its delete.
+__host__ __device__ inline uint64_t make_tagged(uint32_t s, uint32_t d, uint32_t op) {
+  return (static_cast<uint64_t>(s) << 32) | (static_cast<uint64_t>(d) << 1) | op;
+}
+__host__ __device__ inline uint64_t untag(uint64_t t) {
+  return (t & 0xFFFFFFFF00000000ull) | ((t & 0xFFFFFFFFull) >> 1);
+}
+
 __global__ void k_pack(const uint32_t *src, const uint32_t *dst, uint64_t n, uint32_t nv, uint64_t *keys,
                        uint64_t *err) {
   GRID_STRIDE(i, n) {
@@ -326,6 +336,97 @@ __global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, c
   if (t == 0) *nunique = tot;
 }
 
+// Mixed batches of at most kSmallSortKeys edges in all: the inserts
+// (is, id) and deletes (ds, dd) packed as tagged keys, validated, sorted in
+// LDS, and split by the last op of each edge (deletes come after inserts,
+// so a batch that inserts and deletes an edge leaves it absent, as applying
+// the inserts and then the deletes does): edges inserted into I (count
+// *ni_out), edges deleted into D (count *nd_out).  Clears the batch scalars.
+__global__ void __launch_bounds__(1024) k_pack_sort_small_mixed(const uint32_t *is, const uint32_t *id, uint32_t ni,
+                                                                const uint32_t *ds, const uint32_t *dd, uint32_t nd,
+                                                                uint32_t nv, uint64_t *I, uint64_t *D,
+                                                                uint64_t *err, uint64_t *ni_out,
+                                                                uint64_t *nd_out) {
+  __shared__ uint64_t s[kSmallSortKeys];
+  __shared__ uint32_t s_w[16];
+  const uint32_t t = threadIdx.x;
+  const uint32_t count = ni + nd;
+  if (t < 14) err[t] = 0;  // the batch scalars (the look-back error words [14], [15] stay)
+  __syncthreads();
+  uint32_t P = 1;
+  while (P < count) P <<= 1;
+  bool bad = false;
+  for (uint32_t i = t; i < P; i += blockDim.x) {
+    uint64_t v = ~0ull;
+    if (i < count) {
+      const uint32_t a = i < ni ? is[i] : ds[i - ni], b = i < ni ? id[i] : dd[i - ni];
+      if (a >= nv || b >= nv) {
+        bad = true;
+        v = 0;
+      } else {
+        v = make_tagged(a, b, i < ni ? 0u : 1u);
+      }
+    }
+    s[i] = v;
+  }
+  if (bad) atomicOr(reinterpret_cast<unsigned long long *>(err), 1ull);
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = t; i < P; i += blockDim.x) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = s[i], b = s[ixj];
+          if ((a > b) == ((i & k) == 0)) {
+            s[i] = b;
+            s[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // the last key of each edge decides: thread t takes keys 4t .. 4t+3
+  bool fi[4], fd[4];
+  uint32_t ci = 0, cd = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) {
+    const uint32_t i = 4 * t + j;
+    const bool last = i < count && (i + 1 == count || (s[i] >> 1) != (s[i + 1] >> 1));
+    fi[j] = last && (s[i] & 1) == 0;
+    fd[j] = last && (s[i] & 1) != 0;
+    ci += fi[j] ? 1u : 0u;
+    cd += fd[j] ? 1u : 0u;
+  }
+  uint32_t toti, totd;
+  uint32_t pi = wg_excl_scan(ci, s_w, &toti);
+  uint32_t pd = wg_excl_scan(cd, s_w, &totd);
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) {
+    const uint64_t k = untag(s[4 * t + j]);
+    if (fi[j]) I[pi++] = k;
+    if (fd[j]) D[pd++] = k;
+  }
+  if (t == 0) {
+    *ni_out = toti;
+    *nd_out = totd;
+  }
+}
+
+// Radix path of mixed batches: tagged keys.
+__global__ void k_pack_mixed(const uint32_t *is, const uint32_t *id, uint64_t ni, const uint32_t *ds,
+                             const uint32_t *dd, uint64_t nd, uint32_t nv, uint64_t *keys, uint64_t *err) {
+  GRID_STRIDE(i, ni + nd) {
+    const uint32_t s = i < ni ? is[i] : ds[i - ni], d = i < ni ? id[i] : dd[i - ni];
+    if (s >= nv || d >= nv) {
+      atomicOr(reinterpret_cast<unsigned long long *>(err), 1ull);
+      keys[i] = 0;
+    } else {
+      keys[i] = make_tagged(s, d, i < ni ? 0u : 1u);
+    }
+  }
+}
+
 // Batches of more than kSmallSortKeys and at most kBucketSortMax pairs: a
 // bucket sort.  The compressed key c = src << bits | dst (vertex ids below
 // 2^bits) orders like the key; its top 12 bits pick one of 4096 buckets,
@@ -341,37 +442,45 @@ constexpr uint64_t kBucketSortMax = 1u << 18;
 constexpr uint64_t kSortOverflow = 256;
 constexpr uint32_t kBucketBlock = 256, kBucketItems = 8, kBucketTile = kBucketBlock * kBucketItems;
 
+template <bool kTagged>
 __device__ inline uint32_t bucket_of(uint64_t key, uint32_t bits, uint32_t shift) {
-  const uint64_t c = (static_cast<uint64_t>(key_src(key)) << bits) | key_dst(key);
+  const uint64_t k = kTagged ? untag(key) : key;
+  const uint64_t c = (static_cast<uint64_t>(key_src(k)) << bits) | key_dst(k);
   return static_cast<uint32_t>(c >> shift);
 }
 
+// Pack, validate and count per bucket.  Items [0, count) come from
+// (src, dst); with kTagged, items [count, count + count2) from (src2, dst2)
+// with op 1, the first ones with op 0.
+template <bool kTagged>
 __global__ void __launch_bounds__(kBucketBlock) k_bucket_count(const uint32_t *src, const uint32_t *dst,
-                                                               uint32_t count, uint32_t nv, uint32_t bits,
-                                                               uint32_t shift, uint64_t *packed, uint32_t *bcnt,
-                                                               uint64_t *err) {
+                                                               uint32_t count, const uint32_t *src2,
+                                                               const uint32_t *dst2, uint32_t count2, uint32_t nv,
+                                                               uint32_t bits, uint32_t shift, uint64_t *packed,
+                                                               uint32_t *bcnt, uint64_t *err) {
   __shared__ uint32_t s_c[kBuckets];
   const uint32_t t = threadIdx.x;
   for (uint32_t i = t; i < kBuckets; i += kBucketBlock) s_c[i] = 0;
   __syncthreads();
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBucketTile;
+  const uint64_t total = static_cast<uint64_t>(count) + (kTagged ? count2 : 0u);
   uint32_t ra[kBucketItems], rb[kBucketItems];
 #pragma unroll
   for (uint32_t j = 0; j < kBucketItems; j++) {
     const uint64_t i = base + j * kBucketBlock + t;
-    ra[j] = i < count ? src[i] : 0;
-    rb[j] = i < count ? dst[i] : 0;
+    ra[j] = i < count ? src[i] : (kTagged && i < total ? src2[i - count] : 0);
+    rb[j] = i < count ? dst[i] : (kTagged && i < total ? dst2[i - count] : 0);
   }
   bool bad = false;
 #pragma unroll
   for (uint32_t j = 0; j < kBucketItems; j++) {
     const uint64_t i = base + j * kBucketBlock + t;
-    if (i >= count) continue;
+    if (i >= total) continue;
     uint64_t key = 0;
     if (ra[j] >= nv || rb[j] >= nv) bad = true;
-    else key = make_key(ra[j], rb[j]);
+    else key = kTagged ? make_tagged(ra[j], rb[j], i >= count ? 1u : 0u) : make_key(ra[j], rb[j]);
     packed[i] = key;
-    atomicAdd(&s_c[bucket_of(key, bits, shift)], 1u);
+    atomicAdd(&s_c[bucket_of<kTagged>(key, bits, shift)], 1u);
   }
   if (bad) atomicOr(reinterpret_cast<unsigned long long *>(err), 1ull);
   __syncthreads();
@@ -407,6 +516,7 @@ __device__ inline void bucket_starts(const uint32_t *bcnt, uint32_t *s_off, uint
   __syncthreads();
 }
 
+template <bool kTagged>
 __global__ void __launch_bounds__(kBucketBlock) k_bucket_scatter(const uint64_t *packed, uint32_t count,
                                                                  uint32_t bits, uint32_t shift,
                                                                  const uint32_t *bcnt, uint32_t *bcur,
@@ -428,7 +538,7 @@ __global__ void __launch_bounds__(kBucketBlock) k_bucket_scatter(const uint64_t 
 #pragma unroll
   for (uint32_t j = 0; j < kBucketItems; j++) {
     const uint64_t i = base + j * kBucketBlock + t;
-    r[j] = i < count ? atomicAdd(&s_c[bucket_of(k[j], bits, shift)], 1u) : 0u;
+    r[j] = i < count ? atomicAdd(&s_c[bucket_of<kTagged>(k[j], bits, shift)], 1u) : 0u;
   }
   __syncthreads();
   // one global reservation per bucket this block holds keys of
@@ -439,7 +549,7 @@ __global__ void __launch_bounds__(kBucketBlock) k_bucket_scatter(const uint64_t 
   for (uint32_t j = 0; j < kBucketItems; j++) {
     const uint64_t i = base + j * kBucketBlock + t;
     if (i >= count) continue;
-    const uint32_t pos = s_off[bucket_of(k[j], bits, shift)] + r[j];
+    const uint32_t pos = s_off[bucket_of<kTagged>(k[j], bits, shift)] + r[j];
     if (pos < count) out[pos] = k[j];  // always, unless the counters were stale
   }
 }
@@ -1241,6 +1351,18 @@ struct SelUnique {
   __device__ bool pred(uint64_t i) const { return i == 0 || k[i] != k[i - 1]; }
   __device__ void emit(uint64_t i, uint64_t pos) const { out[pos] = k[i]; }
 };
+// Sorted tagged keys: the last key of each edge is its final op; inserted
+// edges go to output 1 (I), deleted ones to output 2 (D), untagged.
+struct SelSplitOps {
+  const uint64_t *t;
+  uint64_t n;
+  uint64_t *I, *D;
+  __device__ uint32_t pick(uint64_t i) const {
+    const bool last = i + 1 == n || (t[i] >> 1) != (t[i + 1] >> 1);
+    return last ? 1u + static_cast<uint32_t>(t[i] & 1) : 0u;
+  }
+  __device__ void emit(uint64_t i, uint32_t which, uint64_t pos) const { (which == 1 ? I : D)[pos] = untag(t[i]); }
+};
 struct SelRunStart {
   const uint64_t *k;
   uint32_t *beg;
@@ -1969,8 +2091,10 @@ __global__ void __launch_bounds__(256) k_tree_small_apply(GView g, const uint64_
 // ------------------------------------------------------------ delete batch
 // One thread per run of existing keys: remove inline hits, mark secondary
 // deletes, and record how many inline slots must be refilled.
+// *pma_taken (k_bump's counter of PMA elements moved inline) is cleared here.
 __global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_beg, uint32_t nruns,
-                              uint8_t *sec_route, uint32_t *run_bump) {
+                              uint8_t *sec_route, uint32_t *run_bump, uint64_t *pma_taken) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *pma_taken = 0;
   GRID_STRIDE(r, nruns) {
     uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
     uint32_t s = key_src(R[b0]);
@@ -2541,10 +2665,10 @@ bool Graph::prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_
     sbkeys_.ensure(count, stream_);
     const uint32_t shift = 2 * bits > kBucketBits ? 2 * bits - kBucketBits : 0;
     const unsigned nb = static_cast<unsigned>((count + kBucketTile - 1) / kBucketTile);
-    k_bucket_count<<<nb, kBucketBlock, 0, stream_>>>(src, dst, static_cast<uint32_t>(count), n_, bits, shift,
-                                                     k0_.get(), sbcnt_.get(), dsc_.get());
+    k_bucket_count<false><<<nb, kBucketBlock, 0, stream_>>>(src, dst, static_cast<uint32_t>(count), nullptr, nullptr,
+                                                            0u, n_, bits, shift, k0_.get(), sbcnt_.get(), dsc_.get());
     TG_CHECK_LAUNCH();
-    k_bucket_scatter<<<nb, kBucketBlock, 0, stream_>>>(k0_.get(), static_cast<uint32_t>(count), bits, shift,
+    k_bucket_scatter<false><<<nb, kBucketBlock, 0, stream_>>>(k0_.get(), static_cast<uint32_t>(count), bits, shift,
                                                        sbcnt_.get(), sbcur_.get(), sboff_.get(), sbkeys_.get());
     TG_CHECK_LAUNCH();
     k_bucket_sort<<<kBuckets, kBucketBlock, 0, stream_>>>(sbkeys_.get(), static_cast<uint32_t>(count),
@@ -2560,6 +2684,59 @@ bool Graph::prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_
   }
   select(SelUnique{k0_.get(), k1_.get()}, count, dsc_.get() + 1, scan_scratch, stream_);
   return false;
+}
+
+// The mixed batch's sort: inserts (is, id, ni) and deletes (ds, dd, nd)
+// as tagged keys, sorted, split by each edge's last op into I (k1_, count
+// in dsc_[1]) and D (kD_, count in dsc_[9]); error bits in dsc_[0].
+// Returns true when the bucket sort ran (see prepare_keys_async).
+bool Graph::prepare_mixed_async(const uint32_t *is, const uint32_t *id, uint64_t ni, const uint32_t *ds,
+                                const uint32_t *dd, uint64_t nd, bool radix) {
+  const uint64_t count = ni + nd;
+  k1_.ensure(count + 1, stream_);
+  kD_.ensure(count + 1, stream_);
+  if (count <= kSmallSortKeys) {
+    // also clears the batch scalars
+    k_pack_sort_small_mixed<<<1, 1024, 0, stream_>>>(is, id, static_cast<uint32_t>(ni), ds, dd,
+                                                     static_cast<uint32_t>(nd), n_, k1_.get(), kD_.get(), dsc_.get(),
+                                                     dsc_.get() + 1, dsc_.get() + 9);
+    TG_CHECK_LAUNCH();
+    return false;
+  }
+  k0_.ensure(count, stream_);
+  uint32_t bits = 1;
+  while (bits < 32 && (1ull << bits) < n_) bits++;
+  fill_u64(dsc_.get(), 0, kErrSort, stream_);  // the look-back error words stay until checked
+  const uint64_t *sorted = nullptr;
+  if (count <= kBucketSortMax && !radix) {
+    if (sbcnt_.ensure(kBuckets, stream_)) fill_u32(sbcnt_.get(), 0, kBuckets, stream_);
+    if (sbcur_.ensure(kBuckets, stream_)) fill_u32(sbcur_.get(), 0, kBuckets, stream_);
+    sboff_.ensure(kBuckets, stream_);
+    sbkeys_.ensure(count, stream_);
+    // compressed key (src << bits | dst) of 2 * bits bits; the op bit below
+    // it does not take part in the bucket
+    const uint32_t shift = 2 * bits > kBucketBits ? 2 * bits - kBucketBits : 0;
+    const unsigned nb = static_cast<unsigned>((count + kBucketTile - 1) / kBucketTile);
+    k_bucket_count<true><<<nb, kBucketBlock, 0, stream_>>>(is, id, static_cast<uint32_t>(ni), ds, dd,
+                                                           static_cast<uint32_t>(nd), n_, bits, shift, k0_.get(),
+                                                           sbcnt_.get(), dsc_.get());
+    TG_CHECK_LAUNCH();
+    k_bucket_scatter<true><<<nb, kBucketBlock, 0, stream_>>>(k0_.get(), static_cast<uint32_t>(count), bits, shift,
+                                                             sbcnt_.get(), sbcur_.get(), sboff_.get(), sbkeys_.get());
+    TG_CHECK_LAUNCH();
+    k_bucket_sort<<<kBuckets, kBucketBlock, 0, stream_>>>(sbkeys_.get(), static_cast<uint32_t>(count),
+                                                          sboff_.get(), sbcnt_.get(), sbcur_.get(), dsc_.get());
+    TG_CHECK_LAUNCH();
+    sorted = sbkeys_.get();
+  } else {
+    k_pack_mixed<<<grid_for(count, 256), 256, 0, stream_>>>(is, id, ni, ds, dd, nd, n_, k0_.get(), dsc_.get());
+    TG_CHECK_LAUNCH();
+    radix_sort_keys(k0_.get(), count, bits + 1, bits, sort_, stream_);  // dst << 1 | op, then src
+    sorted = k0_.get();
+  }
+  select2(SelSplitOps{sorted, count, k1_.get(), kD_.get()}, count, dsc_.get() + 1, dsc_.get() + 9, scan_scratch,
+          stream_);
+  return count <= kBucketSortMax && !radix;
 }
 
 // Internal consistency flags raised by kernels during the batch.
@@ -3044,6 +3221,9 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
     if (!bucket || !(h[0] & kSortOverflow)) break;
   }
   check_prep(h);
+  stats_.last_ins_keys = h[1];
+  stats_.last_ins_sources = h[2];
+  stats_.last_del_keys = stats_.last_del_sources = 0;
   if (h[1] == 0) return;
   insert_runs(R, h[1], static_cast<uint32_t>(h[2]));  // run_beg[nruns] = nR is written
 }
@@ -3261,6 +3441,9 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
     if (!bucket || !(h[0] & kSortOverflow)) break;
   }
   check_prep(h);
+  stats_.last_ins_keys = stats_.last_ins_sources = 0;
+  stats_.last_del_keys = h[2];
+  stats_.last_del_sources = h[3];
   if (h[2] == 0) return;
   delete_runs(k2_.get(), h[2], static_cast<uint32_t>(h[3]));  // run_beg[nruns] = nR is written
 }
@@ -3292,6 +3475,90 @@ void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
   delete_runs(R, nR, static_cast<uint32_t>(h[3]));
 }
 
+// A mixed batch: the inserts, then the deletes (the driver's add_edge_batch
+// followed by its remove_edge loop, src/graph_bm.cc:354-378), with one sort
+// and one preparation read for both: each edge's last op decides (an edge
+// both inserted and deleted ends absent), deletes of absent edges are
+// no-ops.  The delete half's existence check runs before the insert half
+// changes the graph: the two halves hold disjoint edges, so inserting one
+// cannot change whether an edge of the other exists.
+void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, const uint32_t *ds,
+                         const uint32_t *dd, uint64_t nd) {
+  const uint64_t count = ni + nd;
+  if (count == 0) return;
+  if (ws_) throw Error(-1, "weighted graph: updates without weights");
+  if (count > 0x7FFFFFFFull) throw Error(-1, "mixed batch larger than 2^31 edges");
+  if (read_only_) throw Error(-1, "graph is a query-only snapshot");
+  const bool small = count <= kSmallSortKeys;
+  uint64_t h[kScalars];
+  for (int pass = 0; pass < 2; pass++) {
+    const bool bucket = prepare_mixed_async(is, id, ni, ds, dd, nd, pass == 1);
+    GView g = view();
+    const uint64_t *I = k1_.get(), *D = kD_.get();
+    f0_.ensure(count + 1, stream_);
+    f1_.ensure(count + 1, stream_);
+    fD_.ensure(count + 1, stream_);
+    kE_.ensure(count + 1, stream_);
+    s32_[0].ensure(count + 1, stream_);
+    s32_[1].ensure(count + 1, stream_);
+    // inserted edges: existence, new-key prefix, source runs (run_beg in
+    // s32_[0], count in dsc_[2]); deleted edges: the existing ones into kE_
+    // (count dsc_[10]), their runs in s32_[1] (count dsc_[11])
+    if (small) {
+      k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, I, dsc_.get() + 1, count, f0_.get(), f1_.get());
+      TG_CHECK_LAUNCH();
+      k_small_runs<true><<<1, 1024, 0, stream_>>>(I, f1_.get(), static_cast<uint32_t>(count), nullptr,
+                                                  s32_[0].get(), dsc_.get());
+      TG_CHECK_LAUNCH();
+      k_exists_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, D, dsc_.get() + 9, count, fD_.get());
+      TG_CHECK_LAUNCH();
+      k_small_runs<false><<<1, 1024, 0, stream_>>>(D, fD_.get(), static_cast<uint32_t>(count), kE_.get(),
+                                                   s32_[1].get(), dsc_.get() + 8);
+      TG_CHECK_LAUNCH();
+    } else if (count <= kBucketSortMax) {
+      const uint32_t nb = static_cast<uint32_t>((count + kPrepBlock - 1) / kPrepBlock);
+      uint32_t epoch = scan_scratch.lb.begin(2ull * nb, stream_);
+      k_insert_prep<1, true><<<nb, kPrepBlock, 0, stream_>>>(
+          g, I, dsc_.get() + 1, count, f0_.get(), f1_.get(), s32_[0].get(), dsc_.get() + 2,
+          scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb, epoch);
+      TG_CHECK_LAUNCH();
+      epoch = scan_scratch.lb.begin(nb, stream_);
+      k_delete_prep<<<nb, kPrepBlock, 0, stream_>>>(g, D, dsc_.get() + 9, kE_.get(), dsc_.get() + 10,
+                                                    scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(),
+                                                    scan_scratch.lb.err_word(), nb, epoch);
+      TG_CHECK_LAUNCH();
+      select(SelRunStartN{kE_.get(), s32_[1].get(), dsc_.get() + 10}, count, dsc_.get() + 11, scan_scratch, stream_);
+    } else {
+      k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, I, dsc_.get() + 1, count, f0_.get(), f1_.get());
+      TG_CHECK_LAUNCH();
+      const uint32_t nb = static_cast<uint32_t>((count + 8 * kPrepBlock - 1) / (8 * kPrepBlock));
+      const uint32_t epoch = scan_scratch.lb.begin(2ull * nb, stream_);
+      k_insert_prep<8, false><<<nb, kPrepBlock, 0, stream_>>>(
+          g, I, dsc_.get() + 1, count, f0_.get(), f1_.get(), s32_[0].get(), dsc_.get() + 2,
+          scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb, epoch);
+      TG_CHECK_LAUNCH();
+      k_exists_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, D, dsc_.get() + 9, count, fD_.get());
+      TG_CHECK_LAUNCH();
+      select(SelFlagged{D, fD_.get(), kE_.get()}, count, dsc_.get() + 10, scan_scratch, stream_);
+      select(SelRunStartN{kE_.get(), s32_[1].get(), dsc_.get() + 10}, count, dsc_.get() + 11, scan_scratch, stream_);
+    }
+    if (debug_fail & 1u) {
+      debug_fail &= ~1u;
+      fill_u64(dsc_.get() + kErrScan, 1, 1, stream_);
+    }
+    read_scalars(dsc_.get(), h, kScalars, stream_);
+    if (!bucket || !(h[0] & kSortOverflow)) break;
+  }
+  check_prep(h);
+  const uint64_t nI = h[1], nIruns = h[2], nE = h[10], nEruns = h[11];
+  stats_.last_ins_keys = nI;
+  stats_.last_ins_sources = nIruns;
+  stats_.last_del_keys = nE;
+  stats_.last_del_sources = nEruns;
+  if (nI) insert_runs(k1_.get(), nI, static_cast<uint32_t>(nIruns));
+  if (nE) delete_runs(kE_.get(), nE, static_cast<uint32_t>(nEruns), s32_[1].get());
+}
+
 // Scratch of the delete's common path (see presize_insert).  Inline
 // refills number at most the deleted keys.
 void Graph::presize_delete(uint64_t nR, uint32_t nruns) {
@@ -3307,24 +3574,24 @@ void Graph::presize_delete(uint64_t nR, uint32_t nruns) {
   s32_[30].ensure(1 + 3 * std::min<uint64_t>(nR, kTreeSmallKeys), stream_);
 }
 
-void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
+void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint32_t *run_beg) {
   presize_delete(nR, nruns);
-  mutate([&] { delete_runs_apply(R, nR, nruns); });
+  mutate([&] { delete_runs_apply(R, nR, nruns, run_beg ? run_beg : s32_[0].get()); });
 }
 
 // The delete proper: R (in k2_) holds existing keys only and s32_[0] their
 // run starts with run_beg[nruns] = nR.
-void Graph::delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
+void Graph::delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint32_t *run_beg) {
   lists_dirty_ = true;
   ++epoch_;
   GView g = view();
-  DevBuf<uint32_t> &run_beg = s32_[0], &run_bump = s32_[12], &bump_off = s32_[13];
+  DevBuf<uint32_t> &run_bump = s32_[12], &bump_off = s32_[13];
   DevBuf<uint8_t> &sec_route = s8_[5];
   run_bump.ensure(nruns + 1, stream_);
   bump_off.ensure(nruns + 1, stream_);
   sec_route.ensure(nR + 1, stream_);
-  k_delete_runs<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, sec_route.get(),
-                                                           run_bump.get());
+  k_delete_runs<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg, nruns, sec_route.get(), run_bump.get(),
+                                                           dsc_.get() + 9);
   TG_CHECK_LAUNCH();
   if (debug_fail & 2u) {
     debug_fail &= ~2u;
@@ -3369,7 +3636,7 @@ void Graph::delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
       // PMA refills are removed from the PMA by k_bump itself (count in
       // dsc_[9]); the second pass, after a B-tree fallback, redoes only
       // the B-tree vertices
-      k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg.get(), nruns, run_bump.get(), bump_off.get(),
+      k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg, nruns, run_bump.get(), bump_off.get(),
                                                         bkeys.get(), broute.get(), pass == 1, dsc_.get() + 9);
       TG_CHECK_LAUNCH();
       tree_dirty_ = true;

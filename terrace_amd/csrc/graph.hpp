@@ -57,6 +57,9 @@ struct Stats {
   uint64_t promotions = 0;
   uint64_t tree_small_updates = 0;  // tree updates done in place
   uint64_t pma_leaf_merges = 0;     // PMA inserts merged inside their leaves
+  // the last batch: unique inserted keys and their sources, existing
+  // deleted keys and their sources
+  uint64_t last_ins_keys = 0, last_ins_sources = 0, last_del_keys = 0, last_del_sources = 0;
 };
 
 class Graph {
@@ -74,6 +77,11 @@ class Graph {
   // inputs are not modified.  Any order, duplicates allowed.
   void insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t count);
   void delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t count);
+  // A mixed batch (device pointers): the ni inserts, then the nd deletes,
+  // with one sort and preparation for both; equals insert_batch(is, id, ni)
+  // followed by delete_batch(ds, dd, nd).
+  void update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, const uint32_t *ds, const uint32_t *dd,
+                    uint64_t nd);
   // Sorted unique keys (src<<32|dst) already in HBM.
   void insert_sorted(const uint64_t *keys, uint64_t n);
   void delete_sorted(const uint64_t *keys, uint64_t n);
@@ -213,6 +221,8 @@ class Graph {
   // scratch
   SortScratch sort_;
   DevBuf<uint64_t> k0_, k1_, k2_, k3_;
+  DevBuf<uint64_t> kD_, kE_;  // mixed batches: deleted edges, the existing ones among them
+  DevBuf<uint32_t> fD_;       // mixed batches: existence flags of the deleted edges
   DevBuf<uint32_t> f0_, f1_, f2_, f3_;
   DevBuf<uint64_t> g0_, g1_, g2_, g3_;
   DevBuf<uint32_t> u0_, u1_, u2_, u3_, u4_;
@@ -243,11 +253,14 @@ class Graph {
   void compute_bounds();
   // true when the bucket sort ran (its overflow bit may then be set)
   bool prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_t count, bool radix = false);
+  bool prepare_mixed_async(const uint32_t *is, const uint32_t *id, uint64_t ni, const uint32_t *ds,
+                           const uint32_t *dd, uint64_t nd, bool radix = false);
   // presize, then *_apply under mutate()
   void insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
-  void delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
+  // run_beg: the runs' starts (null: s32_[0])
+  void delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint32_t *run_beg = nullptr);
   void insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns);
-  void delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns);
+  void delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint32_t *run_beg);
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
   void check_batch_flags();
   void check_flags(const uint64_t *h);

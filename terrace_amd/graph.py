@@ -100,6 +100,19 @@ class Graph:
         n = len(s) if edge_count is None else int(edge_count)
         check(lib().tg_delete_batch(self._h, ptr(s), ptr(d), n))
 
+    def update_batch(self, ins_src, ins_dst, del_src, del_dst) -> None:
+        """A mixed batch: add_edge_batch(ins) then remove_edge of every
+        (del) edge (src/graph_bm.cc:354-378), prepared together in one call."""
+        a, b, c, d = _u32(ins_src), _u32(ins_dst), _u32(del_src), _u32(del_dst)
+        if len(a) != len(b) or len(c) != len(d):
+            raise ValueError("source and destination arrays differ in length")
+        check(lib().tg_update_batch(self._h, ptr(a), ptr(b), len(a), ptr(c), ptr(d), len(c)))
+
+    def update_batch_dev(self, ins_src: DeviceBuffer, ins_dst: DeviceBuffer, n_ins: int, del_src: DeviceBuffer,
+                         del_dst: DeviceBuffer, n_del: int) -> None:
+        check(lib().tg_update_batch_dev(self._h, ins_src.ptr, ins_dst.ptr, int(n_ins), del_src.ptr, del_dst.ptr,
+                                        int(n_del)))
+
     def add_edge_batch_dev(self, src: DeviceBuffer, dst: DeviceBuffer, count: int) -> None:
         check(lib().tg_insert_batch_dev(self._h, src.ptr, dst.ptr, int(count)))
 
@@ -186,11 +199,11 @@ class Graph:
         return out
 
     def stats(self) -> dict:
-        out = np.zeros(10, np.uint64)
+        out = np.zeros(14, np.uint64)
         check(lib().tg_stats(self._h, ptr(out)))
         keys = ["pma_rebuilds", "pma_windows", "pma_window_leaves", "tree_blocks_written",
                 "promotions", "trees", "pma_slots", "tree_small_updates", "pma_leaf_merges",
-                "pma_elements"]
+                "pma_elements", "last_ins_keys", "last_ins_sources", "last_del_keys", "last_del_sources"]
         return {k: int(v) for k, v in zip(keys, out)}
 
     def apply_ops(self, src, dst, ops) -> None:

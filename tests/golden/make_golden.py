@@ -13,7 +13,8 @@ Fixtures:
                    (n, seed, a, b, c)
   graph_*.npz      a static build plus insert batches: inputs (sorted unique
                    directed keys), the adjacency after every step (degree
-                   array + neighbours in NeighborIterator order), then BFS
+                   array + neighbours in NeighborIterator order) and the
+                   B-tree tier membership after every step, then BFS
                    parents, PR (10 iterations), CC labels, BC and the
                    triangle count on the final graph.
 
@@ -97,6 +98,7 @@ def make_graph(name, n, base_keys, batches, src, iters=10):
             raw = np.fromfile(prefix + f".adj{k}", np.uint32)
             data[f"deg{k}"] = raw[:n]
             data[f"nbrs{k}"] = raw[n:]
+            data[f"btree{k}"] = np.fromfile(prefix + f".btree{k}", np.uint8)
         data["bfs_parents"] = np.fromfile(prefix + ".bfs", np.int32)
         data["pr"] = np.fromfile(prefix + ".pr", np.float64)
         data["cc"] = np.fromfile(prefix + ".cc", np.uint32)

@@ -3,8 +3,10 @@
 The graphs come from the device generator (bit-exact with the oracle's rMAT
 at smaller sizes, tests/test_gpu_parity.py) and are pinned at full size by
 the counts measured from the reference's generator: directed edges, maximum
-degree, one vertex's degree and the vertices per tier.  Updates are checked
-against the set model, and BFS, PageRank and CC against the oracle."""
+degree, one vertex's degree and the vertices per tier.  Every update batch is
+checked against the set model, and BFS, PageRank, CC, BC and TC against the
+oracle.  Each workload keeps one graph for its module (fixtures), and each
+check is its own test."""
 import numpy as np
 import pytest
 
@@ -51,71 +53,125 @@ def assert_adjacency(G, keys, n, what):
     return ref
 
 
-def check_queries(G, g, src, what):
+def mixed_batch(G, keys, n, seed, size, rng, probs, sources=None):
+    """One update batch of `size` directed edges, 90% inserts (fresh rMAT
+    edges, both directions; or, with `sources`, edges between those vertices
+    and random active ones, both directions) and 10% deletes of existing edges (both directions),
+    applied as the benchmark applies it: one mixed call (the inserts, then
+    the deletes).  Returns the new key set (the set model)."""
+    if sources is None:
+        ins = oracle.build_keys(*oracle.rmat_edges(n, seed, *probs, max(1, int(size * 0.9) // 2)))
+    else:
+        active = np.nonzero(oracle.CSR(keys, n).degrees())[0]
+        cnt = int(size * 0.9)
+        bs = sources[rng.integers(0, len(sources), cnt)].astype(np.uint64)
+        bd = active[rng.integers(0, len(active), cnt)].astype(np.uint64)
+        ins = sym((bs << np.uint64(32)) | bd)  # both directions: the graph stays symmetric
+        ins = ins[(ins >> np.uint64(32)) != (ins & np.uint64(0xFFFFFFFF))]
+    dels = sym(keys[rng.choice(len(keys), max(1, int(size * 0.1) // 2), replace=False)])
+    G.update_batch(*split(ins), *split(dels))
+    return oracle.set_difference(oracle.set_union(keys, ins), dels)
+
+
+def check_queries(G, g, src, what, bc=True):
     parents, levels = tg.BFS_with_edge_map(G, src, with_levels=True)
     lv = oracle.bfs_levels(g, src)
     assert np.array_equal(levels, lv), f"{what}: BFS levels"
     assert oracle.bfs_check_parents(g, src, parents, lv) == 0, f"{what}: BFS parents"
     assert oracle.rel_l1(tg.PR_S(G, 10), oracle.pagerank(g, 10)) <= FP_TOL, f"{what}: PR"
     assert np.array_equal(tg.CC(G), oracle.cc(g)), f"{what}: CC"
+    if bc:
+        got = tg.BC(G, src)
+        ref = oracle.bc(g, src)
+        reach = lv != UNREACHED
+        assert np.all(np.isnan(got[~reach])), f"{what}: BC unreachable not NaN"
+        assert oracle.rel_l1(got[reach], ref[reach]) <= FP_TOL, f"{what}: BC"
 
 
-def test_plain_workload_full_size():
-    """rMAT scale 22, 69M generated edges (a=.5 b=.1 c=.1): 135,910,260
-    directed edges, maximum degree 1,441, deg(9) = 707, 17 B-tree vertices.
-    Two mixed batches (about 9e4 inserts generated with ith_rand(1000 + t),
-    1e4 deletes of existing edges in both directions) match the set model;
-    BFS from 9, PageRank 10 iterations and CC match the oracle."""
+# ------------------------------------------------------------- plain workload
+@pytest.fixture(scope="module")
+def plain():
+    """rMAT scale 22, 69M generated edges (a=.5 b=.1 c=.1), built on the
+    device from the generator's output in HBM."""
     n = 1 << 22
-    say("full size plain workload: build")
+    say("plain workload: build")
     s, d, w, keys = device_graph(n, (0.5, 0.1, 0.1), 69_000_000)
-    assert len(keys) == 135_910_260
     G = tg.Graph(n)
     G.add_edge_batch_dev(s, d, w)
     del s, d
+    st = {"G": G, "keys": keys, "n": n, "rng": np.random.default_rng(5)}
+    yield st
+    G.close()
+
+
+def test_plain_static_graph(plain):
+    """135,910,260 directed edges, maximum degree 1,441, deg(9) = 707,
+    17 B-tree vertices (the reference generator's counts)."""
+    G, keys, n = plain["G"], plain["keys"], plain["n"]
+    assert len(keys) == 135_910_260
     deg = G.degrees()
     assert int(deg.max()) == 1441 and int(deg[9]) == 707
     assert int((G.tiers() == 2).sum()) == 17
     assert_adjacency(G, keys, n, "static")
-    rng = np.random.default_rng(5)
-    for t in range(2):
-        say("full size plain workload: batch", t)
-        seed = oracle.aspen_ith_rand(1000 + t) & 0xFFFFFFFF
-        ins = oracle.build_keys(*oracle.rmat_edges(n, seed, 0.5, 0.1, 0.1, 45000))
-        dels = sym(keys[rng.choice(len(keys), 5000, replace=False)])
-        G.add_edge_batch(*split(ins))
-        keys = oracle.set_union(keys, ins)
-        G.remove_edge_batch(*split(dels))
-        keys = oracle.set_difference(keys, dels)
-        ref = assert_adjacency(G, keys, n, f"batch {t}")
-    say("full size plain workload: queries")
-    check_queries(G, ref, 9, "plain full size")
-    G.close()
 
 
-def test_skew_workload_full_size():
-    """Skewed rMAT scale 22, 120M generated edges (a=.57 b=.19 c=.19):
-    224,914,926 directed edges, maximum degree 243,995; vertices with edges
-    by tier: 1,685,167 inline only, 989,920 PMA, 35,443 B-tree.  A 1e6-edge
-    batch whose sources are the top-1% degree vertices, inserted and then
-    deleted, leaves the adjacency as it was (round trip); BFS from the
-    largest hub, PageRank 10 iterations and CC match the oracle."""
+@pytest.mark.parametrize("size", [100_000, 100_000, 1_000, 10_000_000])
+def test_plain_mixed_batch(plain, size):
+    """Mixed batches of 1e5 (twice), 1e3 and 1e7 update edges (90% inserts
+    of fresh rMAT edges generated with ith_rand(1000 + t), 10% deletes of
+    existing edges in both directions): the adjacency equals the set model
+    after each."""
+    plain["t"] = t = plain.get("t", -1) + 1
+    say("plain workload: mixed batch", t, size)
+    seed = oracle.aspen_ith_rand(1000 + t) & 0xFFFFFFFF
+    plain["keys"] = mixed_batch(plain["G"], plain["keys"], plain["n"], seed, size, plain["rng"], (0.5, 0.1, 0.1))
+    assert_adjacency(plain["G"], plain["keys"], plain["n"], f"batch {t} of {size}")
+
+
+def test_plain_queries(plain):
+    """BFS from 9, PageRank 10 iterations, CC and BC from 9 on the graph
+    after the batches equal the oracle."""
+    say("plain workload: queries")
+    g = oracle.CSR(plain["keys"], plain["n"])
+    check_queries(plain["G"], g, 9, "plain full size")
+
+
+# ------------------------------------------------------------- skew workload
+@pytest.fixture(scope="module")
+def skew():
+    """Skewed rMAT scale 22, 120M generated edges (a=.57 b=.19 c=.19)."""
     n = 1 << 22
-    say("full size skew workload: build")
+    say("skew workload: build")
     s, d, w, keys = device_graph(n, (0.57, 0.19, 0.19), 120_000_000)
-    assert len(keys) == 224_914_926
     G = tg.Graph(n)
     G.add_edge_batch_dev(s, d, w)
     del s, d
+    st = {"G": G, "keys": keys, "n": n, "rng": np.random.default_rng(9)}
+    yield st
+    G.close()
+
+
+def test_skew_static_graph(skew):
+    """224,914,926 directed edges, maximum degree 243,995; vertices with
+    edges by tier: 1,685,167 inline only, 989,920 PMA, 35,443 B-tree."""
+    G, keys, n = skew["G"], skew["keys"], skew["n"]
+    assert len(keys) == 224_914_926
     deg = G.degrees()
     assert int(deg.max()) == 243_995
     tiers = G.tiers()
     assert int(((tiers == 0) & (deg > 0)).sum()) == 1_685_167
     assert int((tiers == 1).sum()) == 989_920
     assert int((tiers == 2).sum()) == 35_443
-    ref = assert_adjacency(G, keys, n, "static")
-    say("full size skew workload: hub round trip")
-    rng = np.random.default_rng(9)
+    assert_adjacency(G, keys, n, "static")
+
+
+def test_skew_hub_round_trip(skew):
+    """A 1e6-edge batch whose sources are the top-1% degree vertices,
+    inserted and then its new edges deleted, leaves the adjacency as it
+    was."""
+    G, keys, n, rng = skew["G"], skew["keys"], skew["n"], skew["rng"]
+    say("skew workload: hub round trip")
+    deg = G.degrees()
     top = np.argsort(deg)[-(n // 100):]
     active = np.nonzero(deg)[0]
     bs = top[rng.integers(0, len(top), 1_000_000)].astype(np.uint64)
@@ -127,6 +183,38 @@ def test_skew_workload_full_size():
     assert_adjacency(G, oracle.set_union(keys, batch), n, "hub insert")
     G.remove_edge_batch(*split(fresh))
     assert_adjacency(G, keys, n, "hub round trip")
-    say("full size skew workload: queries")
-    check_queries(G, ref, int(np.argmax(deg)), "skew full size")
-    G.close()
+
+
+def test_skew_hub_mixed_batch(skew):
+    """A 1e6-edge mixed batch skewed toward the top-1% degree vertices (90%
+    inserts from them, 10% deletes of existing edges) equals the set
+    model."""
+    G, n, rng = skew["G"], skew["n"], skew["rng"]
+    say("skew workload: hub-targeted mixed batch")
+    deg = G.degrees()
+    top = np.argsort(deg)[-(n // 100):]
+    skew["keys"] = mixed_batch(G, skew["keys"], n, 0, 1_000_000, rng, None, sources=top)
+    assert_adjacency(G, skew["keys"], n, "hub mixed batch")
+
+
+def test_skew_queries(skew):
+    """BFS from the largest hub, PageRank 10 iterations, CC and BC equal the
+    oracle."""
+    say("skew workload: queries")
+    g = oracle.CSR(skew["keys"], skew["n"])
+    check_queries(skew["G"], g, int(np.argmax(g.degrees())), "skew full size")
+
+
+def test_skew_triangle_count(skew):
+    """TC (the degree-ordered device count) and TC.h's edgeMap + countF
+    form both equal the oracle's count_common sum over every edge."""
+    say("skew workload: TC")
+    g = oracle.CSR(skew["keys"], skew["n"])
+    want = oracle.tc(g)
+    assert tg.TC(skew["G"]) == want
+    cnt = tg.DeviceBuffer(8)
+    cnt.upload(np.zeros(1, np.uint64))
+    vs = tg.VertexSubset(None, 0, all=True, max_el=skew["n"])
+    tg.edgeMap(skew["G"], vs, tg.OP_TC, tg.OpArgs(count=cnt.ptr), output=False)
+    assert int(cnt.download(np.uint64, 1)[0]) == want
+    cnt.close()

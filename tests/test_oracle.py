@@ -282,6 +282,79 @@ def test_tier_routing_rule():
     assert t["btree"][0] == 0
 
 
+@pytest.mark.parametrize("name", ["graph_s10", "graph_skew12", "graph_promote11"])
+def test_tier_model_matches_reference_build(name):
+    """The tier model's B-tree membership after the static build and after
+    every batch equals the reference build's own (vertices whose
+    aux_neighbors is set), dumped by oracle/_ref/ref_harness."""
+    z = load(name)
+    n = int(z["n"][0])
+    tiers = oracle.tier_model(n)
+    keys = np.zeros(0, np.uint64)
+    for k in range(int(z["nbatches"][0]) + 1):
+        b = z["base"] if k == 0 else z[f"batch{k - 1}"]
+        oracle.tier_insert(tiers, oracle.CSR(keys, n), b)
+        keys = oracle.set_union(keys, b)
+        assert np.array_equal(tiers["btree"], z[f"btree{k}"]), (name, k)
+        assert np.array_equal(tiers["degree"], z[f"deg{k}"]), (name, k)
+    if name != "graph_s10":
+        assert z[f"btree{int(z['nbatches'][0])}"].any()
+
+
+# --------------------------------------------------------------- CSR builders
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_edges_to_csr_equals_key_sort(symmetric):
+    """The row-wise CSR builder equals the CSR of the sorted unique key
+    set (duplicates, self-loops, an empty row, a hub row)."""
+    rng = np.random.default_rng(21)
+    n = 3000
+    s = np.concatenate([rng.integers(0, n, 50_000), np.full(5000, 7), [5, 5, 9]]).astype(np.uint32)
+    d = np.concatenate([rng.integers(0, n, 50_000), rng.integers(0, n, 5000), [5, 6, 6]]).astype(np.uint32)
+    s[s == 11] = 12  # vertex 11 has no out-edges of its own
+    got = oracle.CSR.from_edges(n, s, d, symmetric=symmetric)
+    want = oracle.CSR(oracle.build_keys(s, d, symmetric=symmetric), n)
+    assert got.m == want.m
+    assert np.array_equal(got.off, want.off)
+    assert np.array_equal(got.nbr[:got.m], want.nbr[:want.m])
+
+
+def test_csr_update_equals_set_model():
+    rng = np.random.default_rng(22)
+    n = 2048
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 5, 0.57, 0.19, 0.19, 20000))
+    g = oracle.CSR(keys, n)
+    ins = oracle.sort_unique((rng.integers(0, n, 4000).astype(np.uint64) << np.uint64(32))
+                             | rng.integers(0, n, 4000).astype(np.uint64))
+    dels = oracle.sort_unique(np.concatenate([keys[rng.integers(0, len(keys), 3000)], ins[:100],
+                                              (np.uint64(n - 1) << np.uint64(32)) + np.arange(5, dtype=np.uint64)]))
+    got = g.updated(ins, dels)
+    want = oracle.CSR(oracle.set_difference(oracle.set_union(keys, ins), dels), n)
+    assert got.m == want.m
+    assert np.array_equal(got.off, want.off)
+    assert np.array_equal(got.nbr[:got.m], want.nbr[:want.m])
+
+
+# --------------------------------------------------------------- local triangle count
+def test_ltc_brute_force_and_sum():
+    """Triangles through v: brute force over pairs of neighbours on random
+    graphs, the sum over all v equals 3 x TC, and K_n gives C(n-1, 2)."""
+    rng = np.random.default_rng(8)
+    for n, m in ((30, 150), (50, 400)):
+        edges = {(int(a), int(b)) for a, b in rng.integers(0, n, (m, 2)) if a != b}
+        g = undirected(n, edges)
+        adj = np.zeros((n, n), bool)
+        for a, b in edges:
+            adj[a, b] = adj[b, a] = True
+        for v in range(n):
+            nb = np.nonzero(adj[v])[0]
+            brute = sum(1 for a, b in itertools.combinations(nb, 2) if adj[a, b])
+            assert oracle.ltc(g, v) == brute, (n, v)
+        assert sum(oracle.ltc(g, v) for v in range(n)) == 3 * oracle.tc(g)
+    k = 9
+    g = undirected(k, [(a, b) for a, b in itertools.combinations(range(k), 2)])
+    assert all(oracle.ltc(g, v) == (k - 1) * (k - 2) // 2 for v in range(k))
+
+
 # --------------------------------------------------------------- planted mistakes
 def test_planted_mistakes_are_caught():
     z = load("graph_skew12")
