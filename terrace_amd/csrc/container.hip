@@ -2739,6 +2739,21 @@ bool Graph::prepare_mixed_async(const uint32_t *is, const uint32_t *id, uint64_t
   return count <= kBucketSortMax && !radix;
 }
 
+void Graph::trim_scratch() noexcept {
+  (void)hipStreamSynchronize(stream_);
+  auto trim = [](auto &b) {
+    if (b.cap > kKeepScratch) b.release();
+  };
+  trim(k0_), trim(k1_), trim(k2_), trim(k3_), trim(kD_), trim(kE_), trim(fD_);
+  trim(f0_), trim(f1_), trim(f2_), trim(f3_), trim(g0_), trim(g2_), trim(g3_);
+  trim(u0_), trim(u1_), trim(u2_), trim(u3_), trim(u4_), trim(kick_), trim(route_), trim(tmp_old_);
+  trim(ops_val_), trim(ops_ins_), trim(ops_del_), trim(sbkeys_), trim(sort_.alt), trim(sort_.alt_vals);
+  trim(wv0_), trim(wv1_), trim(wput_);
+  for (auto &b : s32_) trim(b);
+  for (auto &b : s8_) trim(b);
+  for (auto &b : s64_) trim(b);
+}
+
 // Internal consistency flags raised by kernels during the batch.
 void Graph::check_batch_flags() {
   uint64_t h[kScalars];
@@ -3169,6 +3184,7 @@ void Graph::tree_delete_cow(const uint64_t *keys, uint64_t k) {
 // ------------------------------------------------------------ batches
 void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
   if (count == 0) return;
+  TrimAfter trim_after{*this, count > kKeepScratch};
   if (ws_) throw Error(-1, "weighted graph: insert with weights");
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
@@ -3396,6 +3412,7 @@ void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
 
 void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
   if (count == 0) return;
+  TrimAfter trim_after{*this, count > kKeepScratch};
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
   // sort, unique, the existing keys and their source runs sized by count;
@@ -3486,6 +3503,7 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
                          const uint32_t *dd, uint64_t nd) {
   const uint64_t count = ni + nd;
   if (count == 0) return;
+  TrimAfter trim_after{*this, count > kKeepScratch};
   if (ws_) throw Error(-1, "weighted graph: updates without weights");
   if (count > 0x7FFFFFFFull) throw Error(-1, "mixed batch larger than 2^31 edges");
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
@@ -3704,6 +3722,7 @@ void Graph::enable_weights() {
 
 void Graph::insert_batch_weighted(const uint32_t *src, const uint32_t *dst, const uint32_t *w, uint64_t count) {
   if (count == 0) return;
+  TrimAfter trim_after{*this, count > kKeepScratch};
   if (!ws_) throw Error(-1, "graph is not weighted");
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
@@ -3755,6 +3774,7 @@ void Graph::edge_weights(const uint32_t *src, const uint32_t *dst, uint64_t coun
 // one (both ops are idempotent set updates).
 void Graph::apply_ops(const uint32_t *src, const uint32_t *dst, const uint32_t *ops, uint64_t count) {
   if (count == 0) return;
+  TrimAfter trim_after{*this, count > kKeepScratch};
   if (ws_) throw Error(-1, "weighted graph: coalesced ops carry no weights");
   if (count > 0xFFFFFFFFull) throw Error(-1, "batch larger than 2^32 edges");
   k0_.ensure(count, stream_);

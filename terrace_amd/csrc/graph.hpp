@@ -249,6 +249,20 @@ class Graph {
   DevBuf<uint32_t> sbcnt_, sbcur_, sboff_;
   DevBuf<uint64_t> sbkeys_;
 
+  // Per-batch scratch above kKeepScratch elements is released when a batch
+  // that large ends (a static build of 1.5B edges would otherwise hold tens
+  // of GB of sort and flag buffers for the graph's life); smaller buffers
+  // stay, so steady-state batches allocate nothing.
+  static constexpr uint64_t kKeepScratch = 1ull << 25;
+  void trim_scratch() noexcept;
+  struct TrimAfter {  // releases the scratch when a batch of `count` ends
+    Graph &g;
+    bool on;
+    ~TrimAfter() {
+      if (on) g.trim_scratch();
+    }
+  };
+
   void init_pma(uint64_t nleaves);
   void compute_bounds();
   // true when the bucket sort ran (its overflow bit may then be set)

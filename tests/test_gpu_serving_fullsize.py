@@ -132,6 +132,7 @@ def test_serving_stream_alone_vs_mix(serving):
             assert x["version"] == nu
     say("serving workload: replay one request at a time on each version")
     order = sorted(range(REQUESTS), key=lambda i: (res[i]["version"], 0 if reqs[i][0] == "U" else 1, i))
+    serving.pop("G1").close()  # HBM for the replay's graph
     serving["G2"] = device_graph()
     P = tg.Server(serving["G2"], mode="plain", mem_budget=160 << 30)
     pids = [submit(P, reqs[i], 0.0) for i in order]
