@@ -5,8 +5,8 @@ a=0.5 b=0.1 c=0.1 (divisor 2^32), self-loops dropped, symmetrized and
 deduplicated (135.9M directed edges), built on the device.  A step is one
 mixed update batch of --batch directed edge updates: 90% inserts (fresh
 rMAT edges, both directions) and 10% deletes of edges of the graph (both
-directions), applied as one insert call then one delete call.  All batch
-inputs are generated into HBM before the timed region.
+directions), applied as one mixed call (the inserts, then the deletes).
+All batch inputs are generated into HBM before the timed region.
 
 After the timed steps the run also times BFS, PageRank (10 iterations,
 FP64), CC, insert batches of 10^3 and 10^7 edges, and reports them under
@@ -29,6 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per mixed update step of 1e5 edges at the default workload,
+# summed over the step's kernels from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs scaled by a 1 GiB copy kernel (FETCH_SIZE x 2, WRITE_SIZE
+# x 1); None until that profile exists.
+UPDATE_STEP_TRAFFIC_DEFAULT = None
 # HBM bytes per PageRank iteration (k_prs_pull + k_prs_fix + k_prs_reduce)
 # at the default workload (scale 22, 69M generated edges), from rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy kernel
@@ -77,9 +82,18 @@ def make_batches(tg, G, n, scale_a, batch, count, seed0, rng, off, nbr):
 
 
 def apply_batch(G, bt):
+    """One mixed step: the inserts, then the deletes, in one call
+    (tg_update_batch_dev)."""
     s, d, w, bs, bd, nd = bt
-    G.add_edge_batch_dev(s, d, w)
-    G.remove_edge_batch_dev(bs, bd, nd)
+    G.update_batch_dev(s, d, w, bs, bd, nd)
+
+
+def step_bytes(st, b_ins, b_del):
+    """Algorithmic HBM bytes of one mixed step: inserts 8b + 4b + 16u (read
+    the pairs, write the ids, read and write each source's metadata),
+    deletes 8b + 16u, with u the batch's distinct sources as counted on the
+    device for the step (tg_stats)."""
+    return 12 * b_ins + 16 * st["last_ins_sources"] + 8 * b_del + 16 * st["last_del_sources"]
 
 
 def copy_probe():
@@ -618,8 +632,7 @@ def skew_bench(args):
         batches.append((dev, len(ins_s), len(del_s), (ins_s, ins_d, del_s, del_d)))
     # batch 0 (warm-up) is checked against the host set model
     dev, ni, nd, host = batches[0]
-    G.add_edge_batch_dev(dev[0], dev[1], ni)
-    G.remove_edge_batch_dev(dev[2], dev[3], nd)
+    G.update_batch_dev(dev[0], dev[1], ni, dev[2], dev[3], nd)
     keys = (np.repeat(np.arange(n, dtype=np.uint64), np.diff(off).astype(np.int64)) << np.uint64(32)) | nbr
     del nbr
     ins = np.unique((host[0].astype(np.uint64) << np.uint64(32)) | host[1])
@@ -641,8 +654,7 @@ def skew_bench(args):
     t0 = time.perf_counter()
     tot = 0
     for dev, ni, nd, _ in batches[1:]:
-        G.add_edge_batch_dev(dev[0], dev[1], ni)
-        G.remove_edge_batch_dev(dev[2], dev[3], nd)
+        G.update_batch_dev(dev[0], dev[1], ni, dev[2], dev[3], nd)
         tot += ni + nd
     tg.lib().tg_device_sync()
     t = time.perf_counter() - t0
@@ -654,7 +666,7 @@ def skew_bench(args):
         f"{res['update_edges_per_s']:.3e} edges/s")
     G.close()
     out = {
-        "metric": "inserted edges/s (hub-targeted 1e6 batches, skewed graph)",
+        "metric": "updated edges/s (hub-targeted 1e6 mixed batches, skewed graph)",
         "value": res["update_edges_per_s"],
         "unit": "edges/s",
         "n_gpus": 1,
@@ -764,8 +776,30 @@ def main():
     log(f"updates: {upd} in {t_steps:.4f} s -> {value:.3e} edges/s")
 
     extra = {"static_build_edges_per_s": m_static / t_static, "static_edges": m_static,
-             "insert_share_edges_per_s": ins / t_steps}
-    roofline = None
+             "insert_share_edges_per_s": ins / t_steps, "delete_share_edges_per_s": (upd - ins) / t_steps}
+    # Roofline of the headline work: the mixed update step, timed with HIP
+    # events on the graph's stream over as many further steps as timed
+    # (fresh batches), each step's algorithmic bytes from the sources the
+    # device counted for it.
+    prof = make_batches(tg, G, n, probs, args.batch, args.steps, 3000, rng, off, nbr)
+    G.profile(True)
+    G.profile_reset()
+    step_b = []
+    for bt in prof:
+        apply_batch(G, bt)
+        step_b.append(step_bytes(G.stats(), bt[2], bt[5]))
+    step_ms, step_n = G.profile_get("update_step")
+    G.profile(False)
+    del prof
+    step_avg_ms = step_ms / max(step_n, 1)
+    achieved = (sum(step_b) / len(step_b)) / (step_avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": UPDATE_STEP_TRAFFIC_DEFAULT if default_workload else None,
+                "kernel": "mixed update step (tg_update_batch_dev: sort, existence, tier merges, PMA and "
+                          "B-tree updates; every kernel of the step)",
+                "avg_ms": step_avg_ms, "launches": step_n, "algorithmic_bytes": sum(step_b) / len(step_b),
+                "bytes_formula": "12 b_ins + 16 u_ins + 8 b_del + 16 u_del"}
+    log(f"update step: {step_avg_ms:.4f} ms (events), {roofline['algorithmic_bytes']:.3e} B, frac {roofline['frac']:.2e}")
     if not args.no_extra:
         src = 9
         deg = G.degrees()
@@ -809,7 +843,7 @@ def main():
             pull_ms_, pull_n_ = G.profile_get("prs_pull")
             red_ms_, red_n_ = G.profile_get("prs_reduce")
             fix_ms_, fix_n_ = G.profile_get("prs_fix")
-            roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            extra["pr_roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS,
                         "traffic": PR_ITER_TRAFFIC_DEFAULT if default_workload else None,
                         "kernel": "PageRank iteration: k_prs_pull + k_prs_fix + k_prs_reduce (XCD-sliced pull)",
@@ -823,7 +857,7 @@ def main():
             pma_bytes = 4 * tc_["pma_edges"] + 24 * tc_["pma_vertices"]
             pma_avg = pma_ms / max(pma_n, 1)
             achieved = pma_bytes / (pma_avg * 1e-3) / 1e9
-            roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            extra["pr_roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                         "kernel": "k_pull_pma<PullSum> (PageRank dense pull over PMA-tier vertices)",
                         "avg_ms": pma_avg, "launches": pma_n, "algorithmic_bytes": pma_bytes, "units": tc_}
@@ -905,8 +939,7 @@ def main():
         tg.lib().tg_device_sync()
         t0 = time.perf_counter()
         for hs, hd, xs, xd in hb:
-            G.add_edge_batch(hs, hd)
-            G.remove_edge_batch(xs, xd)
+            G.update_batch(hs, hd, xs, xd)
         tg.lib().tg_device_sync()
         dt = time.perf_counter() - t0
         extra["update_edges_per_s_host_arrays_b1e5"] = sum(len(h[0]) + len(h[2]) for h in hb) / dt
@@ -917,7 +950,7 @@ def main():
         copy_probe()
     cpu = None if args.no_cpu else cpu_baseline(args.quick, args.batch)
     out = {
-        "metric": "inserted edges/s",
+        "metric": f"updated edges/s (mixed batches of {args.batch}: 90% inserts, 10% deletes)",
         "value": value,
         "unit": "edges/s",
         "n_gpus": 1,

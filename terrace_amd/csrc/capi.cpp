@@ -191,7 +191,20 @@ int tg_update_batch_dev(tg_graph *G, const uint32_t *ins_src, const uint32_t *in
   return guard([&] {
     NEED(G && (n_ins == 0 || (ins_src && ins_dst)) && (n_del == 0 || (del_src && del_dst)), "null argument");
     std::lock_guard<std::mutex> l(G->g.mu);
+    if (!G->t.profiling) {
+      G->g.update_batch(ins_src, ins_dst, n_ins, del_src, del_dst, n_del);
+      return;
+    }
+    // profiled: the whole step on the graph's stream, as "update_step"
+    struct Ev {
+      hipEvent_t e = nullptr;
+      Ev() { TG_HIP(hipEventCreate(&e)); }
+      ~Ev() { (void)hipEventDestroy(e); }
+    } a, b;
+    TG_HIP(hipEventRecord(a.e, G->g.stream()));
     G->g.update_batch(ins_src, ins_dst, n_ins, del_src, del_dst, n_del);
+    TG_HIP(hipEventRecord(b.e, G->g.stream()));
+    G->t.record("update_step", a.e, b.e);
   });
 }
 

@@ -223,6 +223,14 @@ __device__ inline uint64_t lower_bound_dev(const T *a, uint64_t n, T x) {
   return lo;
 }
 
+// A count the device computed, bounded by the launch's capacity (null: the
+// capacity itself).
+__device__ inline uint64_t dev_count(const uint64_t *p, uint64_t cap) {
+  if (!p) return cap;
+  const uint64_t v = *p;
+  return v < cap ? v : cap;
+}
+
 #define GRID_STRIDE(i, n)                                                                  \
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < (n); \
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
@@ -597,11 +605,13 @@ __global__ void __launch_bounds__(kBucketBlock) k_bucket_sort(uint64_t *keys, ui
 // workgroup, order kept): PMA keys into P (count sc[0]), B-tree keys into T
 // (sc[1]); and the exclusive prefix of per-run counts (promotions, inline
 // refills) over the runs (total at ppos[nruns] and sc[2]).
-__global__ void __launch_bounds__(1024) k_small_route(const uint64_t *K, const uint8_t *route, uint32_t nk,
+__global__ void __launch_bounds__(1024) k_small_route(const uint64_t *K, const uint8_t *route, uint32_t cap,
                                                       uint64_t *P, uint64_t *T, const uint32_t *promote,
-                                                      uint32_t nruns, uint32_t *ppos, uint64_t *sc) {
+                                                      uint32_t nruns, uint32_t *ppos, uint64_t *sc,
+                                                      const uint64_t *nkdev = nullptr) {
   __shared__ uint32_t s_w[16];
   const uint32_t t = threadIdx.x;
+  const uint32_t nk = static_cast<uint32_t>(dev_count(nkdev, cap));
   uint32_t rt[4], cp = 0, ct = 0;
   uint64_t k[4];
 #pragma unroll
@@ -1103,8 +1113,10 @@ __global__ void k_place_kicked(const uint64_t *R, const uint32_t *ex, const uint
 }
 
 // ------------------------------------------------------------ PMA kernels
-__global__ void k_leaf_of(GView g, const uint64_t *keys, uint64_t k, uint32_t *leaf, uint64_t *fit_flag) {
+__global__ void k_leaf_of(GView g, const uint64_t *keys, uint64_t cap, uint32_t *leaf, uint64_t *fit_flag,
+                          const uint64_t *kdev) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *fit_flag = 0;  // set later by k_pma_leaf_fit
+  const uint64_t k = dev_count(kdev, cap);
   GRID_STRIDE(i, k) {
     const uint64_t key = keys[i];
     leaf[i] = static_cast<uint32_t>(pma_leaf_of(g, key, g.vb[key_src(key)].aux));
@@ -1178,7 +1190,8 @@ __global__ void k_clear_marks(GView g, const uint32_t *T, const uint64_t *ntp, c
 
 // Also closes Tstart (Tstart[nt] = k) for the kernels after it.
 __global__ void k_pma_leaf_fit(GView g, uint64_t up0, const uint32_t *T, uint32_t *Tstart, const uint64_t *ntp,
-                               uint32_t k, uint64_t *flag) {
+                               uint32_t cap, uint64_t *flag, const uint64_t *kdev) {
+  const uint32_t k = static_cast<uint32_t>(dev_count(kdev, cap));
   const uint32_t nt = static_cast<uint32_t>(*ntp);
   if (blockIdx.x == 0 && threadIdx.x == 0) Tstart[nt] = k;
   GRID_STRIDE(t, nt) {
@@ -1411,7 +1424,9 @@ struct SelRoutes {
   const uint64_t *k;
   const uint8_t *route;
   uint64_t *P, *T;
+  const uint64_t *np = nullptr;  // keys past *np are not listed (null: all)
   __device__ uint32_t pick(uint64_t i) const {
+    if (np && i >= *np) return 0u;
     const uint8_t r = route[i];
     return r == kRoutePma ? 1u : (r == kRouteTree ? 2u : 0u);
   }
@@ -1420,7 +1435,8 @@ struct SelRoutes {
 struct SelTouched {
   const uint32_t *leaf;
   uint32_t *T, *Tstart;
-  __device__ bool pred(uint64_t i) const { return i == 0 || leaf[i] != leaf[i - 1]; }
+  const uint64_t *np = nullptr;  // keys past *np are not listed (null: all)
+  __device__ bool pred(uint64_t i) const { return (!np || i < *np) && (i == 0 || leaf[i] != leaf[i - 1]); }
   __device__ void emit(uint64_t i, uint64_t pos) const {
     T[pos] = leaf[i];
     Tstart[pos] = static_cast<uint32_t>(i);
@@ -1617,7 +1633,9 @@ __global__ void __launch_bounds__(256) k_tree_up(uint32_t *pcnt, uint32_t H, uin
 }
 
 // PMA delete: clear the slot, decrement the count tree, flag begin fixes.
-__global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t k, uint64_t *fix_slot, uint64_t *err) {
+__global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t cap, uint64_t *fix_slot, uint64_t *err,
+                             const uint64_t *kdev) {
+  const uint64_t k = dev_count(kdev, cap);
   GRID_STRIDE(i, k) {
     uint32_t s = key_src(keys[i]), d = key_dst(keys[i]);
     const uint64_t begin = g.vb[s].aux;
@@ -1666,7 +1684,9 @@ __device__ inline uint64_t next_begin(const GView &g, uint32_t s, uint64_t p) {
 
 // New begin for vertices whose first PMA element was deleted: the next
 // occupied slot if it still belongs to the vertex.
-__global__ void k_pma_fix_begin(GView g, const uint64_t *keys, uint64_t k, const uint64_t *fix_slot) {
+__global__ void k_pma_fix_begin(GView g, const uint64_t *keys, uint64_t cap, const uint64_t *fix_slot,
+                                const uint64_t *kdev) {
+  const uint64_t k = dev_count(kdev, cap);
   GRID_STRIDE(i, k) {
     uint64_t slot = fix_slot[i];
     if (slot == ~0ull) continue;
@@ -1972,10 +1992,24 @@ __global__ void k_tree_commit(GView g, const TreeRun *tr, uint32_t ntr, const ui
 // position of its block (groups in any order).  *status = 1 when a group
 // does not fit, else 0.
 template <bool kInsert>
-__global__ void __launch_bounds__(1024) k_tree_small_groups(GView g, const uint64_t *keys, uint32_t k, uint32_t *gl,
-                                                            uint64_t *status) {
+__global__ void __launch_bounds__(1024) k_tree_small_groups(GView g, const uint64_t *keys, uint32_t cap, uint32_t *gl,
+                                                            uint64_t *status, const uint64_t *kdev,
+                                                            const uint64_t *gate) {
   __shared__ uint64_t s_gj[Graph::kTreeSmallKeys];
   __shared__ uint32_t s_ng, s_fail;
+  // kdev: the key count computed on the device (null: cap); more than
+  // kTreeSmallKeys keys do not fit (status 1).  gate: when *gate is nonzero
+  // the update must wait for the host (status 2); nothing is written.
+  const uint64_t kk = kdev ? *kdev : cap;
+  const uint32_t refuse = gate && *gate ? 2u : (kk > Graph::kTreeSmallKeys || kk > cap ? 1u : 0u);
+  if (refuse) {
+    if (threadIdx.x == 0) {
+      gl[0] = 0;
+      *status = refuse;
+    }
+    return;
+  }
+  const uint32_t k = static_cast<uint32_t>(kk);
   if (threadIdx.x == 0) {
     s_ng = 0;
     s_fail = 0;
@@ -2877,23 +2911,23 @@ void Graph::pma_insert(const uint64_t *keys, uint64_t k) {
 // dsc_[8]) and the in-leaf merge, which writes nothing unless every touched
 // leaf fits (fit flag in dsc_[7]).  Needs pma_elems_ + k within the root
 // bound.
-void Graph::pma_fast_launch(const uint64_t *keys, uint64_t k) {
+void Graph::pma_fast_launch(const uint64_t *keys, uint64_t k, const uint64_t *kdev) {
   GView g = view();
   DevBuf<uint32_t> &T = f1_, &Tstart = f2_;
   u0_.ensure(k + 1, stream_);  // leaf of key
   T.ensure(k + 1, stream_);
   Tstart.ensure(k + 1, stream_);
   uint64_t *ntp = dsc_.get() + 8, *fitp = dsc_.get() + 7;
-  k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get(), fitp);
+  k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get(), fitp, kdev);
   TG_CHECK_LAUNCH();
-  select(SelTouched{u0_.get(), T.get(), Tstart.get()}, k, ntp, scan_scratch, stream_);
+  select(SelTouched{u0_.get(), T.get(), Tstart.get(), kdev}, k, ntp, scan_scratch, stream_);
   // kLeafLanes lanes per touched leaf (at most min(k, leaves) of them)
   const uint64_t most = k < (1ull << pma_H_) ? k : (1ull << pma_H_);
   const uint64_t wg64 = (most + kMergeGroups - 1) / kMergeGroups;
   if (wg64 > 0x7FFFFFFFull) throw Error(-1, "PMA merge: too many keys");
   const unsigned wg = static_cast<unsigned>(wg64);
   k_pma_leaf_fit<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_.up[0], T.get(), Tstart.get(), ntp,
-                                                         static_cast<uint32_t>(k), fitp);
+                                                         static_cast<uint32_t>(k), fitp, kdev);
   TG_CHECK_LAUNCH();
   k_pma_leaf_merge<<<wg, 256, 0, stream_>>>(g, T.get(), Tstart.get(), ntp, keys, fitp, dsc_.get());
   TG_CHECK_LAUNCH();
@@ -2962,32 +2996,44 @@ void Graph::pma_window_insert(const uint64_t *keys, uint64_t k) {
   stats_.pma_window_leaves += ntasks;
 }
 
-void Graph::pma_delete(const uint64_t *keys, uint64_t k) {
-  if (k == 0) return;
+// Delete up to cap present keys (kdev: their count on the device); the
+// caller lowers pma_elems_ by the count.
+void Graph::pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev) {
   GView g = view();
-  g2_.ensure(k, stream_);
-  k_pma_delete<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g2_.get(), dsc_.get());
+  g2_.ensure(cap + 1, stream_);
+  k_pma_delete<<<grid_for(cap, 256), 256, 0, stream_>>>(g, keys, cap, g2_.get(), dsc_.get(), kdev);
   TG_CHECK_LAUNCH();
   tree_dirty_ = true;
-  k_pma_fix_begin<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, g2_.get());
+  k_pma_fix_begin<<<grid_for(cap, 256), 256, 0, stream_>>>(g, keys, cap, g2_.get(), kdev);
   TG_CHECK_LAUNCH();
+}
+
+void Graph::pma_delete(const uint64_t *keys, uint64_t k) {
+  if (k == 0) return;
+  pma_delete_launch(keys, k, nullptr);
   pma_elems_ -= k;  // every key is present (a missing one raises an error flag)
 }
 
 // ------------------------------------------------------------ tree update
 // Launch the in-place tree update (status in dsc_[13]: 0 = done).
-void Graph::tree_small_launch(const uint64_t *keys, uint64_t k, bool insert) {
+// Up to min(cap, kTreeSmallKeys) keys (kdev: their count on the device;
+// more keys give status 1); the groups kernel lists the blocks to rewrite,
+// the apply kernel's grid covers the most groups cap keys can make.
+void Graph::tree_small_launch(const uint64_t *keys, uint64_t cap, bool insert, const uint64_t *kdev,
+                              const uint64_t *gate, uint64_t *status) {
   GView g = view();
   DevBuf<uint32_t> &gl = s32_[30];
-  gl.ensure(1 + 3 * k, stream_);
-  const uint32_t kk = static_cast<uint32_t>(k);
-  const unsigned grid = static_cast<unsigned>(k < 1024 ? k : 1024);
+  const uint64_t most = cap < kTreeSmallKeys ? cap : kTreeSmallKeys;
+  gl.ensure(1 + 3 * most, stream_);
+  const uint32_t kk = static_cast<uint32_t>(most);
+  const unsigned grid = static_cast<unsigned>(most == 0 ? 1 : (most < 1024 ? most : 1024));
+  uint64_t *st = status ? status : dsc_.get() + 13;
   if (insert) {
-    k_tree_small_groups<true><<<1, 1024, 0, stream_>>>(g, keys, kk, gl.get(), dsc_.get() + 13);
+    k_tree_small_groups<true><<<1, 1024, 0, stream_>>>(g, keys, kk, gl.get(), st, kdev, gate);
     TG_CHECK_LAUNCH();
     k_tree_small_apply<true><<<grid, 256, 0, stream_>>>(g, keys, gl.get());
   } else {
-    k_tree_small_groups<false><<<1, 1024, 0, stream_>>>(g, keys, kk, gl.get(), dsc_.get() + 13);
+    k_tree_small_groups<false><<<1, 1024, 0, stream_>>>(g, keys, kk, gl.get(), st, kdev, gate);
     TG_CHECK_LAUNCH();
     k_tree_small_apply<false><<<grid, 256, 0, stream_>>>(g, keys, gl.get());
   }
@@ -3325,7 +3371,7 @@ void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
                                                          run_c.get(), run_nko.get(), kick_old.get(), k2_.get(),
                                                          kroute.get());
   TG_CHECK_LAUNCH();
-  // split kicked keys by tier and count promotions: one host read
+  // split kicked keys by tier and count promotions (counts stay on the device)
   k3_.ensure(nR + 1, stream_);
   g0_.ensure(nR + 1, stream_);
   if (nR <= kSmallSortKeys) {
@@ -3338,11 +3384,32 @@ void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
     exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_,
                        dsc_.get() + 5);
   }
-  uint64_t h[3];
-  read_scalars(dsc_.get() + 3, h, 3, stream_);
-  uint64_t npma = h[0], ntree = h[1];
-  uint32_t np = static_cast<uint32_t>(h[2]);
+  // The in-place tree update and the in-leaf PMA merge (disjoint data) run
+  // on the device's counts (PMA keys dsc_[3], tree keys [4]) with no host
+  // read before them.  The tree update waits (status 2) while promotions
+  // ([5]) are pending: their trees must exist first.  One read then brings
+  // every count and outcome back; a part that did not fit takes its general
+  // pass.  The PMA merge is launched when every key could go to the PMA
+  // within the root bound (else the general insert decides after the read).
+  const bool pma_launched = pma_elems_ + nR <= bounds_.up[pma_H_];
+  tree_small_launch(g0_.get(), nR, true, dsc_.get() + 4, dsc_.get() + 5);
+  if (pma_launched) pma_fast_launch(k3_.get(), nR, dsc_.get() + 3);
+  uint64_t hs[kScalars];
+  read_scalars(dsc_.get(), hs, kScalars, stream_);
+  const uint64_t npma = hs[3];
+  uint64_t ntree = hs[4];
+  const uint32_t np = static_cast<uint32_t>(hs[5]);
+  bool tree_ok = ntree == 0 || hs[13] == 0;
+  const bool pma_ok = npma == 0 || (pma_launched && hs[7] == 0);
+  if (ntree && tree_ok) stats_.tree_small_updates++;
+  if (npma && pma_ok) pma_fast_done(npma, hs[8]);
+  if (np == 0 && tree_ok && pma_ok) {
+    check_flags(hs);
+    return;
+  }
   // promotions: new trees for vertices routed to the B-tier the first time
+  // (their PMA runs are read after the in-leaf merge, which keeps each run
+  // contiguous and its begin slot exact)
   uint64_t next = 0;
   DevBuf<uint64_t> &ext = s64_[0];
   if (np) {
@@ -3383,29 +3450,15 @@ void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
     while (bits < 32 && (1ull << bits) < n_) bits++;
     radix_sort_keys(g0_.get(), ntree, bits, bits, sort_, stream_);
   }
-  // the in-place tree update and the in-leaf PMA merge (disjoint data) are
-  // launched together and their outcomes read back at once; a part that
-  // did not fit then takes its general pass
-  const bool tree_try = ntree > 0 && ntree <= kTreeSmallKeys;
-  const bool pma_try = npma > 0 && pma_elems_ + npma <= bounds_.up[pma_H_];
-  if (tree_try) tree_small_launch(g0_.get(), ntree, true);
-  if (pma_try) pma_fast_launch(k3_.get(), npma);
-  // one read: the batch's error flags with the fit flag ([7]), touched
-  // leaves ([8]) and the tree update status ([13])
-  uint64_t hs[kScalars];
-  read_scalars(dsc_.get(), hs, kScalars, stream_);
-  const bool tree_ok = ntree == 0 || (tree_try && hs[13] == 0);
-  const bool pma_ok = npma == 0 || (pma_try && hs[7] == 0);
-  if (ntree && tree_ok) stats_.tree_small_updates++;
-  if (npma && pma_ok) pma_fast_done(npma, hs[8]);
-  if (tree_ok && pma_ok) {
-    check_flags(hs);
-    return;
+  if (!tree_ok || next) {
+    // waited for promotions (status 2): the in-place update may fit now;
+    // did not fit in place (status 1): copy-on-write
+    if (np) tree_insert(g0_.get(), ntree);
+    else tree_insert_cow(g0_.get(), ntree);
   }
-  if (!tree_ok) tree_insert_cow(g0_.get(), ntree);
   if (!pma_ok) {
-    if (pma_try) pma_window_insert(k3_.get(), npma);
-    else pma_insert(k3_.get(), npma);  // beyond the root bound: rebuild
+    if (pma_launched) pma_window_insert(k3_.get(), npma);
+    else pma_insert(k3_.get(), npma);  // may be beyond the root bound: the general insert decides
   }
   check_batch_flags();
 }
@@ -3628,85 +3681,73 @@ void Graph::delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns, co
     exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_,
                        dsc_.get() + 6);
   }
-  uint64_t h[3];
-  read_scalars(dsc_.get() + 4, h, 3, stream_);
-  uint64_t npma = h[0], ntree = h[1], nb = h[2];
-  pma_delete(k3_.get(), npma);
-  // the in-place tree delete is launched without waiting for its status:
-  // the status comes back with the refill counts (or the final flags), and
-  // the refill is redone after the copy-on-write pass when it did not fit
-  const bool tree_try = ntree > 0 && ntree <= kTreeSmallKeys;
-  if (tree_try) tree_small_launch(g0_.get(), ntree, false);
-  else tree_delete(g0_.get(), ntree);
-  bool tree_pending = tree_try;  // status not read yet
-  uint64_t hs[kScalars];
-  // refill inline slots from the secondary tier (positional writes: running
-  // it again after a fallback gives the same result)
-  if (nb) {
-    DevBuf<uint64_t> &bkeys = s64_[1], &bpma = s64_[2], &btree = s64_[3];
-    DevBuf<uint8_t> &broute = s8_[6];
-    bkeys.ensure(nb + 1, stream_);
-    broute.ensure(nb + 1, stream_);
-    bpma.ensure(nb + 1, stream_);
-    btree.ensure(nb + 1, stream_);
-    for (int pass = 0; pass < 2; pass++) {
-      g = view();
-      // PMA refills are removed from the PMA by k_bump itself (count in
-      // dsc_[9]); the second pass, after a B-tree fallback, redoes only
-      // the B-tree vertices
-      k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg, nruns, run_bump.get(), bump_off.get(),
-                                                        bkeys.get(), broute.get(), pass == 1, dsc_.get() + 9);
+  // Every stage below runs on the device's counts (PMA deletes dsc_[4],
+  // tree deletes [5], inline refills [6]) with no host read between them:
+  // the PMA deletes, the in-place tree delete (status [13]), the inline
+  // refill from the smallest secondary neighbours (k_bump; it removes the
+  // PMA elements it moves itself and counts them in [9]), the refill's tier
+  // split ([16] PMA keys: none, k_bump routes them nowhere; [17] tree keys)
+  // and the refill's in-place tree delete (status [19]), which waits when
+  // the first tree delete did not fit.  One read then brings every count and
+  // outcome back.
+  DevBuf<uint64_t> &bkeys = s64_[1], &bpma = s64_[2], &btree = s64_[3];
+  DevBuf<uint8_t> &broute = s8_[6];
+  DevBuf<uint32_t> &none = s32_[31];  // no per-run counts for the refill's split
+  pma_delete_launch(k3_.get(), nR, dsc_.get() + 4);
+  tree_small_launch(g0_.get(), nR, false, dsc_.get() + 5, nullptr, dsc_.get() + 13);
+  auto refill = [&](bool trees_only) {
+    g = view();
+    k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg, nruns, run_bump.get(), bump_off.get(),
+                                                      bkeys.get(), broute.get(), trees_only, dsc_.get() + 9);
+    TG_CHECK_LAUNCH();
+    tree_dirty_ = true;
+    if (nR <= kSmallSortKeys) {
+      k_small_route<<<1, 1024, 0, stream_>>>(bkeys.get(), broute.get(), static_cast<uint32_t>(nR), bpma.get(),
+                                             btree.get(), nullptr, 0, none.get(), dsc_.get() + 16, dsc_.get() + 6);
       TG_CHECK_LAUNCH();
-      tree_dirty_ = true;
-      if (nb <= kSmallSortKeys) {
-        DevBuf<uint32_t> &none = s32_[31];  // no per-run counts here
-        none.ensure(1, stream_);
-        k_small_route<<<1, 1024, 0, stream_>>>(bkeys.get(), broute.get(), static_cast<uint32_t>(nb), bpma.get(),
-                                               btree.get(), nullptr, 0, none.get(), dsc_.get() + 4);
-        TG_CHECK_LAUNCH();
-      } else {
-        select2(SelRoutes{bkeys.get(), broute.get(), bpma.get(), btree.get()}, nb, dsc_.get() + 4, dsc_.get() + 5,
-                scan_scratch, stream_);
-      }
-      read_scalars(dsc_.get(), hs, kScalars, stream_);
-      if (!tree_pending) break;
-      tree_pending = false;
-      if (hs[13] == 0) {
-        stats_.tree_small_updates++;
-        break;
-      }
-      tree_delete_cow(g0_.get(), ntree);  // then the refill again
+    } else {
+      select2(SelRoutes{bkeys.get(), broute.get(), bpma.get(), btree.get(), dsc_.get() + 6}, nR, dsc_.get() + 16,
+              dsc_.get() + 17, scan_scratch, stream_);
     }
-    pma_elems_ -= hs[9];          // PMA elements the refill moved inline
-    pma_delete(bpma.get(), hs[4]);  // none: k_bump removed them
-    const uint64_t nbt = hs[5];
-    const bool bt_try = nbt > 0 && nbt <= kTreeSmallKeys;
-    if (bt_try) tree_small_launch(btree.get(), nbt, false);
-    else if (nbt) tree_delete(btree.get(), nbt);
-    // the batch's flags (and the tree status); the refill's read already
-    // holds them when nothing was launched after it
-    if (hs[4] || nbt) read_scalars(dsc_.get(), hs, kScalars, stream_);
-    if (bt_try) {
-      if (hs[13] == 0) {
-        stats_.tree_small_updates++;
-      } else {
-        tree_delete_cow(btree.get(), nbt);
-        read_scalars(dsc_.get(), hs, kScalars, stream_);
-      }
+  };
+  refill(false);
+  tree_small_launch(btree.get(), nR, false, dsc_.get() + 17, dsc_.get() + 13, dsc_.get() + 19);
+  uint64_t hs[kScalars];
+  read_scalars(dsc_.get(), hs, kScalars, stream_);
+  const uint64_t npma = hs[4], ntree = hs[5], nb = hs[6];
+  pma_elems_ -= npma + hs[9];  // deleted PMA keys, PMA elements the refill moved inline
+  bool fallback = false;
+  if (ntree && hs[13] == 0) stats_.tree_small_updates++;
+  if (ntree && hs[13] != 0) {
+    // the in-place tree delete did not fit: copy-on-write, then the refill
+    // of the B-tree vertices again (its writes are positional, so running
+    // it again gives the same result) and the delete of its tree keys
+    fallback = true;
+    tree_delete_cow(g0_.get(), ntree);
+    if (nb) {
+      fill_u64(dsc_.get() + 9, 0, 1, stream_);  // a trees-only refill moves no PMA element
+      refill(true);
+      uint64_t h2[2];
+      read_scalars(dsc_.get() + 16, h2, 2, stream_);
+      pma_delete(bpma.get(), h2[0]);
+      tree_delete(btree.get(), h2[1]);
     }
-  } else {
-    read_scalars(dsc_.get(), hs, kScalars, stream_);
-    if (tree_pending) {
-      if (hs[13] == 0) {
-        stats_.tree_small_updates++;
-      } else {
-        tree_delete_cow(g0_.get(), ntree);
-        read_scalars(dsc_.get(), hs, kScalars, stream_);
-      }
+  } else if (hs[17]) {
+    if (hs[19] == 0) {
+      stats_.tree_small_updates++;
+    } else {
+      fallback = true;
+      tree_delete_cow(btree.get(), hs[17]);
     }
+  }
+  if (hs[16]) {  // refilled PMA keys are removed by k_bump itself; kept for safety
+    fallback = true;
+    pma_delete(bpma.get(), hs[16]);
   }
   if (ws_) {
     ws_->erase(R, nR, stream_);  // R holds exactly the deleted (existing) keys
+    check_batch_flags();
+  } else if (fallback) {
     check_batch_flags();
   } else {
     check_flags(hs);

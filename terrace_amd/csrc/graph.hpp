@@ -230,8 +230,10 @@ class Graph {
   DevBuf<uint8_t> route_;
   DevBuf<uint64_t> scal_;  // small device scalars
   // batch scalars: [0] error bits, [1..] counts read back in one copy,
-  // [7] PMA leaf-fit flag, [13] status of the in-place tree update
-  static constexpr int kScalars = 16;
+  // [7] PMA leaf-fit flag, [13] status of the in-place tree update,
+  // [16..18] the inline refill's tier split, [19] status of the refill's
+  // in-place tree delete
+  static constexpr int kScalars = 24;
   static constexpr int kErrSort = 14, kErrScan = 15;  // look-back error words (sort, scans/selects)
   DevBuf<uint64_t> dsc_;
   DevBuf<uint64_t> cc_err_, cc_off_;  // count_common: id check; CSR offsets of the TC edgeMap
@@ -291,7 +293,9 @@ class Graph {
   // PMA inserts: pma_insert = in-leaf merge when every touched leaf fits,
   // else the window pass (or a rebuild beyond the root bound)
   void pma_insert(const uint64_t *keys, uint64_t k);
-  void pma_fast_launch(const uint64_t *keys, uint64_t k);
+  // kdev: the key count on the device, at most cap (null: cap keys)
+  void pma_fast_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev = nullptr);
+  void pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev);
   void pma_fast_done(uint64_t k, uint64_t nt);
   void pma_window_insert(const uint64_t *keys, uint64_t k);
   void pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total);
@@ -302,7 +306,9 @@ class Graph {
   // in-place update of up to 4096 tree keys when every touched block keeps
   // between 1 and 256 keys; false (nothing written) otherwise
   bool tree_small(const uint64_t *keys, uint64_t k, bool insert);
-  void tree_small_launch(const uint64_t *keys, uint64_t k, bool insert);
+  // status (null: dsc_[13]): 0 done, 1 did not fit, 2 gate set
+  void tree_small_launch(const uint64_t *keys, uint64_t cap, bool insert, const uint64_t *kdev = nullptr,
+                         const uint64_t *gate = nullptr, uint64_t *status = nullptr);
   void tree_insert_cow(const uint64_t *keys, uint64_t k);
   void tree_delete_cow(const uint64_t *keys, uint64_t k);
   void ensure_blocks(uint64_t need);
