@@ -350,8 +350,16 @@ int tg_profile_get(tg_graph *g, const char *name, double *ms, uint64_t *launches
  * [8] PMA inserts merged inside their leaves (no window redistribution),
  * [9] elements held in the PMA; the last update batch: [10] unique
  * inserted edges (present ones included), [11] their distinct sources,
- * [12] deleted edges that existed, [13] their distinct sources */
-int tg_stats(tg_graph *g, uint64_t *out /* [14] */);
+ * [12] deleted edges that existed, [13] their distinct sources;
+ * [14] underfull PMA windows respread after deletes, [15] PMA arrays shrunk
+ * after deletes */
+int tg_stats(tg_graph *g, uint64_t *out /* [16] */);
+/* Element count of every PMA leaf (64 slots each), the per-leaf counts
+ * that the density bounds of src/PMA.cc:382-394 are checked on (the
+ * reference's check_no_full_leaves, PMA.cc, is the same kind of
+ * diagnostic).  Writes min(cap, leaves) counts; *leaves gets the number of
+ * leaves. */
+int tg_pma_leaf_counts(tg_graph *g, uint32_t *out, uint64_t cap, uint64_t *leaves);
 /* Failure injection for tests of the error paths; no counterpart in the
  * reference.  what = 1: the next update batch's preparation reports a
  * look-back error (TG_EINTERNAL, graph unchanged and still valid); what = 2:

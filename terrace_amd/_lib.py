@@ -153,6 +153,7 @@ _SIGS = {
     "tg_profile_get": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_uint64)]),
     "tg_stats": (ctypes.c_int, [_P, _P]),
+    "tg_pma_leaf_counts": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P]),
     "tg_tier_counts": (ctypes.c_int, [_P, _P]),
     "tg_debug_fail_next": (ctypes.c_int, [_P, ctypes.c_int]),
     "tg_count_common": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64, _P]),

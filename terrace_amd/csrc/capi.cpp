@@ -1132,6 +1132,15 @@ int tg_stats(tg_graph *G, uint64_t *out) {
     out[11] = s.last_ins_sources;
     out[12] = s.last_del_keys;
     out[13] = s.last_del_sources;
+    out[14] = s.pma_low_windows;
+    out[15] = s.pma_shrinks;
+  });
+}
+
+int tg_pma_leaf_counts(tg_graph *G, uint32_t *out, uint64_t cap, uint64_t *leaves) {
+  return guard([&] {
+    NEED(G && leaves && (out || cap == 0), "null argument");
+    *leaves = G->g.pma_leaf_counts(out, cap);
   });
 }
 

@@ -1162,6 +1162,27 @@ __global__ void k_cover(GView g, const uint32_t *T, const uint64_t *ntp, const u
   }
 }
 
+// Deletes: the window of each underfull leaf T[t] is its lowest ancestor
+// whose count is above that level's lower bound (the climb of
+// src/PMA.cc:1646-1712); the root when none is (an array already at its
+// smallest size).
+__global__ void k_climb_low(GView g, Bounds bd, const uint32_t *T, const uint64_t *ntp, uint8_t *mark,
+                            uint32_t *chosen) {
+  const uint32_t nt = static_cast<uint32_t>(*ntp);
+  GRID_STRIDE(t, nt) {
+    const uint32_t f = T[t];
+    uint32_t L = g.H;
+    for (uint32_t l = 1; l < g.H; l++) {
+      if (g.pcnt[tree_level_off(g.H, l) + (f >> l)] > bd.lo[l]) {
+        L = l;
+        break;
+      }
+    }
+    mark[tree_level_off(g.H, L) + (f >> L)] = 1;
+    chosen[t] = L;
+  }
+}
+
 // Window record: lo leaf, level, key range, old element count.
 struct Win {
   uint64_t lo;
@@ -1442,6 +1463,17 @@ struct SelTouched {
     Tstart[pos] = static_cast<uint32_t>(i);
   }
 };
+// leaves holding at most lo0 elements (start offsets all 0: no new keys)
+struct SelLowLeaf {
+  const uint32_t *pcnt;
+  uint32_t lo0;
+  uint32_t *T, *Tstart;
+  __device__ bool pred(uint64_t i) const { return pcnt[i] <= lo0; }
+  __device__ void emit(uint64_t i, uint64_t pos) const {
+    T[pos] = static_cast<uint32_t>(i);
+    Tstart[pos] = 0;
+  }
+};
 // last op of each key (keys sorted, arrival order kept inside a key)
 struct SelLastOp {
   const uint64_t *k;
@@ -1633,8 +1665,11 @@ __global__ void __launch_bounds__(256) k_tree_up(uint32_t *pcnt, uint32_t H, uin
 }
 
 // PMA delete: clear the slot, decrement the count tree, flag begin fixes.
+// A leaf left at or below the leaf lower bound lo0 raises *low (the batch
+// then respreads the underfull windows, as src/PMA.cc:1636-1746 does per
+// removal).
 __global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t cap, uint64_t *fix_slot, uint64_t *err,
-                             const uint64_t *kdev) {
+                             const uint64_t *kdev, uint32_t lo0, uint64_t *low) {
   const uint64_t k = dev_count(kdev, cap);
   GRID_STRIDE(i, k) {
     uint32_t s = key_src(keys[i]), d = key_dst(keys[i]);
@@ -1647,7 +1682,7 @@ __global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t cap, uint64
     }
     g.pdst[slot] = kEmpty;
     uint64_t leaf = slot >> kLeafShift;
-    atomicSub(&g.pcnt[leaf], 1u);
+    if (atomicSub(&g.pcnt[leaf], 1u) - 1u <= lo0) atomicOr(reinterpret_cast<unsigned long long *>(low), 1ull);
     if (begin == slot) fix_slot[i] = slot;
   }
 }
@@ -2127,8 +2162,11 @@ __global__ void __launch_bounds__(256) k_tree_small_apply(GView g, const uint64_
 // deletes, and record how many inline slots must be refilled.
 // *pma_taken (k_bump's counter of PMA elements moved inline) is cleared here.
 __global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_beg, uint32_t nruns,
-                              uint8_t *sec_route, uint32_t *run_bump, uint64_t *pma_taken) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *pma_taken = 0;
+                              uint8_t *sec_route, uint32_t *run_bump, uint64_t *pma_taken, uint64_t *low) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *pma_taken = 0;
+    *low = 0;
+  }
   GRID_STRIDE(r, nruns) {
     uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
     uint32_t s = key_src(R[b0]);
@@ -2181,7 +2219,8 @@ __global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_be
 // fallback; their refill is already done).
 __global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, uint32_t nruns,
                        const uint32_t *run_bump, const uint32_t *bump_off, uint64_t *bump_keys,
-                       uint8_t *bump_route, bool trees_only, uint64_t *pma_taken) {
+                       uint8_t *bump_route, bool trees_only, uint64_t *pma_taken, uint32_t lo0,
+                       uint64_t *low) {
   GRID_STRIDE(r, nruns) {
     uint32_t y = run_bump[r];
     if (y == 0) continue;
@@ -2217,7 +2256,8 @@ __global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, uint
           bump_keys[out + got] = make_key(s, d);
           bump_route[out + got] = kRouteNone;  // removed here
           g.pdst[p] = kEmpty;
-          atomicSub(&g.pcnt[p >> kLeafShift], 1u);
+          if (atomicSub(&g.pcnt[p >> kLeafShift], 1u) - 1u <= lo0)
+            atomicOr(reinterpret_cast<unsigned long long *>(low), 1ull);
           got++;
         }
         p++;
@@ -2587,7 +2627,9 @@ void Graph::compute_bounds() {
     double lo = 0.25 - 0.125 * depth / H;
     double slots = static_cast<double>(kLeafSlots) * std::ldexp(1.0, static_cast<int>(l));
     bounds_.up[l] = static_cast<uint64_t>(std::floor(up * slots));
-    bounds_.lo[l] = static_cast<uint64_t>(std::ceil(lo * slots));
+    // a window is underfull when count / slots <= lower (the test of
+    // src/PMA.cc:1646), that is count <= floor(lower * slots)
+    bounds_.lo[l] = static_cast<uint64_t>(std::floor(lo * slots));
   }
 }
 
@@ -2831,6 +2873,16 @@ void Graph::check_flags(const uint64_t *h) {
 }
 
 // ------------------------------------------------------------ PMA insert
+uint64_t Graph::pma_leaf_counts(uint32_t *host, uint64_t cap) {
+  const uint64_t nl = 1ull << pma_H_;
+  const uint64_t n = cap < nl ? cap : nl;
+  if (n) {
+    TG_HIP(hipMemcpyAsync(host, pcnt_.get(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    TG_HIP(hipStreamSynchronize(stream_));
+  }
+  return nl;
+}
+
 void Graph::rebuild_count_tree() {
   if (!tree_dirty_) return;
   tree_dirty_ = false;
@@ -2943,32 +2995,43 @@ void Graph::pma_fast_done(uint64_t k, uint64_t nt) {
 
 // The window pass over the touched leaves pma_fast_launch found (a leaf did
 // not fit): density windows, gather and redistribution.
-void Graph::pma_window_insert(const uint64_t *keys, uint64_t k) {
+void Graph::pma_window_insert(const uint64_t *keys, uint64_t k) { pma_respread(keys, k, k, false); }
+
+// Density windows over the leaves T (f1_; their count in dsc_[8], at most
+// cap), each window's share of the sorted new keys from Tstart (f2_), then
+// gather and even redistribution of old and new elements.  Inserts climb to
+// the lowest window within its upper bound; after deletes (low, no keys)
+// the underfull leaves climb to the lowest window above its lower bound.
+void Graph::pma_respread(const uint64_t *keys, uint64_t k, uint64_t cap, bool low) {
   rebuild_count_tree();
   GView g = view();
   uint64_t nl = 1ull << pma_H_;
   const uint64_t root_old = pma_elems_;
   DevBuf<uint32_t> &T = f1_, &Tstart = f2_, &chosen = f3_;
-  u3_.ensure(k + 1, stream_);  // cover, then task -> window map
-  chosen.ensure(k + 1, stream_);
+  u3_.ensure(cap + 1, stream_);  // cover, then task -> window map
+  chosen.ensure(cap + 1, stream_);
   uint64_t *ntp = dsc_.get() + 8, *nwp = dsc_.get() + 9;
-  k_climb<<<grid_for(k, 256), 256, 0, stream_>>>(g, bounds_, T.get(), Tstart.get(), ntp, mark_.get(), chosen.get());
+  if (low)
+    k_climb_low<<<grid_for(cap, 256), 256, 0, stream_>>>(g, bounds_, T.get(), ntp, mark_.get(), chosen.get());
+  else
+    k_climb<<<grid_for(cap, 256), 256, 0, stream_>>>(g, bounds_, T.get(), Tstart.get(), ntp, mark_.get(),
+                                                     chosen.get());
   TG_CHECK_LAUNCH();
-  k_cover<<<grid_for(k, 256), 256, 0, stream_>>>(g, T.get(), ntp, mark_.get(), chosen.get(), u3_.get());
+  k_cover<<<grid_for(cap, 256), 256, 0, stream_>>>(g, T.get(), ntp, mark_.get(), chosen.get(), u3_.get());
   TG_CHECK_LAUNCH();
   DevBuf<uint8_t> &winbuf = route_;
-  winbuf.ensure((k + 1) * sizeof(Win) + 64, stream_);
+  winbuf.ensure((cap + 1) * sizeof(Win) + 64, stream_);
   Win *win = reinterpret_cast<Win *>(winbuf.get());
-  kick_.ensure(k + 2, stream_);  // window leaf counts -> task_base (zeros past nw)
-  fill_u32(kick_.get(), 0, k + 1, stream_);
-  select(SelOwner{g, T.get(), Tstart.get(), u3_.get(), ntp, win, kick_.get()}, k, nwp, scan_scratch, stream_);
-  k_clear_marks<<<grid_for(k, 256), 256, 0, stream_>>>(g, T.get(), ntp, chosen.get(), mark_.get());
+  kick_.ensure(cap + 2, stream_);  // window leaf counts -> task_base (zeros past nw)
+  fill_u32(kick_.get(), 0, cap + 1, stream_);
+  select(SelOwner{g, T.get(), Tstart.get(), u3_.get(), ntp, win, kick_.get()}, cap, nwp, scan_scratch, stream_);
+  k_clear_marks<<<grid_for(cap, 256), 256, 0, stream_>>>(g, T.get(), ntp, chosen.get(), mark_.get());
   TG_CHECK_LAUNCH();
   uint32_t *task_base = kick_.get();
-  // scanning k entries (zeros past nw) keeps nw on the device: task_base[w]
-  // for w >= nw equals the task total, so window searches over k are exact
-  exclusive_scan_u32(task_base, task_base, k, task_base + k, scan_scratch, stream_);
-  k_u32_to_u64<<<1, 1, 0, stream_>>>(task_base + k, dsc_.get() + 10);
+  // scanning cap entries (zeros past nw) keeps nw on the device: task_base[w]
+  // for w >= nw equals the task total, so window searches over cap are exact
+  exclusive_scan_u32(task_base, task_base, cap, task_base + cap, scan_scratch, stream_);
+  k_u32_to_u64<<<1, 1, 0, stream_>>>(task_base + cap, dsc_.get() + 10);
   TG_CHECK_LAUNCH();
   uint64_t h[3];
   read_scalars(dsc_.get() + 8, h, 3, stream_);  // nt, nw, ntasks
@@ -2994,6 +3057,49 @@ void Graph::pma_window_insert(const uint64_t *keys, uint64_t k) {
   pma_elems_ += k;
   stats_.pma_windows += nw;
   stats_.pma_window_leaves += ntasks;
+  if (low) stats_.pma_low_windows += nw;
+}
+
+// After deletes left a leaf at or below the leaf lower bound: an array whose
+// root is at or below its own lower bound shrinks (half_list,
+// src/PMA.cc:722-813; here straight to the size a rebuild gives the count,
+// density between 1/4 and 1/2, never below two leaves); otherwise every
+// underfull leaf's window is respread.  The gather checks each leaf's
+// count against its slots, as on inserts.
+void Graph::pma_rebalance_low() {
+  const uint64_t nl = 1ull << pma_H_;
+  if (pma_elems_ == 0) {
+    if (nl > 2) {
+      pdst_.release();
+      psrc_.release();
+      pcnt_.release();
+      fence_.release();
+      mark_.release();
+      init_pma(2);
+      tree_dirty_ = false;  // a fresh array's counts are all zero
+      stats_.pma_shrinks++;
+    }
+    return;
+  }
+  if (nl > 2 && pma_elems_ <= bounds_.lo[pma_H_]) {
+    rebuild_count_tree();
+    pma_rebuild(nullptr, 0, pma_elems_);
+    stats_.pma_shrinks++;
+    return;
+  }
+  rebuild_count_tree();
+  DevBuf<uint32_t> &T = f1_, &Tstart = f2_;
+  T.ensure(nl + 1, stream_);
+  Tstart.ensure(nl + 1, stream_);
+  fill_u32(Tstart.get(), 0, nl + 1, stream_);
+  select(SelLowLeaf{pcnt_.get(), static_cast<uint32_t>(bounds_.lo[0]), T.get(), Tstart.get()}, nl,
+         dsc_.get() + 8, scan_scratch, stream_);
+  const uint64_t nt = read_scalar(dsc_.get() + 8, stream_);
+  if (nt == 0) return;
+  // no new keys: k_fill reads nothing from the key array when a window's key
+  // range is empty, any valid pointer serves
+  tmp_old_.ensure(pma_elems_ + 1, stream_);
+  pma_respread(tmp_old_.get(), 0, nt, true);
 }
 
 // Delete up to cap present keys (kdev: their count on the device); the
@@ -3001,7 +3107,9 @@ void Graph::pma_window_insert(const uint64_t *keys, uint64_t k) {
 void Graph::pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev) {
   GView g = view();
   g2_.ensure(cap + 1, stream_);
-  k_pma_delete<<<grid_for(cap, 256), 256, 0, stream_>>>(g, keys, cap, g2_.get(), dsc_.get(), kdev);
+  k_pma_delete<<<grid_for(cap, 256), 256, 0, stream_>>>(g, keys, cap, g2_.get(), dsc_.get(), kdev,
+                                                         static_cast<uint32_t>(bounds_.lo[0]),
+                                                         dsc_.get() + kLowFlag);
   TG_CHECK_LAUNCH();
   tree_dirty_ = true;
   k_pma_fix_begin<<<grid_for(cap, 256), 256, 0, stream_>>>(g, keys, cap, g2_.get(), kdev);
@@ -3433,6 +3541,7 @@ void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
       k_extract_pma<<<grid_for(static_cast<uint64_t>(np) * 64, 256), 256, 0, stream_>>>(g, pv.get(), pcount.get(),
                                                                                        pext.get(), np, ext.get());
       TG_CHECK_LAUNCH();
+      fill_u64(dsc_.get() + kLowFlag, 0, 1, stream_);
       pma_delete(ext.get(), next);
     }
     ensure_trees(ntrees_ + np);
@@ -3461,6 +3570,8 @@ void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
     else pma_insert(k3_.get(), npma);  // may be beyond the root bound: the general insert decides
   }
   check_batch_flags();
+  // the promoted runs left the PMA: respread what they left underfull
+  if (next && read_scalar(dsc_.get() + kLowFlag, stream_)) pma_rebalance_low();
 }
 
 void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
@@ -3662,7 +3773,7 @@ void Graph::delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns, co
   bump_off.ensure(nruns + 1, stream_);
   sec_route.ensure(nR + 1, stream_);
   k_delete_runs<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg, nruns, sec_route.get(), run_bump.get(),
-                                                           dsc_.get() + 9);
+                                                           dsc_.get() + 9, dsc_.get() + kLowFlag);
   TG_CHECK_LAUNCH();
   if (debug_fail & 2u) {
     debug_fail &= ~2u;
@@ -3698,7 +3809,8 @@ void Graph::delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns, co
   auto refill = [&](bool trees_only) {
     g = view();
     k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg, nruns, run_bump.get(), bump_off.get(),
-                                                      bkeys.get(), broute.get(), trees_only, dsc_.get() + 9);
+                                                      bkeys.get(), broute.get(), trees_only, dsc_.get() + 9,
+                                                      static_cast<uint32_t>(bounds_.lo[0]), dsc_.get() + kLowFlag);
     TG_CHECK_LAUNCH();
     tree_dirty_ = true;
     if (nR <= kSmallSortKeys) {
@@ -3752,6 +3864,9 @@ void Graph::delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns, co
   } else {
     check_flags(hs);
   }
+  // the flag word is only set after k_delete_runs cleared it: the fallback's
+  // deletes add to it
+  if (fallback ? read_scalar(dsc_.get() + kLowFlag, stream_) != 0 : hs[kLowFlag] != 0) pma_rebalance_low();
 }
 
 // ------------------------------------------------------------ weights

@@ -57,6 +57,8 @@ struct Stats {
   uint64_t promotions = 0;
   uint64_t tree_small_updates = 0;  // tree updates done in place
   uint64_t pma_leaf_merges = 0;     // PMA inserts merged inside their leaves
+  uint64_t pma_low_windows = 0;     // underfull PMA windows respread after deletes
+  uint64_t pma_shrinks = 0;         // PMA arrays shrunk after deletes
   // the last batch: unique inserted keys and their sources, existing
   // deleted keys and their sources
   uint64_t last_ins_keys = 0, last_ins_sources = 0, last_del_keys = 0, last_del_sources = 0;
@@ -126,6 +128,8 @@ class Graph {
   uint32_t num_trees() const { return ntrees_; }
   uint64_t pma_slots() const { return pma_N_; }
   uint64_t pma_elements() const { return pma_elems_; }
+  // leaf counts into host memory (at most cap); returns the number of leaves
+  uint64_t pma_leaf_counts(uint32_t *host, uint64_t cap);
   const Stats &stats() const { return stats_; }
   // True once a batch failed after it had begun changing the graph (an
   // allocation for the graph's own growth, or an internal check): the
@@ -232,9 +236,11 @@ class Graph {
   // batch scalars: [0] error bits, [1..] counts read back in one copy,
   // [7] PMA leaf-fit flag, [13] status of the in-place tree update,
   // [16..18] the inline refill's tier split, [19] status of the refill's
-  // in-place tree delete
+  // in-place tree delete, [20] a PMA delete left a leaf at or below the
+  // leaf lower bound
   static constexpr int kScalars = 24;
   static constexpr int kErrSort = 14, kErrScan = 15;  // look-back error words (sort, scans/selects)
+  static constexpr int kLowFlag = 20;
   DevBuf<uint64_t> dsc_;
   DevBuf<uint64_t> cc_err_, cc_off_;  // count_common: id check; CSR offsets of the TC edgeMap
   DevBuf<uint32_t> cc_nbr_;
@@ -298,6 +304,8 @@ class Graph {
   void pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev);
   void pma_fast_done(uint64_t k, uint64_t nt);
   void pma_window_insert(const uint64_t *keys, uint64_t k);
+  void pma_respread(const uint64_t *keys, uint64_t k, uint64_t cap, bool low);
+  void pma_rebalance_low();
   void pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total);
   void rebuild_count_tree();
   void pma_delete(const uint64_t *keys, uint64_t k);

@@ -199,12 +199,21 @@ class Graph:
         return out
 
     def stats(self) -> dict:
-        out = np.zeros(14, np.uint64)
+        out = np.zeros(16, np.uint64)
         check(lib().tg_stats(self._h, ptr(out)))
         keys = ["pma_rebuilds", "pma_windows", "pma_window_leaves", "tree_blocks_written",
                 "promotions", "trees", "pma_slots", "tree_small_updates", "pma_leaf_merges",
-                "pma_elements", "last_ins_keys", "last_ins_sources", "last_del_keys", "last_del_sources"]
+                "pma_elements", "last_ins_keys", "last_ins_sources", "last_del_keys", "last_del_sources",
+                "pma_low_windows", "pma_shrinks"]
         return {k: int(v) for k, v in zip(keys, out)}
+
+    def pma_leaf_counts(self) -> np.ndarray:
+        """Element count of every PMA leaf (64 slots each)."""
+        nl = ctypes.c_uint64()
+        check(lib().tg_pma_leaf_counts(self._h, None, 0, ctypes.byref(nl)))
+        out = np.zeros(nl.value, np.uint32)
+        check(lib().tg_pma_leaf_counts(self._h, ptr(out), nl.value, ctypes.byref(nl)))
+        return out
 
     def apply_ops(self, src, dst, ops) -> None:
         """Coalesced updates: ops[i] 0 = insert, 1 = delete (src[i], dst[i]),
