@@ -28,42 +28,71 @@ constexpr uint64_t kInvalidKey = ~0ull;
 constexpr uint8_t kRouteNone = 0, kRoutePma = 1, kRouteTree = 2;
 
 // ------------------------------------------------------------ device search
-__device__ inline uint64_t pma_find_leaf(const GView &g, uint64_t key) {
-  uint64_t lo = 0, hi = 1ull << g.H;
-  while (lo < hi) {
-    uint64_t mid = (lo + hi) >> 1;
-    if (g.fence[mid] <= key) lo = mid + 1;
-    else hi = mid;
+// The searches below are chains of dependent HBM reads, so each step reads
+// 15 or 16 fences (or index keys) at once and narrows the range 16-fold:
+// one round trip per 16x instead of one per 2x.
+
+// Last leaf in [lo, hi) whose fence <= key, given fence[lo] <= key (or
+// lo == 0) and fence[hi] > key (or hi == the leaf count).  Fences never
+// decrease.
+__device__ inline uint64_t fence_narrow(const GView &g, uint64_t key, uint64_t lo, uint64_t hi) {
+  while (hi - lo > 1) {
+    const uint64_t step = (hi - lo + 15) >> 4;
+    uint64_t f[15];
+#pragma unroll
+    for (uint32_t x = 1; x < 16; x++) {
+      const uint64_t i = lo + x * step;
+      f[x - 1] = i < hi ? g.fence[i] : ~0ull;
+    }
+    uint64_t nlo = lo;
+#pragma unroll
+    for (uint32_t x = 1; x < 16; x++)
+      if (lo + x * step < hi && f[x - 1] <= key) nlo = lo + x * step;
+    hi = nlo + step < hi ? nlo + step : hi;
+    lo = nlo;
   }
-  return lo == 0 ? 0 : lo - 1;
+  return lo;
+}
+
+// Leaf holding key: the last leaf whose fence <= key (leaf 0 when none).
+__device__ inline uint64_t pma_find_leaf(const GView &g, uint64_t key) { return fence_narrow(g, key, 0, 1ull << g.H); }
+
+// From leaf lo with fence[lo] <= key: fences lo+1 .. lo+16 at once, then
+// lo+16, lo+32, .. lo+256 and so on (a vertex's run spans a few leaves, so
+// one round trip usually brackets the key), then the narrowing above.
+__device__ inline uint64_t fence_gallop(const GView &g, uint64_t key, uint64_t lo) {
+  const uint64_t nl = 1ull << g.H;
+  uint64_t step = 1, hi = nl;
+  while (true) {
+    uint64_t f[16];
+#pragma unroll
+    for (uint32_t x = 1; x <= 16; x++) {
+      const uint64_t i = lo + x * step;
+      f[x - 1] = i < nl ? g.fence[i] : ~0ull;
+    }
+    uint32_t below = 0;  // samples inside the array with fence <= key (a prefix)
+#pragma unroll
+    for (uint32_t x = 1; x <= 16; x++) below += (lo + x * step < nl && f[x - 1] <= key) ? 1u : 0u;
+    if (below < 16) {
+      const uint64_t h = lo + (below + 1) * step;
+      hi = h < nl ? h : nl;
+      lo += below * step;
+      break;
+    }
+    lo += 16 * step;
+    step <<= 4;
+  }
+  return fence_narrow(g, key, lo, hi);
 }
 
 // The same leaf, searched from the leaf of begin_slot (the vertex's first
-// PMA slot) onwards.  Fences never decrease, so when fence[start] <= key the
-// answer is at or after the start, and a galloping search finds it in about
-// 2 log2(distance) fence reads instead of H; the vertex's keys span a few
-// leaves.  Any other start (none, a tree id, or a leaf past the key) takes
-// the full binary search, so a stale begin slot costs time, never the result.
+// PMA slot) onwards.  Any other start (none, a tree id, or a leaf past the
+// key) takes the full search, so a stale begin slot costs time, never the
+// result.
 __device__ inline uint64_t pma_find_leaf_from(const GView &g, uint64_t key, uint64_t begin_slot) {
-  const uint64_t nl = 1ull << g.H;
-  uint64_t lo = begin_slot >> kLeafShift;
-  if (lo >= nl || g.fence[lo] > key) return pma_find_leaf(g, key);
-  uint64_t step = 1, hi = nl;
-  while (lo + step < nl) {
-    if (g.fence[lo + step] > key) {
-      hi = lo + step;
-      break;
-    }
-    lo += step;
-    step <<= 1;
-  }
-  // fence[lo] <= key, and fence[hi] > key unless hi == nl
-  while (hi - lo > 1) {
-    const uint64_t mid = (lo + hi) >> 1;
-    if (g.fence[mid] <= key) lo = mid;
-    else hi = mid;
-  }
-  return lo;
+  const uint64_t lo = begin_slot >> kLeafShift;
+  if (lo >= (1ull << g.H) || g.fence[lo] > key) return pma_find_leaf(g, key);
+  return fence_gallop(g, key, lo);
 }
 
 // Leaf of key when it is the begin leaf L0 (fence[L0] <= key < fence[L0 + 1],
@@ -73,6 +102,7 @@ __device__ inline uint64_t pma_leaf_of(const GView &g, uint64_t key, uint64_t be
   if (L0 + 1 < (1ull << g.H)) {
     const uint64_t f0 = g.fence[L0], f1 = g.fence[L0 + 1];
     if (f0 <= key && key < f1) return L0;
+    if (f0 <= key) return fence_gallop(g, key, L0 + 1);
   }
   return pma_find_leaf_from(g, key, begin);
 }
@@ -94,19 +124,23 @@ __device__ inline uint64_t leaf_match(const GView &g, const uint4 *v, uint64_t b
 // Slot of (s,d) in the PMA, or ~0 when absent (begin: the vertex's aux
 // word).  The begin leaf's destinations are loaded with its two fences, so
 // a key inside the begin leaf costs one round trip (and the source check
-// where a destination matches); other keys take the fence search.
+// where a destination matches); other keys take the searches above.
 __device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d, uint64_t begin) {
   const uint64_t key = make_key(s, d);
   const uint64_t L0 = begin >> kLeafShift;
   uint4 v[kLeafSlots / 4];
+  uint64_t leaf;
   if (L0 + 1 < (1ull << g.H)) {
     const uint64_t f0 = g.fence[L0], f1 = g.fence[L0 + 1];
     const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + (L0 << kLeafShift));
 #pragma unroll
     for (uint32_t i = 0; i < kLeafSlots / 4; i++) v[i] = pd[i];
     if (f0 <= key && key < f1) return leaf_match(g, v, L0 << kLeafShift, s, d);
+    leaf = f0 <= key ? fence_gallop(g, key, L0 + 1) : pma_find_leaf(g, key);
+  } else {
+    leaf = pma_find_leaf_from(g, key, begin);
   }
-  const uint64_t base = pma_find_leaf_from(g, key, begin) << kLeafShift;
+  const uint64_t base = leaf << kLeafShift;
   const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + base);
 #pragma unroll
   for (uint32_t i = 0; i < kLeafSlots / 4; i++) v[i] = pd[i];
@@ -114,25 +148,35 @@ __device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d, u
 }
 
 // Index position j of the leaf block of tree t that may hold d
-// (last block whose first key <= d, or 0).  Trees of at most 16 blocks
-// (up to 4096 neighbours) load every first key at once.
+// (last block whose first key <= d, or 0): 16-way narrowing, 15 index keys
+// read at once per step, then the last (at most 16) candidates at once.
 __device__ inline uint32_t tree_block_pos(const GView &g, uint32_t t, uint32_t d) {
   const uint64_t off = g.toff[t];
   const uint32_t nb = g.tnblk[t];
-  if (nb <= 16) {
-    uint32_t j = 0;
+  uint32_t lo = 0, hi = nb;  // imin[lo] <= d (or lo == 0), imin[hi] > d (or hi == nb)
+  while (hi - lo > 16) {
+    const uint32_t step = (hi - lo + 15) >> 4;
+    uint32_t m[15];
+#pragma unroll
+    for (uint32_t x = 1; x < 16; x++) {
+      const uint32_t i = lo + x * step;
+      m[x - 1] = i < hi ? g.imin[off + i] : ~0u;
+    }
+    uint32_t nlo = lo;
 #pragma unroll
     for (uint32_t x = 1; x < 16; x++)
-      if (x < nb && g.imin[off + x] <= d) j = x;
-    return j;
+      if (lo + x * step < hi && m[x - 1] <= d) nlo = lo + x * step;
+    hi = nlo + step < hi ? nlo + step : hi;
+    lo = nlo;
   }
-  uint32_t lo = 0, hi = nb;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (g.imin[off + mid] <= d) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo == 0 ? 0 : lo - 1;
+  uint32_t m[15];
+#pragma unroll
+  for (uint32_t x = 1; x < 16; x++) m[x - 1] = lo + x < hi ? g.imin[off + lo + x] : ~0u;
+  uint32_t j = lo;
+#pragma unroll
+  for (uint32_t x = 1; x < 16; x++)
+    if (lo + x < hi && m[x - 1] <= d) j = lo + x;
+  return j;
 }
 
 // d among the first cnt keys of a leaf block (kBlk = 256 slots, sorted):
@@ -781,19 +825,16 @@ constexpr int kPrepBlock = 256;
 // kItems keys per thread.  kLookup: the existence flags are computed here
 // (one key per thread keeps the lookups' memory parallelism) and written
 // to ex; otherwise ex holds them already.
+// The work of block b (in start order) of nblocks.
 template <int kItems, bool kLookup>
-__global__ void __launch_bounds__(kPrepBlock) k_insert_prep(GView g, const uint64_t *R, const uint64_t *np,
-                                                            uint64_t cap, uint32_t *ex, uint32_t *nidx,
-                                                            uint32_t *run_beg, uint64_t *nruns, uint64_t *state,
-                                                            uint32_t *misc, uint32_t *err, uint32_t nblocks,
-                                                            uint32_t epoch) {
+__device__ inline void insert_prep_block(uint32_t b, GView g, const uint64_t *R, const uint64_t *np, uint64_t cap,
+                                         uint32_t *ex, uint32_t *nidx, uint32_t *run_beg, uint64_t *nruns,
+                                         uint64_t *state, uint32_t *err, uint32_t nblocks, uint32_t epoch) {
   constexpr int kPrepItems = kItems, kPrepTile = kPrepBlock * kItems;
   __shared__ uint32_t wcnt[2][kPrepItems][kPrepBlock / 64];
   __shared__ uint32_t wpre[2][kPrepItems][kPrepBlock / 64];
-  __shared__ uint32_t s_ticket, s_run[2];
+  __shared__ uint32_t s_run[2];
   __shared__ uint64_t s_off[2];
-  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
-  if (b >= nblocks) return;
   const uint64_t n = *np;
   const uint32_t t = threadIdx.x, w = t >> 6;
   const uint64_t lt = lanemask_lt();
@@ -855,19 +896,27 @@ __global__ void __launch_bounds__(kPrepBlock) k_insert_prep(GView g, const uint6
   }
 }
 
+template <int kItems, bool kLookup>
+__global__ void __launch_bounds__(kPrepBlock) k_insert_prep(GView g, const uint64_t *R, const uint64_t *np,
+                                                            uint64_t cap, uint32_t *ex, uint32_t *nidx,
+                                                            uint32_t *run_beg, uint64_t *nruns, uint64_t *state,
+                                                            uint32_t *misc, uint32_t *err, uint32_t nblocks,
+                                                            uint32_t epoch) {
+  __shared__ uint32_t s_ticket;
+  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
+  if (b >= nblocks) return;
+  insert_prep_block<kItems, kLookup>(b, g, R, np, cap, ex, nidx, run_beg, nruns, state, err, nblocks, epoch);
+}
+
 // Delete preparation of a batch of kSmallSortKeys to kBucketSortMax keys in
 // one single-pass launch: the existing keys among the first *np of R
 // compacted into out in order (count into *nout), one key per thread (the
 // lookups keep their memory parallelism).
-__global__ void __launch_bounds__(kPrepBlock) k_delete_prep(GView g, const uint64_t *R, const uint64_t *np,
-                                                            uint64_t *out, uint64_t *nout, uint64_t *state,
-                                                            uint32_t *misc, uint32_t *err, uint32_t nblocks,
-                                                            uint32_t epoch) {
+__device__ inline void delete_prep_block(uint32_t b, GView g, const uint64_t *R, const uint64_t *np, uint64_t *out,
+                                         uint64_t *nout, uint64_t *state, uint32_t *err, uint32_t nblocks,
+                                         uint32_t epoch) {
   __shared__ uint32_t wcnt[kPrepBlock / 64];
-  __shared__ uint32_t s_ticket;
   __shared__ uint64_t s_off;
-  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
-  if (b >= nblocks) return;
   const uint64_t n = *np;
   const uint32_t t = threadIdx.x, w = t >> 6;
   const uint64_t idx = static_cast<uint64_t>(b) * kPrepBlock + t;
@@ -895,6 +944,36 @@ __global__ void __launch_bounds__(kPrepBlock) k_delete_prep(GView g, const uint6
   }
   __syncthreads();
   if (e) out[s_off + before + r] = key;
+}
+
+__global__ void __launch_bounds__(kPrepBlock) k_delete_prep(GView g, const uint64_t *R, const uint64_t *np,
+                                                            uint64_t *out, uint64_t *nout, uint64_t *state,
+                                                            uint32_t *misc, uint32_t *err, uint32_t nblocks,
+                                                            uint32_t epoch) {
+  __shared__ uint32_t s_ticket;
+  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
+  if (b >= nblocks) return;
+  delete_prep_block(b, g, R, np, out, nout, state, err, nblocks, epoch);
+}
+
+// Both halves of a mixed batch in one launch, so the two latency-bound
+// lookup passes overlap: blocks in start order [0, nb) prepare the inserted
+// keys I (look-back words state[0, 2 nb)), blocks [nb, 2 nb) the deleted
+// keys D (words state[2 nb, 3 nb)).  A block of either half only waits on
+// blocks of its half that started before it.
+__global__ void __launch_bounds__(kPrepBlock) k_mixed_prep(GView g, const uint64_t *I, const uint64_t *ni,
+                                                           uint64_t cap, uint32_t *ex, uint32_t *nidx,
+                                                           uint32_t *run_beg, uint64_t *nruns, const uint64_t *D,
+                                                           const uint64_t *nd, uint64_t *E, uint64_t *ne,
+                                                           uint64_t *state, uint32_t *misc, uint32_t *err,
+                                                           uint32_t nb, uint32_t epoch) {
+  __shared__ uint32_t s_ticket;
+  const uint32_t b = lbk::ticket(misc, err, 2 * nb, &s_ticket);
+  if (b >= 2 * nb) return;
+  if (b < nb)
+    insert_prep_block<1, true>(b, g, I, ni, cap, ex, nidx, run_beg, nruns, state, err, nb, epoch);
+  else
+    delete_prep_block(b - nb, g, D, nd, E, ne, state + 2ull * nb, err, nb, epoch);
 }
 
 // ------------------------------------------------------------ inline merge
@@ -3699,15 +3778,11 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
       TG_CHECK_LAUNCH();
     } else if (count <= kBucketSortMax) {
       const uint32_t nb = static_cast<uint32_t>((count + kPrepBlock - 1) / kPrepBlock);
-      uint32_t epoch = scan_scratch.lb.begin(2ull * nb, stream_);
-      k_insert_prep<1, true><<<nb, kPrepBlock, 0, stream_>>>(
-          g, I, dsc_.get() + 1, count, f0_.get(), f1_.get(), s32_[0].get(), dsc_.get() + 2,
-          scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb, epoch);
-      TG_CHECK_LAUNCH();
-      epoch = scan_scratch.lb.begin(nb, stream_);
-      k_delete_prep<<<nb, kPrepBlock, 0, stream_>>>(g, D, dsc_.get() + 9, kE_.get(), dsc_.get() + 10,
-                                                    scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(),
-                                                    scan_scratch.lb.err_word(), nb, epoch);
+      const uint32_t epoch = scan_scratch.lb.begin(3ull * nb, stream_);
+      k_mixed_prep<<<2 * nb, kPrepBlock, 0, stream_>>>(
+          g, I, dsc_.get() + 1, count, f0_.get(), f1_.get(), s32_[0].get(), dsc_.get() + 2, D, dsc_.get() + 9,
+          kE_.get(), dsc_.get() + 10, scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(),
+          scan_scratch.lb.err_word(), nb, epoch);
       TG_CHECK_LAUNCH();
       select(SelRunStartN{kE_.get(), s32_[1].get(), dsc_.get() + 10}, count, dsc_.get() + 11, scan_scratch, stream_);
     } else {
