@@ -256,11 +256,14 @@ int tg_apply_ops(tg_graph *g, const uint32_t *src, const uint32_t *dst, const ui
  * report[12]: thr_bfs, thr_ms, thr_cc, pull_block, then (reference ms,
  * tuned ms) for BFS (3 sources), MS-BFS (64 sources), CC and PageRank (3
  * iterations).  Results never depend on the tuning.
- * tuning[7]: thr_bfs, thr_ms, thr_cc, pull_block (64/128/256/512), pr_graph
- * (1 = PageRank iterations replayed from a captured HIP graph), pr_sliced
- * (1 = PageRank over an XCD-sliced adjacency copy when n <= 6M: each XCD's
- * L2 holds one slice of the ranks), push_wave (push frontiers up to this
- * size give each PMA vertex a 64-lane wave, larger ones 16 lanes). */
+ * tuning[8]: thr_bfs, thr_ms, thr_cc, pull_block (64/128/256/512), pr_graph
+ * (1 = PageRank iterations replayed from a captured HIP graph, kept per
+ * graph version), pr_sliced (1 = PageRank over an XCD-sliced adjacency copy
+ * when n <= 6M: each XCD's L2 holds one slice of the ranks), push_wave (push
+ * frontiers up to this size give each PMA vertex a 64-lane wave, larger
+ * ones 16 lanes), level_graph (1 = BFS and CC levels, whose directions are
+ * chosen on the device, replayed from a captured HIP graph of two levels;
+ * 0 = the same kernels launched one by one). */
 int tg_autotune(tg_graph *g, double *report);
 int tg_get_tuning(tg_graph *g, uint32_t *tuning);
 int tg_set_tuning(tg_graph *g, const uint32_t *tuning);

@@ -828,6 +828,7 @@ int tg_get_tuning(tg_graph *G, uint32_t *out) {
     out[4] = T.pr_graph;
     out[5] = T.pr_sliced;
     out[6] = T.push_wave;
+    out[7] = T.level_graph;
   });
 }
 
@@ -836,9 +837,9 @@ int tg_set_tuning(tg_graph *G, const uint32_t *in) {
     NEED(G && in, "null argument");
     NEED(in[0] >= 1 && in[1] >= 1 && in[2] >= 1, "thresholds must be >= 1");
     NEED(in[3] == 64 || in[3] == 128 || in[3] == 256 || in[3] == 512, "pull block must be 64, 128, 256 or 512");
-    NEED(in[4] <= 1 && in[5] <= 1, "pr_graph and pr_sliced must be 0 or 1");
+    NEED(in[4] <= 1 && in[5] <= 1 && in[7] <= 1, "pr_graph, pr_sliced and level_graph must be 0 or 1");
     std::lock_guard<std::mutex> l(G->g.mu);
-    G->g.tune = tg::Graph::Tuning{in[0], in[1], in[2], in[3], in[4], in[5], in[6]};
+    G->g.tune = tg::Graph::Tuning{in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7]};
   });
 }
 

@@ -168,21 +168,25 @@ class Graph {
 
   // Launch parameters chosen by the autotuner; the whole-algorithm calls
   // use them.  thr_*: edgeMap push/pull switch (dense when n / thr <=
-  // |frontier|, include/Map.cpp:66-74, reference value 20; CC starts at 5,
-  // which the autotuner picked on the scale-22 plain and skewed rMAT
-  // graphs); pull_block: workgroup size of the inline and PMA pull
-  // kernels.  Results do not depend on them.
+  // |frontier|, include/Map.cpp:66-74, reference value 20; BFS starts at
+  // 40 and CC at 5, the fastest in threshold sweeps on the scale-22 plain
+  // and skewed rMAT graphs); pull_block: workgroup size of the inline and
+  // PMA pull kernels.  Results do not depend on them.
   // pr_graph: PageRank iterations replayed from a captured HIP graph.
   // pr_sliced: PageRank over the XCD-sliced adjacency copy when the graph
   // has at most 6M vertices (else the tiered pull).
   // push_wave: push frontiers up to this size give each PMA vertex a wave
   // (64 lanes), larger ones a 16-lane group.
+  // level_graph: BFS and CC levels (directions chosen on the device)
+  // replayed from a captured HIP graph of two levels; 0 launches the same
+  // kernels one by one.
   struct Tuning {
-    uint32_t thr_bfs = 20, thr_ms = 20, thr_cc = 5;
+    uint32_t thr_bfs = 40, thr_ms = 20, thr_cc = 5;
     uint32_t pull_block = 256;
     uint32_t pr_graph = 1;
     uint32_t pr_sliced = 1;
     uint32_t push_wave = 16384;
+    uint32_t level_graph = 1;
   };
   Tuning tune;
 

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 
 namespace tg {
 
@@ -55,6 +56,15 @@ __device__ inline double wave_sum_d(double v) {
 
 enum Mode { kFirst = 0, kExcl = 1, kShared = 2 };
 
+// A launch whose work depends on a word the device decides (the direction
+// of a level, run without a host read): the kernel returns at once unless
+// *p == want.  p = null: always runs.
+struct Gate {
+  const uint32_t *p = nullptr;
+  uint32_t want = 0;
+  __device__ bool off() const { return p != nullptr && *p != want; }
+};
+
 // ---------------------------------------------------------------- pull ops
 // BFS pull (BFS_F, include/BFS.h:27-39; map_dense_vs_not_all early exit,
 // include/graph.h:1490-1613): parent = first frontier neighbour in sorted
@@ -65,6 +75,7 @@ struct PullBFS {
   uint32_t *level;
   uint32_t depth;
   uint32_t *obits;
+  const uint32_t *depth_p = nullptr;  // non-null: the level number is read here
   using Acc = uint32_t;
   static constexpr bool kEarly = true;
   static constexpr bool kOrdered = false;
@@ -85,7 +96,7 @@ struct PullBFS {
     } else {
       parent[i] = static_cast<int32_t>(a);
     }
-    if (level) level[i] = depth;
+    if (level) level[i] = depth_p ? *depth_p : depth;
     bit_set(obits, i);
   }
 };
@@ -95,6 +106,7 @@ struct PullBFS {
 struct PullCC {
   const uint32_t *fbits;  // null = all
   uint32_t *ids;
+  uint32_t *obits = nullptr;  // non-null: vertices whose ID this pull lowers (IDs != prevIDs)
   using Acc = uint32_t;
   static constexpr bool kEarly = false;
   static constexpr bool kOrdered = false;
@@ -110,8 +122,14 @@ struct PullCC {
   __device__ static bool found(Acc) { return false; }
   __device__ void commit(uint32_t i, Acc a, Mode m) const {
     if (a == 0xFFFFFFFFu) return;
-    if (m == kShared) atomicMin(&ids[i], a);
-    else if (a < ids[i]) ids[i] = a;
+    bool lowered;
+    if (m == kShared) {
+      lowered = atomicMin(&ids[i], a) > a;
+    } else {
+      lowered = a < ids[i];
+      if (lowered) ids[i] = a;
+    }
+    if (lowered && obits) bit_set(obits, i);
   }
 };
 
@@ -260,14 +278,26 @@ __device__ inline void load_vblock(const VBlock *vb, uint32_t v, uint32_t (&w)[1
   w[12] = q3.x; w[13] = q3.y; w[14] = q3.z; w[15] = q3.w;
 }
 
+// Pull bodies as device functions of (workgroup index, workgroup count), so
+// the three tiers can also run as block ranges of one launch.
+//
+// fused: the PMA pull of the same launch covers PMA vertices whole (their
+// inline slots too), so they are skipped here, and a B-tree vertex's inline
+// part commits as a shared commit (its tree tasks run alongside).
 template <class Op>
-__global__ void __launch_bounds__(512) k_pull_inline(GView g, Op op) {
-  GRID_STRIDE(i, g.n) {
+__device__ inline void pull_inline_body(const GView &g, const Op &op, uint64_t bid, uint64_t nblk,
+                                        bool fused = false) {
+  for (uint64_t i = bid * blockDim.x + threadIdx.x; i < g.n; i += nblk * blockDim.x) {
     uint32_t v = static_cast<uint32_t>(i);
     if (op.skip(v)) continue;
     uint32_t w[16];
     load_vblock(g.vb, v, w);
     uint32_t D = w[0];
+    Mode mode = kFirst;
+    if (fused) {
+      if (w[15] & static_cast<uint32_t>(kAuxTree >> 32)) mode = kShared;  // a B-tree vertex
+      else if (D > kInline) continue;                                     // a PMA vertex
+    }
     uint32_t lim = D < kInline ? D : kInline;
     typename Op::Acc a = Op::ident();
     bool done = false;
@@ -279,20 +309,29 @@ __global__ void __launch_bounds__(512) k_pull_inline(GView g, Op op) {
         if (Op::kEarly && Op::found(a)) done = true;
       }
     }
-    op.commit(v, a, kFirst);
+    op.commit(v, a, mode);
   }
+}
+
+template <class Op>
+__global__ void __launch_bounds__(512) k_pull_inline(GView g, Op op, Gate gate) {
+  if (gate.off()) return;
+  pull_inline_body(g, op, blockIdx.x, gridDim.x);
 }
 
 // 16-lane group per PMA vertex; each lane covers 4 slots with one 16-byte
 // load, so a group reads 64 slots (256 bytes) per step and a wave works on
 // four vertices at once.
+// fused: the group also takes the vertex's inline slots (lanes 0..12) and
+// commits the vertex alone.
 template <class Op>
-__global__ void __launch_bounds__(512) k_pull_pma(GView g, Op op, const uint32_t *list, uint32_t cnt) {
+__device__ inline void pull_pma_body(const GView &g, const Op &op, const uint32_t *list, uint32_t cnt, uint64_t bid,
+                                     uint64_t nblk, bool fused = false) {
   const uint32_t lane = threadIdx.x & 63, gl = lane & 15;
   const uint64_t gmask = 0xFFFFull << (lane & 48);
   const uint64_t lt = lanemask_lt() & gmask;
-  uint64_t grp = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 4;
-  uint64_t ngrp = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 4;
+  uint64_t grp = (bid * blockDim.x + threadIdx.x) >> 4;
+  uint64_t ngrp = (nblk * blockDim.x) >> 4;
   for (uint64_t w = grp; w < cnt; w += ngrp) {
     uint32_t v = list[w];
     if (op.skip(v)) continue;  // uniform within the group
@@ -302,10 +341,20 @@ __global__ void __launch_bounds__(512) k_pull_pma(GView g, Op op, const uint32_t
     uint64_t p = begin & ~3ull;
     uint32_t got = 0;
     typename Op::Acc a = Op::ident();
+    // the run's first 64 slots, loaded before the inline slots are tested
+    uint4 q = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+    if (p + gl * 4 < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + p + gl * 4);
+    if (fused) {
+      // the 13 inline slots, the vertex's smallest neighbours (a PMA vertex
+      // holds all 13)
+      if (gl < kInline) {
+        const uint32_t u = b.nbr[gl];
+        if (op.member(u)) a = op.value_e(v, u);
+      }
+      if (Op::kEarly && (__ballot(Op::found(a)) & gmask) != 0) need = 0;
+    }
     while (got < need && p < g.N) {
       uint64_t slot = p + gl * 4;
-      uint4 q = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
-      if (slot < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + slot);
       uint32_t d[4] = {q.x, q.y, q.z, q.w};
       bool val[4];
 #pragma unroll
@@ -327,10 +376,18 @@ __global__ void __launch_bounds__(512) k_pull_pma(GView g, Op op, const uint32_t
       got += __popcll(B0) + __popcll(B1) + __popcll(B2) + __popcll(B3);
       p += 64;
       if (Op::kEarly && (__ballot(hit) & gmask) != 0) break;
+      q = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+      if (got < need && p + gl * 4 < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + p + gl * 4);
     }
     a = Op::reduce16(a, gmask);
-    if (gl == 0) op.commit(v, a, kExcl);
+    if (gl == 0) op.commit(v, a, fused ? kFirst : kExcl);
   }
+}
+
+template <class Op>
+__global__ void __launch_bounds__(512) k_pull_pma(GView g, Op op, const uint32_t *list, uint32_t cnt, Gate gate) {
+  if (gate.off()) return;
+  pull_pma_body(g, op, list, cnt, blockIdx.x, gridDim.x);
 }
 
 // One wave per (tree vertex, leaf block) task; the task's vertex and block
@@ -339,10 +396,10 @@ __global__ void __launch_bounds__(512) k_pull_pma(GView g, Op op, const uint32_t
 // parts != null: the task's partial goes to parts[t] and k_tree_parts adds
 // the partials of each vertex in leaf order (deterministic FP64 sums).
 template <class Op>
-__global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_t *task_v, const uint32_t *task_b,
-                                                   uint64_t ntasks, typename Op::Acc *parts) {
-  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
-  const uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+__device__ inline void pull_tree_body(const GView &g, const Op &op, const uint32_t *task_v, const uint32_t *task_b,
+                                      uint64_t ntasks, typename Op::Acc *parts, uint64_t bid, uint64_t nblk) {
+  const uint64_t wave = (bid * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = (nblk * blockDim.x) >> 6;
   const uint32_t lane = threadIdx.x & 63;
   for (uint64_t t = wave; t < ntasks; t += nwaves) {
     const uint32_t v = task_v[t];
@@ -367,10 +424,35 @@ __global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_
   }
 }
 
+template <class Op>
+__global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_t *task_v, const uint32_t *task_b,
+                                                   uint64_t ntasks, typename Op::Acc *parts, Gate gate) {
+  if (gate.off()) return;
+  pull_tree_body(g, op, task_v, task_b, ntasks, parts, blockIdx.x, gridDim.x);
+}
+
+// The three pulls in one launch of 256-thread workgroups: [0, nb_inline)
+// inline, then nb_pma PMA workgroups, then the tree tasks (ops whose tree
+// partials need no ordered sum).  Each inline or PMA vertex is pulled by one
+// thread or group alone (plain commits); a B-tree vertex's inline part and
+// tree tasks run alongside each other and commit atomically.
+template <class Op>
+__global__ void __launch_bounds__(256) k_pull_all(GView g, Op op, const uint32_t *pma, uint32_t npma,
+                                                  const uint32_t *task_v, const uint32_t *task_b, uint64_t ntasks,
+                                                  uint32_t nb_inline, uint32_t nb_pma, Gate gate) {
+  static_assert(!Op::kOrdered, "ordered tree sums take the separate launches");
+  if (gate.off()) return;
+  const uint32_t b = blockIdx.x;
+  if (b < nb_inline) pull_inline_body(g, op, b, nb_inline, true);
+  else if (b < nb_inline + nb_pma) pull_pma_body(g, op, pma, npma, b - nb_inline, nb_pma, true);
+  else pull_tree_body(g, op, task_v, task_b, ntasks, nullptr, b - nb_inline - nb_pma, gridDim.x - nb_inline - nb_pma);
+}
+
 // One thread per tree vertex: its task partials in leaf order, one commit.
 template <class Op>
 __global__ void k_tree_parts(Op op, const uint32_t *list, uint32_t cnt, const uint64_t *task_off,
-                             const typename Op::Acc *parts) {
+                             const typename Op::Acc *parts, Gate gate) {
+  if (gate.off()) return;
   GRID_STRIDE(li, cnt) {
     uint32_t v = list[li];
     if (op.skip(v)) continue;
@@ -386,10 +468,11 @@ struct PushBFS {
   int32_t *parent;
   uint32_t *level;
   uint32_t depth;
+  const uint32_t *depth_p = nullptr;  // non-null: the level number is read here
   __device__ bool apply(uint32_t s, uint32_t d) const {
     if (parent[d] != -1) return false;
     if (atomicCAS(&parent[d], -1, static_cast<int32_t>(s)) != -1) return false;
-    if (level) level[d] = depth;
+    if (level) level[d] = depth_p ? *depth_p : depth;
     return true;
   }
 };
@@ -488,30 +571,65 @@ __device__ inline void push_emit(const Op &op, const PushOut &o, WaveQueue &wq, 
 
 // Thread per frontier vertex: inline neighbours; vertices with a secondary
 // run are appended to the PMA / tree lists.
+//
+// One frontier vertex s per lane (ok: the lane has one); the whole wave
+// calls it.
 template <class Op>
-__global__ void __launch_bounds__(256) k_push_inline(GView g, Op op, PushOut o, const uint32_t *fq, uint32_t fcount,
-                                                     uint32_t *sec_pma, uint32_t *sec_tree, uint32_t *sec_cnt) {
+__device__ inline void push_inline_one(const GView &g, const Op &op, const PushOut &o, WaveQueue &out,
+                                       WaveQueue &qp, WaveQueue &qt, bool ok, uint32_t s) {
+  uint32_t wv[16];
+  load_vblock(g.vb, s, wv);
+  uint32_t D = ok ? wv[0] : 0;
+  uint32_t lim = D < kInline ? D : kInline;
+#pragma unroll
+  for (uint32_t k = 0; k < kInline; k++) push_emit(op, o, out, s, wv[1 + k], k < lim);
+  bool sec = ok && D > kInline;
+  uint64_t aux = (static_cast<uint64_t>(wv[15]) << 32) | wv[14];
+  bool tree = sec && (aux & kAuxTree);
+  qp.push(sec && !tree, s);
+  qt.push(tree, s);
+}
+
+// fbits / from_bits (device levels): when *from_bits is set the frontier
+// is read from its bitmap (fbits, words words over n vertices) instead of
+// the queue, which a pull level does not write.
+template <class Op>
+__global__ void __launch_bounds__(256) k_push_inline(GView g, Op op, PushOut o, const uint32_t *fq, uint32_t fcount_h,
+                                                     uint32_t *sec_pma, uint32_t *sec_tree, uint32_t *sec_cnt,
+                                                     const uint32_t *fcount_d, Gate gate,
+                                                     const uint32_t *fbits = nullptr, uint64_t words = 0,
+                                                     uint32_t n = 0, const uint32_t *from_bits = nullptr) {
+  if (gate.off()) return;
+  // the frontier's size: on the device when fcount_d is given
+  const uint32_t fcount = fcount_d ? *fcount_d : fcount_h;
   __shared__ uint32_t lq[3][4][kWQ];
   const uint32_t w = threadIdx.x >> 6;
   WaveQueue out{lq[0][w], o.queue, o.count, 0};
   WaveQueue qp{lq[1][w], sec_pma, sec_cnt + 0, 0};
   WaveQueue qt{lq[2][w], sec_tree, sec_cnt + 1, 0};
-  for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < fcount;
-       i0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint64_t i = i0 + threadIdx.x;
-    bool ok = i < fcount;
-    uint32_t s = ok ? fq[i] : 0;
-    uint32_t wv[16];
-    load_vblock(g.vb, s, wv);
-    uint32_t D = ok ? wv[0] : 0;
-    uint32_t lim = D < kInline ? D : kInline;
-#pragma unroll
-    for (uint32_t k = 0; k < kInline; k++) push_emit(op, o, out, s, wv[1 + k], k < lim);
-    bool sec = ok && D > kInline;
-    uint64_t aux = (static_cast<uint64_t>(wv[15]) << 32) | wv[14];
-    bool tree = sec && (aux & kAuxTree);
-    qp.push(sec && !tree, s);
-    qt.push(tree, s);
+  if (from_bits && *from_bits) {
+    for (uint64_t w0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); w0 < words;
+         w0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+      const uint64_t wi = w0 + threadIdx.x;
+      uint32_t x = wi < words ? fbits[wi] : 0u;
+      if (wi < words && (wi + 1) * 32 > n) {
+        const uint32_t keep = n - static_cast<uint32_t>(wi * 32);
+        x &= keep >= 32 ? 0xFFFFFFFFu : ((1u << keep) - 1u);
+      }
+      while (__ballot(x != 0) != 0) {  // wave-uniform: every lane takes part in the pushes
+        const bool ok = x != 0;
+        const uint32_t s = ok ? static_cast<uint32_t>(wi * 32 + (__ffs(x) - 1)) : 0u;
+        x &= x - 1;
+        push_inline_one(g, op, o, out, qp, qt, ok, s);
+      }
+    }
+  } else {
+    for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); i0 < fcount;
+         i0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+      uint64_t i = i0 + threadIdx.x;
+      bool ok = i < fcount;
+      push_inline_one(g, op, o, out, qp, qt, ok, ok ? fq[i] : 0u);
+    }
   }
   out.flush();
   qp.flush();
@@ -527,18 +645,16 @@ __global__ void __launch_bounds__(256) k_push_inline(GView g, Op op, PushOut o, 
 // Large frontiers use groups of 16 (four vertices per wave); small ones,
 // whose vertices are few and heavy, a whole wave (256 slots per step).
 template <class Op, int GS>
-__global__ void __launch_bounds__(256) k_push_pma(GView g, Op op, PushOut o, const uint32_t *list,
-                                                  const uint32_t *cnt_ptr) {
+__device__ inline void push_pma_body(const GView &g, const Op &op, const PushOut &o, const uint32_t *list,
+                                     uint32_t cnt, uint32_t *lq_wave, uint64_t bid, uint64_t nblk) {
   static_assert(GS == 16 || GS == 64, "group of 16 lanes or a wave");
   constexpr int GPW = 64 / GS;  // groups per wave
-  __shared__ uint32_t lq[4][kWQ];
-  const uint32_t cnt = *cnt_ptr;
   const uint32_t lane = threadIdx.x & 63, gl = lane & (GS - 1), gi = lane / GS;
   const uint64_t gmask = GS == 64 ? ~0ull : (0xFFFFull << (lane & 48));
   const uint64_t lt = lanemask_lt() & gmask;
-  WaveQueue out{lq[threadIdx.x >> 6], o.queue, o.count, 0};
-  uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
-  uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  WaveQueue out{lq_wave, o.queue, o.count, 0};
+  uint64_t wave = (bid * blockDim.x + threadIdx.x) >> 6;
+  uint64_t nwaves = (nblk * blockDim.x) >> 6;
   for (uint64_t base = wave * GPW; base < cnt; base += nwaves * GPW) {
     uint64_t w = base + gi;
     bool have = w < cnt;
@@ -574,24 +690,42 @@ __global__ void __launch_bounds__(256) k_push_pma(GView g, Op op, PushOut o, con
   out.flush();
 }
 
+template <class Op, int GS>
+__global__ void __launch_bounds__(256) k_push_pma(GView g, Op op, PushOut o, const uint32_t *list,
+                                                  const uint32_t *cnt_ptr, Gate gate) {
+  if (gate.off()) return;
+  __shared__ uint32_t lq[4][kWQ];
+  push_pma_body<Op, GS>(g, op, o, list, *cnt_ptr, lq[threadIdx.x >> 6], blockIdx.x, gridDim.x);
+}
+
+// Both PMA push widths in one launch, the width chosen on the device
+// (words w64, w16: 1 for the width this level uses, both 0 when it does
+// not push).
+template <class Op>
+__global__ void __launch_bounds__(256) k_push_pma_dev(GView g, Op op, PushOut o, const uint32_t *list,
+                                                      const uint32_t *cnt_ptr, const uint32_t *w64,
+                                                      const uint32_t *w16) {
+  __shared__ uint32_t lq[4][kWQ];
+  if (*w64) push_pma_body<Op, 64>(g, op, o, list, *cnt_ptr, lq[threadIdx.x >> 6], blockIdx.x, gridDim.x);
+  else if (*w16) push_pma_body<Op, 16>(g, op, o, list, *cnt_ptr, lq[threadIdx.x >> 6], blockIdx.x, gridDim.x);
+}
+
 // Frontier B-tree vertices, 256 ids (one leaf block) per workgroup step.
 // Up to kPushTreeSpread of them: every workgroup takes a stride of each
 // vertex's blocks (a hub's blocks spread over the grid); more: one
 // workgroup per vertex.  The count is read on the device.
 constexpr uint32_t kPushTreeSpread = 64;
 template <class Op>
-__global__ void __launch_bounds__(256) k_push_tree(GView g, Op op, PushOut o, const uint32_t *list,
-                                                   const uint32_t *cnt_ptr) {
-  __shared__ uint32_t lq[4][kWQ];
-  const uint32_t cnt = *cnt_ptr;
-  WaveQueue out{lq[threadIdx.x >> 6], o.queue, o.count, 0};
+__device__ inline void push_tree_body(const GView &g, const Op &op, const PushOut &o, const uint32_t *list,
+                                      uint32_t cnt, uint32_t *lq_wave, uint32_t bid, uint32_t nblk) {
+  WaveQueue out{lq_wave, o.queue, o.count, 0};
   const bool spread = cnt <= kPushTreeSpread;
-  for (uint64_t w = spread ? 0 : blockIdx.x; w < cnt; w += spread ? 1 : gridDim.x) {
+  for (uint64_t w = spread ? 0 : bid; w < cnt; w += spread ? 1 : nblk) {
     uint32_t s = list[w];
     uint32_t tr = static_cast<uint32_t>(g.vb[s].aux);
     uint64_t off = g.toff[tr];
     uint32_t nb = g.tnblk[tr];
-    for (uint32_t j = spread ? blockIdx.x : 0; j < nb; j += spread ? gridDim.x : 1) {
+    for (uint32_t j = spread ? bid : 0; j < nb; j += spread ? nblk : 1) {
       uint32_t blk = g.iblk[off + j];
       uint32_t c = g.bcnt[blk];
       bool valid = threadIdx.x < c;
@@ -601,6 +735,38 @@ __global__ void __launch_bounds__(256) k_push_tree(GView g, Op op, PushOut o, co
   }
   out.flush();
 }
+
+template <class Op>
+__global__ void __launch_bounds__(256) k_push_tree(GView g, Op op, PushOut o, const uint32_t *list,
+                                                   const uint32_t *cnt_ptr, Gate gate) {
+  if (gate.off()) return;
+  __shared__ uint32_t lq[4][kWQ];
+  push_tree_body(g, op, o, list, *cnt_ptr, lq[threadIdx.x >> 6], blockIdx.x, gridDim.x);
+}
+
+// ---------------------------------------------------------------- device levels
+// BFS and CC levels run on counts that stay on the device: a one-thread
+// plan kernel picks each level's direction (the rule of include/Map.cpp:
+// 66-74, dense when n / threshold <= |frontier|) and every kernel of the
+// level returns at once unless its direction was picked (Gate).  Two
+// frontiers alternate (level x -> y); the host reads the counts only
+// between groups of levels.  Control words:
+// Words that kernels read (mode, gates, depth) sit in one 128-byte line,
+// each count that a level's kernels update atomically in a line of its own:
+// reads of a line that other waves are updating with atomics are slow.
+constexpr uint32_t kLcHasQ = 2;    // [2], [3]: the frontier's queue holds its members
+constexpr uint32_t kLcMode = 4;    // this level: 0 idle (empty frontier), 1 push, 2 pull
+constexpr uint32_t kLcDepth = 7;   // levels run
+constexpr uint32_t kLcPush64 = 8;  // 1: this push gives each PMA vertex a whole wave
+constexpr uint32_t kLcPush16 = 9;  // 1: this push gives each PMA vertex 16 lanes
+constexpr uint32_t kLcBuildQ = 10; // 1: the push reads the frontier from its bitmap (no queue)
+constexpr uint32_t kLcCnt = 32;    // [32], [64]: frontier sizes (kLcCnt + kLcStride * x)
+constexpr uint32_t kLcStride = 32;
+constexpr uint32_t kLcSec = 96;    // [96], [97]: push lists of PMA and B-tree vertices
+constexpr uint32_t kLcDone = 112;  // workgroups of k_lvl_finish done
+constexpr uint32_t kLcWords = 128;
+__host__ __device__ constexpr uint32_t lc_cnt(uint32_t x) { return kLcCnt + kLcStride * x; }
+constexpr uint32_t kModePush = 1, kModePull = 2;
 
 // ---------------------------------------------------------------- helpers
 __global__ void k_bits_from_queue(const uint32_t *q, uint64_t cnt, uint32_t *bits) {
@@ -650,6 +816,86 @@ __global__ void k_pr_norm(GView g, const double *p, double *q) {
 // CC: output = vertices whose label changed this round.
 __global__ void k_cc_changed(const uint32_t *ids, const uint32_t *prev, uint32_t n, uint32_t *obits) {
   GRID_STRIDE(i, n) if (ids[i] != prev[i]) bit_set(obits, static_cast<uint32_t>(i));
+}
+
+
+// Plans level x -> y from x's size: its direction (idle when x is empty),
+// the PMA push width, whether the push must build x's queue; clears the
+// level's counters.  Counts written by atomics in the same launch are read
+// with device-scope atomic loads (the XCDs' L2 caches are not coherent).
+__device__ inline void lvl_plan(uint32_t *c, uint32_t x, uint32_t y, uint32_t n, uint32_t thr, uint32_t push_wave) {
+  const uint32_t cnt = __hip_atomic_load(&c[lc_cnt(x)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t mode = cnt == 0 ? 0u : (n / thr <= cnt ? kModePull : kModePush);
+  c[kLcMode] = mode;
+  c[lc_cnt(y)] = 0;
+  c[kLcHasQ + y] = mode == kModePush ? 1u : 0u;  // a push writes the queue and the bitmap
+  c[kLcSec] = 0;
+  c[kLcSec + 1] = 0;
+  c[kLcPush64] = mode == kModePush && cnt <= push_wave ? 1u : 0u;
+  c[kLcPush16] = mode == kModePush && cnt > push_wave ? 1u : 0u;
+  c[kLcBuildQ] = mode == kModePush && c[kLcHasQ + x] == 0 ? 1u : 0u;
+  if (mode) c[kLcDepth]++;
+}
+
+// Ends level x -> y: the size of a pull's output bitmap into y's count (a
+// push counted its appends), clears x's bitmap (the next level's output),
+// and the last workgroup to finish plans level y -> x.
+__global__ void __launch_bounds__(256) k_lvl_finish(uint32_t *c, uint32_t x, uint32_t y, uint32_t *xbits,
+                                                    const uint32_t *ybits, uint64_t words, uint32_t n, uint32_t thr,
+                                                    uint32_t push_wave) {
+  __shared__ uint32_t part[4];
+  __shared__ bool last;
+  const bool counting = c[kLcMode] == kModePull;
+  uint32_t sum = 0;
+  GRID_STRIDE(w, words) {
+    if (counting) {
+      uint32_t v = ybits[w];
+      const uint64_t hi = (w + 1) * 32;
+      if (hi > n) {
+        const uint32_t keep = n - static_cast<uint32_t>(w * 32);
+        v &= keep >= 32 ? 0xFFFFFFFFu : ((1u << keep) - 1u);
+      }
+      sum += __popc(v);
+    }
+    xbits[w] = 0;
+  }
+  for (uint32_t o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(&c[lc_cnt(y)], t);
+    __threadfence();
+    last = atomicAdd(&c[kLcDone], 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    c[kLcDone] = 0;
+    lvl_plan(c, y, x, n, thr, push_wave);
+  }
+}
+
+constexpr uint64_t kFinishGrid = 128;  // few workgroups: the last one plans the next level
+
+// BFS start: frontier 0 = {src} (queue and bitmap; both bitmaps cleared
+// before), and the plan of level 0 -> 1.
+__global__ void k_lvl_init_single(uint32_t *c, uint32_t *queue0, uint32_t *bits0, uint32_t src, uint32_t n,
+                                  uint32_t thr, uint32_t push_wave) {
+  for (uint32_t i = 0; i < kLcWords; i++) c[i] = 0;
+  c[lc_cnt(0)] = 1;
+  c[kLcHasQ] = 1;
+  queue0[0] = src;
+  bits0[src >> 5] |= 1u << (src & 31);
+  lvl_plan(c, 0, 1, n, thr, push_wave);
+}
+
+// CC start: control words for a first round that pulls from every vertex
+// into frontier 0 (that round runs before the device levels).
+__global__ void k_lvl_init_all(uint32_t *c) {
+  for (uint32_t i = 0; i < kLcWords; i++) c[i] = 0;
+  c[kLcMode] = kModePull;
+  c[kLcDepth] = 1;
 }
 
 __global__ void k_mark_visited(const uint32_t *q, uint64_t cnt, uint8_t *visited) {
@@ -1691,9 +1937,11 @@ void Traversal::prepare_tree_tasks() {
   }
 }
 
+constexpr uint64_t kPullGrid = 8192;
+
 template <class Op>
 static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierLists &L, const uint32_t *tree_list,
-                        uint32_t ntree, const uint64_t *task_off, uint64_t ntasks) {
+                        uint32_t ntree, const uint64_t *task_off, uint64_t ntasks, Gate gate = Gate{}) {
   hipStream_t s = T.stream();
   GView v = g.view();
   // events: [0] start, [1] after inline, [2] after PMA, [3] after tree
@@ -1703,11 +1951,14 @@ static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierL
     TG_HIP(hipEventRecord(e[0], s));
   }
   const unsigned B = g.tune.pull_block;
-  k_pull_inline<Op><<<grid_for(v.n, B), B, 0, s>>>(v, op);
+  // grids of at most kPullGrid workgroups (the kernels loop): a launch that
+  // its gate turns off costs little
+  k_pull_inline<Op><<<grid_for(v.n, B, kPullGrid), B, 0, s>>>(v, op, gate);
   TG_CHECK_LAUNCH();
   if (T.profiling) TG_HIP(hipEventRecord(e[1], s));
   if (L.npma) {
-    k_pull_pma<Op><<<grid_for(static_cast<uint64_t>(L.npma) * 16, B), B, 0, s>>>(v, op, L.pma, L.npma);
+    k_pull_pma<Op><<<grid_for(static_cast<uint64_t>(L.npma) * 16, B, kPullGrid), B, 0, s>>>(v, op, L.pma, L.npma,
+                                                                                          gate);
     TG_CHECK_LAUNCH();
   }
   if (T.profiling) TG_HIP(hipEventRecord(e[2], s));
@@ -1718,11 +1969,11 @@ static void launch_pull(Traversal &T, Graph &g, const Op &op, const Graph::TierL
       pb.ensure(ntasks * sizeof(typename Op::Acc), s);
       parts = reinterpret_cast<typename Op::Acc *>(pb.get());
     }
-    k_pull_tree<Op><<<grid_for(ntasks * 64, 256, 1u << 20), 256, 0, s>>>(v, op, T.tree_task_vertex(),
-                                                                         T.tree_task_block(), ntasks, parts);
+    k_pull_tree<Op><<<grid_for(ntasks * 64, 256, kPullGrid), 256, 0, s>>>(v, op, T.tree_task_vertex(),
+                                                                         T.tree_task_block(), ntasks, parts, gate);
     TG_CHECK_LAUNCH();
     if (parts) {
-      k_tree_parts<Op><<<grid_for(ntree, 256), 256, 0, s>>>(op, tree_list, ntree, task_off, parts);
+      k_tree_parts<Op><<<grid_for(ntree, 256), 256, 0, s>>>(op, tree_list, ntree, task_off, parts, gate);
       TG_CHECK_LAUNCH();
     }
   }
@@ -1819,17 +2070,20 @@ static void launch_push(Traversal &T, Graph &g, const Op &op, Frontier &in, Fron
     TG_HIP(hipEventRecord(e0, s));
   }
   uint32_t fc = static_cast<uint32_t>(in.count);
-  k_push_inline<Op><<<grid_for(fc, 256), 256, 0, s>>>(v, op, o, in.queue.get(), fc, sp.get(), st.get(), cnt.get());
+  k_push_inline<Op><<<grid_for(fc, 256), 256, 0, s>>>(v, op, o, in.queue.get(), fc, sp.get(), st.get(), cnt.get(),
+                                                      nullptr, Gate{});
   TG_CHECK_LAUNCH();
   if (fc <= g.tune.push_wave) {
-    k_push_pma<Op, 64><<<grid_for(static_cast<uint64_t>(fc) * 64, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
+    k_push_pma<Op, 64><<<grid_for(static_cast<uint64_t>(fc) * 64, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get(),
+                                                                                     Gate{});
   } else {
-    k_push_pma<Op, 16><<<grid_for(static_cast<uint64_t>(fc) * 16, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get());
+    k_push_pma<Op, 16><<<grid_for(static_cast<uint64_t>(fc) * 16, 256), 256, 0, s>>>(v, op, o, sp.get(), cnt.get(),
+                                                                                     Gate{});
   }
   TG_CHECK_LAUNCH();
   // at least 1024 workgroups, so a few frontier hubs spread their blocks
   const unsigned tg = grid_for(static_cast<uint64_t>(fc) * 256, 256, 1u << 16);
-  k_push_tree<Op><<<tg < 1024 ? 1024 : tg, 256, 0, s>>>(v, op, o, st.get(), cnt.get() + 1);
+  k_push_tree<Op><<<tg < 1024 ? 1024 : tg, 256, 0, s>>>(v, op, o, st.get(), cnt.get() + 1, Gate{});
   TG_CHECK_LAUNCH();
   if (T.profiling) {
     TG_HIP(hipEventRecord(e1, s));
@@ -1887,32 +2141,189 @@ void Traversal::edge_map(Frontier &in, Op op, const OpState &st, Frontier *out, 
   else push(in, op, st, out);
 }
 
+// ---------------------------------------------------------------- captured graphs
+void Traversal::CachedGraph::reset() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  key.clear();
+}
+
+Traversal::~Traversal() {
+  for (CachedGraph *c : {&bfs_graph_, &cc_graph_, &pr_graph_, &prs_graph_}) c->reset();
+}
+
+template <class Body>
+hipGraphExec_t Traversal::cached_graph(CachedGraph &cg, const std::vector<uint64_t> &key, const Body &body) {
+  if (cg.exec && cg.key == key) return cg.exec;
+  cg.reset();
+  hipStream_t s = stream_;
+  TG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  TG_HIP(hipStreamEndCapture(s, &cg.graph));
+  hipError_t e = hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    cg.reset();
+    TG_HIP(e);
+  }
+  cg.key = key;
+  return cg.exec;
+}
+
+// The launch parameters a captured traversal depends on besides its own
+// buffers: the graph's arrays and version, the tier lists, the tree tasks.
+static void key_graph(std::vector<uint64_t> &k, Graph &g, const Graph::TierLists &L, const uint32_t *tree_list,
+                      uint32_t ntree, const uint64_t *task_off, uint64_t ntasks, const uint32_t *task_v,
+                      const uint32_t *task_b) {
+  const GView v = g.view();
+  uint64_t w[(sizeof(GView) + 7) / 8] = {};
+  std::memcpy(w, &v, sizeof(GView));
+  k.insert(k.end(), std::begin(w), std::end(w));
+  k.push_back(g.epoch());
+  k.push_back(reinterpret_cast<uint64_t>(L.pma));
+  k.push_back(L.npma);
+  k.push_back(reinterpret_cast<uint64_t>(tree_list));
+  k.push_back(ntree);
+  k.push_back(reinterpret_cast<uint64_t>(task_off));
+  k.push_back(ntasks);
+  k.push_back(reinterpret_cast<uint64_t>(task_v));
+  k.push_back(reinterpret_cast<uint64_t>(task_b));
+  k.push_back(g.tune.pull_block);
+  k.push_back(g.tune.push_wave);
+}
+
+// ---------------------------------------------------------------- device levels
+void Traversal::level_buffers() {
+  hipStream_t s = stream_;
+  const uint32_t n = g_.num_vertices();
+  const uint64_t words = (static_cast<uint64_t>(n) + 31) / 32;
+  lvl_ctl_.ensure(kLcWords, s);
+  for (int i = 0; i < 2; i++) {
+    lvl_q_[i].ensure(static_cast<uint64_t>(n) + 1, s);
+    lvl_bits_[i].ensure(words + 1, s);
+  }
+  lvl_sec_pma_.ensure(static_cast<uint64_t>(n) + 1, s);
+  lvl_sec_tree_.ensure(static_cast<uint64_t>(n) + 1, s);
+  prepare_tree_tasks();  // also builds the tier lists
+}
+
+// One level x -> y (kernel launches only: it may be captured).  Its plan
+// was made by the previous level's last kernel (or the start kernel).  The
+// inline push reads a frontier that a pull wrote from its bitmap.  Five
+// launches a level (push inline, PMA, B-tree; pull; finish), each costing
+// little when its direction is not taken.  fused: the three tier pulls as
+// one launch (BFS: many levels that end early); otherwise one launch per
+// tier (CC: a few heavy rounds, where the inline pass of every vertex
+// before the PMA pass reads best).
+template <class PushOp, class PullOp>
+void Traversal::run_level(uint32_t x, uint32_t y, uint32_t thr, const PushOp &push_op, const PullOp &pull_op,
+                          bool fused) {
+  hipStream_t s = stream_;
+  const uint32_t n = g_.num_vertices();
+  const uint64_t words = (static_cast<uint64_t>(n) + 31) / 32;
+  uint32_t *c = lvl_ctl_.get();
+  const GView v = g_.view();
+  // push: frontiers below n / thr (grids for the largest; the kernels loop
+  // over the counts on the device)
+  const uint64_t most = std::min<uint64_t>(n, n / thr + 1);
+  PushOut o{lvl_bits_[y].get(), lvl_q_[y].get(), c + lc_cnt(y)};
+  k_push_inline<PushOp><<<grid_for(std::max<uint64_t>(most, words), 256, 4096), 256, 0, s>>>(
+      v, push_op, o, lvl_q_[x].get(), 0, lvl_sec_pma_.get(), lvl_sec_tree_.get(), c + kLcSec, c + lc_cnt(x),
+      Gate{c + kLcMode, kModePush}, lvl_bits_[x].get(), words, n, c + kLcBuildQ);
+  TG_CHECK_LAUNCH();
+  const uint64_t wave_most = std::min<uint64_t>(most, g_.tune.push_wave);
+  const unsigned push_pma_grid = std::max(grid_for(wave_most * 64, 256, 4096), grid_for(most * 16, 256, 1u << 16));
+  const unsigned push_tree_grid = 2048;
+  const Graph::TierLists &L = g_.tier_lists();
+  const uint32_t nb_inline = grid_for(n, 256, kPullGrid);
+  const uint32_t nb_pma = L.npma ? grid_for(static_cast<uint64_t>(L.npma) * 16, 256, kPullGrid) : 0;
+  const uint32_t nb_tree = ntree_tasks_ ? grid_for(ntree_tasks_ * 64, 256, kPullGrid) : 0;
+  k_push_pma_dev<PushOp><<<push_pma_grid, 256, 0, s>>>(v, push_op, o, lvl_sec_pma_.get(), c + kLcSec,
+                                                       c + kLcPush64, c + kLcPush16);
+  TG_CHECK_LAUNCH();
+  k_push_tree<PushOp><<<push_tree_grid, 256, 0, s>>>(v, push_op, o, lvl_sec_tree_.get(), c + kLcSec + 1,
+                                                     Gate{c + kLcMode, kModePush});
+  TG_CHECK_LAUNCH();
+  const Gate pull_gate{c + kLcMode, kModePull};
+  if (fused) {
+    k_pull_all<PullOp><<<nb_inline + nb_pma + nb_tree, 256, 0, s>>>(v, pull_op, L.pma, L.npma, ttask_v_.get(),
+                                                                    ttask_b_.get(), ntree_tasks_, nb_inline, nb_pma,
+                                                                    pull_gate);
+    TG_CHECK_LAUNCH();
+  } else {
+    launch_pull(*this, g_, pull_op, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_, pull_gate);
+  }
+  k_lvl_finish<<<grid_for(words, 256, kFinishGrid), 256, 0, s>>>(c, x, y, lvl_bits_[x].get(), lvl_bits_[y].get(),
+                                                                 words, n, thr, g_.tune.push_wave);
+  TG_CHECK_LAUNCH();
+}
+
+template <class Step>
+uint32_t Traversal::run_levels(CachedGraph &cg, const std::vector<uint64_t> &key, const Step &step) {
+  hipStream_t s = stream_;
+  const uint32_t n = g_.num_vertices();
+  hipGraphExec_t exec = g_.tune.level_graph && !profiling ? cached_graph(cg, key, step) : nullptr;
+  uint32_t h[kLcWords];
+  // four levels between reads while the frontier is large, two once it is
+  // small (the traversal is near its end); a level whose frontier is empty
+  // does nothing
+  uint32_t steps = 2;
+  for (uint64_t it = 0;; it++) {
+    for (uint32_t r = 0; r < steps; r++) {
+      if (exec) TG_HIP(hipGraphLaunch(exec, s));
+      else step();
+    }
+    TG_HIP(hipMemcpyAsync(h, lvl_ctl_.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    if (h[kLcCnt] == 0) break;
+    if (it > n) throw Error(-4, "traversal levels did not finish");  // at most n levels
+    steps = h[kLcCnt] < n / 1024 ? 1 : 2;
+  }
+  return h[kLcDepth];
+}
+
 // ---------------------------------------------------------------- BFS
 uint32_t Traversal::bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t threshold) {
   if (!threshold) threshold = g_.tune.thr_bfs;
   hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
   if (src >= n) throw Error(-1, "source out of range");
+  level_buffers();
+  const uint64_t words = (static_cast<uint64_t>(n) + 31) / 32;
   k_fill_i32<<<grid_for(n, 256), 256, 0, s>>>(parent, -1, n);
   TG_CHECK_LAUNCH();
   if (level) fill_u32(level, 0xFFFFFFFFu, n, s);
   k_set_i32<<<1, 1, 0, s>>>(parent, src, static_cast<int32_t>(src));
   if (level) k_set_u32<<<1, 1, 0, s>>>(level, src, 0);
-  Frontier a, b;
-  frontier_single(a, src);
-  Frontier *cur = &a, *nxt = &b;
-  uint32_t depth = 0;
-  while (cur->count > 0) {
-    depth++;
-    OpState st;
-    st.parent = parent;
-    st.level = level;
-    st.depth = depth;
-    edge_map(*cur, Op::BFS, st, nxt, threshold);
-    std::swap(cur, nxt);
-  }
-  TG_HIP(hipStreamSynchronize(s));
-  return depth;
+  fill_u32(lvl_bits_[0].get(), 0, words, s);
+  fill_u32(lvl_bits_[1].get(), 0, words, s);
+  k_lvl_init_single<<<1, 1, 0, s>>>(lvl_ctl_.get(), lvl_q_[0].get(), lvl_bits_[0].get(), src, n, threshold,
+                                    g_.tune.push_wave);
+  TG_CHECK_LAUNCH();
+  uint32_t *c = lvl_ctl_.get();
+  auto step = [&] {
+    for (uint32_t x = 0; x < 2; x++) {
+      const uint32_t y = 1 - x;
+      PushBFS push{parent, level, 0, c + kLcDepth};
+      PullBFS pull{lvl_bits_[x].get(), parent, level, 0, lvl_bits_[y].get(), c + kLcDepth};
+      run_level(x, y, threshold, push, pull, true);
+    }
+  };
+  std::vector<uint64_t> key{reinterpret_cast<uint64_t>(parent), reinterpret_cast<uint64_t>(level), threshold, n};
+  key_graph(key, g_, g_.tier_lists(), tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_, ttask_v_.get(),
+            ttask_b_.get());
+  for (auto *b : {lvl_ctl_.get(), lvl_q_[0].get(), lvl_q_[1].get(), lvl_bits_[0].get(), lvl_bits_[1].get(),
+                  lvl_sec_pma_.get(), lvl_sec_tree_.get()})
+    key.push_back(reinterpret_cast<uint64_t>(b));
+  return run_levels(bfs_graph_, key, step);
 }
 
 // ---------------------------------------------------------------- PR
@@ -1952,23 +2363,19 @@ void Traversal::pagerank(uint32_t iters, double *out) {
   // the rest as a captured pair of iterations (p -> pn -> p), replayed;
   // the pair holds kernel nodes only (no memset or copy nodes)
   if (g_.tune.pr_graph && !profiling && iters - done >= 2) {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    TG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    iterate(p, pn);
-    iterate(pn, p);
-    TG_HIP(hipStreamEndCapture(s, &graph));
-    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      TG_HIP(e);
-    }
+    // kept for later calls with the same buffers on the same graph version
+    std::vector<uint64_t> key{reinterpret_cast<uint64_t>(p), reinterpret_cast<uint64_t>(pn),
+                              reinterpret_cast<uint64_t>(q), n};
+    key_graph(key, g_, g_.tier_lists(), tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_, ttask_v_.get(),
+              ttask_b_.get());
+    key.push_back(reinterpret_cast<uint64_t>(tree_parts_.get()));
+    hipGraphExec_t exec = cached_graph(pr_graph_, key, [&] {
+      iterate(p, pn);
+      iterate(pn, p);
+    });
     uint32_t reps = (iters - done) / 2;
     for (uint32_t r = 0; r < reps; r++) TG_HIP(hipGraphLaunch(exec, s));
     done += 2 * reps;
-    TG_HIP(hipStreamSynchronize(s));
-    TG_HIP(hipGraphExecDestroy(exec));
-    TG_HIP(hipGraphDestroy(graph));
   }
   for (; done < iters; done++) {
     iterate(p, pn);
@@ -2071,21 +2478,17 @@ void Traversal::pagerank_sliced(uint32_t iters, double *out, uint64_t m) {
     }
   };
   if (g_.tune.pr_graph && !profiling && iters >= 2) {
-    // one iteration (pull, fix, reduce: kernel nodes only) captured and replayed
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    TG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    iterate();
-    TG_HIP(hipStreamEndCapture(s, &graph));
-    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      TG_HIP(e);
-    }
+    // one iteration (pull, fix, reduce: kernel nodes only) captured once
+    // per slice build and set of buffers, replayed
+    std::vector<uint64_t> key{reinterpret_cast<uint64_t>(out), reinterpret_cast<uint64_t>(q), n, prs_epoch_, blocks,
+                              prs_ntiles_};
+    for (auto *b : {prs_off_.get(), prs_nbr_.get(), prs_coord_.get(), prs_crow_.get(), prs_deg_.get()})
+      key.push_back(reinterpret_cast<uint64_t>(b));
+    key.push_back(reinterpret_cast<uint64_t>(prs_part_.get()));
+    key.push_back(reinterpret_cast<uint64_t>(prs_cval_.get()));
+    for (uint32_t i = 0; i < kPrsSlices; i++) key.push_back((static_cast<uint64_t>(pt.tiles[i]) << 32) | pt.base[i]);
+    hipGraphExec_t exec = cached_graph(prs_graph_, key, iterate);
     for (uint32_t r = 0; r < iters; r++) TG_HIP(hipGraphLaunch(exec, s));
-    TG_HIP(hipStreamSynchronize(s));
-    TG_HIP(hipGraphExecDestroy(exec));
-    TG_HIP(hipGraphDestroy(graph));
   } else {
     for (uint32_t r = 0; r < iters; r++) iterate();
   }
@@ -2097,40 +2500,39 @@ uint32_t Traversal::cc(uint32_t *ids, uint32_t threshold) {
   if (!threshold) threshold = g_.tune.thr_cc;
   hipStream_t s = stream_;
   uint32_t n = g_.num_vertices();
-  utmp_.ensure(n + 1, s);
-  DevBuf<uint32_t> prev;
-  prev.ensure(n, s);
+  level_buffers();
+  const uint64_t words = (static_cast<uint64_t>(n) + 31) / 32;
+  uint32_t *c = lvl_ctl_.get();
   k_iota<<<grid_for(n, 256), 256, 0, s>>>(ids, n);
   TG_CHECK_LAUNCH();
-  Frontier a, b;
-  frontier_all(a);
-  Frontier *cur = &a, *nxt = &b;
-  uint32_t rounds = 0;
-  while (cur->count > 0) {
-    rounds++;
-    k_copy_u32<<<grid_for(n, 256), 256, 0, s>>>(ids, prev.get(), n);
+  fill_u32(lvl_bits_[0].get(), 0, words, s);
+  fill_u32(lvl_bits_[1].get(), 0, words, s);
+  // the first round pulls from every vertex (Components.h:103: the frontier
+  // starts as all vertices); its output, the vertices whose IDs it lowered,
+  // is frontier 0 of the device levels
+  k_lvl_init_all<<<1, 1, 0, s>>>(c);
+  TG_CHECK_LAUNCH();
+  {
+    PullCC all{nullptr, ids, lvl_bits_[0].get()};
+    launch_pull(*this, g_, all, g_.tier_lists(), tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
+    k_lvl_finish<<<grid_for(words, 256, kFinishGrid), 256, 0, s>>>(c, 1, 0, lvl_bits_[1].get(), lvl_bits_[0].get(),
+                                                                   words, n, threshold, g_.tune.push_wave);
     TG_CHECK_LAUNCH();
-    OpState st;
-    st.ids = ids;
-    bool dense = cur->all || (n / (threshold ? threshold : 1)) <= cur->count;
-    if (dense) {
-      if (!cur->all) to_bits(*cur);
-      const Graph::TierLists &L = g_.tier_lists();
-      prepare_tree_tasks();
-      PullCC p{cur->all ? nullptr : cur->bits.get(), ids};
-      launch_pull(*this, g_, p, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_);
-      frontier_empty(*nxt);
-      nxt->has_queue = false;
-      k_cc_changed<<<grid_for(n, 256), 256, 0, s>>>(ids, prev.get(), n, nxt->bits.get());
-      TG_CHECK_LAUNCH();
-      nxt->count = popcount_bits(*nxt);
-    } else {
-      push(*cur, Op::CC, st, nxt);
-    }
-    std::swap(cur, nxt);
   }
-  TG_HIP(hipStreamSynchronize(s));
-  return rounds;
+  auto step = [&] {
+    for (uint32_t x = 0; x < 2; x++) {
+      PushCC push{ids};
+      PullCC pull{lvl_bits_[x].get(), ids, lvl_bits_[1 - x].get()};
+      run_level(x, 1 - x, threshold, push, pull, false);
+    }
+  };
+  std::vector<uint64_t> key{reinterpret_cast<uint64_t>(ids), threshold, n};
+  key_graph(key, g_, g_.tier_lists(), tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_, ttask_v_.get(),
+            ttask_b_.get());
+  for (auto *b : {lvl_ctl_.get(), lvl_q_[0].get(), lvl_q_[1].get(), lvl_bits_[0].get(), lvl_bits_[1].get(),
+                  lvl_sec_pma_.get(), lvl_sec_tree_.get()})
+    key.push_back(reinterpret_cast<uint64_t>(b));
+  return run_levels(cc_graph_, key, step);
 }
 
 // ---------------------------------------------------------------- BC

@@ -50,6 +50,9 @@ class Traversal {
   // s = null: use the graph's stream.  Traversals with their own streams
   // may run concurrently on one graph while no update is in flight.
   explicit Traversal(Graph &g, hipStream_t s = nullptr) : g_(g), stream_(s ? s : g.stream()) {}
+  ~Traversal();
+  Traversal(const Traversal &) = delete;
+  Traversal &operator=(const Traversal &) = delete;
   hipStream_t stream() const { return stream_; }
 
   // Frontier helpers
@@ -153,6 +156,32 @@ class Traversal {
   void pull(Frontier &in, Op op, const OpState &st, Frontier *out);
   void push(Frontier &in, Op op, const OpState &st, Frontier *out);
   uint64_t popcount_bits(const Frontier &f);
+
+  // A captured HIP graph and the launch parameters it was captured with.
+  struct CachedGraph {
+    std::vector<uint64_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void reset();
+  };
+  CachedGraph bfs_graph_, cc_graph_, pr_graph_, prs_graph_;
+  // The exec of cg, captured again from body (kernel launches on the
+  // traversal's stream) when key differs from the one it was captured for.
+  template <class Body>
+  hipGraphExec_t cached_graph(CachedGraph &cg, const std::vector<uint64_t> &key, const Body &body);
+
+  // BFS / CC levels on device-resident counts: control words, two
+  // frontiers (queue and bitmap each), the push lists.
+  DevBuf<uint32_t> lvl_ctl_, lvl_q_[2], lvl_bits_[2], lvl_sec_pma_, lvl_sec_tree_;
+  void level_buffers();
+  template <class PushOp, class PullOp>
+  void run_level(uint32_t x, uint32_t y, uint32_t thr, const PushOp &push_op, const PullOp &pull_op,
+                 bool fused_pull);
+  // Runs two-level steps (0 -> 1 -> 0) until frontier 0 is empty, replayed
+  // from a HIP graph captured for key when the graph tuning asks for it;
+  // returns the levels that ran.
+  template <class Step>
+  uint32_t run_levels(CachedGraph &cg, const std::vector<uint64_t> &key, const Step &step);
 };
 
 // Frontiers not yet bound to a graph (the reference's VertexSubset(e, n,

@@ -234,7 +234,7 @@ class Graph:
             out[k] = int(out[k])
         return out
 
-    TUNING_KEYS = ("thr_bfs", "thr_ms", "thr_cc", "pull_block", "pr_graph", "pr_sliced", "push_wave")
+    TUNING_KEYS = ("thr_bfs", "thr_ms", "thr_cc", "pull_block", "pr_graph", "pr_sliced", "push_wave", "level_graph")
 
     def get_tuning(self) -> dict:
         t = np.zeros(len(self.TUNING_KEYS), np.uint32)
