@@ -201,93 +201,75 @@ def cpu_baseline(quick: bool, batch: int):
             "sample": "oracle set-model update, " + sample}
 
 
-def serve_bench(args):
-    """Mixed request stream through the plain and the batched server.
+SERVE_PROBS = (0.57, 0.19, 0.19)
+SERVE_SUMMARY_REQUESTS = 60  # per rate in the default run's serving summary
 
-    rMAT (skewed a=.57 b=.19 c=.19) graph built on the device.  R requests
-    arrive as a Poisson process: 30% update batches (10..1e6 edges,
-    log-uniform, 90% inserts of fresh rMAT edges / 10% deletes of existing
-    edges, both directions), 70% queries (60% BFS, 20% CC, 10% PageRank 10
-    iterations FP64, 10% local triangle count).  The stream is replayed one
-    request at a time (plain) and with continuous batching (snapshot at
-    admission).  Every batched result is then checked against the query run
-    alone on the version it reports: the plain server replays the batched
-    server's order (each update at its version, each query after the updates
-    of its version) on a fresh copy of the graph."""
-    from terrace_amd import graph as tg
 
-    os.environ.setdefault("TG_SERVE_PROGRESS", "1")
-    n = 1 << args.scale
-    probs = (0.57, 0.19, 0.19)
-    seed = aspen(100) & 0xFFFFFFFF
-    rng = np.random.default_rng(2024)
-
-    def build():
-        G = tg.Graph(n)
-        s, d, w = tg.rmat_device(n, seed, *probs, args.edges, nominal=True, symmetric=True, drop_loops=True)
-        G.add_edge_batch_dev(s, d, w)
-        del s, d
-        tg.lib().tg_device_sync()
-        return G
-
-    log(f"serve: building scale {args.scale} skewed graph, {args.edges} generated edges")
-    t0 = time.perf_counter()
-    G1 = build()
-    m_directed = G1.get_num_edges()
-    log(f"serve: graph built in {time.perf_counter() - t0:.2f} s, {m_directed} edges, "
-        f"{G1.get_size() / 2**30:.1f} GiB, tiers {G1.tier_counts()}")
-    off, nbr = G1.export_adjacency()
+def serve_requests(tg, n, G, R, rng):
+    """R requests of the serving mix on graph G's current key set: 30%
+    update batches (10..1e6 edges, log-uniform, 90% inserts of fresh rMAT
+    edges / 10% deletes of existing edges, both directions), 70% queries
+    (60% BFS, 20% CC, 10% PageRank 10 iterations FP64, 10% local triangle
+    count).  Arrival times are set per stream (serve_stream)."""
+    off, nbr = G.export_adjacency()
     m = len(nbr)
-    deg = (off[1:] - off[:-1])
+    deg = off[1:] - off[:-1]
     active = np.nonzero(deg)[0].astype(np.uint32)
-
-    # request stream
-    R = args.requests
-    gaps = rng.exponential(1.0 / args.rate, R)
-    arrivals = np.cumsum(gaps)
     reqs = []
     t_gen = 0
-    for i in range(R):
+    for _ in range(R):
         u = rng.random()
         if u < 0.3:
             size = int(round(10 ** rng.uniform(1, 6)))
             npairs = max(1, int(size * 0.9) // 2)
-            s, d, w = tg.rmat_device(n, aspen(7000 + t_gen) & 0xFFFFFFFF, *probs, npairs, nominal=True,
+            s, d, w = tg.rmat_device(n, aspen(7000 + t_gen) & 0xFFFFFFFF, *SERVE_PROBS, npairs, nominal=True,
                                      symmetric=True, drop_loops=True)
             t_gen += 1
             ins = (s.download(np.uint32, w), d.download(np.uint32, w))
+            s.close(), d.close()
             nd = max(1, int(size * 0.1) // 2)
             idx = rng.integers(0, m, nd)
             ds = (np.searchsorted(off, idx, side="right") - 1).astype(np.uint32)
             dd = nbr[idx]
             dels = (np.concatenate([ds, dd]), np.concatenate([dd, ds]))
-            reqs.append(("U", arrivals[i], ins, dels))
+            reqs.append(("U", ins, dels))
         else:
             q = rng.random()
             v = int(active[rng.integers(0, len(active))])
             if q < 0.6:
-                reqs.append(("BFS", arrivals[i], v))
+                reqs.append(("BFS", v))
             elif q < 0.8:
-                reqs.append(("CC", arrivals[i]))
+                reqs.append(("CC",))
             elif q < 0.9:
-                reqs.append(("PR", arrivals[i]))
+                reqs.append(("PR",))
             else:
-                reqs.append(("LTC", arrivals[i], v))
-    del off, nbr, deg, active
+                reqs.append(("LTC", v))
+    return reqs
+
+
+def serve_stream(tg, build, reqs, arrivals, budget, verify=True):
+    """The stream (requests with their arrival times) through the plain
+    server (one request at a time, in arrival order) and the batched one
+    (continuous batching, snapshot at admission), each on a fresh copy of
+    the graph; then, when verify is set, every batched result against the
+    same request run alone on the version it reports (the plain server
+    replaying the batched server's order on a third copy)."""
+    R = len(reqs)
 
     def submit(S, r, arrival):
         if r[0] == "U":
-            return S.submit(tg.REQ_UPDATE, arrival, ins=r[2], dels=r[3])
+            return S.submit(tg.REQ_UPDATE, arrival, ins=r[1], dels=r[2])
         if r[0] == "BFS":
-            return S.submit(tg.REQ_BFS, arrival, src=r[2])
+            return S.submit(tg.REQ_BFS, arrival, src=r[1])
         if r[0] == "CC":
             return S.submit(tg.REQ_CC, arrival)
         if r[0] == "PR":
             return S.submit(tg.REQ_PR, arrival, iters=10)
-        return S.submit(tg.REQ_LTC, arrival, src=r[2])
+        return S.submit(tg.REQ_LTC, arrival, src=r[1])
 
-    def replay(G, mode, stream):
-        S = tg.Server(G, mode=mode, mem_budget=args.budget << 30)
+    def replay(mode, stream):
+        G = build()
+        S = tg.Server(G, mode=mode, mem_budget=budget << 30)
         ids = [submit(S, r, a) for r, a in stream]
         t0 = time.perf_counter()
         S.run()
@@ -296,25 +278,15 @@ def serve_bench(args):
         summ = S.summary()
         summ["wall_s"] = wall
         S.close()
+        G.close()
         return summ, res
 
-    log(f"serve: replaying {R} requests at {args.rate}/s, plain")
-    sp, rp = replay(G1, "plain", [(r, r[1]) for r in reqs])
-    G1.close()
-    log(f"serve: plain {sp}")
-    G2 = build()
-    log("serve: batched (snapshot at admission)")
-    sb, rb = replay(G2, "admission", [(r, r[1]) for r in reqs])
-    G2.close()
-    log(f"serve: batched {sb}")
+    sp, rp = replay("plain", list(zip(reqs, arrivals)))
+    sb, rb = replay("admission", list(zip(reqs, arrivals)))
     mism = verified = 0
-    if not args.no_verify:
-        # the batched server's linearization, replayed one request at a time
+    if verify:
         order = sorted(range(R), key=lambda i: (rb[i]["version"], 0 if reqs[i][0] == "U" else 1, i))
-        G3 = build()
-        log("serve: verification replay (each query alone on its version)")
-        sv, rv = replay(G3, "plain", [(reqs[i], 0.0) for i in order])
-        G3.close()
+        _, rv = replay("plain", [(reqs[i], 0.0) for i in order])
         for pos, i in enumerate(order):
             r, a, b = reqs[i], rv[pos], rb[i]
             verified += 1
@@ -326,18 +298,95 @@ def serve_bench(args):
                 mism += 1
             elif r[0] == "PR" and abs(a["fvalue"] - b["fvalue"]) > 1e-9 * max(abs(a["fvalue"]), 1e-300):
                 mism += 1
-        log(f"serve: verified {verified} results, {mism} mismatches")
-    # queries admitted later than their arrival-order version
     ahead = sum(1 for i in range(R) if reqs[i][0] != "U" and rb[i]["version"] != rp[i]["version"])
+    keep = ("requests_per_s", "p50_s", "p99_s", "p50_query_s", "p99_query_s", "makespan_s", "groups", "rejected",
+            "wall_s")
+    return {"plain": {k: sp[k] for k in keep if k in sp}, "batched": {k: sb[k] for k in keep if k in sb},
+            "throughput_x": sb["requests_per_s"] / max(sp["requests_per_s"], 1e-9),
+            "p99_ratio": sb["p99_s"] / max(sp["p99_s"], 1e-12),
+            "verified": verified, "result_mismatches": mism, "queries_on_later_version": ahead}
+
+
+def serve_graph_builder(tg, scale, edges):
+    n = 1 << scale
+    seed = aspen(100) & 0xFFFFFFFF
+
+    def build():
+        G = tg.Graph(n)
+        s, d, w = tg.rmat_device(n, seed, *SERVE_PROBS, edges, nominal=True, symmetric=True, drop_loops=True)
+        G.add_edge_batch_dev(s, d, w)
+        s.close(), d.close()
+        tg.lib().tg_device_sync()
+        return G
+    return build
+
+
+def serve_summary(tg, scale, edges, R, budget, rng):
+    """Bounded serving comparison for the default run's extra: R requests
+    of the serving mix at overload (200/s), then at 80% of the plain
+    server's capacity measured in that run (its requests/s), each verified
+    against the alone-on-version replay."""
+    build = serve_graph_builder(tg, scale, edges)
+    G = build()
+    m_directed = G.get_num_edges()
+    reqs = serve_requests(tg, 1 << scale, G, R, rng)
+    G.close()
+    gaps = rng.exponential(1.0, R)
+    out = {"workload": f"rmat scale {scale} skewed (.57/.19/.19), {edges} generated edges "
+                       f"({m_directed} directed); {R} requests: 30% updates of 10..1e6 edges, "
+                       "70% queries (BFS/CC/PR/LTC)",
+           "kinds": {k: sum(1 for r in reqs if r[0] == k) for k in ("U", "BFS", "CC", "PR", "LTC")}}
+    log(f"serve summary: {R} requests at 200/s")
+    over = serve_stream(tg, build, reqs, np.cumsum(gaps / 200.0), budget)
+    out["overload_200_per_s"] = over
+    rate = 0.8 * over["plain"]["requests_per_s"]
+    log(f"serve summary: {R} requests at {rate:.1f}/s (80% of the plain capacity)")
+    near = serve_stream(tg, build, reqs, np.cumsum(gaps / rate), budget)
+    near["rate_per_s"] = rate
+    out["near_capacity"] = near
+    log(f"serve summary: overload x{over['throughput_x']:.2f} p99 ratio {over['p99_ratio']:.3f}; "
+        f"near capacity x{near['throughput_x']:.2f} p99 ratio {near['p99_ratio']:.3f}; mismatches "
+        f"{over['result_mismatches'] + near['result_mismatches']}")
+    return out
+
+
+def serve_bench(args):
+    """Mixed request stream through the plain and the batched server.
+
+    rMAT (skewed a=.57 b=.19 c=.19) graph built on the device; R requests of
+    the serving mix (serve_requests) arrive as a Poisson process at the
+    given rate.  The stream is replayed one request at a time (plain) and
+    with continuous batching (snapshot at admission); every batched result
+    is then checked against the query run alone on the version it reports
+    (serve_stream)."""
+    from terrace_amd import graph as tg
+
+    os.environ.setdefault("TG_SERVE_PROGRESS", "1")
+    n = 1 << args.scale
+    rng = np.random.default_rng(2024)
+    build = serve_graph_builder(tg, args.scale, args.edges)
+    log(f"serve: building scale {args.scale} skewed graph, {args.edges} generated edges")
+    t0 = time.perf_counter()
+    G1 = build()
+    m_directed = G1.get_num_edges()
+    log(f"serve: graph built in {time.perf_counter() - t0:.2f} s, {m_directed} edges, "
+        f"{G1.get_size() / 2**30:.1f} GiB, tiers {G1.tier_counts()}")
+    R = args.requests
+    reqs = serve_requests(tg, n, G1, R, rng)
+    G1.close()
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, R))
+    log(f"serve: {R} requests at {args.rate}/s")
+    res = serve_stream(tg, build, reqs, arrivals, args.budget, verify=not args.no_verify)
+    log(f"serve: plain {res['plain']}, batched {res['batched']}, {res['result_mismatches']} mismatches")
     kinds = {k: sum(1 for r in reqs if r[0] == k) for k in ("U", "BFS", "CC", "PR", "LTC")}
     out = {
         "metric": "mixed-stream requests/s (batched server)",
-        "value": sb["requests_per_s"],
+        "value": res["batched"]["requests_per_s"],
         "unit": "requests/s",
         "n_gpus": 1,
         "steps": R,
         "warmup": 0,
-        "ms_per_step": sb["makespan_s"] / R * 1e3,
+        "ms_per_step": res["batched"]["makespan_s"] / R * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -346,11 +395,7 @@ def serve_bench(args):
         "config": {"workload": f"serving: rmat scale {args.scale} skewed (.57/.19/.19), {args.edges} generated "
                                f"edges; {R} requests at {args.rate}/s", "vertices": n,
                    "directed_edges": m_directed, "requests": kinds, "parallelism": "single GPU"},
-        "serve": {"plain": sp, "batched": sb, "mode": "snapshot at admission",
-                  "throughput_x": sb["requests_per_s"] / max(sp["requests_per_s"], 1e-9),
-                  "p99_ratio": sb["p99_s"] / max(sp["p99_s"], 1e-12),
-                  "verified": verified, "result_mismatches": mism,
-                  "queries_on_later_version": ahead},
+        "serve": dict(res, mode="snapshot at admission"),
     }
     print(json.dumps(out))
 
@@ -709,6 +754,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="request arrivals per second")
     ap.add_argument("--budget", type=int, default=160, help="server memory budget, GiB")
     ap.add_argument("--no-verify", action="store_true", help="serve: skip the verification replay")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="default run: skip the bounded serving summary (scale 26) in extra")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -890,10 +937,26 @@ def main():
         _check(tg.lib().tg_cc_dev(G.handle, cc_d.ptr, ctypes.byref(rounds)))
         extra["cc_ms"] = (time.perf_counter() - t0) * 1e3
         # PageRank with its iterations replayed from a captured HIP graph
-        # (the profiled run above launches them one by one)
-        t0 = time.perf_counter()
-        _check(tg.lib().tg_pagerank_dev(G.handle, 10, pr_d.ptr))
-        extra["pr_graph_replay_ms"] = (time.perf_counter() - t0) * 1e3
+        # (the profiled run above launches them one by one; the graph is
+        # captured on the second call on this graph version, replayed after)
+        for _ in range(2):
+            t0 = time.perf_counter()
+            _check(tg.lib().tg_pagerank_dev(G.handle, 10, pr_d.ptr))
+            extra["pr_graph_replay_ms"] = (time.perf_counter() - t0) * 1e3
+        # BFS and CC again on the same version: levels replayed from their
+        # captured two-level graphs (the first calls above launch them one
+        # by one); best of three
+        for name, fn in (("bfs_replay_ms", lambda: tg.lib().tg_bfs_dev(G.handle, src, par_d.ptr, lev_d.ptr,
+                                                                        ctypes.byref(depth))),
+                         ("cc_replay_ms", lambda: tg.lib().tg_cc_dev(G.handle, cc_d.ptr, ctypes.byref(rounds)))):
+            ts = []
+            for _ in range(4):
+                tg.lib().tg_device_sync()
+                t0 = time.perf_counter()
+                _check(fn())
+                ts.append((time.perf_counter() - t0) * 1e3)
+            extra[name] = min(ts[1:])
+        extra["bfs_replay_gteps"] = bfs_edges / (extra["bfs_replay_ms"] * 1e-3) / 1e9
         for b_ in (par_d, lev_d, pr_d, cc_d):
             b_.close()
         for bsz in (() if args.no_extra_batches else (1000, 10_000_000)):
@@ -946,6 +1009,13 @@ def main():
         extra["memory_bytes"] = G.get_size()
         extra["stats"] = G.stats()
 
+    if not args.no_extra and not args.no_serve and default_workload:
+        # the serving workload (scale 26 skewed) at a bounded size, beside
+        # the headline: plain versus continuously batched, near capacity and
+        # at overload, every result verified
+        G.close()
+        extra["serving"] = serve_summary(tg, 26, 775_000_000, SERVE_SUMMARY_REQUESTS, args.budget,
+                                         np.random.default_rng(2025))
     if args.copy_probe:
         copy_probe()
     cpu = None if args.no_cpu else cpu_baseline(args.quick, args.batch)

@@ -2157,6 +2157,13 @@ Traversal::~Traversal() {
 template <class Body>
 hipGraphExec_t Traversal::cached_graph(CachedGraph &cg, const std::vector<uint64_t> &key, const Body &body) {
   if (cg.exec && cg.key == key) return cg.exec;
+  // capture on the second call with the same parameters: a graph version
+  // that is queried once (an update between every two queries) never pays
+  // for a capture it would not replay
+  if (cg.seen != key) {
+    cg.seen = key;
+    return nullptr;
+  }
   cg.reset();
   hipStream_t s = stream_;
   TG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -2374,7 +2381,14 @@ void Traversal::pagerank(uint32_t iters, double *out) {
       iterate(pn, p);
     });
     uint32_t reps = (iters - done) / 2;
-    for (uint32_t r = 0; r < reps; r++) TG_HIP(hipGraphLaunch(exec, s));
+    for (uint32_t r = 0; r < reps; r++) {
+      if (exec) {
+        TG_HIP(hipGraphLaunch(exec, s));
+      } else {
+        iterate(p, pn);
+        iterate(pn, p);
+      }
+    }
     done += 2 * reps;
   }
   for (; done < iters; done++) {
@@ -2488,7 +2502,10 @@ void Traversal::pagerank_sliced(uint32_t iters, double *out, uint64_t m) {
     key.push_back(reinterpret_cast<uint64_t>(prs_cval_.get()));
     for (uint32_t i = 0; i < kPrsSlices; i++) key.push_back((static_cast<uint64_t>(pt.tiles[i]) << 32) | pt.base[i]);
     hipGraphExec_t exec = cached_graph(prs_graph_, key, iterate);
-    for (uint32_t r = 0; r < iters; r++) TG_HIP(hipGraphLaunch(exec, s));
+    for (uint32_t r = 0; r < iters; r++) {
+      if (exec) TG_HIP(hipGraphLaunch(exec, s));
+      else iterate();
+    }
   } else {
     for (uint32_t r = 0; r < iters; r++) iterate();
   }

@@ -159,14 +159,17 @@ class Traversal {
 
   // A captured HIP graph and the launch parameters it was captured with.
   struct CachedGraph {
-    std::vector<uint64_t> key;
+    std::vector<uint64_t> key;   // what exec was captured for
+    std::vector<uint64_t> seen;  // the parameters of the last call
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void reset();
   };
   CachedGraph bfs_graph_, cc_graph_, pr_graph_, prs_graph_;
   // The exec of cg, captured again from body (kernel launches on the
-  // traversal's stream) when key differs from the one it was captured for.
+  // traversal's stream) when key differs from the one it was captured for
+  // and equals the previous call's; null (the caller launches body's
+  // kernels itself) on the first call with a key.
   template <class Body>
   hipGraphExec_t cached_graph(CachedGraph &cg, const std::vector<uint64_t> &key, const Body &body);
 
