@@ -760,9 +760,11 @@ constexpr uint32_t kLcDepth = 7;   // levels run
 constexpr uint32_t kLcPush64 = 8;  // 1: this push gives each PMA vertex a whole wave
 constexpr uint32_t kLcPush16 = 9;  // 1: this push gives each PMA vertex 16 lanes
 constexpr uint32_t kLcBuildQ = 10; // 1: the push reads the frontier from its bitmap (no queue)
+constexpr uint32_t kLcNlev = 11;   // BC: levels listed so far
 constexpr uint32_t kLcCnt = 32;    // [32], [64]: frontier sizes (kLcCnt + kLcStride * x)
 constexpr uint32_t kLcStride = 32;
 constexpr uint32_t kLcSec = 96;    // [96], [97]: push lists of PMA and B-tree vertices
+constexpr uint32_t kLcApp = 104;   // BC: level-list append cursor
 constexpr uint32_t kLcDone = 112;  // workgroups of k_lvl_finish done
 constexpr uint32_t kLcWords = 128;
 __host__ __device__ constexpr uint32_t lc_cnt(uint32_t x) { return kLcCnt + kLcStride * x; }
@@ -834,6 +836,7 @@ __device__ inline void lvl_plan(uint32_t *c, uint32_t x, uint32_t y, uint32_t n,
   c[kLcPush64] = mode == kModePush && cnt <= push_wave ? 1u : 0u;
   c[kLcPush16] = mode == kModePush && cnt > push_wave ? 1u : 0u;
   c[kLcBuildQ] = mode == kModePush && c[kLcHasQ + x] == 0 ? 1u : 0u;
+  c[kLcApp] = 0;
   if (mode) c[kLcDepth]++;
 }
 
@@ -877,6 +880,96 @@ __global__ void __launch_bounds__(256) k_lvl_finish(uint32_t *c, uint32_t x, uin
 }
 
 constexpr uint64_t kFinishGrid = 128;  // few workgroups: the last one plans the next level
+
+// The same end of a BC forward level (BC.h:120-131), which also lists the
+// level: the new frontier's vertices are appended to lv at loff[nlev] (a
+// push's queue in its order, a pull's bitmap in word order) and marked
+// visited, and the last workgroup closes the list (loff[nlev + 1], nlev + 1)
+// before it plans the next level.
+__global__ void __launch_bounds__(256) k_lvl_finish_bc(uint32_t *c, uint32_t x, uint32_t y, uint32_t *xbits,
+                                                       const uint32_t *ybits, const uint32_t *yqueue, uint64_t words,
+                                                       uint32_t n, uint32_t thr, uint32_t push_wave, uint32_t *lv,
+                                                       uint64_t *loff, uint8_t *visited) {
+  __shared__ uint32_t part[4];
+  __shared__ bool last;
+  const uint32_t mode = c[kLcMode];
+  const bool pulled = mode == kModePull;
+  const uint64_t base = loff[c[kLcNlev]];
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t sum = 0;
+  for (uint64_t w0 = blockIdx.x * static_cast<uint64_t>(blockDim.x); w0 < words;
+       w0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t wi = w0 + threadIdx.x;
+    uint32_t v = 0;
+    if (wi < words) {
+      if (pulled) {
+        v = ybits[wi];
+        if ((wi + 1) * 32 > n) {
+          const uint32_t keep = n - static_cast<uint32_t>(wi * 32);
+          v &= keep >= 32 ? 0xFFFFFFFFu : ((1u << keep) - 1u);
+        }
+      }
+      xbits[wi] = 0;
+    }
+    const uint32_t k = __popc(v);
+    sum += k;
+    uint32_t incl = k;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    uint32_t at = 0;
+    if (lane == 63 && incl) at = atomicAdd(&c[kLcApp], incl);
+    at = __shfl(at, 63, 64);
+    uint64_t pos = base + at + incl - k;
+    while (v) {
+      const uint32_t u = static_cast<uint32_t>(wi * 32 + (__ffs(v) - 1));
+      lv[pos++] = u;
+      visited[u] = 1;
+      v &= v - 1;
+    }
+  }
+  if (mode == kModePush) {
+    const uint32_t cy = c[lc_cnt(y)];
+    GRID_STRIDE(i, cy) {
+      const uint32_t u = yqueue[i];
+      lv[base + i] = u;
+      visited[u] = 1;
+    }
+  }
+  for (uint32_t o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(&c[lc_cnt(y)], t);
+    __threadfence();
+    last = atomicAdd(&c[kLcDone], 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    c[kLcDone] = 0;
+    const uint32_t cy = __hip_atomic_load(&c[lc_cnt(y)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (mode != 0 && cy != 0) {
+      const uint32_t nlev = c[kLcNlev];
+      loff[nlev + 1] = base + cy;
+      c[kLcNlev] = nlev + 1;
+    }
+    lvl_plan(c, y, x, n, thr, push_wave);
+  }
+}
+
+// BC start: the source as level 0 (lv[0], loff = {0, 1}), visited, one path.
+__global__ void k_lvl_init_bc(uint32_t *c, uint32_t *lv, uint64_t *loff, uint8_t *visited, double *paths,
+                              uint32_t src) {
+  c[kLcNlev] = 1;
+  lv[0] = src;
+  loff[0] = 0;
+  loff[1] = 1;
+  visited[src] = 1;
+  paths[src] = 1.0;
+}
 
 // BFS start: frontier 0 = {src} (queue and bitmap; both bitmaps cleared
 // before), and the plan of level 0 -> 1.
@@ -2151,7 +2244,7 @@ void Traversal::CachedGraph::reset() {
 }
 
 Traversal::~Traversal() {
-  for (CachedGraph *c : {&bfs_graph_, &cc_graph_, &pr_graph_, &prs_graph_}) c->reset();
+  for (CachedGraph *c : {&bfs_graph_, &cc_graph_, &bc_graph_, &pr_graph_, &prs_graph_}) c->reset();
 }
 
 template <class Body>
@@ -2232,7 +2325,7 @@ void Traversal::level_buffers() {
 // before the PMA pass reads best).
 template <class PushOp, class PullOp>
 void Traversal::run_level(uint32_t x, uint32_t y, uint32_t thr, const PushOp &push_op, const PullOp &pull_op,
-                          bool fused) {
+                          bool fused, uint32_t *bc_lv, uint64_t *bc_loff, uint8_t *bc_visited) {
   hipStream_t s = stream_;
   const uint32_t n = g_.num_vertices();
   const uint64_t words = (static_cast<uint64_t>(n) + 31) / 32;
@@ -2260,16 +2353,24 @@ void Traversal::run_level(uint32_t x, uint32_t y, uint32_t thr, const PushOp &pu
                                                      Gate{c + kLcMode, kModePush});
   TG_CHECK_LAUNCH();
   const Gate pull_gate{c + kLcMode, kModePull};
-  if (fused) {
-    k_pull_all<PullOp><<<nb_inline + nb_pma + nb_tree, 256, 0, s>>>(v, pull_op, L.pma, L.npma, ttask_v_.get(),
-                                                                    ttask_b_.get(), ntree_tasks_, nb_inline, nb_pma,
-                                                                    pull_gate);
-    TG_CHECK_LAUNCH();
-  } else {
-    launch_pull(*this, g_, pull_op, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_, pull_gate);
+  bool fused_done = false;
+  if constexpr (!PullOp::kOrdered) {  // ordered (FP64) tree sums take the separate launches
+    if (fused) {
+      k_pull_all<PullOp><<<nb_inline + nb_pma + nb_tree, 256, 0, s>>>(v, pull_op, L.pma, L.npma, ttask_v_.get(),
+                                                                      ttask_b_.get(), ntree_tasks_, nb_inline,
+                                                                      nb_pma, pull_gate);
+      TG_CHECK_LAUNCH();
+      fused_done = true;
+    }
   }
-  k_lvl_finish<<<grid_for(words, 256, kFinishGrid), 256, 0, s>>>(c, x, y, lvl_bits_[x].get(), lvl_bits_[y].get(),
-                                                                 words, n, thr, g_.tune.push_wave);
+  if (!fused_done) launch_pull(*this, g_, pull_op, L, tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_, pull_gate);
+  if (bc_lv)
+    k_lvl_finish_bc<<<grid_for(words, 256, kFinishGrid), 256, 0, s>>>(
+        c, x, y, lvl_bits_[x].get(), lvl_bits_[y].get(), lvl_q_[y].get(), words, n, thr, g_.tune.push_wave, bc_lv,
+        bc_loff, bc_visited);
+  else
+    k_lvl_finish<<<grid_for(words, 256, kFinishGrid), 256, 0, s>>>(c, x, y, lvl_bits_[x].get(), lvl_bits_[y].get(),
+                                                                   words, n, thr, g_.tune.push_wave);
   TG_CHECK_LAUNCH();
 }
 
@@ -2565,45 +2666,63 @@ void Traversal::bc(uint32_t src, double *out, uint32_t threshold) {
   uint8_t *visited = btmp_.get();
   fill_f64(paths, 0.0, n, s);
   TG_HIP(hipMemsetAsync(visited, 0, n, s));
-  k_set_u8<<<1, 1, 0, s>>>(visited, src, 1);
-  k_set_f64<<<1, 1, 0, s>>>(paths, src, 1.0);
-  // level lists, concatenated
-  DevBuf<uint32_t> lv;
-  lv.ensure(static_cast<uint64_t>(n) + 1, s);
-  std::vector<uint64_t> loff{0};
-  k_set_u32<<<1, 1, 0, s>>>(lv.get(), 0, src);
-  loff.push_back(1);
-  Frontier a, b;
-  frontier_single(a, src);
-  Frontier *cur = &a, *nxt = &b;
-  while (true) {
-    OpState st;
-    st.val = paths;
-    st.visited = visited;
-    edge_map(*cur, Op::BC_FWD, st, nxt, threshold);
-    if (nxt->count == 0) break;
-    to_queue(*nxt);
-    TG_HIP(hipMemcpyAsync(lv.get() + loff.back(), nxt->queue.get(), nxt->count * sizeof(uint32_t),
-                          hipMemcpyDeviceToDevice, s));
-    loff.push_back(loff.back() + nxt->count);
-    k_mark_visited<<<grid_for(nxt->count, 256), 256, 0, s>>>(nxt->queue.get(), nxt->count, visited);
-    TG_CHECK_LAUNCH();
-    std::swap(cur, nxt);
-  }
+  // forward (BC.h:110-131): levels on device-resident counts, each level's
+  // vertices listed in lv (level r = lv[loff[r] .. loff[r + 1]))
+  level_buffers();
+  bc_lv_.ensure(static_cast<uint64_t>(n) + 1, s);
+  bc_loff_.ensure(static_cast<uint64_t>(n) + 2, s);
+  tree_parts_.ensure(ntree_tasks_ * sizeof(SumAcc) + 16, s);  // the ordered tree pull's partials
+  const uint64_t words = (static_cast<uint64_t>(n) + 31) / 32;
+  fill_u32(lvl_bits_[0].get(), 0, words, s);
+  fill_u32(lvl_bits_[1].get(), 0, words, s);
+  k_lvl_init_single<<<1, 1, 0, s>>>(lvl_ctl_.get(), lvl_q_[0].get(), lvl_bits_[0].get(), src, n, threshold,
+                                    g_.tune.push_wave);
+  TG_CHECK_LAUNCH();
+  k_lvl_init_bc<<<1, 1, 0, s>>>(lvl_ctl_.get(), bc_lv_.get(), bc_loff_.get(), visited, paths, src);
+  TG_CHECK_LAUNCH();
+  auto step = [&] {
+    for (uint32_t x = 0; x < 2; x++) {
+      const uint32_t y = 1 - x;
+      PushSum push{paths, visited};
+      PullSum pull{lvl_bits_[x].get(), paths, paths, visited, lvl_bits_[y].get(), false};
+      run_level(x, y, threshold, push, pull, false, bc_lv_.get(), bc_loff_.get(), visited);
+    }
+  };
+  std::vector<uint64_t> key{reinterpret_cast<uint64_t>(paths), reinterpret_cast<uint64_t>(visited), threshold, n,
+                            reinterpret_cast<uint64_t>(bc_lv_.get()), reinterpret_cast<uint64_t>(bc_loff_.get()),
+                            reinterpret_cast<uint64_t>(tree_parts_.get())};
+  key_graph(key, g_, g_.tier_lists(), tree_list_, ntree_list_, ttask_off_.get(), ntree_tasks_, ttask_v_.get(),
+            ttask_b_.get());
+  for (auto *b : {lvl_ctl_.get(), lvl_q_[0].get(), lvl_q_[1].get(), lvl_bits_[0].get(), lvl_bits_[1].get(),
+                  lvl_sec_pma_.get(), lvl_sec_tree_.get()})
+    key.push_back(reinterpret_cast<uint64_t>(b));
+  run_levels(bc_graph_, key, step);
+  // the level offsets (one read), then the backward pass with counts known
+  uint32_t hc[kLcWords];
+  TG_HIP(hipMemcpyAsync(hc, lvl_ctl_.get(), sizeof(hc), hipMemcpyDeviceToHost, s));
+  TG_HIP(hipStreamSynchronize(s));
+  const uint32_t nl = hc[kLcNlev];
+  std::vector<uint64_t> loff(nl + 1);
+  TG_HIP(hipMemcpyAsync(loff.data(), bc_loff_.get(), (nl + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  TG_HIP(hipStreamSynchronize(s));
+  DevBuf<uint32_t> &lv = bc_lv_;
   size_t nlev = loff.size() - 1;  // non-empty levels
   k_recip<<<grid_for(n, 256), 256, 0, s>>>(paths, n);
   TG_CHECK_LAUNCH();
   TG_HIP(hipMemsetAsync(visited, 0, n, s));
   fill_f64(dep, 0.0, n, s);
+  // backward (BC.h:133-150), over the listed levels: counts known on the
+  // host, so no level waits for a read
+  Frontier &f = bc_back_;
+  f.n = n;
+  f.all = false;
   for (size_t r = nlev; r-- > 0;) {
     if (r + 1 < nlev) {
-      Frontier f;
-      f.n = n;
       f.queue.ensure(loff[r + 2] - loff[r + 1] + 1, s);
-      TG_HIP(hipMemcpyAsync(f.queue.get(), lv.get() + loff[r + 1], (loff[r + 2] - loff[r + 1]) * sizeof(uint32_t),
-                            hipMemcpyDeviceToDevice, s));
+      copy_bytes(f.queue.get(), lv.get() + loff[r + 1], (loff[r + 2] - loff[r + 1]) * sizeof(uint32_t), s);
       f.count = loff[r + 2] - loff[r + 1];
       f.has_queue = true;
+      f.has_bits = false;
       OpState st;
       st.val = dep;
       st.visited = visited;

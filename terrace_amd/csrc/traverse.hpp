@@ -165,7 +165,7 @@ class Traversal {
     hipGraphExec_t exec = nullptr;
     void reset();
   };
-  CachedGraph bfs_graph_, cc_graph_, pr_graph_, prs_graph_;
+  CachedGraph bfs_graph_, cc_graph_, bc_graph_, pr_graph_, prs_graph_;
   // The exec of cg, captured again from body (kernel launches on the
   // traversal's stream) when key differs from the one it was captured for
   // and equals the previous call's; null (the caller launches body's
@@ -176,10 +176,17 @@ class Traversal {
   // BFS / CC levels on device-resident counts: control words, two
   // frontiers (queue and bitmap each), the push lists.
   DevBuf<uint32_t> lvl_ctl_, lvl_q_[2], lvl_bits_[2], lvl_sec_pma_, lvl_sec_tree_;
+  // BC: the forward levels' vertex lists and offsets, the backward frontier
+  DevBuf<uint32_t> bc_lv_;
+  DevBuf<uint64_t> bc_loff_;
+  Frontier bc_back_;
   void level_buffers();
+  // bc_*: a BC forward level (its vertices listed into bc_lv / bc_loff and
+  // marked visited)
   template <class PushOp, class PullOp>
   void run_level(uint32_t x, uint32_t y, uint32_t thr, const PushOp &push_op, const PullOp &pull_op,
-                 bool fused_pull);
+                 bool fused_pull, uint32_t *bc_lv = nullptr, uint64_t *bc_loff = nullptr,
+                 uint8_t *bc_visited = nullptr);
   // Runs two-level steps (0 -> 1 -> 0) until frontier 0 is empty, replayed
   // from a HIP graph captured for key when the graph tuning asks for it;
   // returns the levels that ran.

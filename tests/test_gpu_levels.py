@@ -1,10 +1,12 @@
-"""GPU tests of BFS and CC levels run on device-resident counts (each
-level's direction chosen on the device, include/Map.cpp:66-74) and of the
-captured HIP graphs that replay them and PageRank's iterations.
+"""GPU tests of BFS, CC and BC forward levels run on device-resident
+counts (each level's direction chosen on the device, include/Map.cpp:66-74)
+and of the captured HIP graphs that replay them and PageRank's iterations.
 
 * Captured and one-by-one launches give the same BFS levels and CC labels,
   equal to the oracle, at thresholds that force all-push, all-pull and
-  mixed directions (pull -> push levels build the queue from the bitmap).
+  mixed directions (a push after a pull reads the frontier's bitmap); BC
+  within 1e-9 of the oracle both ways (its pushes add FP64 values with
+  atomics, so runs agree to rounding, not bit for bit).
 * A captured graph replays several times, with other work and host
   synchronisation between the replays, and every replay equals the eager
   result; after an update batch the next call captures again and follows
@@ -51,6 +53,12 @@ def graph():
     G.close()
 
 
+def check_bc(got, ref, what):
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(got), fin), f"{what}: unreachable set"
+    assert oracle.rel_l1(got[fin], ref[fin]) <= FP_TOL, what
+
+
 def check_bfs(G, g, src, what):
     parents, levels = tg.BFS_with_edge_map(G, src, with_levels=True)
     lv = oracle.bfs_levels(g, src)
@@ -70,11 +78,16 @@ def test_levels_captured_equal_eager(graph, thr):
     eager = [check_bfs(G, g, s, f"eager thr {thr}") for s in srcs]
     cc_eager = tg.CC(G)
     assert np.array_equal(cc_eager, oracle.cc(g)), "eager CC"
+    bc_ref = {s: oracle.bc(g, s) for s in srcs[1:]}
+    for s in srcs[1:]:
+        check_bc(tg.BC(G, s), bc_ref[s], f"eager BC thr {thr}")
     G.set_tuning(level_graph=1)
     for rep in range(3):
         for s, lv in zip(srcs, eager):
             assert np.array_equal(check_bfs(G, g, s, f"captured thr {thr} rep {rep}"), lv)
         assert np.array_equal(tg.CC(G), cc_eager), f"captured CC rep {rep}"
+        for s in srcs[1:]:
+            check_bc(tg.BC(G, s), bc_ref[s], f"captured BC thr {thr} rep {rep}")
         G.export_adjacency()  # other work and a host synchronisation between replays
     G.set_tuning(thr_bfs=40, thr_cc=5, level_graph=1)
 
@@ -95,6 +108,7 @@ def test_levels_follow_updates(graph):
         g = oracle.CSR(keys, n)
         check_bfs(G, g, src, f"round {rnd}")
         assert np.array_equal(tg.CC(G), oracle.cc(g)), f"round {rnd}: CC"
+        check_bc(tg.BC(G, src), oracle.bc(g, src), f"round {rnd}: BC")
         pr = tg.PR_S(G, 10)
         assert oracle.rel_l1(pr, oracle.pagerank(g, 10)) <= FP_TOL, f"round {rnd}: PR"
         assert np.array_equal(tg.PR_S(G, 10), pr), f"round {rnd}: PR replay"
