@@ -270,8 +270,16 @@ int tg_set_tuning(tg_graph *g, const uint32_t *tuning);
 int tg_ltc(tg_graph *g, const uint32_t *verts, uint32_t q, uint64_t *counts);
 
 /* ---- request loop (serving) --------------------------------------------
- * No counterpart in the reference.  A server replays submitted requests at
- * their arrival offsets (seconds from tg_server_run) against one graph.
+ * No counterpart in the reference.  A server runs requests at their arrival
+ * offsets (seconds from tg_server_run / tg_server_start) against one graph.
+ * tg_server_run replays the requests submitted before it and returns when
+ * they are done.  tg_server_start runs the same loop on a thread of its own
+ * and returns at once: tg_server_submit stays open (from any thread), a
+ * request submitted while the loop runs arrives at its submission time (or
+ * at arrival_s, if later), tg_server_wait waits for one request, and
+ * tg_server_finish closes the input, waits for every request and returns
+ * the loop's status.  The loop holds the graph while it runs: other calls
+ * on that graph wait until tg_server_finish.
  * batched = 0: one request at a time in arrival order (the plain path).
  * batched = 1: continuous batching; every query still runs on the version
  * that contains exactly the updates that arrived before it.
@@ -307,12 +315,19 @@ int tg_server_create(tg_graph *g, int batched, uint64_t mem_budget, int keep_res
 int tg_server_destroy(tg_server *s);
 int tg_server_submit(tg_server *s, const tg_request *r, uint64_t *id);
 int tg_server_run(tg_server *s);
+int tg_server_start(tg_server *s);
+int tg_server_finish(tg_server *s);
+/* *status = the request's status after waiting up to timeout_s seconds for
+ * it to complete (1: still pending, or the loop is not running) */
+int tg_server_wait(tg_server *s, uint64_t id, double timeout_s, int *status);
+/* while a request is pending only status and arrival_s are filled in */
 int tg_server_result(tg_server *s, uint64_t id, tg_result *out);
 /* kept result array (keep_results=1): BFS levels / CC labels as uint32[n],
  * PR ranks as double[n] */
 int tg_server_result_data(tg_server *s, uint64_t id, void *out, uint64_t bytes);
 /* [0] makespan s, [1] requests/s, [2] p50 s, [3] p99 s, [4] p50 update,
- * [5] p99 update, [6] p50 query, [7] p99 query, [8] groups, [9] rejected */
+ * [5] p99 update, [6] p50 query, [7] p99 query, [8] groups, [9] rejected
+ * (over the completed requests) */
 int tg_server_summary(tg_server *s, double *out /* [10] */);
 
 /* ---- workload and device memory helpers -------------------------------

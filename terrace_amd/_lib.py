@@ -133,6 +133,9 @@ _SIGS = {
     "tg_server_destroy": (ctypes.c_int, [_P]),
     "tg_server_submit": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint64)]),
     "tg_server_run": (ctypes.c_int, [_P]),
+    "tg_server_start": (ctypes.c_int, [_P]),
+    "tg_server_finish": (ctypes.c_int, [_P]),
+    "tg_server_wait": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
     "tg_server_result": (ctypes.c_int, [_P, ctypes.c_uint64, _P]),
     "tg_server_result_data": (ctypes.c_int, [_P, ctypes.c_uint64, _P, ctypes.c_uint64]),
     "tg_server_summary": (ctypes.c_int, [_P, _P]),

@@ -963,17 +963,43 @@ int tg_server_submit(tg_server *S, const tg_request *r, uint64_t *id) {
 int tg_server_run(tg_server *S) {
   return guard([&] {
     NEED(S, "null server");
+    // before the graph's lock, which a running loop holds
+    NEED(!S->s->busy(), "the server's loop is already running");
     std::lock_guard<std::mutex> l(S->owner->g.mu);
     S->s->run();
+  });
+}
+
+int tg_server_start(tg_server *S) {
+  return guard([&] {
+    NEED(S, "null server");
+    S->s->start(&S->owner->g.mu);
+  });
+}
+
+int tg_server_finish(tg_server *S) {
+  return guard([&] {
+    NEED(S, "null server");
+    S->s->finish();
+  });
+}
+
+int tg_server_wait(tg_server *S, uint64_t id, double timeout_s, int *status) {
+  return guard([&] {
+    NEED(S && status, "null argument");
+    *status = S->s->wait(id, timeout_s);
   });
 }
 
 int tg_server_result(tg_server *S, uint64_t id, tg_result *out) {
   return guard([&] {
     NEED(S && out, "null argument");
+    const int st = S->s->status(id);
     const tg::Request &r = S->s->result(id);
-    out->status = r.status;
+    *out = tg_result{};
+    out->status = st;
     out->arrival_s = r.arrival;
+    if (st == 1) return;  // pending: the other fields are not written yet
     out->start_s = r.start;
     out->done_s = r.done;
     out->version = r.version;
@@ -987,6 +1013,7 @@ int tg_server_result(tg_server *S, uint64_t id, tg_result *out) {
 int tg_server_result_data(tg_server *S, uint64_t id, void *out, uint64_t bytes) {
   return guard([&] {
     NEED(S && out, "null argument");
+    NEED(S->s->status(id) != 1, "request still pending");
     const tg::Request &r = S->s->result(id);
     if (!r.r32.empty()) {
       NEED(bytes >= r.r32.size() * 4, "buffer too small");

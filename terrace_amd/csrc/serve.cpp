@@ -54,6 +54,10 @@ Server::Server(Graph &g, int mode, uint64_t mem_budget, bool keep_results)
 
 Server::~Server() {
   try {
+    finish();  // a loop still running on its own thread ends first
+  } catch (...) {
+  }
+  try {
     drain();
   } catch (...) {
   }
@@ -97,9 +101,9 @@ void Server::launch_group(QuerySlot &sl, std::vector<uint64_t> ids) {
 }
 
 void Server::reject(Request &r) {
-  r.status = -2;
   r.start = r.done = now();
   r.version = version_;
+  publish(r, -2);
   note_done(1);
 }
 
@@ -125,17 +129,79 @@ double Server::now() const { return wall() - t0_; }
 void Server::note_done(uint64_t k) {
   uint64_t before = completed_.fetch_add(k);
   if (!progress_) return;
-  uint64_t total = reqs_.size(), step = std::max<uint64_t>(1, total / 10);
+  uint64_t total;
+  {
+    std::lock_guard<std::mutex> l(mu_);
+    total = total_;
+  }
+  uint64_t step = std::max<uint64_t>(1, total / 10);
   if ((before + k) / step != before / step || before + k == total)
     std::fprintf(stderr, "[serve] mode %d: %llu / %llu requests done at %.2f s\n", mode_,
                  static_cast<unsigned long long>(before + k), static_cast<unsigned long long>(total), now());
 }
 
 uint64_t Server::submit(Request r) {
-  r.status = 1;
-  r.footprint = footprint(r);
-  reqs_.push_back(std::move(r));
-  return reqs_.size() - 1;
+  uint64_t id;
+  {
+    std::lock_guard<std::mutex> l(mu_);
+    if (running_) {
+      if (closed_) throw Error(-1, "the server's input is closed while its loop runs");
+      // a live request arrives now (or at its own offset, if later): the
+      // loop has taken in only requests that arrived before this moment
+      r.arrival = std::max(r.arrival, now());
+    }
+    r.status = 1;
+    reqs_.push_back(std::move(r));
+    id = reqs_.size() - 1;
+    total_++;
+    if (running_) {
+      inbox_.push_back(id);
+      input_seq_++;
+    }
+  }
+  input_cv_.notify_all();
+  return id;
+}
+
+Request &Server::req(uint64_t id) {
+  std::lock_guard<std::mutex> l(mu_);
+  return reqs_.at(id);
+}
+
+// Results are written before the status leaves pending, under mu_: a
+// reader that sees the status (under mu_) sees the results.
+void Server::publish(Request &r, int status) {
+  {
+    std::lock_guard<std::mutex> l(mu_);
+    r.status = status;
+  }
+  done_cv_.notify_all();
+}
+
+int Server::status(uint64_t id) const {
+  std::lock_guard<std::mutex> l(mu_);
+  if (id >= reqs_.size()) throw Error(-1, "unknown request id");
+  return reqs_[id].status;
+}
+
+const Request &Server::result(uint64_t id) const {
+  std::lock_guard<std::mutex> l(mu_);
+  if (id >= reqs_.size()) throw Error(-1, "unknown request id");
+  return reqs_[id];
+}
+
+bool Server::busy() const {
+  std::lock_guard<std::mutex> l(mu_);
+  return running_ || loop_.joinable();
+}
+
+int Server::wait(uint64_t id, double timeout_s) {
+  std::unique_lock<std::mutex> l(mu_);
+  if (id >= reqs_.size()) throw Error(-1, "unknown request id");
+  const Request &r = reqs_[id];
+  done_cv_.wait_for(l, std::chrono::duration<double>(std::max(0.0, timeout_s)),
+                    [&] { return r.status != 1 || !running_; });
+  return r.status;
 }
 
 // Device bytes a request needs while it runs (scratch, outputs, growth).
@@ -159,11 +225,11 @@ uint64_t Server::footprint(const Request &r) const {
 void Server::exec_update_group(const std::vector<uint64_t> &ids) {
   hipStream_t s = g_.stream();
   uint64_t total = 0;
-  for (uint64_t id : ids) total += reqs_[id].us.size();
+  for (uint64_t id : ids) total += req(id).us.size();
   double st = now();
   if (!batched_) {
     // plain: the request's own inserts, then its deletes
-    Request &r = reqs_[ids[0]];
+    Request &r = req(ids[0]);
     std::vector<uint32_t> is, id, ds, dd;
     for (size_t i = 0; i < r.us.size(); i++) {
       if (r.uop[i] == 0) {
@@ -192,7 +258,7 @@ void Server::exec_update_group(const std::vector<uint64_t> &ids) {
     hd.reserve(total);
     ho.reserve(total);
     for (uint64_t rid : ids) {
-      const Request &r = reqs_[rid];
+      const Request &r = req(rid);
       hs.insert(hs.end(), r.us.begin(), r.us.end());
       hd.insert(hd.end(), r.ud.begin(), r.ud.end());
       ho.insert(ho.end(), r.uop.begin(), r.uop.end());
@@ -212,12 +278,12 @@ void Server::exec_update_group(const std::vector<uint64_t> &ids) {
   // version base+k+1, and no query runs between them
   double t = now();
   for (uint64_t id : ids) {
-    Request &r = reqs_[id];
+    Request &r = req(id);
     r.start = st;
     r.done = t;
     r.version = ++version_;
-    r.status = 0;
     r.group = groups_;
+    publish(r, 0);
   }
   groups_++;
   note_done(ids.size());
@@ -273,8 +339,8 @@ void Server::exec_query_plain(Request &r) {
   }
   r.done = now();
   r.version = version_;
-  r.status = 0;
   r.group = groups_++;
+  publish(r, 0);
   note_done(1);
 }
 
@@ -282,7 +348,7 @@ void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64
   const uint32_t n = g_.num_vertices();
   std::vector<uint64_t> bfs, cc, pr, ltc;
   for (uint64_t id : ids) {
-    switch (reqs_[id].type) {
+    switch (req(id).type) {
       case kReqBfs: bfs.push_back(id); break;
       case kReqCc: cc.push_back(id); break;
       case kReqPr: pr.push_back(id); break;
@@ -293,12 +359,12 @@ void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64
   (void)slot.snap->tier_lists();  // shared, read-only while the group runs
   double st = now();
   auto finish = [&](uint64_t id) {
-    Request &r = reqs_[id];
+    Request &r = req(id);
     r.start = st;
     r.done = now();
     r.version = ver;
     r.group = grp;
-    r.status = 0;
+    publish(r, 0);
   };
   std::vector<std::thread> th;
   std::vector<std::string> errors(kLanes);
@@ -310,11 +376,11 @@ void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64
         for (size_t c0 = 0; c0 < bfs.size(); c0 += 64) {
           uint32_t k = static_cast<uint32_t>(std::min<size_t>(64, bfs.size() - c0));
           std::vector<uint32_t> srcs(k);
-          for (uint32_t j = 0; j < k; j++) srcs[j] = reqs_[bfs[c0 + j]].src;
+          for (uint32_t j = 0; j < k; j++) srcs[j] = req(bfs[c0 + j]).src;
           slot.res32[0].ensure(static_cast<uint64_t>(k) * n, s);
           T.bfs_multi(srcs.data(), k, slot.res32[0].get());
           for (uint32_t j = 0; j < k; j++) {
-            Request &r = reqs_[bfs[c0 + j]];
+            Request &r = req(bfs[c0 + j]);
             const uint32_t *lv = slot.res32[0].get() + static_cast<uint64_t>(j) * n;
             r.digest = T.digest(lv, nullptr, n);
             if (keep_) {
@@ -344,8 +410,9 @@ void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64
           TG_HIP(hipStreamSynchronize(s));
         }
         for (uint64_t id : cc) {
-          reqs_[id].digest = d;
-          if (keep_) reqs_[id].r32 = h;
+          Request &r = req(id);
+          r.digest = d;
+          if (keep_) r.r32 = h;
           finish(id);
         }
       } catch (const std::exception &e) {
@@ -358,7 +425,7 @@ void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64
         Traversal &T = *slot.trav[2];
         hipStream_t s = T.stream();
         std::map<uint32_t, std::vector<uint64_t>> by_iters;
-        for (uint64_t id : pr) by_iters[reqs_[id].iters].push_back(id);
+        for (uint64_t id : pr) by_iters[req(id).iters].push_back(id);
         slot.res64.ensure(n, s);
         for (auto &kv : by_iters) {
           T.pagerank(kv.first, slot.res64.get());
@@ -369,9 +436,10 @@ void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64
           double sum = 0;
           for (double x : h) sum += x;
           for (uint64_t id : kv.second) {
-            reqs_[id].digest = d;
-            reqs_[id].fvalue = sum;
-            if (keep_) reqs_[id].r64 = h;
+            Request &r = req(id);
+            r.digest = d;
+            r.fvalue = sum;
+            if (keep_) r.r64 = h;
             finish(id);
           }
         }
@@ -384,11 +452,11 @@ void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64
       try {
         Traversal &T = *slot.trav[3];
         std::vector<uint32_t> vs(ltc.size());
-        for (size_t i = 0; i < ltc.size(); i++) vs[i] = reqs_[ltc[i]].src;
+        for (size_t i = 0; i < ltc.size(); i++) vs[i] = req(ltc[i]).src;
         std::vector<uint64_t> out(ltc.size());
         T.ltc(vs.data(), static_cast<uint32_t>(vs.size()), out.data());
         for (size_t i = 0; i < ltc.size(); i++) {
-          reqs_[ltc[i]].value = out[i];
+          req(ltc[i]).value = out[i];
           finish(ltc[i]);
         }
       } catch (const std::exception &e) {
@@ -401,25 +469,131 @@ void Server::exec_query_group(QuerySlot &slot, std::vector<uint64_t> ids, uint64
   note_done(ids.size());
 }
 
-void Server::run() {
-  std::vector<uint64_t> order(reqs_.size());
-  for (uint64_t i = 0; i < order.size(); i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint64_t a, uint64_t b) { return reqs_[a].arrival < reqs_[b].arrival; });
-  (void)g_.tier_lists();
+void Server::begin(bool live) {
+  std::lock_guard<std::mutex> l(mu_);
+  if (running_) throw Error(-1, "the server's loop is already running");
+  running_ = true;
+  closed_ = !live;
+  loop_code_ = 0;
+  loop_msg_.clear();
+  inbox_.clear();
+  for (uint64_t i = 0; i < reqs_.size(); i++)
+    if (reqs_[i].status == 1) inbox_.push_back(i);
+  input_seq_++;
   completed_ = 0;
   t0_ = wall();
-  if (mode_ == kServeAdmission) {
-    run_admission(order);
-    return;
+}
+
+void Server::end() {
+  {
+    std::lock_guard<std::mutex> l(mu_);
+    running_ = false;
+    closed_ = true;
   }
-  size_t next = 0;  // first request (in arrival order) not yet executed
-  while (next < order.size()) {
-    double t = now();
-    Request &head = reqs_[order[next]];
+  done_cv_.notify_all();
+}
+
+void Server::run() {
+  if (loop_.joinable()) throw Error(-1, "the server's loop is running on its own thread: finish() it first");
+  begin(false);
+  try {
+    loop_body();
+  } catch (...) {
+    end();
+    throw;
+  }
+  end();
+}
+
+void Server::start(std::mutex *hold) {
+  if (loop_.joinable()) throw Error(-1, "the server's loop is already running");
+  begin(true);
+  loop_ = std::thread([this, hold] {
+    std::unique_lock<std::mutex> graph_lock;
+    if (hold) graph_lock = std::unique_lock<std::mutex>(*hold);
+    int code = 0;
+    std::string msg;
+    try {
+      loop_body();
+    } catch (const Error &e) {
+      code = e.code;
+      msg = e.what();
+    } catch (const std::exception &e) {
+      code = -4;
+      msg = e.what();
+    }
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      loop_code_ = code;
+      loop_msg_ = msg;
+    }
+    end();
+  });
+}
+
+void Server::finish() {
+  {
+    std::lock_guard<std::mutex> l(mu_);
+    closed_ = true;
+    input_seq_++;
+  }
+  input_cv_.notify_all();
+  if (loop_.joinable()) loop_.join();
+  int code;
+  std::string msg;
+  {
+    std::lock_guard<std::mutex> l(mu_);
+    code = loop_code_;
+    msg = loop_msg_;
+    loop_code_ = 0;
+  }
+  if (code) throw Error(code, "server loop: " + msg);
+}
+
+bool Server::take_input(std::vector<uint64_t> &order, size_t next, double &t, uint64_t &seen) {
+  std::lock_guard<std::mutex> l(mu_);
+  seen = input_seq_;
+  if (!inbox_.empty()) {
+    for (uint64_t id : inbox_) {
+      reqs_[id].footprint = footprint(reqs_[id]);
+      order.push_back(id);
+    }
+    inbox_.clear();
+    std::stable_sort(order.begin() + static_cast<std::ptrdiff_t>(next), order.end(),
+                     [&](uint64_t a, uint64_t b) { return reqs_[a].arrival < reqs_[b].arrival; });
+  }
+  // taken under mu_: a request submitted after this arrives at t or later
+  t = now();
+  return closed_;
+}
+
+void Server::wait_input(uint64_t seen, double seconds) {
+  std::unique_lock<std::mutex> l(mu_);
+  input_cv_.wait_for(l, std::chrono::duration<double>(seconds), [&] { return input_seq_ != seen; });
+}
+
+void Server::loop_body() {
+  (void)g_.tier_lists();
+  if (mode_ == kServeAdmission) run_admission();
+  else run_strict();
+}
+
+void Server::run_strict() {
+  std::vector<uint64_t> order;  // requests taken in, in arrival order
+  size_t next = 0;              // first one not yet executed
+  uint64_t seen = 0;
+  while (true) {
+    double t = 0;
+    const bool closed = take_input(order, next, t, seen);
+    if (next >= order.size()) {
+      if (closed) break;
+      wait_input(seen, 1.0);
+      continue;
+    }
+    Request &head = req(order[next]);
     if (head.arrival > t) {
       double wait = head.arrival - t;
-      if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
+      if (wait > 200e-6) wait_input(seen, wait - 100e-6);
       continue;
     }
     // a request larger than the whole budget can never be admitted
@@ -440,7 +614,7 @@ void Server::run() {
     uint64_t used = 0;
     size_t j = next;
     while (j < order.size()) {
-      Request &r = reqs_[order[j]];
+      Request &r = req(order[j]);
       if (r.arrival > t || (r.type == kReqUpdate) != upd) break;
       if (r.footprint > budget_) break;  // rejected on its own turn
       if (!ids.empty() && used + r.footprint > budget_) break;
@@ -458,17 +632,21 @@ void Server::run() {
   drain();
 }
 
-void Server::run_admission(const std::vector<uint64_t> &order) {
-  size_t next = 0;               // first request (arrival order) not yet taken in
+void Server::run_admission() {
+  std::vector<uint64_t> order;   // requests taken in, in arrival order
+  size_t next = 0;               // first one not yet taken from order
   std::vector<uint64_t> pool;    // arrived queries waiting for a slot, in arrival order
-  while (next < order.size() || !pool.empty()) {
-    double t = now();
+  uint64_t seen = 0;
+  while (true) {
+    double t = 0;
+    const bool closed = take_input(order, next, t, seen);
+    if (closed && next >= order.size() && pool.empty()) break;
     // take in every arrival: updates are applied now, in arrival order
     std::vector<uint64_t> upd;
     uint64_t used_u = 0;
-    while (next < order.size() && reqs_[order[next]].arrival <= t) {
+    while (next < order.size() && req(order[next]).arrival <= t) {
       uint64_t id = order[next++];
-      Request &r = reqs_[id];
+      Request &r = req(id);
       if (r.footprint > budget_) {
         reject(r);
         continue;
@@ -493,7 +671,7 @@ void Server::run_admission(const std::vector<uint64_t> &order) {
         uint64_t used = 0;
         size_t k = 0;
         for (; k < pool.size(); k++) {
-          const Request &r = reqs_[pool[k]];
+          const Request &r = req(pool[k]);
           if (!ids.empty() && used + r.footprint > budget_) break;
           used += r.footprint;
           ids.push_back(pool[k]);
@@ -503,23 +681,22 @@ void Server::run_admission(const std::vector<uint64_t> &order) {
         continue;
       }
     }
-    // idle: wait for the next arrival, or poll for a slot
-    double wait = next < order.size() ? reqs_[order[next]].arrival - now() : 1.0;
+    // idle: wait for the next arrival or new input, or poll for a slot
+    double wait = next < order.size() ? req(order[next]).arrival - now() : 1.0;
     if (!pool.empty()) wait = std::min(wait, 50e-6);
-    if (wait > 20e-6) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
+    if (wait > 20e-6) wait_input(seen, std::min(wait, 1e-3));
   }
   drain();
 }
 
 std::vector<double> Server::summary() const {
+  std::lock_guard<std::mutex> l(mu_);
   std::vector<double> all, up, q;
   double first = 1e300, last = 0;
   double rejected = 0;
   for (const Request &r : reqs_) {
-    if (r.status != 0) {
-      rejected++;
-      continue;
-    }
+    if (r.status < 0) rejected++;
+    if (r.status != 0) continue;
     double lat = r.done - r.arrival;
     all.push_back(lat);
     (r.type == kReqUpdate ? up : q).push_back(lat);

@@ -24,10 +24,20 @@
 //           waited (a linearizable order: those updates and the query were
 //           in flight together).  Each result carries its version and
 //           equals the query run alone on that version.
+//
+// run() replays the requests submitted before it and returns when they are
+// done.  start() runs the same loop on a thread of its own and keeps the
+// input open: a request submitted while the loop runs arrives at the time
+// of its submission (or at its own arrival offset, if later), and is
+// ordered, batched and versioned like the rest; finish() closes the input,
+// waits for every request and joins the loop.
 #pragma once
 
 #include <atomic>
+#include <condition_variable>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -75,11 +85,26 @@ class Server {
  public:
   Server(Graph &g, int mode, uint64_t mem_budget, bool keep_results);
   ~Server();
+  // Thread-safe; callable before run()/start() and while the loop runs.
   uint64_t submit(Request r);
   void run();
-  const Request &result(uint64_t id) const { return reqs_.at(id); }
+  // The loop on its own thread, holding *hold (if not null) while it runs.
+  void start(std::mutex *hold);
+  // Close the input, wait for every request, join the loop thread; throws
+  // the loop's error, if it failed.
+  void finish();
+  // Status of request id (1 pending, 0 done, < 0 error) after waiting up to
+  // timeout_s for it to leave pending (or for the loop to end).
+  int wait(uint64_t id, double timeout_s);
+  // The loop is running (run() or start() until finish()).  run(), start()
+  // and finish() are called from one controlling thread.
+  bool busy() const;
+  int status(uint64_t id) const;
+  // A request's fields; stable once its status is no longer pending.
+  const Request &result(uint64_t id) const;
   // [0] makespan s, [1] requests/s, [2] p50 s, [3] p99 s, [4] p50 update,
   // [5] p99 update, [6] p50 query, [7] p99 query, [8] groups, [9] rejected
+  // (completed requests only)
   std::vector<double> summary() const;
 
  private:
@@ -88,7 +113,18 @@ class Server {
   bool batched_;
   uint64_t budget_;
   bool keep_;
-  std::vector<Request> reqs_;
+  // Requests by id.  A deque keeps references valid while submit() appends;
+  // mu_ guards the container, the input list and every status change.
+  std::deque<Request> reqs_;
+  mutable std::mutex mu_;
+  std::condition_variable input_cv_;  // new input or closed input
+  mutable std::condition_variable done_cv_;  // a request completed, or the loop ended
+  std::vector<uint64_t> inbox_;       // submitted, not yet taken in by the loop
+  uint64_t input_seq_ = 0;            // bumped by every submit and by finish()
+  bool running_ = false, closed_ = true;
+  std::thread loop_;
+  int loop_code_ = 0;
+  std::string loop_msg_;
   uint64_t version_ = 0;
   uint64_t groups_ = 0;
   static constexpr int kLanes = QuerySlot::kLanes;
@@ -106,9 +142,20 @@ class Server {
   // progress on stderr every tenth of the stream (TG_SERVE_PROGRESS=1)
   bool progress_ = false;
   std::atomic<uint64_t> completed_{0};
+  uint64_t total_ = 0;  // requests submitted (under mu_)
   void note_done(uint64_t k);
 
   double now() const;
+  Request &req(uint64_t id);
+  void publish(Request &r, int status);
+  void begin(bool live);
+  void end();
+  void loop_body();
+  // Takes the submitted requests into order (the unexecuted tail, from
+  // next, kept in arrival order); t = the server clock at that moment.
+  // Returns true when the input is closed.
+  bool take_input(std::vector<uint64_t> &order, size_t next, double &t, uint64_t &seen);
+  void wait_input(uint64_t seen, double seconds);
   uint64_t footprint(const Request &r) const;
   void exec_update_group(const std::vector<uint64_t> &ids);
   void exec_query_plain(Request &r);
@@ -117,7 +164,8 @@ class Server {
   QuerySlot *free_slot();
   void launch_group(QuerySlot &sl, std::vector<uint64_t> ids);
   void reject(Request &r);
-  void run_admission(const std::vector<uint64_t> &order);
+  void run_strict();
+  void run_admission();
   void drain();
 };
 

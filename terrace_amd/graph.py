@@ -453,26 +453,42 @@ class Server:
         check(lib().tg_server_create(G.handle, m, int(mem_budget), int(keep_results),
                                      ctypes.byref(h)))
         self._h, self._g = h, G
-        self._keep = []  # host arrays referenced until submit returns
 
     def submit(self, type: int, arrival: float, src: int = 0, iters: int = 10,
                ins=None, dels=None) -> int:
         r = Request(type=type, arrival_s=float(arrival), src=int(src), iters=int(iters))
+        keep = []  # host arrays referenced until submit returns (it copies them)
         if ins is not None:
             s, d = _u32(ins[0]), _u32(ins[1])
             r.ins_src, r.ins_dst, r.n_ins = ptr(s), ptr(d), len(s)
-            self._keep += [s, d]
+            keep += [s, d]
         if dels is not None:
             s, d = _u32(dels[0]), _u32(dels[1])
             r.del_src, r.del_dst, r.n_del = ptr(s), ptr(d), len(s)
-            self._keep += [s, d]
+            keep += [s, d]
         rid = ctypes.c_uint64()
         check(lib().tg_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
-        self._keep.clear()
+        del keep
         return rid.value
 
     def run(self) -> None:
         check(lib().tg_server_run(self._h))
+
+    def start(self) -> None:
+        """Run the loop on a thread of its own; submit() stays open and a
+        request submitted now arrives at its submission time."""
+        check(lib().tg_server_start(self._h))
+
+    def finish(self) -> None:
+        """Close the input, wait for every request and join the loop."""
+        check(lib().tg_server_finish(self._h))
+
+    def wait(self, rid: int, timeout: float = 60.0) -> int:
+        """Status of request rid (1 pending, 0 done, < 0 error) after waiting
+        up to timeout seconds for it to complete."""
+        st = ctypes.c_int()
+        check(lib().tg_server_wait(self._h, int(rid), float(timeout), ctypes.byref(st)))
+        return st.value
 
     def result(self, rid: int) -> dict:
         r = Result()

@@ -265,3 +265,78 @@ def test_server_admission_rejects_oversized():
     assert S.result(b)["status"] == -2
     assert S.summary()["rejected"] == 1
     S.close(), G.close()
+
+
+def submit_one(S, t, req):
+    return submit_all(S, [(t, req)])[0]
+
+
+@pytest.mark.parametrize("mode", ["plain", "strict", "admission"])
+def test_server_live_submission(mode):
+    """Requests submitted while the loop runs (tg_server_start ...
+    tg_server_finish): a request completes while the input is still open,
+    every live request arrives at its submission time, the update requests
+    apply in submission order, and every query equals the oracle on the
+    version it reports -- exactly the updates submitted before it (plain,
+    strict), at least those (admission).  A request with a later arrival
+    offset stays pending until then, and finish waits for it."""
+    import time
+    n = 1 << 12
+    print("[gpu-test] server live submission", mode, flush=True)
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 61, 0.57, 0.19, 0.19, 50000))
+    rng = np.random.default_rng(29)
+    stream = build_stream(n, keys, rng)
+    G = make_graph(n, keys)
+    S = tg.Server(G, mode=mode, keep_results=True)
+    pre = 4  # submitted before the loop starts, arriving at its start
+    ids = [submit_one(S, 0.0, req) for _, req in stream[:pre]]
+    S.start()
+    with pytest.raises(tg.TerraceError):
+        S.run()  # the loop is already running
+    assert S.wait(ids[0], 60.0) == 0  # done while the input is open
+    open_done = 0
+    for k, (_, req) in enumerate(stream[pre:]):
+        ids.append(submit_one(S, 0.0, req))
+        if k % 9 == 4:
+            time.sleep(0.003)
+        if k % 20 == 10:
+            assert S.wait(ids[-1], 60.0) == 0
+            open_done += 1
+    # a request that arrives 0.3 s after its submission
+    late_t = S.result(ids[-1])["arrival_s"] + 0.3
+    late = S.submit(tg.REQ_CC, late_t)
+    assert S.wait(late, 0.0) == 1 and S.result(late)["status"] == 1
+    S.finish()
+    assert open_done >= 2
+    r_late = S.result(late)
+    assert r_late["status"] == 0 and r_late["start_s"] >= late_t
+    states = oracle_states(keys, stream)
+    seen_updates, last_arrival = 0, 0.0
+    for (_, req), i in zip(stream, ids):
+        r = S.result(i)
+        assert r["status"] == 0, (req[0], r)
+        assert r["arrival_s"] >= last_arrival  # submission order is arrival order
+        last_arrival = r["arrival_s"]
+        if req[0] == "U":
+            seen_updates += 1
+            assert r["version"] == seen_updates
+            continue
+        if mode == "admission":
+            assert seen_updates <= r["version"] <= len(states) - 1, (req, r, seen_updates)
+        else:
+            assert r["version"] == seen_updates, (req, r, seen_updates)
+        g = oracle.CSR(states[r["version"]], n)
+        if req[0] == "BFS":
+            assert np.array_equal(S.result_data(i, np.uint32), oracle.bfs_levels(g, req[1]))
+        elif req[0] == "CC":
+            assert np.array_equal(S.result_data(i, np.uint32), oracle.cc(g))
+        elif req[0] == "PR":
+            assert oracle.rel_l1(S.result_data(i, np.float64), oracle.pagerank(g, 10)) <= 1e-9
+        else:
+            assert r["value"] == oracle.ltc(g, req[1])
+    assert np.array_equal(S.result_data(late, np.uint32), oracle.cc(oracle.CSR(states[-1], n)))
+    off, nbr = G.export_adjacency()
+    ref = oracle.CSR(states[-1], n)
+    assert np.array_equal(nbr, ref.nbr[:ref.m]) and np.array_equal(off, ref.off)
+    assert S.summary()["rejected"] == 0
+    S.close(), G.close()
