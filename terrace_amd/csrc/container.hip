@@ -976,6 +976,47 @@ __global__ void __launch_bounds__(kPrepBlock) k_mixed_prep(GView g, const uint64
     delete_prep_block(b - nb, g, D, nd, E, ne, state + 2ull * nb, err, nb, epoch);
 }
 
+// ------------------------------------------------------------ gates
+// A batch's update kernels are launched before the host has read how the
+// preparation went.  The first kernel of each half decides from the batch
+// scalars whether the half runs and publishes the key and run counts the
+// half's later kernels use: zero when it does not run, so such a half
+// writes nothing to the graph.
+//
+// The preparation raised no error: no id out of range, no bucket beyond its
+// capacity, no look-back wait given up.
+__device__ inline bool prep_clean(const uint64_t *dsc) {
+  return (dsc[0] & (1ull | kSortOverflow)) == 0 && dsc[bsw::kErrSort] == 0 && dsc[bsw::kErrScan] == 0;
+}
+
+// The insert half of the same batch completed on the device: no promotion
+// pending, the in-place tree update done (or no tree keys), the in-leaf PMA
+// merge done (or no PMA keys).
+__device__ inline bool insert_half_done(const uint64_t *dsc, bool pma_launched) {
+  return dsc[5] == 0 && (dsc[4] == 0 || dsc[13] == 0) && (dsc[3] == 0 || (pma_launched && dsc[7] == 0));
+}
+
+// The counts of a half: cw[0] keys, cw[1] runs (at most the capacities)
+// when ok, else zero; block 0 publishes them with the outcome at gw.
+struct HalfCounts {
+  uint64_t keys;
+  uint32_t runs;
+};
+__device__ inline HalfCounts gate_counts(bool ok, const uint64_t *cw, uint64_t cap_keys, uint32_t cap_runs,
+                                         uint64_t *gw) {
+  HalfCounts c{0, 0};
+  if (ok) {
+    c.keys = dev_count(cw, cap_keys);
+    c.runs = static_cast<uint32_t>(dev_count(cw + 1, cap_runs));
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    gw[0] = c.keys;
+    gw[1] = c.runs;
+    gw[2] = ok ? 1u : 2u;
+  }
+  return c;
+}
+
 // ------------------------------------------------------------ inline merge
 // One thread per source run of the unique sorted batch R.  ex[] marks keys
 // already present; nidx[] is the exclusive count of new keys (size |R|+1).
@@ -1076,11 +1117,19 @@ __device__ inline bool inline_merge_window(GView g, const uint64_t *R, const uin
   return true;
 }
 
+// The insert half's first kernel: its gate (gate null: open) and counts
+// (cw, published at gw), then one thread per run.  promote[] is zero past
+// the runs up to cap_runs, so scans over the capacity count promotions.
 __global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
-                               const uint32_t *run_beg, uint32_t nruns, uint8_t *run_route,
-                               uint32_t *run_c, uint32_t *run_nko, uint32_t *kick_old, uint32_t *promote,
-                               uint32_t *old_deg, uint8_t *kroute, uint64_t nR) {
+                               const uint32_t *run_beg, const uint64_t *cw, const uint64_t *gate, uint64_t *gw,
+                               uint64_t cap_keys, uint32_t cap_runs, uint8_t *run_route, uint32_t *run_c,
+                               uint32_t *run_nko, uint32_t *kick_old, uint32_t *promote, uint32_t *old_deg,
+                               uint8_t *kroute) {
+  const HalfCounts hc = gate_counts(!gate || prep_clean(gate), cw, cap_keys, cap_runs, gw);
+  const uint64_t nR = hc.keys;
+  const uint32_t nruns = hc.runs;
   GRID_STRIDE(i, nR) kroute[i] = kRouteNone;  // the kicked keys' routes start empty
+  GRID_STRIDE(r, cap_runs) if (r >= nruns) promote[r] = 0u;
   GRID_STRIDE(r, nruns) {
     uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
     uint32_t s = key_src(R[b0]);
@@ -1157,10 +1206,12 @@ __device__ inline void place_kicked_old(uint32_t r, const uint64_t *R, const uin
 // Place the kicked keys of every run, sorted, into kick[run_beg ..]:
 // new keys beyond the first c of the run, merged with the old inline
 // entries that were pushed out.
-__global__ void k_place_kicked(const uint64_t *R, const uint32_t *ex, const uint32_t *nidx, uint64_t nR,
-                               const uint32_t *run_beg, uint32_t nruns, const uint8_t *run_route,
-                               const uint32_t *run_c, const uint32_t *run_nko, const uint32_t *kick_old,
-                               uint64_t *kick, uint8_t *kroute) {
+// gw: the counts k_inline_merge published.
+__global__ void k_place_kicked(const uint64_t *R, const uint32_t *ex, const uint32_t *nidx, const uint64_t *gw,
+                               const uint32_t *run_beg, const uint8_t *run_route, const uint32_t *run_c,
+                               const uint32_t *run_nko, const uint32_t *kick_old, uint64_t *kick, uint8_t *kroute) {
+  const uint64_t nR = gw[0];
+  const uint32_t nruns = static_cast<uint32_t>(gw[1]);
   GRID_STRIDE(i, nR) {
     if (ex[i]) continue;
     // run of i: last r with run_beg[r] <= i
@@ -2240,12 +2291,21 @@ __global__ void __launch_bounds__(256) k_tree_small_apply(GView g, const uint64_
 // One thread per run of existing keys: remove inline hits, mark secondary
 // deletes, and record how many inline slots must be refilled.
 // *pma_taken (k_bump's counter of PMA elements moved inline) is cleared here.
-__global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_beg, uint32_t nruns,
-                              uint8_t *sec_route, uint32_t *run_bump, uint64_t *pma_taken, uint64_t *low) {
+// The delete half's first kernel: its gate (gate null: open; spec: also the
+// insert half of the same batch must have completed) and counts (cw,
+// published at gw).  run_bump[] is zero past the runs up to cap_runs.
+__global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_beg, const uint64_t *cw,
+                              const uint64_t *gate, bool spec, bool pma_launched, uint64_t *gw, uint64_t cap_keys,
+                              uint32_t cap_runs, uint8_t *sec_route, uint32_t *run_bump, uint64_t *pma_taken,
+                              uint64_t *low) {
+  const bool ok = !gate || (prep_clean(gate) && (!spec || insert_half_done(gate, pma_launched)));
+  const HalfCounts hc = gate_counts(ok, cw, cap_keys, cap_runs, gw);
+  const uint32_t nruns = hc.runs;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *pma_taken = 0;
-    *low = 0;
+    if (ok) *low = 0;
   }
+  GRID_STRIDE(r, cap_runs) if (r >= nruns) run_bump[r] = 0u;
   GRID_STRIDE(r, nruns) {
     uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
     uint32_t s = key_src(R[b0]);
@@ -2296,10 +2356,12 @@ __global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_be
 // B-tree vertices: the moved keys are routed to the tree delete.  With
 // trees_only, PMA vertices are skipped (a second pass after the B-tree
 // fallback; their refill is already done).
-__global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, uint32_t nruns,
+// gw: the counts k_delete_runs published.
+__global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, const uint64_t *gw,
                        const uint32_t *run_bump, const uint32_t *bump_off, uint64_t *bump_keys,
                        uint8_t *bump_route, bool trees_only, uint64_t *pma_taken, uint32_t lo0,
                        uint64_t *low) {
+  const uint32_t nruns = static_cast<uint32_t>(gw[1]);
   GRID_STRIDE(r, nruns) {
     uint32_t y = run_bump[r];
     if (y == 0) continue;
@@ -3425,6 +3487,8 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   // stays on the device until the one read below
   const uint64_t *R = nullptr;
   const bool small = count <= kSmallSortKeys;
+  const bool fused = count <= kBucketSortMax;  // the insert launched before the read
+  bool pma_launched = false;
   uint64_t h[kScalars];
   // a bucket sort that overflowed is redone by radix (second pass)
   for (int pass = 0; pass < 2; pass++) {
@@ -3465,16 +3529,33 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
       debug_fail &= ~1u;
       fill_u64(dsc_.get() + kErrScan, 1, 1, stream_);
     }
+    if (fused) {
+      // the insert on the device's counts (dsc_[1], [2]), gated on the
+      // preparation's error words, before the one read
+      presize_insert(count, static_cast<uint32_t>(count));
+      mutate([&] { pma_launched = insert_launch(R, count, static_cast<uint32_t>(count), dsc_.get() + 1, dsc_.get()); });
+    }
     // counts, error bits and the look-back error words in one read
     read_scalars(dsc_.get(), h, kScalars, stream_);
     if (!bucket || !(h[0] & kSortOverflow)) break;
   }
-  check_prep(h);
+  if (fused && h[bsw::kGate + 2] != 1) {
+    check_prep(h);  // the gate closed on a preparation error: nothing was written
+    throw Error(-4, "insert batch: the update gate closed without an error word");
+  }
+  if (!fused) check_prep(h);
   stats_.last_ins_keys = h[1];
   stats_.last_ins_sources = h[2];
   stats_.last_del_keys = stats_.last_del_sources = 0;
+  if (fused) {
+    mutate([&] {
+      if (insert_finish(h, R, pma_launched)) check_flags(h);
+    });
+    return;
+  }
   if (h[1] == 0) return;
-  insert_runs(R, h[1], static_cast<uint32_t>(h[2]));  // run_beg[nruns] = nR is written
+  // run_beg[nruns] = nR is written; the counts stay in dsc_[1], [2]
+  insert_runs(R, h[1], static_cast<uint32_t>(h[2]), dsc_.get() + 1);
 }
 
 // Insert sorted unique keys (R must not alias the batch scratch buffers).
@@ -3489,16 +3570,17 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
   k_exists2<<<grid_for(nR, 256), 256, 0, stream_>>>(g, R, nR, ex, nidx);
   TG_CHECK_LAUNCH();
   exclusive_scan_u32(nidx, nidx, nR, nidx + nR, scan_scratch, stream_);
-  // source runs
+  // source runs (count in dsc_[2]; the key count goes to dsc_[1] beside it)
   DevBuf<uint32_t> &run_beg = s32_[0];
   run_beg.ensure(nR + 1, stream_);
   select(SelRunStart{R, run_beg.get()}, nR, dsc_.get() + 2, scan_scratch, stream_);
   k_put_u32<<<1, 1, 0, stream_>>>(run_beg.get(), dsc_.get() + 2, static_cast<uint32_t>(nR));
   TG_CHECK_LAUNCH();
+  fill_u64(dsc_.get() + 1, nR, 1, stream_);
   uint64_t h[kScalars];
   read_scalars(dsc_.get(), h, kScalars, stream_);
   check_prep(h);
-  insert_runs(R, nR, static_cast<uint32_t>(h[2]));
+  insert_runs(R, nR, static_cast<uint32_t>(h[2]), dsc_.get() + 1);
 }
 
 // Scratch of the insert's common path, bounded by the batch: sized before
@@ -3517,14 +3599,23 @@ void Graph::presize_insert(uint64_t nR, uint32_t nruns) {
   s32_[30].ensure(1 + 3 * std::min<uint64_t>(nR, kTreeSmallKeys), stream_);
 }
 
-void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns) {
+void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint64_t *cw) {
   presize_insert(nR, nruns);
-  mutate([&] { insert_runs_apply(R, nR, nruns); });
+  mutate([&] {
+    const bool pma_launched = insert_launch(R, nR, nruns, cw, nullptr);
+    uint64_t hs[kScalars];
+    read_scalars(dsc_.get(), hs, kScalars, stream_);
+    if (insert_finish(hs, R, pma_launched)) check_flags(hs);
+  });
 }
 
 // The insert proper: existence flags in f0_, the new-key prefix in f1_ and
-// the run starts (with run_beg[nruns] = nR) in s32_[0] are ready.
-void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
+// the run starts (with run_beg[nruns] = nR) in s32_[0] are ready; the key
+// and run counts are on the device at cw.  Every kernel is sized by the
+// capacities and reads the counts k_inline_merge published, so nothing
+// here waits for the host.
+bool Graph::insert_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint64_t *cw,
+                          const uint64_t *gate) {
   lists_dirty_ = true;
   ++epoch_;
   GView g = view();
@@ -3532,20 +3623,22 @@ void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   DevBuf<uint32_t> &run_beg = s32_[0], &run_c = s32_[2], &run_nko = s32_[3], &kick_old = s32_[4],
                    &promote = s32_[5], &ppos = s32_[6];
   DevBuf<uint8_t> &run_route = s8_[3], &kroute = s8_[4];
-  run_c.ensure(nruns + 1, stream_);
-  run_nko.ensure(nruns + 1, stream_);
-  kick_old.ensure(static_cast<uint64_t>(nruns) * kInline + 1, stream_);
-  promote.ensure(nruns + 1, stream_);
-  ppos.ensure(nruns + 1, stream_);
-  run_route.ensure(nruns + 1, stream_);
-  kroute.ensure(nR + 1, stream_);
+  run_c.ensure(capRuns + 1, stream_);
+  run_nko.ensure(capRuns + 1, stream_);
+  kick_old.ensure(static_cast<uint64_t>(capRuns) * kInline + 1, stream_);
+  promote.ensure(capRuns + 1, stream_);
+  ppos.ensure(capRuns + 1, stream_);
+  run_route.ensure(capRuns + 1, stream_);
+  kroute.ensure(capR + 1, stream_);
   // the old degrees of promoted vertices are needed after the merge
   DevBuf<uint32_t> &old_deg = s32_[7];
-  old_deg.ensure(nruns + 1, stream_);
+  old_deg.ensure(capRuns + 1, stream_);
+  uint64_t *gw = dsc_.get() + bsw::kGate;
   // (also clears the kicked keys' routes)
-  k_inline_merge<<<grid_for(nR, 256), 256, 0, stream_>>>(g, R, ex, nidx, run_beg.get(), nruns, run_route.get(),
-                                                         run_c.get(), run_nko.get(), kick_old.get(), promote.get(),
-                                                         old_deg.get(), kroute.get(), nR);
+  k_inline_merge<<<grid_for(capR, 256), 256, 0, stream_>>>(g, R, ex, nidx, run_beg.get(), cw, gate, gw, capR, capRuns,
+                                                           run_route.get(), run_c.get(), run_nko.get(),
+                                                           kick_old.get(), promote.get(), old_deg.get(),
+                                                           kroute.get());
   TG_CHECK_LAUNCH();
   if (debug_fail & 2u) {
     debug_fail &= ~2u;
@@ -3553,36 +3646,38 @@ void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   }
   // kicked keys, sorted per run, placed in kick buffer k2_ (only routed
   // entries are read back)
-  k2_.ensure(nR + 1, stream_);
-  k_place_kicked<<<grid_for(nR, 256), 256, 0, stream_>>>(R, ex, nidx, nR, run_beg.get(), nruns, run_route.get(),
-                                                         run_c.get(), run_nko.get(), kick_old.get(), k2_.get(),
-                                                         kroute.get());
+  k2_.ensure(capR + 1, stream_);
+  k_place_kicked<<<grid_for(capR, 256), 256, 0, stream_>>>(R, ex, nidx, gw, run_beg.get(), run_route.get(),
+                                                           run_c.get(), run_nko.get(), kick_old.get(), k2_.get(),
+                                                           kroute.get());
   TG_CHECK_LAUNCH();
-  // split kicked keys by tier and count promotions (counts stay on the device)
-  k3_.ensure(nR + 1, stream_);
-  g0_.ensure(nR + 1, stream_);
-  if (nR <= kSmallSortKeys) {
-    k_small_route<<<1, 1024, 0, stream_>>>(k2_.get(), kroute.get(), static_cast<uint32_t>(nR), k3_.get(), g0_.get(),
-                                           promote.get(), nruns, ppos.get(), dsc_.get() + 3);
+  // split kicked keys by tier and count promotions (counts stay on the
+  // device: PMA keys dsc_[3], tree keys [4], promotions [5])
+  k3_.ensure(capR + 1, stream_);
+  g0_.ensure(capR + 1, stream_);
+  if (capR <= kSmallSortKeys) {
+    k_small_route<<<1, 1024, 0, stream_>>>(k2_.get(), kroute.get(), static_cast<uint32_t>(capR), k3_.get(),
+                                           g0_.get(), promote.get(), capRuns, ppos.get(), dsc_.get() + 3, gw);
     TG_CHECK_LAUNCH();
   } else {
-    select2(SelRoutes{k2_.get(), kroute.get(), k3_.get(), g0_.get()}, nR, dsc_.get() + 3, dsc_.get() + 4,
+    select2(SelRoutes{k2_.get(), kroute.get(), k3_.get(), g0_.get(), gw}, capR, dsc_.get() + 3, dsc_.get() + 4,
             scan_scratch, stream_);
-    exclusive_scan_u32(promote.get(), ppos.get(), nruns, ppos.get() + nruns, scan_scratch, stream_,
+    exclusive_scan_u32(promote.get(), ppos.get(), capRuns, ppos.get() + capRuns, scan_scratch, stream_,
                        dsc_.get() + 5);
   }
   // The in-place tree update and the in-leaf PMA merge (disjoint data) run
-  // on the device's counts (PMA keys dsc_[3], tree keys [4]) with no host
-  // read before them.  The tree update waits (status 2) while promotions
-  // ([5]) are pending: their trees must exist first.  One read then brings
-  // every count and outcome back; a part that did not fit takes its general
-  // pass.  The PMA merge is launched when every key could go to the PMA
-  // within the root bound (else the general insert decides after the read).
-  const bool pma_launched = pma_elems_ + nR <= bounds_.up[pma_H_];
-  tree_small_launch(g0_.get(), nR, true, dsc_.get() + 4, dsc_.get() + 5);
-  if (pma_launched) pma_fast_launch(k3_.get(), nR, dsc_.get() + 3);
-  uint64_t hs[kScalars];
-  read_scalars(dsc_.get(), hs, kScalars, stream_);
+  // on the device's counts.  The tree update waits (status 2) while
+  // promotions are pending: their trees must exist first.  The PMA merge is
+  // launched when every key could go to the PMA within the root bound (else
+  // the general insert decides after the read).
+  const bool pma_launched = pma_elems_ + capR <= bounds_.up[pma_H_];
+  tree_small_launch(g0_.get(), capR, true, dsc_.get() + 4, dsc_.get() + 5);
+  if (pma_launched) pma_fast_launch(k3_.get(), capR, dsc_.get() + 3);
+  return pma_launched;
+}
+
+bool Graph::insert_finish(const uint64_t *hs, const uint64_t *R, bool pma_launched) {
+  const uint32_t nruns = static_cast<uint32_t>(hs[bsw::kGate + 1]);
   const uint64_t npma = hs[3];
   uint64_t ntree = hs[4];
   const uint32_t np = static_cast<uint32_t>(hs[5]);
@@ -3590,10 +3685,9 @@ void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   const bool pma_ok = npma == 0 || (pma_launched && hs[7] == 0);
   if (ntree && tree_ok) stats_.tree_small_updates++;
   if (npma && pma_ok) pma_fast_done(npma, hs[8]);
-  if (np == 0 && tree_ok && pma_ok) {
-    check_flags(hs);
-    return;
-  }
+  if (np == 0 && tree_ok && pma_ok) return true;
+  GView g = view();
+  DevBuf<uint32_t> &run_beg = s32_[0], &promote = s32_[5], &ppos = s32_[6], &old_deg = s32_[7];
   // promotions: new trees for vertices routed to the B-tier the first time
   // (their PMA runs are read after the in-leaf merge, which keeps each run
   // contiguous and its begin slot exact)
@@ -3651,6 +3745,7 @@ void Graph::insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns) {
   check_batch_flags();
   // the promoted runs left the PMA: respread what they left underfull
   if (next && read_scalar(dsc_.get() + kLowFlag, stream_)) pma_rebalance_low();
+  return false;
 }
 
 void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t count) {
@@ -3661,6 +3756,8 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   // sort, unique, the existing keys and their source runs sized by count;
   // the three counts come back in one read
   const bool small = count <= kSmallSortKeys;
+  const bool fused = count <= kBucketSortMax;  // the delete launched before the read
+  uint64_t *dsd = dsc_.get() + bsw::kDel;
   uint64_t h[kScalars];
   // a bucket sort that overflowed is redone by radix (second pass)
   for (int pass = 0; pass < 2; pass++) {
@@ -3697,15 +3794,33 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
       debug_fail &= ~1u;
       fill_u64(dsc_.get() + kErrScan, 1, 1, stream_);
     }
+    if (fused) {
+      // the delete on the device's counts (dsc_[2], [3]), gated on the
+      // preparation's error words, before the one read
+      presize_delete(count, static_cast<uint32_t>(count));
+      mutate([&] {
+        delete_launch(k2_.get(), count, static_cast<uint32_t>(count), s32_[0].get(), dsc_.get() + 2, dsc_.get(),
+                      false, false, dsd);
+      });
+    }
     read_scalars(dsc_.get(), h, kScalars, stream_);
     if (!bucket || !(h[0] & kSortOverflow)) break;
   }
-  check_prep(h);
+  if (fused && h[bsw::kDel + bsw::kGate + 2] != 1) {
+    check_prep(h);  // the gate closed on a preparation error: nothing was written
+    throw Error(-4, "delete batch: the update gate closed without an error word");
+  }
+  if (!fused) check_prep(h);
   stats_.last_ins_keys = stats_.last_ins_sources = 0;
   stats_.last_del_keys = h[2];
   stats_.last_del_sources = h[3];
+  if (fused) {
+    mutate([&] { delete_finish(h, k2_.get(), s32_[0].get(), dsd); });
+    return;
+  }
   if (h[2] == 0) return;
-  delete_runs(k2_.get(), h[2], static_cast<uint32_t>(h[3]));  // run_beg[nruns] = nR is written
+  // run_beg[nruns] = nR is written; the counts stay in dsc_[2], [3]
+  delete_runs(k2_.get(), h[2], static_cast<uint32_t>(h[3]), dsc_.get() + 2);
 }
 
 // Delete sorted unique keys (absent keys are ignored).
@@ -3732,16 +3847,32 @@ void Graph::delete_sorted(const uint64_t *R0, uint64_t nR0) {
   TG_CHECK_LAUNCH();
   read_scalars(dsc_.get(), h, kScalars, stream_);
   check_prep(h);
-  delete_runs(R, nR, static_cast<uint32_t>(h[3]));
+  delete_runs(R, nR, static_cast<uint32_t>(h[3]), dsc_.get() + 2);  // counts in dsc_[2], [3]
+}
+
+// Two counts written to the device (bsw::kPut) for a half launched on
+// counts the host knows.
+const uint64_t *Graph::put_counts(uint64_t keys, uint64_t runs) {
+  fill_u64(dsc_.get() + bsw::kPut, keys, 1, stream_);
+  fill_u64(dsc_.get() + bsw::kPut + 1, runs, 1, stream_);
+  return dsc_.get() + bsw::kPut;
 }
 
 // A mixed batch: the inserts, then the deletes (the driver's add_edge_batch
 // followed by its remove_edge loop, src/graph_bm.cc:354-378), with one sort
-// and one preparation read for both: each edge's last op decides (an edge
-// both inserted and deleted ends absent), deletes of absent edges are
-// no-ops.  The delete half's existence check runs before the insert half
-// changes the graph: the two halves hold disjoint edges, so inserting one
-// cannot change whether an edge of the other exists.
+// and one preparation for both: each edge's last op decides (an edge both
+// inserted and deleted ends absent), deletes of absent edges are no-ops.
+// The delete half's existence check runs before the insert half changes the
+// graph: the two halves hold disjoint edges, so inserting one cannot change
+// whether an edge of the other exists.
+//
+// Up to kBucketSortMax edges the whole batch is launched before the host
+// reads anything: the preparation, the insert half (gated on the
+// preparation's error words) and the delete half (gated also on the insert
+// half completing on the device), then one read.  When the insert half
+// needs a general pass (a promotion, a tree block or PMA leaf that does not
+// fit in place), the delete half has written nothing; it runs again after
+// that pass.
 void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, const uint32_t *ds,
                          const uint32_t *dd, uint64_t nd) {
   const uint64_t count = ni + nd;
@@ -3751,7 +3882,10 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
   if (count > 0x7FFFFFFFull) throw Error(-1, "mixed batch larger than 2^31 edges");
   if (read_only_) throw Error(-1, "graph is a query-only snapshot");
   const bool small = count <= kSmallSortKeys;
+  const bool fused = count <= kBucketSortMax;
+  uint64_t *dsd = dsc_.get() + bsw::kDel;
   uint64_t h[kScalars];
+  bool pma_launched = false;
   for (int pass = 0; pass < 2; pass++) {
     const bool bucket = prepare_mixed_async(is, id, ni, ds, dd, nd, pass == 1);
     GView g = view();
@@ -3803,17 +3937,56 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
       debug_fail &= ~1u;
       fill_u64(dsc_.get() + kErrScan, 1, 1, stream_);
     }
+    if (fused) {
+      // both halves on the device's counts (inserted keys and runs in
+      // dsc_[1], [2]; existing deleted keys and runs in dsc_[10], [11]),
+      // their scratch sized by the batch before the first write
+      presize_insert(count, static_cast<uint32_t>(count));
+      presize_delete(count, static_cast<uint32_t>(count));
+      mutate([&] {
+        pma_launched = insert_launch(k1_.get(), count, static_cast<uint32_t>(count), dsc_.get() + 1, dsc_.get());
+        delete_launch(kE_.get(), count, static_cast<uint32_t>(count), s32_[1].get(), dsc_.get() + 10, dsc_.get(),
+                      true, pma_launched, dsd);
+      });
+    }
     read_scalars(dsc_.get(), h, kScalars, stream_);
     if (!bucket || !(h[0] & kSortOverflow)) break;
   }
-  check_prep(h);
+  if (fused && h[bsw::kGate + 2] != 1) {
+    // the gate closed on a preparation error: nothing was written
+    check_prep(h);
+    throw Error(-4, "mixed batch: the update gate closed without an error word");
+  }
+  if (!fused) check_prep(h);
   const uint64_t nI = h[1], nIruns = h[2], nE = h[10], nEruns = h[11];
   stats_.last_ins_keys = nI;
   stats_.last_ins_sources = nIruns;
   stats_.last_del_keys = nE;
   stats_.last_del_sources = nEruns;
-  if (nI) insert_runs(k1_.get(), nI, static_cast<uint32_t>(nIruns));
-  if (nE) delete_runs(kE_.get(), nE, static_cast<uint32_t>(nEruns), s32_[1].get());
+  if (!fused) {
+    if (nI) insert_runs(k1_.get(), nI, static_cast<uint32_t>(nIruns), dsc_.get() + 1);
+    // the insert's general passes may have used dsc_[10], [11]
+    if (nE) delete_runs(kE_.get(), nE, static_cast<uint32_t>(nEruns), put_counts(nE, nEruns), s32_[1].get());
+    return;
+  }
+  mutate([&] {
+    const bool ins_done = insert_finish(h, k1_.get(), pma_launched);
+    const bool del_ran = h[bsw::kDel + bsw::kGate + 2] == 1;
+    if (ins_done != del_ran) {
+      check_flags(h);  // a look-back wait given up in the insert half closes the delete's gate
+      throw Error(-4, "mixed batch: the delete half's gate disagrees with the insert half's outcome");
+    }
+    if (del_ran) {
+      delete_finish(h, kE_.get(), s32_[1].get(), dsd);
+    } else if (nE) {
+      // the insert's general passes are done; the delete half now
+      delete_launch(kE_.get(), nE, static_cast<uint32_t>(nEruns), s32_[1].get(), put_counts(nE, nEruns), nullptr,
+                    false, false, dsd);
+      uint64_t hs[kScalars];
+      read_scalars(dsc_.get(), hs, kScalars, stream_);
+      delete_finish(hs, kE_.get(), s32_[1].get(), dsd);
+    }
+  });
 }
 
 // Scratch of the delete's common path (see presize_insert).  Inline
@@ -3831,105 +4004,123 @@ void Graph::presize_delete(uint64_t nR, uint32_t nruns) {
   s32_[30].ensure(1 + 3 * std::min<uint64_t>(nR, kTreeSmallKeys), stream_);
 }
 
-void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint32_t *run_beg) {
+void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint64_t *cw, const uint32_t *run_beg) {
   presize_delete(nR, nruns);
-  mutate([&] { delete_runs_apply(R, nR, nruns, run_beg ? run_beg : s32_[0].get()); });
+  const uint32_t *rb = run_beg ? run_beg : s32_[0].get();
+  uint64_t *dsd = dsc_.get() + bsw::kDel;
+  mutate([&] {
+    delete_launch(R, nR, nruns, rb, cw, nullptr, false, false, dsd);
+    uint64_t hs[kScalars];
+    read_scalars(dsc_.get(), hs, kScalars, stream_);
+    delete_finish(hs, R, rb, dsd);
+  });
 }
 
-// The delete proper: R (in k2_) holds existing keys only and s32_[0] their
-// run starts with run_beg[nruns] = nR.
-void Graph::delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint32_t *run_beg) {
+// The inline refill from the smallest secondary neighbours (k_bump; it
+// removes the PMA elements it moves itself and counts them in ds[9]) and
+// its tier split (ds[16] PMA keys: none, k_bump routes them nowhere; ds[17]
+// tree keys; the refill count is ds[6]).
+void Graph::delete_refill(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint32_t *run_beg, uint64_t *ds,
+                          bool trees_only) {
+  GView g = view();
+  DevBuf<uint32_t> &run_bump = s32_[12], &bump_off = s32_[13];
+  DevBuf<uint64_t> &bkeys = s64_[1], &bpma = s64_[2], &btree = s64_[3];
+  DevBuf<uint8_t> &broute = s8_[6];
+  DevBuf<uint32_t> &none = s32_[31];  // no per-run counts for the refill's split
+  const uint64_t *gw = ds + bsw::kGate;
+  k_bump<<<grid_for(capRuns, 256), 256, 0, stream_>>>(g, R, run_beg, gw, run_bump.get(), bump_off.get(), bkeys.get(),
+                                                      broute.get(), trees_only, ds + 9,
+                                                      static_cast<uint32_t>(bounds_.lo[0]), dsc_.get() + kLowFlag);
+  TG_CHECK_LAUNCH();
+  tree_dirty_ = true;
+  if (capR <= kSmallSortKeys) {
+    k_small_route<<<1, 1024, 0, stream_>>>(bkeys.get(), broute.get(), static_cast<uint32_t>(capR), bpma.get(),
+                                           btree.get(), nullptr, 0, none.get(), ds + 16, ds + 6);
+    TG_CHECK_LAUNCH();
+  } else {
+    select2(SelRoutes{bkeys.get(), broute.get(), bpma.get(), btree.get(), ds + 6}, capR, ds + 16, ds + 17,
+            scan_scratch, stream_);
+  }
+}
+
+// The delete proper: R holds existing keys only, run_beg their run starts
+// with run_beg[nruns] = nR, the counts on the device at cw.  Every stage
+// runs on the device's counts (PMA deletes ds[4], tree deletes ds[5],
+// inline refills ds[6]) with no host read between them: the PMA deletes,
+// the in-place tree delete (status ds[13]), the inline refill, its tier
+// split and the refill's in-place tree delete (status ds[19]), which waits
+// when the first tree delete did not fit.
+void Graph::delete_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint32_t *run_beg,
+                          const uint64_t *cw, const uint64_t *gate, bool spec, bool pma_launched, uint64_t *ds) {
   lists_dirty_ = true;
   ++epoch_;
   GView g = view();
   DevBuf<uint32_t> &run_bump = s32_[12], &bump_off = s32_[13];
   DevBuf<uint8_t> &sec_route = s8_[5];
-  run_bump.ensure(nruns + 1, stream_);
-  bump_off.ensure(nruns + 1, stream_);
-  sec_route.ensure(nR + 1, stream_);
-  k_delete_runs<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg, nruns, sec_route.get(), run_bump.get(),
-                                                           dsc_.get() + 9, dsc_.get() + kLowFlag);
+  run_bump.ensure(capRuns + 1, stream_);
+  bump_off.ensure(capRuns + 1, stream_);
+  sec_route.ensure(capR + 1, stream_);
+  uint64_t *gw = ds + bsw::kGate;
+  k_delete_runs<<<grid_for(capR, 256), 256, 0, stream_>>>(g, R, run_beg, cw, gate, spec, pma_launched, gw, capR,
+                                                          capRuns, sec_route.get(), run_bump.get(), ds + 9,
+                                                          dsc_.get() + kLowFlag);
   TG_CHECK_LAUNCH();
   if (debug_fail & 2u) {
     debug_fail &= ~2u;
     throw Error(-2, "injected allocation failure after the inline deletes");
   }
   // secondary deletes by tier, and how many inline slots to refill
-  k3_.ensure(nR + 1, stream_);
-  g0_.ensure(nR + 1, stream_);
-  if (nR <= kSmallSortKeys) {
-    k_small_route<<<1, 1024, 0, stream_>>>(R, sec_route.get(), static_cast<uint32_t>(nR), k3_.get(), g0_.get(),
-                                           run_bump.get(), nruns, bump_off.get(), dsc_.get() + 4);
+  k3_.ensure(capR + 1, stream_);
+  g0_.ensure(capR + 1, stream_);
+  if (capR <= kSmallSortKeys) {
+    k_small_route<<<1, 1024, 0, stream_>>>(R, sec_route.get(), static_cast<uint32_t>(capR), k3_.get(), g0_.get(),
+                                           run_bump.get(), capRuns, bump_off.get(), ds + 4, gw);
     TG_CHECK_LAUNCH();
   } else {
-    select2(SelRoutes{R, sec_route.get(), k3_.get(), g0_.get()}, nR, dsc_.get() + 4, dsc_.get() + 5, scan_scratch,
-            stream_);
-    exclusive_scan_u32(run_bump.get(), bump_off.get(), nruns, bump_off.get() + nruns, scan_scratch, stream_,
-                       dsc_.get() + 6);
+    select2(SelRoutes{R, sec_route.get(), k3_.get(), g0_.get(), gw}, capR, ds + 4, ds + 5, scan_scratch, stream_);
+    exclusive_scan_u32(run_bump.get(), bump_off.get(), capRuns, bump_off.get() + capRuns, scan_scratch, stream_,
+                       ds + 6);
   }
-  // Every stage below runs on the device's counts (PMA deletes dsc_[4],
-  // tree deletes [5], inline refills [6]) with no host read between them:
-  // the PMA deletes, the in-place tree delete (status [13]), the inline
-  // refill from the smallest secondary neighbours (k_bump; it removes the
-  // PMA elements it moves itself and counts them in [9]), the refill's tier
-  // split ([16] PMA keys: none, k_bump routes them nowhere; [17] tree keys)
-  // and the refill's in-place tree delete (status [19]), which waits when
-  // the first tree delete did not fit.  One read then brings every count and
-  // outcome back.
-  DevBuf<uint64_t> &bkeys = s64_[1], &bpma = s64_[2], &btree = s64_[3];
-  DevBuf<uint8_t> &broute = s8_[6];
-  DevBuf<uint32_t> &none = s32_[31];  // no per-run counts for the refill's split
-  pma_delete_launch(k3_.get(), nR, dsc_.get() + 4);
-  tree_small_launch(g0_.get(), nR, false, dsc_.get() + 5, nullptr, dsc_.get() + 13);
-  auto refill = [&](bool trees_only) {
-    g = view();
-    k_bump<<<grid_for(nruns, 256), 256, 0, stream_>>>(g, R, run_beg, nruns, run_bump.get(), bump_off.get(),
-                                                      bkeys.get(), broute.get(), trees_only, dsc_.get() + 9,
-                                                      static_cast<uint32_t>(bounds_.lo[0]), dsc_.get() + kLowFlag);
-    TG_CHECK_LAUNCH();
-    tree_dirty_ = true;
-    if (nR <= kSmallSortKeys) {
-      k_small_route<<<1, 1024, 0, stream_>>>(bkeys.get(), broute.get(), static_cast<uint32_t>(nR), bpma.get(),
-                                             btree.get(), nullptr, 0, none.get(), dsc_.get() + 16, dsc_.get() + 6);
-      TG_CHECK_LAUNCH();
-    } else {
-      select2(SelRoutes{bkeys.get(), broute.get(), bpma.get(), btree.get(), dsc_.get() + 6}, nR, dsc_.get() + 16,
-              dsc_.get() + 17, scan_scratch, stream_);
-    }
-  };
-  refill(false);
-  tree_small_launch(btree.get(), nR, false, dsc_.get() + 17, dsc_.get() + 13, dsc_.get() + 19);
-  uint64_t hs[kScalars];
-  read_scalars(dsc_.get(), hs, kScalars, stream_);
-  const uint64_t npma = hs[4], ntree = hs[5], nb = hs[6];
-  pma_elems_ -= npma + hs[9];  // deleted PMA keys, PMA elements the refill moved inline
+  pma_delete_launch(k3_.get(), capR, ds + 4);
+  tree_small_launch(g0_.get(), capR, false, ds + 5, nullptr, ds + 13);
+  delete_refill(R, capR, capRuns, run_beg, ds, false);
+  tree_small_launch(s64_[3].get(), capR, false, ds + 17, ds + 13, ds + 19);
+}
+
+void Graph::delete_finish(const uint64_t *hs, const uint64_t *R, const uint32_t *run_beg, uint64_t *ds) {
+  const uint64_t *hd = hs + (ds - dsc_.get());
+  const uint64_t nR = hd[bsw::kGate];
+  const uint32_t nruns = static_cast<uint32_t>(hd[bsw::kGate + 1]);
+  const uint64_t npma = hd[4], ntree = hd[5], nb = hd[6];
+  DevBuf<uint64_t> &bpma = s64_[2], &btree = s64_[3];
+  pma_elems_ -= npma + hd[9];  // deleted PMA keys, PMA elements the refill moved inline
   bool fallback = false;
-  if (ntree && hs[13] == 0) stats_.tree_small_updates++;
-  if (ntree && hs[13] != 0) {
+  if (ntree && hd[13] == 0) stats_.tree_small_updates++;
+  if (ntree && hd[13] != 0) {
     // the in-place tree delete did not fit: copy-on-write, then the refill
     // of the B-tree vertices again (its writes are positional, so running
     // it again gives the same result) and the delete of its tree keys
     fallback = true;
     tree_delete_cow(g0_.get(), ntree);
     if (nb) {
-      fill_u64(dsc_.get() + 9, 0, 1, stream_);  // a trees-only refill moves no PMA element
-      refill(true);
+      fill_u64(ds + 9, 0, 1, stream_);  // a trees-only refill moves no PMA element
+      delete_refill(R, nR, nruns, run_beg, ds, true);
       uint64_t h2[2];
-      read_scalars(dsc_.get() + 16, h2, 2, stream_);
+      read_scalars(ds + 16, h2, 2, stream_);
       pma_delete(bpma.get(), h2[0]);
       tree_delete(btree.get(), h2[1]);
     }
-  } else if (hs[17]) {
-    if (hs[19] == 0) {
+  } else if (hd[17]) {
+    if (hd[19] == 0) {
       stats_.tree_small_updates++;
     } else {
       fallback = true;
-      tree_delete_cow(btree.get(), hs[17]);
+      tree_delete_cow(btree.get(), hd[17]);
     }
   }
-  if (hs[16]) {  // refilled PMA keys are removed by k_bump itself; kept for safety
+  if (hd[16]) {  // refilled PMA keys are removed by k_bump itself; kept for safety
     fallback = true;
-    pma_delete(bpma.get(), hs[16]);
+    pma_delete(bpma.get(), hd[16]);
   }
   if (ws_) {
     ws_->erase(R, nR, stream_);  // R holds exactly the deleted (existing) keys

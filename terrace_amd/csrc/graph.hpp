@@ -44,6 +44,30 @@ __host__ __device__ inline uint64_t tree_level_off(uint32_t H, uint32_t l) {
   return 2 * nl - ((2 * nl) >> l);
 }
 
+// Words of a graph's batch scalars: one device array of u64 that a batch
+// reads back in one copy.
+//   [0] error bits; [1..13] the counts and in-place statuses of one batch
+//   (insert half: [3] PMA keys, [4] tree keys, [5] promotions, [7] PMA
+//   leaf-fit flag, [8] touched leaves, [13] in-place tree status);
+//   [14], [15] look-back error words (sort, scans/selects);
+//   [16..19] the inline refill's tier split and tree status;
+//   [20] a PMA delete left a leaf at or below the leaf lower bound;
+//   [21..23] the insert half's gate: the key and run counts its kernels
+//   use (zero when it does not run) and 1 ran / 2 did not run;
+//   [24..47] the delete half's words, at the same offsets from kDel;
+//   [48], [49] key and run counts the host writes for a half it launches
+//   on counts it knows.
+// A mixed batch launches both halves before any read, so the delete half
+// keeps its counts apart from the insert half's.
+namespace bsw {
+constexpr int kErrSort = 14, kErrScan = 15;
+constexpr int kLowFlag = 20;
+constexpr int kGate = 21;  // [kGate] keys, [kGate + 1] runs, [kGate + 2] 1 ran / 2 did not
+constexpr int kDel = 24;   // the delete half's block
+constexpr int kPut = 48;
+constexpr int kCount = 56;
+}  // namespace bsw
+
 struct Bounds {
   uint64_t up[48];  // max elements a window at level l may hold
   uint64_t lo[48];  // min elements a window at level l should hold
@@ -237,14 +261,10 @@ class Graph {
   DevBuf<uint32_t> kick_;
   DevBuf<uint8_t> route_;
   DevBuf<uint64_t> scal_;  // small device scalars
-  // batch scalars: [0] error bits, [1..] counts read back in one copy,
-  // [7] PMA leaf-fit flag, [13] status of the in-place tree update,
-  // [16..18] the inline refill's tier split, [19] status of the refill's
-  // in-place tree delete, [20] a PMA delete left a leaf at or below the
-  // leaf lower bound
-  static constexpr int kScalars = 24;
-  static constexpr int kErrSort = 14, kErrScan = 15;  // look-back error words (sort, scans/selects)
-  static constexpr int kLowFlag = 20;
+  // batch scalars (word layout: namespace bsw)
+  static constexpr int kScalars = bsw::kCount;
+  static constexpr int kErrSort = bsw::kErrSort, kErrScan = bsw::kErrScan;
+  static constexpr int kLowFlag = bsw::kLowFlag;
   DevBuf<uint64_t> dsc_;
   DevBuf<uint64_t> cc_err_, cc_off_;  // count_common: id check; CSR offsets of the TC edgeMap
   DevBuf<uint32_t> cc_nbr_;
@@ -281,12 +301,32 @@ class Graph {
   bool prepare_keys_async(const uint32_t *src, const uint32_t *dst, uint64_t count, bool radix = false);
   bool prepare_mixed_async(const uint32_t *is, const uint32_t *id, uint64_t ni, const uint32_t *ds,
                            const uint32_t *dd, uint64_t nd, bool radix = false);
-  // presize, then *_apply under mutate()
-  void insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns);
+  // presize, then launch + read + finish under mutate().  cw: the key and
+  // run counts on the device (two adjacent words), at most nR and nruns.
+  void insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint64_t *cw);
   // run_beg: the runs' starts (null: s32_[0])
-  void delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint32_t *run_beg = nullptr);
-  void insert_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns);
-  void delete_runs_apply(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint32_t *run_beg);
+  void delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const uint64_t *cw,
+                   const uint32_t *run_beg = nullptr);
+  // The insert half's kernels, sized by capacities (capR keys, capRuns
+  // runs), on the counts in cw; gate (the batch scalars, or null) closes
+  // the half when the preparation raised an error.  Returns whether the
+  // in-leaf PMA merge was launched.
+  bool insert_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint64_t *cw, const uint64_t *gate);
+  // After the read: the insert's outcome, and its general passes when the
+  // in-place paths did not take every key.  True when the in-place paths
+  // took them all (flags not yet checked).
+  bool insert_finish(const uint64_t *hs, const uint64_t *R, bool pma_launched);
+  // The delete half's kernels, its words at ds.  spec: launched behind the
+  // insert half of the same batch, it runs only when that half completed
+  // on the device (pma_launched: the insert's in-leaf merge was launched).
+  void delete_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint32_t *run_beg,
+                     const uint64_t *cw, const uint64_t *gate, bool spec, bool pma_launched, uint64_t *ds);
+  void delete_finish(const uint64_t *hs, const uint64_t *R, const uint32_t *run_beg, uint64_t *ds);
+  // k_bump and the refill's tier split (ds[16] PMA keys, ds[17] tree keys)
+  void delete_refill(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint32_t *run_beg, uint64_t *ds,
+                     bool trees_only);
+  // keys and runs written to the device (bsw::kPut); returns their address
+  const uint64_t *put_counts(uint64_t keys, uint64_t runs);
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
   void check_batch_flags();
   void check_flags(const uint64_t *h);
