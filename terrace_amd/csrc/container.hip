@@ -645,58 +645,6 @@ __global__ void __launch_bounds__(kBucketBlock) k_bucket_sort(uint64_t *keys, ui
   for (uint32_t i = t; i < n; i += kBucketBlock) keys[start + i] = s[i];
 }
 
-// The tier split of a small batch's keys (at most kSmallSortKeys, one
-// workgroup, order kept): PMA keys into P (count sc[0]), B-tree keys into T
-// (sc[1]); and the exclusive prefix of per-run counts (promotions, inline
-// refills) over the runs (total at ppos[nruns] and sc[2]).
-__global__ void __launch_bounds__(1024) k_small_route(const uint64_t *K, const uint8_t *route, uint32_t cap,
-                                                      uint64_t *P, uint64_t *T, const uint32_t *promote,
-                                                      uint32_t nruns, uint32_t *ppos, uint64_t *sc,
-                                                      const uint64_t *nkdev = nullptr) {
-  __shared__ uint32_t s_w[16];
-  const uint32_t t = threadIdx.x;
-  const uint32_t nk = static_cast<uint32_t>(dev_count(nkdev, cap));
-  uint32_t rt[4], cp = 0, ct = 0;
-  uint64_t k[4];
-#pragma unroll
-  for (uint32_t j = 0; j < 4; j++) {
-    const uint32_t i = 4 * t + j;
-    rt[j] = i < nk ? route[i] : 0u;
-    k[j] = rt[j] ? K[i] : 0;
-    cp += rt[j] == kRoutePma ? 1u : 0u;
-    ct += rt[j] == kRouteTree ? 1u : 0u;
-  }
-  uint32_t tp, tt;
-  uint32_t pp = wg_excl_scan(cp, s_w, &tp);
-  uint32_t pt = wg_excl_scan(ct, s_w, &tt);
-#pragma unroll
-  for (uint32_t j = 0; j < 4; j++) {
-    if (rt[j] == kRoutePma) P[pp++] = k[j];
-    if (rt[j] == kRouteTree) T[pt++] = k[j];
-  }
-  uint32_t x[4], c = 0;
-#pragma unroll
-  for (uint32_t j = 0; j < 4; j++) {
-    const uint32_t i = 4 * t + j;
-    x[j] = i < nruns ? promote[i] : 0u;
-    c += x[j];
-  }
-  uint32_t tq;
-  uint32_t pre = wg_excl_scan(c, s_w, &tq);
-#pragma unroll
-  for (uint32_t j = 0; j < 4; j++) {
-    const uint32_t i = 4 * t + j;
-    if (i < nruns) ppos[i] = pre;
-    pre += x[j];
-  }
-  if (t == 0) {
-    ppos[nruns] = tq;
-    sc[0] = tp;
-    sc[1] = tt;
-    sc[2] = tq;
-  }
-}
-
 // After the existence check of a small batch (at most kSmallSortKeys keys,
 // one workgroup).  Insert: nex (flags, 0 past the unique count) becomes its
 // exclusive prefix over [0, cap] and the source runs of R are listed, with
@@ -1017,6 +965,57 @@ __device__ inline HalfCounts gate_counts(bool ok, const uint64_t *cw, uint64_t c
   return c;
 }
 
+// ---- Per-run kernels with their own offsets
+// The first kernel of each half gives one thread to each run (256 runs per
+// block) and computes, besides the run's update, the run's share of the
+// half's output lists (keys for the PMA, for the B-tree, ...).  Their
+// exclusive prefixes over the launch come from a block scan and the
+// launch's decoupled look-back (one column per list), so the kernel writes
+// its lists in run order itself: no flag array, no separate select or scan.
+constexpr uint32_t kRunBlock = 256;
+
+// C <= 4 columns (one look-back wave each).  pre[c]: this thread's
+// exclusive prefix over the launch; tot[c]: the prefix through this block
+// (the grand total in the last block).  Called by every thread of the block.
+template <int C>
+__device__ inline void run_block_scan(const uint32_t (&x)[C], uint32_t (&pre)[C], uint32_t (&tot)[C], uint64_t *state,
+                                      uint32_t b, uint32_t *err, uint32_t epoch) {
+  static_assert(C >= 1 && C <= static_cast<int>(kRunBlock / 64), "one look-back wave per column");
+  __shared__ uint32_t s_w[C][kRunBlock / 64];
+  __shared__ uint32_t s_off[C], s_agg[C];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc[C];
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    uint32_t v = x[c];
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(v, o, 64);
+      if (lane >= o) v += y;
+    }
+    inc[c] = v;
+    if (lane == 63) s_w[c][w] = v;
+  }
+  __syncthreads();
+  if (w < static_cast<uint32_t>(C)) {  // wave c walks column c
+    uint32_t agg = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kRunBlock / 64; i++) agg += s_w[w][i];
+    const uint32_t off = lbk::lookback_wave(state, C, b, w, err, epoch, agg);
+    if (lane == 0) {
+      s_off[w] = off;
+      s_agg[w] = agg;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    uint32_t base = s_off[c];
+    for (uint32_t i = 0; i < w; i++) base += s_w[c][i];
+    pre[c] = base + inc[c] - x[c];
+    tot[c] = s_off[c] + s_agg[c];
+  }
+}
+
 // ------------------------------------------------------------ inline merge
 // One thread per source run of the unique sorted batch R.  ex[] marks keys
 // already present; nidx[] is the exclusive count of new keys (size |R|+1).
@@ -1034,10 +1033,18 @@ __device__ inline HalfCounts gate_counts(bool ok, const uint64_t *cw, uint64_t c
 // a run it does not cover.
 constexpr uint32_t kMergeRun = 16;
 
+// What the merge decided for one run.
+struct RunMerge {
+  uint8_t route = kRouteNone;
+  uint32_t c = 0;     // new keys that entered the inline slots
+  uint32_t nko = 0;   // old inline entries pushed out (into ko[0, nko), sorted)
+  uint32_t nnew = 0;  // new keys of the run
+  uint32_t promote = 0;
+};
+
 __device__ inline bool inline_merge_window(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
-                                           uint32_t b0, uint32_t b1, uint32_t s, uint64_t r, uint8_t *run_route,
-                                           uint32_t *run_c, uint32_t *run_nko, uint32_t *kick_old,
-                                           uint32_t *promote, uint32_t *old_deg) {
+                                           uint32_t b0, uint32_t b1, uint32_t s, RunMerge &m, uint32_t *ko,
+                                           uint32_t &old_degree) {
   const uint32_t L = b1 - b0;
   uint32_t rd[kMergeRun];
   bool rnew[kMergeRun];
@@ -1055,7 +1062,7 @@ __device__ inline bool inline_merge_window(GView g, const uint64_t *R, const uin
   const uint32_t w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
                           q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
   const uint32_t D = w[0];
-  old_deg[r] = D;  // promotions need the degree before the merge
+  old_degree = D;  // promotions need the degree before the merge
   const uint32_t lim = D < kInline ? D : kInline;
   const bool is_tree = (w[15] & 0x80000000u) != 0;  // aux bit 63
   const uint32_t nnew = L > kMergeRun ? nidx[b1] - nidx[b0] : wnew;
@@ -1092,7 +1099,7 @@ __device__ inline bool inline_merge_window(GView g, const uint64_t *R, const uin
     if (rank_old[p] < kInline) {
       if (rank_old[p] != p) vb.nbr[rank_old[p]] = w[1 + p];
     } else {
-      kick_old[r * kInline + (p - kept_old)] = w[1 + p];
+      ko[p - kept_old] = w[1 + p];
     }
   }
   // new keys: rank = inline entries below + new keys before it in the run
@@ -1110,136 +1117,163 @@ __device__ inline bool inline_merge_window(GView g, const uint64_t *R, const uin
     before++;
   }
   vb.degree = D + nnew;
-  run_route[r] = route;
-  run_c[r] = c;
-  run_nko[r] = lim - kept_old;
-  promote[r] = (route == kRouteTree && !is_tree) ? 1u : 0u;
+  m.route = route;
+  m.c = c;
+  m.nko = lim - kept_old;
+  m.nnew = nnew;
+  m.promote = (route == kRouteTree && !is_tree) ? 1u : 0u;
   return true;
 }
 
-// The insert half's first kernel: its gate (gate null: open) and counts
-// (cw, published at gw), then one thread per run.  promote[] is zero past
-// the runs up to cap_runs, so scans over the capacity count promotions.
-__global__ void k_inline_merge(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
-                               const uint32_t *run_beg, const uint64_t *cw, const uint64_t *gate, uint64_t *gw,
-                               uint64_t cap_keys, uint32_t cap_runs, uint8_t *run_route, uint32_t *run_c,
-                               uint32_t *run_nko, uint32_t *kick_old, uint32_t *promote, uint32_t *old_deg,
-                               uint8_t *kroute) {
+// The serial merge for the runs the window does not cover.
+__device__ inline void inline_merge_serial(GView g, const uint64_t *R, const uint32_t *ex, const uint32_t *nidx,
+                                           uint32_t b0, uint32_t b1, uint32_t s, RunMerge &m, uint32_t *ko,
+                                           uint32_t &old_degree) {
+  VBlock &vb = g.vb[s];
+  uint32_t D = vb.degree;
+  old_degree = D;  // promotions need the degree before the merge
+  uint32_t lim = D < kInline ? D : kInline;
+  uint32_t inl[kInline];
+  for (uint32_t i = 0; i < kInline; i++) inl[i] = i < lim ? vb.nbr[i] : 0;
+  uint64_t size = b1 - b0;  // run length including keys already present
+  uint32_t nnew = nidx[b1] - nidx[b0];
+  // k = |inline ∪ run| as the reference counts it (graph.h:290-330)
+  uint32_t dup_inline = 0;
+  for (uint32_t p = 0; p < lim; p++) {
+    uint64_t x = make_key(s, inl[p]);
+    uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, x);
+    if (lb < b1 && R[lb] == x) dup_inline++;
+  }
+  uint64_t kk = static_cast<uint64_t>(lim) + size - dup_inline;
+  bool is_tree = (vb.aux & kAuxTree) != 0;
+  bool overflow = (D == 0) ? (size > kInline) : (kk > kInline);
+  uint8_t route = kRouteNone;
+  if (overflow) route = (is_tree || static_cast<uint64_t>(D) + size > kMediumDegree) ? kRouteTree : kRoutePma;
+  // merge the inline slots with the new keys, keep the 13 smallest
+  uint32_t out[kInline];
+  uint32_t ni = 0, p = 0, c = 0;
+  uint32_t q = b0;
+  while (ni < kInline) {
+    while (q < b1 && ex[q]) q++;
+    bool have_a = p < lim, have_b = q < b1;
+    if (!have_a && !have_b) break;
+    if (have_a && (!have_b || inl[p] < key_dst(R[q]))) {
+      out[ni++] = inl[p++];
+    } else {
+      out[ni++] = key_dst(R[q]);
+      q++;
+      c++;
+    }
+  }
+  uint32_t nko = lim - p;
+  for (uint32_t i = 0; i < nko; i++) ko[i] = inl[p + i];
+  for (uint32_t i = 0; i < ni; i++) vb.nbr[i] = out[i];
+  vb.degree = D + nnew;
+  m.route = route;
+  m.c = c;
+  m.nko = nko;
+  m.nnew = nnew;
+  m.promote = (route == kRouteTree && !is_tree) ? 1u : 0u;
+}
+
+// The insert half's first kernel, one thread per run of the unique sorted
+// batch R (ex[] marks keys already present, nidx[] the exclusive count of
+// new keys): its gate (gate null: open) and counts (cw, published at gw);
+// the run's inline merge (new inline slots and degree, its route); then the
+// keys the run kicks out of the inline tier: the new keys beyond the first
+// c, merged with the old inline entries pushed out, written sorted to the
+// run's offset in the PMA list P or the B-tree list T.  The offsets and the
+// promotion prefix (ppos) come from the run scan; the block's keys are
+// placed by all its threads (a run's keys are contiguous in R).  Totals:
+// sc[0] PMA keys, sc[1] tree keys, sc[2] promotions.
+__global__ void __launch_bounds__(kRunBlock) k_insert_runs(GView g, const uint64_t *R, const uint32_t *ex,
+                                                           const uint32_t *nidx, const uint32_t *run_beg,
+                                                           const uint64_t *cw, const uint64_t *gate, uint64_t *gw,
+                                                           uint64_t cap_keys, uint32_t cap_runs, uint64_t *P,
+                                                           uint64_t *T, uint32_t *promote, uint32_t *ppos,
+                                                           uint32_t *old_deg, uint64_t *sc, uint64_t *state,
+                                                           uint32_t *misc, uint32_t *err, uint32_t nblocks,
+                                                           uint32_t epoch) {
+  __shared__ uint32_t s_ticket;
+  __shared__ uint32_t s_b0[kRunBlock + 1], s_nb0[kRunBlock], s_c[kRunBlock], s_off[kRunBlock];
+  __shared__ uint8_t s_route[kRunBlock], s_nko[kRunBlock];
+  __shared__ uint32_t s_ko[kRunBlock][kInline];
+  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
+  if (b >= nblocks) return;
   const HalfCounts hc = gate_counts(!gate || prep_clean(gate), cw, cap_keys, cap_runs, gw);
-  const uint64_t nR = hc.keys;
-  const uint32_t nruns = hc.runs;
-  GRID_STRIDE(i, nR) kroute[i] = kRouteNone;  // the kicked keys' routes start empty
-  GRID_STRIDE(r, cap_runs) if (r >= nruns) promote[r] = 0u;
-  GRID_STRIDE(r, nruns) {
-    uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
-    uint32_t s = key_src(R[b0]);
-    if (inline_merge_window(g, R, ex, nidx, b0, b1, s, r, run_route, run_c, run_nko, kick_old, promote, old_deg))
-      continue;
-    VBlock &vb = g.vb[s];
-    uint32_t D = vb.degree;
-    old_deg[r] = D;  // promotions need the degree before the merge
-    uint32_t lim = D < kInline ? D : kInline;
-    uint32_t inl[kInline];
-    for (uint32_t i = 0; i < kInline; i++) inl[i] = i < lim ? vb.nbr[i] : 0;
-    uint64_t size = b1 - b0;  // run length including keys already present
-    uint32_t nnew = nidx[b1] - nidx[b0];
-    // k = |inline ∪ run| as the reference counts it (graph.h:290-330)
-    uint32_t dup_inline = 0;
-    for (uint32_t p = 0; p < lim; p++) {
-      uint64_t x = make_key(s, inl[p]);
-      uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, x);
-      if (lb < b1 && R[lb] == x) dup_inline++;
-    }
-    uint64_t kk = static_cast<uint64_t>(lim) + size - dup_inline;
-    bool is_tree = (vb.aux & kAuxTree) != 0;
-    bool overflow = (D == 0) ? (size > kInline) : (kk > kInline);
-    uint8_t route = kRouteNone;
-    if (overflow) route = (is_tree || static_cast<uint64_t>(D) + size > kMediumDegree) ? kRouteTree : kRoutePma;
-    // merge the inline slots with the new keys, keep the 13 smallest
-    uint32_t out[kInline];
-    uint32_t ni = 0, p = 0, c = 0;
-    uint32_t q = b0;
-    while (ni < kInline) {
-      while (q < b1 && ex[q]) q++;
-      bool have_a = p < lim, have_b = q < b1;
-      if (!have_a && !have_b) break;
-      if (have_a && (!have_b || inl[p] < key_dst(R[q]))) {
-        out[ni++] = inl[p++];
-      } else {
-        out[ni++] = key_dst(R[q]);
-        q++;
-        c++;
-      }
-    }
-    uint32_t nko = lim - p;
-    for (uint32_t i = 0; i < nko; i++) kick_old[static_cast<uint64_t>(r) * kInline + i] = inl[p + i];
-    for (uint32_t i = 0; i < ni; i++) vb.nbr[i] = out[i];
-    vb.degree = D + nnew;
-    run_route[r] = route;
-    run_c[r] = c;
-    run_nko[r] = nko;
-    promote[r] = (route == kRouteTree && !is_tree) ? 1u : 0u;
+  const uint32_t t = threadIdx.x;
+  const uint64_t r0 = static_cast<uint64_t>(b) * kRunBlock;
+  const uint32_t nlive = r0 < hc.runs ? static_cast<uint32_t>(hc.runs - r0 < kRunBlock ? hc.runs - r0 : kRunBlock) : 0u;
+  const bool live = t < nlive;
+  const uint64_t r = r0 + t;
+  RunMerge m;
+  uint32_t b0 = 0, b1 = 0, nb0 = 0;
+  if (live) {
+    b0 = run_beg[r];
+    b1 = run_beg[r + 1];
+    nb0 = nidx[b0];
+    const uint32_t s = key_src(R[b0]);
+    uint32_t od = 0;
+    if (!inline_merge_window(g, R, ex, nidx, b0, b1, s, m, s_ko[t], od))
+      inline_merge_serial(g, R, ex, nidx, b0, b1, s, m, s_ko[t], od);
+    old_deg[r] = od;
   }
-}
-
-// The old inline entries a run pushed out, placed among its kicked keys.
-__device__ inline void place_kicked_old(uint32_t r, const uint64_t *R, const uint32_t *nidx,
-                                        const uint32_t *run_beg, const uint8_t *run_route, const uint32_t *run_c,
-                                        const uint32_t *run_nko, const uint32_t *kick_old, uint64_t *kick,
-                                        uint8_t *kroute) {
-  uint8_t route = run_route[r];
-  uint32_t nko = run_nko[r];
-  if (route == kRouteNone || nko == 0) return;
-  uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
-  uint32_t s = key_src(R[b0]);
-  uint32_t c = run_c[r];
-  for (uint32_t e = 0; e < nko; e++) {
-    uint32_t x = kick_old[static_cast<uint64_t>(r) * kInline + e];
-    uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, make_key(s, x));
-    uint32_t new_less = nidx[lb] - nidx[b0] - c;  // new tail keys below x
-    uint64_t pos = b0 + e + new_less;
-    kick[pos] = make_key(s, x);
-    kroute[pos] = route;
+  const uint32_t K = m.route != kRouteNone ? m.nnew - m.c + m.nko : 0u;  // keys kicked out of the inline tier
+  const uint32_t x[3] = {m.route == kRoutePma ? K : 0u, m.route == kRouteTree ? K : 0u, m.promote};
+  uint32_t pre[3], tot[3];
+  run_block_scan<3>(x, pre, tot, state, b, err, epoch);
+  if (live) {
+    promote[r] = m.promote;
+    ppos[r] = pre[2];
   }
-}
-
-// Place the kicked keys of every run, sorted, into kick[run_beg ..]:
-// new keys beyond the first c of the run, merged with the old inline
-// entries that were pushed out.
-// gw: the counts k_inline_merge published.
-__global__ void k_place_kicked(const uint64_t *R, const uint32_t *ex, const uint32_t *nidx, const uint64_t *gw,
-                               const uint32_t *run_beg, const uint8_t *run_route, const uint32_t *run_c,
-                               const uint32_t *run_nko, const uint32_t *kick_old, uint64_t *kick, uint8_t *kroute) {
-  const uint64_t nR = gw[0];
-  const uint32_t nruns = static_cast<uint32_t>(gw[1]);
-  GRID_STRIDE(i, nR) {
+  if (live) s_b0[t] = b0;  // s_b0[nlive] is the end (thread 0, below)
+  s_nb0[t] = nb0;
+  s_c[t] = m.c;
+  s_off[t] = m.route == kRoutePma ? pre[0] : pre[1];
+  s_route[t] = m.route;
+  s_nko[t] = static_cast<uint8_t>(m.nko);
+  if (t == 0) s_b0[nlive] = nlive ? run_beg[r0 + nlive] : 0u;  // run_beg[nruns] = nR
+  __syncthreads();
+  if (b == nblocks - 1 && t == 0) {
+    sc[0] = tot[0];
+    sc[1] = tot[1];
+    sc[2] = tot[2];
+  }
+  if (nlive == 0) return;
+  // the old inline entries each run pushed out, at their ranks among its
+  // kicked keys (new tail keys below them + their own order)
+  if (live && m.route != kRouteNone) {
+    uint64_t *out = (m.route == kRoutePma ? P : T) + s_off[t];
+    const uint32_t s = key_src(R[b0]);
+    for (uint32_t e = 0; e < m.nko; e++) {
+      const uint32_t xo = s_ko[t][e];
+      const uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, make_key(s, xo));
+      const uint32_t new_less = nidx[lb] - nb0 - m.c;
+      out[e + new_less] = make_key(s, xo);
+    }
+  }
+  // the new keys beyond the first c of each run, by every thread over the
+  // block's keys: rank among the new keys, plus the pushed-out entries below
+  const uint32_t kb = s_b0[0], ke = s_b0[nlive];
+  for (uint32_t i = kb + t; i < ke; i += kRunBlock) {
     if (ex[i]) continue;
-    // run of i: last r with run_beg[r] <= i
-    uint32_t lo = 0, hi = nruns;
-    while (lo < hi) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (run_beg[mid] <= i) lo = mid + 1;
+    uint32_t lo = 0, hi = nlive;  // run: last q with s_b0[q] <= i
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_b0[mid] <= i) lo = mid;
       else hi = mid;
     }
-    uint32_t r = lo - 1;
-    uint8_t route = run_route[r];
+    const uint8_t route = s_route[lo];
     if (route == kRouteNone) continue;
-    uint32_t b0 = run_beg[r];
-    uint32_t j = nidx[i] - nidx[b0];
-    uint32_t c = run_c[r];
+    const uint32_t j = nidx[i] - s_nb0[lo];
+    const uint32_t c = s_c[lo];
     if (j < c) continue;
-    uint32_t d = key_dst(R[i]);
-    uint32_t nko = run_nko[r];
-    const uint32_t *ko = kick_old + static_cast<uint64_t>(r) * kInline;
+    const uint32_t d = key_dst(R[i]);
+    const uint32_t nko = s_nko[lo];
     uint32_t less = 0;
-    while (less < nko && ko[less] < d) less++;
-    uint64_t pos = b0 + (j - c) + less;
-    kick[pos] = R[i];
-    kroute[pos] = route;
+    while (less < nko && s_ko[lo][less] < d) less++;
+    (route == kRoutePma ? P : T)[s_off[lo] + (j - c) + less] = R[i];
   }
-  // and the old inline entries each run pushed out (their own positions)
-  GRID_STRIDE(r, nruns) place_kicked_old(static_cast<uint32_t>(r), R, nidx, run_beg, run_route, run_c, run_nko,
-                                         kick_old, kick, kroute);
 }
 
 // ------------------------------------------------------------ PMA kernels
@@ -1571,18 +1605,6 @@ struct SelTiers {
   }
 };
 // Keys of the two secondary tiers: 1 = PMA (into P), 2 = B-tree (into T).
-struct SelRoutes {
-  const uint64_t *k;
-  const uint8_t *route;
-  uint64_t *P, *T;
-  const uint64_t *np = nullptr;  // keys past *np are not listed (null: all)
-  __device__ uint32_t pick(uint64_t i) const {
-    if (np && i >= *np) return 0u;
-    const uint8_t r = route[i];
-    return r == kRoutePma ? 1u : (r == kRouteTree ? 2u : 0u);
-  }
-  __device__ void emit(uint64_t i, uint32_t which, uint64_t pos) const { (which == 1 ? P : T)[pos] = k[i]; }
-};
 struct SelTouched {
   const uint32_t *leaf;
   uint32_t *T, *Tstart;
@@ -2288,27 +2310,47 @@ __global__ void __launch_bounds__(256) k_tree_small_apply(GView g, const uint64_
 }
 
 // ------------------------------------------------------------ delete batch
-// One thread per run of existing keys: remove inline hits, mark secondary
-// deletes, and record how many inline slots must be refilled.
-// *pma_taken (k_bump's counter of PMA elements moved inline) is cleared here.
-// The delete half's first kernel: its gate (gate null: open; spec: also the
-// insert half of the same batch must have completed) and counts (cw,
-// published at gw).  run_bump[] is zero past the runs up to cap_runs.
-__global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_beg, const uint64_t *cw,
-                              const uint64_t *gate, bool spec, bool pma_launched, uint64_t *gw, uint64_t cap_keys,
-                              uint32_t cap_runs, uint8_t *sec_route, uint32_t *run_bump, uint64_t *pma_taken,
-                              uint64_t *low) {
+// The delete half's first kernel, one thread per run of existing keys: its
+// gate (gate null: open; spec: also the insert half of the same batch must
+// have completed) and counts (cw, published at gw); the run's inline hits
+// removed (kept entries move down over them, the degree drops by the run's
+// length); how many inline slots the refill must fill (run_bump); and the
+// run's secondary deletes written, in key order, to its offset in the PMA
+// list P or the B-tree list T.  The offsets come from the run scan, as do
+// each B-tree run's offset for the tree keys its refill moves (tbump_off).
+// Totals: ds[4] PMA deletes, ds[5] tree deletes, ds[6] inline refills,
+// ds[17] refill keys for the tree delete (ds[16], refill keys for the PMA,
+// is 0: k_bump removes those itself).  ds[9] (k_bump's counter of PMA
+// elements moved inline) and, when the half runs, *low are cleared here.
+__global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_beg,
+                                                           const uint64_t *cw, const uint64_t *gate, bool spec,
+                                                           bool pma_launched, uint64_t *gw, uint64_t cap_keys,
+                                                           uint32_t cap_runs, uint64_t *P, uint64_t *T,
+                                                           uint32_t *run_bump, uint32_t *tbump_off, uint64_t *ds,
+                                                           uint64_t *low, uint64_t *state, uint32_t *misc,
+                                                           uint32_t *err, uint32_t nblocks, uint32_t epoch) {
+  __shared__ uint32_t s_ticket;
+  __shared__ uint32_t s_b0[kRunBlock + 1], s_off[kRunBlock];
+  __shared__ uint8_t s_tier[kRunBlock], s_nhit[kRunBlock];
+  __shared__ uint32_t s_hit[kRunBlock][kInline];  // the inline entries the run deletes, sorted
+  const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
+  if (b >= nblocks) return;
   const bool ok = !gate || (prep_clean(gate) && (!spec || insert_half_done(gate, pma_launched)));
   const HalfCounts hc = gate_counts(ok, cw, cap_keys, cap_runs, gw);
-  const uint32_t nruns = hc.runs;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *pma_taken = 0;
+  const uint32_t t = threadIdx.x;
+  if (blockIdx.x == 0 && t == 0) {
+    ds[9] = 0;
     if (ok) *low = 0;
   }
-  GRID_STRIDE(r, cap_runs) if (r >= nruns) run_bump[r] = 0u;
-  GRID_STRIDE(r, nruns) {
-    uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
-    uint32_t s = key_src(R[b0]);
+  const uint64_t r0 = static_cast<uint64_t>(b) * kRunBlock;
+  const uint32_t nlive = r0 < hc.runs ? static_cast<uint32_t>(hc.runs - r0 < kRunBlock ? hc.runs - r0 : kRunBlock) : 0u;
+  const bool live = t < nlive;
+  const uint64_t r = r0 + t;
+  uint32_t nsec = 0, y = 0, nhit = 0;
+  uint8_t tier = kRoutePma;
+  if (live) {
+    const uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
+    const uint32_t s = key_src(R[b0]);
     // the block as 16 words (degree, nbr[0..12], aux): registers only, the
     // inline hits as a 13-bit mask (no run-time indexed private array)
     const uint4 *vp = reinterpret_cast<const uint4 *>(g.vb + s);
@@ -2317,15 +2359,24 @@ __global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_be
                             q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
     const uint32_t D = w[0];
     const uint32_t lim = D < kInline ? D : kInline;
-    const uint8_t tier = (w[15] & 0x80000000u) ? kRouteTree : kRoutePma;  // aux bit 63: B-tree
+    tier = (w[15] & 0x80000000u) ? kRouteTree : kRoutePma;  // aux bit 63: B-tree
+    const uint32_t len = b1 - b0;
     uint32_t hit = 0;
-    for (uint32_t q = b0; q < b1; q++) {
-      const uint32_t d = key_dst(R[q]);
-      uint32_t m = 0;
+    if (len <= 2 * kInline) {
+      for (uint32_t q = b0; q < b1; q++) {
+        const uint32_t d = key_dst(R[q]);
 #pragma unroll
-      for (uint32_t p = 0; p < kInline; p++) m |= (p < lim && w[1 + p] == d) ? (1u << p) : 0u;
-      hit |= m;
-      sec_route[q] = m ? kRouteNone : tier;
+        for (uint32_t p = 0; p < kInline; p++) hit |= (p < lim && w[1 + p] == d) ? (1u << p) : 0u;
+      }
+    } else {
+      // a long run: look each inline entry up among the run's keys
+#pragma unroll
+      for (uint32_t p = 0; p < kInline; p++) {
+        if (p >= lim) continue;
+        const uint64_t key = make_key(s, w[1 + p]);
+        const uint64_t lb = b0 + lower_bound_dev(R + b0, len, key);
+        if (lb < b1 && R[lb] == key) hit |= 1u << p;
+      }
     }
     // kept inline entries move down over the deleted ones, in order
     VBlock &vb = g.vb[s];
@@ -2335,32 +2386,73 @@ __global__ void k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_be
       if (p < lim && !((hit >> p) & 1u)) {
         if (keep != p) vb.nbr[keep] = w[1 + p];
         keep++;
+      } else if (p < lim) {
+        s_hit[t][nhit++] = w[1 + p];
       }
     }
-    uint32_t x = lim - keep;  // inline entries deleted
-    uint32_t len = b1 - b0;
-    uint32_t sec_before = D > kInline ? D - kInline : 0;
-    uint32_t sec_after = sec_before - (len - x);
-    uint32_t y = x < sec_after ? x : sec_after;
+    const uint32_t x = lim - keep;  // inline entries deleted
+    const uint32_t sec_before = D > kInline ? D - kInline : 0;
+    const uint32_t sec_after = sec_before - (len - x);
+    y = x < sec_after ? x : sec_after;
     vb.degree = D - len;
     run_bump[r] = y;
+    nsec = len - x;
+    s_b0[t] = b0;
+    if (t == 0) s_b0[nlive] = run_beg[r0 + nlive];  // run_beg[nruns] = nR
+  }
+  const uint32_t xs[4] = {tier == kRoutePma ? nsec : 0u, tier == kRouteTree ? nsec : 0u, y,
+                          tier == kRouteTree ? y : 0u};
+  uint32_t pre[4], tot[4];
+  run_block_scan<4>(xs, pre, tot, state, b, err, epoch);
+  if (live) tbump_off[r] = pre[3];
+  s_off[t] = tier == kRoutePma ? pre[0] : pre[1];
+  s_tier[t] = tier;
+  s_nhit[t] = static_cast<uint8_t>(nhit);
+  __syncthreads();
+  if (b == nblocks - 1 && t == 0) {
+    ds[4] = tot[0];
+    ds[5] = tot[1];
+    ds[6] = tot[2];
+    ds[16] = 0;
+    ds[17] = tot[3];
+  }
+  if (nlive == 0) return;
+  // the secondary deletes, by every thread over the block's keys: a key's
+  // position is its rank in the run minus the inline hits below it
+  const uint32_t kb = s_b0[0], ke = s_b0[nlive];
+  for (uint32_t i = kb + t; i < ke; i += kRunBlock) {
+    uint32_t lo = 0, hi = nlive;  // run: last q with s_b0[q] <= i
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_b0[mid] <= i) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t d = key_dst(R[i]);
+    const uint32_t nh = s_nhit[lo];
+    uint32_t less = 0;
+    bool inl = false;
+    for (uint32_t e = 0; e < nh; e++) {
+      const uint32_t v = s_hit[lo][e];
+      less += v < d ? 1u : 0u;
+      inl |= v == d;
+    }
+    if (inl) continue;
+    (s_tier[lo] == kRoutePma ? P : T)[s_off[lo] + (i - s_b0[lo]) - less] = R[i];
   }
 }
 
 // Refill inline slots from the smallest secondary neighbours (graph.h:729-780,
-// batched): thread per run with y > 0; emits the moved keys for deletion
-// from the secondary tier.
+// batched): thread per run with y > 0.
 // PMA vertices: the moved elements are removed from the PMA here (slot
 // emptied, leaf count lowered, begin pointer moved to the next element of
-// the vertex; their count added to *pma_taken) and routed to no tier.
-// B-tree vertices: the moved keys are routed to the tree delete.  With
-// trees_only, PMA vertices are skipped (a second pass after the B-tree
-// fallback; their refill is already done).
-// gw: the counts k_delete_runs published.
+// the vertex; their count added to *pma_taken).
+// B-tree vertices: the moved keys are written to tree_keys at the run's
+// offset (tbump_off) for the tree delete.  With trees_only, PMA vertices
+// are skipped (a second pass after the B-tree fallback; their refill is
+// already done).  gw: the counts k_delete_runs published.
 __global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, const uint64_t *gw,
-                       const uint32_t *run_bump, const uint32_t *bump_off, uint64_t *bump_keys,
-                       uint8_t *bump_route, bool trees_only, uint64_t *pma_taken, uint32_t lo0,
-                       uint64_t *low) {
+                       const uint32_t *run_bump, const uint32_t *tbump_off, uint64_t *tree_keys, bool trees_only,
+                       uint64_t *pma_taken, uint32_t lo0, uint64_t *low) {
   const uint32_t nruns = static_cast<uint32_t>(gw[1]);
   GRID_STRIDE(r, nruns) {
     uint32_t y = run_bump[r];
@@ -2371,20 +2463,19 @@ __global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, cons
     uint32_t D = vb.degree;  // already the post-delete degree
     // inline currently holds min(D,13) - y entries
     uint32_t have = (D < kInline ? D : kInline) - y;
-    uint64_t out = bump_off[r];
     uint32_t got = 0;
     if (vb.aux & kAuxTree) {
       uint32_t t = static_cast<uint32_t>(vb.aux);
       uint64_t off = g.toff[t];
       uint32_t nb = g.tnblk[t];
+      uint64_t out = tbump_off[r];
       for (uint32_t j = 0; j < nb && got < y; j++) {
         uint32_t b = g.iblk[off + j];
         uint32_t c = g.bcnt[b];
         for (uint32_t e = 0; e < c && got < y; e++) {
           uint32_t d = g.bkeys[static_cast<uint64_t>(b) * kBlk + e];
           vb.nbr[have + got] = d;
-          bump_keys[out + got] = make_key(s, d);
-          bump_route[out + got] = kRouteTree;
+          tree_keys[out + got] = make_key(s, d);
           got++;
         }
       }
@@ -2394,8 +2485,6 @@ __global__ void k_bump(GView g, const uint64_t *R, const uint32_t *run_beg, cons
         uint32_t d = g.pdst[p];
         if (d != kEmpty) {
           vb.nbr[have + got] = d;
-          bump_keys[out + got] = make_key(s, d);
-          bump_route[out + got] = kRouteNone;  // removed here
           g.pdst[p] = kEmpty;
           if (atomicSub(&g.pcnt[p >> kLeafShift], 1u) - 1u <= lo0)
             atomicOr(reinterpret_cast<unsigned long long *>(low), 1ull);
@@ -3587,11 +3676,7 @@ void Graph::insert_sorted(const uint64_t *R, uint64_t nR) {
 // the first write.  The fallbacks (promotion, the copy-on-write tree pass,
 // the PMA window pass and growth) size theirs by counts known only later.
 void Graph::presize_insert(uint64_t nR, uint32_t nruns) {
-  for (int i : {2, 3, 5, 6, 7}) s32_[i].ensure(nruns + 1, stream_);
-  s32_[4].ensure(static_cast<uint64_t>(nruns) * kInline + 1, stream_);
-  s8_[3].ensure(nruns + 1, stream_);
-  s8_[4].ensure(nR + 1, stream_);
-  k2_.ensure(nR + 1, stream_);
+  for (int i : {5, 6, 7}) s32_[i].ensure(nruns + 1, stream_);
   k3_.ensure(nR + 1, stream_);
   g0_.ensure(nR + 1, stream_);
   u0_.ensure(nR + 1, stream_);
@@ -3619,51 +3704,25 @@ bool Graph::insert_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, co
   lists_dirty_ = true;
   ++epoch_;
   GView g = view();
-  uint32_t *ex = f0_.get(), *nidx = f1_.get();
-  DevBuf<uint32_t> &run_beg = s32_[0], &run_c = s32_[2], &run_nko = s32_[3], &kick_old = s32_[4],
-                   &promote = s32_[5], &ppos = s32_[6];
-  DevBuf<uint8_t> &run_route = s8_[3], &kroute = s8_[4];
-  run_c.ensure(capRuns + 1, stream_);
-  run_nko.ensure(capRuns + 1, stream_);
-  kick_old.ensure(static_cast<uint64_t>(capRuns) * kInline + 1, stream_);
+  DevBuf<uint32_t> &run_beg = s32_[0], &promote = s32_[5], &ppos = s32_[6], &old_deg = s32_[7];
   promote.ensure(capRuns + 1, stream_);
   ppos.ensure(capRuns + 1, stream_);
-  run_route.ensure(capRuns + 1, stream_);
-  kroute.ensure(capR + 1, stream_);
-  // the old degrees of promoted vertices are needed after the merge
-  DevBuf<uint32_t> &old_deg = s32_[7];
-  old_deg.ensure(capRuns + 1, stream_);
-  uint64_t *gw = dsc_.get() + bsw::kGate;
-  // (also clears the kicked keys' routes)
-  k_inline_merge<<<grid_for(capR, 256), 256, 0, stream_>>>(g, R, ex, nidx, run_beg.get(), cw, gate, gw, capR, capRuns,
-                                                           run_route.get(), run_c.get(), run_nko.get(),
-                                                           kick_old.get(), promote.get(), old_deg.get(),
-                                                           kroute.get());
+  old_deg.ensure(capRuns + 1, stream_);  // promotions need the degrees before the merge
+  k3_.ensure(capR + 1, stream_);
+  g0_.ensure(capR + 1, stream_);
+  // the inline merge of every run and the kicked keys split by tier (PMA
+  // keys into k3_, count dsc_[3]; tree keys into g0_, [4]; promotions [5])
+  const uint32_t nb = static_cast<uint32_t>(std::max<uint64_t>(1, (capRuns + kRunBlock - 1) / kRunBlock));
+  const uint32_t epoch = scan_scratch.lb.begin(3ull * nb, stream_);
+  k_insert_runs<<<nb, kRunBlock, 0, stream_>>>(g, R, f0_.get(), f1_.get(), run_beg.get(), cw, gate,
+                                               dsc_.get() + bsw::kGate, capR, capRuns, k3_.get(), g0_.get(),
+                                               promote.get(), ppos.get(), old_deg.get(), dsc_.get() + 3,
+                                               scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(),
+                                               scan_scratch.lb.err_word(), nb, epoch);
   TG_CHECK_LAUNCH();
   if (debug_fail & 2u) {
     debug_fail &= ~2u;
     throw Error(-2, "injected allocation failure after the inline merge");
-  }
-  // kicked keys, sorted per run, placed in kick buffer k2_ (only routed
-  // entries are read back)
-  k2_.ensure(capR + 1, stream_);
-  k_place_kicked<<<grid_for(capR, 256), 256, 0, stream_>>>(R, ex, nidx, gw, run_beg.get(), run_route.get(),
-                                                           run_c.get(), run_nko.get(), kick_old.get(), k2_.get(),
-                                                           kroute.get());
-  TG_CHECK_LAUNCH();
-  // split kicked keys by tier and count promotions (counts stay on the
-  // device: PMA keys dsc_[3], tree keys [4], promotions [5])
-  k3_.ensure(capR + 1, stream_);
-  g0_.ensure(capR + 1, stream_);
-  if (capR <= kSmallSortKeys) {
-    k_small_route<<<1, 1024, 0, stream_>>>(k2_.get(), kroute.get(), static_cast<uint32_t>(capR), k3_.get(),
-                                           g0_.get(), promote.get(), capRuns, ppos.get(), dsc_.get() + 3, gw);
-    TG_CHECK_LAUNCH();
-  } else {
-    select2(SelRoutes{k2_.get(), kroute.get(), k3_.get(), g0_.get(), gw}, capR, dsc_.get() + 3, dsc_.get() + 4,
-            scan_scratch, stream_);
-    exclusive_scan_u32(promote.get(), ppos.get(), capRuns, ppos.get() + capRuns, scan_scratch, stream_,
-                       dsc_.get() + 5);
   }
   // The in-place tree update and the in-leaf PMA merge (disjoint data) run
   // on the device's counts.  The tree update waits (status 2) while
@@ -3994,13 +4053,10 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
 void Graph::presize_delete(uint64_t nR, uint32_t nruns) {
   s32_[12].ensure(nruns + 1, stream_);
   s32_[13].ensure(nruns + 1, stream_);
-  s32_[31].ensure(1, stream_);
-  s8_[5].ensure(nR + 1, stream_);
-  s8_[6].ensure(nR + 1, stream_);
   k3_.ensure(nR + 1, stream_);
   g0_.ensure(nR + 1, stream_);
   g2_.ensure(nR + 1, stream_);
-  for (int i : {1, 2, 3}) s64_[i].ensure(nR + 1, stream_);
+  s64_[3].ensure(nR + 1, stream_);
   s32_[30].ensure(1 + 3 * std::min<uint64_t>(nR, kTreeSmallKeys), stream_);
 }
 
@@ -4017,30 +4073,17 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const ui
 }
 
 // The inline refill from the smallest secondary neighbours (k_bump; it
-// removes the PMA elements it moves itself and counts them in ds[9]) and
-// its tier split (ds[16] PMA keys: none, k_bump routes them nowhere; ds[17]
-// tree keys; the refill count is ds[6]).
-void Graph::delete_refill(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint32_t *run_beg, uint64_t *ds,
+// removes the PMA elements it moves itself and counts them in ds[9]; the
+// tree keys it moves go to s64_[3] at the offsets k_delete_runs gave, their
+// count in ds[17]).
+void Graph::delete_refill(const uint64_t *R, uint32_t capRuns, const uint32_t *run_beg, uint64_t *ds,
                           bool trees_only) {
   GView g = view();
-  DevBuf<uint32_t> &run_bump = s32_[12], &bump_off = s32_[13];
-  DevBuf<uint64_t> &bkeys = s64_[1], &bpma = s64_[2], &btree = s64_[3];
-  DevBuf<uint8_t> &broute = s8_[6];
-  DevBuf<uint32_t> &none = s32_[31];  // no per-run counts for the refill's split
-  const uint64_t *gw = ds + bsw::kGate;
-  k_bump<<<grid_for(capRuns, 256), 256, 0, stream_>>>(g, R, run_beg, gw, run_bump.get(), bump_off.get(), bkeys.get(),
-                                                      broute.get(), trees_only, ds + 9,
+  k_bump<<<grid_for(capRuns, 256), 256, 0, stream_>>>(g, R, run_beg, ds + bsw::kGate, s32_[12].get(), s32_[13].get(),
+                                                      s64_[3].get(), trees_only, ds + 9,
                                                       static_cast<uint32_t>(bounds_.lo[0]), dsc_.get() + kLowFlag);
   TG_CHECK_LAUNCH();
   tree_dirty_ = true;
-  if (capR <= kSmallSortKeys) {
-    k_small_route<<<1, 1024, 0, stream_>>>(bkeys.get(), broute.get(), static_cast<uint32_t>(capR), bpma.get(),
-                                           btree.get(), nullptr, 0, none.get(), ds + 16, ds + 6);
-    TG_CHECK_LAUNCH();
-  } else {
-    select2(SelRoutes{bkeys.get(), broute.get(), bpma.get(), btree.get(), ds + 6}, capR, ds + 16, ds + 17,
-            scan_scratch, stream_);
-  }
 }
 
 // The delete proper: R holds existing keys only, run_beg their run starts
@@ -4055,35 +4098,27 @@ void Graph::delete_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, co
   lists_dirty_ = true;
   ++epoch_;
   GView g = view();
-  DevBuf<uint32_t> &run_bump = s32_[12], &bump_off = s32_[13];
-  DevBuf<uint8_t> &sec_route = s8_[5];
-  run_bump.ensure(capRuns + 1, stream_);
-  bump_off.ensure(capRuns + 1, stream_);
-  sec_route.ensure(capR + 1, stream_);
-  uint64_t *gw = ds + bsw::kGate;
-  k_delete_runs<<<grid_for(capR, 256), 256, 0, stream_>>>(g, R, run_beg, cw, gate, spec, pma_launched, gw, capR,
-                                                          capRuns, sec_route.get(), run_bump.get(), ds + 9,
-                                                          dsc_.get() + kLowFlag);
+  s32_[12].ensure(capRuns + 1, stream_);  // inline refills per run
+  s32_[13].ensure(capRuns + 1, stream_);  // B-tree runs: offset of their refill keys
+  k3_.ensure(capR + 1, stream_);
+  g0_.ensure(capR + 1, stream_);
+  s64_[3].ensure(capR + 1, stream_);
+  // inline deletes of every run and the secondary deletes split by tier
+  // (PMA keys into k3_, count ds[4]; tree keys into g0_, ds[5])
+  const uint32_t nb = static_cast<uint32_t>(std::max<uint64_t>(1, (capRuns + kRunBlock - 1) / kRunBlock));
+  const uint32_t epoch = scan_scratch.lb.begin(4ull * nb, stream_);
+  k_delete_runs<<<nb, kRunBlock, 0, stream_>>>(g, R, run_beg, cw, gate, spec, pma_launched, ds + bsw::kGate, capR,
+                                               capRuns, k3_.get(), g0_.get(), s32_[12].get(), s32_[13].get(), ds,
+                                               dsc_.get() + kLowFlag, scan_scratch.lb.state.get(),
+                                               scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb, epoch);
   TG_CHECK_LAUNCH();
   if (debug_fail & 2u) {
     debug_fail &= ~2u;
     throw Error(-2, "injected allocation failure after the inline deletes");
   }
-  // secondary deletes by tier, and how many inline slots to refill
-  k3_.ensure(capR + 1, stream_);
-  g0_.ensure(capR + 1, stream_);
-  if (capR <= kSmallSortKeys) {
-    k_small_route<<<1, 1024, 0, stream_>>>(R, sec_route.get(), static_cast<uint32_t>(capR), k3_.get(), g0_.get(),
-                                           run_bump.get(), capRuns, bump_off.get(), ds + 4, gw);
-    TG_CHECK_LAUNCH();
-  } else {
-    select2(SelRoutes{R, sec_route.get(), k3_.get(), g0_.get(), gw}, capR, ds + 4, ds + 5, scan_scratch, stream_);
-    exclusive_scan_u32(run_bump.get(), bump_off.get(), capRuns, bump_off.get() + capRuns, scan_scratch, stream_,
-                       ds + 6);
-  }
   pma_delete_launch(k3_.get(), capR, ds + 4);
   tree_small_launch(g0_.get(), capR, false, ds + 5, nullptr, ds + 13);
-  delete_refill(R, capR, capRuns, run_beg, ds, false);
+  delete_refill(R, capRuns, run_beg, ds, false);
   tree_small_launch(s64_[3].get(), capR, false, ds + 17, ds + 13, ds + 19);
 }
 
@@ -4092,7 +4127,7 @@ void Graph::delete_finish(const uint64_t *hs, const uint64_t *R, const uint32_t 
   const uint64_t nR = hd[bsw::kGate];
   const uint32_t nruns = static_cast<uint32_t>(hd[bsw::kGate + 1]);
   const uint64_t npma = hd[4], ntree = hd[5], nb = hd[6];
-  DevBuf<uint64_t> &bpma = s64_[2], &btree = s64_[3];
+  DevBuf<uint64_t> &btree = s64_[3];
   pma_elems_ -= npma + hd[9];  // deleted PMA keys, PMA elements the refill moved inline
   bool fallback = false;
   if (ntree && hd[13] == 0) stats_.tree_small_updates++;
@@ -4104,11 +4139,8 @@ void Graph::delete_finish(const uint64_t *hs, const uint64_t *R, const uint32_t 
     tree_delete_cow(g0_.get(), ntree);
     if (nb) {
       fill_u64(ds + 9, 0, 1, stream_);  // a trees-only refill moves no PMA element
-      delete_refill(R, nR, nruns, run_beg, ds, true);
-      uint64_t h2[2];
-      read_scalars(ds + 16, h2, 2, stream_);
-      pma_delete(bpma.get(), h2[0]);
-      tree_delete(btree.get(), h2[1]);
+      delete_refill(R, nruns, run_beg, ds, true);
+      tree_delete(btree.get(), hd[17]);  // the same count: B-tree runs refill as before
     }
   } else if (hd[17]) {
     if (hd[19] == 0) {
@@ -4117,10 +4149,6 @@ void Graph::delete_finish(const uint64_t *hs, const uint64_t *R, const uint32_t 
       fallback = true;
       tree_delete_cow(btree.get(), hd[17]);
     }
-  }
-  if (hd[16]) {  // refilled PMA keys are removed by k_bump itself; kept for safety
-    fallback = true;
-    pma_delete(bpma.get(), hd[16]);
   }
   if (ws_) {
     ws_->erase(R, nR, stream_);  // R holds exactly the deleted (existing) keys

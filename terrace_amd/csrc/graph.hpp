@@ -322,9 +322,8 @@ class Graph {
   void delete_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint32_t *run_beg,
                      const uint64_t *cw, const uint64_t *gate, bool spec, bool pma_launched, uint64_t *ds);
   void delete_finish(const uint64_t *hs, const uint64_t *R, const uint32_t *run_beg, uint64_t *ds);
-  // k_bump and the refill's tier split (ds[16] PMA keys, ds[17] tree keys)
-  void delete_refill(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint32_t *run_beg, uint64_t *ds,
-                     bool trees_only);
+  // k_bump: the inline refill (tree keys it moves into s64_[3], count ds[17])
+  void delete_refill(const uint64_t *R, uint32_t capRuns, const uint32_t *run_beg, uint64_t *ds, bool trees_only);
   // keys and runs written to the device (bsw::kPut); returns their address
   const uint64_t *put_counts(uint64_t keys, uint64_t runs);
   uint64_t compact_keys(const uint64_t *in, const uint32_t *flags, uint64_t n, uint64_t *out);
