@@ -88,12 +88,14 @@ def apply_batch(G, bt):
     G.update_batch_dev(s, d, w, bs, bd, nd)
 
 
-def step_bytes(st, b_ins, b_del):
-    """Algorithmic HBM bytes of one mixed step: inserts 8b + 4b + 16u (read
+def step_bytes(st1, st0, b_ins, b_del):
+    """Algorithmic HBM bytes of mixed steps: inserts 8b + 4b + 16u (read
     the pairs, write the ids, read and write each source's metadata),
-    deletes 8b + 16u, with u the batch's distinct sources as counted on the
-    device for the step (tg_stats)."""
-    return 12 * b_ins + 16 * st["last_ins_sources"] + 8 * b_del + 16 * st["last_del_sources"]
+    deletes 8b + 16u, with u the batches' distinct sources as counted on
+    the device (tg_stats after minus before the steps)."""
+    u_ins = st1["sum_ins_sources"] - st0["sum_ins_sources"]
+    u_del = st1["sum_del_sources"] - st0["sum_del_sources"]
+    return 12 * b_ins + 16 * u_ins + 8 * b_del + 16 * u_del
 
 
 def copy_probe():
@@ -812,11 +814,20 @@ def main():
     for i in range(args.warmup):
         apply_batch(G, batches[i])
     tg.lib().tg_device_sync()
+    # the timed steps also carry the roofline's HIP events: one before the
+    # first step on the graph's stream and one after the last (nothing is
+    # recorded between steps)
+    st0 = G.stats()
+    G.profile(True)
+    G.profile_reset()
     t0 = time.perf_counter()
     for i in range(args.steps):
         apply_batch(G, batches[args.warmup + i])
     tg.lib().tg_device_sync()
     t_steps = time.perf_counter() - t0
+    step_ms, step_n = G.profile_get("update_step")
+    G.profile(False)
+    st1 = G.stats()
     upd = sum(b[2] + b[5] for b in batches[args.warmup:])
     ins = sum(b[2] for b in batches[args.warmup:])
     value = upd / t_steps
@@ -824,27 +835,17 @@ def main():
 
     extra = {"static_build_edges_per_s": m_static / t_static, "static_edges": m_static,
              "insert_share_edges_per_s": ins / t_steps, "delete_share_edges_per_s": (upd - ins) / t_steps}
-    # Roofline of the headline work: the mixed update step, timed with HIP
-    # events on the graph's stream over as many further steps as timed
-    # (fresh batches), each step's algorithmic bytes from the sources the
-    # device counted for it.
-    prof = make_batches(tg, G, n, probs, args.batch, args.steps, 3000, rng, off, nbr)
-    G.profile(True)
-    G.profile_reset()
-    step_b = []
-    for bt in prof:
-        apply_batch(G, bt)
-        step_b.append(step_bytes(G.stats(), bt[2], bt[5]))
-    step_ms, step_n = G.profile_get("update_step")
-    G.profile(False)
-    del prof
+    # Roofline of the headline work: the mixed update step over the timed
+    # steps, its time from the HIP events above, its algorithmic bytes from
+    # the sources the device counted for the steps.
     step_avg_ms = step_ms / max(step_n, 1)
-    achieved = (sum(step_b) / len(step_b)) / (step_avg_ms * 1e-3) / 1e9
+    alg_bytes = step_bytes(st1, st0, ins, upd - ins) / max(step_n, 1)
+    achieved = alg_bytes / (step_avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": UPDATE_STEP_TRAFFIC_DEFAULT if default_workload else None,
                 "kernel": "mixed update step (tg_update_batch_dev: sort, existence, tier merges, PMA and "
                           "B-tree updates; every kernel of the step)",
-                "avg_ms": step_avg_ms, "launches": step_n, "algorithmic_bytes": sum(step_b) / len(step_b),
+                "avg_ms": step_avg_ms, "launches": step_n, "algorithmic_bytes": alg_bytes,
                 "bytes_formula": "12 b_ins + 16 u_ins + 8 b_del + 16 u_del"}
     log(f"update step: {step_avg_ms:.4f} ms (events), {roofline['algorithmic_bytes']:.3e} B, frac {roofline['frac']:.2e}")
     if not args.no_extra:

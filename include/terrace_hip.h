@@ -358,7 +358,11 @@ int tg_device_sync(void);
 
 /* ---- measurement -------------------------------------------------------
  * With profiling on, kernels of interest are bracketed by hipEvents on
- * the graph's stream; tg_profile_get sums their durations by name. */
+ * the graph's stream; tg_profile_get sums their durations by name.
+ * "update_step" is the span of tg_update_batch_dev calls since the last
+ * reset: an event before the first call, and one that tg_profile_get
+ * records on the graph's stream, with launches = the calls (nothing is
+ * recorded between calls). */
 int tg_profile_enable(tg_graph *g, int on);
 int tg_profile_reset(tg_graph *g);
 int tg_profile_get(tg_graph *g, const char *name, double *ms, uint64_t *launches);
@@ -368,10 +372,12 @@ int tg_profile_get(tg_graph *g, const char *name, double *ms, uint64_t *launches
  * [8] PMA inserts merged inside their leaves (no window redistribution),
  * [9] elements held in the PMA; the last update batch: [10] unique
  * inserted edges (present ones included), [11] their distinct sources,
- * [12] deleted edges that existed, [13] their distinct sources;
- * [14] underfull PMA windows respread after deletes, [15] PMA arrays shrunk
- * after deletes */
-int tg_stats(tg_graph *g, uint64_t *out /* [16] */);
+ * [12] deleted edges that existed, [13] the distinct sources of the
+ * deleted edges (of those that existed, for batches of more than 2^18
+ * edges); [14] underfull PMA windows respread after deletes, [15] PMA
+ * arrays shrunk after deletes; since the graph was created: [16] the sum
+ * of [11] and [17] the sum of [13] over the update batches */
+int tg_stats(tg_graph *g, uint64_t *out /* [18] */);
 /* Element count of every PMA leaf (64 slots each), the per-leaf counts
  * that the density bounds of src/PMA.cc:382-394 are checked on (the
  * reference's check_no_full_leaves, PMA.cc, is the same kind of

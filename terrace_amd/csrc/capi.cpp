@@ -25,7 +25,15 @@ struct tg_graph {
   tg::DevBuf<int32_t> s_i32;
   tg::DevBuf<double> s_f64;
   int device = 0;  // the device current when the graph was created
+  // profiled update steps: one span of stream events from the first step
+  // after a reset to the end of the latest, over `steps` calls
+  hipEvent_t span_a = nullptr, span_b = nullptr;
+  uint64_t span_steps = 0;
   explicit tg_graph(uint32_t n) : g(n), t(g) { TG_HIP(hipGetDevice(&device)); }
+  ~tg_graph() {
+    if (span_a) (void)hipEventDestroy(span_a);
+    if (span_b) (void)hipEventDestroy(span_b);
+  }
 };
 
 struct tg_frontier {
@@ -195,16 +203,15 @@ int tg_update_batch_dev(tg_graph *G, const uint32_t *ins_src, const uint32_t *in
       G->g.update_batch(ins_src, ins_dst, n_ins, del_src, del_dst, n_del);
       return;
     }
-    // profiled: the whole step on the graph's stream, as "update_step"
-    struct Ev {
-      hipEvent_t e = nullptr;
-      Ev() { TG_HIP(hipEventCreate(&e)); }
-      ~Ev() { (void)hipEventDestroy(e); }
-    } a, b;
-    TG_HIP(hipEventRecord(a.e, G->g.stream()));
+    // profiled: consecutive steps on the graph's stream, as "update_step":
+    // an event before the first step after a reset; tg_profile_get records
+    // the closing one on the stream (nothing is recorded between steps)
+    if (!G->span_a) {
+      TG_HIP(hipEventCreate(&G->span_a));
+      TG_HIP(hipEventRecord(G->span_a, G->g.stream()));
+    }
     G->g.update_batch(ins_src, ins_dst, n_ins, del_src, del_dst, n_del);
-    TG_HIP(hipEventRecord(b.e, G->g.stream()));
-    G->t.record("update_step", a.e, b.e);
+    G->span_steps++;
   });
 }
 
@@ -1118,6 +1125,9 @@ int tg_profile_reset(tg_graph *G) {
   return guard([&] {
     NEED(G, "null graph");
     G->t.ktimes.clear();
+    if (G->span_a) (void)hipEventDestroy(G->span_a);
+    G->span_a = nullptr;
+    G->span_steps = 0;
   });
 }
 
@@ -1131,6 +1141,15 @@ int tg_profile_get(tg_graph *G, const char *name, double *ms, uint64_t *launches
         *ms = k.ms;
         *launches = k.launches;
       }
+    if (std::strcmp(name, "update_step") == 0 && G->span_a && G->span_steps) {
+      float span = 0;
+      if (!G->span_b) TG_HIP(hipEventCreate(&G->span_b));
+      TG_HIP(hipEventRecord(G->span_b, G->g.stream()));
+      TG_HIP(hipEventSynchronize(G->span_b));
+      TG_HIP(hipEventElapsedTime(&span, G->span_a, G->span_b));
+      *ms = span;
+      *launches = G->span_steps;
+    }
   });
 }
 
@@ -1162,6 +1181,8 @@ int tg_stats(tg_graph *G, uint64_t *out) {
     out[13] = s.last_del_sources;
     out[14] = s.pma_low_windows;
     out[15] = s.pma_shrinks;
+    out[16] = s.sum_ins_sources;
+    out[17] = s.sum_del_sources;
   });
 }
 

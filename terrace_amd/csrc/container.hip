@@ -125,7 +125,9 @@ __device__ inline uint64_t leaf_match(const GView &g, const uint4 *v, uint64_t b
 // word).  The begin leaf's destinations are loaded with its two fences, so
 // a key inside the begin leaf costs one round trip (and the source check
 // where a destination matches); other keys take the searches above.
-__device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d, uint64_t begin) {
+// leaf_out (optional): the leaf that holds or would hold the key.
+__device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d, uint64_t begin,
+                                       uint32_t *leaf_out = nullptr) {
   const uint64_t key = make_key(s, d);
   const uint64_t L0 = begin >> kLeafShift;
   uint4 v[kLeafSlots / 4];
@@ -135,11 +137,15 @@ __device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d, u
     const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + (L0 << kLeafShift));
 #pragma unroll
     for (uint32_t i = 0; i < kLeafSlots / 4; i++) v[i] = pd[i];
-    if (f0 <= key && key < f1) return leaf_match(g, v, L0 << kLeafShift, s, d);
+    if (f0 <= key && key < f1) {
+      if (leaf_out) *leaf_out = static_cast<uint32_t>(L0);
+      return leaf_match(g, v, L0 << kLeafShift, s, d);
+    }
     leaf = f0 <= key ? fence_gallop(g, key, L0 + 1) : pma_find_leaf(g, key);
   } else {
     leaf = pma_find_leaf_from(g, key, begin);
   }
+  if (leaf_out) *leaf_out = static_cast<uint32_t>(leaf);
   const uint64_t base = leaf << kLeafShift;
   const uint4 *pd = reinterpret_cast<const uint4 *>(g.pdst + base);
 #pragma unroll
@@ -210,7 +216,9 @@ __device__ inline bool tree_has(const GView &g, uint32_t t, uint32_t d) {
 
 // Graph::is_edge (graph.h:643-673): inline first, then the secondary tier.
 // The 64-byte block is loaded whole (four 16-byte loads issued together).
-__device__ inline bool edge_exists(const GView &g, uint32_t s, uint32_t d) {
+// leaf_out (optional): the PMA leaf searched for a key of a PMA-tier vertex
+// (the leaf that holds or would hold it), else left as it is.
+__device__ inline bool edge_exists(const GView &g, uint32_t s, uint32_t d, uint32_t *leaf_out = nullptr) {
   const uint4 *vp = reinterpret_cast<const uint4 *>(g.vb + s);
   const uint4 q0 = vp[0], q1 = vp[1], q2 = vp[2], q3 = vp[3];
   // degree, nbr[0..12], aux (low, high)
@@ -226,7 +234,7 @@ __device__ inline bool edge_exists(const GView &g, uint32_t s, uint32_t d) {
   if (D <= kInline || d < w[kInline]) return false;
   const uint64_t aux = static_cast<uint64_t>(w[14]) | (static_cast<uint64_t>(w[15]) << 32);
   if (aux & kAuxTree) return tree_has(g, static_cast<uint32_t>(aux), d);
-  return pma_slot_of(g, s, d, aux) != ~0ull;
+  return pma_slot_of(g, s, d, aux, leaf_out) != ~0ull;
 }
 
 // Merge path: number of A elements among the first m of merge(A, B)
@@ -774,10 +782,13 @@ constexpr int kPrepBlock = 256;
 // (one key per thread keeps the lookups' memory parallelism) and written
 // to ex; otherwise ex holds them already.
 // The work of block b (in start order) of nblocks.
+// leaf (optional, with kLookup): per key the PMA leaf its lookup searched,
+// ~0 when the lookup did not reach the PMA.
 template <int kItems, bool kLookup>
 __device__ inline void insert_prep_block(uint32_t b, GView g, const uint64_t *R, const uint64_t *np, uint64_t cap,
                                          uint32_t *ex, uint32_t *nidx, uint32_t *run_beg, uint64_t *nruns,
-                                         uint64_t *state, uint32_t *err, uint32_t nblocks, uint32_t epoch) {
+                                         uint64_t *state, uint32_t *err, uint32_t nblocks, uint32_t epoch,
+                                         uint32_t *leaf = nullptr) {
   constexpr int kPrepItems = kItems, kPrepTile = kPrepBlock * kItems;
   __shared__ uint32_t wcnt[2][kPrepItems][kPrepBlock / 64];
   __shared__ uint32_t wpre[2][kPrepItems][kPrepBlock / 64];
@@ -795,7 +806,13 @@ __device__ inline void insert_prep_block(uint32_t b, GView g, const uint64_t *R,
     st[i] = false;
     if (idx < n) {
       const uint64_t key = R[idx];
-      e[i] = kLookup ? edge_exists(g, key_src(key), key_dst(key)) : ex[idx] != 0;
+      if (kLookup) {
+        uint32_t lf = ~0u;
+        e[i] = edge_exists(g, key_src(key), key_dst(key), leaf ? &lf : nullptr);
+        if (leaf) leaf[idx] = lf;
+      } else {
+        e[i] = ex[idx] != 0;
+      }
       st[i] = idx == 0 || key_src(R[idx - 1]) != key_src(key);
     }
   }
@@ -907,21 +924,29 @@ __global__ void __launch_bounds__(kPrepBlock) k_delete_prep(GView g, const uint6
 // Both halves of a mixed batch in one launch, so the two latency-bound
 // lookup passes overlap: blocks in start order [0, nb) prepare the inserted
 // keys I (look-back words state[0, 2 nb)), blocks [nb, 2 nb) the deleted
-// keys D (words state[2 nb, 3 nb)).  A block of either half only waits on
-// blocks of its half that started before it.
+// keys D (words state[2 nb, 4 nb)).  A block of either half only waits on
+// blocks of its half that started before it.  The deleted keys get the
+// same outputs as the inserted ones: existence flags (dex), the prefix of
+// the absent keys (dnidx; the existing keys before i are i - dnidx[i]) and
+// their source runs (drun_beg, count *druns), so the delete half runs over
+// D itself and no compaction is needed.  ileaf: the PMA leaf each inserted
+// key's lookup searched (~0 when it did not reach the PMA).
 __global__ void __launch_bounds__(kPrepBlock) k_mixed_prep(GView g, const uint64_t *I, const uint64_t *ni,
                                                            uint64_t cap, uint32_t *ex, uint32_t *nidx,
-                                                           uint32_t *run_beg, uint64_t *nruns, const uint64_t *D,
-                                                           const uint64_t *nd, uint64_t *E, uint64_t *ne,
-                                                           uint64_t *state, uint32_t *misc, uint32_t *err,
-                                                           uint32_t nb, uint32_t epoch) {
+                                                           uint32_t *run_beg, uint64_t *nruns, uint32_t *ileaf,
+                                                           const uint64_t *D,
+                                                           const uint64_t *nd, uint32_t *dex, uint32_t *dnidx,
+                                                           uint32_t *drun_beg, uint64_t *druns, uint64_t *state,
+                                                           uint32_t *misc, uint32_t *err, uint32_t nb,
+                                                           uint32_t epoch) {
   __shared__ uint32_t s_ticket;
   const uint32_t b = lbk::ticket(misc, err, 2 * nb, &s_ticket);
   if (b >= 2 * nb) return;
   if (b < nb)
-    insert_prep_block<1, true>(b, g, I, ni, cap, ex, nidx, run_beg, nruns, state, err, nb, epoch);
+    insert_prep_block<1, true>(b, g, I, ni, cap, ex, nidx, run_beg, nruns, state, err, nb, epoch, ileaf);
   else
-    delete_prep_block(b - nb, g, D, nd, E, ne, state + 2ull * nb, err, nb, epoch);
+    insert_prep_block<1, true>(b - nb, g, D, nd, cap, dex, dnidx, drun_beg, druns, state + 2ull * nb, err, nb,
+                               epoch);
 }
 
 // ------------------------------------------------------------ gates
@@ -1184,13 +1209,18 @@ __device__ inline void inline_merge_serial(GView g, const uint64_t *R, const uin
 // c, merged with the old inline entries pushed out, written sorted to the
 // run's offset in the PMA list P or the B-tree list T.  The offsets and the
 // promotion prefix (ppos) come from the run scan; the block's keys are
-// placed by all its threads (a run's keys are contiguous in R).  Totals:
-// sc[0] PMA keys, sc[1] tree keys, sc[2] promotions.
+// placed by all its threads (a run's keys are contiguous in R).  Each PMA
+// key also gets the leaf its preparation lookup searched (Pleaf, from
+// ileaf when given; ~0 when unknown, for k_leaf_of to search; the PMA does
+// not change before the in-leaf merge reads it).  Totals: sc[0] PMA keys, sc[1] tree keys,
+// sc[2] promotions.
 __global__ void __launch_bounds__(kRunBlock) k_insert_runs(GView g, const uint64_t *R, const uint32_t *ex,
-                                                           const uint32_t *nidx, const uint32_t *run_beg,
+                                                           const uint32_t *nidx, const uint32_t *ileaf,
+                                                           const uint32_t *run_beg,
                                                            const uint64_t *cw, const uint64_t *gate, uint64_t *gw,
                                                            uint64_t cap_keys, uint32_t cap_runs, uint64_t *P,
-                                                           uint64_t *T, uint32_t *promote, uint32_t *ppos,
+                                                           uint32_t *Pleaf, uint64_t *T, uint32_t *promote,
+                                                           uint32_t *ppos,
                                                            uint32_t *old_deg, uint64_t *sc, uint64_t *state,
                                                            uint32_t *misc, uint32_t *err, uint32_t nblocks,
                                                            uint32_t epoch) {
@@ -1202,6 +1232,7 @@ __global__ void __launch_bounds__(kRunBlock) k_insert_runs(GView g, const uint64
   if (b >= nblocks) return;
   const HalfCounts hc = gate_counts(!gate || prep_clean(gate), cw, cap_keys, cap_runs, gw);
   const uint32_t t = threadIdx.x;
+
   const uint64_t r0 = static_cast<uint64_t>(b) * kRunBlock;
   const uint32_t nlive = r0 < hc.runs ? static_cast<uint32_t>(hc.runs - r0 < kRunBlock ? hc.runs - r0 : kRunBlock) : 0u;
   const bool live = t < nlive;
@@ -1243,13 +1274,20 @@ __global__ void __launch_bounds__(kRunBlock) k_insert_runs(GView g, const uint64
   // the old inline entries each run pushed out, at their ranks among its
   // kicked keys (new tail keys below them + their own order)
   if (live && m.route != kRouteNone) {
-    uint64_t *out = (m.route == kRoutePma ? P : T) + s_off[t];
+    const bool pma = m.route == kRoutePma;
+    const uint64_t o = s_off[t];
     const uint32_t s = key_src(R[b0]);
     for (uint32_t e = 0; e < m.nko; e++) {
       const uint32_t xo = s_ko[t][e];
-      const uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, make_key(s, xo));
+      const uint64_t key = make_key(s, xo);
+      const uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, key);
       const uint32_t new_less = nidx[lb] - nb0 - m.c;
-      out[e + new_less] = make_key(s, xo);
+      if (pma) {
+        P[o + e + new_less] = key;
+        Pleaf[o + e + new_less] = ~0u;  // searched by k_leaf_of
+      } else {
+        T[o + e + new_less] = key;
+      }
     }
   }
   // the new keys beyond the first c of each run, by every thread over the
@@ -1268,20 +1306,30 @@ __global__ void __launch_bounds__(kRunBlock) k_insert_runs(GView g, const uint64
     const uint32_t j = nidx[i] - s_nb0[lo];
     const uint32_t c = s_c[lo];
     if (j < c) continue;
-    const uint32_t d = key_dst(R[i]);
+    const uint64_t key = R[i];
+    const uint32_t d = key_dst(key);
     const uint32_t nko = s_nko[lo];
     uint32_t less = 0;
     while (less < nko && s_ko[lo][less] < d) less++;
-    (route == kRoutePma ? P : T)[s_off[lo] + (j - c) + less] = R[i];
+    const uint64_t pos = s_off[lo] + (j - c) + less;
+    if (route == kRoutePma) {
+      P[pos] = key;
+      Pleaf[pos] = ileaf ? ileaf[i] : ~0u;  // ~0: searched by k_leaf_of
+    } else {
+      T[pos] = key;
+    }
   }
 }
 
 // ------------------------------------------------------------ PMA kernels
+// The leaf of each key; with only_unknown, only of the keys whose leaf
+// reads ~0 (the others came from the preparation's lookups).
 __global__ void k_leaf_of(GView g, const uint64_t *keys, uint64_t cap, uint32_t *leaf, uint64_t *fit_flag,
-                          const uint64_t *kdev) {
+                          const uint64_t *kdev, bool only_unknown) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *fit_flag = 0;  // set later by k_pma_leaf_fit
   const uint64_t k = dev_count(kdev, cap);
   GRID_STRIDE(i, k) {
+    if (only_unknown && leaf[i] != ~0u) continue;
     const uint64_t key = keys[i];
     leaf[i] = static_cast<uint32_t>(pma_leaf_of(g, key, g.vb[key_src(key)].aux));
   }
@@ -1414,12 +1462,12 @@ __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t 
   uint64_t leaf = 0, base = 0;
   uint32_t nb = 0, c = 0;
   uint32_t sv[kLaneSlots] = {};  // old sources of this lane's slots (kept in emptied slots)
+  uint32_t dd[kLaneSlots] = {};  // old destinations (a quad that stays the same is not stored)
   if (live) {
     leaf = T[t];
     base = leaf << kLeafShift;
     const uint32_t kb = Tstart[t];
     nb = Tstart[t + 1] - kb;
-    uint32_t dd[kLaneSlots];
 #pragma unroll
     for (uint32_t h = 0; h < kLaneQuads; h++) {
       const uint4 dv = reinterpret_cast<const uint4 *>(g.pdst + base)[kLaneQuads * l + h];
@@ -1483,8 +1531,12 @@ __global__ void __launch_bounds__(256) k_pma_leaf_merge(GView g, const uint32_t 
     ns.y = o0 + r + 1 < total ? key_src(out[r + 1]) : sv[r + 1];
     ns.z = o0 + r + 2 < total ? key_src(out[r + 2]) : sv[r + 2];
     ns.w = o0 + r + 3 < total ? key_src(out[r + 3]) : sv[r + 3];
-    reinterpret_cast<uint4 *>(g.pdst + base)[kLaneQuads * l + h] = nd;
-    reinterpret_cast<uint4 *>(g.psrc + base)[kLaneQuads * l + h] = ns;
+    // the slots before the first new key keep their elements (unless holes
+    // left by deletes shift them): only quads that change are written
+    if (nd.x != dd[r] || nd.y != dd[r + 1] || nd.z != dd[r + 2] || nd.w != dd[r + 3])
+      reinterpret_cast<uint4 *>(g.pdst + base)[kLaneQuads * l + h] = nd;
+    if (ns.x != sv[r] || ns.y != sv[r + 1] || ns.z != sv[r + 2] || ns.w != sv[r + 3])
+      reinterpret_cast<uint4 *>(g.psrc + base)[kLaneQuads * l + h] = ns;
   }
   // Run starts in this leaf move the vertex's begin pointer.  A pointer
   // into this leaf (the run started here) moves with a CAS, which fails
@@ -1816,29 +1868,6 @@ __global__ void __launch_bounds__(256) k_tree_up(uint32_t *pcnt, uint32_t H, uin
   }
 }
 
-// PMA delete: clear the slot, decrement the count tree, flag begin fixes.
-// A leaf left at or below the leaf lower bound lo0 raises *low (the batch
-// then respreads the underfull windows, as src/PMA.cc:1636-1746 does per
-// removal).
-__global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t cap, uint64_t *fix_slot, uint64_t *err,
-                             const uint64_t *kdev, uint32_t lo0, uint64_t *low) {
-  const uint64_t k = dev_count(kdev, cap);
-  GRID_STRIDE(i, k) {
-    uint32_t s = key_src(keys[i]), d = key_dst(keys[i]);
-    const uint64_t begin = g.vb[s].aux;
-    uint64_t slot = pma_slot_of(g, s, d, begin);
-    fix_slot[i] = ~0ull;
-    if (slot == ~0ull) {
-      atomicOr(reinterpret_cast<unsigned long long *>(err), 4ull);
-      continue;
-    }
-    g.pdst[slot] = kEmpty;
-    uint64_t leaf = slot >> kLeafShift;
-    if (atomicSub(&g.pcnt[leaf], 1u) - 1u <= lo0) atomicOr(reinterpret_cast<unsigned long long *>(low), 1ull);
-    if (begin == slot) fix_slot[i] = slot;
-  }
-}
-
 // Begin slot of vertex s from slot p on: the first occupied slot at or
 // after p if it holds an element of s, else kPmaNone (the PMA is sorted, so
 // the vertex has no element later).  A leaf at a time: its count and all
@@ -1869,16 +1898,53 @@ __device__ inline uint64_t next_begin(const GView &g, uint32_t s, uint64_t p) {
   return kPmaNone;
 }
 
-// New begin for vertices whose first PMA element was deleted: the next
-// occupied slot if it still belongs to the vertex.
-__global__ void k_pma_fix_begin(GView g, const uint64_t *keys, uint64_t cap, const uint64_t *fix_slot,
-                                const uint64_t *kdev) {
+// First slot at or after p holding an element of s that this batch does
+// not delete, or kPmaNone.  keys[j, k) are the batch's PMA deletes after
+// the caller's (sorted, so those of s come first); the slots they empty may
+// read either way while the other threads work, so each element found is
+// checked against them (a merge of two sorted lists).  Empty leaves are
+// skipped by their counts.
+__device__ inline uint64_t next_begin_after(const GView &g, uint32_t s, uint64_t p, const uint64_t *keys, uint64_t j,
+                                            uint64_t k) {
+  while (p < g.N) {
+    const uint64_t leaf = p >> kLeafShift, end = (leaf + 1) << kLeafShift;
+    if (g.pcnt[leaf] == 0) {
+      p = end;
+      continue;
+    }
+    for (; p < end; p++) {
+      const uint32_t d = g.pdst[p];
+      if (d == kEmpty) continue;
+      if (g.psrc[p] != s) return kPmaNone;  // the PMA is sorted: no later element of s
+      const uint64_t key = make_key(s, d);
+      while (j < k && keys[j] < key) j++;
+      if (j < k && keys[j] == key) continue;  // deleted by this batch
+      return p;
+    }
+  }
+  return kPmaNone;
+}
+
+// PMA delete: clear the slot and lower the leaf count.  The thread whose key
+// is its vertex's first element also moves the vertex's begin pointer to
+// the first element the batch keeps.  A leaf left at or below the leaf
+// lower bound lo0 raises *low (the batch then respreads the underfull
+// windows, as src/PMA.cc:1636-1746 does per removal).
+__global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t cap, uint64_t *err, const uint64_t *kdev,
+                             uint32_t lo0, uint64_t *low) {
   const uint64_t k = dev_count(kdev, cap);
   GRID_STRIDE(i, k) {
-    uint64_t slot = fix_slot[i];
-    if (slot == ~0ull) continue;
-    const uint32_t s = key_src(keys[i]);
-    g.vb[s].aux = next_begin(g, s, slot + 1);
+    const uint32_t s = key_src(keys[i]), d = key_dst(keys[i]);
+    const uint64_t begin = g.vb[s].aux;
+    const uint64_t slot = pma_slot_of(g, s, d, begin);
+    if (slot == ~0ull) {
+      atomicOr(reinterpret_cast<unsigned long long *>(err), 4ull);
+      continue;
+    }
+    g.pdst[slot] = kEmpty;
+    const uint64_t leaf = slot >> kLeafShift;
+    if (atomicSub(&g.pcnt[leaf], 1u) - 1u <= lo0) atomicOr(reinterpret_cast<unsigned long long *>(low), 1ull);
+    if (begin == slot) g.vb[s].aux = next_begin_after(g, s, slot + 1, keys, i + 1, k);
   }
 }
 
@@ -2310,7 +2376,9 @@ __global__ void __launch_bounds__(256) k_tree_small_apply(GView g, const uint64_
 }
 
 // ------------------------------------------------------------ delete batch
-// The delete half's first kernel, one thread per run of existing keys: its
+// The delete half's first kernel, one thread per run of deleted keys (exf:
+// their existence flags and dnidx the prefix of the absent ones; null when
+// every key of R exists): its
 // gate (gate null: open; spec: also the insert half of the same batch must
 // have completed) and counts (cw, published at gw); the run's inline hits
 // removed (kept entries move down over them, the degree drops by the run's
@@ -2320,9 +2388,11 @@ __global__ void __launch_bounds__(256) k_tree_small_apply(GView g, const uint64_
 // each B-tree run's offset for the tree keys its refill moves (tbump_off).
 // Totals: ds[4] PMA deletes, ds[5] tree deletes, ds[6] inline refills,
 // ds[17] refill keys for the tree delete (ds[16], refill keys for the PMA,
-// is 0: k_bump removes those itself).  ds[9] (k_bump's counter of PMA
-// elements moved inline) and, when the half runs, *low are cleared here.
-__global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64_t *R, const uint32_t *run_beg,
+// is 0: k_bump removes those itself); ds[3] the existing keys deleted.
+// ds[9] (k_bump's counter of PMA elements moved inline) and, when the half
+// runs, *low are cleared here.
+__global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64_t *R, const uint32_t *exf,
+                                                           const uint32_t *dnidx, const uint32_t *run_beg,
                                                            const uint64_t *cw, const uint64_t *gate, bool spec,
                                                            bool pma_launched, uint64_t *gw, uint64_t cap_keys,
                                                            uint32_t cap_runs, uint64_t *P, uint64_t *T,
@@ -2330,7 +2400,7 @@ __global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64
                                                            uint64_t *low, uint64_t *state, uint32_t *misc,
                                                            uint32_t *err, uint32_t nblocks, uint32_t epoch) {
   __shared__ uint32_t s_ticket;
-  __shared__ uint32_t s_b0[kRunBlock + 1], s_off[kRunBlock];
+  __shared__ uint32_t s_b0[kRunBlock + 1], s_off[kRunBlock], s_nb0[kRunBlock];
   __shared__ uint8_t s_tier[kRunBlock], s_nhit[kRunBlock];
   __shared__ uint32_t s_hit[kRunBlock][kInline];  // the inline entries the run deletes, sorted
   const uint32_t b = lbk::ticket(misc, err, nblocks, &s_ticket);
@@ -2340,16 +2410,20 @@ __global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64
   const uint32_t t = threadIdx.x;
   if (blockIdx.x == 0 && t == 0) {
     ds[9] = 0;
+    ds[3] = exf ? hc.keys - dnidx[hc.keys] : hc.keys;  // the existing keys deleted
     if (ok) *low = 0;
   }
   const uint64_t r0 = static_cast<uint64_t>(b) * kRunBlock;
   const uint32_t nlive = r0 < hc.runs ? static_cast<uint32_t>(hc.runs - r0 < kRunBlock ? hc.runs - r0 : kRunBlock) : 0u;
   const bool live = t < nlive;
   const uint64_t r = r0 + t;
-  uint32_t nsec = 0, y = 0, nhit = 0;
+  uint32_t nsec = 0, y = 0, nhit = 0, nb0 = 0;
   uint8_t tier = kRoutePma;
   if (live) {
     const uint32_t b0 = run_beg[r], b1 = run_beg[r + 1];
+    nb0 = exf ? dnidx[b0] : 0u;
+    // existing keys of the run (an absent key matches no inline entry)
+    const uint32_t len = (b1 - b0) - (exf ? dnidx[b1] - nb0 : 0u);
     const uint32_t s = key_src(R[b0]);
     // the block as 16 words (degree, nbr[0..12], aux): registers only, the
     // inline hits as a 13-bit mask (no run-time indexed private array)
@@ -2360,9 +2434,8 @@ __global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64
     const uint32_t D = w[0];
     const uint32_t lim = D < kInline ? D : kInline;
     tier = (w[15] & 0x80000000u) ? kRouteTree : kRoutePma;  // aux bit 63: B-tree
-    const uint32_t len = b1 - b0;
     uint32_t hit = 0;
-    if (len <= 2 * kInline) {
+    if (b1 - b0 <= 2 * kInline) {
       for (uint32_t q = b0; q < b1; q++) {
         const uint32_t d = key_dst(R[q]);
 #pragma unroll
@@ -2374,7 +2447,7 @@ __global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64
       for (uint32_t p = 0; p < kInline; p++) {
         if (p >= lim) continue;
         const uint64_t key = make_key(s, w[1 + p]);
-        const uint64_t lb = b0 + lower_bound_dev(R + b0, len, key);
+        const uint64_t lb = b0 + lower_bound_dev(R + b0, b1 - b0, key);
         if (lb < b1 && R[lb] == key) hit |= 1u << p;
       }
     }
@@ -2394,7 +2467,7 @@ __global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64
     const uint32_t sec_before = D > kInline ? D - kInline : 0;
     const uint32_t sec_after = sec_before - (len - x);
     y = x < sec_after ? x : sec_after;
-    vb.degree = D - len;
+    if (len) vb.degree = D - len;
     run_bump[r] = y;
     nsec = len - x;
     s_b0[t] = b0;
@@ -2405,6 +2478,7 @@ __global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64
   uint32_t pre[4], tot[4];
   run_block_scan<4>(xs, pre, tot, state, b, err, epoch);
   if (live) tbump_off[r] = pre[3];
+  s_nb0[t] = nb0;
   s_off[t] = tier == kRoutePma ? pre[0] : pre[1];
   s_tier[t] = tier;
   s_nhit[t] = static_cast<uint8_t>(nhit);
@@ -2418,9 +2492,11 @@ __global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64
   }
   if (nlive == 0) return;
   // the secondary deletes, by every thread over the block's keys: a key's
-  // position is its rank in the run minus the inline hits below it
+  // position is its rank among the run's existing keys minus the inline
+  // hits below it
   const uint32_t kb = s_b0[0], ke = s_b0[nlive];
   for (uint32_t i = kb + t; i < ke; i += kRunBlock) {
+    if (exf && !exf[i]) continue;
     uint32_t lo = 0, hi = nlive;  // run: last q with s_b0[q] <= i
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -2437,7 +2513,8 @@ __global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64
       inl |= v == d;
     }
     if (inl) continue;
-    (s_tier[lo] == kRoutePma ? P : T)[s_off[lo] + (i - s_b0[lo]) - less] = R[i];
+    const uint32_t rank = (i - s_b0[lo]) - (exf ? dnidx[i] - s_nb0[lo] : 0u);
+    (s_tier[lo] == kRoutePma ? P : T)[s_off[lo] + rank - less] = R[i];
   }
 }
 
@@ -2800,6 +2877,8 @@ __global__ void k_flag_tree_change(const Grp *grp, uint32_t ng, uint32_t *f) {
 Graph::Graph(uint32_t n) : n_(n) {
   if (n == 0 || n > 0x7FFFFFFFu) throw Error(-1, "vertex count must be in [1, 2^31)");
   TG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  TG_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+  for (auto &e : ev_) TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   vb_.ensure(n, stream_);
   TG_HIP(hipMemsetAsync(vb_.get(), 0, sizeof(VBlock) * static_cast<uint64_t>(n), stream_));
   k_init_aux<<<grid_for(n, 256), 256, 0, stream_>>>(vb_.get(), n);
@@ -2818,10 +2897,26 @@ Graph::Graph(uint32_t n) : n_(n) {
 }
 
 Graph::~Graph() {
+  if (side_) {
+    (void)hipStreamSynchronize(side_);
+    (void)hipStreamDestroy(side_);
+  }
+  for (auto &e : ev_)
+    if (e) (void)hipEventDestroy(e);
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
   }
+}
+
+void Graph::fork_side(int e) {
+  TG_HIP(hipEventRecord(ev_[e], stream_));
+  TG_HIP(hipStreamWaitEvent(side_, ev_[e], 0));
+}
+
+void Graph::join_side(int e) {
+  TG_HIP(hipEventRecord(ev_[e], side_));
+  TG_HIP(hipStreamWaitEvent(stream_, ev_[e], 0));
 }
 
 GView Graph::view() const {
@@ -3193,14 +3288,14 @@ void Graph::pma_insert(const uint64_t *keys, uint64_t k) {
 // dsc_[8]) and the in-leaf merge, which writes nothing unless every touched
 // leaf fits (fit flag in dsc_[7]).  Needs pma_elems_ + k within the root
 // bound.
-void Graph::pma_fast_launch(const uint64_t *keys, uint64_t k, const uint64_t *kdev) {
+void Graph::pma_fast_launch(const uint64_t *keys, uint64_t k, const uint64_t *kdev, bool leaves_known) {
   GView g = view();
   DevBuf<uint32_t> &T = f1_, &Tstart = f2_;
   u0_.ensure(k + 1, stream_);  // leaf of key
   T.ensure(k + 1, stream_);
   Tstart.ensure(k + 1, stream_);
   uint64_t *ntp = dsc_.get() + 8, *fitp = dsc_.get() + 7;
-  k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get(), fitp, kdev);
+  k_leaf_of<<<grid_for(k, 256), 256, 0, stream_>>>(g, keys, k, u0_.get(), fitp, kdev, leaves_known);
   TG_CHECK_LAUNCH();
   select(SelTouched{u0_.get(), T.get(), Tstart.get(), kdev}, k, ntp, scan_scratch, stream_);
   // kLeafLanes lanes per touched leaf (at most min(k, leaves) of them)
@@ -3336,14 +3431,11 @@ void Graph::pma_rebalance_low() {
 // caller lowers pma_elems_ by the count.
 void Graph::pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev) {
   GView g = view();
-  g2_.ensure(cap + 1, stream_);
-  k_pma_delete<<<grid_for(cap, 256), 256, 0, stream_>>>(g, keys, cap, g2_.get(), dsc_.get(), kdev,
+  k_pma_delete<<<grid_for(cap, 256), 256, 0, stream_>>>(g, keys, cap, dsc_.get(), kdev,
                                                          static_cast<uint32_t>(bounds_.lo[0]),
                                                          dsc_.get() + kLowFlag);
   TG_CHECK_LAUNCH();
   tree_dirty_ = true;
-  k_pma_fix_begin<<<grid_for(cap, 256), 256, 0, stream_>>>(g, keys, cap, g2_.get(), kdev);
-  TG_CHECK_LAUNCH();
 }
 
 void Graph::pma_delete(const uint64_t *keys, uint64_t k) {
@@ -3358,22 +3450,23 @@ void Graph::pma_delete(const uint64_t *keys, uint64_t k) {
 // more keys give status 1); the groups kernel lists the blocks to rewrite,
 // the apply kernel's grid covers the most groups cap keys can make.
 void Graph::tree_small_launch(const uint64_t *keys, uint64_t cap, bool insert, const uint64_t *kdev,
-                              const uint64_t *gate, uint64_t *status) {
+                              const uint64_t *gate, uint64_t *status, hipStream_t s) {
   GView g = view();
   DevBuf<uint32_t> &gl = s32_[30];
   const uint64_t most = cap < kTreeSmallKeys ? cap : kTreeSmallKeys;
-  gl.ensure(1 + 3 * most, stream_);
+  gl.ensure(1 + 3 * most, stream_);  // presized on the update path: no reallocation under way
   const uint32_t kk = static_cast<uint32_t>(most);
   const unsigned grid = static_cast<unsigned>(most == 0 ? 1 : (most < 1024 ? most : 1024));
   uint64_t *st = status ? status : dsc_.get() + 13;
+  if (!s) s = stream_;
   if (insert) {
-    k_tree_small_groups<true><<<1, 1024, 0, stream_>>>(g, keys, kk, gl.get(), st, kdev, gate);
+    k_tree_small_groups<true><<<1, 1024, 0, s>>>(g, keys, kk, gl.get(), st, kdev, gate);
     TG_CHECK_LAUNCH();
-    k_tree_small_apply<true><<<grid, 256, 0, stream_>>>(g, keys, gl.get());
+    k_tree_small_apply<true><<<grid, 256, 0, s>>>(g, keys, gl.get());
   } else {
-    k_tree_small_groups<false><<<1, 1024, 0, stream_>>>(g, keys, kk, gl.get(), st, kdev, gate);
+    k_tree_small_groups<false><<<1, 1024, 0, s>>>(g, keys, kk, gl.get(), st, kdev, gate);
     TG_CHECK_LAUNCH();
-    k_tree_small_apply<false><<<grid, 256, 0, stream_>>>(g, keys, gl.get());
+    k_tree_small_apply<false><<<grid, 256, 0, s>>>(g, keys, gl.get());
   }
   TG_CHECK_LAUNCH();
 }
@@ -3635,6 +3728,7 @@ void Graph::insert_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   if (!fused) check_prep(h);
   stats_.last_ins_keys = h[1];
   stats_.last_ins_sources = h[2];
+  stats_.sum_ins_sources += h[2];
   stats_.last_del_keys = stats_.last_del_sources = 0;
   if (fused) {
     mutate([&] {
@@ -3700,7 +3794,7 @@ void Graph::insert_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const ui
 // capacities and reads the counts k_inline_merge published, so nothing
 // here waits for the host.
 bool Graph::insert_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint64_t *cw,
-                          const uint64_t *gate) {
+                          const uint64_t *gate, const uint32_t *ileaf) {
   lists_dirty_ = true;
   ++epoch_;
   GView g = view();
@@ -3709,13 +3803,15 @@ bool Graph::insert_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, co
   ppos.ensure(capRuns + 1, stream_);
   old_deg.ensure(capRuns + 1, stream_);  // promotions need the degrees before the merge
   k3_.ensure(capR + 1, stream_);
+  u0_.ensure(capR + 1, stream_);
   g0_.ensure(capR + 1, stream_);
   // the inline merge of every run and the kicked keys split by tier (PMA
-  // keys into k3_, count dsc_[3]; tree keys into g0_, [4]; promotions [5])
+  // keys into k3_, count dsc_[3], their leaves into u0_; tree keys into g0_,
+  // [4]; promotions [5])
   const uint32_t nb = static_cast<uint32_t>(std::max<uint64_t>(1, (capRuns + kRunBlock - 1) / kRunBlock));
   const uint32_t epoch = scan_scratch.lb.begin(3ull * nb, stream_);
-  k_insert_runs<<<nb, kRunBlock, 0, stream_>>>(g, R, f0_.get(), f1_.get(), run_beg.get(), cw, gate,
-                                               dsc_.get() + bsw::kGate, capR, capRuns, k3_.get(), g0_.get(),
+  k_insert_runs<<<nb, kRunBlock, 0, stream_>>>(g, R, f0_.get(), f1_.get(), ileaf, run_beg.get(), cw, gate,
+                                               dsc_.get() + bsw::kGate, capR, capRuns, k3_.get(), u0_.get(), g0_.get(),
                                                promote.get(), ppos.get(), old_deg.get(), dsc_.get() + 3,
                                                scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(),
                                                scan_scratch.lb.err_word(), nb, epoch);
@@ -3729,9 +3825,14 @@ bool Graph::insert_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, co
   // promotions are pending: their trees must exist first.  The PMA merge is
   // launched when every key could go to the PMA within the root bound (else
   // the general insert decides after the read).
+  // A large batch runs the tree update on the side stream, beside the PMA
+  // kernels, and joins it back before anything after the half.
   const bool pma_launched = pma_elems_ + capR <= bounds_.up[pma_H_];
-  tree_small_launch(g0_.get(), capR, true, dsc_.get() + 4, dsc_.get() + 5);
-  if (pma_launched) pma_fast_launch(k3_.get(), capR, dsc_.get() + 3);
+  const bool side = capR >= kSideMin;
+  if (side) fork_side(0);
+  tree_small_launch(g0_.get(), capR, true, dsc_.get() + 4, dsc_.get() + 5, nullptr, side ? side_ : nullptr);
+  if (pma_launched) pma_fast_launch(k3_.get(), capR, dsc_.get() + 3, true);
+  if (side) join_side(1);
   return pma_launched;
 }
 
@@ -3858,8 +3959,8 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
       // preparation's error words, before the one read
       presize_delete(count, static_cast<uint32_t>(count));
       mutate([&] {
-        delete_launch(k2_.get(), count, static_cast<uint32_t>(count), s32_[0].get(), dsc_.get() + 2, dsc_.get(),
-                      false, false, dsd);
+        delete_launch(k2_.get(), nullptr, nullptr, count, static_cast<uint32_t>(count), s32_[0].get(),
+                      dsc_.get() + 2, dsc_.get(), false, false, dsd);
       });
     }
     read_scalars(dsc_.get(), h, kScalars, stream_);
@@ -3873,6 +3974,7 @@ void Graph::delete_batch(const uint32_t *src, const uint32_t *dst, uint64_t coun
   stats_.last_ins_keys = stats_.last_ins_sources = 0;
   stats_.last_del_keys = h[2];
   stats_.last_del_sources = h[3];
+  stats_.sum_del_sources += h[3];
   if (fused) {
     mutate([&] { delete_finish(h, k2_.get(), s32_[0].get(), dsd); });
     return;
@@ -3945,40 +4047,49 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
   uint64_t *dsd = dsc_.get() + bsw::kDel;
   uint64_t h[kScalars];
   bool pma_launched = false;
+  // deleted edges: fused, the unique keys D (kD_, count dsc_[9]) with their
+  // existence flags (fD_), the prefix of the absent ones (s32_[2]) and their
+  // source runs (s32_[1], count dsc_[10]); otherwise the existing ones
+  // compacted into kE_ (count dsc_[10]), their runs in s32_[1] (dsc_[11])
+  uint32_t *dnidx = nullptr;
+  uint32_t *ileaf = nullptr;  // the inserted keys' PMA leaves from the preparation (bucket path)
   for (int pass = 0; pass < 2; pass++) {
     const bool bucket = prepare_mixed_async(is, id, ni, ds, dd, nd, pass == 1);
+    ileaf = nullptr;
     GView g = view();
     const uint64_t *I = k1_.get(), *D = kD_.get();
     f0_.ensure(count + 1, stream_);
     f1_.ensure(count + 1, stream_);
     fD_.ensure(count + 1, stream_);
-    kE_.ensure(count + 1, stream_);
     s32_[0].ensure(count + 1, stream_);
     s32_[1].ensure(count + 1, stream_);
+    s32_[2].ensure(count + 1, stream_);
+    s32_[3].ensure(count + 1, stream_);
+    dnidx = s32_[2].get();
     // inserted edges: existence, new-key prefix, source runs (run_beg in
-    // s32_[0], count in dsc_[2]); deleted edges: the existing ones into kE_
-    // (count dsc_[10]), their runs in s32_[1] (count dsc_[11])
+    // s32_[0], count in dsc_[2])
     if (small) {
       k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, I, dsc_.get() + 1, count, f0_.get(), f1_.get());
       TG_CHECK_LAUNCH();
       k_small_runs<true><<<1, 1024, 0, stream_>>>(I, f1_.get(), static_cast<uint32_t>(count), nullptr,
                                                   s32_[0].get(), dsc_.get());
       TG_CHECK_LAUNCH();
-      k_exists_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, D, dsc_.get() + 9, count, fD_.get());
+      k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, D, dsc_.get() + 9, count, fD_.get(), dnidx);
       TG_CHECK_LAUNCH();
-      k_small_runs<false><<<1, 1024, 0, stream_>>>(D, fD_.get(), static_cast<uint32_t>(count), kE_.get(),
-                                                   s32_[1].get(), dsc_.get() + 8);
+      k_small_runs<true><<<1, 1024, 0, stream_>>>(D, dnidx, static_cast<uint32_t>(count), nullptr, s32_[1].get(),
+                                                  dsc_.get() + 8);
       TG_CHECK_LAUNCH();
     } else if (count <= kBucketSortMax) {
       const uint32_t nb = static_cast<uint32_t>((count + kPrepBlock - 1) / kPrepBlock);
-      const uint32_t epoch = scan_scratch.lb.begin(3ull * nb, stream_);
+      const uint32_t epoch = scan_scratch.lb.begin(4ull * nb, stream_);
+      ileaf = s32_[3].get();
       k_mixed_prep<<<2 * nb, kPrepBlock, 0, stream_>>>(
-          g, I, dsc_.get() + 1, count, f0_.get(), f1_.get(), s32_[0].get(), dsc_.get() + 2, D, dsc_.get() + 9,
-          kE_.get(), dsc_.get() + 10, scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(),
+          g, I, dsc_.get() + 1, count, f0_.get(), f1_.get(), s32_[0].get(), dsc_.get() + 2, ileaf, D, dsc_.get() + 9,
+          fD_.get(), dnidx, s32_[1].get(), dsc_.get() + 10, scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(),
           scan_scratch.lb.err_word(), nb, epoch);
       TG_CHECK_LAUNCH();
-      select(SelRunStartN{kE_.get(), s32_[1].get(), dsc_.get() + 10}, count, dsc_.get() + 11, scan_scratch, stream_);
     } else {
+      kE_.ensure(count + 1, stream_);
       k_exists2_n<<<grid_for(count, 256), 256, 0, stream_>>>(g, I, dsc_.get() + 1, count, f0_.get(), f1_.get());
       TG_CHECK_LAUNCH();
       const uint32_t nb = static_cast<uint32_t>((count + 8 * kPrepBlock - 1) / (8 * kPrepBlock));
@@ -3998,14 +4109,15 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
     }
     if (fused) {
       // both halves on the device's counts (inserted keys and runs in
-      // dsc_[1], [2]; existing deleted keys and runs in dsc_[10], [11]),
-      // their scratch sized by the batch before the first write
+      // dsc_[1], [2]; deleted keys and runs in dsc_[9], [10]), their scratch
+      // sized by the batch before the first write
       presize_insert(count, static_cast<uint32_t>(count));
       presize_delete(count, static_cast<uint32_t>(count));
       mutate([&] {
-        pma_launched = insert_launch(k1_.get(), count, static_cast<uint32_t>(count), dsc_.get() + 1, dsc_.get());
-        delete_launch(kE_.get(), count, static_cast<uint32_t>(count), s32_[1].get(), dsc_.get() + 10, dsc_.get(),
-                      true, pma_launched, dsd);
+        pma_launched =
+            insert_launch(k1_.get(), count, static_cast<uint32_t>(count), dsc_.get() + 1, dsc_.get(), ileaf);
+        delete_launch(kD_.get(), fD_.get(), dnidx, count, static_cast<uint32_t>(count), s32_[1].get(),
+                      dsc_.get() + 9, dsc_.get(), true, pma_launched, dsd);
       });
     }
     read_scalars(dsc_.get(), h, kScalars, stream_);
@@ -4017,17 +4129,23 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
     throw Error(-4, "mixed batch: the update gate closed without an error word");
   }
   if (!fused) check_prep(h);
-  const uint64_t nI = h[1], nIruns = h[2], nE = h[10], nEruns = h[11];
+  const uint64_t nI = h[1], nIruns = h[2];
   stats_.last_ins_keys = nI;
   stats_.last_ins_sources = nIruns;
-  stats_.last_del_keys = nE;
-  stats_.last_del_sources = nEruns;
+  stats_.sum_ins_sources += nIruns;
   if (!fused) {
+    const uint64_t nE = h[10], nEruns = h[11];
+    stats_.last_del_keys = nE;
+    stats_.last_del_sources = nEruns;
+    stats_.sum_del_sources += nEruns;
     if (nI) insert_runs(k1_.get(), nI, static_cast<uint32_t>(nIruns), dsc_.get() + 1);
     // the insert's general passes may have used dsc_[10], [11]
     if (nE) delete_runs(kE_.get(), nE, static_cast<uint32_t>(nEruns), put_counts(nE, nEruns), s32_[1].get());
     return;
   }
+  const uint64_t nD = h[9], nDruns = h[10];
+  stats_.last_del_sources = nDruns;  // sources of the deleted keys
+  stats_.sum_del_sources += nDruns;
   mutate([&] {
     const bool ins_done = insert_finish(h, k1_.get(), pma_launched);
     const bool del_ran = h[bsw::kDel + bsw::kGate + 2] == 1;
@@ -4036,14 +4154,19 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
       throw Error(-4, "mixed batch: the delete half's gate disagrees with the insert half's outcome");
     }
     if (del_ran) {
-      delete_finish(h, kE_.get(), s32_[1].get(), dsd);
-    } else if (nE) {
-      // the insert's general passes are done; the delete half now
-      delete_launch(kE_.get(), nE, static_cast<uint32_t>(nEruns), s32_[1].get(), put_counts(nE, nEruns), nullptr,
-                    false, false, dsd);
+      stats_.last_del_keys = h[bsw::kDel + 3];
+      delete_finish(h, kD_.get(), s32_[1].get(), dsd);
+    } else if (nD) {
+      // the insert's general passes are done (they may have used dsc_[9],
+      // [10]); the delete half now
+      delete_launch(kD_.get(), fD_.get(), dnidx, nD, static_cast<uint32_t>(nDruns), s32_[1].get(),
+                    put_counts(nD, nDruns), nullptr, false, false, dsd);
       uint64_t hs[kScalars];
       read_scalars(dsc_.get(), hs, kScalars, stream_);
-      delete_finish(hs, kE_.get(), s32_[1].get(), dsd);
+      stats_.last_del_keys = hs[bsw::kDel + 3];
+      delete_finish(hs, kD_.get(), s32_[1].get(), dsd);
+    } else {
+      stats_.last_del_keys = 0;
     }
   });
 }
@@ -4055,7 +4178,6 @@ void Graph::presize_delete(uint64_t nR, uint32_t nruns) {
   s32_[13].ensure(nruns + 1, stream_);
   k3_.ensure(nR + 1, stream_);
   g0_.ensure(nR + 1, stream_);
-  g2_.ensure(nR + 1, stream_);
   s64_[3].ensure(nR + 1, stream_);
   s32_[30].ensure(1 + 3 * std::min<uint64_t>(nR, kTreeSmallKeys), stream_);
 }
@@ -4065,7 +4187,7 @@ void Graph::delete_runs(const uint64_t *R, uint64_t nR, uint32_t nruns, const ui
   const uint32_t *rb = run_beg ? run_beg : s32_[0].get();
   uint64_t *dsd = dsc_.get() + bsw::kDel;
   mutate([&] {
-    delete_launch(R, nR, nruns, rb, cw, nullptr, false, false, dsd);
+    delete_launch(R, nullptr, nullptr, nR, nruns, rb, cw, nullptr, false, false, dsd);
     uint64_t hs[kScalars];
     read_scalars(dsc_.get(), hs, kScalars, stream_);
     delete_finish(hs, R, rb, dsd);
@@ -4093,8 +4215,9 @@ void Graph::delete_refill(const uint64_t *R, uint32_t capRuns, const uint32_t *r
 // the in-place tree delete (status ds[13]), the inline refill, its tier
 // split and the refill's in-place tree delete (status ds[19]), which waits
 // when the first tree delete did not fit.
-void Graph::delete_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint32_t *run_beg,
-                          const uint64_t *cw, const uint64_t *gate, bool spec, bool pma_launched, uint64_t *ds) {
+void Graph::delete_launch(const uint64_t *R, const uint32_t *exf, const uint32_t *dnidx, uint64_t capR,
+                          uint32_t capRuns, const uint32_t *run_beg, const uint64_t *cw, const uint64_t *gate, bool spec,
+                          bool pma_launched, uint64_t *ds) {
   lists_dirty_ = true;
   ++epoch_;
   GView g = view();
@@ -4107,7 +4230,7 @@ void Graph::delete_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, co
   // (PMA keys into k3_, count ds[4]; tree keys into g0_, ds[5])
   const uint32_t nb = static_cast<uint32_t>(std::max<uint64_t>(1, (capRuns + kRunBlock - 1) / kRunBlock));
   const uint32_t epoch = scan_scratch.lb.begin(4ull * nb, stream_);
-  k_delete_runs<<<nb, kRunBlock, 0, stream_>>>(g, R, run_beg, cw, gate, spec, pma_launched, ds + bsw::kGate, capR,
+  k_delete_runs<<<nb, kRunBlock, 0, stream_>>>(g, R, exf, dnidx, run_beg, cw, gate, spec, pma_launched, ds + bsw::kGate, capR,
                                                capRuns, k3_.get(), g0_.get(), s32_[12].get(), s32_[13].get(), ds,
                                                dsc_.get() + kLowFlag, scan_scratch.lb.state.get(),
                                                scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb, epoch);
@@ -4116,8 +4239,13 @@ void Graph::delete_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, co
     debug_fail &= ~2u;
     throw Error(-2, "injected allocation failure after the inline deletes");
   }
+  // the tree delete beside the PMA delete for a large batch (the refill
+  // reads both tiers after the join)
+  const bool side = capR >= kSideMin;
+  if (side) fork_side(2);
+  tree_small_launch(g0_.get(), capR, false, ds + 5, nullptr, ds + 13, side ? side_ : nullptr);
   pma_delete_launch(k3_.get(), capR, ds + 4);
-  tree_small_launch(g0_.get(), capR, false, ds + 5, nullptr, ds + 13);
+  if (side) join_side(3);
   delete_refill(R, capRuns, run_beg, ds, false);
   tree_small_launch(s64_[3].get(), capR, false, ds + 17, ds + 13, ds + 19);
 }

@@ -86,6 +86,8 @@ struct Stats {
   // the last batch: unique inserted keys and their sources, existing
   // deleted keys and their sources
   uint64_t last_ins_keys = 0, last_ins_sources = 0, last_del_keys = 0, last_del_sources = 0;
+  // over every update batch since the graph was created
+  uint64_t sum_ins_sources = 0, sum_del_sources = 0;
 };
 
 class Graph {
@@ -220,6 +222,15 @@ class Graph {
  private:
   uint32_t n_;
   hipStream_t stream_ = nullptr;
+  // A second stream for the B-tree part of a large batch, which touches
+  // no PMA data: it runs beside the PMA kernels, forked from and joined
+  // back to stream_ by events (every fork is joined before the batch's
+  // read, so synchronising stream_ covers it).
+  hipStream_t side_ = nullptr;
+  hipEvent_t ev_[4] = {};
+  static constexpr uint64_t kSideMin = 1u << 14;  // batches from this many keys use it
+  void fork_side(int e);  // side_ waits for stream_'s work so far
+  void join_side(int e);  // stream_ waits for side_'s work so far
   DevBuf<VBlock> vb_;
   // PMA
   uint64_t pma_N_ = 0;
@@ -309,9 +320,11 @@ class Graph {
                    const uint32_t *run_beg = nullptr);
   // The insert half's kernels, sized by capacities (capR keys, capRuns
   // runs), on the counts in cw; gate (the batch scalars, or null) closes
-  // the half when the preparation raised an error.  Returns whether the
-  // in-leaf PMA merge was launched.
-  bool insert_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint64_t *cw, const uint64_t *gate);
+  // the half when the preparation raised an error; ileaf (optional): the
+  // PMA leaf each key's existence lookup searched (~0: none).  Returns
+  // whether the in-leaf PMA merge was launched.
+  bool insert_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint64_t *cw, const uint64_t *gate,
+                     const uint32_t *ileaf = nullptr);
   // After the read: the insert's outcome, and its general passes when the
   // in-place paths did not take every key.  True when the in-place paths
   // took them all (flags not yet checked).
@@ -319,8 +332,11 @@ class Graph {
   // The delete half's kernels, its words at ds.  spec: launched behind the
   // insert half of the same batch, it runs only when that half completed
   // on the device (pma_launched: the insert's in-leaf merge was launched).
-  void delete_launch(const uint64_t *R, uint64_t capR, uint32_t capRuns, const uint32_t *run_beg,
-                     const uint64_t *cw, const uint64_t *gate, bool spec, bool pma_launched, uint64_t *ds);
+  // R: the deleted keys, with exf their existence flags and dnidx the prefix
+  // of the absent ones (both null: every key of R exists).
+  void delete_launch(const uint64_t *R, const uint32_t *exf, const uint32_t *dnidx, uint64_t capR,
+                     uint32_t capRuns, const uint32_t *run_beg, const uint64_t *cw, const uint64_t *gate, bool spec,
+                     bool pma_launched, uint64_t *ds);
   void delete_finish(const uint64_t *hs, const uint64_t *R, const uint32_t *run_beg, uint64_t *ds);
   // k_bump: the inline refill (tree keys it moves into s64_[3], count ds[17])
   void delete_refill(const uint64_t *R, uint32_t capRuns, const uint32_t *run_beg, uint64_t *ds, bool trees_only);
@@ -342,8 +358,11 @@ class Graph {
   // PMA inserts: pma_insert = in-leaf merge when every touched leaf fits,
   // else the window pass (or a rebuild beyond the root bound)
   void pma_insert(const uint64_t *keys, uint64_t k);
-  // kdev: the key count on the device, at most cap (null: cap keys)
-  void pma_fast_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev = nullptr);
+  // kdev: the key count on the device, at most cap (null: cap keys);
+  // leaves_known: u0_ holds each key's leaf or ~0 where it must be searched
+  // (k_insert_runs wrote it)
+  void pma_fast_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev = nullptr,
+                       bool leaves_known = false);
   void pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev);
   void pma_fast_done(uint64_t k, uint64_t nt);
   void pma_window_insert(const uint64_t *keys, uint64_t k);
@@ -359,7 +378,7 @@ class Graph {
   bool tree_small(const uint64_t *keys, uint64_t k, bool insert);
   // status (null: dsc_[13]): 0 done, 1 did not fit, 2 gate set
   void tree_small_launch(const uint64_t *keys, uint64_t cap, bool insert, const uint64_t *kdev = nullptr,
-                         const uint64_t *gate = nullptr, uint64_t *status = nullptr);
+                         const uint64_t *gate = nullptr, uint64_t *status = nullptr, hipStream_t s = nullptr);
   void tree_insert_cow(const uint64_t *keys, uint64_t k);
   void tree_delete_cow(const uint64_t *keys, uint64_t k);
   void ensure_blocks(uint64_t need);

@@ -199,12 +199,12 @@ class Graph:
         return out
 
     def stats(self) -> dict:
-        out = np.zeros(16, np.uint64)
+        out = np.zeros(18, np.uint64)
         check(lib().tg_stats(self._h, ptr(out)))
         keys = ["pma_rebuilds", "pma_windows", "pma_window_leaves", "tree_blocks_written",
                 "promotions", "trees", "pma_slots", "tree_small_updates", "pma_leaf_merges",
                 "pma_elements", "last_ins_keys", "last_ins_sources", "last_del_keys", "last_del_sources",
-                "pma_low_windows", "pma_shrinks"]
+                "pma_low_windows", "pma_shrinks", "sum_ins_sources", "sum_del_sources"]
         return {k: int(v) for k, v in zip(keys, out)}
 
     def pma_leaf_counts(self) -> np.ndarray:
