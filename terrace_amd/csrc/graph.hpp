@@ -154,6 +154,9 @@ class Graph {
   uint32_t num_trees() const { return ntrees_; }
   uint64_t pma_slots() const { return pma_N_; }
   uint64_t pma_elements() const { return pma_elems_; }
+  // an upper bound on the edges, known on the host: the inline slots, the
+  // PMA's elements and the capacity of the tree blocks in use
+  uint64_t edge_bound() const { return 13ull * n_ + pma_elems_ + 256ull * blk_top_; }
   // leaf counts into host memory (at most cap); returns the number of leaves
   uint64_t pma_leaf_counts(uint32_t *host, uint64_t cap);
   const Stats &stats() const { return stats_; }

@@ -216,7 +216,10 @@ uint64_t Server::footprint(const Request &r) const {
     }
     case kReqBfs: return 4 * n + n / 2;   // one lane of a multi-source run
     case kReqCc: return 12 * n;
-    case kReqPr: return 24 * n;
+    // ranks and scratch (24 B per vertex), and the sliced adjacency copy
+    // PageRank builds per graph version: 4 B per edge, 8 row-offset words
+    // and 8 partial sums per vertex, and the degrees
+    case kReqPr: return 24 * n + 4 * g_.edge_bound() + 100 * n;
     case kReqLtc: return 1 << 16;
     default: return 0;
   }

@@ -1081,7 +1081,7 @@ __global__ void __launch_bounds__(256) k_tc(const uint64_t *off, const uint32_t 
 }
 
 // ---------------------------------------------------------------- PageRank, XCD-sliced pull
-// A graph with n <= kPrSlicedMaxN runs PageRank over a slice-major copy of
+// A graph with at most 2^32 edges runs PageRank over a slice-major copy of
 // its adjacency.  The source ids are cut into eight ranges holding equal
 // edge counts (cut points from the CSR offsets, exact for a symmetric
 // graph); slice k lists, for every vertex v in order, v's neighbours that
@@ -1100,7 +1100,6 @@ __global__ void __launch_bounds__(256) k_tc(const uint64_t *off, const uint32_t 
 // adds the eight slice partials in slice order: the result is the same on
 // every run.
 constexpr uint32_t kPrSlices = kPrsSlices;
-constexpr uint32_t kPrSlicedMaxN = 6u << 20;
 constexpr uint32_t kPrsItems = 8;  // merge-path steps per thread
 constexpr uint32_t kPrsBlock = 256;
 constexpr uint32_t kPrsTile = kPrsItems * kPrsBlock;
@@ -2437,7 +2436,7 @@ uint32_t Traversal::bfs(uint32_t src, int32_t *parent, uint32_t *level, uint32_t
 // ---------------------------------------------------------------- PR
 void Traversal::pagerank(uint32_t iters, double *out) {
   uint32_t n = g_.num_vertices();
-  if (n <= kPrSlicedMaxN && g_.tune.pr_sliced) {
+  if (g_.tune.pr_sliced) {
     // the sliced copy indexes edges with 32 bits; denser graphs take the tiered pull
     const uint64_t m = prs_epoch_ == g_.epoch() ? prs_m_ : g_.num_edges();
     if (m <= 0xFFFFFFFFull) {
