@@ -58,9 +58,11 @@ def base_graph(n, m0, seed):
 
 @pytest.mark.parametrize("size", [1000, 50_000, 300_000])
 def test_prep_failure_leaves_graph_unchanged(size):
-    """Injected look-back error in the preparation of an insert and of a
-    delete batch (small, bucket and radix sizes): TG_EINTERNAL, no change,
-    and the same batches then apply normally."""
+    """Injected look-back error in the preparation of an insert, a delete
+    and a mixed batch (small, bucket and radix sizes; up to 2^18 edges the
+    batch's update kernels are already launched, behind the device-side
+    gate): TG_EINTERNAL, no change, and the same batches then apply
+    normally."""
     n = 1 << 14
     say("prep failure", size)
     G, keys = base_graph(n, 150_000, 11)
@@ -69,10 +71,13 @@ def test_prep_failure_leaves_graph_unchanged(size):
     ins = oracle.sort_unique((rng.integers(0, n, size).astype(np.uint64) << np.uint64(32))
                              | rng.integers(0, n, size).astype(np.uint64))
     dels = oracle.sort_unique(keys[rng.choice(len(keys), min(size, len(keys) // 2), replace=False)])
-    for kind, batch in (("insert", ins), ("delete", dels)):
+    calls = (("insert", lambda: G.add_edge_batch(*split(ins))),
+             ("delete", lambda: G.remove_edge_batch(*split(dels))),
+             ("mixed", lambda: G.update_batch(*split(ins[: size // 2]), *split(dels[: size // 2]))))
+    for kind, call in calls:
         tg.lib().tg_debug_fail_next(G.handle, 1)
         with pytest.raises(TerraceError) as e:
-            (G.add_edge_batch if kind == "insert" else G.remove_edge_batch)(*split(batch))
+            call()
         assert e.value.code == TG_EINTERNAL, kind
         after = adjacency(G)
         assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1]), kind
@@ -84,7 +89,7 @@ def test_prep_failure_leaves_graph_unchanged(size):
     G.close()
 
 
-@pytest.mark.parametrize("kind", ["insert", "delete"])
+@pytest.mark.parametrize("kind", ["insert", "delete", "mixed"])
 def test_failure_after_first_write_marks_graph_invalid(kind):
     n = 1 << 12
     say("mid-batch failure", kind)
@@ -93,7 +98,10 @@ def test_failure_after_first_write_marks_graph_invalid(kind):
         *oracle.rmat_edges(n, 77, 0.5, 0.1, 0.1, 5000))
     tg.lib().tg_debug_fail_next(G.handle, 2)
     with pytest.raises(TerraceError) as e:
-        (G.add_edge_batch if kind == "insert" else G.remove_edge_batch)(*split(batch))
+        if kind == "mixed":
+            G.update_batch(*split(batch), *split(keys[:500]))
+        else:
+            (G.add_edge_batch if kind == "insert" else G.remove_edge_batch)(*split(batch))
     assert e.value.code == TG_EINTERNAL
     for call in (lambda: G.add_edge_batch(*split(batch)),
                  lambda: G.export_adjacency(),

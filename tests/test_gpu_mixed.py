@@ -101,3 +101,38 @@ def test_update_batch_bucket_overflow_and_errors():
     assert e.value.code == -1
     assert_set_model(G, keys, n, "after the rejected batch")
     G.close()
+
+
+@pytest.mark.parametrize("n_ins,n_del", [(1500, 500), (60_000, 20_000)])
+def test_insert_fallbacks_rerun_the_delete_half(n_ins, n_del):
+    """Mixed batches whose insert half needs the host's general passes: new
+    B-trees (promotions of vertices pushed past 1024 neighbours) and PMA
+    leaves that overflow their bound (the window pass).  The delete half,
+    launched behind the insert half, must then have written nothing and run
+    again after those passes; the adjacency equals the set model after
+    every batch, and the deletes' own counts are reported."""
+    n = 1 << 13
+    say("insert fallbacks", n_ins, n_del)
+    rng = np.random.default_rng(n_ins + 5)
+    keys = oracle.build_keys(*oracle.rmat_edges(n, 29, 0.5, 0.1, 0.1, 20_000))
+    G = tg.Graph(n)
+    G.add_edge_batch(*split(keys))
+    st0 = G.stats()
+    for rnd in range(3):
+        # a few sources get 1100 new neighbours each (promotions), the rest
+        # random edges crowding the PMA's leaves; deletes of present edges
+        hub = rng.choice(np.nonzero(G.degrees() < 200)[0], 3, replace=False).astype(np.uint64)
+        heavy = np.concatenate([(h << np.uint64(32)) | rng.permutation(n)[:1100].astype(np.uint64) for h in hub])
+        light = (rng.integers(0, n, n_ins).astype(np.uint64) << np.uint64(32)) | rng.integers(0, n, n_ins).astype(
+            np.uint64)
+        ins = np.concatenate([heavy, light])
+        dels = keys[rng.choice(len(keys), min(n_del, len(keys)), replace=False)]
+        G.update_batch(*split(ins), *split(dels))
+        keys = oracle.set_difference(oracle.set_union(keys, oracle.sort_unique(ins)), oracle.sort_unique(dels))
+        assert_set_model(G, keys, n, f"round {rnd}")
+        st = G.stats()
+        assert st["last_del_keys"] == len(oracle.sort_unique(dels)), f"round {rnd}"
+    st = G.stats()
+    assert st["promotions"] > st0["promotions"]
+    assert st["pma_windows"] + st["pma_rebuilds"] > st0["pma_windows"] + st0["pma_rebuilds"]
+    G.close()
