@@ -271,7 +271,9 @@ def serve_stream(tg, build, reqs, arrivals, budget, verify=True):
 
     def replay(mode, stream):
         G = build()
-        S = tg.Server(G, mode=mode, mem_budget=budget << 30)
+        t_c = time.perf_counter()
+        S = tg.Server(G, mode=mode, mem_budget=budget << 30)  # reserves the lanes' working memory
+        startup = time.perf_counter() - t_c
         ids = [submit(S, r, a) for r, a in stream]
         t0 = time.perf_counter()
         S.run()
@@ -279,6 +281,7 @@ def serve_stream(tg, build, reqs, arrivals, budget, verify=True):
         res = [S.result(i) for i in ids]
         summ = S.summary()
         summ["wall_s"] = wall
+        summ["startup_s"] = startup
         S.close()
         G.close()
         return summ, res
@@ -302,7 +305,7 @@ def serve_stream(tg, build, reqs, arrivals, budget, verify=True):
                 mism += 1
     ahead = sum(1 for i in range(R) if reqs[i][0] != "U" and rb[i]["version"] != rp[i]["version"])
     keep = ("requests_per_s", "p50_s", "p99_s", "p50_query_s", "p99_query_s", "makespan_s", "groups", "rejected",
-            "wall_s")
+            "wall_s", "startup_s")
     return {"plain": {k: sp[k] for k in keep if k in sp}, "batched": {k: sb[k] for k in keep if k in sb},
             "throughput_x": sb["requests_per_s"] / max(sp["requests_per_s"], 1e-9),
             "p99_ratio": sb["p99_s"] / max(sp["p99_s"], 1e-12),

@@ -4582,7 +4582,10 @@ void Graph::copy_from(Graph &src, bool query_only) {
   TG_HIP(hipStreamSynchronize(stream_));
   cp(vb_.get(), src.vb_.get(), sizeof(VBlock) * static_cast<uint64_t>(n_));
   if (pma_N_ != src.pma_N_) {
-    pdst_.release();
+    // a query-only copy keeps a slot array that is already large enough
+    // (repeated snapshots of a growing graph then do not reallocate); an
+    // updatable copy starts its update-side state afresh
+    if (!query_only || pdst_.cap < src.pma_N_) pdst_.release();
     psrc_.release();
     pcnt_.release();
     fence_.release();

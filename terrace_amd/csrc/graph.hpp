@@ -194,6 +194,13 @@ class Graph {
 
   // Mutation counter: changes whenever the adjacency may have changed.
   uint64_t epoch() const { return epoch_; }
+  // Drops what was derived from the current version (the tier lists, and
+  // every traversal cache keyed by the epoch) without changing the
+  // adjacency: the next query rebuilds them as after an update.
+  void touch() {
+    lists_dirty_ = true;
+    ++epoch_;
+  }
 
   // Launch parameters chosen by the autotuner; the whole-algorithm calls
   // use them.  thr_*: edgeMap push/pull switch (dense when n / thr <=

@@ -50,6 +50,87 @@ Server::Server(Graph &g, int mode, uint64_t mem_budget, bool keep_results)
       slots_.push_back(std::move(sl));
     }
   }
+  reserve();
+}
+
+// Lanes allocate their working memory (frontier queues and bitmaps, the
+// sliced adjacency copy of PageRank, level and result arrays, the
+// snapshot's own arrays) on first use and grow it by a quarter when a
+// later call needs more.  On this device hipMalloc of a few GiB can take
+// hundreds of milliseconds while earlier frees are reclaimed, and hipFree
+// waits for the whole device, so a lane allocating mid-stream stalls every
+// other lane.  Each lane therefore runs one query of its kind here, on the
+// graph as it is: BFS from up to 64 sources among the vertices with the
+// longest lists, CC, one PageRank iteration and one local triangle count.
+// What those queries leave cached (tier lists, tree tasks, PageRank
+// slices) is dropped afterwards, so the first requests still pay for
+// their own work.
+void Server::reserve() {
+  const uint32_t n = g_.num_vertices();
+  if (n == 0) return;
+  auto probes = [n](Graph &g, hipStream_t s, std::vector<uint32_t> &src, uint32_t &ltc_v) {
+    const Graph::TierLists &L = g.tier_lists();
+    src.clear();
+    uint32_t nt = std::min<uint32_t>(L.ntree, 64);
+    uint32_t np = std::min<uint32_t>(L.npma, 64 - nt);
+    src.resize(nt + np);
+    if (nt) TG_HIP(hipMemcpyAsync(src.data(), L.tree, nt * 4ull, hipMemcpyDeviceToHost, s));
+    if (np) TG_HIP(hipMemcpyAsync(src.data() + nt, L.pma, np * 4ull, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    if (src.empty()) src.push_back(0);
+    // the triangle count probes a PMA vertex (moderate degree): a hub's
+    // count can take far longer than the memory it reserves is worth
+    ltc_v = np ? src[nt] : src[0];
+  };
+  std::vector<uint32_t> src;
+  uint32_t lv = 0;
+  uint64_t c = 0;
+  if (!batched_) {
+    Traversal &T = *trav_[0];
+    hipStream_t s = T.stream();
+    probes(g_, s, src, lv);
+    par_.ensure(n, s);
+    res32_[0].ensure(n, s);
+    T.bfs(src[0], par_.get(), res32_[0].get());
+    (void)T.digest(res32_[0].get(), nullptr, n);
+    res32_[1].ensure(n, s);
+    T.cc(res32_[1].get());
+    res64_.ensure(n, s);
+    T.pagerank(1, res64_.get());
+    T.ltc(&lv, 1, &c);
+    TG_HIP(hipStreamSynchronize(s));
+    g_.touch();
+    return;
+  }
+  for (auto &p : slots_) {
+    QuerySlot &sl = *p;
+    sl.snap->copy_from(g_, true);
+    probes(*sl.snap, sl.streams[0], src, lv);
+    const uint32_t k = static_cast<uint32_t>(src.size());
+    {
+      Traversal &T = *sl.trav[0];
+      // a group runs its BFS requests 64 sources per call
+      sl.res32[0].ensure(64ull * n, T.stream());
+      T.bfs_multi(src.data(), k, sl.res32[0].get());
+      (void)T.digest(sl.res32[0].get(), nullptr, n);
+    }
+    {
+      Traversal &T = *sl.trav[1];
+      sl.res32[1].ensure(n, T.stream());
+      T.cc(sl.res32[1].get());
+      (void)T.digest(sl.res32[1].get(), nullptr, n);
+    }
+    {
+      Traversal &T = *sl.trav[2];
+      sl.res64.ensure(n, T.stream());
+      T.pagerank(1, sl.res64.get());
+      (void)T.digest(nullptr, reinterpret_cast<const uint64_t *>(sl.res64.get()), n);
+    }
+    sl.trav[3]->ltc(&lv, 1, &c);
+    for (int i = 0; i < kLanes; i++) TG_HIP(hipStreamSynchronize(sl.streams[i]));
+    sl.snap->touch();
+  }
+  g_.touch();
 }
 
 Server::~Server() {

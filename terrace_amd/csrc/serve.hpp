@@ -164,6 +164,9 @@ class Server {
   QuerySlot *free_slot();
   void launch_group(QuerySlot &sl, std::vector<uint64_t> ids);
   void reject(Request &r);
+  // Start-up: every lane runs one query of its kind, so each allocates
+  // its working memory before the first request (see serve.cpp)
+  void reserve();
   void run_strict();
   void run_admission();
   void drain();

@@ -2792,10 +2792,9 @@ uint32_t Traversal::bfs_multi(const uint32_t *srcs_host, uint32_t k, uint32_t *l
   fill_u64(ms_fr_.get(), 0, n, s);
   fill_u64(ms_nx_.get(), 0, n, s);
   fill_u32(levels, 0xFFFFFFFFu, static_cast<uint64_t>(k) * n, s);
-  DevBuf<uint32_t> dsrc;
-  dsrc.ensure(k, s);
-  TG_HIP(hipMemcpyAsync(dsrc.get(), hs.data(), k * 4, hipMemcpyHostToDevice, s));
-  k_ms_init<<<1, 64, 0, s>>>(dsrc.get(), k, n, ms_vis_.get(), ms_fr_.get(), levels);
+  qv_.ensure(k, s);
+  TG_HIP(hipMemcpyAsync(qv_.get(), hs.data(), k * 4, hipMemcpyHostToDevice, s));
+  k_ms_init<<<1, 64, 0, s>>>(qv_.get(), k, n, ms_vis_.get(), ms_fr_.get(), levels);
   TG_CHECK_LAUNCH();
   std::vector<uint32_t> uniq(hs);
   std::sort(uniq.begin(), uniq.end());
@@ -2855,19 +2854,17 @@ void Traversal::bc_multi(const uint32_t *srcs_host, uint32_t k, double *out, uin
   fill_u64(ms_fr_.get(), 0, n, s);
   fill_u64(ms_nx_.get(), 0, n, s);
   fill_f64(bc_sigma_.get(), 0.0, nl, s);
-  DevBuf<uint32_t> dsrc;
-  dsrc.ensure(k, s);
-  TG_HIP(hipMemcpyAsync(dsrc.get(), hs.data(), k * 4, hipMemcpyHostToDevice, s));
-  k_msbc_init<<<1, 64, 0, s>>>(dsrc.get(), k, ms_vis_.get(), ms_fr_.get(), bc_sigma_.get());
+  qv_.ensure(k, s);
+  TG_HIP(hipMemcpyAsync(qv_.get(), hs.data(), k * 4, hipMemcpyHostToDevice, s));
+  k_msbc_init<<<1, 64, 0, s>>>(qv_.get(), k, ms_vis_.get(), ms_fr_.get(), bc_sigma_.get());
   TG_CHECK_LAUNCH();
   // level 0: the sources with their lane masks
   std::vector<uint64_t> loff{0};
   bc_list_.ensure(n, s);
   bc_mask_.ensure(n, s);
-  DevBuf<uint64_t> cntd;
-  cntd.ensure(1, s);
-  select(SelMask{ms_fr_.get(), bc_list_.get(), bc_mask_.get()}, n, cntd.get(), scan_, s);
-  loff.push_back(read_scalar(cntd.get(), s));
+  qcnt_.ensure(1, s);
+  select(SelMask{ms_fr_.get(), bc_list_.get(), bc_mask_.get()}, n, qcnt_.get(), scan_, s);
+  loff.push_back(read_scalar(qcnt_.get(), s));
   Frontier a, b;
   a.n = n;
   a.queue.ensure(loff[1] + 1, s);
@@ -2900,7 +2897,7 @@ void Traversal::bc_multi(const uint32_t *srcs_host, uint32_t k, double *out, uin
     uint64_t base = loff.back();
     bc_list_.ensure(base + nxt->count + 1, s, true, base);
     bc_mask_.ensure(base + nxt->count + 1, s, true, base);
-    select(SelMask{ms_fr_.get(), bc_list_.get() + base, bc_mask_.get() + base}, n, cntd.get(), scan_, s);
+    select(SelMask{ms_fr_.get(), bc_list_.get() + base, bc_mask_.get() + base}, n, qcnt_.get(), scan_, s);
     loff.push_back(base + nxt->count);
     k_msbc_level<<<grid_for(nxt->count * 64, 256, 1u << 18), 256, 0, s>>>(
         g_.view(), bc_list_.get() + base, bc_mask_.get() + base, nxt->count, bc_pm_.get(), bc_sigma_.get(),
@@ -3056,29 +3053,29 @@ void Traversal::ltc(const uint32_t *verts_host, uint32_t q, uint64_t *out_host) 
   if (q == 0) return;
   for (uint32_t i = 0; i < q; i++)
     if (verts_host[i] >= n) throw Error(-1, "vertex out of range");
-  DevBuf<uint32_t> dv;
-  DevBuf<uint64_t> off;
-  DevBuf<unsigned long long> cnt;
-  dv.ensure(q, s);
-  off.ensure(q + 1, s);
-  cnt.ensure(q, s);
-  TG_HIP(hipMemcpyAsync(dv.get(), verts_host, q * 4, hipMemcpyHostToDevice, s));
-  k_deg_of<<<grid_for(q, 256), 256, 0, s>>>(g_.view(), dv.get(), q, off.get());
+  qv_.ensure(q, s);
+  qoff_.ensure(q + 1, s);
+  qcnt_.ensure(q, s);
+  uint32_t *dv = qv_.get();
+  uint64_t *off = qoff_.get();
+  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(qcnt_.get());
+  TG_HIP(hipMemcpyAsync(dv, verts_host, q * 4, hipMemcpyHostToDevice, s));
+  k_deg_of<<<grid_for(q, 256), 256, 0, s>>>(g_.view(), dv, q, off);
   TG_CHECK_LAUNCH();
-  exclusive_scan_u64(off.get(), off.get(), q, off.get() + q, scan_, s);
-  uint64_t total = read_scalar(off.get() + q, s);
+  exclusive_scan_u64(off, off, q, off + q, scan_, s);
+  uint64_t total = read_scalar(off + q, s);
   utmp_.ensure(total + 1, s);
-  fill_u64(reinterpret_cast<uint64_t *>(cnt.get()), 0, q, s);
+  fill_u64(qcnt_.get(), 0, q, s);
   if (total) {
-    k_gather_lists<<<grid_for(static_cast<uint64_t>(q) * 64, 256), 256, 0, s>>>(g_.view(), dv.get(), q, off.get(),
+    k_gather_lists<<<grid_for(static_cast<uint64_t>(q) * 64, 256), 256, 0, s>>>(g_.view(), dv, q, off,
                                                                                 utmp_.get());
     TG_CHECK_LAUNCH();
-    k_ltc_count<<<grid_for(total * 64, 256), 256, 0, s>>>(g_.view(), dv.get(), q, off.get(), utmp_.get(), total,
-                                                          cnt.get());
+    k_ltc_count<<<grid_for(total * 64, 256), 256, 0, s>>>(g_.view(), dv, q, off, utmp_.get(), total,
+                                                          cnt);
     TG_CHECK_LAUNCH();
   }
   std::vector<unsigned long long> h(q);
-  TG_HIP(hipMemcpyAsync(h.data(), cnt.get(), q * 8, hipMemcpyDeviceToHost, s));
+  TG_HIP(hipMemcpyAsync(h.data(), cnt, q * 8, hipMemcpyDeviceToHost, s));
   TG_HIP(hipStreamSynchronize(s));
   for (uint32_t i = 0; i < q; i++) out_host[i] = h[i] / 2;
 }

@@ -133,6 +133,12 @@ class Traversal {
   DevBuf<uint64_t> csr_off_;
   DevBuf<uint32_t> csr_nbr_;
   DevBuf<uint64_t> red_;
+  // per-call inputs and counters (sources, query vertices, their offsets
+  // and counts), kept so that a call allocates nothing once warm: hipFree
+  // waits for the whole device, so a call freeing a temporary would stall
+  // work on every other stream
+  DevBuf<uint32_t> qv_;
+  DevBuf<uint64_t> qoff_, qcnt_;
   DevBuf<uint64_t> ms_vis_, ms_fr_, ms_nx_;
   DevBuf<uint64_t> bc_pm_, bc_mask_;  // multi-source BC: dense masks, per-level masks
   DevBuf<uint32_t> bc_list_;          // per-level vertex lists
