@@ -218,7 +218,10 @@ __device__ inline bool tree_has(const GView &g, uint32_t t, uint32_t d) {
 // The 64-byte block is loaded whole (four 16-byte loads issued together).
 // leaf_out (optional): the PMA leaf searched for a key of a PMA-tier vertex
 // (the leaf that holds or would hold it), else left as it is.
-__device__ inline bool edge_exists(const GView &g, uint32_t s, uint32_t d, uint32_t *leaf_out = nullptr) {
+// slot_out (optional): the PMA slot that holds the key when the lookup
+// found it there (left unchanged otherwise).
+__device__ inline bool edge_exists(const GView &g, uint32_t s, uint32_t d, uint32_t *leaf_out = nullptr,
+                                   uint64_t *slot_out = nullptr) {
   const uint4 *vp = reinterpret_cast<const uint4 *>(g.vb + s);
   const uint4 q0 = vp[0], q1 = vp[1], q2 = vp[2], q3 = vp[3];
   // degree, nbr[0..12], aux (low, high)
@@ -234,7 +237,9 @@ __device__ inline bool edge_exists(const GView &g, uint32_t s, uint32_t d, uint3
   if (D <= kInline || d < w[kInline]) return false;
   const uint64_t aux = static_cast<uint64_t>(w[14]) | (static_cast<uint64_t>(w[15]) << 32);
   if (aux & kAuxTree) return tree_has(g, static_cast<uint32_t>(aux), d);
-  return pma_slot_of(g, s, d, aux, leaf_out) != ~0ull;
+  const uint64_t sl = pma_slot_of(g, s, d, aux, leaf_out);
+  if (slot_out) *slot_out = sl;
+  return sl != ~0ull;
 }
 
 // Merge path: number of A elements among the first m of merge(A, B)
@@ -407,13 +412,14 @@ __global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, c
 // exclusive prefix of the new (I) or absent (D) keys over [0, count] and the
 // source runs (I: runI, count sc[2]; D: runD, count sc[10]; each with its
 // terminator); exI, exD the existence flags, leafI the PMA leaf each
-// inserted key's lookup searched (~0: none).  Clears the batch scalars.
+// inserted key's lookup searched (~0: none), slotD the PMA slot of each
+// deleted key (~0: not in the PMA).  Clears the batch scalars.
 __global__ void __launch_bounds__(1024) k_small_mixed_prep(GView g, const uint32_t *is, const uint32_t *id,
                                                            uint32_t ni, const uint32_t *ds, const uint32_t *dd,
                                                            uint32_t nd, uint64_t *I, uint64_t *D, uint64_t *sc,
                                                            uint32_t *exI, uint32_t *nidxI, uint32_t *runI,
                                                            uint32_t *leafI, uint32_t *exD, uint32_t *nidxD,
-                                                           uint32_t *runD) {
+                                                           uint32_t *runD, uint64_t *slotD) {
   __shared__ uint64_t s[kSmallSortKeys], sD[kSmallSortKeys];
   __shared__ uint8_t s_f[kSmallSortKeys];  // per lookup item: new (I) / absent (D)
   __shared__ uint32_t s_w[16];
@@ -493,12 +499,14 @@ __global__ void __launch_bounds__(1024) k_small_mixed_prep(GView g, const uint32
     const bool ins = q < nI;
     const uint64_t key = ins ? s[q] : sD[q - nI];
     uint32_t lf = ~0u;
-    const bool e = edge_exists(g, key_src(key), key_dst(key), ins ? &lf : nullptr);
+    uint64_t sl = ~0ull;
+    const bool e = edge_exists(g, key_src(key), key_dst(key), ins ? &lf : nullptr, ins ? nullptr : &sl);
     if (ins) {
       exI[q] = e ? 1u : 0u;
       leafI[q] = lf;
     } else {
       exD[q - nI] = e ? 1u : 0u;
+      slotD[q - nI] = sl;
     }
     s_f[q] = e ? 0 : 1;
   }
@@ -862,12 +870,13 @@ constexpr int kPrepBlock = 256;
 // to ex; otherwise ex holds them already.
 // The work of block b (in start order) of nblocks.
 // leaf (optional, with kLookup): per key the PMA leaf its lookup searched,
-// ~0 when the lookup did not reach the PMA.
+// ~0 when the lookup did not reach the PMA.  slot (optional, with
+// kLookup): per key the PMA slot that holds it, ~0 when it is not there.
 template <int kItems, bool kLookup>
 __device__ inline void insert_prep_block(uint32_t b, GView g, const uint64_t *R, const uint64_t *np, uint64_t cap,
                                          uint32_t *ex, uint32_t *nidx, uint32_t *run_beg, uint64_t *nruns,
                                          uint64_t *state, uint32_t *err, uint32_t nblocks, uint32_t epoch,
-                                         uint32_t *leaf = nullptr) {
+                                         uint32_t *leaf = nullptr, uint64_t *slot = nullptr) {
   constexpr int kPrepItems = kItems, kPrepTile = kPrepBlock * kItems;
   __shared__ uint32_t wcnt[2][kPrepItems][kPrepBlock / 64];
   __shared__ uint32_t wpre[2][kPrepItems][kPrepBlock / 64];
@@ -887,8 +896,10 @@ __device__ inline void insert_prep_block(uint32_t b, GView g, const uint64_t *R,
       const uint64_t key = R[idx];
       if (kLookup) {
         uint32_t lf = ~0u;
-        e[i] = edge_exists(g, key_src(key), key_dst(key), leaf ? &lf : nullptr);
+        uint64_t sl = ~0ull;
+        e[i] = edge_exists(g, key_src(key), key_dst(key), leaf ? &lf : nullptr, slot ? &sl : nullptr);
         if (leaf) leaf[idx] = lf;
+        if (slot) slot[idx] = sl;
       } else {
         e[i] = ex[idx] != 0;
       }
@@ -1009,13 +1020,16 @@ __global__ void __launch_bounds__(kPrepBlock) k_delete_prep(GView g, const uint6
 // the absent keys (dnidx; the existing keys before i are i - dnidx[i]) and
 // their source runs (drun_beg, count *druns), so the delete half runs over
 // D itself and no compaction is needed.  ileaf: the PMA leaf each inserted
-// key's lookup searched (~0 when it did not reach the PMA).
+// key's lookup searched (~0 when it did not reach the PMA); dslot: the PMA
+// slot of each deleted key (~0 when it is not in the PMA), a hint for the
+// PMA delete.
 __global__ void __launch_bounds__(kPrepBlock) k_mixed_prep(GView g, const uint64_t *I, const uint64_t *ni,
                                                            uint64_t cap, uint32_t *ex, uint32_t *nidx,
                                                            uint32_t *run_beg, uint64_t *nruns, uint32_t *ileaf,
                                                            const uint64_t *D,
                                                            const uint64_t *nd, uint32_t *dex, uint32_t *dnidx,
-                                                           uint32_t *drun_beg, uint64_t *druns, uint64_t *state,
+                                                           uint32_t *drun_beg, uint64_t *druns, uint64_t *dslot,
+                                                           uint64_t *state,
                                                            uint32_t *misc, uint32_t *err, uint32_t nb,
                                                            uint32_t epoch) {
   __shared__ uint32_t s_ticket;
@@ -1025,7 +1039,7 @@ __global__ void __launch_bounds__(kPrepBlock) k_mixed_prep(GView g, const uint64
     insert_prep_block<1, true>(b, g, I, ni, cap, ex, nidx, run_beg, nruns, state, err, nb, epoch, ileaf);
   else
     insert_prep_block<1, true>(b - nb, g, D, nd, cap, dex, dnidx, drun_beg, druns, state + 2ull * nb, err, nb,
-                               epoch);
+                               epoch, nullptr, dslot);
 }
 
 // ------------------------------------------------------------ gates
@@ -2009,13 +2023,21 @@ __device__ inline uint64_t next_begin_after(const GView &g, uint32_t s, uint64_t
 // the first element the batch keeps.  A leaf left at or below the leaf
 // lower bound lo0 raises *low (the batch then respreads the underfull
 // windows, as src/PMA.cc:1636-1746 does per removal).
+// hint (optional): per key the slot where the batch's preparation found it.
+// The insert half may have moved it since, so a hint is used only while the
+// slot still holds the key; otherwise the key is searched for.
 __global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t cap, uint64_t *err, const uint64_t *kdev,
-                             uint32_t lo0, uint64_t *low) {
+                             uint32_t lo0, uint64_t *low, const uint64_t *hint) {
   const uint64_t k = dev_count(kdev, cap);
   GRID_STRIDE(i, k) {
     const uint32_t s = key_src(keys[i]), d = key_dst(keys[i]);
     const uint64_t begin = g.vb[s].aux;
-    const uint64_t slot = pma_slot_of(g, s, d, begin);
+    uint64_t slot = ~0ull;
+    if (hint) {
+      const uint64_t h = hint[i];
+      if (h < g.N && g.pdst[h] == d && g.psrc[h] == s) slot = h;
+    }
+    if (slot == ~0ull) slot = pma_slot_of(g, s, d, begin);
     if (slot == ~0ull) {
       atomicOr(reinterpret_cast<unsigned long long *>(err), 4ull);
       continue;
@@ -2469,14 +2491,16 @@ __global__ void __launch_bounds__(256) k_tree_small_apply(GView g, const uint64_
 // ds[17] refill keys for the tree delete (ds[16], refill keys for the PMA,
 // is 0: k_bump removes those itself); ds[3] the existing keys deleted.
 // ds[9] (k_bump's counter of PMA elements moved inline) and, when the half
-// runs, *low are cleared here.
+// runs, *low are cleared here.  dslot (optional): per key of R its PMA slot
+// as the preparation found it; each key written to P takes its slot to Ph.
 __global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64_t *R, const uint32_t *exf,
                                                            const uint32_t *dnidx, const uint32_t *run_beg,
                                                            const uint64_t *cw, const uint64_t *gate, bool spec,
                                                            bool pma_launched, uint64_t *gw, uint64_t cap_keys,
                                                            uint32_t cap_runs, uint64_t *P, uint64_t *T,
                                                            uint32_t *run_bump, uint32_t *tbump_off, uint64_t *ds,
-                                                           uint64_t *low, uint64_t *state, uint32_t *misc,
+                                                           uint64_t *low, const uint64_t *dslot, uint64_t *Ph,
+                                                           uint64_t *state, uint32_t *misc,
                                                            uint32_t *err, uint32_t nblocks, uint32_t epoch) {
   __shared__ uint32_t s_ticket;
   __shared__ uint32_t s_b0[kRunBlock + 1], s_off[kRunBlock], s_nb0[kRunBlock];
@@ -2593,7 +2617,13 @@ __global__ void __launch_bounds__(kRunBlock) k_delete_runs(GView g, const uint64
     }
     if (inl) continue;
     const uint32_t rank = (i - s_b0[lo]) - (exf ? dnidx[i] - s_nb0[lo] : 0u);
-    (s_tier[lo] == kRoutePma ? P : T)[s_off[lo] + rank - less] = R[i];
+    const uint32_t pos = s_off[lo] + rank - less;
+    if (s_tier[lo] == kRoutePma) {
+      P[pos] = R[i];
+      if (dslot) Ph[pos] = dslot[i];
+    } else {
+      T[pos] = R[i];
+    }
   }
 }
 
@@ -3221,7 +3251,7 @@ void Graph::trim_scratch() noexcept {
   trim(f0_), trim(f1_), trim(f2_), trim(f3_), trim(g0_), trim(g2_), trim(g3_);
   trim(u0_), trim(u1_), trim(u2_), trim(u3_), trim(u4_), trim(kick_), trim(route_), trim(tmp_old_);
   trim(ops_val_), trim(ops_ins_), trim(ops_del_), trim(sbkeys_), trim(sort_.alt), trim(sort_.alt_vals);
-  trim(wv0_), trim(wv1_), trim(wput_);
+  trim(wv0_), trim(wv1_), trim(wput_), trim(dslot_), trim(dhint_);
   for (auto &b : s32_) trim(b);
   for (auto &b : s8_) trim(b);
   for (auto &b : s64_) trim(b);
@@ -3501,11 +3531,11 @@ void Graph::pma_rebalance_low() {
 
 // Delete up to cap present keys (kdev: their count on the device); the
 // caller lowers pma_elems_ by the count.
-void Graph::pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev) {
+void Graph::pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev, const uint64_t *hint) {
   GView g = view();
   k_pma_delete<<<grid_for(cap, 256), 256, 0, stream_>>>(g, keys, cap, dsc_.get(), kdev,
                                                          static_cast<uint32_t>(bounds_.lo[0]),
-                                                         dsc_.get() + kLowFlag);
+                                                         dsc_.get() + kLowFlag, hint);
   TG_CHECK_LAUNCH();
   tree_dirty_ = true;
 }
@@ -4125,11 +4155,13 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
   // compacted into kE_ (count dsc_[10]), their runs in s32_[1] (dsc_[11])
   uint32_t *dnidx = nullptr;
   uint32_t *ileaf = nullptr;  // the inserted keys' PMA leaves from the preparation (bucket path)
+  uint64_t *dslot = nullptr;  // the deleted keys' PMA slots from the preparation (fused paths)
   for (int pass = 0; pass < 2; pass++) {
     k1_.ensure(count + 1, stream_);
     kD_.ensure(count + 1, stream_);
     const bool bucket = small ? false : prepare_mixed_async(is, id, ni, ds, dd, nd, pass == 1);
     ileaf = nullptr;
+    dslot = nullptr;
     GView g = view();
     const uint64_t *I = k1_.get(), *D = kD_.get();
     f0_.ensure(count + 1, stream_);
@@ -4145,19 +4177,23 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
     if (small) {
       // the whole preparation in one workgroup (also clears the batch scalars)
       ileaf = s32_[3].get();
+      dslot_.ensure(count + 1, stream_);
+      dslot = dslot_.get();
       k_small_mixed_prep<<<1, 1024, 0, stream_>>>(g, is, id, static_cast<uint32_t>(ni), ds, dd,
                                                   static_cast<uint32_t>(nd), k1_.get(), kD_.get(), dsc_.get(),
                                                   f0_.get(), f1_.get(), s32_[0].get(), ileaf, fD_.get(), dnidx,
-                                                  s32_[1].get());
+                                                  s32_[1].get(), dslot);
       TG_CHECK_LAUNCH();
     } else if (count <= kBucketSortMax) {
       const uint32_t nb = static_cast<uint32_t>((count + kPrepBlock - 1) / kPrepBlock);
       const uint32_t epoch = scan_scratch.lb.begin(4ull * nb, stream_);
       ileaf = s32_[3].get();
+      dslot_.ensure(count + 1, stream_);
+      dslot = dslot_.get();
       k_mixed_prep<<<2 * nb, kPrepBlock, 0, stream_>>>(
           g, I, dsc_.get() + 1, count, f0_.get(), f1_.get(), s32_[0].get(), dsc_.get() + 2, ileaf, D, dsc_.get() + 9,
-          fD_.get(), dnidx, s32_[1].get(), dsc_.get() + 10, scan_scratch.lb.state.get(), scan_scratch.lb.misc.get(),
-          scan_scratch.lb.err_word(), nb, epoch);
+          fD_.get(), dnidx, s32_[1].get(), dsc_.get() + 10, dslot, scan_scratch.lb.state.get(),
+          scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb, epoch);
       TG_CHECK_LAUNCH();
     } else {
       kE_.ensure(count + 1, stream_);
@@ -4188,7 +4224,7 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
         pma_launched =
             insert_launch(k1_.get(), count, static_cast<uint32_t>(count), dsc_.get() + 1, dsc_.get(), ileaf);
         delete_launch(kD_.get(), fD_.get(), dnidx, count, static_cast<uint32_t>(count), s32_[1].get(),
-                      dsc_.get() + 9, dsc_.get(), true, pma_launched, dsd);
+                      dsc_.get() + 9, dsc_.get(), true, pma_launched, dsd, dslot);
       });
     }
     read_scalars(dsc_.get(), h, kScalars, stream_);
@@ -4231,7 +4267,7 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
       // the insert's general passes are done (they may have used dsc_[9],
       // [10]); the delete half now
       delete_launch(kD_.get(), fD_.get(), dnidx, nD, static_cast<uint32_t>(nDruns), s32_[1].get(),
-                    put_counts(nD, nDruns), nullptr, false, false, dsd);
+                    put_counts(nD, nDruns), nullptr, false, false, dsd, dslot);
       uint64_t hs[kScalars];
       read_scalars(dsc_.get(), hs, kScalars, stream_);
       stats_.last_del_keys = hs[bsw::kDel + 3];
@@ -4250,6 +4286,7 @@ void Graph::presize_delete(uint64_t nR, uint32_t nruns) {
   k3_.ensure(nR + 1, stream_);
   g0_.ensure(nR + 1, stream_);
   s64_[3].ensure(nR + 1, stream_);
+  dhint_.ensure(nR + 1, stream_);
   s32_[30].ensure(1 + 3 * std::min<uint64_t>(nR, kTreeSmallKeys), stream_);
 }
 
@@ -4288,7 +4325,7 @@ void Graph::delete_refill(const uint64_t *R, uint32_t capRuns, const uint32_t *r
 // when the first tree delete did not fit.
 void Graph::delete_launch(const uint64_t *R, const uint32_t *exf, const uint32_t *dnidx, uint64_t capR,
                           uint32_t capRuns, const uint32_t *run_beg, const uint64_t *cw, const uint64_t *gate, bool spec,
-                          bool pma_launched, uint64_t *ds) {
+                          bool pma_launched, uint64_t *ds, const uint64_t *dslot) {
   lists_dirty_ = true;
   ++epoch_;
   GView g = view();
@@ -4297,13 +4334,15 @@ void Graph::delete_launch(const uint64_t *R, const uint32_t *exf, const uint32_t
   k3_.ensure(capR + 1, stream_);
   g0_.ensure(capR + 1, stream_);
   s64_[3].ensure(capR + 1, stream_);
+  if (dslot) dhint_.ensure(capR + 1, stream_);
   // inline deletes of every run and the secondary deletes split by tier
   // (PMA keys into k3_, count ds[4]; tree keys into g0_, ds[5])
   const uint32_t nb = static_cast<uint32_t>(std::max<uint64_t>(1, (capRuns + kRunBlock - 1) / kRunBlock));
   const uint32_t epoch = scan_scratch.lb.begin(4ull * nb, stream_);
   k_delete_runs<<<nb, kRunBlock, 0, stream_>>>(g, R, exf, dnidx, run_beg, cw, gate, spec, pma_launched, ds + bsw::kGate, capR,
                                                capRuns, k3_.get(), g0_.get(), s32_[12].get(), s32_[13].get(), ds,
-                                               dsc_.get() + kLowFlag, scan_scratch.lb.state.get(),
+                                               dsc_.get() + kLowFlag, dslot, dslot ? dhint_.get() : nullptr,
+                                               scan_scratch.lb.state.get(),
                                                scan_scratch.lb.misc.get(), scan_scratch.lb.err_word(), nb, epoch);
   TG_CHECK_LAUNCH();
   if (debug_fail & 2u) {
@@ -4315,7 +4354,7 @@ void Graph::delete_launch(const uint64_t *R, const uint32_t *exf, const uint32_t
   const bool side = capR >= kSideMin;
   if (side) fork_side(2);
   tree_small_launch(g0_.get(), capR, false, ds + 5, nullptr, ds + 13, side ? side_ : nullptr);
-  pma_delete_launch(k3_.get(), capR, ds + 4);
+  pma_delete_launch(k3_.get(), capR, ds + 4, dslot ? dhint_.get() : nullptr);
   if (side) join_side(3);
   delete_refill(R, capRuns, run_beg, ds, false);
   tree_small_launch(s64_[3].get(), capR, false, ds + 17, ds + 13, ds + 19);

@@ -297,6 +297,9 @@ class Graph {
   DevBuf<uint32_t> s32_[32];
   DevBuf<uint8_t> s8_[8];
   DevBuf<uint64_t> s64_[4];
+  // mixed batches: the PMA slot of each deleted key as the preparation
+  // found it, and the same hints for the PMA delete list
+  DevBuf<uint64_t> dslot_, dhint_;
   // bucket sort of mid-size batches: per-bucket counts and cursors (zero
   // between calls), bucket starts, the bucketed keys
   DevBuf<uint32_t> sbcnt_, sbcur_, sboff_;
@@ -343,10 +346,11 @@ class Graph {
   // insert half of the same batch, it runs only when that half completed
   // on the device (pma_launched: the insert's in-leaf merge was launched).
   // R: the deleted keys, with exf their existence flags and dnidx the prefix
-  // of the absent ones (both null: every key of R exists).
+  // of the absent ones (both null: every key of R exists).  dslot (optional):
+  // per key of R the PMA slot the preparation found it in (a hint).
   void delete_launch(const uint64_t *R, const uint32_t *exf, const uint32_t *dnidx, uint64_t capR,
                      uint32_t capRuns, const uint32_t *run_beg, const uint64_t *cw, const uint64_t *gate, bool spec,
-                     bool pma_launched, uint64_t *ds);
+                     bool pma_launched, uint64_t *ds, const uint64_t *dslot = nullptr);
   void delete_finish(const uint64_t *hs, const uint64_t *R, const uint32_t *run_beg, uint64_t *ds);
   // k_bump: the inline refill (tree keys it moves into s64_[3], count ds[17])
   void delete_refill(const uint64_t *R, uint32_t capRuns, const uint32_t *run_beg, uint64_t *ds, bool trees_only);
@@ -373,7 +377,8 @@ class Graph {
   // (k_insert_runs wrote it)
   void pma_fast_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev = nullptr,
                        bool leaves_known = false);
-  void pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev);
+  // hint (optional): per key a slot that may still hold it
+  void pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t *kdev, const uint64_t *hint = nullptr);
   void pma_fast_done(uint64_t k, uint64_t nt);
   void pma_window_insert(const uint64_t *keys, uint64_t k);
   void pma_respread(const uint64_t *keys, uint64_t k, uint64_t cap, bool low);
