@@ -205,6 +205,7 @@ def cpu_baseline(quick: bool, batch: int):
 
 SERVE_PROBS = (0.57, 0.19, 0.19)
 SERVE_SUMMARY_REQUESTS = 60  # per rate in the default run's serving summary
+SUSTAINED_BATCHES = 60  # the default run's sustained update figure
 
 
 def serve_requests(tg, n, G, R, rng):
@@ -1010,6 +1011,23 @@ def main():
         tg.lib().tg_device_sync()
         dt = time.perf_counter() - t0
         extra["update_edges_per_s_host_arrays_b1e5"] = sum(len(h[0]) + len(h[2]) for h in hb) / dt
+        # sustained: 60 more mixed batches of the headline size as one timed
+        # region, including the batches that promote a vertex to the B-tree
+        # tier (about one in twelve here), which take the general passes
+        bt = make_batches(tg, G, n, probs, args.batch, SUSTAINED_BATCHES, 9000, rng, off, nbr)
+        sa = G.stats()
+        tg.lib().tg_device_sync()
+        t0 = time.perf_counter()
+        for b_ in bt:
+            apply_batch(G, b_)
+        tg.lib().tg_device_sync()
+        dt = time.perf_counter() - t0
+        sb = G.stats()
+        extra["update_sustained"] = {
+            "batches": SUSTAINED_BATCHES, "edges_per_s": sum(b_[2] + b_[5] for b_ in bt) / dt,
+            "ms_per_batch": dt / SUSTAINED_BATCHES * 1e3, "promotions": sb["promotions"] - sa["promotions"]}
+        log(f"sustained: {SUSTAINED_BATCHES} batches, {extra['update_sustained']['ms_per_batch']:.4f} ms per batch, "
+            f"{extra['update_sustained']['promotions']} promotions")
         extra["memory_bytes"] = G.get_size()
         extra["stats"] = G.stats()
 

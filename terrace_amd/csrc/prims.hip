@@ -489,8 +489,45 @@ void radix_sort_pairs(uint64_t *keys, uint32_t *vals, uint64_t n, uint32_t lo_bi
   }
 }
 
+namespace {
+constexpr uint32_t kSortSmallKeys = 4096;
+// At most kSortSmallKeys keys: one workgroup sorts them in LDS (bitonic,
+// padded with ~0 to a power of two) and writes them back in place.  Keys
+// carry no values, so the order equals the radix sort's.
+__global__ void __launch_bounds__(1024) k_sort_small_u64(uint64_t *keys, uint32_t n) {
+  __shared__ uint64_t sk[kSortSmallKeys];
+  const uint32_t t = threadIdx.x;
+  uint32_t P = 2;
+  while (P < n) P <<= 1;
+  for (uint32_t i = t; i < P; i += blockDim.x) sk[i] = i < n ? keys[i] : ~0ull;
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = t; i < P; i += blockDim.x) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = sk[i], b = sk[ixj];
+          if ((a > b) == ((i & k) == 0)) {
+            sk[i] = b;
+            sk[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = t; i < n; i += blockDim.x) keys[i] = sk[i];
+}
+}  // namespace
+
 void radix_sort_keys(uint64_t *keys, uint64_t n, uint32_t lo_bits, uint32_t hi_bits, SortScratch &ss,
                      hipStream_t s) {
+  if (n < 2) return;
+  if (n <= kSortSmallKeys) {  // one launch instead of the histogram and a pass per digit
+    k_sort_small_u64<<<1, 1024, 0, s>>>(keys, static_cast<uint32_t>(n));
+    TG_CHECK_LAUNCH();
+    return;
+  }
   radix_sort_pairs(keys, nullptr, n, lo_bits, hi_bits, ss, s);
 }
 

@@ -233,7 +233,9 @@ struct SortScratch {
 };
 
 // Stable LSD radix sort of keys in place over the bit ranges that vary:
-// dst bits [0, lo_bits) and src bits [32, 32 + hi_bits).  8-bit digits.
+// dst bits [0, lo_bits) and src bits [32, 32 + hi_bits) (every other bit
+// zero).  8-bit digits.  At most 4096 keys are sorted by one workgroup in
+// LDS instead (the same order: keys carry no values).
 void radix_sort_keys(uint64_t *keys, uint64_t n, uint32_t lo_bits, uint32_t hi_bits,
                      SortScratch &ss, hipStream_t s);
 
