@@ -2051,9 +2051,12 @@ __global__ void k_pma_delete(GView g, const uint64_t *keys, uint64_t cap, uint64
 
 // ------------------------------------------------------------ promotion
 // One wave per promoted vertex: copy its PMA run (D_old - 13 elements
-// from the begin slot) into ext[ext_off ..].
+// from the begin slot) into ext[ext_off ..] and remove it from the PMA
+// (slots emptied, leaf counts lowered, *low raised when a leaf falls to
+// lo0 or below).  The vertex's begin word is replaced by its tree next.
 __global__ void __launch_bounds__(256) k_extract_pma(GView g, const uint32_t *pv, const uint32_t *pcount,
-                                                     const uint32_t *ext_off, uint32_t np, uint64_t *ext) {
+                                                     const uint32_t *ext_off, uint32_t np, uint64_t *ext,
+                                                     uint32_t lo0, uint64_t *low) {
   uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) >> 6;
   uint64_t nwaves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
   uint32_t lane = threadIdx.x & 63;
@@ -2069,7 +2072,12 @@ __global__ void __launch_bounds__(256) k_extract_pma(GView g, const uint32_t *pv
       bool v = d != kEmpty;
       uint64_t m = __ballot(v);
       uint32_t r = __popcll(m & lanemask_lt());
-      if (v && got + r < need) ext[ext_off[w] + got + r] = make_key(s, d);
+      if (v && got + r < need) {
+        ext[ext_off[w] + got + r] = make_key(s, d);
+        g.pdst[slot] = kEmpty;
+        if (atomicSub(&g.pcnt[slot >> kLeafShift], 1u) - 1u <= lo0)
+          atomicOr(reinterpret_cast<unsigned long long *>(low), 1ull);
+      }
       got += __popcll(m);
       p += 64;
       if (p >= g.N) break;
@@ -3540,12 +3548,6 @@ void Graph::pma_delete_launch(const uint64_t *keys, uint64_t cap, const uint64_t
   tree_dirty_ = true;
 }
 
-void Graph::pma_delete(const uint64_t *keys, uint64_t k) {
-  if (k == 0) return;
-  pma_delete_launch(keys, k, nullptr);
-  pma_elems_ -= k;  // every key is present (a missing one raises an error flag)
-}
-
 // ------------------------------------------------------------ tree update
 // Launch the in-place tree update (status in dsc_[13]: 0 = done).
 // Up to min(cap, kTreeSmallKeys) keys (kdev: their count on the device;
@@ -3973,11 +3975,14 @@ bool Graph::insert_finish(const uint64_t *hs, const uint64_t *R, bool pma_launch
     next = read_scalar(pext.get() + np, stream_);
     ext.ensure(next + 1, stream_);
     if (next) {
-      k_extract_pma<<<grid_for(static_cast<uint64_t>(np) * 64, 256), 256, 0, stream_>>>(g, pv.get(), pcount.get(),
-                                                                                       pext.get(), np, ext.get());
-      TG_CHECK_LAUNCH();
+      // the runs leave the PMA as they are copied out
       fill_u64(dsc_.get() + kLowFlag, 0, 1, stream_);
-      pma_delete(ext.get(), next);
+      k_extract_pma<<<grid_for(static_cast<uint64_t>(np) * 64, 256), 256, 0, stream_>>>(
+          g, pv.get(), pcount.get(), pext.get(), np, ext.get(), static_cast<uint32_t>(bounds_.lo[0]),
+          dsc_.get() + kLowFlag);
+      TG_CHECK_LAUNCH();
+      pma_elems_ -= next;
+      tree_dirty_ = true;
     }
     ensure_trees(ntrees_ + np);
     g = view();
@@ -3995,10 +4000,10 @@ bool Graph::insert_finish(const uint64_t *hs, const uint64_t *R, bool pma_launch
     radix_sort_keys(g0_.get(), ntree, bits, bits, sort_, stream_);
   }
   if (!tree_ok || next) {
-    // waited for promotions (status 2): the in-place update may fit now;
-    // did not fit in place (status 1): copy-on-write
-    if (np) tree_insert(g0_.get(), ntree);
-    else tree_insert_cow(g0_.get(), ntree);
+    // waited for promotions (status 2) or did not fit in place (status 1):
+    // copy-on-write.  After a promotion the in-place update cannot succeed:
+    // each new tree has keys of this batch and no blocks yet.
+    tree_insert_cow(g0_.get(), ntree);
   }
   if (!pma_ok) {
     if (pma_launched) pma_window_insert(k3_.get(), npma);

@@ -385,7 +385,6 @@ class Graph {
   void pma_rebalance_low();
   void pma_rebuild(const uint64_t *keys, uint64_t k, uint64_t total);
   void rebuild_count_tree();
-  void pma_delete(const uint64_t *keys, uint64_t k);
   void tree_insert(const uint64_t *keys, uint64_t k);
   void tree_delete(const uint64_t *keys, uint64_t k);
   // in-place update of up to 4096 tree keys when every touched block keeps
