@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # summed over the step's kernels from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs scaled by a 1 GiB copy kernel (FETCH_SIZE x 2, WRITE_SIZE
 # x 1): profiles/update_step_scale22_hbm_traffic.md.
-UPDATE_STEP_TRAFFIC_DEFAULT = 2.069e8
+UPDATE_STEP_TRAFFIC_DEFAULT = 2.051e8
 # HBM bytes per PageRank iteration (k_prs_pull + k_prs_fix + k_prs_reduce)
 # at the default workload (scale 22, 69M generated edges), from rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy kernel
