@@ -401,30 +401,23 @@ __global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, c
   if (t == 0) *nunique = tot;
 }
 
-// Mixed batches of at most kSmallSortKeys edges in all: the whole
-// preparation in one workgroup.  The inserts (is, id) and deletes (ds, dd)
-// are packed as tagged keys, validated, sorted in LDS and split by the last
-// op of each edge (deletes come after inserts, so a batch that inserts and
+// Mixed batches of at most kSmallSortKeys edges in all: pack, validate,
+// sort and split in one workgroup.  The inserts (is, id) and deletes (ds,
+// dd) are packed as tagged keys, sorted in LDS and split by the last op of
+// each edge (deletes come after inserts, so a batch that inserts and
 // deletes an edge leaves it absent, as applying the inserts and then the
 // deletes does): edges inserted into I (count sc[1]), edges deleted into D
-// (count sc[9]).  Then, for both, the existence lookups (one key per thread
-// while there are at most 1024 keys, all of them issued together), the
-// exclusive prefix of the new (I) or absent (D) keys over [0, count] and the
-// source runs (I: runI, count sc[2]; D: runD, count sc[10]; each with its
-// terminator); exI, exD the existence flags, leafI the PMA leaf each
-// inserted key's lookup searched (~0: none), slotD the PMA slot of each
-// deleted key (~0: not in the PMA).  Clears the batch scalars.
-__global__ void __launch_bounds__(1024) k_small_mixed_prep(GView g, const uint32_t *is, const uint32_t *id,
-                                                           uint32_t ni, const uint32_t *ds, const uint32_t *dd,
-                                                           uint32_t nd, uint64_t *I, uint64_t *D, uint64_t *sc,
-                                                           uint32_t *exI, uint32_t *nidxI, uint32_t *runI,
-                                                           uint32_t *leafI, uint32_t *exD, uint32_t *nidxD,
-                                                           uint32_t *runD, uint64_t *slotD) {
-  __shared__ uint64_t s[kSmallSortKeys], sD[kSmallSortKeys];
-  __shared__ uint8_t s_f[kSmallSortKeys];  // per lookup item: new (I) / absent (D)
+// (count sc[9]).  Clears the batch scalars.  The lookups, prefixes and runs
+// of both halves then run in k_mixed_prep across many workgroups: in this
+// one workgroup the thousand lookup chains of a 1e3 batch took 30 us of the
+// kernel's 50.
+__global__ void __launch_bounds__(1024) k_small_mixed_sort(const uint32_t *is, const uint32_t *id, uint32_t ni,
+                                                           const uint32_t *ds, const uint32_t *dd, uint32_t nd,
+                                                           uint32_t nv, uint64_t *I, uint64_t *D, uint64_t *sc) {
+  __shared__ uint64_t s[kSmallSortKeys];
   __shared__ uint32_t s_w[16];
   const uint32_t t = threadIdx.x;
-  const uint32_t count = ni + nd, nv = g.n;
+  const uint32_t count = ni + nd;
   if (t < 14) sc[t] = 0;  // the batch scalars (the look-back error words [14], [15] stay)
   __syncthreads();
   uint32_t P = 1;
@@ -475,88 +468,16 @@ __global__ void __launch_bounds__(1024) k_small_mixed_prep(GView g, const uint32
     cd += fd[j] ? 1u : 0u;
   }
   uint32_t nI, nD;
-  uint32_t pi = wg_excl_scan(ci, s_w, &nI);  // (its barriers also order the reads of s above)
+  uint32_t pi = wg_excl_scan(ci, s_w, &nI);
   uint32_t pd = wg_excl_scan(cd, s_w, &nD);
 #pragma unroll
   for (uint32_t j = 0; j < 4; j++) {
-    if (fi[j]) {
-      I[pi] = kk[j];
-      s[pi++] = kk[j];
-    }
-    if (fd[j]) {
-      D[pd] = kk[j];
-      sD[pd++] = kk[j];
-    }
+    if (fi[j]) I[pi++] = kk[j];
+    if (fd[j]) D[pd++] = kk[j];
   }
   if (t == 0) {
     sc[1] = nI;
     sc[9] = nD;
-  }
-  __syncthreads();
-  // existence lookups: items [0, nI) the inserted keys, [nI, nI + nD) the
-  // deleted ones
-  for (uint32_t q = t; q < nI + nD; q += blockDim.x) {
-    const bool ins = q < nI;
-    const uint64_t key = ins ? s[q] : sD[q - nI];
-    uint32_t lf = ~0u;
-    uint64_t sl = ~0ull;
-    const bool e = edge_exists(g, key_src(key), key_dst(key), ins ? &lf : nullptr, ins ? nullptr : &sl);
-    if (ins) {
-      exI[q] = e ? 1u : 0u;
-      leafI[q] = lf;
-    } else {
-      exD[q - nI] = e ? 1u : 0u;
-      slotD[q - nI] = sl;
-    }
-    s_f[q] = e ? 0 : 1;
-  }
-  __syncthreads();
-  // the prefixes of the new / absent keys over [0, count] (0 past the keys)
-  for (uint32_t half = 0; half < 2; half++) {
-    const uint32_t n = half ? nD : nI, base = half ? nI : 0;
-    uint32_t *nidx = half ? nidxD : nidxI, *ex = half ? exD : exI;
-    uint32_t x[4], c = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < 4; j++) {
-      const uint32_t i = 4 * t + j;
-      x[j] = i < n ? s_f[base + i] : 0u;
-      c += x[j];
-    }
-    uint32_t tot;
-    uint32_t pre = wg_excl_scan(c, s_w, &tot);
-#pragma unroll
-    for (uint32_t j = 0; j < 4; j++) {
-      const uint32_t i = 4 * t + j;
-      if (i < count) {
-        nidx[i] = pre;
-        if (i >= n) ex[i] = 0;
-      }
-      pre += x[j];
-    }
-    if (t == 0) nidx[count] = tot;
-  }
-  // the source runs of I and of D, each with run_beg[nruns] = n
-  for (uint32_t half = 0; half < 2; half++) {
-    const uint32_t n = half ? nD : nI;
-    const uint64_t *k = half ? sD : s;
-    uint32_t *run_beg = half ? runD : runI;
-    bool f[4];
-    uint32_t c = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < 4; j++) {
-      const uint32_t i = 4 * t + j;
-      f[j] = i < n && (i == 0 || key_src(k[i]) != key_src(k[i - 1]));
-      c += f[j] ? 1u : 0u;
-    }
-    uint32_t tot;
-    uint32_t pos = wg_excl_scan(c, s_w, &tot);
-#pragma unroll
-    for (uint32_t j = 0; j < 4; j++)
-      if (f[j]) run_beg[pos++] = 4 * t + j;
-    if (t == 0) {
-      run_beg[tot] = n;
-      sc[half ? 10 : 2] = tot;
-    }
   }
 }
 
@@ -3213,7 +3134,7 @@ bool Graph::prepare_mixed_async(const uint32_t *is, const uint32_t *id, uint64_t
   const uint64_t count = ni + nd;
   k1_.ensure(count + 1, stream_);
   kD_.ensure(count + 1, stream_);
-  // (batches of at most kSmallSortKeys edges take k_small_mixed_prep instead)
+  // (batches of at most kSmallSortKeys edges are sorted by k_small_mixed_sort instead)
   k0_.ensure(count, stream_);
   uint32_t bits = 1;
   while (bits < 32 && (1ull << bits) < n_) bits++;
@@ -4180,16 +4101,13 @@ void Graph::update_batch(const uint32_t *is, const uint32_t *id, uint64_t ni, co
     // inserted edges: existence, new-key prefix, source runs (run_beg in
     // s32_[0], count in dsc_[2])
     if (small) {
-      // the whole preparation in one workgroup (also clears the batch scalars)
-      ileaf = s32_[3].get();
-      dslot_.ensure(count + 1, stream_);
-      dslot = dslot_.get();
-      k_small_mixed_prep<<<1, 1024, 0, stream_>>>(g, is, id, static_cast<uint32_t>(ni), ds, dd,
-                                                  static_cast<uint32_t>(nd), k1_.get(), kD_.get(), dsc_.get(),
-                                                  f0_.get(), f1_.get(), s32_[0].get(), ileaf, fD_.get(), dnidx,
-                                                  s32_[1].get(), dslot);
+      // sort and split in one workgroup (also clears the batch scalars); the
+      // rest of the preparation as for the bucket path
+      k_small_mixed_sort<<<1, 1024, 0, stream_>>>(is, id, static_cast<uint32_t>(ni), ds, dd,
+                                                  static_cast<uint32_t>(nd), n_, k1_.get(), kD_.get(), dsc_.get());
       TG_CHECK_LAUNCH();
-    } else if (count <= kBucketSortMax) {
+    }
+    if (count <= kBucketSortMax) {
       const uint32_t nb = static_cast<uint32_t>((count + kPrepBlock - 1) / kPrepBlock);
       const uint32_t epoch = scan_scratch.lb.begin(4ull * nb, stream_);
       ileaf = s32_[3].get();
