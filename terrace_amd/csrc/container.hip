@@ -369,21 +369,7 @@ __global__ void __launch_bounds__(1024) k_pack_sort_small(const uint32_t *src, c
   }
   if (bad) atomicOr(reinterpret_cast<unsigned long long *>(err), 1ull);
   __syncthreads();
-  for (uint32_t k = 2; k <= P; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = t; i < P; i += blockDim.x) {
-        const uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          const uint64_t a = s[i], b = s[ixj];
-          if ((a > b) == ((i & k) == 0)) {
-            s[i] = b;
-            s[ixj] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
+  wg_bitonic_sort(s, P);
   // unique, in order: thread t takes keys 4t .. 4t+3
   bool f[4];
   uint32_t c = 0;
@@ -438,21 +424,7 @@ __global__ void __launch_bounds__(1024) k_small_mixed_sort(const uint32_t *is, c
   }
   if (bad) atomicOr(reinterpret_cast<unsigned long long *>(sc), 1ull);
   __syncthreads();
-  for (uint32_t k = 2; k <= P; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = t; i < P; i += blockDim.x) {
-        const uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          const uint64_t a = s[i], b = s[ixj];
-          if ((a > b) == ((i & k) == 0)) {
-            s[i] = b;
-            s[ixj] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
+  wg_bitonic_sort(s, P);
   // the last key of each edge decides: thread t takes keys 4t .. 4t+3
   bool fi[4], fd[4];
   uint64_t kk[4];

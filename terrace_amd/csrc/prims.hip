@@ -501,21 +501,7 @@ __global__ void __launch_bounds__(1024) k_sort_small_u64(uint64_t *keys, uint32_
   while (P < n) P <<= 1;
   for (uint32_t i = t; i < P; i += blockDim.x) sk[i] = i < n ? keys[i] : ~0ull;
   __syncthreads();
-  for (uint32_t k = 2; k <= P; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = t; i < P; i += blockDim.x) {
-        const uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          const uint64_t a = sk[i], b = sk[ixj];
-          if ((a > b) == ((i & k) == 0)) {
-            sk[i] = b;
-            sk[ixj] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
+  wg_bitonic_sort(sk, P);
   for (uint32_t i = t; i < n; i += blockDim.x) keys[i] = sk[i];
 }
 }  // namespace

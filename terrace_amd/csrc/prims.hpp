@@ -42,6 +42,56 @@ struct DevBuf {
   operator T *() const { return ptr; }
 };
 
+// Ascending bitonic sort of s[0, P) in LDS by the whole workgroup: P a power
+// of two, s filled (padded with ~0) and a barrier after the writes; every
+// thread calls it, and it returns after a barrier.  With at most one key
+// per thread a thread keeps its key in a register: the stages whose partner
+// is in the same wave (distance below 64) exchange lanes, and only the
+// wider ones go through s with a barrier pair (10 of the 55 stages of a
+// 1024-key sort, which takes about 9 us instead of 14).  More keys than
+// threads: one compare-exchange per thread and stage over s.
+__device__ inline void wg_bitonic_sort(uint64_t *s, uint32_t P) {
+  const uint32_t t = threadIdx.x;
+  if (P <= blockDim.x) {  // uniform
+    uint64_t v = t < P ? s[t] : ~0ull;
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        uint64_t p;
+        if (j < 64) {
+          p = __shfl_xor(v, static_cast<int>(j), 64);
+        } else {
+          __syncthreads();  // the previous stage's reads of s are done
+          if (t < P) s[t] = v;
+          __syncthreads();
+          p = t < P ? s[t ^ j] : ~0ull;
+        }
+        // the key at the lower index keeps the smaller one when ascending
+        const bool lower = (t & j) == 0, up = (t & k) == 0;
+        v = (lower == up) ? (v < p ? v : p) : (v < p ? p : v);
+      }
+    }
+    __syncthreads();
+    if (t < P) s[t] = v;
+  } else {
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        for (uint32_t i = t; i < P; i += blockDim.x) {
+          const uint32_t ixj = i ^ j;
+          if (ixj > i) {
+            const uint64_t a = s[i], b = s[ixj];
+            if ((a > b) == ((i & k) == 0)) {
+              s[i] = b;
+              s[ixj] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // Single-pass (decoupled look-back) state.  Blocks take tickets in the
 // order they start, so a block only waits on blocks already running; each
 // publishes its own total before it waits.  Words carry the launch's epoch,
