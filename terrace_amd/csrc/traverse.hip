@@ -342,8 +342,14 @@ __device__ inline void pull_pma_body(const GView &g, const Op &op, const uint32_
     uint32_t got = 0;
     typename Op::Acc a = Op::ident();
     // the run's first 64 slots, loaded before the inline slots are tested
+    // (their round trips overlap), except for an early-exit pull (BFS),
+    // which reads them only when no inline slot holds a frontier neighbour:
+    // in the large pull levels most vertices find their parent inline, and
+    // the speculative 256 bytes per vertex cost more than the round trip
+    // (BFS 1.37 against 1.48 ms replayed, A/B in one run)
     uint4 q = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
-    if (p + gl * 4 < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + p + gl * 4);
+    const bool lazy = fused && Op::kEarly;
+    if (!lazy && p + gl * 4 < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + p + gl * 4);
     if (fused) {
       // the 13 inline slots, the vertex's smallest neighbours (a PMA vertex
       // holds all 13)
@@ -353,6 +359,7 @@ __device__ inline void pull_pma_body(const GView &g, const Op &op, const uint32_
       }
       if (Op::kEarly && (__ballot(Op::found(a)) & gmask) != 0) need = 0;
     }
+    if (lazy && got < need && p + gl * 4 < g.N) q = *reinterpret_cast<const uint4 *>(g.pdst + p + gl * 4);
     while (got < need && p < g.N) {
       uint64_t slot = p + gl * 4;
       uint32_t d[4] = {q.x, q.y, q.z, q.w};
