@@ -438,8 +438,8 @@ __global__ void __launch_bounds__(256) k_pull_tree(GView g, Op op, const uint32_
   pull_tree_body(g, op, task_v, task_b, ntasks, parts, blockIdx.x, gridDim.x);
 }
 
-// The three pulls in one launch of 256-thread workgroups: [0, nb_inline)
-// inline, then nb_pma PMA workgroups, then the tree tasks (ops whose tree
+// The three pulls in one launch of 256-thread workgroups: [0, nb_pma) PMA
+// groups, then nb_inline inline workgroups, then the tree tasks (ops whose tree
 // partials need no ordered sum).  Each inline or PMA vertex is pulled by one
 // thread or group alone (plain commits); a B-tree vertex's inline part and
 // tree tasks run alongside each other and commit atomically.
@@ -450,8 +450,11 @@ __global__ void __launch_bounds__(256) k_pull_all(GView g, Op op, const uint32_t
   static_assert(!Op::kOrdered, "ordered tree sums take the separate launches");
   if (gate.off()) return;
   const uint32_t b = blockIdx.x;
-  if (b < nb_inline) pull_inline_body(g, op, b, nb_inline, true);
-  else if (b < nb_inline + nb_pma) pull_pma_body(g, op, pma, npma, b - nb_inline, nb_pma, true);
+  // the PMA groups first: the vertices they resolve are visited by the time
+  // the inline workgroups (dispatched later) reach them, which then skip
+  // their blocks (BFS about 1.5% faster replayed, A/B in one run)
+  if (b < nb_pma) pull_pma_body(g, op, pma, npma, b, nb_pma, true);
+  else if (b < nb_inline + nb_pma) pull_inline_body(g, op, b - nb_pma, nb_inline, true);
   else pull_tree_body(g, op, task_v, task_b, ntasks, nullptr, b - nb_inline - nb_pma, gridDim.x - nb_inline - nb_pma);
 }
 
