@@ -85,13 +85,33 @@ __device__ inline uint64_t fence_gallop(const GView &g, uint64_t key, uint64_t l
   return fence_narrow(g, key, lo, hi);
 }
 
+// The leaf of a key below fence[hi] (hi a leaf index): the 16 fences below
+// hi at once, since a key just under a vertex's run (a key leaving its
+// inline slots, a new smallest secondary neighbour) usually belongs to the
+// leaf before the run's first; else the full search.  Fences do not
+// decrease, so the last one <= key inside the window, when the window's
+// first is <= key, is the last in the array: the same leaf as the search.
+__device__ inline uint64_t fence_back(const GView &g, uint64_t key, uint64_t hi) {
+  const uint64_t lo = hi > 16 ? hi - 16 : 0;
+  uint64_t f[16];
+#pragma unroll
+  for (uint32_t x = 0; x < 16; x++) f[x] = lo + x < hi ? g.fence[lo + x] : ~0ull;
+  if (hi == 0 || f[0] > key) return lo == 0 ? 0 : pma_find_leaf(g, key);  // leaf 0 when none is <= key
+  uint64_t r = lo;
+#pragma unroll
+  for (uint32_t x = 1; x < 16; x++)
+    if (lo + x < hi && f[x] <= key) r = lo + x;
+  return r;
+}
+
 // The same leaf, searched from the leaf of begin_slot (the vertex's first
-// PMA slot) onwards.  Any other start (none, a tree id, or a leaf past the
-// key) takes the full search, so a stale begin slot costs time, never the
+// PMA slot) onwards, or just below it.  Any other start (none, a tree id)
+// takes the full search, so a stale begin slot costs time, never the
 // result.
 __device__ inline uint64_t pma_find_leaf_from(const GView &g, uint64_t key, uint64_t begin_slot) {
   const uint64_t lo = begin_slot >> kLeafShift;
-  if (lo >= (1ull << g.H) || g.fence[lo] > key) return pma_find_leaf(g, key);
+  if (lo >= (1ull << g.H)) return pma_find_leaf(g, key);
+  if (g.fence[lo] > key) return fence_back(g, key, lo);
   return fence_gallop(g, key, lo);
 }
 
@@ -141,7 +161,7 @@ __device__ inline uint64_t pma_slot_of(const GView &g, uint32_t s, uint32_t d, u
       if (leaf_out) *leaf_out = static_cast<uint32_t>(L0);
       return leaf_match(g, v, L0 << kLeafShift, s, d);
     }
-    leaf = f0 <= key ? fence_gallop(g, key, L0 + 1) : pma_find_leaf(g, key);
+    leaf = f0 <= key ? fence_gallop(g, key, L0 + 1) : fence_back(g, key, L0);
   } else {
     leaf = pma_find_leaf_from(g, key, begin);
   }
